@@ -1,0 +1,234 @@
+/*
+ * mlegs_amd.h -- C ABI of the MI355X spectral-transform time step.
+ *
+ * The MLegS reference (UCBCFD/MLegS v1.1.3, Modern Fortran + MPI) exposes its
+ * hot path as Fortran generic interfaces on type(scalar) and the global
+ * transform kit `tfm`.  Each entry point below replaces one of them; the
+ * citation after each declaration names the interface it stands in for.
+ * A Fortran host binds these with iso_c_binding (fortran/mlegs_amd.f90) and
+ * Python binds them with ctypes (mlegs_amd/_lib.py).
+ *
+ * Conventions
+ *   - Every function returns 0 (MLX_OK) or a non-zero MLX_E_* code; the
+ *     message of the last failure is mlx_last_error(ctx).  The library never
+ *     exits the process.  Error classes follow the reference's error flags
+ *     (src/modules/mlegs_envir.f90:39-45).
+ *   - Device memory is owned by the library behind opaque handles; host
+ *     buffers are borrowed for the duration of the call.
+ *   - Host field buffers use exactly the reference's global layout: complex(8)
+ *     e(nrdim, npdim, nzdim), column-major (i fastest), interleaved re/im
+ *     doubles (src/modules/mlegs_scalar.f90:15-49, docs/tutorial/
+ *     transformation.md:58-86).
+ *   - Work is enqueued on the context's HIP stream in call order; only
+ *     downloads, mlx_sync and functions returning host scalars block.
+ *   - One context per host thread; contexts share no mutable state.
+ *   - Only the radial chopping offset (nrchop_offset) is supported; the
+ *     reference's path never sets the azimuthal or axial offsets.
+ *   - Hyperviscosity (hyperpow != 0) is accepted by the kit (it sizes nrdim)
+ *     but the steppers reject it: helmp/ihelmp are not on this path.
+ */
+#ifndef MLEGS_AMD_H
+#define MLEGS_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes (mirror the reference's error flags 201-206) */
+#define MLX_OK          0
+#define MLX_E_LAYOUT    201  /* size / layout / space mismatch            */
+#define MLX_E_ALLOC     202  /* device allocation failure                  */
+#define MLX_E_ARG       203  /* invalid argument                           */
+#define MLX_E_FILE      204  /* reserved (file I/O is not on this path)    */
+#define MLX_E_NONFINITE 205  /* non-finite state (check_stability)         */
+#define MLX_E_HIP       206  /* HIP runtime failure                        */
+
+/* precision of the Legendre contractions (trans and vec2tp) */
+#define MLX_PREC_FP64   0    /* FP64 operands, FP64 accumulation            */
+#define MLX_PREC_FP32   1    /* FP32 operands, FP32 MFMA accumulation       */
+#define MLX_PREC_BF16X3 2    /* 3-term split-BF16 operands, FP32 MFMA acc.   */
+
+typedef struct mlx_ctx mlx_ctx;
+typedef struct mlx_field mlx_field;
+
+/* Run parameters: the module variables of src/modules/mlegs_base.f90:7-41. */
+typedef struct mlx_params {
+  int32_t nr, np, nz;
+  int32_t nrchop, npchop, nzchop;
+  double ell, zlen;
+  double dt, ti, totaltime;
+  int32_t ni, totaln;
+  double visc;
+  int32_t hyperpow;
+  double hypervisc;
+  int32_t is_svv;
+  double svv_cutoff, svv_target, svv_strength, svv_relax;
+} mlx_params;
+
+/* ---- context and kit ---------------------------------------------------- */
+
+/* Create a context on HIP device `device`; stream may be NULL (the context
+ * creates its own) or a hipStream_t.  Replaces the process-wide globals of
+ * mlegs_base / mlegs_envir (src/modules/mlegs_base.f90:7-53). */
+int mlx_ctx_create(const mlx_params* p, int device, void* stream, mlx_ctx** out);
+int mlx_ctx_destroy(mlx_ctx* ctx);
+const char* mlx_last_error(mlx_ctx* ctx);
+int mlx_sync(mlx_ctx* ctx);
+
+/* Build the transform kit on the GPU in FP64: Gauss-Legendre nodes and
+ * weights, log-norms, the normalised P_n^m table, at0/at1, band-operator
+ * coefficients and projection tables.  Replaces tfm%init
+ * (src/submodules/mlegs_spectfm_init.f90:6-154). */
+int mlx_kit_init(mlx_ctx* ctx);
+
+/* Kit sizes: out[0..8] = nrdim, npdim, nzdim, nrh, ne, chopzl, chopzu, nph, npchop
+ * (src/modules/mlegs_spectfm.f90:35-62). */
+int mlx_kit_dims(mlx_ctx* ctx, int32_t* out9);
+
+/* Copy one kit array to the host.  which: "x","w","ln","r","ak","lognorm",
+ * "pf","at0","at1" with the reference's shapes (column-major); n = number
+ * of doubles the host buffer holds.  Replaces direct reads of tfm%...
+ * (src/modules/mlegs_spectfm.f90:17-28). */
+int mlx_kit_get(mlx_ctx* ctx, const char* which, double* host, int64_t n);
+
+/* Legendre-contraction precision used by mlx_trans and mlx_vec2tp. */
+int mlx_set_precision(mlx_ctx* ctx, int prec);
+
+/* ---- fields --------------------------------------------------------------- */
+
+/* Allocate a scalar of the kit's global size, space "PPP", ln = 0, offsets 0.
+ * Replaces scalar%init (src/submodules/mlegs_scalar_init.f90:6-67). */
+int mlx_field_create(mlx_ctx* ctx, mlx_field** out);
+/* scalar%dealloc (src/submodules/mlegs_scalar_init.f90:70-81). */
+int mlx_field_destroy(mlx_ctx* ctx, mlx_field* f);
+/* Deep copy dst = src: data, ln, offsets, space
+ * (src/submodules/mlegs_scalar_init.f90:106-140). */
+int mlx_field_copy(mlx_ctx* ctx, mlx_field* dst, const mlx_field* src);
+/* Upload / download the whole scalar in the reference global layout
+ * (replaces scalar%disassemble / %assemble, mlegs_scalar_dist.f90:70-368).
+ * host holds 2*nrdim*npdim*nzdim doubles; space is 3 chars; offsets[3] are
+ * (nrchop, npchop, nzchop) offsets. */
+int mlx_field_upload(mlx_ctx* ctx, mlx_field* f, const double* host,
+                     int64_t nrdim, int64_t npdim, int64_t nzdim,
+                     const char* space, double ln, const int32_t* offsets);
+int mlx_field_download(mlx_ctx* ctx, const mlx_field* f, double* host,
+                       int64_t nrdim, int64_t npdim, int64_t nzdim,
+                       char* space4, double* ln, int32_t* offsets);
+/* Metadata access without moving the data. */
+int mlx_field_set_meta(mlx_ctx* ctx, mlx_field* f, const char* space, double ln, const int32_t* offsets);
+int mlx_field_get_meta(mlx_ctx* ctx, const mlx_field* f, char* space4, double* ln, int32_t* offsets);
+/* scalar%chop_offset (src/submodules/mlegs_scalar_init.f90:84-103). */
+int mlx_chop_offset(mlx_ctx* ctx, mlx_field* f, int32_t iof1, int32_t iof2, int32_t iof3);
+/* dst = a*dst + b*src on data and ln (the `s%e = a*s%e + b*t%e` statements
+ * of the reference's drivers, e.g. vortical_flow_3d.f90:136-137, 379). */
+int mlx_field_axpby(mlx_ctx* ctx, mlx_field* dst, double a, const mlx_field* src, double b);
+
+/* ---- transforms and masks ------------------------------------------------- */
+
+/* trans(s, space, tfm) (src/modules/mlegs_scalar.f90:147-155;
+ * src/submodules/mlegs_scalar_ops.f90:157-235). */
+int mlx_trans(mlx_ctx* ctx, mlx_field* f, const char* space);
+/* chop (mlegs_scalar.f90:116-123; mlegs_scalar_ops.f90:6-41). */
+int mlx_chop(mlx_ctx* ctx, mlx_field* f);
+/* dealias (mlegs_scalar.f90:126-133; mlegs_scalar_ops.f90:43-70). */
+int mlx_dealias(mlx_ctx* ctx, mlx_field* f);
+/* svv_filter(s, tfm, gain) (mlegs_scalar.f90:136-144; mlegs_scalar_ops.f90:72-155);
+ * *gain is read and updated. */
+int mlx_svv_filter(mlx_ctx* ctx, mlx_field* f, double* gain);
+/* calcat0 / calcat1: values at r=0 / r=inf of m=0 per z slice; out holds
+ * 2*nz doubles (mlegs_scalar_ops.f90:237-309). */
+int mlx_calcat0(mlx_ctx* ctx, const mlx_field* f, double* out);
+int mlx_calcat1(mlx_ctx* ctx, const mlx_field* f, double* out);
+/* zeroat1 (mlegs_scalar.f90:180-187; mlegs_scalar_ops.f90:311-325). */
+int mlx_zeroat1(mlx_ctx* ctx, mlx_field* f);
+
+/* ---- linear operators ---------------------------------------------------- */
+
+/* delsqp / idelsqp (mlegs_scalar_ops.f90:327-416). */
+int mlx_delsqp(mlx_ctx* ctx, mlx_field* f);
+int mlx_idelsqp(mlx_ctx* ctx, mlx_field* f);
+/* xxdx, del2h, del2 (mlegs_scalar_ops.f90:418-573). */
+int mlx_xxdx(mlx_ctx* ctx, mlx_field* f);
+int mlx_del2h(mlx_ctx* ctx, mlx_field* f);
+int mlx_del2(mlx_ctx* ctx, mlx_field* f);
+/* idel2 without / with a prescribed log coefficient
+ * (mlegs_scalar.f90:250-263; idel2_proln :671-760, idel2_preln :575-669). */
+int mlx_idel2(mlx_ctx* ctx, mlx_field* f);
+int mlx_idel2_preln(mlx_ctx* ctx, mlx_field* f, double preln);
+/* ihelm(s, alpha, tfm) (mlegs_scalar.f90:277-285; mlegs_scalar_ops.f90:791-854). */
+int mlx_ihelm(mlx_ctx* ctx, mlx_field* f, double alpha);
+
+/* ---- time steppers (hyperpow = 0) ---------------------------------------- */
+
+/* febe(s, nl, dt, tfm) (mlegs_scalar.f90:350-359; mlegs_scalar_ops.f90:1157-1198). */
+int mlx_febe(mlx_ctx* ctx, mlx_field* s, const mlx_field* nl, double dt);
+/* abcn(s, s_p, nl, nl_p, dt, tfm) (mlegs_scalar.f90:367-376;
+ * mlegs_scalar_ops.f90:1200-1262): s advances, s_p <- s, nl_p <- nl. */
+int mlx_abcn(mlx_ctx* ctx, mlx_field* s, mlx_field* s_p, const mlx_field* nl, mlx_field* nl_p, double dt);
+
+/* ---- vector-field operators ---------------------------------------------- */
+
+/* vecprod: v <- v x u (mlegs_scalar.f90:381-389; mlegs_scalar_ops.f90:1264-1306). */
+int mlx_vecprod(mlx_ctx* ctx, mlx_field* vr, mlx_field* vp, mlx_field* vz,
+                const mlx_field* ur, const mlx_field* up, const mlx_field* uz);
+/* vec2tp: PPP vector -> (psi, chi) in FFF (mlegs_scalar.f90:394-402;
+ * mlegs_scalar_ops.f90:1308-1453). */
+int mlx_vec2tp(mlx_ctx* ctx, const mlx_field* vr, const mlx_field* vp, const mlx_field* vz,
+               mlx_field* psi, mlx_field* chi);
+/* tp2vec: (psi, chi) in FFF -> PPP vector (mlegs_scalar.f90:407-415;
+ * mlegs_scalar_ops.f90:1455-1545). */
+int mlx_tp2vec(mlx_ctx* ctx, const mlx_field* psi, const mlx_field* chi,
+               mlx_field* vr, mlx_field* vp, mlx_field* vz);
+/* tp2curlvec (mlegs_scalar.f90:420-428; mlegs_scalar_ops.f90:1547-1560). */
+int mlx_tp2curlvec(mlx_ctx* ctx, const mlx_field* psi, const mlx_field* chi,
+                   mlx_field* wr, mlx_field* wp, mlx_field* wz);
+
+/* ---- the 3D vortex step (src/apps/vortical_flow_3d.f90) ------------------ */
+
+/* advection_rhs(psi, chi, nlpsi, nlchi, uz) (vortical_flow_3d.f90:353-395);
+ * psi and chi are de-aliased in place as in the reference. */
+int mlx_advection_rhs(mlx_ctx* ctx, mlx_field* psi, mlx_field* chi,
+                      mlx_field* nlpsi, mlx_field* nlchi, const mlx_field* uz);
+/* check_stability (vortical_flow_3d.f90:397-409): returns MLX_E_NONFINITE if
+ * any entry of psi or chi is NaN/Inf. */
+int mlx_check_stability(mlx_ctx* ctx, const mlx_field* psi, const mlx_field* chi);
+
+/* Vortex run state: psi, chi, nlpsi, nlchi, the AB2 history and the SVV
+ * gains kept on the device (the local variables of vortical_flow_3d.f90:26-31). */
+typedef struct mlx_vortex mlx_vortex;
+/* Takes copies of psi, chi (FFF) and uz (PPP). */
+int mlx_vortex_create(mlx_ctx* ctx, const mlx_field* psi, const mlx_field* chi,
+                      const mlx_field* uz, mlx_vortex** out);
+int mlx_vortex_destroy(mlx_ctx* ctx, mlx_vortex* v);
+/* FE-BE + Richardson bootstrap (vortical_flow_3d.f90:116-147). */
+int mlx_vortex_bootstrap(mlx_ctx* ctx, mlx_vortex* v);
+/* nsteps iterations of the AB2/CN loop body (vortical_flow_3d.f90:160-181)
+ * with step size dt; the non-finite check is accumulated on the device and
+ * reported (MLX_E_NONFINITE) when check != 0 at the end of the call. */
+int mlx_vortex_step(mlx_ctx* ctx, mlx_vortex* v, int32_t nsteps, double dt, int32_t check);
+/* Borrow the state fields (0: psi, 1: chi, 2: nlpsi, 3: nlchi, 4: psi_prev,
+ * 5: chi_prev, 6: nlpsi_prev, 7: nlchi_prev, 8: uz) and read the SVV gains. */
+int mlx_vortex_field(mlx_ctx* ctx, mlx_vortex* v, int32_t which, mlx_field** out);
+int mlx_vortex_gains(mlx_ctx* ctx, mlx_vortex* v, double* gain_psi, double* gain_chi);
+
+/* ---- diagnostics (no reference counterpart; FP64 reductions) ------------- */
+
+/* Kinetic energy of a PPP velocity field by FP64 physical quadrature:
+ * 0.5 * sum_{i,j,k} w_i ell^2/(1-x_i)^2 (2pi/np)(zlen/nz) (vr^2+vp^2+vz^2). */
+int mlx_energy(mlx_ctx* ctx, const mlx_field* vr, const mlx_field* vp, const mlx_field* vz, double* out);
+
+/* ---- measurement --------------------------------------------------------- */
+
+/* Time `iters` launches of one kernel family on the context stream with HIP
+ * events (after one warm-up launch) and return the mean duration in ms.
+ * which: "leg_fwd", "leg_bwd", "fft_phi_fwd", "fft_phi_bwd", "fft_z_fwd",
+ * "fft_z_bwd".  The field is used as operand and left in an unspecified
+ * state of the same space. */
+int mlx_time_kernel(mlx_ctx* ctx, mlx_field* f, const char* which, int32_t iters, double* ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MLEGS_AMD_H */

@@ -1,0 +1,340 @@
+"""MI355X-native spectral-transform time step with the MLegS operator API.
+
+Host-side mirror of the reference's Fortran surface (UCBCFD/MLegS v1.1.3):
+``Kit`` is the transform kit ``tfm`` (src/modules/mlegs_spectfm.f90:55-63),
+``Scalar`` is ``type(scalar)`` (src/modules/mlegs_scalar.f90:15-49), and the
+module functions carry the names, argument order and ``space`` strings of
+the generic interfaces in src/modules/mlegs_scalar.f90:96-450.  Every call
+goes through the C ABI of ``libmlegs_amd.so`` to hand-written HIP kernels;
+errors raise ``MlxError`` (the reference stops the program).
+"""
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from ._lib import MlxError, PREC_FP64, PREC_FP32, PREC_BF16X3  # noqa: F401
+from .params import Params, read_input, timestep_set, InputError  # noqa: F401
+
+__all__ = ['Kit', 'Scalar', 'trans', 'chop', 'dealias', 'svv_filter', 'calcat0', 'calcat1', 'zeroat1',
+           'delsqp', 'idelsqp', 'xxdx', 'del2h', 'del2', 'idel2', 'ihelm', 'febe', 'abcn', 'vecprod',
+           'vec2tp', 'tp2vec', 'tp2curlvec', 'advection_rhs', 'check_stability', 'VortexRun', 'energy',
+           'Params', 'read_input', 'timestep_set', 'MlxError', 'InputError']
+
+
+def _params_struct(p):
+    s = _lib.MlxParams()
+    for name, _ in _lib.MlxParams._fields_:
+        v = getattr(p, name)
+        setattr(s, name, int(v) if isinstance(v, bool) else v)
+    return s
+
+
+class Kit:
+    """The transform kit on one GPU (tfm_kit_3d + tfm%init)."""
+
+    def __init__(self, params, device=0, precision=PREC_FP64):
+        self.lib = _lib.load()
+        self.params = params
+        self._ctx = ctypes.c_void_p()
+        ps = _params_struct(params)
+        rc = self.lib.mlx_ctx_create(ctypes.byref(ps), int(device), None, ctypes.byref(self._ctx))
+        if rc != 0:
+            raise MlxError(rc, 'mlx_ctx_create failed')
+        self._check(self.lib.mlx_kit_init(self._ctx))
+        d = (ctypes.c_int32 * 9)()
+        self._check(self.lib.mlx_kit_dims(self._ctx, d))
+        self.nrdim, self.npdim, self.nzdim, self.nrh, self.ne, self.chopzl, self.chopzu, self.nph, _ = list(d)
+        self.dims = (self.nrdim, self.npdim, self.nzdim)
+        self.nr, self.np, self.nz = params.nr, params.np, params.nz
+        self.nrchop, self.npchop, self.nzchop = params.nrchop, params.npchop, params.nzchop
+        self.ell, self.zlen = params.ell, params.zlen
+        self.set_precision(precision)
+        self.x = self.get('x', self.nr)
+        self.w = self.get('w', self.nr)
+        self.r = self.get('r', self.nr)
+        self.lnx = self.get('ln', self.nr)
+        self.ak = self.get('ak', self.nz)
+
+    def _check(self, rc):
+        _lib.check(self._ctx, rc)
+
+    @property
+    def ctx(self):
+        return self._ctx
+
+    def set_precision(self, prec):
+        self._check(self.lib.mlx_set_precision(self._ctx, int(prec)))
+        self.precision = prec
+
+    def get(self, which, n):
+        buf = np.zeros(n, dtype=np.float64)
+        self._check(self.lib.mlx_kit_get(self._ctx, which.encode(), buf.ctypes.data_as(_lib._DP), n))
+        return buf
+
+    def table(self):
+        """pf(nr/2, nrchop+14, npchop) in the reference's shape."""
+        n = self.nrh * self.ne * self.npchop
+        return self.get('pf', n).reshape((self.nrh, self.ne, self.npchop), order='F')
+
+    def lognorm(self):
+        return self.get('lognorm', self.ne * self.npchop).reshape((self.ne, self.npchop), order='F')
+
+    def sync(self):
+        self._check(self.lib.mlx_sync(self._ctx))
+
+    def close(self):
+        if self._ctx:
+            self.lib.mlx_ctx_destroy(self._ctx)
+            self._ctx = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def time_kernel(self, s, which, iters):
+        ms = ctypes.c_double()
+        self._check(self.lib.mlx_time_kernel(self._ctx, s.h, which.encode(), int(iters), ctypes.byref(ms)))
+        return ms.value
+
+
+class Scalar:
+    """type(scalar) on the device: data e(nrdim, npdim, nzdim), ln, chop offsets, space."""
+
+    def __init__(self, tfm, space='PPP', handle=None):
+        self.tfm = tfm
+        self.lib = tfm.lib
+        if handle is None:
+            h = ctypes.c_void_p()
+            tfm._check(self.lib.mlx_field_create(tfm.ctx, ctypes.byref(h)))
+            self.h = h
+            self._owned = True
+            if space != 'PPP':
+                self.set_meta(space=space)
+        else:
+            self.h = handle
+            self._owned = False
+
+    def __del__(self):
+        try:
+            if self._owned and self.h and self.tfm._ctx:
+                self.lib.mlx_field_destroy(self.tfm.ctx, self.h)
+        except Exception:
+            pass
+
+    # -- metadata -----------------------------------------------------------
+    def meta(self):
+        sp = ctypes.create_string_buffer(4)
+        ln = ctypes.c_double()
+        off = (ctypes.c_int32 * 3)()
+        self.tfm._check(self.lib.mlx_field_get_meta(self.tfm.ctx, self.h, sp, ctypes.byref(ln), off))
+        return sp.value.decode(), ln.value, tuple(off)
+
+    @property
+    def space(self):
+        return self.meta()[0]
+
+    @space.setter
+    def space(self, v):
+        self.set_meta(space=v)
+
+    @property
+    def ln(self):
+        return self.meta()[1]
+
+    @ln.setter
+    def ln(self, v):
+        self.set_meta(ln=v)
+
+    def set_meta(self, space=None, ln=None, offsets=None):
+        sp0, ln0, off0 = self.meta()
+        sp = (space if space is not None else sp0).encode()
+        lnv = ln if ln is not None else ln0
+        off = (ctypes.c_int32 * 3)(*(offsets if offsets is not None else off0))
+        self.tfm._check(self.lib.mlx_field_set_meta(self.tfm.ctx, self.h, sp, float(lnv), off))
+
+    def chop_offset(self, iof1, iof2=0, iof3=0):
+        self.tfm._check(self.lib.mlx_chop_offset(self.tfm.ctx, self.h, iof1, iof2, iof3))
+
+    # -- data ---------------------------------------------------------------
+    def upload(self, e, space=None, ln=0.0, offsets=(0, 0, 0)):
+        a = np.asfortranarray(e, dtype=np.complex128)
+        if a.shape != self.tfm.dims:
+            raise ValueError('scalar shape %s does not match the kit %s' % (a.shape, self.tfm.dims))
+        off = (ctypes.c_int32 * 3)(*offsets)
+        sp = (space or self.space).encode()
+        self.tfm._check(self.lib.mlx_field_upload(self.tfm.ctx, self.h, a.ctypes.data_as(_lib._DP),
+                                                  *self.tfm.dims, sp, float(ln), off))
+        return self
+
+    def download(self):
+        out = np.zeros(self.tfm.dims, dtype=np.complex128, order='F')
+        sp = ctypes.create_string_buffer(4)
+        ln = ctypes.c_double()
+        off = (ctypes.c_int32 * 3)()
+        self.tfm._check(self.lib.mlx_field_download(self.tfm.ctx, self.h, out.ctypes.data_as(_lib._DP),
+                                                    *self.tfm.dims, sp, ctypes.byref(ln), off))
+        return out
+
+    @property
+    def e(self):
+        return self.download()
+
+    def copy(self):
+        o = Scalar(self.tfm)
+        self.tfm._check(self.lib.mlx_field_copy(self.tfm.ctx, o.h, self.h))
+        return o
+
+    def assign(self, other):
+        self.tfm._check(self.lib.mlx_field_copy(self.tfm.ctx, self.h, other.h))
+
+    def axpby(self, a, other, b):
+        """self = a*self + b*other (data and ln)."""
+        self.tfm._check(self.lib.mlx_field_axpby(self.tfm.ctx, self.h, float(a), other.h, float(b)))
+
+
+def _c(tfm, rc):
+    tfm._check(rc)
+
+
+def trans(s, space, tfm):
+    _c(tfm, tfm.lib.mlx_trans(tfm.ctx, s.h, space.encode()))
+
+
+def chop(s, tfm):
+    _c(tfm, tfm.lib.mlx_chop(tfm.ctx, s.h))
+
+
+def dealias(s, tfm):
+    _c(tfm, tfm.lib.mlx_dealias(tfm.ctx, s.h))
+
+
+def svv_filter(s, tfm, gain):
+    g = ctypes.c_double(gain)
+    _c(tfm, tfm.lib.mlx_svv_filter(tfm.ctx, s.h, ctypes.byref(g)))
+    return g.value
+
+
+def _calcat(fn, s, tfm):
+    out = np.zeros(2 * tfm.nzdim)
+    _c(tfm, fn(tfm.ctx, s.h, out.ctypes.data_as(_lib._DP)))
+    return out[0::2] + 1j * out[1::2]
+
+
+def calcat0(s, tfm):
+    return _calcat(tfm.lib.mlx_calcat0, s, tfm)
+
+
+def calcat1(s, tfm):
+    return _calcat(tfm.lib.mlx_calcat1, s, tfm)
+
+
+def zeroat1(s, tfm):
+    _c(tfm, tfm.lib.mlx_zeroat1(tfm.ctx, s.h))
+
+
+def delsqp(s, tfm):
+    _c(tfm, tfm.lib.mlx_delsqp(tfm.ctx, s.h))
+
+
+def idelsqp(s, tfm):
+    _c(tfm, tfm.lib.mlx_idelsqp(tfm.ctx, s.h))
+
+
+def xxdx(s, tfm):
+    _c(tfm, tfm.lib.mlx_xxdx(tfm.ctx, s.h))
+
+
+def del2h(s, tfm):
+    _c(tfm, tfm.lib.mlx_del2h(tfm.ctx, s.h))
+
+
+def del2(s, tfm):
+    _c(tfm, tfm.lib.mlx_del2(tfm.ctx, s.h))
+
+
+def idel2(s, tfm, preln=None):
+    if preln is None:
+        _c(tfm, tfm.lib.mlx_idel2(tfm.ctx, s.h))
+    else:
+        _c(tfm, tfm.lib.mlx_idel2_preln(tfm.ctx, s.h, float(preln)))
+
+
+def ihelm(s, alpha, tfm):
+    _c(tfm, tfm.lib.mlx_ihelm(tfm.ctx, s.h, float(alpha)))
+
+
+def febe(s, nl, dt, tfm):
+    _c(tfm, tfm.lib.mlx_febe(tfm.ctx, s.h, nl.h, float(dt)))
+
+
+def abcn(s, s_p, nl, nl_p, dt, tfm):
+    _c(tfm, tfm.lib.mlx_abcn(tfm.ctx, s.h, s_p.h, nl.h, nl_p.h, float(dt)))
+
+
+def vecprod(vr, vp, vz, ur, up, uz, tfm):
+    _c(tfm, tfm.lib.mlx_vecprod(tfm.ctx, vr.h, vp.h, vz.h, ur.h, up.h, uz.h))
+
+
+def vec2tp(vr, vp, vz, psi, chi, tfm):
+    _c(tfm, tfm.lib.mlx_vec2tp(tfm.ctx, vr.h, vp.h, vz.h, psi.h, chi.h))
+
+
+def tp2vec(psi, chi, vr, vp, vz, tfm):
+    _c(tfm, tfm.lib.mlx_tp2vec(tfm.ctx, psi.h, chi.h, vr.h, vp.h, vz.h))
+
+
+def tp2curlvec(psi, chi, wr, wp, wz, tfm):
+    _c(tfm, tfm.lib.mlx_tp2curlvec(tfm.ctx, psi.h, chi.h, wr.h, wp.h, wz.h))
+
+
+def advection_rhs(psi, chi, nlpsi, nlchi, uz, tfm):
+    _c(tfm, tfm.lib.mlx_advection_rhs(tfm.ctx, psi.h, chi.h, nlpsi.h, nlchi.h, uz.h))
+
+
+def check_stability(psi, chi, tfm):
+    _c(tfm, tfm.lib.mlx_check_stability(tfm.ctx, psi.h, chi.h))
+
+
+def energy(vr, vp, vz, tfm):
+    out = ctypes.c_double()
+    _c(tfm, tfm.lib.mlx_energy(tfm.ctx, vr.h, vp.h, vz.h, ctypes.byref(out)))
+    return out.value
+
+
+class VortexRun:
+    """The 3D vortex driver's state on the device (vortical_flow_3d.f90:26-206)."""
+
+    FIELDS = ('psi', 'chi', 'nlpsi', 'nlchi', 'psi_prev', 'chi_prev', 'nlpsi_prev', 'nlchi_prev', 'uz')
+
+    def __init__(self, tfm, psi, chi, uz):
+        self.tfm = tfm
+        self.h = ctypes.c_void_p()
+        _c(tfm, tfm.lib.mlx_vortex_create(tfm.ctx, psi.h, chi.h, uz.h, ctypes.byref(self.h)))
+
+    def __del__(self):
+        try:
+            if self.h and self.tfm._ctx:
+                self.tfm.lib.mlx_vortex_destroy(self.tfm.ctx, self.h)
+        except Exception:
+            pass
+
+    def bootstrap(self):
+        _c(self.tfm, self.tfm.lib.mlx_vortex_bootstrap(self.tfm.ctx, self.h))
+
+    def step(self, nsteps=1, dt=None, check=True):
+        dt = self.tfm.params.dt if dt is None else dt
+        _c(self.tfm, self.tfm.lib.mlx_vortex_step(self.tfm.ctx, self.h, int(nsteps), float(dt), int(bool(check))))
+
+    def field(self, name):
+        i = self.FIELDS.index(name)
+        h = ctypes.c_void_p()
+        _c(self.tfm, self.tfm.lib.mlx_vortex_field(self.tfm.ctx, self.h, i, ctypes.byref(h)))
+        return Scalar(self.tfm, handle=h)
+
+    def gains(self):
+        a, b = ctypes.c_double(), ctypes.c_double()
+        _c(self.tfm, self.tfm.lib.mlx_vortex_gains(self.tfm.ctx, self.h, ctypes.byref(a), ctypes.byref(b)))
+        return a.value, b.value

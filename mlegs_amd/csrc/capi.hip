@@ -1,0 +1,894 @@
+// C ABI of the library: context, kit, fields, and the reference's scalar
+// operations composed from the kernels of kit.hip, fft.hip, legendre.hip and
+// spectral.hip.  Each entry point follows the control flow of the reference
+// routine named in include/mlegs_amd.h; every device step is enqueued on the
+// context stream.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+#include "mlx_internal.h"
+
+using namespace mlx;
+
+namespace {
+
+int space_id(const char *s) {
+  if (!s) return -1;
+  if (!std::strncmp(s, "PPP", 3)) return 0;
+  if (!std::strncmp(s, "PFP", 3)) return 1;
+  if (!std::strncmp(s, "FFP", 3)) return 2;
+  if (!std::strncmp(s, "FFF", 3)) return 3;
+  return -1;
+}
+void set_space(mlx_field *f, const char *s) { std::memcpy(f->space, s, 3), f->space[3] = 0; }
+bool is_fff(const mlx_field *f) { return !std::strncmp(f->space, "FFF", 3); }
+bool is_ppp(const mlx_field *f) { return !std::strncmp(f->space, "PPP", 3); }
+
+void need_kit(mlx_ctx *c) {
+  if (!c->kit.ready) MLX_THROW(MLX_E_ARG, "transformation kit is not initialized");
+}
+
+mlx_field *field_new(mlx_ctx *c) {
+  mlx_field *f = new mlx_field();
+  if (hipMalloc(&f->e, sizeof(double2) * c->nelem) != hipSuccess) {
+    delete f;
+    MLX_THROW(MLX_E_ALLOC, "scalar_init: device allocation failed");
+  }
+  if (hipMalloc(&f->sc, sizeof(double) * 8) != hipSuccess) {
+    (void)hipFree(f->e);
+    delete f;
+    MLX_THROW(MLX_E_ALLOC, "scalar_init: device allocation failed");
+  }
+  HIPCHK(hipMemsetAsync(f->e, 0, sizeof(double2) * c->nelem, c->stream));
+  HIPCHK(hipMemsetAsync(f->sc, 0, sizeof(double) * 8, c->stream));
+  return f;
+}
+
+void field_free(mlx_field *f) {
+  if (!f) return;
+  if (f->e) (void)hipFree(f->e);
+  if (f->sc) (void)hipFree(f->sc);
+  delete f;
+}
+
+// deep copy (mlegs_scalar_init.f90:106-140)
+void fcopy(mlx_ctx *c, mlx_field *d, const mlx_field *s) {
+  if (d == s) return;
+  HIPCHK(hipMemcpyAsync(d->e, s->e, sizeof(double2) * c->nelem, hipMemcpyDeviceToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(d->sc, s->sc, sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+  std::memcpy(d->space, s->space, 4);
+  std::memcpy(d->off, s->off, sizeof(d->off));
+}
+
+// dst takes src's data (buffers are exchanged; src becomes scratch)
+void fmove(mlx_field *d, mlx_field *s) {
+  std::swap(d->e, s->e);
+  std::swap(d->sc, s->sc);
+  std::memcpy(d->space, s->space, 4);
+  std::memcpy(d->off, s->off, sizeof(d->off));
+}
+
+mlx_field *tmp(mlx_ctx *c, size_t i) {
+  if (c->pool.size() <= i) c->pool.resize(i + 1, nullptr);
+  if (!c->pool[i]) c->pool[i] = field_new(c);
+  return c->pool[i];
+}
+
+void swap_scr(mlx_ctx *c, mlx_field *f) { std::swap(f->e, c->scr[0]); }
+
+// ---------------------------------------------------------------------------
+// trans (mlegs_scalar_ops.f90:157-235)
+// ---------------------------------------------------------------------------
+void trans(mlx_ctx *c, mlx_field *f, const char *space) {
+  need_kit(c);
+  int cur = space_id(f->space);
+  if (cur < 0) MLX_THROW(MLX_E_LAYOUT, "trans: scalar space info corrupted (only accepting PPP, PFP, FFP and FFF)");
+  int nw = space_id(space);
+  if (nw < 0) MLX_THROW(MLX_E_ARG, "trans: only taking PPP, PFP, FFP and FFF for spectral transformation");
+  const KitDev &k = c->kit;
+  if (f->off[1] != 0 || f->off[2] != 0) MLX_THROW(MLX_E_ARG, "trans: only radial chopping offsets are supported");
+  if (cur < nw) {
+    while (true) {
+      if (cur == 0) {
+        if (k.np > 1) fft_phi(c, f->e, -1);
+        set_space(f, "PFP");
+      }
+      if (cur == 1) {
+        k_ln_column(c, f->e, f->sc, -1.0);
+        leg_forward(c, f->e, c->scr[0], f->off[0]);
+        swap_scr(c, f);
+        set_space(f, "FFP");
+      }
+      if (cur == 2) {
+        if (k.nz > 1) fft_z(c, f->e, -1);
+        set_space(f, "FFF");
+      }
+      if (cur == 3) break;
+      ++cur;
+      if (cur == nw) break;
+    }
+  }
+  if (cur > nw) {
+    while (true) {
+      if (cur == 3) {
+        if (k.nz > 1) fft_z(c, f->e, +1);
+        set_space(f, "FFP");
+      }
+      if (cur == 2) {
+        leg_backward(c, f->e, c->scr[0], f->off[0]);
+        swap_scr(c, f);
+        k_ln_column(c, f->e, f->sc, +1.0);
+        set_space(f, "PFP");
+      }
+      if (cur == 1) {
+        if (k.np > 1) fft_phi(c, f->e, +1);
+        set_space(f, "PPP");
+      }
+      if (cur == 0) break;
+      --cur;
+      if (cur == nw) break;
+    }
+  }
+}
+
+void chop(mlx_ctx *c, mlx_field *f) { k_chop(c, f->e, f->space, f->off[0]); }
+void dealias(mlx_ctx *c, mlx_field *f) {
+  need_kit(c);
+  k_dealias(c, f->e, f->space);
+}
+
+void to_fff(mlx_ctx *c, mlx_field *f) {
+  if (!is_fff(f)) trans(c, f, "FFF");
+}
+
+void zeroat1(mlx_ctx *c, mlx_field *f) {
+  to_fff(c, f);
+  k_zeroat1(c, f->e, c->kit.nrchop + f->off[0]);
+}
+
+void bandmul(mlx_ctx *c, mlx_field *f, int op) {
+  need_kit(c);
+  to_fff(c, f);
+  k_bandmul(c, f->e, f->sc, op, f->off[0]);
+}
+
+void ihelm(mlx_ctx *c, mlx_field *f, double alpha) {
+  need_kit(c);
+  to_fff(c, f);
+  if (std::fabs(alpha) < 5e-14) MLX_THROW(MLX_E_ARG, "ihelm: alpha equals to zero. Inversion of 0*identity is impossible");
+  k_helm_pre(c, f->e, f->sc, f->sc, alpha);
+  k_helm_solve(c, f->e, alpha, 0, f->off[0]);
+}
+
+void idel2_proln(mlx_ctx *c, mlx_field *f) {
+  need_kit(c);
+  to_fff(c, f);
+  k_helm_solve(c, f->e, 0.0, 1, f->off[0]);
+  k_idel2_post(c, f->e, f->sc);
+}
+
+void check_step_inputs(mlx_ctx *c, std::initializer_list<const mlx_field *> fs) {
+  for (const mlx_field *f : fs)
+    if (!is_fff(f)) MLX_THROW(MLX_E_LAYOUT, "fefe: all input scalars must be in FFF for time stepping");
+  if (c->p.hyperpow != 0) MLX_THROW(MLX_E_ARG, "hyperviscous time stepping (helmp/ihelmp) is not on this path");
+}
+
+// febe (mlegs_scalar_ops.f90:1157-1198)
+void febe(mlx_ctx *c, mlx_field *s, const mlx_field *nl, double dt) {
+  check_step_inputs(c, {s, nl});
+  if (c->p.visc < 5e-14) MLX_THROW(MLX_E_ARG, "febe: inviscid case and no linear term in rhs. semi-implicit time adv is impossible");
+  double a = -1.0 / (dt * c->p.visc);
+  k_abcn_rhs(c, s->e, s->sc, s->e, s->sc, nl->e, nl->sc, nullptr, nullptr, nullptr, dt, a, 0);
+  ihelm(c, s, a);
+}
+
+// abcn (mlegs_scalar_ops.f90:1200-1262); copy_prev = false skips s_p <- s
+// (s_p is never read by the stepper) and exchanges nl_p <- nl by buffer swap
+// when nl may be overwritten afterwards (swap_nl).
+void abcn(mlx_ctx *c, mlx_field *s, mlx_field *s_p, mlx_field *nl, mlx_field *nl_p, double dt, bool copy_prev,
+          bool swap_nl) {
+  check_step_inputs(c, {s, nl, s_p, nl_p});
+  if (c->p.visc < 5e-14) MLX_THROW(MLX_E_ARG, "abcn: inviscid case and no linear term in rhs. semi-implicit time adv is impossible");
+  mlx_field *svis = tmp(c, 10);
+  fcopy(c, svis, s);
+  k_bandmul(c, svis->e, svis->sc, 0, svis->off[0]);
+  double a = -2.0 / (dt * c->p.visc);
+  k_abcn_rhs(c, s->e, s->sc, s->e, s->sc, nl->e, nl->sc, nl_p->e, nl_p->sc, svis->e, dt, a, 1);
+  ihelm(c, s, a);
+  if (copy_prev) fcopy(c, s_p, s);
+  if (swap_nl) fmove(nl_p, nl);
+  else fcopy(c, nl_p, nl);
+}
+
+// tp2vec (vector_reconstruction, mlegs_scalar_ops.f90:1455-1545)
+void tp2vec(mlx_ctx *c, const mlx_field *psi, const mlx_field *chi, mlx_field *vr, mlx_field *vp, mlx_field *vz) {
+  need_kit(c);
+  if (!is_fff(psi) || !is_fff(chi)) MLX_THROW(MLX_E_LAYOUT, "vector_projection: psi and chi must be in FFF");
+  mlx_field *ur = tmp(c, 0), *up = tmp(c, 1), *uz = tmp(c, 2);
+  fcopy(c, ur, chi);
+  ur->off[0] = 3;
+  fcopy(c, up, psi);
+  up->off[0] = 3;
+  fcopy(c, uz, chi);
+  uz->off[0] = 3;
+  k_bandmul(c, ur->e, ur->sc, 2, 3);
+  k_bandmul(c, up->e, up->sc, 2, 3);
+  k_tp2vec_combine(c, ur->e, up->e, psi->e, uz->e, 3);
+  ur->off[0] = 0;
+  up->off[0] = 0;
+  chop(c, ur);
+  chop(c, up);
+  trans(c, ur, "PPP");
+  trans(c, up, "PPP");
+  k_scale_rows_r(c, ur->e, 1);
+  k_scale_rows_r(c, up->e, 1);
+  k_bandmul(c, uz->e, uz->sc, 1, 3);
+  k_axpby(c, uz->e, uz->sc, -1.0, uz->e, uz->sc, 0.0);
+  uz->off[0] = 0;
+  chop(c, uz);
+  trans(c, uz, "PPP");
+  fmove(vr, ur);
+  fmove(vp, up);
+  fmove(vz, uz);
+}
+
+void tp2curlvec(mlx_ctx *c, const mlx_field *psi, const mlx_field *chi, mlx_field *wr, mlx_field *wp,
+                mlx_field *wz) {
+  need_kit(c);
+  mlx_field *md = tmp(c, 3);
+  fcopy(c, md, chi);
+  bandmul(c, md, 0);
+  k_axpby(c, md->e, md->sc, -1.0, md->e, md->sc, 0.0);
+  k_set_scalar(c, md->sc, 0.0);
+  tp2vec(c, md, psi, wr, wp, wz);
+}
+
+void vecprod(mlx_ctx *c, mlx_field *vr, mlx_field *vp, mlx_field *vz, const mlx_field *ur, const mlx_field *up,
+             const mlx_field *uz) {
+  for (const mlx_field *f : {(const mlx_field *)vr, (const mlx_field *)vp, (const mlx_field *)vz, ur, up, uz})
+    if (!is_ppp(f)) MLX_THROW(MLX_E_LAYOUT, "vector_product: to compute v x u, all inputs must be in PPP");
+  k_vecprod(c, vr->e, vp->e, vz->e, ur->e, up->e, uz->e);
+  for (mlx_field *f : {vr, vp, vz}) {
+    k_set_scalar(c, f->sc, 0.0);
+    f->off[0] = f->off[1] = f->off[2] = 0;
+  }
+}
+
+// vec2tp (vector_projection, mlegs_scalar_ops.f90:1308-1453)
+void vec2tp(mlx_ctx *c, const mlx_field *vr, const mlx_field *vp, const mlx_field *vz, mlx_field *psi,
+            mlx_field *chi) {
+  need_kit(c);
+  for (const mlx_field *f : {vr, vp, vz})
+    if (!is_ppp(f)) MLX_THROW(MLX_E_LAYOUT, "vector_projection: vr, vp, vz must be in PPP");
+  const KitDev &k = c->kit;
+  mlx_field *ur = tmp(c, 0), *up = tmp(c, 1), *uz = tmp(c, 2);
+  fcopy(c, ur, vr);
+  fcopy(c, up, vp);
+  fcopy(c, uz, vz);
+  k_set_scalar(c, psi->sc, 0.0);
+  k_set_scalar(c, chi->sc, 0.0);
+  k_scale_rows_r(c, ur->e, 0);
+  k_scale_rows_r(c, up->e, 0);
+  trans(c, ur, "PFP");
+  if (k.nz > 1) fft_z(c, ur->e, -1);
+  trans(c, up, "FFF");
+  double2 *inf = reinterpret_cast<double2 *>(c->dsmall + 2048);
+  k_calcat(c, up->e, k.at1, k.nrchop + up->off[0], inf);
+  leg_backward(c, up->e, c->scr[0], up->off[0]);
+  swap_scr(c, up);
+  k_vec2tp_up_ln(c, up->e, inf, psi->sc);
+  trans(c, uz, "PFP");
+  if (k.nz > 1) fft_z(c, uz->e, -1);
+  vec2tp_project(c, ur->e, up->e, uz->e, psi->e, chi->e, 3);
+  set_space(psi, "FFF");
+  set_space(chi, "FFF");
+  idel2_proln(c, chi);
+  psi->off[0] = 0;
+  chi->off[0] = 0;
+  chop(c, psi);
+  chop(c, chi);
+  zeroat1(c, psi);
+  zeroat1(c, chi);
+}
+
+// advection_rhs (vortical_flow_3d.f90:353-395)
+void advection_rhs(mlx_ctx *c, mlx_field *psi, mlx_field *chi, mlx_field *nlpsi, mlx_field *nlchi,
+                   const mlx_field *uz) {
+  if (!is_fff(psi) || !is_fff(chi)) MLX_THROW(MLX_E_LAYOUT, "advection_rhs: psi and chi must be in FFF");
+  if (!is_ppp(uz)) MLX_THROW(MLX_E_LAYOUT, "advection_rhs: uz must be in PPP");
+  dealias(c, psi);
+  dealias(c, chi);
+  mlx_field *vr = tmp(c, 4), *vp = tmp(c, 5), *vz = tmp(c, 6);
+  mlx_field *wr = tmp(c, 7), *wp = tmp(c, 8), *wz = tmp(c, 9);
+  tp2vec(c, psi, chi, vr, vp, vz);
+  k_axpby(c, vz->e, vz->sc, 1.0, uz->e, nullptr, 1.0);
+  tp2curlvec(c, psi, chi, wr, wp, wz);
+  vecprod(c, vr, vp, vz, wr, wp, wz);
+  vec2tp(c, vr, vp, vz, nlpsi, nlchi);
+  k_set_scalar(c, nlpsi->sc, 0.0);
+  k_set_scalar(c, nlchi->sc, 0.0);
+  dealias(c, nlpsi);
+  dealias(c, nlchi);
+}
+
+void svv(mlx_ctx *c, mlx_field *f, double *gain_dev) {
+  if (!c->p.is_svv) return;
+  if (!is_fff(f)) MLX_THROW(MLX_E_LAYOUT, "svv_filter: scalar must be in FFF space");
+  need_kit(c);
+  k_svv(c, f->e, gain_dev);
+}
+
+void check_finite(mlx_ctx *c, const mlx_field *a, const mlx_field *b, int *flag_dev) {
+  k_nonfinite(c, a->e, b->e, flag_dev);
+}
+
+template <class F>
+int guarded(mlx_ctx *c, F &&fn) {
+  if (!c) return MLX_E_ARG;
+  try {
+    fn();
+    return MLX_OK;
+  } catch (const MlxError &e) {
+    c->err = e.msg;
+    return e.code;
+  } catch (const std::exception &e) {
+    c->err = e.what();
+    return MLX_E_ARG;
+  }
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// vortex run state
+// ---------------------------------------------------------------------------
+struct mlx_vortex {
+  mlx_field *f[9] = {nullptr};  // psi chi nlpsi nlchi psi_prev chi_prev nlpsi_prev nlchi_prev uz
+  double *gains = nullptr;      // device [2]
+  int *flag = nullptr;          // device non-finite flag
+  bool prev_stale = false;      // psi_prev/chi_prev equal psi/chi but were not copied
+};
+
+extern "C" {
+
+int mlx_ctx_create(const mlx_params *p, int device, void *stream, mlx_ctx **out) {
+  if (!p || !out) return MLX_E_ARG;
+  *out = nullptr;
+  mlx_ctx *c = new mlx_ctx();
+  c->p = *p;
+  c->device = device;
+  int rc = guarded(c, [&] {
+    HIPCHK(hipSetDevice(device));
+    if (stream) {
+      c->stream = reinterpret_cast<hipStream_t>(stream);
+    } else {
+      HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+      c->own_stream = true;
+    }
+    HIPCHK(hipMalloc(&c->dsmall, sizeof(double) * 4096));
+    HIPCHK(hipHostMalloc(&c->hsmall, sizeof(double) * 4096));
+  });
+  if (rc != MLX_OK) {
+    delete c;
+    return rc;
+  }
+  *out = c;
+  return MLX_OK;
+}
+
+int mlx_ctx_destroy(mlx_ctx *c) {
+  if (!c) return MLX_OK;
+  (void)hipSetDevice(c->device);
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  for (mlx_field *f : c->pool) field_free(f);
+  c->pool.clear();
+  for (auto &e : c->lu) {
+    (void)hipFree(e.fac);
+    (void)hipFree(e.piv);
+  }
+  for (auto &kv : c->work)
+    if (kv.second.d) (void)hipFree(kv.second.d);
+  for (double2 *s : c->scr)
+    if (s) (void)hipFree(s);
+  kit_free(c->kit);
+  if (c->dsmall) (void)hipFree(c->dsmall);
+  if (c->hsmall) (void)hipHostFree(c->hsmall);
+  if (c->own_stream && c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+  return MLX_OK;
+}
+
+const char *mlx_last_error(mlx_ctx *c) { return c ? c->err.c_str() : "null context"; }
+
+int mlx_sync(mlx_ctx *c) {
+  return guarded(c, [&] { HIPCHK(hipStreamSynchronize(c->stream)); });
+}
+
+int mlx_kit_init(mlx_ctx *c) {
+  return guarded(c, [&] {
+    const mlx_params &p = c->p;
+    auto ok235 = [](int n) {
+      while (n % 2 == 0 || n % 3 == 0 || n % 5 == 0) {
+        if (n % 2 == 0) n /= 2;
+        if (n % 3 == 0) n /= 3;
+        if (n % 5 == 0) n /= 5;
+      }
+      return n == 1;
+    };
+    // tfm_kit_init checks (mlegs_spectfm_init.f90:35-62)
+    if (!(p.nr > 0 && p.nr % 2 == 0)) MLX_THROW(MLX_E_ARG, "tfm_kit_init: nr must be even");
+    if (!(p.np > 0 && (p.np == 1 || p.np % 2 == 0))) MLX_THROW(MLX_E_ARG, "tfm_kit_init: np must be even");
+    if (p.np != 1 && !ok235(p.np)) MLX_THROW(MLX_E_ARG, "tfm_kit_init: np must only have factors of 2, 3 and 5");
+    if (!(p.nz > 0 && (p.nz == 1 || p.nz % 2 == 0))) MLX_THROW(MLX_E_ARG, "tfm_kit_init: nz must be even");
+    if (p.nz != 1 && !ok235(p.nz)) MLX_THROW(MLX_E_ARG, "tfm_kit_init: nz must only have factors of 2, 3 and 5");
+    if (p.nrchop > p.nr) MLX_THROW(MLX_E_ARG, "tfm_kit_init: nrchop must be smaller than or equal to nr");
+    if (p.np != 1 && p.npchop * 2 > p.np + 2) MLX_THROW(MLX_E_ARG, "tfm_kit_init: npchop <= np/2 + 1 must be satisfied");
+    if (p.nz != 1 && p.nzchop * 2 > p.nz + 2) MLX_THROW(MLX_E_ARG, "tfm_kit_init: nzchop <= nz/2 + 1 must be satisfied");
+    if (p.nrchop < 1 || p.npchop < 1 || p.nzchop < 1) MLX_THROW(MLX_E_ARG, "tfm_kit_init: chops must be positive");
+    if (p.np == 1 || p.npchop > p.np / 2 + 1) MLX_THROW(MLX_E_ARG, "tfm_kit_init: this path needs the 2D/3D kit (np > 1)");
+    HIPCHK(hipSetDevice(c->device));
+    kit_build(c);
+    const KitDev &k = c->kit;
+    c->nelem = (size_t)k.nrdim * k.npdim * k.nzdim;
+    for (double2 *&s : c->scr) {
+      if (s) (void)hipFree(s);
+      if (hipMalloc(&s, sizeof(double2) * c->nelem) != hipSuccess) MLX_THROW(MLX_E_ALLOC, "scratch allocation failed");
+      HIPCHK(hipMemsetAsync(s, 0, sizeof(double2) * c->nelem, c->stream));
+    }
+  });
+}
+
+int mlx_kit_dims(mlx_ctx *c, int32_t *o) {
+  return guarded(c, [&] {
+    need_kit(c);
+    const KitDev &k = c->kit;
+    int32_t v[9] = {k.nrdim, k.npdim, k.nzdim, k.nrh, k.ne, k.chopzl, k.chopzu, k.nph, k.npchop};
+    std::memcpy(o, v, sizeof(v));
+  });
+}
+
+int mlx_kit_get(mlx_ctx *c, const char *which, double *host, int64_t n) {
+  return guarded(c, [&] {
+    need_kit(c);
+    const KitDev &k = c->kit;
+    const double *src = nullptr;
+    size_t cnt = 0;
+    std::string w(which ? which : "");
+    if (w == "x") src = k.x, cnt = k.nr;
+    else if (w == "w") src = k.w, cnt = k.nr;
+    else if (w == "ln") src = k.lnx, cnt = k.nr;
+    else if (w == "r") src = k.r, cnt = k.nr;
+    else if (w == "ak") src = k.ak, cnt = k.nz;
+    else if (w == "lognorm") src = k.lognorm, cnt = (size_t)k.ne * k.npchop;
+    else if (w == "pf") src = k.pf, cnt = (size_t)k.nrh * k.ne * k.npchop;
+    else if (w == "at0") src = k.at0, cnt = k.nrchop;
+    else if (w == "at1") src = k.at1, cnt = k.nrchop;
+    else MLX_THROW(MLX_E_ARG, "mlx_kit_get: unknown array " + w);
+    if ((size_t)n < cnt) MLX_THROW(MLX_E_ARG, "mlx_kit_get: host buffer too small");
+    HIPCHK(hipStreamSynchronize(c->stream));
+    HIPCHK(hipMemcpy(host, src, sizeof(double) * cnt, hipMemcpyDeviceToHost));
+  });
+}
+
+int mlx_set_precision(mlx_ctx *c, int prec) {
+  return guarded(c, [&] {
+    if (prec != MLX_PREC_FP64 && prec != MLX_PREC_FP32 && prec != MLX_PREC_BF16X3)
+      MLX_THROW(MLX_E_ARG, "mlx_set_precision: unknown precision");
+    c->prec = prec;
+  });
+}
+
+int mlx_field_create(mlx_ctx *c, mlx_field **out) {
+  return guarded(c, [&] {
+    need_kit(c);
+    *out = field_new(c);
+  });
+}
+
+int mlx_field_destroy(mlx_ctx *c, mlx_field *f) {
+  return guarded(c, [&] {
+    HIPCHK(hipStreamSynchronize(c->stream));
+    field_free(f);
+  });
+}
+
+int mlx_field_copy(mlx_ctx *c, mlx_field *d, const mlx_field *s) {
+  return guarded(c, [&] { fcopy(c, d, s); });
+}
+
+static void check_dims(mlx_ctx *c, int64_t a, int64_t b, int64_t d) {
+  const KitDev &k = c->kit;
+  if (a != k.nrdim || b != k.npdim || d != k.nzdim)
+    MLX_THROW(MLX_E_LAYOUT, "scalar dimension incompatible with the dimension defined in the kit");
+}
+
+int mlx_field_upload(mlx_ctx *c, mlx_field *f, const double *host, int64_t nrdim, int64_t npdim, int64_t nzdim,
+                     const char *space, double ln, const int32_t *offsets) {
+  return guarded(c, [&] {
+    need_kit(c);
+    check_dims(c, nrdim, npdim, nzdim);
+    if (space_id(space) < 0) MLX_THROW(MLX_E_ARG, "upload: space must be PPP, PFP, FFP or FFF");
+    HIPCHK(hipMemcpyAsync(f->e, host, sizeof(double2) * c->nelem, hipMemcpyHostToDevice, c->stream));
+    c->hsmall[0] = ln;
+    HIPCHK(hipMemcpyAsync(f->sc, c->hsmall, sizeof(double), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    set_space(f, space);
+    for (int i = 0; i < 3; ++i) f->off[i] = offsets ? offsets[i] : 0;
+  });
+}
+
+int mlx_field_download(mlx_ctx *c, const mlx_field *f, double *host, int64_t nrdim, int64_t npdim, int64_t nzdim,
+                       char *space4, double *ln, int32_t *offsets) {
+  return guarded(c, [&] {
+    need_kit(c);
+    check_dims(c, nrdim, npdim, nzdim);
+    if (host) HIPCHK(hipMemcpyAsync(host, f->e, sizeof(double2) * c->nelem, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipMemcpyAsync(c->hsmall, f->sc, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (ln) *ln = c->hsmall[0];
+    if (space4) std::memcpy(space4, f->space, 4);
+    if (offsets)
+      for (int i = 0; i < 3; ++i) offsets[i] = f->off[i];
+  });
+}
+
+int mlx_field_set_meta(mlx_ctx *c, mlx_field *f, const char *space, double ln, const int32_t *offsets) {
+  return guarded(c, [&] {
+    if (space_id(space) < 0) MLX_THROW(MLX_E_ARG, "set_meta: space must be PPP, PFP, FFP or FFF");
+    set_space(f, space);
+    k_set_scalar(c, f->sc, ln);
+    if (offsets)
+      for (int i = 0; i < 3; ++i) f->off[i] = offsets[i];
+  });
+}
+
+int mlx_field_get_meta(mlx_ctx *c, const mlx_field *f, char *space4, double *ln, int32_t *offsets) {
+  return mlx_field_download(c, f, nullptr, c->kit.nrdim, c->kit.npdim, c->kit.nzdim, space4, ln, offsets);
+}
+
+int mlx_chop_offset(mlx_ctx *c, mlx_field *f, int32_t a, int32_t b, int32_t d) {
+  return guarded(c, [&] {
+    if (b != 0 || d != 0) MLX_THROW(MLX_E_ARG, "chop_offset: only radial offsets are supported on this path");
+    f->off[0] = a;
+    f->off[1] = b;
+    f->off[2] = d;
+  });
+}
+
+int mlx_field_axpby(mlx_ctx *c, mlx_field *d, double a, const mlx_field *s, double b) {
+  return guarded(c, [&] { k_axpby(c, d->e, d->sc, a, s->e, s->sc, b); });
+}
+
+int mlx_trans(mlx_ctx *c, mlx_field *f, const char *space) {
+  return guarded(c, [&] { trans(c, f, space); });
+}
+int mlx_chop(mlx_ctx *c, mlx_field *f) {
+  return guarded(c, [&] {
+    need_kit(c);
+    chop(c, f);
+  });
+}
+int mlx_dealias(mlx_ctx *c, mlx_field *f) {
+  return guarded(c, [&] { dealias(c, f); });
+}
+
+int mlx_svv_filter(mlx_ctx *c, mlx_field *f, double *gain) {
+  return guarded(c, [&] {
+    if (!c->p.is_svv) return;
+    double *g = c->dsmall + 4000 + 8;
+    c->hsmall[0] = *gain;
+    HIPCHK(hipMemcpyAsync(g, c->hsmall, sizeof(double), hipMemcpyHostToDevice, c->stream));
+    svv(c, f, g);
+    HIPCHK(hipMemcpyAsync(c->hsmall, g, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    *gain = c->hsmall[0];
+  });
+}
+
+static void calcat_host(mlx_ctx *c, const mlx_field *f, const double *tbl, double *out) {
+  need_kit(c);
+  const mlx_field *src = f;
+  if (!is_fff(f)) {
+    mlx_field *st = tmp(c, 11);
+    fcopy(c, st, f);
+    trans(c, st, "FFF");
+    src = st;
+  }
+  double2 *d = reinterpret_cast<double2 *>(c->dsmall + 2048);
+  k_calcat(c, src->e, tbl, c->kit.nrchop + src->off[0], d);
+  HIPCHK(hipMemcpyAsync(out, d, sizeof(double2) * c->kit.nzdim, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+}
+
+int mlx_calcat0(mlx_ctx *c, const mlx_field *f, double *out) {
+  return guarded(c, [&] { calcat_host(c, f, c->kit.at0, out); });
+}
+int mlx_calcat1(mlx_ctx *c, const mlx_field *f, double *out) {
+  return guarded(c, [&] { calcat_host(c, f, c->kit.at1, out); });
+}
+int mlx_zeroat1(mlx_ctx *c, mlx_field *f) {
+  return guarded(c, [&] {
+    need_kit(c);
+    zeroat1(c, f);
+  });
+}
+
+int mlx_delsqp(mlx_ctx *c, mlx_field *f) {
+  return guarded(c, [&] {
+    need_kit(c);
+    to_fff(c, f);
+    k_delsqp(c, f->e, f->sc, f->off[0], 0);
+  });
+}
+int mlx_idelsqp(mlx_ctx *c, mlx_field *f) {
+  return guarded(c, [&] {
+    need_kit(c);
+    to_fff(c, f);
+    k_delsqp(c, f->e, f->sc, f->off[0], 1);
+  });
+}
+int mlx_xxdx(mlx_ctx *c, mlx_field *f) {
+  return guarded(c, [&] { bandmul(c, f, 2); });
+}
+int mlx_del2h(mlx_ctx *c, mlx_field *f) {
+  return guarded(c, [&] { bandmul(c, f, 1); });
+}
+int mlx_del2(mlx_ctx *c, mlx_field *f) {
+  return guarded(c, [&] { bandmul(c, f, 0); });
+}
+int mlx_idel2(mlx_ctx *c, mlx_field *f) {
+  return guarded(c, [&] { idel2_proln(c, f); });
+}
+int mlx_idel2_preln(mlx_ctx *c, mlx_field *f, double preln) {
+  return guarded(c, [&] {
+    need_kit(c);
+    to_fff(c, f);
+    k_idel2_preln00(c, f->e, f->sc, preln, f->off[0]);
+    k_helm_solve(c, f->e, 0.0, 2, f->off[0]);
+  });
+}
+int mlx_ihelm(mlx_ctx *c, mlx_field *f, double alpha) {
+  return guarded(c, [&] { ihelm(c, f, alpha); });
+}
+
+int mlx_febe(mlx_ctx *c, mlx_field *s, const mlx_field *nl, double dt) {
+  return guarded(c, [&] {
+    need_kit(c);
+    febe(c, s, nl, dt);
+  });
+}
+int mlx_abcn(mlx_ctx *c, mlx_field *s, mlx_field *s_p, const mlx_field *nl, mlx_field *nl_p, double dt) {
+  return guarded(c, [&] {
+    need_kit(c);
+    abcn(c, s, s_p, const_cast<mlx_field *>(nl), nl_p, dt, true, false);
+  });
+}
+
+int mlx_vecprod(mlx_ctx *c, mlx_field *vr, mlx_field *vp, mlx_field *vz, const mlx_field *ur, const mlx_field *up,
+                const mlx_field *uz) {
+  return guarded(c, [&] {
+    need_kit(c);
+    vecprod(c, vr, vp, vz, ur, up, uz);
+  });
+}
+
+int mlx_vec2tp(mlx_ctx *c, const mlx_field *vr, const mlx_field *vp, const mlx_field *vz, mlx_field *psi,
+               mlx_field *chi) {
+  return guarded(c, [&] {
+    if (psi == chi || psi == vr || psi == vp || psi == vz || chi == vr || chi == vp || chi == vz)
+      MLX_THROW(MLX_E_ARG, "vec2tp: outputs must be distinct from the inputs");
+    vec2tp(c, vr, vp, vz, psi, chi);
+  });
+}
+
+int mlx_tp2vec(mlx_ctx *c, const mlx_field *psi, const mlx_field *chi, mlx_field *vr, mlx_field *vp,
+               mlx_field *vz) {
+  return guarded(c, [&] {
+    const mlx_field *ins[2] = {psi, chi};
+    for (const mlx_field *o : {(const mlx_field *)vr, (const mlx_field *)vp, (const mlx_field *)vz})
+      if (o == ins[0] || o == ins[1]) MLX_THROW(MLX_E_ARG, "tp2vec: outputs must be distinct from the inputs");
+    tp2vec(c, psi, chi, vr, vp, vz);
+  });
+}
+
+int mlx_tp2curlvec(mlx_ctx *c, const mlx_field *psi, const mlx_field *chi, mlx_field *wr, mlx_field *wp,
+                   mlx_field *wz) {
+  return guarded(c, [&] {
+    const mlx_field *ins[2] = {psi, chi};
+    for (const mlx_field *o : {(const mlx_field *)wr, (const mlx_field *)wp, (const mlx_field *)wz})
+      if (o == ins[0] || o == ins[1]) MLX_THROW(MLX_E_ARG, "tp2curlvec: outputs must be distinct from the inputs");
+    tp2curlvec(c, psi, chi, wr, wp, wz);
+  });
+}
+
+int mlx_advection_rhs(mlx_ctx *c, mlx_field *psi, mlx_field *chi, mlx_field *nlpsi, mlx_field *nlchi,
+                      const mlx_field *uz) {
+  return guarded(c, [&] {
+    need_kit(c);
+    advection_rhs(c, psi, chi, nlpsi, nlchi, uz);
+  });
+}
+
+int mlx_check_stability(mlx_ctx *c, const mlx_field *psi, const mlx_field *chi) {
+  return guarded(c, [&] {
+    int *flag = reinterpret_cast<int *>(c->dsmall + 4000 + 16);
+    HIPCHK(hipMemsetAsync(flag, 0, sizeof(int), c->stream));
+    check_finite(c, psi, chi, flag);
+    int h = 0;
+    HIPCHK(hipMemcpyAsync(&h, flag, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (h) MLX_THROW(MLX_E_NONFINITE, "ERROR: non-finite vortex state");
+  });
+}
+
+int mlx_vortex_create(mlx_ctx *c, const mlx_field *psi, const mlx_field *chi, const mlx_field *uz,
+                      mlx_vortex **out) {
+  return guarded(c, [&] {
+    need_kit(c);
+    if (!is_fff(psi) || !is_fff(chi) || !is_ppp(uz))
+      MLX_THROW(MLX_E_LAYOUT, "vortex: psi and chi must be in FFF and uz in PPP");
+    mlx_vortex *v = new mlx_vortex();
+    try {
+      for (int i = 0; i < 9; ++i) v->f[i] = field_new(c);
+      fcopy(c, v->f[0], psi);
+      fcopy(c, v->f[1], chi);
+      for (int i = 2; i < 8; ++i) set_space(v->f[i], "FFF");
+      fcopy(c, v->f[8], uz);
+      HIPCHK(hipMalloc(&v->gains, sizeof(double) * 2));
+      HIPCHK(hipMalloc(&v->flag, sizeof(int)));
+      HIPCHK(hipMemsetAsync(v->gains, 0, sizeof(double) * 2, c->stream));
+      HIPCHK(hipMemsetAsync(v->flag, 0, sizeof(int), c->stream));
+    } catch (...) {
+      for (mlx_field *f : v->f) field_free(f);
+      delete v;
+      throw;
+    }
+    *out = v;
+  });
+}
+
+int mlx_vortex_destroy(mlx_ctx *c, mlx_vortex *v) {
+  return guarded(c, [&] {
+    if (!v) return;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    for (mlx_field *f : v->f) field_free(f);
+    if (v->gains) (void)hipFree(v->gains);
+    if (v->flag) (void)hipFree(v->flag);
+    delete v;
+  });
+}
+
+// vortical_flow_3d.f90:116-147
+int mlx_vortex_bootstrap(mlx_ctx *c, mlx_vortex *v) {
+  return guarded(c, [&] {
+    need_kit(c);
+    mlx_field *psi = v->f[0], *chi = v->f[1], *nlpsi = v->f[2], *nlchi = v->f[3];
+    mlx_field *psi_prev = v->f[4], *chi_prev = v->f[5], *nlpsi_prev = v->f[6], *nlchi_prev = v->f[7];
+    mlx_field *uz = v->f[8];
+    double dt = c->p.dt;
+    advection_rhs(c, psi, chi, nlpsi, nlchi, uz);
+    fcopy(c, psi_prev, psi);
+    fcopy(c, nlpsi_prev, nlpsi);
+    fcopy(c, chi_prev, chi);
+    fcopy(c, nlchi_prev, nlchi);
+    mlx_field *psi_rich = tmp(c, 12), *chi_rich = tmp(c, 13);
+    fcopy(c, psi_rich, psi);
+    fcopy(c, chi_rich, chi);
+    advection_rhs(c, psi_rich, chi_rich, nlpsi, nlchi, uz);
+    febe(c, psi_rich, nlpsi, dt);
+    febe(c, chi_rich, nlchi, dt);
+    for (int i = 0; i < 2; ++i) {
+      advection_rhs(c, psi, chi, nlpsi, nlchi, uz);
+      febe(c, psi, nlpsi, dt / 2.0);
+      febe(c, chi, nlchi, dt / 2.0);
+    }
+    // psi%e = 2 psi%e - psi_rich%e (ln untouched)
+    k_axpby(c, psi->e, nullptr, 2.0 / 1.0, psi_rich->e, nullptr, -1.0 / 1.0);
+    k_axpby(c, chi->e, nullptr, 2.0 / 1.0, chi_rich->e, nullptr, -1.0 / 1.0);
+    dealias(c, psi);
+    dealias(c, chi);
+    svv(c, psi, v->gains);
+    svv(c, chi, v->gains + 1);
+    zeroat1(c, psi);
+    zeroat1(c, chi);
+    advection_rhs(c, psi, chi, nlpsi, nlchi, uz);
+    v->prev_stale = false;
+  });
+}
+
+// vortical_flow_3d.f90:160-181
+int mlx_vortex_step(mlx_ctx *c, mlx_vortex *v, int32_t nsteps, double dt, int32_t check) {
+  return guarded(c, [&] {
+    need_kit(c);
+    mlx_field *psi = v->f[0], *chi = v->f[1], *nlpsi = v->f[2], *nlchi = v->f[3];
+    mlx_field *psi_prev = v->f[4], *chi_prev = v->f[5], *nlpsi_prev = v->f[6], *nlchi_prev = v->f[7];
+    mlx_field *uz = v->f[8];
+    for (int it = 0; it < nsteps; ++it) {
+      abcn(c, psi, psi_prev, nlpsi, nlpsi_prev, dt, false, true);
+      abcn(c, chi, chi_prev, nlchi, nlchi_prev, dt, false, true);
+      v->prev_stale = true;
+      dealias(c, psi);
+      dealias(c, chi);
+      svv(c, psi, v->gains);
+      svv(c, chi, v->gains + 1);
+      zeroat1(c, psi);
+      zeroat1(c, chi);
+      advection_rhs(c, psi, chi, nlpsi, nlchi, uz);
+      check_finite(c, psi, chi, v->flag);
+    }
+    if (check) {
+      int h = 0;
+      HIPCHK(hipMemcpyAsync(&h, v->flag, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+      HIPCHK(hipStreamSynchronize(c->stream));
+      if (h) MLX_THROW(MLX_E_NONFINITE, "ERROR: non-finite vortex state");
+    }
+  });
+}
+
+int mlx_vortex_field(mlx_ctx *c, mlx_vortex *v, int32_t which, mlx_field **out) {
+  return guarded(c, [&] {
+    if (which < 0 || which > 8) MLX_THROW(MLX_E_ARG, "mlx_vortex_field: index out of range");
+    if (v->prev_stale) {
+      // psi_prev <- psi and chi_prev <- chi after abcn (mlegs_scalar_ops.f90:1256)
+      fcopy(c, v->f[4], v->f[0]);
+      fcopy(c, v->f[5], v->f[1]);
+      v->prev_stale = false;
+    }
+    *out = v->f[which];
+  });
+}
+
+int mlx_vortex_gains(mlx_ctx *c, mlx_vortex *v, double *gp, double *gc) {
+  return guarded(c, [&] {
+    HIPCHK(hipMemcpyAsync(c->hsmall, v->gains, sizeof(double) * 2, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (gp) *gp = c->hsmall[0];
+    if (gc) *gc = c->hsmall[1];
+  });
+}
+
+int mlx_energy(mlx_ctx *c, const mlx_field *vr, const mlx_field *vp, const mlx_field *vz, double *out) {
+  return guarded(c, [&] {
+    need_kit(c);
+    for (const mlx_field *f : {vr, vp, vz})
+      if (!is_ppp(f)) MLX_THROW(MLX_E_LAYOUT, "energy: velocity components must be in PPP");
+    double *d = c->dsmall + 4000 + 24;
+    k_energy(c, vr->e, vp->e, vz->e, d);
+    HIPCHK(hipMemcpyAsync(c->hsmall, d, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    *out = c->hsmall[0];
+  });
+}
+
+int mlx_time_kernel(mlx_ctx *c, mlx_field *f, const char *which, int32_t iters, double *ms) {
+  return guarded(c, [&] {
+    need_kit(c);
+    std::string w(which ? which : "");
+    auto launch = [&]() {
+      if (w == "leg_fwd") leg_forward(c, f->e, c->scr[0], 0, false);
+      else if (w == "leg_bwd") leg_backward(c, f->e, c->scr[0], 0, false);
+      else if (w == "fft_phi_fwd") fft_phi(c, f->e, -1);
+      else if (w == "fft_phi_bwd") fft_phi(c, f->e, +1);
+      else if (w == "fft_z_fwd") fft_z(c, f->e, -1);
+      else if (w == "fft_z_bwd") fft_z(c, f->e, +1);
+      else MLX_THROW(MLX_E_ARG, "mlx_time_kernel: unknown kernel " + w);
+    };
+    if (iters < 1) MLX_THROW(MLX_E_ARG, "mlx_time_kernel: iters must be positive");
+    launch();
+    hipEvent_t a, b;
+    HIPCHK(hipEventCreate(&a));
+    HIPCHK(hipEventCreate(&b));
+    HIPCHK(hipEventRecord(a, c->stream));
+    for (int i = 0; i < iters; ++i) launch();
+    HIPCHK(hipEventRecord(b, c->stream));
+    HIPCHK(hipEventSynchronize(b));
+    float t = 0.f;
+    HIPCHK(hipEventElapsedTime(&t, a, b));
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    *ms = (double)t / iters;
+  });
+}
+
+}  // extern "C"

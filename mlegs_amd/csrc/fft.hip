@@ -1,0 +1,265 @@
+// phi and z Fourier transforms of the field, FP64, in LDS.
+//
+// Replaces horizontal_fft_forward/backward and vertical_fft_forward/backward
+// (src/submodules/mlegs_scalar_ops.f90:1567-1848), which call FFTE's
+// dzfft2d/zdfft2d/zfft1d per row.  Conventions kept from the reference:
+//   phi forward : real row f_0..f_{np-1} packed as e(i, j) = f_{2j} + i f_{2j+1}
+//                 -> C_m = (1/np) sum_t f_t e^{-2 pi i m t/np}, m = 0..np/2
+//   phi backward: f_t = sum_m C_m e^{+2 pi i m t/np} (Hermitian), packed back;
+//                 column np/2+1 (scratch in the reference) is written as 0
+//   z forward   : (1/nz) sum e^{-2 pi i jk/nz};  z backward: sum e^{+...}
+//   every row (including radial padding) and every column is transformed.
+//
+// Design: one workgroup owns TI consecutive radial rows i of one (k) plane
+// (phi) or one (j) column (z); it stages the TI lines in LDS with the line
+// index fastest (coalesced global access, conflict-free LDS), runs a
+// mixed-radix (4,2,3,5) Stockham FFT over the lines, and writes back.  The
+// real-to-complex phi transform is a complex FFT of length np/2 plus the
+// even/odd untangling step.  HBM-bound: one read and one write per element.
+#include "mlx_internal.h"
+
+namespace {
+
+constexpr int kThreads = 256;
+constexpr int kLdsElems = 4096;  // complex doubles per workgroup (64 KiB)
+
+struct Cx {
+  double x, y;
+};
+__device__ inline double2 cmul(double2 a, double2 b) {
+  return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ inline double2 cadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
+__device__ inline double2 csub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
+__device__ inline double2 cmuli(double2 a, double s) { return make_double2(-s * a.y, s * a.x); }  // i*s*a
+
+// DFT of size R on v with sign sg (-1 forward, +1 inverse)
+template <int R>
+__device__ inline void dft(double2 *v, double sg);
+
+template <>
+__device__ inline void dft<2>(double2 *v, double) {
+  double2 a = v[0], b = v[1];
+  v[0] = cadd(a, b);
+  v[1] = csub(a, b);
+}
+template <>
+__device__ inline void dft<4>(double2 *v, double sg) {
+  double2 a0 = cadd(v[0], v[2]), a1 = csub(v[0], v[2]);
+  double2 b0 = cadd(v[1], v[3]), b1 = csub(v[1], v[3]);
+  double2 ib1 = cmuli(b1, sg);  // (sg*i) * (v1 - v3)
+  v[0] = cadd(a0, b0);
+  v[2] = csub(a0, b0);
+  v[1] = cadd(a1, ib1);
+  v[3] = csub(a1, ib1);
+}
+template <>
+__device__ inline void dft<3>(double2 *v, double sg) {
+  const double c1 = -0.5, s1 = 0.86602540378443864676;
+  double2 t1 = cadd(v[1], v[2]);
+  double2 t2 = make_double2(v[0].x + c1 * t1.x, v[0].y + c1 * t1.y);
+  double2 t3 = cmuli(csub(v[1], v[2]), sg * s1);
+  v[0] = cadd(v[0], t1);
+  v[1] = cadd(t2, t3);
+  v[2] = csub(t2, t3);
+}
+template <>
+__device__ inline void dft<5>(double2 *v, double sg) {
+  const double c1 = 0.30901699437494742410, c2 = -0.80901699437494742410;
+  const double s1 = 0.95105651629515357212, s2 = 0.58778525229247312917;
+  double2 a14 = cadd(v[1], v[4]), b14 = csub(v[1], v[4]);
+  double2 a23 = cadd(v[2], v[3]), b23 = csub(v[2], v[3]);
+  double2 r1 = make_double2(v[0].x + c1 * a14.x + c2 * a23.x, v[0].y + c1 * a14.y + c2 * a23.y);
+  double2 r2 = make_double2(v[0].x + c2 * a14.x + c1 * a23.x, v[0].y + c2 * a14.y + c1 * a23.y);
+  double2 i1 = make_double2(s1 * b14.x + s2 * b23.x, s1 * b14.y + s2 * b23.y);
+  double2 i2 = make_double2(s2 * b14.x - s1 * b23.x, s2 * b14.y - s1 * b23.y);
+  double2 j1 = cmuli(i1, sg), j2 = cmuli(i2, sg);
+  v[0] = make_double2(v[0].x + a14.x + a23.x, v[0].y + a14.y + a23.y);
+  v[1] = cadd(r1, j1);
+  v[4] = csub(r1, j1);
+  v[2] = cadd(r2, j2);
+  v[3] = csub(r2, j2);
+}
+
+// One Stockham radix-R pass over TI lines of length N held in buf[t*TI + l].
+template <int R, int MAXB>
+__device__ inline void stockham_pass(double2 *buf, const double2 *tw, int N, int TI, int ns, double sg) {
+  const int NR = N / R;
+  const int total = TI * NR;
+  double2 v[MAXB][R];
+#pragma unroll
+  for (int q = 0; q < MAXB; ++q) {
+    int b = threadIdx.x + q * kThreads;
+    if (b < total) {
+      int l = b % TI, j = b / TI;
+#pragma unroll
+      for (int r = 0; r < R; ++r) v[q][r] = buf[(j + r * NR) * TI + l];
+    }
+  }
+  __syncthreads();
+  const int step0 = N / (ns * R);
+#pragma unroll
+  for (int q = 0; q < MAXB; ++q) {
+    int b = threadIdx.x + q * kThreads;
+    if (b < total) {
+      int l = b % TI, j = b / TI;
+      int k = j % ns;
+#pragma unroll
+      for (int r = 1; r < R; ++r) {
+        double2 t = tw[r * k * step0];
+        if (sg > 0) t.y = -t.y;
+        v[q][r] = cmul(v[q][r], t);
+      }
+      dft<R>(v[q], sg);
+      int idxD = (j / ns) * ns * R + k;
+#pragma unroll
+      for (int r = 0; r < R; ++r) buf[(idxD + r * ns) * TI + l] = v[q][r];
+    }
+  }
+  __syncthreads();
+}
+
+__device__ void lds_fft(double2 *buf, const double2 *tw, const FftPlan &pl, int TI, double sg) {
+  int ns = 1;
+  for (int p = 0; p < pl.npass; ++p) {
+    int R = pl.radix[p];
+    if (R == 4) stockham_pass<4, kLdsElems / 4 / kThreads>(buf, tw, pl.n, TI, ns, sg);
+    else if (R == 2) stockham_pass<2, kLdsElems / 2 / kThreads>(buf, tw, pl.n, TI, ns, sg);
+    else if (R == 3) stockham_pass<3, (kLdsElems / 3 + kThreads - 1) / kThreads>(buf, tw, pl.n, TI, ns, sg);
+    else stockham_pass<5, (kLdsElems / 5 + kThreads - 1) / kThreads>(buf, tw, pl.n, TI, ns, sg);
+    ns *= R;
+  }
+}
+
+// phi transform; grid.x = row tiles, grid.y = z planes
+__global__ void __launch_bounds__(kThreads) k_fft_phi(double2 *e, int nrdim, int npdim, int nzdim, int TI,
+                                                      FftPlan pl, const double2 *twg, const double2 *twh,
+                                                      int dir) {
+  __shared__ double2 buf[kLdsElems];
+  __shared__ double2 tw[512];
+  const int N = pl.n;  // np/2
+  const int i0 = blockIdx.x * TI;
+  const int k = blockIdx.y;
+  const size_t plane = (size_t)nrdim * npdim * k;
+  for (int t = threadIdx.x; t < N; t += kThreads) tw[t] = twg[t];
+  if (dir < 0) {
+    // load packed reals z_j = e(i, j, k), j < N
+    for (int idx = threadIdx.x; idx < TI * N; idx += kThreads) {
+      int l = idx % TI, j = idx / TI;
+      int i = i0 + l;
+      double2 v = make_double2(0.0, 0.0);
+      if (i < nrdim) v = e[plane + (size_t)j * nrdim + i];
+      buf[j * TI + l] = v;
+    }
+    __syncthreads();
+    lds_fft(buf, tw, pl, TI, -1.0);
+    // untangle: C_m = E + W^m O, W = e^{-2 pi i/np}; scale 1/np
+    const double sc = 1.0 / (2.0 * N);
+    for (int idx = threadIdx.x; idx < TI * (N + 1); idx += kThreads) {
+      int l = idx % TI, mm = idx / TI;
+      int i = i0 + l;
+      double2 zm = buf[(mm % N) * TI + l];
+      double2 zc = buf[((N - mm) % N) * TI + l];
+      zc.y = -zc.y;
+      double2 E = make_double2(0.5 * (zm.x + zc.x), 0.5 * (zm.y + zc.y));
+      double2 D = make_double2(0.5 * (zm.x - zc.x), 0.5 * (zm.y - zc.y));
+      double2 O = make_double2(D.y, -D.x);  // -i * D
+      double2 C = cadd(E, cmul(twh[mm], O));
+      if (i < nrdim) e[plane + (size_t)mm * nrdim + i] = make_double2(C.x * sc, C.y * sc);
+    }
+  } else {
+    // Z_m = (C_m + conj C_{N-m}) + i (C_m - conj C_{N-m}) e^{+2 pi i m/np}
+    for (int idx = threadIdx.x; idx < TI * N; idx += kThreads) {
+      int l = idx % TI, mm = idx / TI;
+      int i = i0 + l;
+      double2 Z = make_double2(0.0, 0.0);
+      if (i < nrdim) {
+        double2 cm = e[plane + (size_t)mm * nrdim + i];
+        double2 cc = e[plane + (size_t)(N - mm) * nrdim + i];
+        if (mm == 0) { cm.y = 0.0; cc.y = 0.0; }  // imaginary parts of DC and Nyquist are ignored
+        cc.y = -cc.y;
+        double2 E = cadd(cm, cc);
+        double2 w = twh[mm];
+        w.y = -w.y;
+        double2 O = cmul(csub(cm, cc), w);
+        Z = make_double2(E.x - O.y, E.y + O.x);
+      }
+      buf[mm * TI + l] = Z;
+    }
+    __syncthreads();
+    lds_fft(buf, tw, pl, TI, +1.0);
+    for (int idx = threadIdx.x; idx < TI * (N + 1); idx += kThreads) {
+      int l = idx % TI, j = idx / TI;
+      int i = i0 + l;
+      if (i >= nrdim) continue;
+      double2 v = (j < N) ? buf[j * TI + l] : make_double2(0.0, 0.0);
+      e[plane + (size_t)j * nrdim + i] = v;
+    }
+  }
+}
+
+// z transform; grid.x = row tiles, grid.y = columns j
+__global__ void __launch_bounds__(kThreads) k_fft_z(double2 *e, int nrdim, int npdim, int nzdim, int TI,
+                                                    FftPlan pl, const double2 *twg, int dir) {
+  __shared__ double2 buf[kLdsElems];
+  __shared__ double2 tw[512];
+  const int N = pl.n;
+  const int i0 = blockIdx.x * TI;
+  const int j = blockIdx.y;
+  const size_t col = (size_t)nrdim * j;
+  const size_t zs = (size_t)nrdim * npdim;
+  for (int t = threadIdx.x; t < N; t += kThreads) tw[t] = twg[t];
+  for (int idx = threadIdx.x; idx < TI * N; idx += kThreads) {
+    int l = idx % TI, t = idx / TI;
+    int i = i0 + l;
+    double2 v = make_double2(0.0, 0.0);
+    if (i < nrdim) v = e[col + (size_t)t * zs + i];
+    buf[t * TI + l] = v;
+  }
+  __syncthreads();
+  lds_fft(buf, tw, pl, TI, dir < 0 ? -1.0 : 1.0);
+  const double sc = dir < 0 ? 1.0 / N : 1.0;
+  for (int idx = threadIdx.x; idx < TI * N; idx += kThreads) {
+    int l = idx % TI, t = idx / TI;
+    int i = i0 + l;
+    if (i >= nrdim) continue;
+    double2 v = buf[t * TI + l];
+    e[col + (size_t)t * zs + i] = make_double2(v.x * sc, v.y * sc);
+  }
+}
+
+int tile_rows(int n_line, int nrdim) {
+  int ti = kLdsElems / (n_line + 1);
+  if (ti > 64) ti = 64;
+  if (ti > nrdim) ti = nrdim;
+  if (ti < 1) ti = 1;
+  return ti;
+}
+
+}  // namespace
+
+namespace mlx {
+
+void fft_phi(mlx_ctx *c, double2 *e, int dir) {
+  const KitDev &k = c->kit;
+  if (k.np <= 1) return;
+  if (k.nph > 512) MLX_THROW(MLX_E_ARG, "fft_phi: np/2 above 512 is not supported");
+  int TI = tile_rows(k.nph, k.nrdim);
+  dim3 grid((k.nrdim + TI - 1) / TI, k.nzdim);
+  hipLaunchKernelGGL(k_fft_phi, grid, dim3(kThreads), 0, c->stream, e, k.nrdim, k.npdim, k.nzdim, TI, k.plan_nph,
+                     k.tw_nph, k.tw_half, dir);
+  HIPCHK(hipGetLastError());
+}
+
+void fft_z(mlx_ctx *c, double2 *e, int dir) {
+  const KitDev &k = c->kit;
+  if (k.nz <= 1) return;
+  if (k.nz > 512) MLX_THROW(MLX_E_ARG, "fft_z: nz above 512 is not supported");
+  int TI = tile_rows(k.nz, k.nrdim);
+  dim3 grid((k.nrdim + TI - 1) / TI, k.npdim);
+  hipLaunchKernelGGL(k_fft_z, grid, dim3(kThreads), 0, c->stream, e, k.nrdim, k.npdim, k.nzdim, TI, k.plan_nz,
+                     k.tw_nz, dir);
+  HIPCHK(hipGetLastError());
+}
+
+}  // namespace mlx

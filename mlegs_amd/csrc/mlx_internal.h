@@ -1,0 +1,167 @@
+// Internal declarations of the MI355X spectral-transform library.
+//
+// Device data layout (HBM):
+//   field  e(nrdim, npdim, nzdim) complex<double>, i fastest -- the reference's
+//          global layout, so host upload/download is a plain copy.
+//   kit    x, w, lnx, r (nr); ak (nz); lognorm[m][n] (ne per m);
+//          pf[m][n][x] (nrh x ne per m, x fastest: the contraction dimension
+//          of both Legendre directions is contiguous);
+//          band rows of del2 / xxdx per (m, n); projection tables of vec2tp
+//          V, D, T [m][n][x].
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/mlegs_amd.h"
+
+struct MlxError {
+  int code;
+  std::string msg;
+};
+
+#define MLX_THROW(code, msg) throw MlxError{(code), (msg)}
+#define HIPCHK(x)                                                                  \
+  do {                                                                             \
+    hipError_t e_ = (x);                                                           \
+    if (e_ != hipSuccess)                                                          \
+      throw MlxError{MLX_E_HIP, std::string(#x) + ": " + hipGetErrorString(e_)};   \
+  } while (0)
+
+// ---------------------------------------------------------------------------
+// kit
+// ---------------------------------------------------------------------------
+struct FftPlan {
+  int n;        // transform length
+  int npass;    // number of radix passes
+  int radix[16];
+};
+
+struct KitDev {
+  // sizes (mlegs_spectfm_init.f90:111-145)
+  int nr, np, nz, nrchop, npchop, nzchop, hyperpow;
+  int nrh, nrdim, npdim, nzdim, nph, ne, chopzl, chopzu;
+  int nt;  // rows per m of the projection tables (nrchop + 3)
+  double ell, zlen;
+  // device arrays
+  double *x = nullptr, *w = nullptr, *lnx = nullptr, *r = nullptr, *ak = nullptr;
+  double *lognorm = nullptr;  // [m][n]
+  double *pf = nullptr;       // [m][n][x]
+  float *pf32 = nullptr;      // FP32 copy of pf (same layout)
+  uint16_t *pfbf = nullptr;   // 3 split-BF16 planes of pf, plane stride = nrh*ne*npchop
+  double *at0 = nullptr, *at1 = nullptr;
+  double *d2c = nullptr;      // [m][n][5]: del2 band rows (offsets -2..2), /ell^2, no ak
+  double *xxc = nullptr;      // [m][n][2]: xxdx band rows (offsets -1, +1)
+  double *tV = nullptr, *tD = nullptr, *tT = nullptr;  // [m][n][x], nt rows per m
+  float *tV32 = nullptr, *tD32 = nullptr, *tT32 = nullptr;
+  double2 *tw_nph = nullptr, *tw_nz = nullptr, *tw_half = nullptr;
+  FftPlan plan_nph, plan_nz;
+  // host mirrors
+  std::vector<double> lognorm_h, at1_h, at0_h, ak_h, x_h, w_h;
+  bool ready = false;
+};
+
+// ---------------------------------------------------------------------------
+// fields
+// ---------------------------------------------------------------------------
+struct mlx_field {
+  double2 *e = nullptr;   // nrdim*npdim*nzdim complex
+  double *sc = nullptr;   // device scalars: sc[0] = ln
+  char space[4] = {'P', 'P', 'P', 0};
+  int off[3] = {0, 0, 0};
+};
+
+// cached LU factors of (del2 + alpha) per (m, k) system
+struct LuCache {
+  double alpha;
+  int kind;   // 0: del2 + alpha; 1: idel2 bordered (alpha = 0)
+  int nroff;  // radial chop offset the systems were built for
+  double *fac = nullptr;   // [sys][row][7]
+  int8_t *piv = nullptr;   // [sys][row]
+  uint64_t stamp = 0;
+};
+
+struct mlx_vortex;
+
+// device list of GEMM work items (m, parity, first row, row count)
+struct WorkTable {
+  int4 *d = nullptr;
+  int n = 0;
+};
+
+struct mlx_ctx {
+  std::map<long, WorkTable> work;
+  mlx_params p;
+  int device = 0;
+  hipStream_t stream = nullptr;
+  bool own_stream = false;
+  KitDev kit;
+  int prec = MLX_PREC_FP64;
+  std::string err;
+  size_t nelem = 0;
+  // scratch buffers of one field each, and small device scratch
+  double2 *scr[2] = {nullptr, nullptr};
+  double *dsmall = nullptr;     // 4096 doubles of reduction scratch
+  double *hsmall = nullptr;     // pinned host mirror (4096 doubles)
+  std::vector<mlx_field *> pool;  // temporaries (allocated on demand)
+  std::vector<LuCache> lu;
+  uint64_t lu_clock = 0;
+};
+
+// ---------------------------------------------------------------------------
+// launch helpers (defined in the .hip files)
+// ---------------------------------------------------------------------------
+namespace mlx {
+
+// kit.hip
+void kit_build(mlx_ctx *c);
+void kit_free(KitDev &k);
+
+// fft.hip
+void fft_phi(mlx_ctx *c, double2 *e, int dir);  // dir -1 forward (PPP->PFP), +1 backward
+void fft_z(mlx_ctx *c, double2 *e, int dir);    // dir -1 forward (/nz), +1 backward
+
+// legendre.hip
+void leg_forward(mlx_ctx *c, const double2 *in, double2 *out, int nroff, bool zero = true);
+void leg_backward(mlx_ctx *c, const double2 *in, double2 *out, int nroff, bool zero = true);
+void vec2tp_project(mlx_ctx *c, const double2 *ur, const double2 *up, const double2 *uz,
+                    double2 *psi, double2 *chi, int nroff);
+
+// spectral.hip
+void k_ln_column(mlx_ctx *c, double2 *e, const double *ln, double sign);
+void k_chop(mlx_ctx *c, double2 *e, const char *space, int nroff);
+void k_dealias(mlx_ctx *c, double2 *e, const char *space);
+void k_svv(mlx_ctx *c, double2 *e, double *gain_dev);
+void k_calcat(mlx_ctx *c, const double2 *e, const double *tbl, int nrc, double2 *out);
+void k_zeroat1(mlx_ctx *c, double2 *e, int nrc);
+void k_delsqp(mlx_ctx *c, double2 *e, double *ln, int nroff, int inverse);
+void k_bandmul(mlx_ctx *c, double2 *e, double *ln, int op, int nroff);  // op 0 del2, 1 del2h, 2 xxdx
+void k_helm_pre(mlx_ctx *c, double2 *e, double *ln_out, const double *ln_in, double alpha);
+void k_helm_solve(mlx_ctx *c, double2 *e, double alpha, int kind, int nroff);
+void k_idel2_post(mlx_ctx *c, const double2 *e, double *ln);
+void k_idel2_preln00(mlx_ctx *c, double2 *e, double *ln, double preln, int nroff);
+void k_axpby(mlx_ctx *c, double2 *dst, double *dln, double a, const double2 *src, const double *sln, double b);
+void k_abcn_rhs(mlx_ctx *c, double2 *sh, double *shln, const double2 *s, const double *sln,
+                const double2 *nl, const double *nlln, const double2 *nlp, const double *nlpln,
+                const double2 *svis, double dt, double a, int ab2);
+void k_scale_rows_r(mlx_ctx *c, double2 *e, int inverse);
+void k_tp2vec_combine(mlx_ctx *c, double2 *ur, double2 *up, const double2 *psi, const double2 *uz, int nroff);
+void k_vecprod(mlx_ctx *c, double2 *vr, double2 *vp, double2 *vz, const double2 *ur, const double2 *up,
+               const double2 *uz);
+void k_vec2tp_up_ln(mlx_ctx *c, double2 *up, const double2 *inf, double *psiln);
+void k_nonfinite(mlx_ctx *c, const double2 *a, const double2 *b, int *flag);
+void k_energy(mlx_ctx *c, const double2 *vr, const double2 *vp, const double2 *vz, double *out);
+void k_set_scalar(mlx_ctx *c, double *dst, double v);
+void k_copy_scalar(mlx_ctx *c, double *dst, const double *src, double scale);
+
+}  // namespace mlx
+
+// helpers shared by host code
+static inline int nrcs_of(const KitDev &k, int m, int nroff) {
+  int nrc = k.nrchop + nroff;
+  int v = nrc - m;
+  return v > 0 ? v : 0;
+}

@@ -1,0 +1,721 @@
+"""Scalar-field operations of the oracle (test infrastructure only).
+
+Restates ``src/submodules/mlegs_scalar_ops.f90`` for one process (every
+``loc_st`` is 0 and every ``exchange`` is a no-op on one address space).
+Fields use the reference's global layout e(nrdim, npdim, nzdim), complex128,
+here as numpy arrays of that shape (0-based indices).
+
+Banded solves use scipy's LAPACK ``gbsv`` (partial pivoting), the same
+factorisation as the reference's zgbtrf/zgbtrs (mlegs_bndmat_ops.f90:369-430).
+
+Every z-loop visits the reference's index lists exactly, including the quirk
+that slice k = nzchop is visited twice when nzchop = nz/2+1 (chopzl = chopzu):
+operators that are not idempotent (del2, del2h, xxdx, ihelm, idel2) are then
+applied twice to that slice, as the reference does.
+"""
+import numpy as np
+from scipy.linalg import solve_banded
+
+PI = np.pi
+IU = 1j
+
+
+class Scalar:
+    """type(scalar) on one rank (mlegs_scalar.f90:15-49)."""
+
+    def __init__(self, kit, space='PPP'):
+        self.e = np.zeros((kit.nrdim, kit.npdim, kit.nzdim), dtype=np.complex128)
+        self.ln = 0.0
+        self.nrchop_offset = 0
+        self.npchop_offset = 0
+        self.nzchop_offset = 0
+        self.space = space
+
+    def copy(self):
+        """Deep copy '=' (mlegs_scalar_init.f90:106-140)."""
+        o = Scalar.__new__(Scalar)
+        o.e = self.e.copy()
+        o.ln = self.ln
+        o.nrchop_offset = self.nrchop_offset
+        o.npchop_offset = self.npchop_offset
+        o.nzchop_offset = self.nzchop_offset
+        o.space = self.space
+        return o
+
+    def assign(self, other):
+        self.e = other.e.copy()
+        self.ln = other.ln
+        self.nrchop_offset = other.nrchop_offset
+        self.npchop_offset = other.npchop_offset
+        self.nzchop_offset = other.nzchop_offset
+        self.space = other.space
+
+    def chop_offset(self, iof1, iof2=0, iof3=0):
+        """scalar_chop_offset (mlegs_scalar_init.f90:84-103)."""
+        self.nrchop_offset = iof1
+        self.npchop_offset = iof2
+        self.nzchop_offset = iof3
+
+
+# ----------------------------------------------------------------------------
+# index helpers
+# ----------------------------------------------------------------------------
+def chop_index(s, kit):
+    """chop_index (mlegs_scalar_ops.f90:2012-2063), 3D kit."""
+    if s.npchop_offset != 0 or s.nzchop_offset != 0:
+        raise ValueError('chop_index: only radial chopping offsets are used on this path')
+    nrc = kit.chops[0] + s.nrchop_offset
+    npc = kit.chopp + s.npchop_offset
+    nzc = kit.chopzl + s.nzchop_offset
+    nzcu = kit.chopzu - s.nzchop_offset
+    if nrc > kit.nrdim:
+        raise ValueError('chop_index: chopping in r too large')
+    if npc > kit.npdim:
+        raise ValueError('chop_index: chopping in p too large')
+    nrcs = np.zeros(kit.npchop, dtype=np.int64)
+    for mm in range(npc):
+        nrcs[mm] = max(min(nrc, nrc - kit.m[mm]), 0)
+    return nrc, npc, nzc, nzcu, nrcs
+
+
+def zlist(kit, nzc, nzcu):
+    """0-based z slices in the order of the two kk-loops of del2 (:546-557)."""
+    nz = kit.nzdim
+    ks = list(range(0, min(nz, nzc)))
+    lo = max(nzcu, 1)
+    if lo <= nz:
+        ks += list(range(lo - 1, nz))
+    return ks
+
+
+# ----------------------------------------------------------------------------
+# masks
+# ----------------------------------------------------------------------------
+def chop(s, kit):
+    """chop (mlegs_scalar_ops.f90:6-41)."""
+    nrc, npc, nzc, nzcu, nrcs = chop_index(s, kit)
+    if s.space[0] == 'F':
+        for mm in range(s.e.shape[1]):
+            rc = nrcs[mm] + 1 if mm < len(nrcs) else 1
+            if 1 <= rc:
+                s.e[rc - 1:, mm, :] = 0.0
+            else:
+                s.e[:, mm, :] = 0.0
+    if s.space[1] == 'F':
+        s.e[:, npc:, :] = 0.0
+    if s.space[2] == 'F':
+        if nzc < nzcu:
+            s.e[:, :, nzc:nzcu - 1] = 0.0
+
+
+def dealias(s, kit):
+    """2/3 rule in phi and z (mlegs_scalar_ops.f90:43-70)."""
+    np_, nz = kit.np, kit.nz
+    if np_ > 1 and s.space[1] == 'F':
+        pcut = max(np_ // 3 + 1, 1)
+        s.e[:, pcut:, :] = 0.0
+    if nz > 1 and s.space[2] == 'F':
+        zcut = max(nz // 3 + 1, 1)
+        zupper = nz - zcut + 2
+        # 1-based slices zcut < k < zupper  ->  0-based zcut .. zupper-2
+        if zupper - 1 > zcut:
+            s.e[:, :, zcut:zupper - 1] = 0.0
+
+
+def svv_quality(kit, shape):
+    """q = min(1, max(q_r, q_p, q_z)) per entry (mlegs_scalar_ops.f90:95-115)."""
+    nrchop, np_ = kit.nrchop, kit.np
+    i = np.arange(shape[0], dtype=np.float64)
+    q_r = np.minimum(1.0, np.maximum(i, 0) / max(float(nrchop - 1), 1.0))
+    j = np.arange(shape[1], dtype=np.float64)
+    q_p = np.minimum(1.0, np.abs(j) / max(float(np_ // 2), 1.0))
+    kmax = max(np.max(np.abs(kit.ak)), 1.0)
+    q_z = np.minimum(1.0, np.abs(kit.ak[: shape[2]]) / kmax)
+    q = np.maximum(np.maximum(q_r[:, None, None], q_p[None, :, None]), q_z[None, None, :])
+    return np.minimum(1.0, q)
+
+
+def svv_filter(s, kit, gain, p):
+    """Dynamic spectral vanishing-viscosity filter (mlegs_scalar_ops.f90:72-155).
+
+    Returns the updated gain.
+    """
+    if not p.is_svv:
+        return gain
+    if s.space != 'FFF':
+        raise ValueError('svv_filter: scalar must be in FFF space')
+    a2 = np.abs(s.e) ** 2
+    total = float(np.sum(a2))
+    cutoff = min(max(p.svv_cutoff, 0.0), 0.99)
+    target = max(p.svv_target, 1e-12)
+    q = svv_quality(kit, s.e.shape)
+    tail = float(np.sum(a2[q >= cutoff]))
+    if total <= np.finfo(np.float64).tiny:
+        return gain
+    ratio = tail / total
+    feedback = min(max(ratio / target - 1.0, 0.0), 1.0)
+    relax = min(max(p.svv_relax, 0.0), 1.0)
+    gain = (1.0 - relax) * gain + relax * feedback
+    strength = min(max(p.svv_strength, 0.0) * gain, 1.0)
+    if strength <= 0.0:
+        return gain
+    sel = q > cutoff
+    shape = (q[sel] - cutoff) / (1.0 - cutoff)
+    s.e[sel] *= np.exp(-strength * shape ** 8)
+    return gain
+
+
+# ----------------------------------------------------------------------------
+# FFT kernels (FFTE sign/scale: forward e^{-}, /N; backward e^{+}, unscaled)
+# ----------------------------------------------------------------------------
+def horizontal_fft_forward(s, kit):
+    """Real FFT along phi of packed rows (mlegs_scalar_ops.f90:1567-1641)."""
+    nph = kit.np // 2
+    f = np.empty((s.e.shape[0], 2 * nph, s.e.shape[2]))
+    f[:, 0::2, :] = s.e[:, :nph, :].real
+    f[:, 1::2, :] = s.e[:, :nph, :].imag
+    c = np.fft.rfft(f, axis=1) / (2 * nph)
+    s.e[:, :nph + 1, :] = c
+
+
+def horizontal_fft_backward(s, kit):
+    """Hermitian synthesis along phi, packed output (mlegs_scalar_ops.f90:1645-1717).
+
+    Column np/2+1 is FFTE's scratch in the reference; here it is set to 0.
+    """
+    nph = kit.np // 2
+    f = np.fft.irfft(s.e[:, :nph + 1, :], n=2 * nph, axis=1) * (2 * nph)
+    s.e[:, :nph, :] = f[:, 0::2, :] + IU * f[:, 1::2, :]
+    s.e[:, nph, :] = 0.0
+
+
+def vertical_fft_forward(s, kit):
+    """(1/nz) sum e^{-2 pi i jk/nz} along z for every (row, col) (:1721-1783)."""
+    s.e[:, :, :] = np.fft.fft(s.e, axis=2) / kit.nz
+
+
+def vertical_fft_backward(s, kit):
+    """sum e^{+2 pi i jk/nz} along z for every (row, col) (:1787-1848)."""
+    s.e[:, :, :] = np.fft.ifft(s.e, axis=2) * kit.nz
+
+
+# ----------------------------------------------------------------------------
+# radial (mapped Legendre) transforms
+# ----------------------------------------------------------------------------
+def _nrcs_rtrans(s, kit):
+    nrc = kit.chops[0] + s.nrchop_offset
+    npc = kit.chopp + s.npchop_offset
+    if nrc > kit.nrdim:
+        raise ValueError('rtrans: chopping in r too large')
+    if npc > kit.npdim:
+        raise ValueError('rtrans: chopping in p too large')
+    nrcs = np.array([max(min(nrc, nrc - kit.m[mm]), 0) for mm in range(npc)], dtype=np.int64)
+    return npc, nrcs
+
+
+def rtrans_forward(s, kit):
+    """Gauss quadrature against P_n^m, parity folded (mlegs_scalar_ops.f90:1852-1927)."""
+    npc, nrcs = _nrcs_rtrans(s, kit)
+    nr, nrh = kit.nr, kit.nr // 2
+    se = np.zeros_like(s.e)
+    w = kit.w[:nrh, None]
+    for mm in range(npc):
+        a = s.e[:nrh, mm, :]
+        b = s.e[nr - 1:nrh - 1:-1, mm, :]
+        be = (a + b) * w
+        bo = (a - b) * w
+        nn = nrcs[mm]
+        pf = kit.pf[:, :, mm]
+        if nn >= 1:
+            se[0:nn:2, mm, :] = pf[:, 0:nn:2].T @ be
+        if nn >= 2:
+            se[1:nn:2, mm, :] = pf[:, 1:nn:2].T @ bo
+    s.e = se
+
+
+def rtrans_backward(s, kit):
+    """Synthesis sum_n e_n P_n^m(x), unfolded to +-x (mlegs_scalar_ops.f90:1931-2008)."""
+    npc, nrcs = _nrcs_rtrans(s, kit)
+    nr, nrh = kit.nr, kit.nr // 2
+    se = np.zeros_like(s.e)
+    for mm in range(npc):
+        nn = nrcs[mm]
+        pf = kit.pf[:, :, mm]
+        be = pf[:, 0:nn:2] @ s.e[0:nn:2, mm, :] if nn >= 1 else 0.0
+        bo = pf[:, 1:nn:2] @ s.e[1:nn:2, mm, :] if nn >= 2 else 0.0
+        se[:nrh, mm, :] = be + bo
+        se[nr - 1:nrh - 1:-1, mm, :] = be - bo
+    s.e = se
+
+
+_SPACES = {'PPP': 0, 'PFP': 1, 'FFP': 2, 'FFF': 3}
+
+
+def trans(s, space, kit):
+    """State machine PPP <-> PFP <-> FFP <-> FFF (mlegs_scalar_ops.f90:157-235)."""
+    if s.space not in _SPACES:
+        raise ValueError('trans: scalar space info corrupted (only accepting PPP, PFP, FFP and FFF)')
+    if space not in _SPACES:
+        raise ValueError('trans: only taking PPP, PFP, FFP and FFF for spectral transformation')
+    cur, new = _SPACES[s.space], _SPACES[space]
+    nr = kit.nr
+    if cur < new:
+        while True:
+            if cur == 0:
+                if kit.np > 1:
+                    horizontal_fft_forward(s, kit)
+                s.space = 'PFP'
+            if cur == 1:
+                s.e[:nr, 0, :] -= s.ln * kit.lnx[:nr, None]
+                rtrans_forward(s, kit)
+                s.space = 'FFP'
+            if cur == 2:
+                if kit.nz > 1:
+                    vertical_fft_forward(s, kit)
+                s.space = 'FFF'
+            if cur == 3:
+                break
+            cur += 1
+            if cur == new:
+                break
+    if cur > new:
+        while True:
+            if cur == 3:
+                if kit.nz > 1:
+                    vertical_fft_backward(s, kit)
+                s.space = 'FFP'
+            if cur == 2:
+                rtrans_backward(s, kit)
+                s.e[:nr, 0, :] += s.ln * kit.lnx[:nr, None]
+                s.space = 'PFP'
+            if cur == 1:
+                if kit.np > 1:
+                    horizontal_fft_backward(s, kit)
+                s.space = 'PPP'
+            if cur == 0:
+                break
+            cur -= 1
+            if cur == new:
+                break
+
+
+# ----------------------------------------------------------------------------
+# values at the origin / infinity
+# ----------------------------------------------------------------------------
+def _calcat(s, kit, tbl):
+    nrc, npc, nzc, nzcu, nrcs = chop_index(s, kit)
+    st = s.copy()
+    trans(st, 'FFF', kit)
+    n = min(st.e.shape[0], nrc)
+    return tbl[:n] @ st.e[:n, 0, :]
+
+
+def calcat0(s, kit):
+    """f(r=0) of the m=0 column per z slice (mlegs_scalar_ops.f90:237-272)."""
+    return _calcat(s, kit, kit.at0)
+
+
+def calcat1(s, kit):
+    """f(r=inf) of the m=0 column per z slice (mlegs_scalar_ops.f90:274-309)."""
+    return _calcat(s, kit, kit.at1)
+
+
+def zeroat1(s, kit):
+    """Remove the value at infinity through the n=0 coefficient (:311-325)."""
+    if s.space != 'FFF':
+        trans(s, 'FFF', kit)
+    at1 = calcat1(s, kit)
+    s.e[0, 0, :] -= at1 / kit.at1[0]
+
+
+# ----------------------------------------------------------------------------
+# diagonal operators
+# ----------------------------------------------------------------------------
+def delsqp(s, kit):
+    """(1-x)^-2 del_perp^2: diagonal -n(n+1)/ell^2 (mlegs_scalar_ops.f90:327-366)."""
+    nrc, npc, nzc, nzcu, nrcs = chop_index(s, kit)
+    if s.space != 'FFF':
+        trans(s, 'FFF', kit)
+    so = s.copy()
+    so.ln = 0.0
+    ell = kit.ell
+    for mm in range(npc):
+        nn = nrcs[mm]
+        n = kit.m[mm] + np.arange(nn)
+        so.e[:nn, mm, :] = -s.e[:nn, mm, :] * (n * (n + 1.0))[:, None] / (ell ** 2)
+    so.e[0, 0, 0] = s.ln / ell ** 2 / np.exp(kit.lognorm[0, 0])
+    s.assign(so)
+
+
+def idelsqp(s, kit):
+    """Inverse of delsqp with the log term (mlegs_scalar_ops.f90:368-416)."""
+    nrc, npc, nzc, nzcu, nrcs = chop_index(s, kit)
+    if s.space != 'FFF':
+        trans(s, 'FFF', kit)
+    so = s.copy()
+    ell = kit.ell
+    so.ln = float(np.real(s.e[0, 0, 0])) * ell ** 2 * np.exp(kit.lognorm[0, 0])
+    for mm in range(npc):
+        nn = nrcs[mm]
+        n = (kit.m[mm] + np.arange(nn)).astype(np.float64)
+        fac = np.where(n == 0, 0.0, -(ell ** 2) / np.where(n == 0, 1.0, n) / (n + 1.0))
+        so.e[:nn, mm, :] = s.e[:nn, mm, :] * fac[:, None]
+        if kit.m[mm] == 0 and nn >= 1:
+            so.e[0, mm, :] = 0.0
+    so.e[0, 0, :] = 0.0
+    s.assign(so)
+
+
+# ----------------------------------------------------------------------------
+# banded operators
+# ----------------------------------------------------------------------------
+def xxdx(s, kit):
+    """r d/dr (mlegs_scalar_ops.f90:418-463)."""
+    nrc, npc, nzc, nzcu, nrcs = chop_index(s, kit)
+    if s.space != 'FFF':
+        trans(s, 'FFF', kit)
+    so = s.copy()
+    so.ln = 0.0
+    ks = zlist(kit, nzc, nzcu)
+    for mm in range(npc):
+        nn = nrcs[mm]
+        X = kit.leg_xxdx(kit.m[mm], nn)
+        for k in ks:
+            so.e[:nn, mm, k] = X @ so.e[:nn, mm, k]
+    so.e[0, 0, 0] += 1.0 / np.exp(kit.lognorm[0, 0]) * s.ln
+    so.e[1, 0, 0] += 1.0 / np.exp(kit.lognorm[1, 0]) * s.ln
+    s.assign(so)
+
+
+def _lap_ln(so, kit, ln, sign=1.0):
+    ell2 = kit.ell ** 2
+    so.e[0, 0, 0] += sign * 4.0 / 3.0 / ell2 / np.exp(kit.lognorm[0, 0]) * ln
+    so.e[1, 0, 0] -= sign * 2.0 / 1.0 / ell2 / np.exp(kit.lognorm[1, 0]) * ln
+    so.e[2, 0, 0] += sign * 2.0 / 3.0 / ell2 / np.exp(kit.lognorm[2, 0]) * ln
+
+
+def del2h(s, kit):
+    """Horizontal Laplacian (mlegs_scalar_ops.f90:465-518)."""
+    nrc, npc, nzc, nzcu, nrcs = chop_index(s, kit)
+    if s.space != 'FFF':
+        trans(s, 'FFF', kit)
+    so = s.copy()
+    so.ln = 0.0
+    ks = zlist(kit, nzc, nzcu)
+    for mm in range(npc):
+        nn = nrcs[mm]
+        H = kit.leg_del2h(kit.m[mm], nn)
+        for k in ks:
+            so.e[:nn, mm, k] = H @ so.e[:nn, mm, k]
+    _lap_ln(so, kit, s.ln)
+    s.assign(so)
+
+
+def del2(s, kit):
+    """Laplacian (mlegs_scalar_ops.f90:520-573)."""
+    nrc, npc, nzc, nzcu, nrcs = chop_index(s, kit)
+    if s.space != 'FFF':
+        trans(s, 'FFF', kit)
+    so = s.copy()
+    so.ln = 0.0
+    ks = zlist(kit, nzc, nzcu)
+    for mm in range(npc):
+        nn = nrcs[mm]
+        for k in ks:
+            D = kit.leg_del2(kit.m[mm], kit.ak[k], nn)
+            so.e[:nn, mm, k] = D @ so.e[:nn, mm, k]
+    _lap_ln(so, kit, s.ln)
+    s.assign(so)
+
+
+def _band_solve(A, b, kl=2, ku=2):
+    """LAPACK gbsv on a dense matrix with known bandwidths."""
+    n = A.shape[0]
+    ab = np.zeros((kl + ku + 1, n))
+    for k in range(-kl, ku + 1):
+        d = np.diagonal(A, k)
+        j0 = max(k, 0)
+        ab[ku - k, j0:j0 + len(d)] = d
+    return solve_banded((kl, ku), ab, b)
+
+
+def ihelm(s, alpha, kit):
+    """Solve (del2 + alpha) x = s per (m, k) (mlegs_scalar_ops.f90:791-854)."""
+    nrc, npc, nzc, nzcu, nrcs = chop_index(s, kit)
+    if s.space != 'FFF':
+        trans(s, 'FFF', kit)
+    if abs(alpha) < 5e-14:
+        raise ValueError('ihelm: alpha equals to zero. Inversion of 0*identity is impossible')
+    so = s.copy()
+    so.ln = s.ln / alpha
+    _lap_ln(so, kit, so.ln, sign=-1.0)
+    ks = zlist(kit, nzc, nzcu)
+    for mm in range(npc):
+        nn = nrcs[mm]
+        for k in ks:
+            A = kit.leg_del2(kit.m[mm], kit.ak[k], nn) + alpha * np.eye(nn)
+            so.e[:nn, mm, k] = _band_solve(A, so.e[:nn, mm, k])
+    s.assign(so)
+
+
+def idel2_proln(s, kit):
+    """Inverse Laplacian, log coefficient from the (m,k)=(0,0) solve (:671-760)."""
+    nrc, npc, nzc, nzcu, nrcs = chop_index(s, kit)
+    if s.space != 'FFF':
+        trans(s, 'FFF', kit)
+    so = s.copy()
+    ell2 = kit.ell ** 2
+    ln = kit.lognorm[:, 0]
+    ks = zlist(kit, nzc, nzcu)
+    for mm in range(npc):
+        nn = nrcs[mm]
+        for idx, k in enumerate(ks):
+            A = kit.leg_del2(kit.m[mm], kit.ak[k], nn)
+            if mm == 0 and idx == 0 and k == 0:
+                A[0, 0] += 4.0 / 3.0 / ell2
+                A[1, 0] -= 2.0 / 1.0 / ell2 * np.exp(ln[0] - ln[1])
+                A[2, 0] += 2.0 / 3.0 / ell2 * np.exp(ln[0] - ln[2])
+                so.e[:nn, mm, k] = _band_solve(A, so.e[:nn, mm, k])
+                so.ln = float(np.real(so.e[0, mm, k])) * np.exp(ln[0])
+            else:
+                so.e[:nn, mm, k] = _band_solve(A, so.e[:nn, mm, k])
+    s.assign(so)
+
+
+def idel2_preln(s, kit, preln):
+    """Inverse Laplacian with a prescribed log coefficient (:575-669)."""
+    nrc, npc, nzc, nzcu, nrcs = chop_index(s, kit)
+    if s.space != 'FFF':
+        trans(s, 'FFF', kit)
+    so = s.copy()
+    ell2 = kit.ell ** 2
+    ln = kit.lognorm[:, 0]
+    ks = zlist(kit, nzc, nzcu)
+    for mm in range(npc):
+        nn = nrcs[mm]
+        for idx, k in enumerate(ks):
+            A = kit.leg_del2(kit.m[mm], kit.ak[k], nn)
+            if mm == 0 and idx == 0 and k == 0:
+                P = np.zeros((nn, nn))
+                P[1:nn, :] = A[0:nn - 1, :]
+                P[0, 0] = 1.0
+                P[1, 0] += 4.0 / 3.0 / ell2
+                P[2, 0] -= 2.0 / 1.0 / ell2 * np.exp(ln[0] - ln[1])
+                P[3, 0] += 2.0 / 3.0 / ell2 * np.exp(ln[0] - ln[2])
+                b = so.e[:nn, mm, k].copy()
+                rhs = np.empty_like(b)
+                rhs[1:] = b[:-1]
+                rhs[0] = preln / np.exp(ln[0])
+                so.e[:nn, mm, k] = _band_solve(P, rhs, kl=3, ku=2)
+                so.ln = float(np.real(so.e[0, mm, k])) * np.exp(ln[0])
+            else:
+                so.e[:nn, mm, k] = _band_solve(A, so.e[:nn, mm, k])
+    s.assign(so)
+
+
+# ----------------------------------------------------------------------------
+# time steppers (hyperpow = 0 branch)
+# ----------------------------------------------------------------------------
+def _need_fff(*ss):
+    for s in ss:
+        if s.space != 'FFF':
+            raise ValueError('fefe: all input scalars must be in FFF for time stepping')
+
+
+def febe(s, nl, dt, kit, p):
+    """Forward/backward Euler (mlegs_scalar_ops.f90:1157-1198)."""
+    _need_fff(s, nl)
+    if p.hyperpow != 0:
+        raise NotImplementedError('febe: hyperviscous branch is not on this path')
+    if p.visc < 5e-14:
+        raise ValueError('febe: inviscid case and no linear term in rhs. semi-implicit time adv is impossible')
+    sh = s.copy()
+    sh.e = s.e + dt * nl.e
+    sh.ln = s.ln + dt * nl.ln
+    a = -1.0 / (dt * p.visc)
+    sh.ln = a * sh.ln
+    sh.e = a * sh.e
+    ihelm(sh, a, kit)
+    s.assign(sh)
+
+
+def abcn(s, s_p, nl, nl_p, dt, kit, p):
+    """Adams-Bashforth / Crank-Nicolson (mlegs_scalar_ops.f90:1200-1262)."""
+    _need_fff(s, nl, s_p, nl_p)
+    if p.hyperpow != 0:
+        raise NotImplementedError('abcn: hyperviscous branch is not on this path')
+    sh = s.copy()
+    sh.e = s.e + dt * (1.5 * nl.e - 0.5 * nl_p.e)
+    sh.ln = s.ln + dt * (1.5 * nl.ln - 0.5 * nl_p.ln)
+    svis = s.copy()
+    if p.visc < 5e-14:
+        raise ValueError('abcn: inviscid case and no linear term in rhs. semi-implicit time adv is impossible')
+    del2(svis, kit)
+    svis.e = svis.e * p.visc
+    a = -2.0 / (dt * p.visc)
+    sh.ln = a * (sh.ln + dt / 2.0 * svis.ln)
+    sh.e = a * (sh.e + dt / 2.0 * svis.e)
+    ihelm(sh, a, kit)
+    s.assign(sh)
+    s_p.assign(s)
+    nl_p.assign(nl)
+
+
+# ----------------------------------------------------------------------------
+# vector-field operations
+# ----------------------------------------------------------------------------
+def vecprod(vr, vp, vz, ur, up, uz, kit):
+    """v <- v x u on the two packed phi points (mlegs_scalar_ops.f90:1264-1306)."""
+    for s in (vr, vp, vz, ur, up, uz):
+        if s.space != 'PPP':
+            raise ValueError('vector_product: to compute v x u, all inputs must be in PPP')
+    nr, nph = kit.nr, kit.np // 2
+    sl = (slice(0, nr), slice(0, nph), slice(None))
+    a1, a2, a3 = vr.e[sl].real, vp.e[sl].real, vz.e[sl].real
+    b1, b2, b3 = ur.e[sl].real, up.e[sl].real, uz.e[sl].real
+    c1, c2, c3 = vr.e[sl].imag, vp.e[sl].imag, vz.e[sl].imag
+    d1, d2, d3 = ur.e[sl].imag, up.e[sl].imag, uz.e[sl].imag
+    outs = []
+    for re, im in ((a2 * b3 - a3 * b2, c2 * d3 - c3 * d2),
+                   (a3 * b1 - a1 * b3, c3 * d1 - c1 * d3),
+                   (a1 * b2 - a2 * b1, c1 * d2 - c2 * d1)):
+        o = Scalar(kit, 'PPP')
+        o.e[sl] = re + IU * im
+        outs.append(o)
+    vr.assign(outs[0])
+    vp.assign(outs[1])
+    vz.assign(outs[2])
+
+
+def tp2vec(psi, chi, kit):
+    """(psi, chi) -> (vr, vp, vz) in PPP (vector_reconstruction, :1455-1545)."""
+    if psi.space != 'FFF' or chi.space != 'FFF':
+        raise ValueError('vector_projection: psi and chi must be in FFF')
+    nr = kit.nr
+    r = kit.ell * np.sqrt((1.0 + kit.x) / (1.0 - kit.x))
+    ur = chi.copy(); ur.chop_offset(3)
+    up = psi.copy(); up.chop_offset(3)
+    uz = chi.copy(); uz.chop_offset(3)
+    nrc, npc, nzc, nzcu, nrcs = chop_index(ur, kit)
+    xxdx(ur, kit)
+    xxdx(up, kit)
+    for mm in range(npc):
+        nn = nrcs[mm]
+        mv = kit.m[mm]
+        for k in range(kit.nzdim):
+            if (k + 1 <= nzc) or (k + 1 >= nzcu):
+                kv = kit.ak[k]
+                ur.e[:nn, mm, k] = IU * mv * psi.e[:nn, mm, k] + IU * kv * ur.e[:nn, mm, k]
+                up.e[:nn, mm, k] = -up.e[:nn, mm, k] - mv * kv * uz.e[:nn, mm, k]
+    ur.chop_offset(0); up.chop_offset(0)
+    chop(ur, kit); chop(up, kit)
+    trans(ur, 'PPP', kit); trans(up, 'PPP', kit)
+    ur.e[:nr, :, :] /= r[:nr, None, None]
+    up.e[:nr, :, :] /= r[:nr, None, None]
+    del2h(uz, kit)
+    uz.e = -uz.e
+    uz.chop_offset(0)
+    chop(uz, kit)
+    trans(uz, 'PPP', kit)
+    return ur, up, uz
+
+
+def tp2curlvec(psi, chi, kit):
+    """curl V = tp2vec(-del2 chi, psi) (curl_vector_reconstruction, :1547-1560)."""
+    md = chi.copy()
+    del2(md, kit)
+    md.e = -md.e
+    return tp2vec(md, psi, kit)
+
+
+def vec2tp_tables(kit, mm, nn):
+    """V, D, T projection tables of vector_projection (:1387-1411) for one m."""
+    nrh = kit.nr // 2
+    mv = kit.m[mm]
+    fff = (1.0 - kit.x[:nrh] ** 2) / kit.w[:nrh]
+    pfd = kit.pf[:, :nn + 1, mm] @ kit.leg_xxdx(mv, nn + 1)
+    v = np.zeros((nn, nrh)); d = np.zeros((nn, nrh)); t = np.zeros((nn, nrh))
+    for i in range(1, nn + 1):
+        n = max(1, mv + i - 1)
+        v[i - 1] = kit.pf[:, i - 1, mm] / n / (n + 1) / fff
+        d[i - 1] = pfd[:, i - 1] / n / (n + 1) / fff
+        t[i - 1] = kit.pf[:, i - 1, mm] * kit.w[:nrh]
+    return v, d, t
+
+
+def _eomul(a, b):
+    """Parity-folded product, odd rows <- even part (mlegs_scalar_ops.f90:2067-2104)."""
+    nj = b.shape[0]
+    njh = nj // 2
+    be = b[:njh] + b[nj - 1:njh - 1:-1]
+    bo = b[:njh] - b[nj - 1:njh - 1:-1]
+    c = np.zeros(a.shape[0], dtype=np.complex128)
+    c[0::2] = a[0::2] @ be
+    if a.shape[0] > 1:
+        c[1::2] = a[1::2] @ bo
+    return c
+
+
+def _oemul(a, b):
+    """Parity-folded product, odd rows <- odd part (mlegs_scalar_ops.f90:2108-2145)."""
+    nj = b.shape[0]
+    njh = nj // 2
+    be = b[:njh] + b[nj - 1:njh - 1:-1]
+    bo = b[:njh] - b[nj - 1:njh - 1:-1]
+    c = np.zeros(a.shape[0], dtype=np.complex128)
+    c[0::2] = a[0::2] @ bo
+    if a.shape[0] > 1:
+        c[1::2] = a[1::2] @ be
+    return c
+
+
+def vec2tp(vr, vp, vz, kit):
+    """Toroidal-poloidal projection of a PPP vector field (vector_projection, :1308-1453).
+
+    Returns (psi, chi) in FFF.
+    """
+    for s in (vr, vp, vz):
+        if s.space != 'PPP':
+            raise ValueError('vector_projection: vr, vp, vz must be in PPP')
+    nr = kit.nr
+    r = kit.ell * np.sqrt((1.0 + kit.x) / (1.0 - kit.x))
+    psi = Scalar(kit, 'FFF')
+    chi = Scalar(kit, 'FFF')
+    ur = vr.copy(); up = vp.copy(); uz = vz.copy()
+    ur.e[:nr, :, :] *= r[:nr, None, None]
+    up.e[:nr, :, :] *= r[:nr, None, None]
+    trans(ur, 'PFP', kit)
+    if kit.nz > 1:
+        vertical_fft_forward(ur, kit)
+    ur.space = 'PFF'
+    trans(up, 'FFF', kit)
+    inf = calcat1(up, kit)
+    psi.ln = -0.5 * float(np.real(inf[0]))
+    rtrans_backward(up, kit)
+    up.space = 'PFF'
+    up.e[:nr, 0, 0] += psi.ln * (1.0 + kit.x)
+    trans(uz, 'PFP', kit)
+    if kit.nz > 1:
+        vertical_fft_forward(uz, kit)
+    uz.space = 'PFF'
+    ur.chop_offset(3); up.chop_offset(3); uz.chop_offset(3)
+    nrc, npc, nzc, nzcu, nrcs = chop_index(ur, kit)
+    ks = zlist(kit, nzc, nzcu)
+    for mm in range(npc):
+        nn = nrcs[mm]
+        mv = kit.m[mm]
+        v, d, t = vec2tp_tables(kit, mm, nn)
+        for k in ks:
+            w1 = _eomul(v, ur.e[:nr, mm, k]) if mv != 0 else np.zeros(nn, complex)
+            w2 = _oemul(d, up.e[:nr, mm, k])
+            psi.e[:nn, mm, k] = -IU * mv * w1 - w2
+            w1 = _oemul(d, ur.e[:nr, mm, k])
+            w2 = _eomul(v, up.e[:nr, mm, k]) if mv != 0 else np.zeros(nn, complex)
+            c = _eomul(t, uz.e[:nr, mm, k])
+            kv = kit.ak[k]
+            chi.e[:nn, mm, k] = IU * kv * w1 + mv * kv * w2 - c
+    idel2_proln(chi, kit)
+    psi.chop_offset(0); chi.chop_offset(0)
+    chop(psi, kit); chop(chi, kit)
+    zeroat1(psi, kit); zeroat1(chi, kit)
+    return psi, chi
