@@ -384,8 +384,8 @@ int mlx_ctx_destroy(mlx_ctx *c) {
   for (mlx_field *f : c->pool) field_free(f);
   c->pool.clear();
   for (auto &e : c->lu) {
-    (void)hipFree(e.fac);
-    (void)hipFree(e.piv);
+    (void)hipFree(e.sol);
+    (void)hipFree(e.moff);
   }
   for (auto &kv : c->work)
     if (kv.second.d) (void)hipFree(kv.second.d);

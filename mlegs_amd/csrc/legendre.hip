@@ -319,6 +319,10 @@ namespace mlx {
 void leg_forward(mlx_ctx *c, const double2 *in, double2 *out, int nroff, bool zero) {
   const KitDev &k = c->kit;
   if (k.nrchop + nroff > k.nrdim) MLX_THROW(MLX_E_ARG, "rtrans_forward: chopping in r too large");
+  if (c->prec != MLX_PREC_FP64 && mfma_ok(k, nroff)) {
+    leg_forward_f32(c, in, out);
+    return;
+  }
   if (zero) HIPCHK(hipMemsetAsync(out, 0, sizeof(double2) * c->nelem, c->stream));
   const WorkTable &wt = get_work(c, 0, nroff);
   if (wt.n == 0) return;
@@ -331,6 +335,10 @@ void leg_forward(mlx_ctx *c, const double2 *in, double2 *out, int nroff, bool ze
 void leg_backward(mlx_ctx *c, const double2 *in, double2 *out, int nroff, bool zero) {
   const KitDev &k = c->kit;
   if (k.nrchop + nroff > k.nrdim) MLX_THROW(MLX_E_ARG, "rtrans_backward: chopping in r too large");
+  if (c->prec != MLX_PREC_FP64 && mfma_ok(k, nroff)) {
+    leg_backward_f32(c, in, out);
+    return;
+  }
   if (zero) HIPCHK(hipMemsetAsync(out, 0, sizeof(double2) * c->nelem, c->stream));
   const WorkTable &wt = get_work(c, 1, nroff);
   if (wt.n == 0) return;
@@ -343,6 +351,10 @@ void leg_backward(mlx_ctx *c, const double2 *in, double2 *out, int nroff, bool z
 void vec2tp_project(mlx_ctx *c, const double2 *ur, const double2 *up, const double2 *uz, double2 *psi,
                     double2 *chi, int nroff) {
   const KitDev &k = c->kit;
+  if (c->prec != MLX_PREC_FP64 && mfma_ok(k, 0)) {
+    vec2tp_project_f32(c, ur, up, uz, psi, chi);
+    return;
+  }
   HIPCHK(hipMemsetAsync(psi, 0, sizeof(double2) * c->nelem, c->stream));
   HIPCHK(hipMemsetAsync(chi, 0, sizeof(double2) * c->nelem, c->stream));
   const WorkTable &wt = get_work(c, 2, nroff);

@@ -1,0 +1,465 @@
+// Mixed-precision Legendre contractions on FP32 MFMA (v_mfma_f32_16x16x4_f32).
+//
+// Same contractions as legendre.hip (rtrans_forward / rtrans_backward /
+// vector_projection, src/submodules/mlegs_scalar_ops.f90:1852-2008,
+// 1387-1436) with FP32 operands and FP32 accumulation; fields stay FP64 in
+// HBM and are rounded to FP32 when staged into LDS.  These kernels also write
+// every entry of their output (zeros outside the active set), so no separate
+// zero-fill pass is needed.
+//
+// GEMM orientation.  Both directions are computed transposed: the MFMA "A"
+// operand is the field (rows = real columns j = 2k + re/im, contraction
+// dimension in LDS), the "B" operand is the table (columns = radial index).
+// The 16x16 accumulator layout (lane l holds rows 4(l/16)+r, column l%16)
+// then gives each lane the (re, im) pairs of two axial slices for one radial
+// index, so rows 2a and 2a+1 (both parities) or x and nr-1-x are written as
+// contiguous FP64 pairs.
+//
+// Work decomposition: one workgroup (4 wavefronts) per (m, 32 axial slices);
+// wave w owns 16-row tiles w, w+4, ... of the radial index.  Contraction in
+// chunks of KC = 32 staged in LDS.
+#include "mlx_internal.h"
+
+namespace {
+
+typedef float floatx4 __attribute__((ext_vector_type(4)));
+
+constexpr int KC = 32;        // contraction chunk
+constexpr int KT = 32;        // axial slices per workgroup
+constexpr int BJ = 2 * KT;    // real columns per workgroup
+constexpr int BJP = BJ + 4;   // padded LDS row (floats)
+
+__device__ inline floatx4 mfma(float a, float b, floatx4 c) {
+  return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+}
+
+// ---------------------------------------------------------------------------
+// forward: out(2a+p, m, k) = sum_x P_p(a, x) fold_p(x, k)
+//   table pfg[m][p][x][a] (a padded to AP), fold from FP64 input
+// ---------------------------------------------------------------------------
+template <int AT>  // 16-row tiles per wave: AP = 4 * AT * 16
+__global__ void __launch_bounds__(256) k_leg_fwd_f32(const double2 *__restrict__ in, double2 *__restrict__ out,
+                                                     const float *__restrict__ pfg, const double *__restrict__ w,
+                                                     int nrdim, int npdim, int nz, int nr, int nrh, int nrc,
+                                                     int npc) {
+  constexpr int AP = 4 * AT * 16;
+  constexpr int APP = AP + 4;
+  __shared__ float Bs[2][KC][BJP];
+  __shared__ float As[2][KC][APP];
+  const int m = blockIdx.x;
+  const int k0 = blockIdx.y * KT;
+  const int tid = threadIdx.x, lane = tid % 64, wv = tid / 64;
+  const size_t zs = (size_t)nrdim * npdim;
+  double2 *ocol = out + (size_t)nrdim * m;
+  if (m >= npc) {  // inactive column: zero
+    for (int idx = tid; idx < nrdim * KT; idx += 256) {
+      int i = idx % nrdim, kl = idx / nrdim;
+      if (k0 + kl < nz) ocol[(size_t)(k0 + kl) * zs + i] = make_double2(0.0, 0.0);
+    }
+    return;
+  }
+  const int nn = nrc - m > 0 ? nrc - m : 0;
+  const int Mmax = (nn + 1) / 2;
+  const double2 *icol = in + (size_t)nrdim * m;
+  const float *T = pfg + (size_t)m * 2 * nrh * AP;
+  floatx4 acc[AT][4][2];
+#pragma unroll
+  for (int s = 0; s < AT; ++s)
+#pragma unroll
+    for (int t = 0; t < 4; ++t)
+#pragma unroll
+      for (int p = 0; p < 2; ++p) acc[s][t][p] = floatx4{0.f, 0.f, 0.f, 0.f};
+
+  for (int x0 = 0; x0 < nrh; x0 += KC) {
+    // table chunk: As[p][x][a] = pfg[m][p][x0+x][a]
+    for (int idx = tid; idx < 2 * KC * AP; idx += 256) {
+      int a = idx % AP, x = (idx / AP) % KC, p = idx / (AP * KC);
+      int xg = x0 + x;
+      As[p][x][a] = (xg < nrh) ? T[((size_t)p * nrh + xg) * AP + a] : 0.f;
+    }
+    // field chunk, folded with the quadrature weights
+    for (int idx = tid; idx < KC * KT; idx += 256) {
+      int x = idx % KC, kl = idx / KC;
+      int xg = x0 + x, kg = k0 + kl;
+      float e0 = 0.f, e1 = 0.f, o0 = 0.f, o1 = 0.f;
+      if (xg < nrh && kg < nz) {
+        double2 a = icol[(size_t)kg * zs + xg];
+        double2 b = icol[(size_t)kg * zs + (nr - 1 - xg)];
+        double wx = w[xg];
+        e0 = (float)((a.x + b.x) * wx);
+        e1 = (float)((a.y + b.y) * wx);
+        o0 = (float)((a.x - b.x) * wx);
+        o1 = (float)((a.y - b.y) * wx);
+      }
+      Bs[0][x][2 * kl] = e0;
+      Bs[0][x][2 * kl + 1] = e1;
+      Bs[1][x][2 * kl] = o0;
+      Bs[1][x][2 * kl + 1] = o1;
+    }
+    __syncthreads();
+#pragma unroll 2
+    for (int kk = 0; kk < KC; kk += 4) {
+      const int xr = kk + lane / 16;
+#pragma unroll
+      for (int p = 0; p < 2; ++p) {
+        float fa[4];
+#pragma unroll
+        for (int t = 0; t < 4; ++t) fa[t] = Bs[p][xr][16 * t + lane % 16];
+#pragma unroll
+        for (int s = 0; s < AT; ++s) {
+          const int at = wv + 4 * s;
+          if (at * 16 >= Mmax) continue;
+          float fb = As[p][xr][at * 16 + lane % 16];
+#pragma unroll
+          for (int t = 0; t < 4; ++t) acc[s][t][p] = mfma(fa[t], fb, acc[s][t][p]);
+        }
+      }
+    }
+    __syncthreads();
+  }
+  // epilogue: lane holds (re, im) of slices kl, kl+1 for rows 2a, 2a+1
+#pragma unroll
+  for (int s = 0; s < AT; ++s) {
+    const int a = (wv + 4 * s) * 16 + lane % 16;
+    const int n0 = 2 * a;
+    if (n0 >= nrdim) continue;
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        const int kl = 8 * t + 2 * (lane / 16) + h;
+        const int kg = k0 + kl;
+        if (kg >= nz) continue;
+        double2 ve = (n0 < nn) ? make_double2(acc[s][t][0][2 * h], acc[s][t][0][2 * h + 1]) : make_double2(0.0, 0.0);
+        double2 vo = (n0 + 1 < nn) ? make_double2(acc[s][t][1][2 * h], acc[s][t][1][2 * h + 1])
+                                   : make_double2(0.0, 0.0);
+        double2 *o = ocol + (size_t)kg * zs + n0;
+        o[0] = ve;
+        if (n0 + 1 < nrdim) o[1] = vo;
+      }
+    }
+  }
+  // rows beyond the tiles: zero
+  for (int idx = tid; idx < (nrdim - 2 * AP > 0 ? nrdim - 2 * AP : 0) * KT; idx += 256) {
+    int span = nrdim - 2 * AP;
+    int i = 2 * AP + idx % span, kl = idx / span;
+    if (k0 + kl < nz) ocol[(size_t)(k0 + kl) * zs + i] = make_double2(0.0, 0.0);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// backward: f(x) = be + bo, f(-x) = be - bo, b_p(x, k) = sum_a P(x, 2a+p) e(2a+p, k)
+//   table pf32[m][n][x]
+// ---------------------------------------------------------------------------
+template <int XT>  // 16-column tiles of x per wave: XP = 4 * XT * 16 >= nrh
+__global__ void __launch_bounds__(256) k_leg_bwd_f32(const double2 *__restrict__ in, double2 *__restrict__ out,
+                                                     const float *__restrict__ pf32, int nrdim, int npdim, int nz,
+                                                     int nr, int nrh, int ne, int nrc, int npc) {
+  constexpr int XP = 4 * XT * 16;
+  constexpr int XPP = XP + 4;
+  __shared__ float Es[2][KC][BJP];
+  __shared__ float Ps[2][KC][XPP];
+  const int m = blockIdx.x;
+  const int k0 = blockIdx.y * KT;
+  const int tid = threadIdx.x, lane = tid % 64, wv = tid / 64;
+  const size_t zs = (size_t)nrdim * npdim;
+  double2 *ocol = out + (size_t)nrdim * m;
+  if (m >= npc) {
+    for (int idx = tid; idx < nrdim * KT; idx += 256) {
+      int i = idx % nrdim, kl = idx / nrdim;
+      if (k0 + kl < nz) ocol[(size_t)(k0 + kl) * zs + i] = make_double2(0.0, 0.0);
+    }
+    return;
+  }
+  const int nn = nrc - m > 0 ? nrc - m : 0;
+  const int Me = (nn + 1) / 2;
+  const double2 *icol = in + (size_t)nrdim * m;
+  const float *P = pf32 + (size_t)m * ne * nrh;
+  floatx4 acc[XT][4][2];
+#pragma unroll
+  for (int s = 0; s < XT; ++s)
+#pragma unroll
+    for (int t = 0; t < 4; ++t)
+#pragma unroll
+      for (int p = 0; p < 2; ++p) acc[s][t][p] = floatx4{0.f, 0.f, 0.f, 0.f};
+
+  for (int a0 = 0; a0 < Me; a0 += KC) {
+    for (int idx = tid; idx < 2 * KC * XP; idx += 256) {
+      int x = idx % XP, a = (idx / XP) % KC, p = idx / (XP * KC);
+      int n = 2 * (a0 + a) + p;
+      Ps[p][a][x] = (x < nrh && n < nn) ? P[(size_t)n * nrh + x] : 0.f;
+    }
+    for (int idx = tid; idx < 2 * KC * KT; idx += 256) {
+      int r = idx % (2 * KC), kl = idx / (2 * KC);
+      int n = 2 * a0 + r, kg = k0 + kl;
+      double2 v = make_double2(0.0, 0.0);
+      if (n < nn && kg < nz) v = icol[(size_t)kg * zs + n];
+      Es[r & 1][r >> 1][2 * kl] = (float)v.x;
+      Es[r & 1][r >> 1][2 * kl + 1] = (float)v.y;
+    }
+    __syncthreads();
+#pragma unroll 2
+    for (int kk = 0; kk < KC; kk += 4) {
+      const int ar = kk + lane / 16;
+#pragma unroll
+      for (int p = 0; p < 2; ++p) {
+        float fa[4];
+#pragma unroll
+        for (int t = 0; t < 4; ++t) fa[t] = Es[p][ar][16 * t + lane % 16];
+#pragma unroll
+        for (int s = 0; s < XT; ++s) {
+          const int xt = wv + 4 * s;
+          if (xt * 16 >= nrh) continue;
+          float fb = Ps[p][ar][xt * 16 + lane % 16];
+#pragma unroll
+          for (int t = 0; t < 4; ++t) acc[s][t][p] = mfma(fa[t], fb, acc[s][t][p]);
+        }
+      }
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int s = 0; s < XT; ++s) {
+    const int x = (wv + 4 * s) * 16 + lane % 16;
+    if (x >= nrh) continue;
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        const int kg = k0 + 8 * t + 2 * (lane / 16) + h;
+        if (kg >= nz) continue;
+        double ber = acc[s][t][0][2 * h], bei = acc[s][t][0][2 * h + 1];
+        double bor = acc[s][t][1][2 * h], boi = acc[s][t][1][2 * h + 1];
+        double2 *o = ocol + (size_t)kg * zs;
+        o[x] = make_double2(ber + bor, bei + boi);
+        o[nr - 1 - x] = make_double2(ber - bor, bei - boi);
+      }
+    }
+  }
+  for (int idx = tid; idx < (nrdim - nr) * KT; idx += 256) {
+    int span = nrdim - nr;
+    int i = nr + idx % span, kl = idx / span;
+    if (k0 + kl < nz) ocol[(size_t)(k0 + kl) * zs + i] = make_double2(0.0, 0.0);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// vec2tp projection on MFMA, one parity per workgroup:
+//   psi = V.X1 + D.X2,   X1 = -i m eo(ur),  X2 = -oe(up)
+//   chi = D.X3 + V.X4 + T.X5,  X3 = i kv oe(ur),  X4 = m kv eo(up),  X5 = -eo(uz)
+// eo/oe: even/odd folds b(x) +- b(-x) chosen by the row parity (eomul /
+// oemul, mlegs_scalar_ops.f90:2067-2145).  Only rows < nrchop - m are
+// computed: the three rows of the radial offset are chopped afterwards by
+// the reference.  Tables tg[m][p][x][a] (V, D, T).
+// ---------------------------------------------------------------------------
+template <int AT>
+__global__ void __launch_bounds__(256) k_v2t_f32(const double2 *__restrict__ ur, const double2 *__restrict__ up,
+                                                 const double2 *__restrict__ uz, double2 *__restrict__ psi,
+                                                 double2 *__restrict__ chi, const float *__restrict__ tgV,
+                                                 const float *__restrict__ tgD, const float *__restrict__ tgT,
+                                                 const double *__restrict__ ak, int nrdim, int npdim, int nz, int nr,
+                                                 int nrh, int nrc0, int npc, int nzc, int nzcu) {
+  constexpr int AP = 4 * AT * 16;
+  constexpr int APP = AP + 4;
+  constexpr int KV = 16;  // contraction chunk for this kernel
+  __shared__ float X[5][KV][BJP];
+  __shared__ float Tb[3][KV][APP];
+  const int m = blockIdx.x >> 1, p = blockIdx.x & 1;
+  const int k0 = blockIdx.y * KT;
+  const int tid = threadIdx.x, lane = tid % 64, wv = tid / 64;
+  const size_t zs = (size_t)nrdim * npdim;
+  if (m >= npc) return;
+  const int nn = nrc0 - m > 0 ? nrc0 - m : 0;
+  const int M = (nn - p + 1) / 2;
+  if (M <= 0) return;
+  const double mv = (double)m;
+  const double2 *cr = ur + (size_t)nrdim * m;
+  const double2 *cp = up + (size_t)nrdim * m;
+  const double2 *cz = uz + (size_t)nrdim * m;
+  const size_t tb = ((size_t)m * 2 + p) * nrh * AP;
+  const double seo = (p == 0) ? 1.0 : -1.0;
+  floatx4 aps[AT][4], ach[AT][4];
+#pragma unroll
+  for (int s = 0; s < AT; ++s)
+#pragma unroll
+    for (int t = 0; t < 4; ++t) aps[s][t] = ach[s][t] = floatx4{0.f, 0.f, 0.f, 0.f};
+
+  for (int x0 = 0; x0 < nrh; x0 += KV) {
+    for (int idx = tid; idx < 3 * KV * AP; idx += 256) {
+      int a = idx % AP, x = (idx / AP) % KV, q = idx / (AP * KV);
+      int xg = x0 + x;
+      const float *src = (q == 0) ? tgV : (q == 1) ? tgD : tgT;
+      Tb[q][x][a] = (xg < nrh) ? src[tb + (size_t)xg * AP + a] : 0.f;
+    }
+    for (int idx = tid; idx < KV * KT; idx += 256) {
+      int x = idx % KV, kl = idx / KV;
+      int xg = x0 + x, kg = k0 + kl;
+      double2 x1 = make_double2(0, 0), x2 = x1, x3 = x1, x4 = x1, x5 = x1;
+      if (xg < nrh && kg < nz) {
+        size_t oa = (size_t)kg * zs + xg, ob = (size_t)kg * zs + (nr - 1 - xg);
+        double kv = ak[kg];
+        double2 a = cr[oa], b = cr[ob];
+        double2 reo = make_double2(a.x + seo * b.x, a.y + seo * b.y);  // eo(ur)
+        double2 roe = make_double2(a.x - seo * b.x, a.y - seo * b.y);  // oe(ur)
+        a = cp[oa];
+        b = cp[ob];
+        double2 peo = make_double2(a.x + seo * b.x, a.y + seo * b.y);
+        double2 poe = make_double2(a.x - seo * b.x, a.y - seo * b.y);
+        a = cz[oa];
+        b = cz[ob];
+        double2 zeo = make_double2(a.x + seo * b.x, a.y + seo * b.y);
+        x1 = make_double2(mv * reo.y, -mv * reo.x);          // -i m eo(ur)
+        x2 = make_double2(-poe.x, -poe.y);                   // -oe(up)
+        x3 = make_double2(-kv * roe.y, kv * roe.x);          // i kv oe(ur)
+        x4 = make_double2(mv * kv * peo.x, mv * kv * peo.y); // m kv eo(up)
+        x5 = make_double2(-zeo.x, -zeo.y);                   // -eo(uz)
+      }
+      X[0][x][2 * kl] = (float)x1.x; X[0][x][2 * kl + 1] = (float)x1.y;
+      X[1][x][2 * kl] = (float)x2.x; X[1][x][2 * kl + 1] = (float)x2.y;
+      X[2][x][2 * kl] = (float)x3.x; X[2][x][2 * kl + 1] = (float)x3.y;
+      X[3][x][2 * kl] = (float)x4.x; X[3][x][2 * kl + 1] = (float)x4.y;
+      X[4][x][2 * kl] = (float)x5.x; X[4][x][2 * kl + 1] = (float)x5.y;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int kk = 0; kk < KV; kk += 4) {
+      const int xr = kk + lane / 16;
+      float f[5][4];
+#pragma unroll
+      for (int q = 0; q < 5; ++q)
+#pragma unroll
+        for (int t = 0; t < 4; ++t) f[q][t] = X[q][xr][16 * t + lane % 16];
+#pragma unroll
+      for (int s = 0; s < AT; ++s) {
+        const int at = wv + 4 * s;
+        if (at * 16 >= M) continue;
+        float bv = Tb[0][xr][at * 16 + lane % 16];
+        float bd = Tb[1][xr][at * 16 + lane % 16];
+        float bt = Tb[2][xr][at * 16 + lane % 16];
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+          aps[s][t] = mfma(f[0][t], bv, aps[s][t]);
+          aps[s][t] = mfma(f[1][t], bd, aps[s][t]);
+          ach[s][t] = mfma(f[2][t], bd, ach[s][t]);
+          ach[s][t] = mfma(f[3][t], bv, ach[s][t]);
+          ach[s][t] = mfma(f[4][t], bt, ach[s][t]);
+        }
+      }
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int s = 0; s < AT; ++s) {
+    const int a = (wv + 4 * s) * 16 + lane % 16;
+    if (a >= M) continue;
+    const int n = 2 * a + p;
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        const int kg = k0 + 8 * t + 2 * (lane / 16) + h;
+        if (kg >= nz) continue;
+        if (!((kg + 1 <= nzc) || (kg + 1 >= nzcu))) continue;
+        size_t o = (size_t)kg * zs + (size_t)nrdim * m + n;
+        psi[o] = make_double2(aps[s][t][2 * h], aps[s][t][2 * h + 1]);
+        chi[o] = make_double2(ach[s][t][2 * h], ach[s][t][2 * h + 1]);
+      }
+    }
+  }
+}
+
+// table re-layouts for the forward and projection kernels: dst[m][p][x][a]
+__global__ void k_tbl_pxa(const double *src, float *dst, int nrh, int rows_per_m, int nrows, int AP, int npchop) {
+  size_t n = (size_t)npchop * 2 * nrh * AP;
+  for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (size_t)gridDim.x * blockDim.x) {
+    int a = (int)(t % AP);
+    int x = (int)((t / AP) % nrh);
+    int p = (int)((t / ((size_t)AP * nrh)) % 2);
+    int m = (int)(t / ((size_t)AP * nrh * 2));
+    int r = 2 * a + p;
+    dst[t] = (r < nrows) ? (float)src[((size_t)m * rows_per_m + r) * nrh + x] : 0.f;
+  }
+}
+
+}  // namespace
+
+namespace mlx {
+
+int mfma_tiles(const KitDev &k) {
+  // 16-row tiles per wave needed to cover ceil(nrchop/2) rows with 4 waves
+  int M = (k.nrchop + 1) / 2;
+  int at = (M + 63) / 64;
+  return at;
+}
+
+bool mfma_ok(const KitDev &k, int nroff) {
+  int at = mfma_tiles(k);
+  int xt = (k.nrh + 63) / 64;
+  return nroff == 0 && at >= 1 && at <= 2 && xt >= 1 && xt <= 2;
+}
+
+void mfma_tables(mlx_ctx *c) {
+  KitDev &k = c->kit;
+  if (!mfma_ok(k, 0)) return;
+  int AP = 4 * mfma_tiles(k) * 16;
+  size_t n = (size_t)k.npchop * 2 * k.nrh * AP;
+  k.mfma_ap = AP;
+  auto alloc = [&](float *&p) {
+    if (p) (void)hipFree(p);
+    if (hipMalloc(&p, sizeof(float) * n) != hipSuccess) MLX_THROW(MLX_E_ALLOC, "kit: table allocation failed");
+  };
+  alloc(k.pfg32);
+  alloc(k.tVg32);
+  alloc(k.tDg32);
+  alloc(k.tTg32);
+  int nb = (int)((n + 255) / 256 > 8192 ? 8192 : (n + 255) / 256);
+  hipLaunchKernelGGL(k_tbl_pxa, dim3(nb), dim3(256), 0, c->stream, k.pf, k.pfg32, k.nrh, k.ne, k.ne, AP, k.npchop);
+  hipLaunchKernelGGL(k_tbl_pxa, dim3(nb), dim3(256), 0, c->stream, k.tV, k.tVg32, k.nrh, k.nt, k.nt, AP, k.npchop);
+  hipLaunchKernelGGL(k_tbl_pxa, dim3(nb), dim3(256), 0, c->stream, k.tD, k.tDg32, k.nrh, k.nt, k.nt, AP, k.npchop);
+  hipLaunchKernelGGL(k_tbl_pxa, dim3(nb), dim3(256), 0, c->stream, k.tT, k.tTg32, k.nrh, k.nt, k.nt, AP, k.npchop);
+  HIPCHK(hipGetLastError());
+}
+
+void leg_forward_f32(mlx_ctx *c, const double2 *in, double2 *out) {
+  const KitDev &k = c->kit;
+  dim3 grid(k.npdim, (k.nzdim + KT - 1) / KT);
+  int at = mfma_tiles(k);
+  if (at == 1)
+    hipLaunchKernelGGL(k_leg_fwd_f32<1>, grid, dim3(256), 0, c->stream, in, out, k.pfg32, k.w, k.nrdim, k.npdim,
+                       k.nzdim, k.nr, k.nrh, k.nrchop, k.npchop);
+  else
+    hipLaunchKernelGGL(k_leg_fwd_f32<2>, grid, dim3(256), 0, c->stream, in, out, k.pfg32, k.w, k.nrdim, k.npdim,
+                       k.nzdim, k.nr, k.nrh, k.nrchop, k.npchop);
+  HIPCHK(hipGetLastError());
+}
+
+void leg_backward_f32(mlx_ctx *c, const double2 *in, double2 *out) {
+  const KitDev &k = c->kit;
+  dim3 grid(k.npdim, (k.nzdim + KT - 1) / KT);
+  int xt = (k.nrh + 63) / 64;
+  if (xt == 1)
+    hipLaunchKernelGGL(k_leg_bwd_f32<1>, grid, dim3(256), 0, c->stream, in, out, k.pf32, k.nrdim, k.npdim, k.nzdim,
+                       k.nr, k.nrh, k.ne, k.nrchop, k.npchop);
+  else
+    hipLaunchKernelGGL(k_leg_bwd_f32<2>, grid, dim3(256), 0, c->stream, in, out, k.pf32, k.nrdim, k.npdim, k.nzdim,
+                       k.nr, k.nrh, k.ne, k.nrchop, k.npchop);
+  HIPCHK(hipGetLastError());
+}
+
+void vec2tp_project_f32(mlx_ctx *c, const double2 *ur, const double2 *up, const double2 *uz, double2 *psi,
+                        double2 *chi) {
+  const KitDev &k = c->kit;
+  HIPCHK(hipMemsetAsync(psi, 0, sizeof(double2) * c->nelem, c->stream));
+  HIPCHK(hipMemsetAsync(chi, 0, sizeof(double2) * c->nelem, c->stream));
+  dim3 grid(2 * k.npchop, (k.nzdim + KT - 1) / KT);
+  int at = mfma_tiles(k);
+  if (at == 1)
+    hipLaunchKernelGGL(k_v2t_f32<1>, grid, dim3(256), 0, c->stream, ur, up, uz, psi, chi, k.tVg32, k.tDg32, k.tTg32,
+                       k.ak, k.nrdim, k.npdim, k.nzdim, k.nr, k.nrh, k.nrchop, k.npchop, k.chopzl, k.chopzu);
+  else
+    hipLaunchKernelGGL(k_v2t_f32<2>, grid, dim3(256), 0, c->stream, ur, up, uz, psi, chi, k.tVg32, k.tDg32, k.tTg32,
+                       k.ak, k.nrdim, k.npdim, k.nzdim, k.nr, k.nrh, k.nrchop, k.npchop, k.chopzl, k.chopzu);
+  HIPCHK(hipGetLastError());
+}
+
+}  // namespace mlx

@@ -57,6 +57,9 @@ struct KitDev {
   double *xxc = nullptr;      // [m][n][2]: xxdx band rows (offsets -1, +1)
   double *tV = nullptr, *tD = nullptr, *tT = nullptr;  // [m][n][x], nt rows per m
   float *tV32 = nullptr, *tD32 = nullptr, *tT32 = nullptr;
+  // MFMA-friendly FP32 tables [m][parity][x][a] (a padded to mfma_ap)
+  float *pfg32 = nullptr, *tVg32 = nullptr, *tDg32 = nullptr, *tTg32 = nullptr;
+  int mfma_ap = 0;
   double2 *tw_nph = nullptr, *tw_nz = nullptr, *tw_half = nullptr;
   FftPlan plan_nph, plan_nz;
   // host mirrors
@@ -77,10 +80,13 @@ struct mlx_field {
 // cached LU factors of (del2 + alpha) per (m, k) system
 struct LuCache {
   double alpha;
-  int kind;   // 0: del2 + alpha; 1: idel2 bordered (alpha = 0)
+  int kind;   // 0: del2 + alpha; 1: idel2 bordered (alpha = 0); 2: idel2, (0,0) excluded
   int nroff;  // radial chop offset the systems were built for
-  double *fac = nullptr;   // [sys][row][7]
-  int8_t *piv = nullptr;   // [sys][row]
+  // solve layout: row j of system (m, k) holds 8 doubles
+  //   (L(j+1,j), L(j+2,j), U(j,j), U(j,j+1..j+4), pivot offset)
+  // at sol[((moff[m] + j) * 8 + q) * nz + k]  -- k fastest (coalesced)
+  double *sol = nullptr;
+  int *moff = nullptr;
   uint64_t stamp = 0;
 };
 
@@ -127,6 +133,13 @@ void fft_z(mlx_ctx *c, double2 *e, int dir);    // dir -1 forward (/nz), +1 back
 // legendre.hip
 void leg_forward(mlx_ctx *c, const double2 *in, double2 *out, int nroff, bool zero = true);
 void leg_backward(mlx_ctx *c, const double2 *in, double2 *out, int nroff, bool zero = true);
+// legendre_mfma.hip (FP32 MFMA; valid when mfma_ok)
+bool mfma_ok(const KitDev &k, int nroff);
+void mfma_tables(mlx_ctx *c);
+void leg_forward_f32(mlx_ctx *c, const double2 *in, double2 *out);
+void leg_backward_f32(mlx_ctx *c, const double2 *in, double2 *out);
+void vec2tp_project_f32(mlx_ctx *c, const double2 *ur, const double2 *up, const double2 *uz, double2 *psi,
+                        double2 *chi);
 void vec2tp_project(mlx_ctx *c, const double2 *ur, const double2 *up, const double2 *uz,
                     double2 *psi, double2 *chi, int nroff);
 

@@ -1,0 +1,118 @@
+"""Mixed-precision Legendre contractions (FP32 MFMA) against the FP64 oracle.
+
+Target of the build: spectral coefficients within 1e-6 norm-wise relative
+error of the FP64 result, per field.  FP32 operands with FP32 accumulation
+over nr/2 quadrature points give ~1e-7.
+"""
+import numpy as np
+import pytest
+
+from conftest import phys, rel_err
+import _cases as C
+
+pytestmark = pytest.mark.gpu
+
+TOL_MIXED = 1e-6
+
+
+def _say(*a):
+    print('[case]', *a, flush=True)
+
+
+@pytest.fixture(scope='module', params=['C1', 'ODD'])
+def setup(request):
+    import mlegs_amd as M
+    from oracle import kit as K
+    p = C.params(getattr(C, request.param))
+    _say('kit', request.param)
+    tfm = M.Kit(p, device=0, precision=M.PREC_FP32)
+    okit = K.Kit3D(p)
+    yield request.param, p, tfm, okit
+    tfm.close()
+
+
+def test_trans_fp32(setup):
+    import mlegs_amd as M
+    from oracle import scalar_ops as so
+    name, p, tfm, okit = setup
+    _say('trans fp32', name)
+    e = C.spectral(tfm.dims, p, 11)
+    g, o = C.pair(tfm, okit, e, 'FFF', ln=0.37)
+    M.trans(g, 'PPP', tfm)
+    so.trans(o, 'PPP', okit)
+    err_b = rel_err(phys(g.e, p.nr, p.np // 2), phys(o.e, p.nr, p.np // 2))
+    M.trans(g, 'FFF', tfm)
+    so.trans(o, 'FFF', okit)
+    err_f = rel_err(g.e, o.e)
+    _say('  backward %.2e forward %.2e' % (err_b, err_f))
+    assert err_b < TOL_MIXED and err_f < TOL_MIXED
+    # not silently FP64: FP32 rounding is visible
+    assert err_b > 1e-12
+
+
+def test_vec2tp_fp32(setup):
+    import mlegs_amd as M
+    from oracle import scalar_ops as so
+    name, p, tfm, okit = setup
+    _say('vec2tp fp32', name)
+    vs = [C.physical(tfm.dims, p, 60 + i) for i in range(3)]
+    g = [M.Scalar(tfm).upload(v, space='PPP') for v in vs]
+    o = []
+    for v in vs:
+        s = so.Scalar(okit, 'PPP')
+        s.e = v.copy()
+        o.append(s)
+    gpsi, gchi = M.Scalar(tfm), M.Scalar(tfm)
+    M.vec2tp(g[0], g[1], g[2], gpsi, gchi, tfm)
+    opsi, ochi = so.vec2tp(o[0], o[1], o[2], okit)
+    ep, ec = rel_err(gpsi.e, opsi.e), rel_err(gchi.e, ochi.e)
+    _say('  psi %.2e chi %.2e' % (ep, ec))
+    assert ep < TOL_MIXED and ec < TOL_MIXED
+
+
+def test_vortex_fp32(setup):
+    import mlegs_amd as M
+    from oracle import scalar_ops as so, vortex as V
+    name, p, tfm, okit = setup
+    _say('vortex fp32', name)
+    (gpsi, opsi), (gchi, ochi) = C.qvortex(tfm, okit, p)
+    run = M.VortexRun(tfm, gpsi, gchi, M.Scalar(tfm))
+    orun = V.VortexRun(okit, p, opsi, ochi, so.Scalar(okit, 'PPP'))
+    run.bootstrap()
+    orun.bootstrap()
+    for _ in range(3):
+        run.step(1)
+        orun.step()
+    for nm, ob in (('psi', orun.psi), ('chi', orun.chi)):
+        err = rel_err(run.field(nm).e, ob.e)
+        _say('  %s %.2e' % (nm, err))
+        assert err < TOL_MIXED, nm
+
+
+def test_transform_256_fp32_vs_oracle():
+    """The transform-only workload's size (NR=NP=NZ=256), one field."""
+    import math
+    import mlegs_amd as M
+    from oracle import kit as K, scalar_ops as so
+    from mlegs_amd import inputs
+    _say('256^3 transform fp32/fp64 vs oracle')
+    p = M.Params(nr=256, np=256, nz=256, nrchop=256, npchop=129, nzchop=129, ell=1.0, zlen=2 * math.pi,
+                 hyperpow=0, is_svv=False)
+    tfm = M.Kit(p, device=0, precision=M.PREC_FP64)
+    okit = K.Kit3D(p)
+    e = inputs.transform_field(tfm.dims, (p.nrchop, p.npchop, p.nzchop, p.np, p.nz), field_id=3)
+    o = so.Scalar(okit, 'FFF')
+    o.e = e.copy()
+    so.trans(o, 'PPP', okit)
+    ref_ppp = o.e.copy()
+    so.trans(o, 'FFF', okit)
+    for prec, tol in ((M.PREC_FP64, 1e-12), (M.PREC_FP32, TOL_MIXED)):
+        tfm.set_precision(prec)
+        g = M.Scalar(tfm).upload(e, space='FFF')
+        M.trans(g, 'PPP', tfm)
+        eb = rel_err(phys(g.e, 256, 128), phys(ref_ppp, 256, 128))
+        M.trans(g, 'FFF', tfm)
+        ef = rel_err(g.e, o.e)
+        _say('  prec %d backward %.2e round trip %.2e' % (prec, eb, ef))
+        assert eb < tol and ef < tol
+    tfm.close()
