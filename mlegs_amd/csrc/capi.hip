@@ -383,10 +383,7 @@ int mlx_ctx_destroy(mlx_ctx *c) {
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   for (mlx_field *f : c->pool) field_free(f);
   c->pool.clear();
-  for (auto &e : c->lu) {
-    (void)hipFree(e.sol);
-    (void)hipFree(e.moff);
-  }
+  lu_cache_free(c);
   for (auto &kv : c->work)
     if (kv.second.d) (void)hipFree(kv.second.d);
   for (double2 *s : c->scr)

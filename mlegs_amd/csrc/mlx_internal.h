@@ -77,16 +77,14 @@ struct mlx_field {
   int off[3] = {0, 0, 0};
 };
 
-// cached LU factors of (del2 + alpha) per (m, k) system
+// cached LU factors of (del2 + alpha) per (m, |kappa|) system (layout in solve.hip)
 struct LuCache {
   double alpha;
   int kind;   // 0: del2 + alpha; 1: idel2 bordered (alpha = 0); 2: idel2, (0,0) excluded
   int nroff;  // radial chop offset the systems were built for
-  // solve layout: row j of system (m, k) holds 8 doubles
-  //   (L(j+1,j), L(j+2,j), U(j,j), U(j,j+1..j+4), pivot offset)
-  // at sol[((moff[m] + j) * 8 + q) * nz + k]  -- k fastest (coalesced)
-  double *sol = nullptr;
-  int *moff = nullptr;
+  double *sol = nullptr;     // factor rows
+  uint16_t *pvw = nullptr;   // row-interchange words
+  int *anyp = nullptr;       // per 64-system group: any interchange
   uint64_t stamp = 0;
 };
 
@@ -154,6 +152,7 @@ void k_delsqp(mlx_ctx *c, double2 *e, double *ln, int nroff, int inverse);
 void k_bandmul(mlx_ctx *c, double2 *e, double *ln, int op, int nroff);  // op 0 del2, 1 del2h, 2 xxdx
 void k_helm_pre(mlx_ctx *c, double2 *e, double *ln_out, const double *ln_in, double alpha);
 void k_helm_solve(mlx_ctx *c, double2 *e, double alpha, int kind, int nroff);
+void lu_cache_free(mlx_ctx *c);  // solve.hip
 void k_idel2_post(mlx_ctx *c, const double2 *e, double *ln);
 void k_idel2_preln00(mlx_ctx *c, double2 *e, double *ln, double preln, int nroff);
 void k_axpby(mlx_ctx *c, double2 *dst, double *dln, double a, const double2 *src, const double *sln, double b);

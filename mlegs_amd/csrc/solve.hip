@@ -1,0 +1,401 @@
+// Banded LU of (del2 + alpha) per (m, k) system and its solve, FP64.
+//
+// Replaces lsolve -> zgbtrf/zgbtrs behind ihelm / idel2
+// (src/submodules/mlegs_bndmat_ops.f90:369-448, called from
+// src/submodules/mlegs_scalar_ops.f90:575-854).  The reference rebuilds and
+// refactorises every (m, k) matrix on every call; here each (alpha, kind)
+// set of factors is computed once (gbtf2: partial pivoting, multipliers by
+// the reciprocal pivot, rank-1 updates skipped on zero entries) and cached.
+// The matrix depends on k only through ak(k)^2, which is bit-identical for
+// k and nz - k, so factors are stored once per |kappa|.
+//
+// Solve layout: systems grouped by 64 consecutive m (one wavefront; its 64
+// columns of the field lie in one z-plane).  Each sweep streams its own dense
+// array, row j of lane l at index ((kk*G + g)*nrc + j)*64 + l
+// (kk = |kappa| index, m = 64g + l):
+//   Ff (double2)  L(j+1,j), L(j+2,j)                 -- forward sweep
+//   Fb (double4)  U(j,j), U(j,j+1), U(j,j+2), 0       -- back substitution
+//   Fx (double2)  U(j,j+3), U(j,j+4)                 -- fill-in, pivoting groups
+// Row interchanges are 2 bits per row in one 16-bit word per 8-row chunk.
+// A group without any interchange has no fill-in (U(j,j+3) = U(j,j+4) = 0)
+// and takes the short path.  Waves for k and nz - k are launched next to
+// each other so the second read of their shared factors hits the caches.
+#include <cmath>
+
+#include "mlx_internal.h"
+
+namespace {
+
+constexpr int KL = 2, KU = 2, KV = 4, LDAB = 7;
+constexpr int CH = 8;  // rows per chunk (one pivot word)
+
+struct FacPtrs {
+  double2 *ff;
+  double4 *fb;
+  double2 *fx;
+};
+__host__ __device__ inline FacPtrs fac_ptrs(double *base, size_t nrows) {
+  FacPtrs p;
+  p.ff = reinterpret_cast<double2 *>(base);
+  p.fb = reinterpret_cast<double4 *>(base + 2 * nrows);
+  p.fx = reinterpret_cast<double2 *>(base + 6 * nrows);
+  return p;
+}
+
+__device__ inline int nrcs_dev(int nrc, int npc, int m) {
+  if (m >= npc) return 0;
+  int v = nrc - m;
+  return v > 0 ? v : 0;
+}
+__device__ inline int z_count(int k, int nzc, int nzcu) {
+  return ((k + 1 <= nzc) ? 1 : 0) + ((k + 1 >= nzcu) ? 1 : 0);
+}
+// launch position p -> (k, kk): k = 0; then kk, nz-kk pairs; then nz/2
+__device__ inline void pos_to_k(int p, int nz, int &k, int &kk) {
+  if (nz == 1 || p == 0) {
+    k = 0;
+    kk = 0;
+  } else if (p == nz - 1) {
+    k = nz / 2;
+    kk = nz / 2;
+  } else {
+    kk = (p + 1) / 2;
+    k = (p & 1) ? kk : nz - kk;
+  }
+}
+
+// (del2 + alpha)(i, j) in the normalised basis; bordered (m=0,k=0 of
+// idel2_proln): column 0 gains the 4/3, -2, 2/3 log-term entries
+// (mlegs_scalar_ops.f90:705-711)
+__device__ inline double d2_entry(const double *d2c, const double *lognorm, int m, int ne, int i, int j, double akk,
+                                  double alpha, int bordered, double ell) {
+  int d = j - i + 2;
+  double v = 0.0;
+  if (d >= 0 && d <= 4) {
+    v = d2c[((size_t)m * ne + i) * 5 + d];
+    if (d == 2) v = v - akk + alpha;
+  }
+  if (bordered && j == 0) {
+    double ell2 = ell * ell;
+    if (i == 0) v += 4.0 / 3.0 / ell2;
+    if (i == 1) v -= 2.0 / 1.0 / ell2 * exp(lognorm[0] - lognorm[1]);
+    if (i == 2) v += 2.0 / 3.0 / ell2 * exp(lognorm[0] - lognorm[2]);
+  }
+  return v;
+}
+
+// one thread per (m, kk) system: gbtf2 in LAPACK band storage
+// (element (i, j) at ab[(KV + i - j) + LDAB j], rows 0..KL-1 fill-in)
+__global__ void k_lu_factor(double *fac, int8_t *piv, const double *d2c, const double *lognorm, const double *ak,
+                            int nrdim, int nkk, int ne, int nrc, int npc, double alpha, int kind, double ell,
+                            int *info) {
+  int kk = blockIdx.x * blockDim.x + threadIdx.x;
+  int m = blockIdx.y;
+  if (kk >= nkk || m >= npc) return;
+  int n = nrcs_dev(nrc, npc, m);
+  if (n <= 0) return;
+  if (kind == 2 && m == 0 && kk == 0) return;  // idel2_preln: (0,0) solved densely
+  size_t sys = (size_t)m * nkk + kk;
+  double *ab = fac + sys * (size_t)nrdim * LDAB;
+  int8_t *ip = piv + sys * (size_t)nrdim;
+  double akk = ak[kk] * ak[kk];
+  int bordered = (kind == 1 && m == 0 && kk == 0) ? 1 : 0;
+  for (int j = 0; j < n; ++j) {
+    for (int r = 0; r < LDAB; ++r) ab[r + LDAB * j] = 0.0;
+    for (int i = (j - KU > 0 ? j - KU : 0); i <= j + KL && i < n; ++i)
+      ab[(KV + i - j) + LDAB * j] = d2_entry(d2c, lognorm, m, ne, i, j, akk, alpha, bordered, ell);
+  }
+  int ju = 0;
+  for (int j = 0; j < n; ++j) {
+    int km = (KL < n - 1 - j) ? KL : n - 1 - j;
+    int jp = 0;
+    double best = fabs(ab[KV + LDAB * j]);
+    for (int r = 1; r <= km; ++r) {
+      double v = fabs(ab[KV + r + LDAB * j]);
+      if (v > best) {
+        best = v;
+        jp = r;
+      }
+    }
+    ip[j] = (int8_t)jp;
+    if (ab[KV + jp + LDAB * j] != 0.0) {
+      int jnew = j + KU + jp;
+      if (jnew > n - 1) jnew = n - 1;
+      if (jnew > ju) ju = jnew;
+      if (jp != 0) {
+        for (int c = j; c <= ju; ++c) {
+          double *a = &ab[(KV + j - c) + LDAB * c];
+          double *b = &ab[(KV + j + jp - c) + LDAB * c];
+          double t = *a;
+          *a = *b;
+          *b = t;
+        }
+      }
+      if (km > 0) {
+        double rp = 1.0 / ab[KV + LDAB * j];
+        for (int r = 1; r <= km; ++r) ab[KV + r + LDAB * j] *= rp;
+        for (int c = j + 1; c <= ju; ++c) {
+          double u = ab[(KV + j - c) + LDAB * c];
+          if (u != 0.0)
+            for (int r = 1; r <= km; ++r) ab[(KV + j + r - c) + LDAB * c] -= ab[KV + r + LDAB * j] * u;
+        }
+      }
+    } else {
+      atomicExch(info, 1);
+    }
+  }
+}
+
+__global__ void k_lu_repack(const double *fac, const int8_t *piv, FacPtrs F, uint16_t *pvw, int *anyp, int nrdim,
+                            int nkk, int nrc, int npc, int kind) {
+  int kk = blockIdx.x * blockDim.x + threadIdx.x;
+  int m = blockIdx.y;
+  if (kk >= nkk || m >= npc) return;
+  if (kind == 2 && m == 0 && kk == 0) return;
+  int n = nrcs_dev(nrc, npc, m);
+  const int G = (npc + 63) / 64, g = m / 64, l = m % 64;
+  const int NC = (nrc + CH - 1) / CH;
+  size_t sys = (size_t)m * nkk + kk;
+  const double *ab = fac + sys * (size_t)nrdim * LDAB;
+  const int8_t *ip = piv + sys * (size_t)nrdim;
+  const size_t r0 = ((size_t)kk * G + g) * (size_t)nrc * 64 + l;
+  uint16_t *P = pvw + ((size_t)kk * G + g) * NC * 64 + l;
+  int any = 0;
+  uint16_t word = 0;
+  for (int j = 0; j < n; ++j) {
+    double u[5];
+    for (int d = 0; d <= 4; ++d) u[d] = (j + d < n) ? ab[(KV - d) + LDAB * (j + d)] : 0.0;
+    const size_t o = r0 + (size_t)j * 64;
+    F.ff[o] = make_double2((j + 1 < n) ? ab[KV + 1 + LDAB * j] : 0.0, (j + 2 < n) ? ab[KV + 2 + LDAB * j] : 0.0);
+    double4 fb;
+    fb.x = u[0];
+    fb.y = u[1];
+    fb.z = u[2];
+    fb.w = 0.0;
+    F.fb[o] = fb;
+    F.fx[o] = make_double2(u[3], u[4]);
+    int p = ip[j];
+    any |= p;
+    word |= (uint16_t)(p << (2 * (j % CH)));
+    if (j % CH == CH - 1 || j == n - 1) {
+      P[(size_t)(j / CH) * 64] = word;
+      word = 0;
+    }
+  }
+  if (any) atomicOr(&anyp[(size_t)kk * G + g], 1);
+}
+
+template <bool PIV>
+__device__ __forceinline__ void solve_column(double2 *b, const double2 *__restrict__ Ff,
+                                             const double4 *__restrict__ Fb, const double2 *__restrict__ Fx,
+                                             const uint16_t *P, int n) {
+  const double2 z2 = make_double2(0.0, 0.0);
+  // ---- forward elimination, L y = P b ----
+  double2 w0 = b[0];
+  double2 w1 = (n > 1) ? b[1] : z2;
+  double2 w2 = (n > 2) ? b[2] : z2;
+  for (int j0 = 0; j0 < n - 1; j0 += CH) {
+    double l1[CH], l2[CH];
+    double2 nb[CH];
+    unsigned pw = PIV ? (unsigned)P[(size_t)(j0 / CH) * 64] : 0u;
+#pragma unroll
+    for (int u = 0; u < CH; ++u) {
+      const int j = j0 + u;
+      const double2 f = Ff[(size_t)(j < n ? j : 0) * 64];
+      l1[u] = f.x;
+      l2[u] = f.y;
+      nb[u] = (j + 3 < n) ? b[j + 3] : z2;
+    }
+#pragma unroll
+    for (int u = 0; u < CH; ++u) {
+      const int j = j0 + u;
+      if (j < n - 1) {
+        if (PIV) {
+          const unsigned pv = (pw >> (2 * u)) & 3u;
+          if (pv == 1) { double2 t = w0; w0 = w1; w1 = t; }
+          else if (pv == 2) { double2 t = w0; w0 = w2; w2 = t; }
+        }
+        w1.x -= l1[u] * w0.x;
+        w1.y -= l1[u] * w0.y;
+        if (j + 2 < n) {
+          w2.x -= l2[u] * w0.x;
+          w2.y -= l2[u] * w0.y;
+        }
+        b[j] = w0;
+        w0 = w1;
+        w1 = w2;
+        w2 = nb[u];
+      }
+    }
+  }
+  b[n - 1] = w0;
+  // ---- back substitution, U x = y ----
+  double2 x1 = z2, x2 = z2, x3 = z2, x4 = z2;
+  for (int i0 = n - 1; i0 >= 0; i0 -= CH) {
+    constexpr int NU = PIV ? 5 : 3;
+    double uu[CH][NU];
+    double2 bb[CH];
+#pragma unroll
+    for (int u = 0; u < CH; ++u) {
+      const int i = i0 - u;
+      const size_t o = (size_t)(i >= 0 ? i : 0) * 64;
+      const double4 f = Fb[o];
+      uu[u][0] = f.x;
+      uu[u][1] = f.y;
+      uu[u][2] = f.z;
+      if (PIV) {
+        const double2 fx = Fx[o];
+        uu[u][NU - 2] = fx.x;
+        uu[u][NU - 1] = fx.y;
+      }
+      bb[u] = (i >= 0) ? b[i] : z2;
+    }
+#pragma unroll
+    for (int u = 0; u < CH; ++u) {
+      const int i = i0 - u;
+      if (i >= 0) {
+        double2 acc = bb[u];
+        if (PIV) {
+          if (i + 4 < n) { acc.x -= x4.x * uu[u][NU - 1]; acc.y -= x4.y * uu[u][NU - 1]; }
+          if (i + 3 < n) { acc.x -= x3.x * uu[u][NU - 2]; acc.y -= x3.y * uu[u][NU - 2]; }
+        }
+        if (i + 2 < n) { acc.x -= x2.x * uu[u][2]; acc.y -= x2.y * uu[u][2]; }
+        if (i + 1 < n) { acc.x -= x1.x * uu[u][1]; acc.y -= x1.y * uu[u][1]; }
+        const double d = uu[u][0];
+        double2 xi = make_double2(acc.x / d, acc.y / d);
+        b[i] = xi;
+        x4 = x3;
+        x3 = x2;
+        x2 = x1;
+        x1 = xi;
+      }
+    }
+  }
+}
+
+__global__ void __launch_bounds__(64) k_lu_solve(double2 *e, FacPtrs Fp, const uint16_t *__restrict__ pvw,
+                                                 const int *__restrict__ anyp, int nrdim, int npdim, int nz, int nrc,
+                                                 int npc, int nzc, int nzcu, int skip00) {
+  const int g = blockIdx.x, l = threadIdx.x;
+  int k, kk;
+  pos_to_k(blockIdx.y, nz, k, kk);
+  const int m = 64 * g + l;
+  const int G = (npc + 63) / 64;
+  const int NC = (nrc + CH - 1) / CH;
+  if (k >= nz || m >= npc) return;
+  int cnt = z_count(k, nzc, nzcu);
+  const int n = nrc - m > 0 ? nrc - m : 0;
+  if (skip00 && m == 0 && k == 0) cnt = 0;
+  if (cnt == 0 || n <= 0) return;
+  const size_t r0 = ((size_t)kk * G + g) * (size_t)nrc * 64 + l;
+  const uint16_t *P = pvw + ((size_t)kk * G + g) * NC * 64 + l;
+  double2 *b = e + (size_t)k * nrdim * npdim + (size_t)nrdim * m;
+  const bool piv = anyp[(size_t)kk * G + g] != 0;
+  for (int rep = 0; rep < cnt; ++rep) {
+    if (piv) solve_column<true>(b, Fp.ff + r0, Fp.fb + r0, Fp.fx + r0, P, n);
+    else solve_column<false>(b, Fp.ff + r0, Fp.fb + r0, Fp.fx + r0, P, n);
+  }
+}
+
+// factor rows of all systems: (|kappa| values) x (64-system groups) x rows x 64 lanes
+size_t fac_rows(const KitDev &k, int nroff) {
+  const int nkk = (k.nzdim > 1) ? k.nzdim / 2 + 1 : 1;
+  const int G = (k.npchop + 63) / 64;
+  return (size_t)nkk * G * (k.nrchop + nroff) * 64 + 64;
+}
+
+LuCache &lu_get(mlx_ctx *c, double alpha, int kind, int nroff) {
+  for (auto &e : c->lu)
+    if (e.alpha == alpha && e.kind == kind && e.nroff == nroff) {
+      e.stamp = ++c->lu_clock;
+      return e;
+    }
+  const KitDev &k = c->kit;
+  const int nrc = k.nrchop + nroff;
+  if (nrc > k.nrdim) MLX_THROW(MLX_E_ARG, "chop_index: chopping in r too large");
+  if (c->lu.size() >= 4) {  // evict the least recently used entry
+    size_t victim = 0;
+    for (size_t i = 1; i < c->lu.size(); ++i)
+      if (c->lu[i].stamp < c->lu[victim].stamp) victim = i;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    (void)hipFree(c->lu[victim].sol);
+    (void)hipFree(c->lu[victim].pvw);
+    (void)hipFree(c->lu[victim].anyp);
+    c->lu.erase(c->lu.begin() + victim);
+  }
+  LuCache e;
+  e.alpha = alpha;
+  e.kind = kind;
+  e.nroff = nroff;
+  const int nkk = (k.nzdim > 1) ? k.nzdim / 2 + 1 : 1;
+  const int G = (k.npchop + 63) / 64;
+  const int NC = (nrc + CH - 1) / CH;
+  size_t nsys = (size_t)k.npchop * nkk;
+  const size_t nrows = fac_rows(k, nroff);
+  double *fac = nullptr;
+  int8_t *piv = nullptr;
+  auto cleanup = [&]() {
+    if (fac) (void)hipFree(fac);
+    if (piv) (void)hipFree(piv);
+    if (e.sol) (void)hipFree(e.sol);
+    if (e.pvw) (void)hipFree(e.pvw);
+    if (e.anyp) (void)hipFree(e.anyp);
+  };
+  if (hipMalloc(&fac, sizeof(double) * nsys * k.nrdim * LDAB) != hipSuccess ||
+      hipMalloc(&piv, nsys * k.nrdim) != hipSuccess ||
+      hipMalloc(&e.sol, sizeof(double) * 8 * nrows) != hipSuccess ||
+      hipMalloc(&e.pvw, sizeof(uint16_t) * ((size_t)nkk * G * NC * 64 + 1)) != hipSuccess ||
+      hipMalloc(&e.anyp, sizeof(int) * ((size_t)nkk * G + 1)) != hipSuccess) {
+    cleanup();
+    MLX_THROW(MLX_E_ALLOC, "ihelm: cannot allocate LU factors");
+  }
+  HIPCHK(hipMemsetAsync(e.anyp, 0, sizeof(int) * ((size_t)nkk * G + 1), c->stream));
+  HIPCHK(hipMemsetAsync(e.pvw, 0, sizeof(uint16_t) * ((size_t)nkk * G * NC * 64 + 1), c->stream));
+  int *dinfo = reinterpret_cast<int *>(c->dsmall + 4000);
+  HIPCHK(hipMemsetAsync(dinfo, 0, sizeof(int), c->stream));
+  dim3 grid((nkk + 63) / 64, k.npchop);
+  hipLaunchKernelGGL(k_lu_factor, grid, dim3(64), 0, c->stream, fac, piv, k.d2c, k.lognorm, k.ak, k.nrdim, nkk, k.ne,
+                     nrc, k.npchop, alpha, kind, k.ell, dinfo);
+  hipLaunchKernelGGL(k_lu_repack, grid, dim3(64), 0, c->stream, fac, piv, fac_ptrs(e.sol, nrows), e.pvw, e.anyp,
+                     k.nrdim, nkk, nrc,
+                     k.npchop, kind);
+  HIPCHK(hipGetLastError());
+  int info = 0;
+  HIPCHK(hipMemcpyAsync(&info, dinfo, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  (void)hipFree(fac);
+  (void)hipFree(piv);
+  fac = nullptr;
+  piv = nullptr;
+  if (info != 0) {
+    cleanup();
+    MLX_THROW(MLX_E_ARG, "lurc: lu factorization resulted in failure");
+  }
+  e.stamp = ++c->lu_clock;
+  c->lu.push_back(e);
+  return c->lu.back();
+}
+
+}  // namespace
+
+namespace mlx {
+
+void k_helm_solve(mlx_ctx *c, double2 *e, double alpha, int kind, int nroff) {
+  const KitDev &k = c->kit;
+  LuCache &lu = lu_get(c, alpha, kind, nroff);
+  dim3 grid((k.npchop + 63) / 64, k.nzdim);
+  hipLaunchKernelGGL(k_lu_solve, grid, dim3(64), 0, c->stream, e, fac_ptrs(lu.sol, fac_rows(k, nroff)), lu.pvw, lu.anyp, k.nrdim, k.npdim, k.nzdim,
+                     k.nrchop + nroff, k.npchop, k.chopzl, k.chopzu, kind == 2 ? 1 : 0);
+  HIPCHK(hipGetLastError());
+}
+
+void lu_cache_free(mlx_ctx *c) {
+  for (auto &e : c->lu) {
+    (void)hipFree(e.sol);
+    (void)hipFree(e.pvw);
+    (void)hipFree(e.anyp);
+  }
+  c->lu.clear();
+}
+
+}  // namespace mlx

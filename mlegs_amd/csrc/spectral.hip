@@ -302,170 +302,6 @@ __global__ void k_helm_pre_(double2 *e, double *ln_out, const double *ln_in, con
   e[2].x -= 2.0 / 3.0 / ell2 / exp(lognorm[2]) * l;
 }
 
-// LAPACK band storage for LU with kl = ku = 2: ldab = 7, element (i, j) of the
-// matrix at ab[(kv + i - j) + 7 j] with kv = kl + ku = 4 (0-based).
-constexpr int KL = 2, KU = 2, KV = 4, LDAB = 7;
-
-__device__ inline double d2_entry(const double *d2c, const double *lognorm, int m, int ne, int i, int j, double akk,
-                                  double alpha, int bordered, double ell) {
-  // (del2 + alpha)(i, j) in the normalised basis; bordered (m=0,k=0 of
-  // idel2_proln): column 0 gains 4/3, -2, 2/3 terms (mlegs_scalar_ops.f90:705-711)
-  int d = j - i + 2;
-  double v = 0.0;
-  if (d >= 0 && d <= 4) {
-    v = d2c[((size_t)m * ne + i) * 5 + d];
-    if (d == 2) v = v - akk + alpha;
-  }
-  if (bordered && j == 0) {
-    double ell2 = ell * ell;
-    if (i == 0) v += 4.0 / 3.0 / ell2;
-    if (i == 1) v -= 2.0 / 1.0 / ell2 * exp(lognorm[0] - lognorm[1]);
-    if (i == 2) v += 2.0 / 3.0 / ell2 * exp(lognorm[0] - lognorm[2]);
-  }
-  return v;
-}
-
-// one thread per (m, k) system: gbtf2 on the band of (del2(m, ak) + alpha)
-__global__ void k_lu_factor(double *fac, int8_t *piv, const double *d2c, const double *lognorm, const double *ak,
-                            int nrdim, int nz, int ne, int nrc, int npc, double alpha, int kind, double ell,
-                            int *info) {
-  int k = blockIdx.x * blockDim.x + threadIdx.x;
-  int m = blockIdx.y;
-  if (k >= nz || m >= npc) return;
-  int n = nrcs_dev(nrc, npc, m);
-  if (n <= 0) return;
-  if (kind == 2 && m == 0 && k == 0) return;  // idel2_preln: (0,0) solved densely
-  size_t sys = (size_t)m * nz + k;
-  double *ab = fac + sys * (size_t)nrdim * LDAB;
-  int8_t *ip = piv + sys * (size_t)nrdim;
-  double akk = ak[k] * ak[k];
-  int bordered = (kind == 1 && m == 0 && k == 0) ? 1 : 0;
-  // fill: rows 0..KL-1 of each column are fill-in space
-  for (int j = 0; j < n; ++j) {
-    for (int r = 0; r < LDAB; ++r) ab[r + LDAB * j] = 0.0;
-    for (int i = (j - KU > 0 ? j - KU : 0); i <= j + KL && i < n; ++i)
-      ab[(KV + i - j) + LDAB * j] = d2_entry(d2c, lognorm, m, ne, i, j, akk, alpha, bordered, ell);
-  }
-  int ju = 0;
-  for (int j = 0; j < n; ++j) {
-    int km = (KL < n - 1 - j) ? KL : n - 1 - j;
-    // pivot search (first maximum of |.|, as idamax)
-    int jp = 0;
-    double best = fabs(ab[KV + LDAB * j]);
-    for (int r = 1; r <= km; ++r) {
-      double v = fabs(ab[KV + r + LDAB * j]);
-      if (v > best) { best = v; jp = r; }
-    }
-    ip[j] = (int8_t)jp;
-    if (ab[KV + jp + LDAB * j] != 0.0) {
-      int jnew = j + KU + jp;
-      if (jnew > n - 1) jnew = n - 1;
-      if (jnew > ju) ju = jnew;
-      if (jp != 0) {
-        // swap rows j and j+jp over columns j..ju
-        for (int c = j; c <= ju; ++c) {
-          double *a = &ab[(KV + j - c) + LDAB * c];
-          double *b = &ab[(KV + j + jp - c) + LDAB * c];
-          double t = *a; *a = *b; *b = t;
-        }
-      }
-      if (km > 0) {
-        double rp = 1.0 / ab[KV + LDAB * j];
-        for (int r = 1; r <= km; ++r) ab[KV + r + LDAB * j] *= rp;
-        for (int c = j + 1; c <= ju; ++c) {
-          double u = ab[(KV + j - c) + LDAB * c];
-          if (u != 0.0)
-            for (int r = 1; r <= km; ++r) ab[(KV + j + r - c) + LDAB * c] -= ab[KV + r + LDAB * j] * u;
-        }
-      }
-    } else {
-      atomicExch(info, 1);
-    }
-  }
-}
-
-// Repack the gbtf2 factors of each system into the solve layout (rows
-// contiguous per system in k-interleaved order; U stored by rows so that the
-// back substitution streams forward in memory).
-__global__ void k_lu_repack(const double *fac, const int8_t *piv, double *sol, const int *moff, int nrdim, int nz,
-                            int nrc, int npc, int kind) {
-  int k = blockIdx.x * blockDim.x + threadIdx.x;
-  int m = blockIdx.y;
-  if (k >= nz || m >= npc) return;
-  if (kind == 2 && m == 0 && k == 0) return;
-  int n = nrcs_dev(nrc, npc, m);
-  size_t sys = (size_t)m * nz + k;
-  const double *ab = fac + sys * (size_t)nrdim * LDAB;
-  const int8_t *ip = piv + sys * (size_t)nrdim;
-  double *F = sol + (size_t)moff[m] * 8 * nz + k;
-  for (int j = 0; j < n; ++j) {
-    double *Fj = F + (size_t)j * 8 * nz;
-    Fj[0] = (j + 1 < n) ? ab[KV + 1 + LDAB * j] : 0.0;
-    Fj[(size_t)1 * nz] = (j + 2 < n) ? ab[KV + 2 + LDAB * j] : 0.0;
-    for (int d = 0; d <= 4; ++d) Fj[(size_t)(2 + d) * nz] = (j + d < n) ? ab[(KV - d) + LDAB * (j + d)] : 0.0;
-    Fj[(size_t)7 * nz] = (double)ip[j];
-  }
-}
-
-// one thread per (m, k) system: gbtrs ('N') with real factors and complex rhs.
-// Forward sweep with a 3-entry register window (pivot offsets are <= kl = 2);
-// back substitution row by row, subtracting U(i, i+4) .. U(i, i+1) in the
-// order the column-oriented tbsv of LAPACK applies them.
-__global__ void __launch_bounds__(64) k_lu_solve(double2 *e, const double *__restrict__ sol,
-                                                 const int *__restrict__ moff, int nrdim, int npdim, int nz, int nrc,
-                                                 int npc, int nzc, int nzcu, int skip00) {
-  int k = blockIdx.x * blockDim.x + threadIdx.x;
-  int m = blockIdx.y;
-  if (k >= nz || m >= npc) return;
-  if (skip00 && m == 0 && k == 0) return;
-  int cnt = z_count(k, nzc, nzcu);
-  int n = nrcs_dev(nrc, npc, m);
-  if (cnt == 0 || n <= 0) return;
-  const double *F = sol + (size_t)moff[m] * 8 * nz + k;
-  const size_t rs = (size_t)8 * nz;
-  double2 *b = e + (size_t)k * nrdim * npdim + (size_t)nrdim * m;
-  const double2 z2 = make_double2(0.0, 0.0);
-  for (int rep = 0; rep < cnt; ++rep) {
-    double2 w0 = b[0];
-    double2 w1 = (n > 1) ? b[1] : z2;
-    double2 w2 = (n > 2) ? b[2] : z2;
-    for (int j = 0; j < n - 1; ++j) {
-      const double *Fj = F + (size_t)j * rs;
-      int p = (int)Fj[7 * (size_t)nz];
-      double l1 = Fj[0], l2 = Fj[nz];
-      if (p == 1) { double2 t = w0; w0 = w1; w1 = t; }
-      else if (p == 2) { double2 t = w0; w0 = w2; w2 = t; }
-      w1.x -= l1 * w0.x;
-      w1.y -= l1 * w0.y;
-      if (j + 2 < n) {
-        w2.x -= l2 * w0.x;
-        w2.y -= l2 * w0.y;
-      }
-      b[j] = w0;
-      w0 = w1;
-      w1 = w2;
-      w2 = (j + 3 < n) ? b[j + 3] : z2;
-    }
-    b[n - 1] = w0;
-    double2 x1 = z2, x2 = z2, x3 = z2, x4 = z2;
-    for (int i = n - 1; i >= 0; --i) {
-      const double *Fi = F + (size_t)i * rs;
-      double2 acc = b[i];
-      if (i + 4 < n) { double u = Fi[6 * (size_t)nz]; acc.x -= x4.x * u; acc.y -= x4.y * u; }
-      if (i + 3 < n) { double u = Fi[5 * (size_t)nz]; acc.x -= x3.x * u; acc.y -= x3.y * u; }
-      if (i + 2 < n) { double u = Fi[4 * (size_t)nz]; acc.x -= x2.x * u; acc.y -= x2.y * u; }
-      if (i + 1 < n) { double u = Fi[3 * (size_t)nz]; acc.x -= x1.x * u; acc.y -= x1.y * u; }
-      double d = Fi[2 * (size_t)nz];
-      double2 xi = make_double2(acc.x / d, acc.y / d);
-      b[i] = xi;
-      x4 = x3;
-      x3 = x2;
-      x2 = x1;
-      x1 = xi;
-    }
-  }
-}
-
 __global__ void k_idel2_post_(const double2 *e, double *ln, const double *lognorm) {
   if (threadIdx.x == 0 && blockIdx.x == 0) ln[0] = e[0].x * exp(lognorm[0]);
 }
@@ -668,72 +504,6 @@ inline int grid_for(size_t n) {
   return (int)b;
 }
 
-LuCache &lu_get(mlx_ctx *c, double alpha, int kind, int nroff) {
-  for (auto &e : c->lu)
-    if (e.alpha == alpha && e.kind == kind && e.nroff == nroff) {
-      e.stamp = ++c->lu_clock;
-      return e;
-    }
-  const KitDev &k = c->kit;
-  const int nrc = k.nrchop + nroff;
-  if (nrc > k.nrdim) MLX_THROW(MLX_E_ARG, "chop_index: chopping in r too large");
-  // evict the least recently used entry beyond 4
-  if (c->lu.size() >= 4) {
-    size_t victim = 0;
-    for (size_t i = 1; i < c->lu.size(); ++i)
-      if (c->lu[i].stamp < c->lu[victim].stamp) victim = i;
-    HIPCHK(hipStreamSynchronize(c->stream));
-    (void)hipFree(c->lu[victim].sol);
-    (void)hipFree(c->lu[victim].moff);
-    c->lu.erase(c->lu.begin() + victim);
-  }
-  LuCache e;
-  e.alpha = alpha;
-  e.kind = kind;
-  e.nroff = nroff;
-  std::vector<int> moff(k.npchop + 1, 0);
-  for (int m = 0; m < k.npchop; ++m) moff[m + 1] = moff[m] + nrcs_of(k, m, nroff);
-  size_t nsys = (size_t)k.npchop * k.nzdim;
-  double *fac = nullptr;
-  int8_t *piv = nullptr;
-  auto cleanup = [&]() {
-    if (fac) (void)hipFree(fac);
-    if (piv) (void)hipFree(piv);
-    if (e.sol) (void)hipFree(e.sol);
-    if (e.moff) (void)hipFree(e.moff);
-  };
-  if (hipMalloc(&fac, sizeof(double) * nsys * k.nrdim * LDAB) != hipSuccess ||
-      hipMalloc(&piv, nsys * k.nrdim) != hipSuccess ||
-      hipMalloc(&e.sol, sizeof(double) * ((size_t)moff[k.npchop] * 8 * k.nzdim + 1)) != hipSuccess ||
-      hipMalloc(&e.moff, sizeof(int) * moff.size()) != hipSuccess) {
-    cleanup();
-    MLX_THROW(MLX_E_ALLOC, "ihelm: cannot allocate LU factors");
-  }
-  HIPCHK(hipMemcpyAsync(e.moff, moff.data(), sizeof(int) * moff.size(), hipMemcpyHostToDevice, c->stream));
-  int *dinfo = reinterpret_cast<int *>(c->dsmall + 4000);
-  HIPCHK(hipMemsetAsync(dinfo, 0, sizeof(int), c->stream));
-  dim3 grid((k.nzdim + 63) / 64, k.npchop);
-  hipLaunchKernelGGL(k_lu_factor, grid, dim3(64), 0, c->stream, fac, piv, k.d2c, k.lognorm, k.ak, k.nrdim, k.nzdim,
-                     k.ne, nrc, k.npchop, alpha, kind, k.ell, dinfo);
-  hipLaunchKernelGGL(k_lu_repack, grid, dim3(64), 0, c->stream, fac, piv, e.sol, e.moff, k.nrdim, k.nzdim, nrc,
-                     k.npchop, kind);
-  HIPCHK(hipGetLastError());
-  int info = 0;
-  HIPCHK(hipMemcpyAsync(&info, dinfo, sizeof(int), hipMemcpyDeviceToHost, c->stream));
-  HIPCHK(hipStreamSynchronize(c->stream));
-  (void)hipFree(fac);
-  (void)hipFree(piv);
-  fac = nullptr;
-  piv = nullptr;
-  if (info != 0) {
-    cleanup();
-    MLX_THROW(MLX_E_ARG, "lurc: lu factorization resulted in failure");
-  }
-  e.stamp = ++c->lu_clock;
-  c->lu.push_back(e);
-  return c->lu.back();
-}
-
 }  // namespace
 
 namespace mlx {
@@ -825,15 +595,6 @@ void k_bandmul(mlx_ctx *c, double2 *e, double *ln, int op, int nroff) {
 void k_helm_pre(mlx_ctx *c, double2 *e, double *ln_out, const double *ln_in, double alpha) {
   hipLaunchKernelGGL(k_helm_pre_, dim3(1), dim3(64), 0, c->stream, e, ln_out, ln_in, c->kit.lognorm, alpha,
                      c->kit.ell);
-}
-
-void k_helm_solve(mlx_ctx *c, double2 *e, double alpha, int kind, int nroff) {
-  const KitDev &k = c->kit;
-  LuCache &lu = lu_get(c, alpha, kind, nroff);
-  dim3 grid((k.nzdim + 63) / 64, k.npchop);
-  hipLaunchKernelGGL(k_lu_solve, grid, dim3(64), 0, c->stream, e, lu.sol, lu.moff, k.nrdim, k.npdim, k.nzdim,
-                     k.nrchop + nroff, k.npchop, k.chopzl, k.chopzu, kind == 2 ? 1 : 0);
-  HIPCHK(hipGetLastError());
 }
 
 void k_idel2_post(mlx_ctx *c, const double2 *e, double *ln) {
