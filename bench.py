@@ -3,17 +3,25 @@
 
 Workload (one "step" = one AB2/CN time step of the 3D vortex driver,
 vortical_flow_3d.f90:160-181): NR = NP = NZ = 256, NRCHOP = 256,
-NPCHOP = NZCHOP = 129, L = 1, ZLEN = 2 pi, VISC = 5e-4, dt = 5e-4; initial
-state = centred q = 4 vortex + seeded 1e-4 perturbation (synthetic).  The
-bootstrap (FE-BE + Richardson) runs untimed, then --warmup steps, then
---steps timed steps bracketed by device synchronisation.
+NPCHOP = NZCHOP = 129, L = 1, ZLEN = 2 pi, VISC = 5e-4, dt = 5e-4, SVV on;
+initial state = centred q = 4 vortex + seeded 1e-4 perturbation (synthetic,
+SplitMix64/Box-Muller).  The FE-BE + Richardson bootstrap runs untimed, then
+--warmup steps, then --steps timed steps between device synchronisations.
 
-Printed: one JSON line with the steps/s of the selected precision (--prec,
-default fp32: Legendre contractions on FP32 MFMA, everything else FP64), the
-same-GPU FP64 plain path's steps/s, the spectral error of the mixed path
-against FP64 after the same steps, the roofline of the dominant kernel
-(timed live with HIP events on the library's stream) and a CPU baseline (the
-oracle on the host, bounded sample).
+Printed: one JSON line with
+  value            steps/s of the selected precision (--prec, default fp32:
+                   Legendre contractions on FP32 MFMA, everything else FP64)
+  fp64_steps_per_s the same-GPU FP64 path over the same steps
+  rel_err_vs_fp64  spectral error of the selected precision against FP64:
+                   one step from identical FP64 states (per field), after the
+                   whole run (trajectory), and per transform direction of one
+                   field (the transform-only workload's input)
+  roofline         the kernel with the most time in the timed region, timed
+                   with HIP events on the library's stream around each of its
+                   launches; algorithmic bytes or flops per launch come from
+                   the library (fixed per kernel and chop)
+  cpu_baseline     the oracle (numpy/scipy FP64) timing one AB2/CN step at the
+                   same size on the host
 """
 import argparse
 import json
@@ -26,8 +34,14 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 METRIC = 'steps/s @256³ mixed-prec vs FP64 at ≤1e-6 rel.err; adjoint/forward time ratio'
-PEAK = {'fp64': 78.6e12, 'fp32': 157.3e12, 'bf16x3': 2.5e15}
-HBM = 8.0e12
+PEAK_TFLOPS = {'fp64': 78.6, 'fp32': 157.3, 'bf16x3': 2500.0}
+HBM_GBPS = 8000.0
+HBM_BOUND = ('fft_phi', 'fft_z', 'lu_solve')
+TRAFFIC_FILE = os.path.join(ROOT, 'profiles', 'traffic_256.json')
+
+
+def log(*a):
+    print('[bench]', *a, file=sys.stderr, flush=True)
 
 
 def workload(n=256):
@@ -37,6 +51,7 @@ def workload(n=256):
 
 
 def initial_state(tfm, p, amp=1e-4, q=4.0):
+    """psi, chi of the q-vortex (qvort_dist_tp pattern, one vortex) plus the seeded perturbation."""
     import mlegs_amd as M
     from mlegs_amd import inputs
     dims = tfm.dims
@@ -55,24 +70,149 @@ def initial_state(tfm, p, amp=1e-4, q=4.0):
     return out
 
 
-def log(*a):
-    print('[bench]', *a, file=sys.stderr, flush=True)
+def rel(a, b):
+    import numpy as np
+    return float(np.linalg.norm((a - b).ravel()) / np.linalg.norm(b.ravel()))
 
 
-def run_steps(tfm, p, psi0, chi0, warmup, steps):
+def timed_run(tfm, prec, psi0, chi0, warmup, steps, probe=False):
+    """Bootstrap + warmup untimed, then `steps` timed steps; returns (run, seconds, probes)."""
     import mlegs_amd as M
+    tfm.set_precision(prec)
     run = M.VortexRun(tfm, psi0, chi0, M.Scalar(tfm))
     run.bootstrap()
-    tfm.sync()
     if warmup:
         run.step(warmup, check=True)
     tfm.sync()
+    if probe:
+        tfm.probe_start(64 * max(steps, 1))
     t0 = time.perf_counter()
     run.step(steps, check=False)
     tfm.sync()
     t1 = time.perf_counter()
-    run.step(0, check=True)  # report a non-finite state, if any
-    return run, (t1 - t0)
+    probes = tfm.probe_stop() if probe else None
+    run.step(0, check=True)  # raises on a non-finite state
+    return run, t1 - t0, probes
+
+
+def step_errors(tfm, prec, psi0, chi0, presteps=2):
+    """From the same FP64 state, steps in `prec` against steps in FP64.
+
+    Inside one step the transforms only produce the nonlinear terms N for the
+    next step, so the error of one step is the error of N (nlpsi, nlchi); two
+    steps carry it into psi and chi."""
+    import mlegs_amd as M
+    runs = []
+    for _ in range(2):
+        tfm.set_precision(M.PREC_FP64)
+        r = M.VortexRun(tfm, psi0, chi0, M.Scalar(tfm))
+        r.bootstrap()
+        r.step(presteps, check=True)
+        runs.append(r)
+    out = {}
+    for n, names in ((1, ('nlpsi', 'nlchi')), (2, ('psi', 'chi'))):
+        tfm.set_precision(M.PREC_FP64)
+        runs[0].step(1)
+        tfm.set_precision(prec)
+        runs[1].step(1)
+        out['%d_step' % n] = {nm: rel(runs[1].field(nm).e, runs[0].field(nm).e) for nm in names}
+    del runs
+    return out
+
+
+def transform_error(tfm, p, prec):
+    """Transform-only input (|c| ~ N(0,1) exp(-n/32)): backward and round-trip error vs FP64."""
+    import mlegs_amd as M
+    from mlegs_amd import inputs
+    e = inputs.transform_field(tfm.dims, (p.nrchop, p.npchop, p.nzchop, p.np, p.nz), field_id=3)
+    res = {}
+    for pr in (M.PREC_FP64, prec):
+        tfm.set_precision(pr)
+        g = M.Scalar(tfm).upload(e, space='FFF')
+        M.trans(g, 'PPP', tfm)
+        b = g.e[:p.nr, :p.np // 2].copy()
+        M.trans(g, 'FFF', tfm)
+        res[pr] = (b, g.e.copy())
+    (b64, f64), (bx, fx) = res[M.PREC_FP64], res[prec]
+    return {'backward': rel(bx, b64), 'round_trip': rel(fx, f64)}
+
+
+def roofline(probes, prec):
+    """The probe with the most kernel time; achieved = algorithmic work / kernel time."""
+    name = max(probes, key=lambda k: probes[k][1])
+    n, ms, nbytes, flops = probes[name]
+    per_launch_s = ms * 1e-3 / n
+    if name in HBM_BOUND:
+        achieved = nbytes / n / per_launch_s / 1e9
+        peak, unit, bound = HBM_GBPS, 'GB/s', 'hbm'
+    else:
+        achieved = flops / n / per_launch_s / 1e12
+        peak, unit, bound = PEAK_TFLOPS[prec], 'TFLOP/s', 'mfma'
+    traffic = None
+    if os.path.exists(TRAFFIC_FILE):
+        with open(TRAFFIC_FILE) as f:
+            t = json.load(f)
+        entry = t.get('kernels', {}).get(name)
+        if entry and entry.get('precision') in (prec, 'any'):
+            traffic = entry.get('hbm_bytes_per_launch')
+    return {'bound': bound, 'kernel': name, 'achieved': achieved, 'peak': peak, 'unit': unit,
+            'frac': achieved / peak, 'traffic': traffic,
+            'launches': n, 'avg_launch_us': 1e3 * ms / n,
+            'algorithmic_bytes_per_launch': nbytes / n, 'algorithmic_flops_per_launch': flops / n}
+
+
+def probe_table(probes):
+    out = {}
+    for k, (n, ms, nbytes, flops) in probes.items():
+        if n == 0:
+            continue
+        s = ms * 1e-3 / n
+        out[k] = {'launches': n, 'avg_us': round(1e6 * s, 2), 'total_ms': round(ms, 3),
+                  'GB_per_s': round(nbytes / n / s / 1e9, 1), 'TFLOP_per_s': round(flops / n / s / 1e12, 2)}
+    return out
+
+
+def cpu_baseline(p):
+    """The oracle (numpy/scipy, FP64) on the host: one AB2/CN step at the workload's
+    size from a synthetic state whose history (psi_prev, N, N_prev) is set
+    directly -- the step's cost does not depend on the values, so the
+    bootstrap is not run.  A bounded sample of the workload: one step."""
+    from threadpoolctl import threadpool_info
+    from oracle import kit as K, scalar_ops as so, vortex as V
+    from mlegs_amd import inputs
+    okit = K.Kit3D(p)
+    dims = (okit.nrdim, okit.npdim, okit.nzdim)
+    ch = (p.nrchop, p.npchop, p.nzchop, p.np, p.nz)
+    psi = so.Scalar(okit, 'FFF')
+    psi.e = inputs.perturbation(dims, ch, 1e-4, field_id=1)
+    chi = so.Scalar(okit, 'FFF')
+    chi.e = inputs.perturbation(dims, ch, 1e-4, field_id=2)
+    run = V.VortexRun(okit, p, psi, chi, so.Scalar(okit, 'PPP'))
+    run.psi_prev, run.chi_prev = psi.copy(), chi.copy()
+    run.nlpsi, run.nlchi = so.Scalar(okit, 'FFF'), so.Scalar(okit, 'FFF')
+    run.nlpsi_prev, run.nlchi_prev = so.Scalar(okit, 'FFF'), so.Scalar(okit, 'FFF')
+    t0 = time.perf_counter()
+    run.step()
+    t = time.perf_counter() - t0
+    threads = max([i.get('num_threads', 1) for i in threadpool_info()] + [1])
+    return {'value': 1.0 / t, 'unit': 'steps/s', 'cores': threads, 'kind': 'port',
+            'sample': 'oracle AB2/CN step (numpy/scipy FP64, BLAS on %d threads, FFTs single-threaded) at '
+                      'NR=NP=NZ=%d, one step timed: %.2f s' % (threads, p.nr, t)}
+
+
+def kernel_sample(tfm, p, prec, iters):
+    """Each probed kernel family on one field of the workload's size (plus the copy)."""
+    import mlegs_amd as M
+    from mlegs_amd import inputs
+    tfm.set_precision(prec)
+    e = inputs.transform_field(tfm.dims, (p.nrchop, p.npchop, p.nzchop, p.np, p.nz), field_id=3)
+    out = {}
+    for w in ('copy', 'fft_phi_fwd', 'fft_phi_bwd', 'fft_z_fwd', 'fft_z_bwd', 'leg_fwd', 'leg_bwd', 'v2t',
+              'lu_solve'):
+        g = M.Scalar(tfm).upload(e, space='FFF')
+        out[w] = round(1e3 * tfm.time_kernel(g, w, iters), 2)
+        log(w, out[w], 'us')
+    print(json.dumps({'kernel_sample_us': out, 'iters': iters, 'precision': prec}), flush=True)
 
 
 def main():
@@ -80,117 +220,74 @@ def main():
     ap.add_argument('--gpus', type=int, default=1)
     ap.add_argument('--steps', type=int, default=20)
     ap.add_argument('--warmup', type=int, default=3)
-    ap.add_argument('--prec', default='fp64', choices=['fp64', 'fp32', 'bf16x3'])
+    ap.add_argument('--prec', default='fp32', choices=['fp64', 'fp32', 'bf16x3'])
     ap.add_argument('--size', type=int, default=256)
-    ap.add_argument('--fp64-steps', type=int, default=5, help='timed steps of the FP64 reference run')
-    ap.add_argument('--kernel-iters', type=int, default=20)
     ap.add_argument('--cpu-baseline', type=int, default=1)
+    ap.add_argument('--errors', type=int, default=1, help='measure the errors against FP64')
+    ap.add_argument('--kernel-sample', type=int, default=0,
+                    help='only launch each probed kernel family (and a plain field copy) this many times, '
+                         'for hardware-counter runs; prints their mean durations')
     args = ap.parse_args()
     if args.gpus != 1:
-        print('bench.py: this build runs on one GPU; --gpus %d is not supported (multi-GPU sharding is a '
-              'design note, not part of this build)' % args.gpus, file=sys.stderr)
+        print('bench.py: this build runs on one GPU; --gpus %d is not supported (the multi-GPU z-slab '
+              'sharding is a design note, not part of this build)' % args.gpus, file=sys.stderr)
         sys.exit(2)
 
-    import numpy as np
     import mlegs_amd as M
 
     p = workload(args.size)
     precs = {'fp64': M.PREC_FP64, 'fp32': M.PREC_FP32, 'bf16x3': M.PREC_BF16X3}
+    prec = precs[args.prec]
     log('kit', p.nr, p.np, p.nz)
     tfm = M.Kit(p, device=0, precision=M.PREC_FP64)
+    if args.kernel_sample:
+        kernel_sample(tfm, p, prec, args.kernel_sample)
+        return
     psi0, chi0 = initial_state(tfm, p)
 
-    # FP64 plain path (reference numerics)
-    fp64_rate = None
-    run64 = None
-    if args.prec != 'fp64' or True:
-        tfm.set_precision(M.PREC_FP64)
-        k64 = args.fp64_steps if args.prec != 'fp64' else args.steps
-        w64 = 1 if args.prec != 'fp64' else args.warmup
-        log('fp64 run', w64, k64)
-        run64, t64 = run_steps(tfm, p, psi0, chi0, w64, k64)
-        fp64_rate = k64 / t64
-        ms64 = 1e3 * t64 / k64
-    # selected precision
-    if args.prec == 'fp64':
-        rate, ms, run = fp64_rate, ms64, run64
-        nsteps_total = args.warmup + args.steps
-        err = {'psi': 0.0, 'chi': 0.0}
-    else:
-        tfm.set_precision(precs[args.prec])
-        log(args.prec, 'run', args.warmup, args.steps)
-        run, t = run_steps(tfm, p, psi0, chi0, args.warmup, args.steps)
-        rate = args.steps / t
-        ms = 1e3 * t / args.steps
-        # error vs FP64 after the same number of steps
-        tfm.set_precision(M.PREC_FP64)
-        ref, _ = run_steps(tfm, p, psi0, chi0, args.warmup, args.steps)
-        err = {}
-        for nm in ('psi', 'chi'):
-            a = run.field(nm).e
-            b = ref.field(nm).e
-            err[nm] = float(np.linalg.norm((a - b).ravel()) / np.linalg.norm(b.ravel()))
-        tfm.set_precision(precs[args.prec])
+    log(args.prec, 'run: warmup', args.warmup, 'steps', args.steps)
+    run, t, probes = timed_run(tfm, prec, psi0, chi0, args.warmup, args.steps, probe=True)
+    rate, ms = args.steps / t, 1e3 * t / args.steps
+    roof = roofline(probes, args.prec)
+    log('%.3f ms/step; dominant kernel %s %.1f us/launch' % (ms, roof['kernel'], roof['avg_launch_us']))
 
-    # dominant kernel: the forward Legendre contraction, timed with HIP events
-    nr, nz = p.nr, p.nz
-    S = sum(max(p.nrchop - m, 0) for m in range(p.npchop))
-    f_leg = 4.0 * (nr // 2) * nz * S
-    probe = M.Scalar(tfm).upload(np.zeros(tfm.dims, dtype=np.complex128), space='PFP')
-    kern = {}
-    for w in ('leg_fwd', 'leg_bwd', 'fft_phi_fwd', 'fft_phi_bwd', 'fft_z_fwd', 'fft_z_bwd'):
-        kern[w] = tfm.time_kernel(probe, w, args.kernel_iters)
-    t_leg = kern['leg_fwd'] * 1e-3
-    achieved = f_leg / t_leg
-    peak = PEAK[args.prec]
-    roofline = {'bound': 'mfma', 'kernel': 'leg_fwd (%s)' % args.prec, 'achieved': achieved / 1e12,
-                'peak': peak / 1e12, 'unit': 'TFLOP/s', 'frac': achieved / peak, 'traffic': None,
-                'flops_per_launch': f_leg, 'ms_per_launch': kern['leg_fwd']}
-    nb = 16.0 * p.nr * (p.np // 2) * p.nz * 2
-    kernels_ms = {k: round(v, 4) for k, v in kern.items()}
-    fft_bw = {k: nb / (v * 1e-3) / 1e9 for k, v in kern.items() if k.startswith('fft')}
+    err = None
+    if args.prec == 'fp64':
+        fp64_rate = rate
+    else:
+        log('fp64 run')
+        run64, t64, _ = timed_run(tfm, M.PREC_FP64, psi0, chi0, args.warmup, args.steps)
+        fp64_rate = args.steps / t64
+        if args.errors:
+            err = {'trajectory_%d_steps' % (args.warmup + args.steps):
+                   {nm: rel(run.field(nm).e, run64.field(nm).e) for nm in ('psi', 'chi')}}
+            del run64
+            err.update(step_errors(tfm, prec, psi0, chi0))
+            err['transform'] = transform_error(tfm, p, prec)
+            log('errors', err)
+    del run
+    tfm.close()
 
     cpu = None
     if args.cpu_baseline:
+        log('cpu baseline (oracle step)')
         cpu = cpu_baseline(p)
 
+    dtype = {'fp64': 'f64', 'fp32': 'f32-mfma+f64', 'bf16x3': 'bf16x3-mfma+f64'}[args.prec]
     out = {
         'metric': METRIC, 'value': rate, 'unit': 'steps/s', 'n_gpus': 1, 'steps': args.steps,
         'warmup': args.warmup, 'ms_per_step': ms, 'higher_is_better': True, 'scaling': 'weak',
-        'vs_baseline': None, 'dtype': {'fp64': 'f64', 'fp32': 'f32-mfma+f64', 'bf16x3': 'bf16x3-mfma+f64'}[args.prec],
+        'vs_baseline': None, 'dtype': dtype,
         'data': 'synthetic: centred q=4 vortex + seeded 1e-4 perturbation (SplitMix64/Box-Muller, seed 20261202)',
         'config': {'workload': '3D vortex AB2/CN time step, NR=NP=NZ=%d, NRCHOP=%d, NPCHOP=NZCHOP=%d, L=1, '
-                               'VISC=5e-4, dt=5e-4, SVV on' % (p.nr, p.nrchop, p.npchop),
+                               'ZLEN=2pi, VISC=5e-4, dt=5e-4, SVV on' % (p.nr, p.nrchop, p.npchop),
                    'precision': args.prec},
-        'fp64_steps_per_s': fp64_rate, 'speedup_vs_fp64': (rate / fp64_rate) if fp64_rate else None,
-        'rel_err_vs_fp64': err, 'kernels_ms': kernels_ms,
-        'fft_GBps': {k: round(v, 1) for k, v in fft_bw.items()},
-        'roofline': roofline, 'cpu_baseline': cpu,
+        'fp64_steps_per_s': fp64_rate, 'speedup_vs_fp64': rate / fp64_rate,
+        'rel_err_vs_fp64': err, 'adjoint_forward_ratio': None,
+        'kernels': probe_table(probes),
+        'roofline': roof, 'cpu_baseline': cpu,
     }
     print(json.dumps(out), flush=True)
-
-
-def cpu_baseline(p):
-    """The oracle on the host: one transform round trip of one field at the
-    workload's size (the bounded sample), scaled to steps/s by the number of
-    transform-equivalents per step (13 Legendre-sized contractions)."""
-    import numpy as np
-    from oracle import kit as K, scalar_ops as so
-    from mlegs_amd import inputs
-    t0 = time.perf_counter()
-    okit = K.Kit3D(p)
-    tk = time.perf_counter() - t0
-    e = inputs.transform_field((okit.nrdim, okit.npdim, okit.nzdim), (p.nrchop, p.npchop, p.nzchop, p.np, p.nz))
-    s = so.Scalar(okit, 'FFF')
-    s.e = e
-    t0 = time.perf_counter()
-    so.trans(s, 'PPP', okit)
-    so.trans(s, 'FFF', okit)
-    t = time.perf_counter() - t0
-    per_step = t * 13.0 / 2.0
-    return {'value': 1.0 / per_step, 'unit': 'steps/s', 'cores': os.cpu_count(), 'kind': 'port',
-            'sample': 'oracle (numpy/scipy FP64) one backward+forward transform of one 256^3 field (%.2f s), '
-                      'scaled by 6.5 round-trip equivalents per step; kit build %.1f s' % (t, tk)}
 
 
 if __name__ == '__main__':

@@ -50,6 +50,15 @@ extern "C" {
 #define MLX_PREC_FP32   1    /* FP32 operands, FP32 MFMA accumulation       */
 #define MLX_PREC_BF16X3 2    /* 3-term split-BF16 operands, FP32 MFMA acc.   */
 
+/* kernel probes (measurement): see mlx_probe_start */
+#define MLX_PROBE_FFT_PHI   0  /* phi FFT, one launch per direction       */
+#define MLX_PROBE_FFT_Z     1  /* z FFT                                     */
+#define MLX_PROBE_LEG_FWD   2  /* forward Legendre contraction             */
+#define MLX_PROBE_LEG_BWD   3  /* backward Legendre contraction            */
+#define MLX_PROBE_V2T       4  /* vec2tp projection (5 contractions)       */
+#define MLX_PROBE_LU_SOLVE  5  /* banded LU solve of ihelm / idel2         */
+#define MLX_PROBE_COUNT     6
+
 typedef struct mlx_ctx mlx_ctx;
 typedef struct mlx_field mlx_field;
 
@@ -224,9 +233,21 @@ int mlx_energy(mlx_ctx* ctx, const mlx_field* vr, const mlx_field* vp, const mlx
 /* Time `iters` launches of one kernel family on the context stream with HIP
  * events (after one warm-up launch) and return the mean duration in ms.
  * which: "leg_fwd", "leg_bwd", "fft_phi_fwd", "fft_phi_bwd", "fft_z_fwd",
- * "fft_z_bwd".  The field is used as operand and left in an unspecified
- * state of the same space. */
+ * "fft_z_bwd", "v2t" (the vec2tp projection with the field as all three
+ * components), "lu_solve" (the ihelm solve of the AB2/CN step), "copy" (a
+ * plain copy of the field, 32 bytes per complex entry of HBM traffic).  The
+ * field is used as operand and left in an unspecified state of the same
+ * space. */
 int mlx_time_kernel(mlx_ctx* ctx, mlx_field* f, const char* which, int32_t iters, double* ms);
+
+/* Measurement probes, no counterpart in the reference.  After
+ * mlx_probe_start, every launch of a probed kernel (MLX_PROBE_*) is bracketed
+ * by HIP events on the context stream, up to max_launches per probe, and its
+ * algorithmic bytes and flops are summed.  mlx_probe_stop synchronises,
+ * switches probing off and writes, per probe i, out[4i..4i+3] = launches
+ * recorded, total kernel time in ms, algorithmic bytes, algorithmic flops. */
+int mlx_probe_start(mlx_ctx* ctx, int32_t max_launches);
+int mlx_probe_stop(mlx_ctx* ctx, double* out);
 
 #ifdef __cplusplus
 }

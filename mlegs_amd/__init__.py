@@ -99,6 +99,19 @@ class Kit:
         self._check(self.lib.mlx_time_kernel(self._ctx, s.h, which.encode(), int(iters), ctypes.byref(ms)))
         return ms.value
 
+    PROBES = ('fft_phi', 'fft_z', 'leg_fwd', 'leg_bwd', 'v2t', 'lu_solve')
+
+    def probe_start(self, max_launches=4096):
+        """Bracket every launch of the probed kernels with HIP events from now on."""
+        self._check(self.lib.mlx_probe_start(self._ctx, int(max_launches)))
+
+    def probe_stop(self):
+        """{probe: (launches, total ms, algorithmic bytes, algorithmic flops)} since probe_start."""
+        out = (ctypes.c_double * (4 * len(self.PROBES)))()
+        self._check(self.lib.mlx_probe_stop(self._ctx, out))
+        return {nm: (int(out[4 * i]), out[4 * i + 1], out[4 * i + 2], out[4 * i + 3])
+                for i, nm in enumerate(self.PROBES)}
+
 
 class Scalar:
     """type(scalar) on the device: data e(nrdim, npdim, nzdim), ln, chop offsets, space."""

@@ -94,6 +94,8 @@ SIGNATURES = {
     'mlx_vortex_gains': ([_P, _P, _DP, _DP], ctypes.c_int),
     'mlx_energy': ([_P, _P, _P, _P, _DP], ctypes.c_int),
     'mlx_time_kernel': ([_P, _P, _S, _I, _DP], ctypes.c_int),
+    'mlx_probe_start': ([_P, _I], ctypes.c_int),
+    'mlx_probe_stop': ([_P, _DP], ctypes.c_int),
 }
 
 _lib = None

@@ -101,7 +101,7 @@ void trans(mlx_ctx *c, mlx_field *f, const char *space) {
         set_space(f, "FFP");
       }
       if (cur == 2) {
-        if (k.nz > 1) fft_z(c, f->e, -1);
+        if (k.nz > 1) fft_z(c, f->e, -1, active_pairs(k, f->off[0]));
         set_space(f, "FFF");
       }
       if (cur == 3) break;
@@ -112,7 +112,7 @@ void trans(mlx_ctx *c, mlx_field *f, const char *space) {
   if (cur > nw) {
     while (true) {
       if (cur == 3) {
-        if (k.nz > 1) fft_z(c, f->e, +1);
+        if (k.nz > 1) fft_z(c, f->e, +1, active_pairs(k, f->off[0]));
         set_space(f, "FFP");
       }
       if (cur == 2) {
@@ -271,7 +271,7 @@ void vec2tp(mlx_ctx *c, const mlx_field *vr, const mlx_field *vp, const mlx_fiel
   k_scale_rows_r(c, ur->e, 0);
   k_scale_rows_r(c, up->e, 0);
   trans(c, ur, "PFP");
-  if (k.nz > 1) fft_z(c, ur->e, -1);
+  if (k.nz > 1) fft_z(c, ur->e, -1, (double)k.nr * k.npchop);
   trans(c, up, "FFF");
   double2 *inf = reinterpret_cast<double2 *>(c->dsmall + 2048);
   k_calcat(c, up->e, k.at1, k.nrchop + up->off[0], inf);
@@ -279,7 +279,7 @@ void vec2tp(mlx_ctx *c, const mlx_field *vr, const mlx_field *vp, const mlx_fiel
   swap_scr(c, up);
   k_vec2tp_up_ln(c, up->e, inf, psi->sc);
   trans(c, uz, "PFP");
-  if (k.nz > 1) fft_z(c, uz->e, -1);
+  if (k.nz > 1) fft_z(c, uz->e, -1, (double)k.nr * k.npchop);
   vec2tp_project(c, ur->e, up->e, uz->e, psi->e, chi->e, 3);
   set_space(psi, "FFF");
   set_space(chi, "FFF");
@@ -384,6 +384,8 @@ int mlx_ctx_destroy(mlx_ctx *c) {
   for (mlx_field *f : c->pool) field_free(f);
   c->pool.clear();
   lu_cache_free(c);
+  for (auto &v : c->probe.ev)
+    for (hipEvent_t e : v) (void)hipEventDestroy(e);
   for (auto &kv : c->work)
     if (kv.second.d) (void)hipFree(kv.second.d);
   for (double2 *s : c->scr)
@@ -867,8 +869,11 @@ int mlx_time_kernel(mlx_ctx *c, mlx_field *f, const char *which, int32_t iters, 
       else if (w == "leg_bwd") leg_backward(c, f->e, c->scr[0], 0, false);
       else if (w == "fft_phi_fwd") fft_phi(c, f->e, -1);
       else if (w == "fft_phi_bwd") fft_phi(c, f->e, +1);
-      else if (w == "fft_z_fwd") fft_z(c, f->e, -1);
-      else if (w == "fft_z_bwd") fft_z(c, f->e, +1);
+      else if (w == "fft_z_fwd") fft_z(c, f->e, -1, active_pairs(c->kit, 0));
+      else if (w == "fft_z_bwd") fft_z(c, f->e, +1, active_pairs(c->kit, 0));
+      else if (w == "v2t") vec2tp_project(c, f->e, f->e, f->e, c->scr[0], c->scr[1], 3);
+      else if (w == "lu_solve") k_helm_solve(c, f->e, -2.0 / (c->p.dt * c->p.visc), 0, 0);
+      else if (w == "copy") k_copy_field(c, c->scr[0], f->e);
       else MLX_THROW(MLX_E_ARG, "mlx_time_kernel: unknown kernel " + w);
     };
     if (iters < 1) MLX_THROW(MLX_E_ARG, "mlx_time_kernel: iters must be positive");
@@ -885,6 +890,45 @@ int mlx_time_kernel(mlx_ctx *c, mlx_field *f, const char *which, int32_t iters, 
     (void)hipEventDestroy(a);
     (void)hipEventDestroy(b);
     *ms = (double)t / iters;
+  });
+}
+
+int mlx_probe_start(mlx_ctx *c, int32_t max_launches) {
+  return guarded(c, [&] {
+    if (max_launches < 1) MLX_THROW(MLX_E_ARG, "mlx_probe_start: max_launches must be positive");
+    ProbeSet &p = c->probe;
+    for (int i = 0; i < MLX_PROBE_COUNT; ++i) {
+      while ((int)p.ev[i].size() < 2 * max_launches) {
+        hipEvent_t e;
+        HIPCHK(hipEventCreate(&e));
+        p.ev[i].push_back(e);
+      }
+      p.used[i] = 0;
+      p.bytes[i] = 0.0;
+      p.flops[i] = 0.0;
+    }
+    p.cap = max_launches;
+    p.on = true;
+  });
+}
+
+int mlx_probe_stop(mlx_ctx *c, double *out) {
+  return guarded(c, [&] {
+    ProbeSet &p = c->probe;
+    p.on = false;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    for (int i = 0; i < MLX_PROBE_COUNT; ++i) {
+      double tot = 0.0;
+      for (int j = 0; j < p.used[i]; ++j) {
+        float t = 0.f;
+        HIPCHK(hipEventElapsedTime(&t, p.ev[i][2 * j], p.ev[i][2 * j + 1]));
+        tot += t;
+      }
+      out[4 * i] = p.used[i];
+      out[4 * i + 1] = tot;
+      out[4 * i + 2] = p.bytes[i];
+      out[4 * i + 3] = p.flops[i];
+    }
   });
 }
 

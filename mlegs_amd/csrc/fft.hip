@@ -16,6 +16,8 @@
 // mixed-radix (4,2,3,5) Stockham FFT over the lines, and writes back.  The
 // real-to-complex phi transform is a complex FFT of length np/2 plus the
 // even/odd untangling step.  HBM-bound: one read and one write per element.
+#include <cmath>
+
 #include "mlx_internal.h"
 
 namespace {
@@ -246,17 +248,21 @@ void fft_phi(mlx_ctx *c, double2 *e, int dir) {
   if (k.nph > 512) MLX_THROW(MLX_E_ARG, "fft_phi: np/2 above 512 is not supported");
   int TI = tile_rows(k.nph, k.nrdim);
   dim3 grid((k.nrdim + TI - 1) / TI, k.nzdim);
+  // packed reals in, nph+1 modes out (or back): nr rows x nz slices
+  ProbeScope probe(c, MLX_PROBE_FFT_PHI, 16.0 * k.nr * k.nz * (2.0 * k.nph + 1),
+                   2.5 * k.np * std::log2((double)k.np) * k.nr * k.nz);
   hipLaunchKernelGGL(k_fft_phi, grid, dim3(kThreads), 0, c->stream, e, k.nrdim, k.npdim, k.nzdim, TI, k.plan_nph,
                      k.tw_nph, k.tw_half, dir);
   HIPCHK(hipGetLastError());
 }
 
-void fft_z(mlx_ctx *c, double2 *e, int dir) {
+void fft_z(mlx_ctx *c, double2 *e, int dir, double ncol) {
   const KitDev &k = c->kit;
   if (k.nz <= 1) return;
   if (k.nz > 512) MLX_THROW(MLX_E_ARG, "fft_z: nz above 512 is not supported");
   int TI = tile_rows(k.nz, k.nrdim);
   dim3 grid((k.nrdim + TI - 1) / TI, k.npdim);
+  ProbeScope probe(c, MLX_PROBE_FFT_Z, 32.0 * ncol * k.nz, 5.0 * k.nz * std::log2((double)k.nz) * ncol);
   hipLaunchKernelGGL(k_fft_z, grid, dim3(kThreads), 0, c->stream, e, k.nrdim, k.npdim, k.nzdim, TI, k.plan_nz,
                      k.tw_nz, dir);
   HIPCHK(hipGetLastError());

@@ -424,6 +424,7 @@ void leg_forward_f32(mlx_ctx *c, const double2 *in, double2 *out) {
   const KitDev &k = c->kit;
   dim3 grid(k.npdim, (k.nzdim + KT - 1) / KT);
   int at = mfma_tiles(k);
+  ProbeScope probe(c, MLX_PROBE_LEG_FWD, leg_bytes(k, 0, 4), leg_flops(k, 0));
   if (at == 1)
     hipLaunchKernelGGL(k_leg_fwd_f32<1>, grid, dim3(256), 0, c->stream, in, out, k.pfg32, k.w, k.nrdim, k.npdim,
                        k.nzdim, k.nr, k.nrh, k.nrchop, k.npchop);
@@ -437,6 +438,7 @@ void leg_backward_f32(mlx_ctx *c, const double2 *in, double2 *out) {
   const KitDev &k = c->kit;
   dim3 grid(k.npdim, (k.nzdim + KT - 1) / KT);
   int xt = (k.nrh + 63) / 64;
+  ProbeScope probe(c, MLX_PROBE_LEG_BWD, leg_bytes(k, 0, 4), leg_flops(k, 0));
   if (xt == 1)
     hipLaunchKernelGGL(k_leg_bwd_f32<1>, grid, dim3(256), 0, c->stream, in, out, k.pf32, k.nrdim, k.npdim, k.nzdim,
                        k.nr, k.nrh, k.ne, k.nrchop, k.npchop);
@@ -453,6 +455,7 @@ void vec2tp_project_f32(mlx_ctx *c, const double2 *ur, const double2 *up, const 
   HIPCHK(hipMemsetAsync(chi, 0, sizeof(double2) * c->nelem, c->stream));
   dim3 grid(2 * k.npchop, (k.nzdim + KT - 1) / KT);
   int at = mfma_tiles(k);
+  ProbeScope probe(c, MLX_PROBE_V2T, v2t_bytes(k, 0, 4), 5.0 * leg_flops(k, 0));
   if (at == 1)
     hipLaunchKernelGGL(k_v2t_f32<1>, grid, dim3(256), 0, c->stream, ur, up, uz, psi, chi, k.tVg32, k.tDg32, k.tTg32,
                        k.ak, k.nrdim, k.npdim, k.nzdim, k.nr, k.nrh, k.nrchop, k.npchop, k.chopzl, k.chopzu);

@@ -90,6 +90,18 @@ struct LuCache {
 
 struct mlx_vortex;
 
+// Kernel probes: while on, HIP events are recorded on the context stream
+// around each launch of the probed kernels, with the launch's algorithmic
+// bytes and flops (the data it must move once / the arithmetic it must do).
+struct ProbeSet {
+  bool on = false;
+  int cap = 0;
+  std::vector<hipEvent_t> ev[MLX_PROBE_COUNT];
+  int used[MLX_PROBE_COUNT] = {};
+  double bytes[MLX_PROBE_COUNT] = {};
+  double flops[MLX_PROBE_COUNT] = {};
+};
+
 // device list of GEMM work items (m, parity, first row, row count)
 struct WorkTable {
   int4 *d = nullptr;
@@ -113,7 +125,37 @@ struct mlx_ctx {
   std::vector<mlx_field *> pool;  // temporaries (allocated on demand)
   std::vector<LuCache> lu;
   uint64_t lu_clock = 0;
+  ProbeSet probe;
 };
+
+// Records the opening event on construction and the closing one on
+// destruction when probing is on and the probe has room left.
+struct ProbeScope {
+  mlx_ctx *c;
+  int id;
+  int slot = -1;
+  ProbeScope(mlx_ctx *c_, int id_, double bytes, double flops) : c(c_), id(id_) {
+    ProbeSet &p = c->probe;
+    if (!p.on || p.used[id] >= p.cap) return;
+    slot = p.used[id]++;
+    p.bytes[id] += bytes;
+    p.flops[id] += flops;
+    (void)hipEventRecord(p.ev[id][2 * slot], c->stream);
+  }
+  ~ProbeScope() {
+    if (slot >= 0) (void)hipEventRecord(c->probe.ev[id][2 * slot + 1], c->stream);
+  }
+};
+
+// active (n, m) pairs for a radial chop offset: sum over m < npchop of nrcs(m)
+static inline double active_pairs(const KitDev &k, int nroff) {
+  double s = 0;
+  for (int m = 0; m < k.npchop; ++m) {
+    int v = k.nrchop + nroff - m;
+    s += v > 0 ? v : 0;
+  }
+  return s;
+}
 
 // ---------------------------------------------------------------------------
 // launch helpers (defined in the .hip files)
@@ -126,7 +168,9 @@ void kit_free(KitDev &k);
 
 // fft.hip
 void fft_phi(mlx_ctx *c, double2 *e, int dir);  // dir -1 forward (PPP->PFP), +1 backward
-void fft_z(mlx_ctx *c, double2 *e, int dir);    // dir -1 forward (/nz), +1 backward
+// ncol: the columns the caller needs transformed (algorithmic work of the
+// launch); the kernel transforms every column of the field.
+void fft_z(mlx_ctx *c, double2 *e, int dir, double ncol);  // dir -1 forward (/nz), +1 backward
 
 // legendre.hip
 void leg_forward(mlx_ctx *c, const double2 *in, double2 *out, int nroff, bool zero = true);
@@ -156,6 +200,7 @@ void lu_cache_free(mlx_ctx *c);  // solve.hip
 void k_idel2_post(mlx_ctx *c, const double2 *e, double *ln);
 void k_idel2_preln00(mlx_ctx *c, double2 *e, double *ln, double preln, int nroff);
 void k_axpby(mlx_ctx *c, double2 *dst, double *dln, double a, const double2 *src, const double *sln, double b);
+void k_copy_field(mlx_ctx *c, double2 *dst, const double2 *src);
 void k_abcn_rhs(mlx_ctx *c, double2 *sh, double *shln, const double2 *s, const double *sln,
                 const double2 *nl, const double *nlln, const double2 *nlp, const double *nlpln,
                 const double2 *svis, double dt, double a, int ab2);
@@ -170,6 +215,25 @@ void k_set_scalar(mlx_ctx *c, double *dst, double v);
 void k_copy_scalar(mlx_ctx *c, double *dst, const double *src, double scale);
 
 }  // namespace mlx
+
+// Algorithmic work of one Legendre contraction of a whole field (one
+// direction): the parity-folded real table times complex data, FMA = 2 flops,
+// 4 (nr/2) nz S flops; bytes = physical columns m < npchop read (or written)
+// once, spectral coefficients written (or read) once, table read once at
+// tbytes per entry.
+static inline double leg_flops(const KitDev &k, int nroff) {
+  return 4.0 * k.nrh * k.nzdim * active_pairs(k, nroff);
+}
+static inline double leg_bytes(const KitDev &k, int nroff, int tbytes) {
+  double S = active_pairs(k, nroff);
+  return 16.0 * k.nr * k.npchop * k.nzdim + 16.0 * S * k.nzdim + (double)tbytes * k.nrh * S;
+}
+// vec2tp projection: three physical fields in, two spectral fields out,
+// tables V, D, T
+static inline double v2t_bytes(const KitDev &k, int nroff, int tbytes) {
+  double S = active_pairs(k, nroff);
+  return 48.0 * k.nr * k.npchop * k.nzdim + 32.0 * S * k.nzdim + 3.0 * tbytes * k.nrh * S;
+}
 
 // helpers shared by host code
 static inline int nrcs_of(const KitDev &k, int m, int nroff) {

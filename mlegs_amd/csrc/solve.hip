@@ -384,6 +384,10 @@ void k_helm_solve(mlx_ctx *c, double2 *e, double alpha, int kind, int nroff) {
   const KitDev &k = c->kit;
   LuCache &lu = lu_get(c, alpha, kind, nroff);
   dim3 grid((k.npchop + 63) / 64, k.nzdim);
+  // right-hand sides read and written once, factors of every |kappa| read once
+  const double S = active_pairs(k, nroff);
+  const int nkk = (k.nzdim > 1) ? k.nzdim / 2 + 1 : 1;
+  ProbeScope probe(c, MLX_PROBE_LU_SOLVE, 32.0 * S * k.nzdim + 48.0 * S * nkk, 2.0 * 4.0 * 5.0 * S * k.nzdim);
   hipLaunchKernelGGL(k_lu_solve, grid, dim3(64), 0, c->stream, e, fac_ptrs(lu.sol, fac_rows(k, nroff)), lu.pvw, lu.anyp, k.nrdim, k.npdim, k.nzdim,
                      k.nrchop + nroff, k.npchop, k.chopzl, k.chopzu, kind == 2 ? 1 : 0);
   HIPCHK(hipGetLastError());

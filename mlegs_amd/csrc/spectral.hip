@@ -490,6 +490,12 @@ __global__ void k_sum_parts(const double *part, int nb, double *out) {
   if (threadIdx.x == 0) out[0] = t;
 }
 
+// plain copy of a whole field (also the known-size reference for HBM counters)
+__global__ void k_copy_field_(double2 *__restrict__ d, const double2 *__restrict__ s, size_t n) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+    d[i] = s[i];
+}
+
 __global__ void k_set_scalar_(double *d, double v) {
   if (threadIdx.x == 0 && blockIdx.x == 0) d[0] = v;
 }
@@ -616,6 +622,10 @@ void k_idel2_preln00(mlx_ctx *c, double2 *e, double *ln, double preln, int nroff
 void k_axpby(mlx_ctx *c, double2 *dst, double *dln, double a, const double2 *src, const double *sln, double b) {
   hipLaunchKernelGGL(k_axpby_, dim3(grid_for(c->nelem)), dim3(kT), 0, c->stream, dst, dln, a, src, sln, b,
                      c->nelem);
+}
+
+void k_copy_field(mlx_ctx *c, double2 *dst, const double2 *src) {
+  hipLaunchKernelGGL(k_copy_field_, dim3(grid_for(c->nelem)), dim3(kT), 0, c->stream, dst, src, c->nelem);
 }
 
 void k_abcn_rhs(mlx_ctx *c, double2 *sh, double *shln, const double2 *s, const double *sln, const double2 *nl,

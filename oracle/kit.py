@@ -226,3 +226,36 @@ class Kit3D:
         g = self._del2_core(am, nn)
         g[np.arange(nn), np.arange(nn)] -= akval ** 2
         return self._scale(g, am)
+
+    # ------------------------------------------------------------------
+    # the same operators as diagonals: band[d + h, i] = M[i, i + d]
+    # (zero where i + d is outside 0..nn-1), h = half-bandwidth.  The
+    # scaling exp(lognorm_j - lognorm_i) touches only band entries, and
+    # del2 = del2h - ak^2 I because the scaling leaves the diagonal alone.
+    # ------------------------------------------------------------------
+    def _band_of(self, dense, h):
+        nn = dense.shape[0]
+        band = np.zeros((2 * h + 1, nn))
+        for d in range(-h, h + 1):
+            diag = np.diagonal(dense, d)
+            if d >= 0:
+                band[d + h, :len(diag)] = diag
+            else:
+                band[d + h, -d:] = diag
+        return band
+
+    def band_xxdx(self, mval, nn):
+        """Diagonals (offsets -1..1) of leg_xxdx."""
+        key = ('x', abs(int(mval)), nn)
+        cache = self.__dict__.setdefault('_band_cache', {})
+        if key not in cache:
+            cache[key] = self._band_of(self.leg_xxdx(mval, nn), 1)
+        return cache[key]
+
+    def band_del2h(self, mval, nn):
+        """Diagonals (offsets -2..2) of leg_del2h."""
+        key = ('h', abs(int(mval)), nn)
+        cache = self.__dict__.setdefault('_band_cache', {})
+        if key not in cache:
+            cache[key] = self._band_of(self.leg_del2h(mval, nn), 2)
+        return cache[key]

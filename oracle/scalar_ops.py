@@ -15,6 +15,7 @@ applied twice to that slice, as the reference does.
 """
 import numpy as np
 from scipy.linalg import solve_banded
+from scipy.linalg.lapack import zgbsv
 
 PI = np.pi
 IU = 1j
@@ -369,6 +370,35 @@ def idelsqp(s, kit):
 # ----------------------------------------------------------------------------
 # banded operators
 # ----------------------------------------------------------------------------
+def _slices(ks):
+    """z slices of a kk-loop list and how often each is visited."""
+    u, c = np.unique(np.asarray(ks, dtype=np.int64), return_counts=True)
+    return u, c
+
+
+def _band_apply(band, X):
+    """Y[i] = sum_d band[d + h, i] X[i + d] along the first axis of X."""
+    h = band.shape[0] // 2
+    nn = X.shape[0]
+    Y = np.zeros_like(X)
+    for d in range(-h, h + 1):
+        c = band[d + h]
+        if d >= 0:
+            if nn - d > 0:
+                Y[:nn - d] += c[:nn - d, None] * X[d:]
+        elif nn + d > 0:
+            Y[-d:] += c[-d:, None] * X[:nn + d]
+    return Y
+
+
+def _apply_per_slice(so, mm, nn, ks, fn):
+    """Apply fn to the (nn x slices) block of column mm once per visit of each slice."""
+    u, c = _slices(ks)
+    for rep in range(1, int(c.max()) + 1 if len(c) else 1):
+        sel = u[c >= rep]
+        so.e[:nn, mm, sel] = fn(so.e[:nn, mm, sel], sel)
+
+
 def xxdx(s, kit):
     """r d/dr (mlegs_scalar_ops.f90:418-463)."""
     nrc, npc, nzc, nzcu, nrcs = chop_index(s, kit)
@@ -379,9 +409,10 @@ def xxdx(s, kit):
     ks = zlist(kit, nzc, nzcu)
     for mm in range(npc):
         nn = nrcs[mm]
-        X = kit.leg_xxdx(kit.m[mm], nn)
-        for k in ks:
-            so.e[:nn, mm, k] = X @ so.e[:nn, mm, k]
+        if nn == 0:
+            continue
+        B = kit.band_xxdx(kit.m[mm], nn)
+        _apply_per_slice(so, mm, nn, ks, lambda X, sel: _band_apply(B, X))
     so.e[0, 0, 0] += 1.0 / np.exp(kit.lognorm[0, 0]) * s.ln
     so.e[1, 0, 0] += 1.0 / np.exp(kit.lognorm[1, 0]) * s.ln
     s.assign(so)
@@ -404,15 +435,16 @@ def del2h(s, kit):
     ks = zlist(kit, nzc, nzcu)
     for mm in range(npc):
         nn = nrcs[mm]
-        H = kit.leg_del2h(kit.m[mm], nn)
-        for k in ks:
-            so.e[:nn, mm, k] = H @ so.e[:nn, mm, k]
+        if nn == 0:
+            continue
+        H = kit.band_del2h(kit.m[mm], nn)
+        _apply_per_slice(so, mm, nn, ks, lambda X, sel: _band_apply(H, X))
     _lap_ln(so, kit, s.ln)
     s.assign(so)
 
 
 def del2(s, kit):
-    """Laplacian (mlegs_scalar_ops.f90:520-573)."""
+    """Laplacian (mlegs_scalar_ops.f90:520-573): del2h - ak^2 per slice."""
     nrc, npc, nzc, nzcu, nrcs = chop_index(s, kit)
     if s.space != 'FFF':
         trans(s, 'FFF', kit)
@@ -421,11 +453,28 @@ def del2(s, kit):
     ks = zlist(kit, nzc, nzcu)
     for mm in range(npc):
         nn = nrcs[mm]
-        for k in ks:
-            D = kit.leg_del2(kit.m[mm], kit.ak[k], nn)
-            so.e[:nn, mm, k] = D @ so.e[:nn, mm, k]
+        if nn == 0:
+            continue
+        H = kit.band_del2h(kit.m[mm], nn)
+        _apply_per_slice(so, mm, nn, ks,
+                         lambda X, sel: _band_apply(H, X) - (kit.ak[sel] ** 2)[None, :] * X)
     _lap_ln(so, kit, s.ln)
     s.assign(so)
+
+
+def _ab(band, shift):
+    """LAPACK band storage (kl = ku = 2) of the diagonals plus shift on the main one."""
+    nn = band.shape[1]
+    ab = np.zeros((5, nn))
+    for d in range(-2, 3):
+        # A[i, i + d] sits at ab[2 - d, i + d]
+        c = band[d + 2]
+        if d >= 0 and nn - d > 0:
+            ab[2 - d, d:] = c[:nn - d]
+        elif d < 0 and nn + d > 0:
+            ab[2 - d, :nn + d] = c[-d:]
+    ab[2] = ab[2] + shift
+    return ab
 
 
 def _band_solve(A, b, kl=2, ku=2):
@@ -437,6 +486,26 @@ def _band_solve(A, b, kl=2, ku=2):
         j0 = max(k, 0)
         ab[ku - k, j0:j0 + len(d)] = d
     return solve_banded((kl, ku), ab, b)
+
+
+def _gbsv(ab, b):
+    """LAPACK zgbsv (kl = ku = 2, partial pivoting), as zgbtrf + zgbtrs."""
+    abx = np.zeros((7, ab.shape[1]), dtype=np.complex128)
+    abx[2:] = ab
+    _, _, x, info = zgbsv(2, 2, abx, b)
+    if info != 0:
+        raise ValueError('lurc: lu factorization resulted in failure')
+    return x
+
+
+def _helm_solves(so, kit, mm, nn, ks, alpha):
+    """(del2 + alpha) x = b on every visit of every slice of column mm."""
+    H = kit.band_del2h(kit.m[mm], nn)
+    for k in ks:
+        ab = _ab(H, -(kit.ak[k] ** 2))
+        if alpha != 0.0:
+            ab[2] = ab[2] + alpha
+        so.e[:nn, mm, k] = _gbsv(ab, so.e[:nn, mm, k])
 
 
 def ihelm(s, alpha, kit):
@@ -452,9 +521,9 @@ def ihelm(s, alpha, kit):
     ks = zlist(kit, nzc, nzcu)
     for mm in range(npc):
         nn = nrcs[mm]
-        for k in ks:
-            A = kit.leg_del2(kit.m[mm], kit.ak[k], nn) + alpha * np.eye(nn)
-            so.e[:nn, mm, k] = _band_solve(A, so.e[:nn, mm, k])
+        if nn == 0:
+            continue
+        _helm_solves(so, kit, mm, nn, ks, alpha)
     s.assign(so)
 
 
@@ -469,16 +538,22 @@ def idel2_proln(s, kit):
     ks = zlist(kit, nzc, nzcu)
     for mm in range(npc):
         nn = nrcs[mm]
-        for idx, k in enumerate(ks):
-            A = kit.leg_del2(kit.m[mm], kit.ak[k], nn)
-            if mm == 0 and idx == 0 and k == 0:
-                A[0, 0] += 4.0 / 3.0 / ell2
-                A[1, 0] -= 2.0 / 1.0 / ell2 * np.exp(ln[0] - ln[1])
-                A[2, 0] += 2.0 / 3.0 / ell2 * np.exp(ln[0] - ln[2])
-                so.e[:nn, mm, k] = _band_solve(A, so.e[:nn, mm, k])
-                so.ln = float(np.real(so.e[0, mm, k])) * np.exp(ln[0])
-            else:
-                so.e[:nn, mm, k] = _band_solve(A, so.e[:nn, mm, k])
+        if nn == 0:
+            continue
+        first00 = (mm == 0 and ks and ks[0] == 0)
+        if first00:
+            # bordered system: column 0 carries the log term's Laplacian
+            ab = _ab(kit.band_del2h(0, nn), -(kit.ak[0] ** 2))
+            ab[2, 0] += 4.0 / 3.0 / ell2
+            if nn > 1:
+                ab[3, 0] -= 2.0 / 1.0 / ell2 * np.exp(ln[0] - ln[1])
+            if nn > 2:
+                ab[4, 0] += 2.0 / 3.0 / ell2 * np.exp(ln[0] - ln[2])
+            so.e[:nn, 0, 0] = _gbsv(ab, so.e[:nn, 0, 0])
+            so.ln = float(np.real(so.e[0, 0, 0])) * np.exp(ln[0])
+            _helm_solves(so, kit, mm, nn, ks[1:], 0.0)
+        else:
+            _helm_solves(so, kit, mm, nn, ks, 0.0)
     s.assign(so)
 
 
@@ -493,23 +568,26 @@ def idel2_preln(s, kit, preln):
     ks = zlist(kit, nzc, nzcu)
     for mm in range(npc):
         nn = nrcs[mm]
+        if nn == 0:
+            continue
         for idx, k in enumerate(ks):
+            if not (mm == 0 and idx == 0 and k == 0):
+                _helm_solves(so, kit, mm, nn, [k], 0.0)
+                continue
+            # the first equation prescribes the log term; the rest shift down a row
             A = kit.leg_del2(kit.m[mm], kit.ak[k], nn)
-            if mm == 0 and idx == 0 and k == 0:
-                P = np.zeros((nn, nn))
-                P[1:nn, :] = A[0:nn - 1, :]
-                P[0, 0] = 1.0
-                P[1, 0] += 4.0 / 3.0 / ell2
-                P[2, 0] -= 2.0 / 1.0 / ell2 * np.exp(ln[0] - ln[1])
-                P[3, 0] += 2.0 / 3.0 / ell2 * np.exp(ln[0] - ln[2])
-                b = so.e[:nn, mm, k].copy()
-                rhs = np.empty_like(b)
-                rhs[1:] = b[:-1]
-                rhs[0] = preln / np.exp(ln[0])
-                so.e[:nn, mm, k] = _band_solve(P, rhs, kl=3, ku=2)
-                so.ln = float(np.real(so.e[0, mm, k])) * np.exp(ln[0])
-            else:
-                so.e[:nn, mm, k] = _band_solve(A, so.e[:nn, mm, k])
+            P = np.zeros((nn, nn))
+            P[1:nn, :] = A[0:nn - 1, :]
+            P[0, 0] = 1.0
+            P[1, 0] += 4.0 / 3.0 / ell2
+            P[2, 0] -= 2.0 / 1.0 / ell2 * np.exp(ln[0] - ln[1])
+            P[3, 0] += 2.0 / 3.0 / ell2 * np.exp(ln[0] - ln[2])
+            b = so.e[:nn, mm, k].copy()
+            rhs = np.empty_like(b)
+            rhs[1:] = b[:-1]
+            rhs[0] = preln / np.exp(ln[0])
+            so.e[:nn, mm, k] = _band_solve(P, rhs, kl=3, ku=2)
+            so.ln = float(np.real(so.e[0, mm, k])) * np.exp(ln[0])
     s.assign(so)
 
 
@@ -644,12 +722,15 @@ def vec2tp_tables(kit, mm, nn):
 
 
 def _eomul(a, b):
-    """Parity-folded product, odd rows <- even part (mlegs_scalar_ops.f90:2067-2104)."""
+    """Parity-folded product, odd rows <- even part (mlegs_scalar_ops.f90:2067-2104).
+
+    b holds the nr collocation values along its first axis (further axes are
+    independent columns)."""
     nj = b.shape[0]
     njh = nj // 2
     be = b[:njh] + b[nj - 1:njh - 1:-1]
     bo = b[:njh] - b[nj - 1:njh - 1:-1]
-    c = np.zeros(a.shape[0], dtype=np.complex128)
+    c = np.zeros((a.shape[0],) + b.shape[1:], dtype=np.complex128)
     c[0::2] = a[0::2] @ be
     if a.shape[0] > 1:
         c[1::2] = a[1::2] @ bo
@@ -662,7 +743,7 @@ def _oemul(a, b):
     njh = nj // 2
     be = b[:njh] + b[nj - 1:njh - 1:-1]
     bo = b[:njh] - b[nj - 1:njh - 1:-1]
-    c = np.zeros(a.shape[0], dtype=np.complex128)
+    c = np.zeros((a.shape[0],) + b.shape[1:], dtype=np.complex128)
     c[0::2] = a[0::2] @ bo
     if a.shape[0] > 1:
         c[1::2] = a[1::2] @ be
@@ -700,20 +781,26 @@ def vec2tp(vr, vp, vz, kit):
     uz.space = 'PFF'
     ur.chop_offset(3); up.chop_offset(3); uz.chop_offset(3)
     nrc, npc, nzc, nzcu, nrcs = chop_index(ur, kit)
-    ks = zlist(kit, nzc, nzcu)
+    # a slice visited twice is recomputed from the same inputs: once suffices
+    ks = np.unique(np.asarray(zlist(kit, nzc, nzcu), dtype=np.int64))
+    kv = kit.ak[ks][None, :]
     for mm in range(npc):
         nn = nrcs[mm]
         mv = kit.m[mm]
+        if nn == 0:
+            continue
         v, d, t = vec2tp_tables(kit, mm, nn)
-        for k in ks:
-            w1 = _eomul(v, ur.e[:nr, mm, k]) if mv != 0 else np.zeros(nn, complex)
-            w2 = _oemul(d, up.e[:nr, mm, k])
-            psi.e[:nn, mm, k] = -IU * mv * w1 - w2
-            w1 = _oemul(d, ur.e[:nr, mm, k])
-            w2 = _eomul(v, up.e[:nr, mm, k]) if mv != 0 else np.zeros(nn, complex)
-            c = _eomul(t, uz.e[:nr, mm, k])
-            kv = kit.ak[k]
-            chi.e[:nn, mm, k] = IU * kv * w1 + mv * kv * w2 - c
+        r_ = ur.e[:nr, mm][:, ks]
+        p_ = up.e[:nr, mm][:, ks]
+        z_ = uz.e[:nr, mm][:, ks]
+        zero = np.zeros((nn, len(ks)), complex)
+        w1 = _eomul(v, r_) if mv != 0 else zero
+        w2 = _oemul(d, p_)
+        psi.e[:nn, mm, ks] = -IU * mv * w1 - w2
+        w1 = _oemul(d, r_)
+        w2 = _eomul(v, p_) if mv != 0 else zero
+        c = _eomul(t, z_)
+        chi.e[:nn, mm, ks] = IU * kv * w1 + mv * kv * w2 - c
     idel2_proln(chi, kit)
     psi.chop_offset(0); chi.chop_offset(0)
     chop(psi, kit); chop(chi, kit)
