@@ -5,7 +5,7 @@
  * hot path as Fortran generic interfaces on type(scalar) and the global
  * transform kit `tfm`.  Each entry point below replaces one of them; the
  * citation after each declaration names the interface it stands in for.
- * A Fortran host binds these with iso_c_binding (fortran/mlegs_amd.f90) and
+ * A Fortran host binds these with iso_c_binding (mlegs_amd/fortran/mlegs_amd.f90) and
  * Python binds them with ctypes (mlegs_amd/_lib.py).
  *
  * Conventions
@@ -239,6 +239,17 @@ int mlx_energy(mlx_ctx* ctx, const mlx_field* vr, const mlx_field* vp, const mlx
  * field is used as operand and left in an unspecified state of the same
  * space. */
 int mlx_time_kernel(mlx_ctx* ctx, mlx_field* f, const char* which, int32_t iters, double* ms);
+
+/* Seeded synthetic spectral field on the host (no context, no GPU): writes
+ * 2*nrdim*npdim*nzdim doubles in the reference's global FFF layout.
+ * chops = (nrchop, npchop, nzchop, np, nz).  c(n,m,kappa) = a * (g1 + i g2)/sqrt(2)
+ * with g1, g2 ~ N(0,1) from SplitMix64 keyed by (seed, field_id, n, m, kappa)
+ * and Box-Muller, a = amp*exp(-(n+m+|kappa|)/scale) (kind 0, perturbation) or
+ * exp(-n/scale) (kind 1, transform-workload field); zero outside the chops,
+ * m = 0 plane Hermitian in kappa with real (n,0,0), Nyquist m and kappa zero.
+ * No counterpart in the reference (its drivers call the compiler's rand()). */
+int mlx_seed_spectral(double* host, int64_t nrdim, int64_t npdim, int64_t nzdim, const int32_t* chops,
+                      uint64_t seed, int32_t field_id, double amp, double scale, int32_t kind);
 
 /* Measurement probes, no counterpart in the reference.  After
  * mlx_probe_start, every launch of a probed kernel (MLX_PROBE_*) is bracketed

@@ -95,6 +95,8 @@ SIGNATURES = {
     'mlx_energy': ([_P, _P, _P, _P, _DP], ctypes.c_int),
     'mlx_time_kernel': ([_P, _P, _S, _I, _DP], ctypes.c_int),
     'mlx_probe_start': ([_P, _I], ctypes.c_int),
+    'mlx_seed_spectral': ([_DP, _I64, _I64, _I64, _IP, ctypes.c_uint64, _I, ctypes.c_double, ctypes.c_double, _I],
+                          ctypes.c_int),
     'mlx_probe_stop': ([_P, _DP], ctypes.c_int),
 }
 

@@ -1,8 +1,11 @@
-"""Build the HIP library in-tree: ``mlegs_amd/libmlegs_amd.so`` for gfx950.
+"""Build in-tree: ``mlegs_amd/libmlegs_amd.so`` for gfx950 and the Fortran host.
 
 Each ``csrc/*.hip`` is compiled to an object in ``mlegs_amd/_build`` (in
 parallel), then linked into one shared library exposing the C ABI declared
-in ``include/mlegs_amd.h``.
+in ``include/mlegs_amd.h``.  The Fortran module ``fortran/mlegs_amd.f90``
+(iso_c_binding interface of that ABI) and the driver
+``fortran/vortical_flow_3d.f90`` are then compiled with amdflang into
+``mlegs_amd/bin/vortical_flow_3d``, which finds the library through its rpath.
 """
 import concurrent.futures
 import os
@@ -15,6 +18,9 @@ CSRC = os.path.join(HERE, 'csrc')
 OBJ = os.path.join(HERE, '_build')
 LIB = os.path.join(HERE, 'libmlegs_amd.so')
 HIPCC = os.environ.get('HIPCC', '/opt/rocm/bin/hipcc')
+FC = os.environ.get('FC_MLEGS', '/opt/rocm/bin/amdflang')
+FSRC = os.path.join(HERE, 'fortran')
+BIN = os.path.join(HERE, 'bin')
 ARCH = os.environ.get('MLEGS_AMD_ARCH', 'gfx950')
 
 CFLAGS = [
@@ -46,7 +52,25 @@ def build(verbose=True):
     os.replace(tmp, LIB)
     if verbose:
         print('built', LIB)
+    build_fortran(verbose)
     return LIB
+
+
+def build_fortran(verbose=True):
+    """The Fortran host module and the vortex driver linked against the library."""
+    os.makedirs(BIN, exist_ok=True)
+    mod_obj = os.path.join(OBJ, 'mlegs_amd_f90.o')
+    steps = [
+        [FC, '-O2', '-c', os.path.join(FSRC, 'mlegs_amd.f90'), '-module-dir', OBJ, '-o', mod_obj],
+        [FC, '-O2', os.path.join(FSRC, 'vortical_flow_3d.f90'), mod_obj, '-I', OBJ,
+         '-L' + HERE, '-lmlegs_amd', '-Wl,-rpath,$ORIGIN/..', '-o', os.path.join(BIN, 'vortical_flow_3d')],
+    ]
+    for cmd in steps:
+        r = subprocess.run(cmd, capture_output=True, text=True, cwd=OBJ)
+        if r.returncode != 0:
+            raise RuntimeError('amdflang failed:\n%s\n%s' % (' '.join(cmd), r.stderr))
+    if verbose:
+        print('built', os.path.join(BIN, 'vortical_flow_3d'))
 
 
 if __name__ == '__main__':

@@ -114,3 +114,19 @@ def to_fortran_bytes(a):
 
 def from_fortran(buf, dims):
     return np.asarray(buf, dtype=np.complex128).reshape(dims, order='F')
+
+
+def seed_spectral_native(dims, chops, seed=DEFAULT_SEED, field_id=0, amp=1.0, scale=4.0, kind=0):
+    """The same field from the library's host generator (mlx_seed_spectral).
+
+    kind 0: perturbation(amp, scale); kind 1: transform_field(decay=scale)."""
+    import ctypes
+    from . import _lib
+    lib = _lib.load()
+    out = np.zeros(2 * int(np.prod(dims)), dtype=np.float64)
+    ch = (ctypes.c_int32 * 5)(*[int(v) for v in chops])
+    rc = lib.mlx_seed_spectral(out.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), dims[0], dims[1], dims[2], ch,
+                               int(seed), int(field_id), float(amp), float(scale), int(kind))
+    if rc != 0:
+        raise ValueError('mlx_seed_spectral: invalid arguments (code %d)' % rc)
+    return out.view(np.complex128).reshape(dims, order='F')

@@ -1,5 +1,6 @@
 """The seeded synthetic inputs shared by the GPU path and the oracle."""
 import numpy as np
+import pytest
 
 from mlegs_amd import inputs
 
@@ -67,3 +68,23 @@ def test_fortran_order_round_trip():
     f = inputs.to_fortran_bytes(a)
     assert f.flags.f_contiguous
     np.testing.assert_array_equal(inputs.from_fortran(f.ravel(order='F'), (2, 3, 4)), a)
+
+
+@pytest.mark.parametrize('kind', [0, 1])
+def test_native_generator_matches_numpy(kind):
+    """mlx_seed_spectral (host C++ in the library, no GPU needed) and the numpy
+    definition agree."""
+    import os
+    from mlegs_amd import _lib
+    if not os.path.exists(_lib.LIB_PATH):
+        from mlegs_amd import build
+        build.build()
+    dims, chops = (19, 7, 8), (16, 7, 5, 12, 8)
+    if kind == 0:
+        ref = inputs.perturbation(dims, chops, 1e-3, scale=4.0, field_id=6)
+        got = inputs.seed_spectral_native(dims, chops, field_id=6, amp=1e-3, scale=4.0, kind=0)
+    else:
+        ref = inputs.transform_field(dims, chops, field_id=6, decay=5.0)
+        got = inputs.seed_spectral_native(dims, chops, field_id=6, scale=5.0, kind=1)
+    np.testing.assert_allclose(got, ref, rtol=1e-13, atol=1e-300)
+    assert np.array_equal(got == 0, ref == 0)
