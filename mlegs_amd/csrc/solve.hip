@@ -290,6 +290,22 @@ __global__ void __launch_bounds__(64) k_lu_solve(double2 *e, FacPtrs Fp, const u
   const size_t r0 = ((size_t)kk * G + g) * (size_t)nrc * 64 + l;
   const uint16_t *P = pvw + ((size_t)kk * G + g) * NC * 64 + l;
   double2 *b = e + (size_t)k * nrdim * npdim + (size_t)nrdim * m;
+  // An exactly zero right-hand side has the zero solution.  De-aliased fields
+  // are zero on a third of the kappa slices and above m = np/3, so a wave whose
+  // columns are all zero returns here; a non-zero column usually stops the
+  // scan at its first rows.
+  bool nonzero = false;
+  for (int i0 = 0; i0 < n && !nonzero; i0 += CH) {
+#pragma unroll
+    for (int u = 0; u < CH; ++u) {
+      const int i = i0 + u;
+      if (i < n) {
+        const double2 v = b[i];
+        nonzero |= (v.x != 0.0) || (v.y != 0.0);
+      }
+    }
+  }
+  if (!__any(nonzero)) return;
   const bool piv = anyp[(size_t)kk * G + g] != 0;
   for (int rep = 0; rep < cnt; ++rep) {
     if (piv) solve_column<true>(b, Fp.ff + r0, Fp.fb + r0, Fp.fx + r0, P, n);
