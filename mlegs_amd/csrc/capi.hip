@@ -80,7 +80,8 @@ void swap_scr(mlx_ctx *c, mlx_field *f) { std::swap(f->e, c->scr[0]); }
 // ---------------------------------------------------------------------------
 // trans (mlegs_scalar_ops.f90:157-235)
 // ---------------------------------------------------------------------------
-void trans(mlx_ctx *c, mlx_field *f, const char *space) {
+// rscale < 0: the backward phi transform also divides rows i < nr by r_i
+void trans(mlx_ctx *c, mlx_field *f, const char *space, int rscale = 0) {
   need_kit(c);
   int cur = space_id(f->space);
   if (cur < 0) MLX_THROW(MLX_E_LAYOUT, "trans: scalar space info corrupted (only accepting PPP, PFP, FFP and FFF)");
@@ -91,7 +92,7 @@ void trans(mlx_ctx *c, mlx_field *f, const char *space) {
   if (cur < nw) {
     while (true) {
       if (cur == 0) {
-        if (k.np > 1) fft_phi(c, f->e, -1);
+        if (k.np > 1) fft_phi(c, f->e, f->e, -1);
         set_space(f, "PFP");
       }
       if (cur == 1) {
@@ -122,7 +123,8 @@ void trans(mlx_ctx *c, mlx_field *f, const char *space) {
         set_space(f, "PFP");
       }
       if (cur == 1) {
-        if (k.np > 1) fft_phi(c, f->e, +1);
+        if (k.np > 1) fft_phi(c, f->e, f->e, +1, rscale);
+        else if (rscale < 0) k_scale_rows_r(c, f->e, 1);
         set_space(f, "PPP");
       }
       if (cur == 0) break;
@@ -201,47 +203,45 @@ void abcn(mlx_ctx *c, mlx_field *s, mlx_field *s_p, mlx_field *nl, mlx_field *nl
   else fcopy(c, nl_p, nl);
 }
 
-// tp2vec (vector_reconstruction, mlegs_scalar_ops.f90:1455-1545)
+// Back half of tp2vec: the three FFF components (offset 0, no log term) to
+// PPP; ur and up are divided by r in the last phi transform.
+void tp2vec_back(mlx_ctx *c, mlx_field *ur, mlx_field *up, mlx_field *uz) {
+  for (mlx_field *f : {ur, up, uz}) {
+    set_space(f, "FFF");
+    f->off[0] = f->off[1] = f->off[2] = 0;
+    k_set_scalar(c, f->sc, 0.0);
+  }
+  trans(c, ur, "PPP", -1);
+  trans(c, up, "PPP", -1);
+  trans(c, uz, "PPP");
+}
+
+// tp2vec (vector_reconstruction, mlegs_scalar_ops.f90:1455-1545): the +3-offset
+// band products, the combination and the chop run in one column pass
 void tp2vec(mlx_ctx *c, const mlx_field *psi, const mlx_field *chi, mlx_field *vr, mlx_field *vp, mlx_field *vz) {
   need_kit(c);
   if (!is_fff(psi) || !is_fff(chi)) MLX_THROW(MLX_E_LAYOUT, "vector_projection: psi and chi must be in FFF");
   mlx_field *ur = tmp(c, 0), *up = tmp(c, 1), *uz = tmp(c, 2);
-  fcopy(c, ur, chi);
-  ur->off[0] = 3;
-  fcopy(c, up, psi);
-  up->off[0] = 3;
-  fcopy(c, uz, chi);
-  uz->off[0] = 3;
-  k_bandmul(c, ur->e, ur->sc, 2, 3);
-  k_bandmul(c, up->e, up->sc, 2, 3);
-  k_tp2vec_combine(c, ur->e, up->e, psi->e, uz->e, 3);
-  ur->off[0] = 0;
-  up->off[0] = 0;
-  chop(c, ur);
-  chop(c, up);
-  trans(c, ur, "PPP");
-  trans(c, up, "PPP");
-  k_scale_rows_r(c, ur->e, 1);
-  k_scale_rows_r(c, up->e, 1);
-  k_bandmul(c, uz->e, uz->sc, 1, 3);
-  k_axpby(c, uz->e, uz->sc, -1.0, uz->e, uz->sc, 0.0);
-  uz->off[0] = 0;
-  chop(c, uz);
-  trans(c, uz, "PPP");
+  k_tp2vec_front(c, ur->e, up->e, uz->e, psi->e, chi->e, psi->sc, chi->sc, 0, 0);
+  tp2vec_back(c, ur, up, uz);
   fmove(vr, ur);
   fmove(vp, up);
   fmove(vz, uz);
 }
 
+// tp2curlvec (curl_vector_reconstruction, :1547-1560): curl V = tp2vec(-del2 chi, psi),
+// with -del2 chi formed inside the same column pass
 void tp2curlvec(mlx_ctx *c, const mlx_field *psi, const mlx_field *chi, mlx_field *wr, mlx_field *wp,
                 mlx_field *wz) {
   need_kit(c);
-  mlx_field *md = tmp(c, 3);
-  fcopy(c, md, chi);
-  bandmul(c, md, 0);
-  k_axpby(c, md->e, md->sc, -1.0, md->e, md->sc, 0.0);
-  k_set_scalar(c, md->sc, 0.0);
-  tp2vec(c, md, psi, wr, wp, wz);
+  if (!is_fff(psi) || !is_fff(chi)) MLX_THROW(MLX_E_LAYOUT, "vector_projection: psi and chi must be in FFF");
+  mlx_field *ur = tmp(c, 0), *up = tmp(c, 1), *uz = tmp(c, 2);
+  // del2 of md = chi runs over the rows of chi's own radial offset
+  k_tp2vec_front(c, ur->e, up->e, uz->e, chi->e, psi->e, chi->sc, psi->sc, 1, chi->off[0]);
+  tp2vec_back(c, ur, up, uz);
+  fmove(wr, ur);
+  fmove(wp, up);
+  fmove(wz, uz);
 }
 
 void vecprod(mlx_ctx *c, mlx_field *vr, mlx_field *vp, mlx_field *vz, const mlx_field *ur, const mlx_field *up,
@@ -255,6 +255,21 @@ void vecprod(mlx_ctx *c, mlx_field *vr, mlx_field *vp, mlx_field *vz, const mlx_
   }
 }
 
+// dst = PFP form of the PPP field src (optionally r-weighted): metadata and log
+// term of src, then the phi transform from src into dst
+void phi_forward_from(mlx_ctx *c, mlx_field *dst, const mlx_field *src, int rscale) {
+  const KitDev &k = c->kit;
+  std::memcpy(dst->off, src->off, sizeof(dst->off));
+  k_copy_scalar(c, dst->sc, src->sc, 1.0);
+  if (k.np > 1) {
+    fft_phi(c, dst->e, src->e, -1, rscale);
+  } else {
+    HIPCHK(hipMemcpyAsync(dst->e, src->e, sizeof(double2) * c->nelem, hipMemcpyDeviceToDevice, c->stream));
+    if (rscale > 0) k_scale_rows_r(c, dst->e, 0);
+  }
+  set_space(dst, "PFP");
+}
+
 // vec2tp (vector_projection, mlegs_scalar_ops.f90:1308-1453)
 void vec2tp(mlx_ctx *c, const mlx_field *vr, const mlx_field *vp, const mlx_field *vz, mlx_field *psi,
             mlx_field *chi) {
@@ -263,14 +278,13 @@ void vec2tp(mlx_ctx *c, const mlx_field *vr, const mlx_field *vp, const mlx_fiel
     if (!is_ppp(f)) MLX_THROW(MLX_E_LAYOUT, "vector_projection: vr, vp, vz must be in PPP");
   const KitDev &k = c->kit;
   mlx_field *ur = tmp(c, 0), *up = tmp(c, 1), *uz = tmp(c, 2);
-  fcopy(c, ur, vr);
-  fcopy(c, up, vp);
-  fcopy(c, uz, vz);
+  // ur = r vr, up = r vp, uz = vz, each taken to PFP by an out-of-place phi
+  // transform that applies the r weight on its input
+  phi_forward_from(c, ur, vr, 1);
+  phi_forward_from(c, up, vp, 1);
+  phi_forward_from(c, uz, vz, 0);
   k_set_scalar(c, psi->sc, 0.0);
   k_set_scalar(c, chi->sc, 0.0);
-  k_scale_rows_r(c, ur->e, 0);
-  k_scale_rows_r(c, up->e, 0);
-  trans(c, ur, "PFP");
   if (k.nz > 1) fft_z(c, ur->e, -1, (double)k.nr * k.npchop);
   trans(c, up, "FFF");
   double2 *inf = reinterpret_cast<double2 *>(c->dsmall + 2048);
@@ -278,7 +292,6 @@ void vec2tp(mlx_ctx *c, const mlx_field *vr, const mlx_field *vp, const mlx_fiel
   leg_backward(c, up->e, c->scr[0], up->off[0]);
   swap_scr(c, up);
   k_vec2tp_up_ln(c, up->e, inf, psi->sc);
-  trans(c, uz, "PFP");
   if (k.nz > 1) fft_z(c, uz->e, -1, (double)k.nr * k.npchop);
   vec2tp_project(c, ur->e, up->e, uz->e, psi->e, chi->e, 3);
   set_space(psi, "FFF");
@@ -867,8 +880,8 @@ int mlx_time_kernel(mlx_ctx *c, mlx_field *f, const char *which, int32_t iters, 
     auto launch = [&]() {
       if (w == "leg_fwd") leg_forward(c, f->e, c->scr[0], 0, false);
       else if (w == "leg_bwd") leg_backward(c, f->e, c->scr[0], 0, false);
-      else if (w == "fft_phi_fwd") fft_phi(c, f->e, -1);
-      else if (w == "fft_phi_bwd") fft_phi(c, f->e, +1);
+      else if (w == "fft_phi_fwd") fft_phi(c, f->e, f->e, -1);
+      else if (w == "fft_phi_bwd") fft_phi(c, f->e, f->e, +1);
       else if (w == "fft_z_fwd") fft_z(c, f->e, -1, active_pairs(c->kit, 0));
       else if (w == "fft_z_bwd") fft_z(c, f->e, +1, active_pairs(c->kit, 0));
       else if (w == "v2t") vec2tp_project(c, f->e, f->e, f->e, c->scr[0], c->scr[1], 3);

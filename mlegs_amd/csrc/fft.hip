@@ -133,10 +133,15 @@ __device__ void lds_fft(double2 *buf, const double2 *tw, const FftPlan &pl, int 
   }
 }
 
-// phi transform; grid.x = row tiles, grid.y = z planes
-__global__ void __launch_bounds__(kThreads) k_fft_phi(double2 *e, int nrdim, int npdim, int nzdim, int TI,
-                                                      FftPlan pl, const double2 *twg, const double2 *twh,
-                                                      int dir) {
+// phi transform; grid.x = row tiles, grid.y = z planes.  Reads `in`, writes
+// `out` (they may be the same field).  rscale > 0 multiplies the input rows
+// i < nr by r_i before the forward transform; rscale < 0 divides the output
+// rows i < nr by r_i after the backward one (the r-weighting of vec2tp and
+// tp2vec, mlegs_scalar_ops.f90:1330-1331, 1511-1512).
+__global__ void __launch_bounds__(kThreads) k_fft_phi(double2 *out, const double2 *in, int nrdim, int npdim,
+                                                      int nzdim, int TI, FftPlan pl, const double2 *twg,
+                                                      const double2 *twh, int dir, const double *r, int nr,
+                                                      int rscale) {
   __shared__ double2 buf[kLdsElems];
   __shared__ double2 tw[512];
   const int N = pl.n;  // np/2
@@ -150,7 +155,11 @@ __global__ void __launch_bounds__(kThreads) k_fft_phi(double2 *e, int nrdim, int
       int l = idx % TI, j = idx / TI;
       int i = i0 + l;
       double2 v = make_double2(0.0, 0.0);
-      if (i < nrdim) v = e[plane + (size_t)j * nrdim + i];
+      if (i < nrdim) v = in[plane + (size_t)j * nrdim + i];
+      if (rscale > 0 && i < nr) {
+        const double rr = r[i];
+        v = make_double2(v.x * rr, v.y * rr);
+      }
       buf[j * TI + l] = v;
     }
     __syncthreads();
@@ -167,7 +176,7 @@ __global__ void __launch_bounds__(kThreads) k_fft_phi(double2 *e, int nrdim, int
       double2 D = make_double2(0.5 * (zm.x - zc.x), 0.5 * (zm.y - zc.y));
       double2 O = make_double2(D.y, -D.x);  // -i * D
       double2 C = cadd(E, cmul(twh[mm], O));
-      if (i < nrdim) e[plane + (size_t)mm * nrdim + i] = make_double2(C.x * sc, C.y * sc);
+      if (i < nrdim) out[plane + (size_t)mm * nrdim + i] = make_double2(C.x * sc, C.y * sc);
     }
   } else {
     // Z_m = (C_m + conj C_{N-m}) + i (C_m - conj C_{N-m}) e^{+2 pi i m/np}
@@ -176,8 +185,8 @@ __global__ void __launch_bounds__(kThreads) k_fft_phi(double2 *e, int nrdim, int
       int i = i0 + l;
       double2 Z = make_double2(0.0, 0.0);
       if (i < nrdim) {
-        double2 cm = e[plane + (size_t)mm * nrdim + i];
-        double2 cc = e[plane + (size_t)(N - mm) * nrdim + i];
+        double2 cm = in[plane + (size_t)mm * nrdim + i];
+        double2 cc = in[plane + (size_t)(N - mm) * nrdim + i];
         if (mm == 0) { cm.y = 0.0; cc.y = 0.0; }  // imaginary parts of DC and Nyquist are ignored
         cc.y = -cc.y;
         double2 E = cadd(cm, cc);
@@ -195,7 +204,11 @@ __global__ void __launch_bounds__(kThreads) k_fft_phi(double2 *e, int nrdim, int
       int i = i0 + l;
       if (i >= nrdim) continue;
       double2 v = (j < N) ? buf[j * TI + l] : make_double2(0.0, 0.0);
-      e[plane + (size_t)j * nrdim + i] = v;
+      if (rscale < 0 && i < nr) {
+        const double rr = r[i];
+        v = make_double2(v.x / rr, v.y / rr);
+      }
+      out[plane + (size_t)j * nrdim + i] = v;
     }
   }
 }
@@ -242,7 +255,7 @@ int tile_rows(int n_line, int nrdim) {
 
 namespace mlx {
 
-void fft_phi(mlx_ctx *c, double2 *e, int dir) {
+void fft_phi(mlx_ctx *c, double2 *out, const double2 *in, int dir, int rscale) {
   const KitDev &k = c->kit;
   if (k.np <= 1) return;
   if (k.nph > 512) MLX_THROW(MLX_E_ARG, "fft_phi: np/2 above 512 is not supported");
@@ -251,8 +264,8 @@ void fft_phi(mlx_ctx *c, double2 *e, int dir) {
   // packed reals in, nph+1 modes out (or back): nr rows x nz slices
   ProbeScope probe(c, MLX_PROBE_FFT_PHI, 16.0 * k.nr * k.nz * (2.0 * k.nph + 1),
                    2.5 * k.np * std::log2((double)k.np) * k.nr * k.nz);
-  hipLaunchKernelGGL(k_fft_phi, grid, dim3(kThreads), 0, c->stream, e, k.nrdim, k.npdim, k.nzdim, TI, k.plan_nph,
-                     k.tw_nph, k.tw_half, dir);
+  hipLaunchKernelGGL(k_fft_phi, grid, dim3(kThreads), 0, c->stream, out, in, k.nrdim, k.npdim, k.nzdim, TI,
+                     k.plan_nph, k.tw_nph, k.tw_half, dir, k.r, k.nr, rscale);
   HIPCHK(hipGetLastError());
 }
 

@@ -167,7 +167,9 @@ void kit_build(mlx_ctx *c);
 void kit_free(KitDev &k);
 
 // fft.hip
-void fft_phi(mlx_ctx *c, double2 *e, int dir);  // dir -1 forward (PPP->PFP), +1 backward
+// dir -1 forward (PPP->PFP), +1 backward; out may equal in; rscale: +1 multiply
+// input rows i < nr by r_i (forward), -1 divide output rows by r_i (backward)
+void fft_phi(mlx_ctx *c, double2 *out, const double2 *in, int dir, int rscale = 0);
 // ncol: the columns the caller needs transformed (algorithmic work of the
 // launch); the kernel transforms every column of the field.
 void fft_z(mlx_ctx *c, double2 *e, int dir, double ncol);  // dir -1 forward (/nz), +1 backward
@@ -206,6 +208,10 @@ void k_abcn_rhs(mlx_ctx *c, double2 *sh, double *shln, const double2 *s, const d
                 const double2 *svis, double dt, double a, int ab2);
 void k_scale_rows_r(mlx_ctx *c, double2 *e, int inverse);
 void k_tp2vec_combine(mlx_ctx *c, double2 *ur, double2 *up, const double2 *psi, const double2 *uz, int nroff);
+// mode 0: (P, C) = (psi, chi); mode 1: P = -del2(p_in = chi) over the rows of
+// chi's radial offset md_off, C = c_in = psi
+void k_tp2vec_front(mlx_ctx *c, double2 *ur, double2 *up, double2 *uz, const double2 *p_in, const double2 *c_in,
+                    const double *p_ln, const double *c_ln, int mode, int md_off);
 void k_vecprod(mlx_ctx *c, double2 *vr, double2 *vp, double2 *vz, const double2 *ur, const double2 *up,
                const double2 *uz);
 void k_vec2tp_up_ln(mlx_ctx *c, double2 *up, const double2 *inf, double *psiln);

@@ -430,6 +430,142 @@ __global__ void k_tp2vec_combine_(double2 *ur, double2 *up, const double2 *psi, 
   }
 }
 
+// ---------------------------------------------------------------------------
+// Front half of tp2vec / tp2curlvec in one pass per (m, kappa) column
+// (vector_reconstruction, mlegs_scalar_ops.f90:1455-1545; curl_vector_
+// reconstruction, :1547-1560).  With P, C the two input scalars (mode 0:
+// P = psi, C = chi; mode 1: P = -del2(chi) as tp2curlvec forms it, C = psi)
+// and all band products taken over the rows of the +3 radial offset:
+//   ur = i m P + i kappa xxdx(C),  up = -xxdx(P) - m kappa C,  uz = -del2h(C),
+// then chopped to the offset-0 rows, columns and slices (rows, columns and
+// slices outside are written as 0).  Every band product is applied as many
+// times as the reference's two kk-loops visit the slice, and the log-term
+// corrections of the (m, kappa) = (0, 0) column are added after the band
+// products, as in xxdx / del2h / del2.  The results have no log term.
+// ---------------------------------------------------------------------------
+__device__ inline double2 band_row(const double2 *x, int i, int n, int op, int m, int ne, const double *d2c,
+                                   const double *xxc, double akk) {
+  double2 s = make_double2(0.0, 0.0);
+  if (op < 2) {
+    const double *b = d2c + ((size_t)m * ne + i) * 5;
+#pragma unroll
+    for (int d = 0; d < 5; ++d) {
+      const int j = i + d - 2;
+      if (j < 0 || j >= n) continue;
+      double c = b[d];
+      if (d == 2) c = c - akk;
+      s.x += c * x[j].x;
+      s.y += c * x[j].y;
+    }
+  } else {
+    const double *b = xxc + ((size_t)m * ne + i) * 2;
+    if (i >= 1) { s.x += b[0] * x[i - 1].x; s.y += b[0] * x[i - 1].y; }
+    if (i + 1 < n) { s.x += b[1] * x[i + 1].x; s.y += b[1] * x[i + 1].y; }
+  }
+  return s;
+}
+
+// y = B^cnt x over rows < n with tmp as scratch (all in LDS); y may alias x
+__device__ inline void band_pow(double2 *y, const double2 *x, double2 *tmp, int n, int cnt, int op, int m, int ne,
+                                const double *d2c, const double *xxc, double akk) {
+  const double2 *src = x;
+  for (int rep = 0; rep < cnt; ++rep) {
+    for (int i = threadIdx.x; i < n; i += blockDim.x) tmp[i] = band_row(src, i, n, op, m, ne, d2c, xxc, akk);
+    __syncthreads();
+    for (int i = threadIdx.x; i < n; i += blockDim.x) y[i] = tmp[i];
+    __syncthreads();
+    src = y;
+  }
+}
+
+__global__ void __launch_bounds__(64) k_tp2vec_front_(double2 *ur, double2 *up, double2 *uz, const double2 *p_in,
+                                                      const double2 *c_in, const double *p_ln, const double *c_ln,
+                                                      int mode, int md_off, const double *d2c, const double *xxc,
+                                                      const double *lognorm, const double *ak, int nrdim, int npdim,
+                                                      int nz, int ne, int nrc, int npc, int nzc, int nzcu,
+                                                      double ell) {
+  extern __shared__ double2 lds[];
+  double2 *P = lds, *C = lds + nrdim, *A = lds + 2 * nrdim, *B = lds + 3 * nrdim;
+  const int m = blockIdx.x, k = blockIdx.y;
+  const size_t off = (size_t)k * nrdim * npdim + (size_t)nrdim * m;
+  const int cnt = z_count(k, nzc, nzcu);
+  const int nn0 = nrcs_dev(nrc, npc, m);
+  const double2 zero = make_double2(0.0, 0.0);
+  if (cnt == 0 || nn0 == 0) {
+    for (int i = threadIdx.x; i < nrdim; i += blockDim.x) ur[off + i] = up[off + i] = uz[off + i] = zero;
+    return;
+  }
+  int nn3 = nrcs_dev(nrc + 3, npc, m);
+  if (nn3 > nrdim) nn3 = nrdim;
+  const bool c00 = (m == 0 && k == 0);
+  const double ell2 = ell * ell;
+  const double akk = ak[k] * ak[k];
+  double lnP = 0.0;
+  const double lnC = c_ln[0];
+  if (mode == 0) {
+    lnP = p_ln[0];
+    for (int i = threadIdx.x; i < nn3; i += blockDim.x) P[i] = p_in[off + i];
+  } else {
+    // md = -del2(chi) with the copy semantics of del2 (rows past the offset-0
+    // system keep chi's values), log terms of chi added, then negated
+    for (int i = threadIdx.x; i < nn3; i += blockDim.x) P[i] = p_in[off + i];
+    __syncthreads();
+    int nmd = nrcs_dev(nrc + md_off, npc, m);
+    if (nmd > nn3) nmd = nn3;
+    band_pow(P, P, A, nmd, cnt, 0, m, ne, d2c, xxc, akk);
+    if (c00 && threadIdx.x == 0) {
+      const double l = p_ln[0];
+      P[0].x += 4.0 / 3.0 / ell2 / exp(lognorm[0]) * l;
+      if (nn3 > 1) P[1].x -= 2.0 / 1.0 / ell2 / exp(lognorm[1]) * l;
+      if (nn3 > 2) P[2].x += 2.0 / 3.0 / ell2 / exp(lognorm[2]) * l;
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < nn3; i += blockDim.x) P[i] = make_double2(-P[i].x, -P[i].y);
+  }
+  for (int i = threadIdx.x; i < nn3; i += blockDim.x) C[i] = c_in[off + i];
+  __syncthreads();
+  const double mv = (double)m, kv = ak[k];
+  // ur = i m P + i kappa xxdx(C)
+  band_pow(A, C, B, nn3, cnt, 2, m, ne, d2c, xxc, 0.0);
+  if (c00 && threadIdx.x == 0) {
+    A[0].x += 1.0 / exp(lognorm[0]) * lnC;
+    if (nn3 > 1) A[1].x += 1.0 / exp(lognorm[1]) * lnC;
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < nrdim; i += blockDim.x) {
+    double2 v = zero;
+    if (i < nn0) v = make_double2(-mv * P[i].y - kv * A[i].y, mv * P[i].x + kv * A[i].x);
+    ur[off + i] = v;
+  }
+  __syncthreads();
+  // up = -xxdx(P) - m kappa C
+  band_pow(A, P, B, nn3, cnt, 2, m, ne, d2c, xxc, 0.0);
+  if (c00 && threadIdx.x == 0) {
+    A[0].x += 1.0 / exp(lognorm[0]) * lnP;
+    if (nn3 > 1) A[1].x += 1.0 / exp(lognorm[1]) * lnP;
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < nrdim; i += blockDim.x) {
+    double2 v = zero;
+    if (i < nn0) v = make_double2(-A[i].x - mv * kv * C[i].x, -A[i].y - mv * kv * C[i].y);
+    up[off + i] = v;
+  }
+  __syncthreads();
+  // uz = -del2h(C)
+  band_pow(A, C, B, nn3, cnt, 1, m, ne, d2c, xxc, 0.0);
+  if (c00 && threadIdx.x == 0) {
+    A[0].x += 4.0 / 3.0 / ell2 / exp(lognorm[0]) * lnC;
+    if (nn3 > 1) A[1].x -= 2.0 / 1.0 / ell2 / exp(lognorm[1]) * lnC;
+    if (nn3 > 2) A[2].x += 2.0 / 3.0 / ell2 / exp(lognorm[2]) * lnC;
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < nrdim; i += blockDim.x) {
+    double2 v = zero;
+    if (i < nn0) v = make_double2(-A[i].x, -A[i].y);
+    uz[off + i] = v;
+  }
+}
+
 // v <- v x u on the two packed phi points (mlegs_scalar_ops.f90:1284-1296)
 __global__ void k_vecprod_(double2 *vr, double2 *vp, double2 *vz, const double2 *ur, const double2 *up,
                            const double2 *uz, int nr, int nph, int nrdim, int npdim, int nz) {
@@ -645,6 +781,20 @@ void k_tp2vec_combine(mlx_ctx *c, double2 *ur, double2 *up, const double2 *psi, 
   const KitDev &k = c->kit;
   hipLaunchKernelGGL(k_tp2vec_combine_, dim3(grid_for(c->nelem)), dim3(kT), 0, c->stream, ur, up, psi, uz, k.ak,
                      k.nrdim, k.npdim, k.nzdim, k.nrchop + nroff, k.npchop, k.chopzl, k.chopzu);
+}
+
+void k_tp2vec_front(mlx_ctx *c, double2 *ur, double2 *up, double2 *uz, const double2 *p_in, const double2 *c_in,
+                    const double *p_ln, const double *c_ln, int mode, int md_off) {
+  const KitDev &k = c->kit;
+  if (k.nrchop + 3 > k.nrdim || k.nrchop + 3 > k.ne) MLX_THROW(MLX_E_ARG, "chop_index: chopping in r too large");
+  if (md_off < 0 || md_off > 3 || k.nrchop + md_off > k.nrdim)
+    MLX_THROW(MLX_E_ARG, "curl_vector_reconstruction: chopping offset of chi out of range");
+  dim3 grid(k.npdim, k.nzdim);
+  size_t lds = 4 * sizeof(double2) * (size_t)k.nrdim;
+  hipLaunchKernelGGL(k_tp2vec_front_, grid, dim3(64), lds, c->stream, ur, up, uz, p_in, c_in, p_ln, c_ln, mode,
+                     md_off, k.d2c, k.xxc, k.lognorm, k.ak, k.nrdim, k.npdim, k.nzdim, k.ne, k.nrchop, k.npchop, k.chopzl,
+                     k.chopzu, k.ell);
+  HIPCHK(hipGetLastError());
 }
 
 void k_vecprod(mlx_ctx *c, double2 *vr, double2 *vp, double2 *vz, const double2 *ur, const double2 *up,

@@ -200,6 +200,29 @@ def test_tp_operators(setup):
     assert abs(nchi.ln - onchi.ln) < 1e-10 * max(1, abs(onchi.ln))
 
 
+def test_tp_operators_unchopped_with_offsets(setup):
+    """tp2vec / tp2curlvec on unchopped inputs with log terms and a radial chop
+    offset on chi: the copy semantics of del2 (rows past chi's system keep
+    their values) and the +3-offset band products show up in the result."""
+    import mlegs_amd as M
+    from oracle import scalar_ops as so
+    name, p, tfm, okit = setup
+    _say('tp ops unchopped', name)
+    nr, nph = p.nr, p.np // 2
+    gpsi, opsi = C.pair(tfm, okit, C.spectral(tfm.dims, p, 41, admissible=False), 'FFF', ln=0.3)
+    gchi, ochi = C.pair(tfm, okit, C.spectral(tfm.dims, p, 42, admissible=False), 'FFF', ln=-0.2,
+                        offsets=(2, 0, 0))
+    vr, vp, vz = M.Scalar(tfm), M.Scalar(tfm), M.Scalar(tfm)
+    M.tp2vec(gpsi, gchi, vr, vp, vz, tfm)
+    ovr, ovp, ovz = so.tp2vec(opsi, ochi, okit)
+    for a, b in ((vr, ovr), (vp, ovp), (vz, ovz)):
+        assert rel_err(phys(a.e, nr, nph), phys(b.e, nr, nph)) < 1e-11
+    M.tp2curlvec(gpsi, gchi, vr, vp, vz, tfm)
+    ovr, ovp, ovz = so.tp2curlvec(opsi, ochi, okit)
+    for a, b in ((vr, ovr), (vp, ovp), (vz, ovz)):
+        assert rel_err(phys(a.e, nr, nph), phys(b.e, nr, nph)) < 1e-11
+
+
 def test_vortex_steps(setup):
     import mlegs_amd as M
     from oracle import scalar_ops as so, vortex as V
