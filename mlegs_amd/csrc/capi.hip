@@ -89,6 +89,7 @@ void trans(mlx_ctx *c, mlx_field *f, const char *space, int rscale = 0) {
   if (nw < 0) MLX_THROW(MLX_E_ARG, "trans: only taking PPP, PFP, FFP and FFF for spectral transformation");
   const KitDev &k = c->kit;
   if (f->off[1] != 0 || f->off[2] != 0) MLX_THROW(MLX_E_ARG, "trans: only radial chopping offsets are supported");
+  const int start = cur;
   if (cur < nw) {
     while (true) {
       if (cur == 0) {
@@ -102,7 +103,11 @@ void trans(mlx_ctx *c, mlx_field *f, const char *space, int rscale = 0) {
         set_space(f, "FFP");
       }
       if (cur == 2) {
-        if (k.nz > 1) fft_z(c, f->e, -1, active_pairs(k, f->off[0]));
+        // after rtrans_forward in this call the inactive lines are zero: only
+        // the active (n, m) lines need the z transform
+        const bool act = start <= 1;
+        if (k.nz > 1)
+          fft_z(c, f->e, -1, active_pairs(k, f->off[0]), act ? k.nrchop + f->off[0] : 0, act ? k.npchop : 0);
         set_space(f, "FFF");
       }
       if (cur == 3) break;
@@ -113,7 +118,10 @@ void trans(mlx_ctx *c, mlx_field *f, const char *space, int rscale = 0) {
   if (cur > nw) {
     while (true) {
       if (cur == 3) {
-        if (k.nz > 1) fft_z(c, f->e, +1, active_pairs(k, f->off[0]));
+        // when rtrans_backward follows, it reads the active (n, m) lines only
+        const bool act = nw <= 1;
+        if (k.nz > 1)
+          fft_z(c, f->e, +1, active_pairs(k, f->off[0]), act ? k.nrchop + f->off[0] : 0, act ? k.npchop : 0);
         set_space(f, "FFP");
       }
       if (cur == 2) {
@@ -285,14 +293,15 @@ void vec2tp(mlx_ctx *c, const mlx_field *vr, const mlx_field *vp, const mlx_fiel
   phi_forward_from(c, uz, vz, 0);
   k_set_scalar(c, psi->sc, 0.0);
   k_set_scalar(c, chi->sc, 0.0);
-  if (k.nz > 1) fft_z(c, ur->e, -1, (double)k.nr * k.npchop);
+  // the projection reads columns m < npchop only
+  if (k.nz > 1) fft_z(c, ur->e, -1, (double)k.nr * k.npchop, 0, k.npchop);
   trans(c, up, "FFF");
   double2 *inf = reinterpret_cast<double2 *>(c->dsmall + 2048);
   k_calcat(c, up->e, k.at1, k.nrchop + up->off[0], inf);
   leg_backward(c, up->e, c->scr[0], up->off[0]);
   swap_scr(c, up);
   k_vec2tp_up_ln(c, up->e, inf, psi->sc);
-  if (k.nz > 1) fft_z(c, uz->e, -1, (double)k.nr * k.npchop);
+  if (k.nz > 1) fft_z(c, uz->e, -1, (double)k.nr * k.npchop, 0, k.npchop);
   vec2tp_project(c, ur->e, up->e, uz->e, psi->e, chi->e, 3);
   set_space(psi, "FFF");
   set_space(chi, "FFF");

@@ -243,6 +243,238 @@ __global__ void __launch_bounds__(kThreads) k_fft_z(double2 *e, int nrdim, int n
   }
 }
 
+// ---------------------------------------------------------------------------
+// Fast path for power-of-two lengths N = N1 * N2 (N1, N2 <= 16): four-step
+// FFT.  Stage 1 loads x[n2 + N2 n1] of a line straight into registers, runs
+// the N1-point DFT over n1 there and applies W_N^{n2 k1}; one LDS exchange;
+// stage 2 runs the N2-point DFT over n2 and writes X[k1 + N1 k2].  A
+// workgroup owns TI consecutive rows (lines) of one column (z) or plane (phi),
+// so every global access is a run of TI consecutive complex numbers.
+// The real phi transform adds its even/odd untangling (forward) or tangling
+// (backward) through a second LDS exchange.
+// ---------------------------------------------------------------------------
+__host__ __device__ constexpr double c16(int q) {
+  // cos(2 pi q / 16), q = 0..7
+  return q == 0 ? 1.0 : q == 1 ? 0.92387953251128675613 : q == 2 ? 0.70710678118654752440
+       : q == 3 ? 0.38268343236508977173 : q == 4 ? 0.0 : q == 5 ? -0.38268343236508977173
+       : q == 6 ? -0.70710678118654752440 : -0.92387953251128675613;
+}
+__host__ __device__ constexpr double s16(int q) {
+  // sin(2 pi q / 16), q = 0..7
+  return q == 0 ? 0.0 : q == 1 ? 0.38268343236508977173 : q == 2 ? 0.70710678118654752440
+       : q == 3 ? 0.92387953251128675613 : q == 4 ? 1.0 : q == 5 ? 0.92387953251128675613
+       : q == 6 ? 0.70710678118654752440 : 0.38268343236508977173;
+}
+__host__ __device__ constexpr int bitrev(int i, int n) {
+  int r = 0;
+  for (int b = 1; b < n; b <<= 1) {
+    r = (r << 1) | (i & 1);
+    i >>= 1;
+  }
+  return r;
+}
+
+// in-register radix-2 DIT DFT of size N (N <= 16), sign sg of the exponent
+template <int N>
+__device__ __forceinline__ void dft_reg(double2 *v, double sg) {
+  double2 t[N];
+#pragma unroll
+  for (int i = 0; i < N; ++i) t[i] = v[bitrev(i, N)];
+#pragma unroll
+  for (int len = 2; len <= N; len <<= 1) {
+    const int h = len / 2;
+#pragma unroll
+    for (int i0 = 0; i0 < N; i0 += len) {
+#pragma unroll
+      for (int j = 0; j < h; ++j) {
+        const double2 a = t[i0 + j];
+        double2 b = t[i0 + j + h];
+        if (j != 0) {
+          const int q = j * (16 / len);
+          const double cw = c16(q), sw = sg * s16(q);
+          b = make_double2(b.x * cw - b.y * sw, b.x * sw + b.y * cw);
+        }
+        t[i0 + j] = make_double2(a.x + b.x, a.y + b.y);
+        t[i0 + j + h] = make_double2(a.x - b.x, a.y - b.y);
+      }
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < N; ++i) v[i] = t[i];
+}
+
+__device__ __forceinline__ double2 twid(const double2 *tw, int idx, double sg) {
+  double2 w = tw[idx];
+  if (sg > 0) w.y = -w.y;
+  return w;
+}
+
+// MODE 0: complex transform (z); MODE 1: real forward (phi); MODE 2: real backward (phi)
+template <int N1, int N2, int TI, int MODE>
+__global__ void __launch_bounds__(kThreads) k_fft4(double2 *out, const double2 *in, size_t es, size_t gs, int nrow,
+                                                   int act_nrc, const double2 *tw, const double2 *twh, double sg,
+                                                   double scale, const double *r, int nr, int rscale) {
+  constexpr int N = N1 * N2;
+  constexpr int T1 = (TI * N2 + kThreads - 1) / kThreads;
+  constexpr int T2 = (TI * N1 + kThreads - 1) / kThreads;
+  __shared__ double2 buf[TI * N];
+  __shared__ double2 nyq[MODE == 2 ? TI : 1];
+  const int i0 = blockIdx.x * TI;
+  const size_t base = (size_t)blockIdx.y * gs;
+  int rlim = nrow;
+  if (act_nrc > 0) {
+    const int a = act_nrc - (int)blockIdx.y;
+    rlim = a < 0 ? 0 : (a < nrow ? a : nrow);
+  }
+  if (i0 >= rlim) return;  // the whole tile lies outside the active rows
+  const int tid = threadIdx.x;
+
+  if (MODE == 2) {
+    // stage 0: C_m, m = 0..N, into LDS (C_N apart)
+    for (int b = tid; b < TI * (N + 1); b += kThreads) {
+      const int l = b % TI, m = b / TI, i = i0 + l;
+      const double2 v = (i < rlim) ? in[base + (size_t)m * es + i] : make_double2(0.0, 0.0);
+      if (m < N) buf[m * TI + l] = v;
+      else nyq[l] = v;
+    }
+    __syncthreads();
+  }
+
+  // stage 1
+  double2 y1[T1][N1];
+#pragma unroll
+  for (int t = 0; t < T1; ++t) {
+    const int b = tid + t * kThreads;
+    if (b < TI * N2) {
+      const int l = b % TI, n2 = b / TI, i = i0 + l;
+      double2 v[N1];
+#pragma unroll
+      for (int n1 = 0; n1 < N1; ++n1) {
+        const int j = n2 + N2 * n1;
+        if (MODE == 2) {
+          double2 cm = buf[j * TI + l];
+          double2 cc = (j == 0) ? nyq[l] : buf[(N - j) * TI + l];
+          if (j == 0) { cm.y = 0.0; cc.y = 0.0; }  // imaginary parts of DC and Nyquist are ignored
+          cc.y = -cc.y;
+          const double2 E = cadd(cm, cc);
+          double2 w = twh[j];
+          w.y = -w.y;
+          const double2 O = cmul(csub(cm, cc), w);
+          v[n1] = make_double2(E.x - O.y, E.y + O.x);
+        } else {
+          double2 x = (i < rlim) ? in[base + (size_t)j * es + i] : make_double2(0.0, 0.0);
+          if (MODE == 1 && rscale > 0 && i < nr) {
+            const double rr = r[i];
+            x = make_double2(x.x * rr, x.y * rr);
+          }
+          v[n1] = x;
+        }
+      }
+      dft_reg<N1>(v, sg);
+#pragma unroll
+      for (int k1 = 1; k1 < N1; ++k1) {
+        const int q = (n2 * k1) & (N - 1);
+        if (q) v[k1] = cmul(v[k1], twid(tw, q, sg));
+      }
+#pragma unroll
+      for (int k1 = 0; k1 < N1; ++k1) y1[t][k1] = v[k1];
+    }
+  }
+  if (MODE == 2) __syncthreads();  // stage 0 data fully consumed
+#pragma unroll
+  for (int t = 0; t < T1; ++t) {
+    const int b = tid + t * kThreads;
+    if (b < TI * N2) {
+      const int l = b % TI, n2 = b / TI;
+#pragma unroll
+      for (int k1 = 0; k1 < N1; ++k1) buf[(k1 * N2 + n2) * TI + l] = y1[t][k1];
+    }
+  }
+  __syncthreads();
+
+  // stage 2
+  double2 y2[T2][N2];
+#pragma unroll
+  for (int t = 0; t < T2; ++t) {
+    const int b = tid + t * kThreads;
+    if (b < TI * N1) {
+      const int l = b % TI, k1 = b / TI;
+#pragma unroll
+      for (int n2 = 0; n2 < N2; ++n2) y2[t][n2] = buf[(k1 * N2 + n2) * TI + l];
+      dft_reg<N2>(y2[t], sg);
+    }
+  }
+  if (MODE == 0 || MODE == 2) {
+#pragma unroll
+    for (int t = 0; t < T2; ++t) {
+      const int b = tid + t * kThreads;
+      if (b < TI * N1) {
+        const int l = b % TI, k1 = b / TI, i = i0 + l;
+        if (i < rlim) {
+          double dv = 1.0;
+          if (MODE == 2 && rscale < 0 && i < nr) dv = r[i];
+#pragma unroll
+          for (int k2 = 0; k2 < N2; ++k2) {
+            double2 x = y2[t][k2];
+            if (MODE == 0) x = make_double2(x.x * scale, x.y * scale);
+            else if (dv != 1.0) x = make_double2(x.x / dv, x.y / dv);
+            out[base + (size_t)(k1 + N1 * k2) * es + i] = x;
+          }
+          if (MODE == 2 && k1 == 0) out[base + (size_t)N * es + i] = make_double2(0.0, 0.0);
+        }
+      }
+    }
+    return;
+  }
+  // MODE 1: natural order into LDS, then C_m = E + W^m O, m = 0..N, scaled 1/np
+  __syncthreads();
+#pragma unroll
+  for (int t = 0; t < T2; ++t) {
+    const int b = tid + t * kThreads;
+    if (b < TI * N1) {
+      const int l = b % TI, k1 = b / TI;
+#pragma unroll
+      for (int k2 = 0; k2 < N2; ++k2) buf[(k1 + N1 * k2) * TI + l] = y2[t][k2];
+    }
+  }
+  __syncthreads();
+  for (int b = tid; b < TI * (N + 1); b += kThreads) {
+    const int l = b % TI, m = b / TI, i = i0 + l;
+    const double2 zm = buf[(m % N) * TI + l];
+    double2 zc = buf[((N - m) % N) * TI + l];
+    zc.y = -zc.y;
+    const double2 E = make_double2(0.5 * (zm.x + zc.x), 0.5 * (zm.y + zc.y));
+    const double2 D = make_double2(0.5 * (zm.x - zc.x), 0.5 * (zm.y - zc.y));
+    const double2 O = make_double2(D.y, -D.x);  // -i * D
+    const double2 C = cadd(E, cmul(twh[m], O));
+    if (i < rlim) out[base + (size_t)m * es + i] = make_double2(C.x * scale, C.y * scale);
+  }
+}
+
+template <int MODE>
+bool launch_fft4(mlx_ctx *c, int N, dim3 grid_y, double2 *out, const double2 *in, size_t es, size_t gs, int nrow,
+                 int act_nrc, const double2 *tw, const double2 *twh, double sg, double scale, int rscale) {
+  const KitDev &k = c->kit;
+#define MLX_FFT4(N1_, N2_, TI_)                                                                               \
+  do {                                                                                                        \
+    dim3 grid((nrow + TI_ - 1) / TI_, grid_y.y);                                                              \
+    hipLaunchKernelGGL((k_fft4<N1_, N2_, TI_, MODE>), grid, dim3(kThreads), 0, c->stream, out, in, es, gs, nrow, \
+                       act_nrc, tw, twh, sg, scale, k.r, k.nr, rscale);                                      \
+    return true;                                                                                              \
+  } while (0)
+  switch (N) {
+    case 16: MLX_FFT4(4, 4, 64);
+    case 32: MLX_FFT4(4, 8, 64);
+    case 64: MLX_FFT4(8, 8, 32);
+    case 128: MLX_FFT4(8, 16, 16);
+    case 256:
+      if (MODE == 0) MLX_FFT4(16, 16, 16);  // 64 KiB of LDS
+      MLX_FFT4(16, 16, 8);                  // the real modes also hold the Nyquist column
+    default: return false;
+  }
+#undef MLX_FFT4
+}
+
 int tile_rows(int n_line, int nrdim) {
   int ti = kLdsElems / (n_line + 1);
   if (ti > 64) ti = 64;
@@ -264,18 +496,33 @@ void fft_phi(mlx_ctx *c, double2 *out, const double2 *in, int dir, int rscale) {
   // packed reals in, nph+1 modes out (or back): nr rows x nz slices
   ProbeScope probe(c, MLX_PROBE_FFT_PHI, 16.0 * k.nr * k.nz * (2.0 * k.nph + 1),
                    2.5 * k.np * std::log2((double)k.np) * k.nr * k.nz);
+  const size_t plane = (size_t)k.nrdim * k.npdim;
+  bool fast = (dir < 0) ? launch_fft4<1>(c, k.nph, dim3(1, k.nzdim), out, in, k.nrdim, plane, k.nrdim, 0, k.tw_nph,
+                                         k.tw_half, -1.0, 1.0 / k.np, rscale)
+                        : launch_fft4<2>(c, k.nph, dim3(1, k.nzdim), out, in, k.nrdim, plane, k.nrdim, 0, k.tw_nph,
+                                         k.tw_half, +1.0, 1.0, rscale);
+  if (fast) {
+    HIPCHK(hipGetLastError());
+    return;
+  }
   hipLaunchKernelGGL(k_fft_phi, grid, dim3(kThreads), 0, c->stream, out, in, k.nrdim, k.npdim, k.nzdim, TI,
                      k.plan_nph, k.tw_nph, k.tw_half, dir, k.r, k.nr, rscale);
   HIPCHK(hipGetLastError());
 }
 
-void fft_z(mlx_ctx *c, double2 *e, int dir, double ncol) {
+void fft_z(mlx_ctx *c, double2 *e, int dir, double ncol, int act_nrc, int ncols) {
   const KitDev &k = c->kit;
   if (k.nz <= 1) return;
   if (k.nz > 512) MLX_THROW(MLX_E_ARG, "fft_z: nz above 512 is not supported");
+  if (ncols <= 0 || ncols > k.npdim) ncols = k.npdim;
+  ProbeScope probe(c, MLX_PROBE_FFT_Z, 32.0 * ncol * k.nz, 5.0 * k.nz * std::log2((double)k.nz) * ncol);
+  if (launch_fft4<0>(c, k.nz, dim3(1, ncols), e, e, (size_t)k.nrdim * k.npdim, k.nrdim, k.nrdim, act_nrc, k.tw_nz,
+                     nullptr, dir < 0 ? -1.0 : 1.0, dir < 0 ? 1.0 / k.nz : 1.0, 0)) {
+    HIPCHK(hipGetLastError());
+    return;
+  }
   int TI = tile_rows(k.nz, k.nrdim);
   dim3 grid((k.nrdim + TI - 1) / TI, k.npdim);
-  ProbeScope probe(c, MLX_PROBE_FFT_Z, 32.0 * ncol * k.nz, 5.0 * k.nz * std::log2((double)k.nz) * ncol);
   hipLaunchKernelGGL(k_fft_z, grid, dim3(kThreads), 0, c->stream, e, k.nrdim, k.npdim, k.nzdim, TI, k.plan_nz,
                      k.tw_nz, dir);
   HIPCHK(hipGetLastError());

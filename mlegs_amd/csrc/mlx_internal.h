@@ -170,9 +170,12 @@ void kit_free(KitDev &k);
 // dir -1 forward (PPP->PFP), +1 backward; out may equal in; rscale: +1 multiply
 // input rows i < nr by r_i (forward), -1 divide output rows by r_i (backward)
 void fft_phi(mlx_ctx *c, double2 *out, const double2 *in, int dir, int rscale = 0);
-// ncol: the columns the caller needs transformed (algorithmic work of the
-// launch); the kernel transforms every column of the field.
-void fft_z(mlx_ctx *c, double2 *e, int dir, double ncol);  // dir -1 forward (/nz), +1 backward
+// dir -1 forward (/nz), +1 backward.  ncol: the (row, column) lines the caller
+// needs transformed (the launch's algorithmic work).  act_nrc > 0: only rows
+// i < act_nrc - j of column j need transforming; ncols > 0: only columns
+// j < ncols.  Lines outside are left as they are (the caller guarantees they
+// are zero or never read); the generic kernel transforms every line.
+void fft_z(mlx_ctx *c, double2 *e, int dir, double ncol, int act_nrc = 0, int ncols = 0);
 
 // legendre.hip
 void leg_forward(mlx_ctx *c, const double2 *in, double2 *out, int nroff, bool zero = true);

@@ -18,6 +18,12 @@ ODD = dict(nr=40, np=40, nz=30, nrchop=36, npchop=21, nzchop=16, ell=1.5, zlen=4
            visc=1e-2, dt=1e-3, hyperpow=0, hypervisc=0.0, is_svv=True)
 
 
+# power-of-two phi and z lengths (np/2 = 32, nz = 16): the four-step FFT kernels;
+# nzchop = nz/2 + 1 repeats the Nyquist slice in the kk-loops
+POW2 = dict(nr=32, np=64, nz=16, nrchop=30, npchop=20, nzchop=9, ell=1.2, zlen=2.5,
+            visc=2e-3, dt=1e-3, hyperpow=0, hypervisc=0.0, is_svv=True)
+
+
 def params(cfg, **kw):
     d = dict(cfg)
     d.update(kw)
