@@ -41,13 +41,15 @@ template <int AT>  // 16-row tiles per wave: AP = 4 * AT * 16
 __global__ void __launch_bounds__(256) k_leg_fwd_f32(const double2 *__restrict__ in, double2 *__restrict__ out,
                                                      const float *__restrict__ pfg, const double *__restrict__ w,
                                                      int nrdim, int npdim, int nz, int nr, int nrh, int nrc,
-                                                     int npc) {
+                                                     int npc, int nyt) {
   constexpr int AP = 4 * AT * 16;
   constexpr int APP = AP + 4;
   __shared__ float Bs[2][KC][BJP];
   __shared__ float As[2][KC][APP];
-  const int m = blockIdx.x;
-  const int k0 = blockIdx.y * KT;
+  int m, yt;
+  xcd_map(blockIdx.x, nyt, m, yt);
+  if (m >= npdim) return;
+  const int k0 = yt * KT;
   const int tid = threadIdx.x, lane = tid % 64, wv = tid / 64;
   const size_t zs = (size_t)nrdim * npdim;
   double2 *ocol = out + (size_t)nrdim * m;
@@ -154,13 +156,15 @@ __global__ void __launch_bounds__(256) k_leg_fwd_f32(const double2 *__restrict__
 template <int XT>  // 16-column tiles of x per wave: XP = 4 * XT * 16 >= nrh
 __global__ void __launch_bounds__(256) k_leg_bwd_f32(const double2 *__restrict__ in, double2 *__restrict__ out,
                                                      const float *__restrict__ pf32, int nrdim, int npdim, int nz,
-                                                     int nr, int nrh, int ne, int nrc, int npc) {
+                                                     int nr, int nrh, int ne, int nrc, int npc, int nyt) {
   constexpr int XP = 4 * XT * 16;
   constexpr int XPP = XP + 4;
   __shared__ float Es[2][KC][BJP];
   __shared__ float Ps[2][KC][XPP];
-  const int m = blockIdx.x;
-  const int k0 = blockIdx.y * KT;
+  int m, yt;
+  xcd_map(blockIdx.x, nyt, m, yt);
+  if (m >= npdim) return;
+  const int k0 = yt * KT;
   const int tid = threadIdx.x, lane = tid % 64, wv = tid / 64;
   const size_t zs = (size_t)nrdim * npdim;
   double2 *ocol = out + (size_t)nrdim * m;
@@ -258,14 +262,16 @@ __global__ void __launch_bounds__(256) k_v2t_f32(const double2 *__restrict__ ur,
                                                  double2 *__restrict__ chi, const float *__restrict__ tgV,
                                                  const float *__restrict__ tgD, const float *__restrict__ tgT,
                                                  const double *__restrict__ ak, int nrdim, int npdim, int nz, int nr,
-                                                 int nrh, int nrc0, int npc, int nzc, int nzcu) {
+                                                 int nrh, int nrc0, int npc, int nzc, int nzcu, int nyt) {
   constexpr int AP = 4 * AT * 16;
   constexpr int APP = AP + 4;
   constexpr int KV = 16;  // contraction chunk for this kernel
   __shared__ float X[5][KV][BJP];
   __shared__ float Tb[3][KV][APP];
-  const int m = blockIdx.x >> 1, p = blockIdx.x & 1;
-  const int k0 = blockIdx.y * KT;
+  int u, yt;
+  xcd_map(blockIdx.x, nyt, u, yt);
+  const int m = u >> 1, p = u & 1;
+  const int k0 = yt * KT;
   const int tid = threadIdx.x, lane = tid % 64, wv = tid / 64;
   const size_t zs = (size_t)nrdim * npdim;
   if (m >= npc) return;
@@ -422,29 +428,31 @@ void mfma_tables(mlx_ctx *c) {
 
 void leg_forward_f32(mlx_ctx *c, const double2 *in, double2 *out) {
   const KitDev &k = c->kit;
-  dim3 grid(k.npdim, (k.nzdim + KT - 1) / KT);
+  const int nyt = (k.nzdim + KT - 1) / KT;
+  dim3 grid = xcd_grid(k.npdim, nyt);
   int at = mfma_tiles(k);
   ProbeScope probe(c, MLX_PROBE_LEG_FWD, leg_bytes(k, 0, 4), leg_flops(k, 0));
   if (at == 1)
     hipLaunchKernelGGL(k_leg_fwd_f32<1>, grid, dim3(256), 0, c->stream, in, out, k.pfg32, k.w, k.nrdim, k.npdim,
-                       k.nzdim, k.nr, k.nrh, k.nrchop, k.npchop);
+                       k.nzdim, k.nr, k.nrh, k.nrchop, k.npchop, nyt);
   else
     hipLaunchKernelGGL(k_leg_fwd_f32<2>, grid, dim3(256), 0, c->stream, in, out, k.pfg32, k.w, k.nrdim, k.npdim,
-                       k.nzdim, k.nr, k.nrh, k.nrchop, k.npchop);
+                       k.nzdim, k.nr, k.nrh, k.nrchop, k.npchop, nyt);
   HIPCHK(hipGetLastError());
 }
 
 void leg_backward_f32(mlx_ctx *c, const double2 *in, double2 *out) {
   const KitDev &k = c->kit;
-  dim3 grid(k.npdim, (k.nzdim + KT - 1) / KT);
+  const int nyt = (k.nzdim + KT - 1) / KT;
+  dim3 grid = xcd_grid(k.npdim, nyt);
   int xt = (k.nrh + 63) / 64;
   ProbeScope probe(c, MLX_PROBE_LEG_BWD, leg_bytes(k, 0, 4), leg_flops(k, 0));
   if (xt == 1)
     hipLaunchKernelGGL(k_leg_bwd_f32<1>, grid, dim3(256), 0, c->stream, in, out, k.pf32, k.nrdim, k.npdim, k.nzdim,
-                       k.nr, k.nrh, k.ne, k.nrchop, k.npchop);
+                       k.nr, k.nrh, k.ne, k.nrchop, k.npchop, nyt);
   else
     hipLaunchKernelGGL(k_leg_bwd_f32<2>, grid, dim3(256), 0, c->stream, in, out, k.pf32, k.nrdim, k.npdim, k.nzdim,
-                       k.nr, k.nrh, k.ne, k.nrchop, k.npchop);
+                       k.nr, k.nrh, k.ne, k.nrchop, k.npchop, nyt);
   HIPCHK(hipGetLastError());
 }
 
@@ -453,15 +461,16 @@ void vec2tp_project_f32(mlx_ctx *c, const double2 *ur, const double2 *up, const 
   const KitDev &k = c->kit;
   HIPCHK(hipMemsetAsync(psi, 0, sizeof(double2) * c->nelem, c->stream));
   HIPCHK(hipMemsetAsync(chi, 0, sizeof(double2) * c->nelem, c->stream));
-  dim3 grid(2 * k.npchop, (k.nzdim + KT - 1) / KT);
+  const int nyt = (k.nzdim + KT - 1) / KT;
+  dim3 grid = xcd_grid(2 * k.npchop, nyt);
   int at = mfma_tiles(k);
   ProbeScope probe(c, MLX_PROBE_V2T, v2t_bytes(k, 0, 4), 5.0 * leg_flops(k, 0));
   if (at == 1)
     hipLaunchKernelGGL(k_v2t_f32<1>, grid, dim3(256), 0, c->stream, ur, up, uz, psi, chi, k.tVg32, k.tDg32, k.tTg32,
-                       k.ak, k.nrdim, k.npdim, k.nzdim, k.nr, k.nrh, k.nrchop, k.npchop, k.chopzl, k.chopzu);
+                       k.ak, k.nrdim, k.npdim, k.nzdim, k.nr, k.nrh, k.nrchop, k.npchop, k.chopzl, k.chopzu, nyt);
   else
     hipLaunchKernelGGL(k_v2t_f32<2>, grid, dim3(256), 0, c->stream, ur, up, uz, psi, chi, k.tVg32, k.tDg32, k.tTg32,
-                       k.ak, k.nrdim, k.npdim, k.nzdim, k.nr, k.nrh, k.nrchop, k.npchop, k.chopzl, k.chopzu);
+                       k.ak, k.nrdim, k.npdim, k.nzdim, k.nr, k.nrh, k.nrchop, k.npchop, k.chopzl, k.chopzu, nyt);
   HIPCHK(hipGetLastError());
 }
 

@@ -244,6 +244,18 @@ static inline double v2t_bytes(const KitDev &k, int nroff, int tbytes) {
   return 48.0 * k.nr * k.npchop * k.nzdim + 32.0 * S * k.nzdim + 3.0 * tbytes * k.nrh * S;
 }
 
+// XCD-aware work mapping.  Workgroups are dispatched round-robin over the 8
+// XCDs (linear id L runs on XCD L % 8), each with its own L2.  The nyt
+// workgroups of one column u (they all read the same table slice) get ids
+// with the same L % 8, so the slice is fetched from HBM into one L2 only.
+constexpr int kXcd = 8;
+__device__ inline void xcd_map(int L, int nyt, int &u, int &y) {
+  const int x = L % kXcd, q = L / kXcd;
+  y = q % nyt;
+  u = (q / nyt) * kXcd + x;
+}
+inline dim3 xcd_grid(int nu, int nyt) { return dim3(((nu + kXcd - 1) / kXcd) * kXcd * nyt); }
+
 // helpers shared by host code
 static inline int nrcs_of(const KitDev &k, int m, int nroff) {
   int nrc = k.nrchop + nroff;
