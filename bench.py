@@ -34,7 +34,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 METRIC = 'steps/s @256³ mixed-prec vs FP64 at ≤1e-6 rel.err; adjoint/forward time ratio'
-PEAK_TFLOPS = {'fp64': 78.6, 'fp32': 157.3, 'bf16x3': 2500.0}
+PEAK_TFLOPS = {'fp64': 78.6, 'fp32': 157.3}
 HBM_GBPS = 8000.0
 HBM_BOUND = ('fft_phi', 'fft_z', 'lu_solve')
 TRAFFIC_FILE = os.path.join(ROOT, 'profiles', 'traffic_256.json')
@@ -220,7 +220,7 @@ def main():
     ap.add_argument('--gpus', type=int, default=1)
     ap.add_argument('--steps', type=int, default=20)
     ap.add_argument('--warmup', type=int, default=3)
-    ap.add_argument('--prec', default='fp32', choices=['fp64', 'fp32', 'bf16x3'])
+    ap.add_argument('--prec', default='fp32', choices=['fp64', 'fp32'])
     ap.add_argument('--size', type=int, default=256)
     ap.add_argument('--cpu-baseline', type=int, default=1)
     ap.add_argument('--errors', type=int, default=1, help='measure the errors against FP64')
@@ -236,7 +236,7 @@ def main():
     import mlegs_amd as M
 
     p = workload(args.size)
-    precs = {'fp64': M.PREC_FP64, 'fp32': M.PREC_FP32, 'bf16x3': M.PREC_BF16X3}
+    precs = {'fp64': M.PREC_FP64, 'fp32': M.PREC_FP32}
     prec = precs[args.prec]
     log('kit', p.nr, p.np, p.nz)
     tfm = M.Kit(p, device=0, precision=M.PREC_FP64)
@@ -273,7 +273,7 @@ def main():
         log('cpu baseline (oracle step)')
         cpu = cpu_baseline(p)
 
-    dtype = {'fp64': 'f64', 'fp32': 'f32-mfma+f64', 'bf16x3': 'bf16x3-mfma+f64'}[args.prec]
+    dtype = {'fp64': 'f64', 'fp32': 'f32-mfma+f64'}[args.prec]
     out = {
         'metric': METRIC, 'value': rate, 'unit': 'steps/s', 'n_gpus': 1, 'steps': args.steps,
         'warmup': args.warmup, 'ms_per_step': ms, 'higher_is_better': True, 'scaling': 'weak',

@@ -48,7 +48,8 @@ extern "C" {
 /* precision of the Legendre contractions (trans and vec2tp) */
 #define MLX_PREC_FP64   0    /* FP64 operands, FP64 accumulation            */
 #define MLX_PREC_FP32   1    /* FP32 operands, FP32 MFMA accumulation       */
-#define MLX_PREC_BF16X3 2    /* 3-term split-BF16 operands, FP32 MFMA acc.   */
+#define MLX_PREC_BF16X3 2    /* 3-term split-BF16 on MFMA: reserved; rejected
+                                (MLX_E_ARG) by this build                   */
 
 /* kernel probes (measurement): see mlx_probe_start */
 #define MLX_PROBE_FFT_PHI   0  /* phi FFT, one launch per direction       */
@@ -102,7 +103,8 @@ int mlx_kit_dims(mlx_ctx* ctx, int32_t* out9);
  * (src/modules/mlegs_spectfm.f90:17-28). */
 int mlx_kit_get(mlx_ctx* ctx, const char* which, double* host, int64_t n);
 
-/* Legendre-contraction precision used by mlx_trans and mlx_vec2tp. */
+/* Legendre-contraction precision used by mlx_trans and mlx_vec2tp:
+ * MLX_PREC_FP64 or MLX_PREC_FP32 (MLX_PREC_BF16X3 returns MLX_E_ARG). */
 int mlx_set_precision(mlx_ctx* ctx, int prec);
 
 /* ---- fields --------------------------------------------------------------- */

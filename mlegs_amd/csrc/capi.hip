@@ -494,8 +494,9 @@ int mlx_kit_get(mlx_ctx *c, const char *which, double *host, int64_t n) {
 
 int mlx_set_precision(mlx_ctx *c, int prec) {
   return guarded(c, [&] {
-    if (prec != MLX_PREC_FP64 && prec != MLX_PREC_FP32 && prec != MLX_PREC_BF16X3)
-      MLX_THROW(MLX_E_ARG, "mlx_set_precision: unknown precision");
+    if (prec == MLX_PREC_BF16X3)
+      MLX_THROW(MLX_E_ARG, "mlx_set_precision: the split-BF16 contractions are not part of this build");
+    if (prec != MLX_PREC_FP64 && prec != MLX_PREC_FP32) MLX_THROW(MLX_E_ARG, "mlx_set_precision: unknown precision");
     c->prec = prec;
   });
 }

@@ -180,29 +180,6 @@ __global__ void k_to_f32(size_t n, const double *a, float *b) {
   if (i < n) b[i] = (float)a[i];
 }
 
-// a = b0 + b1 + b2 with b_k bf16 (round-to-nearest-even at each stage).
-__device__ inline uint16_t f32_to_bf16_rne(float f) {
-  uint32_t u = __float_as_uint(f);
-  uint32_t lsb = (u >> 16) & 1u;
-  u += 0x7fffu + lsb;
-  return (uint16_t)(u >> 16);
-}
-__device__ inline float bf16_to_f32(uint16_t h) { return __uint_as_float(((uint32_t)h) << 16); }
-
-__global__ void k_to_bf16x3(size_t n, const double *a, uint16_t *b) {
-  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  double v = a[i];
-  uint16_t h0 = f32_to_bf16_rne((float)v);
-  double r1 = v - (double)bf16_to_f32(h0);
-  uint16_t h1 = f32_to_bf16_rne((float)r1);
-  double r2 = r1 - (double)bf16_to_f32(h1);
-  uint16_t h2 = f32_to_bf16_rne((float)r2);
-  b[i] = h0;
-  b[i + n] = h1;
-  b[i + 2 * n] = h2;
-}
-
 FftPlan make_plan(int n) {
   FftPlan p;
   std::memset(&p, 0, sizeof(p));
@@ -244,7 +221,7 @@ T *dalloc(size_t n) {
 namespace mlx {
 
 void kit_free(KitDev &k) {
-  void *ptrs[] = {k.x, k.w, k.lnx, k.r, k.ak, k.lognorm, k.pf, k.pf32, k.pfbf, k.at0, k.at1, k.d2c,
+  void *ptrs[] = {k.x, k.w, k.lnx, k.r, k.ak, k.lognorm, k.pf, k.pf32, k.at0, k.at1, k.d2c,
                   k.xxc, k.tV, k.tD, k.tT, k.tV32, k.tD32, k.tT32, k.tw_nph, k.tw_nz, k.tw_half,
                   k.pfg32, k.tVg32, k.tDg32, k.tTg32};
   for (void *p : ptrs)
@@ -254,7 +231,6 @@ void kit_free(KitDev &k) {
   k.pf32 = k.tV32 = k.tD32 = k.tT32 = nullptr;
   k.pfg32 = k.tVg32 = k.tDg32 = k.tTg32 = nullptr;
   k.mfma_ap = 0;
-  k.pfbf = nullptr;
   k.tw_nph = k.tw_nz = k.tw_half = nullptr;
   k.ready = false;
 }
@@ -286,7 +262,6 @@ void kit_build(mlx_ctx *c) {
   k.lognorm = dalloc<double>((size_t)k.ne * k.npchop);
   k.pf = dalloc<double>((size_t)k.nrh * k.ne * k.npchop);
   k.pf32 = dalloc<float>((size_t)k.nrh * k.ne * k.npchop);
-  k.pfbf = dalloc<uint16_t>((size_t)3 * k.nrh * k.ne * k.npchop);
   k.at0 = dalloc<double>(k.nrchop);
   k.at1 = dalloc<double>(k.nrchop);
   k.d2c = dalloc<double>((size_t)k.ne * k.npchop * 5);
@@ -316,7 +291,6 @@ void kit_build(mlx_ctx *c) {
                      k.npchop, k.nt, k.x, k.w, k.pf, k.xxc, k.tV, k.tD, k.tT);
   size_t npf = (size_t)k.nrh * k.ne * k.npchop;
   hipLaunchKernelGGL(k_to_f32, dim3((npf + 255) / 256), dim3(256), 0, s, npf, k.pf, k.pf32);
-  hipLaunchKernelGGL(k_to_bf16x3, dim3((npf + 255) / 256), dim3(256), 0, s, npf, k.pf, k.pfbf);
   hipLaunchKernelGGL(k_to_f32, dim3((ntab + 255) / 256), dim3(256), 0, s, ntab, k.tV, k.tV32);
   hipLaunchKernelGGL(k_to_f32, dim3((ntab + 255) / 256), dim3(256), 0, s, ntab, k.tD, k.tD32);
   hipLaunchKernelGGL(k_to_f32, dim3((ntab + 255) / 256), dim3(256), 0, s, ntab, k.tT, k.tT32);

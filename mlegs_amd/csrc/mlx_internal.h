@@ -51,7 +51,6 @@ struct KitDev {
   double *lognorm = nullptr;  // [m][n]
   double *pf = nullptr;       // [m][n][x]
   float *pf32 = nullptr;      // FP32 copy of pf (same layout)
-  uint16_t *pfbf = nullptr;   // 3 split-BF16 planes of pf, plane stride = nrh*ne*npchop
   double *at0 = nullptr, *at1 = nullptr;
   double *d2c = nullptr;      // [m][n][5]: del2 band rows (offsets -2..2), /ell^2, no ak
   double *xxc = nullptr;      // [m][n][2]: xxdx band rows (offsets -1, +1)

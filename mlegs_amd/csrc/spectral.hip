@@ -478,7 +478,7 @@ __device__ inline void band_pow(double2 *y, const double2 *x, double2 *tmp, int 
   }
 }
 
-__global__ void __launch_bounds__(64) k_tp2vec_front_(double2 *ur, double2 *up, double2 *uz, const double2 *p_in,
+__global__ void __launch_bounds__(256) k_tp2vec_front_(double2 *ur, double2 *up, double2 *uz, const double2 *p_in,
                                                       const double2 *c_in, const double *p_ln, const double *c_ln,
                                                       int mode, int md_off, const double *d2c, const double *xxc,
                                                       const double *lognorm, const double *ak, int nrdim, int npdim,
@@ -486,7 +486,8 @@ __global__ void __launch_bounds__(64) k_tp2vec_front_(double2 *ur, double2 *up, 
                                                       double ell) {
   extern __shared__ double2 lds[];
   double2 *P = lds, *C = lds + nrdim, *A = lds + 2 * nrdim, *B = lds + 3 * nrdim;
-  const int m = blockIdx.x, k = blockIdx.y;
+  // kappa fastest: consecutive workgroups share m and its band coefficients
+  const int k = blockIdx.x, m = blockIdx.y;
   const size_t off = (size_t)k * nrdim * npdim + (size_t)nrdim * m;
   const int cnt = z_count(k, nzc, nzcu);
   const int nn0 = nrcs_dev(nrc, npc, m);
@@ -502,6 +503,65 @@ __global__ void __launch_bounds__(64) k_tp2vec_front_(double2 *ur, double2 *up, 
   const double akk = ak[k] * ak[k];
   double lnP = 0.0;
   const double lnC = c_ln[0];
+  if (cnt == 1) {
+    // one application of each band product: every output row comes straight
+    // from the staged columns
+    const double mv1 = (double)m, kv1 = ak[k];
+    if (mode == 0) {
+      lnP = p_ln[0];
+      for (int i = threadIdx.x; i < nn3; i += blockDim.x) {
+        P[i] = p_in[off + i];
+        C[i] = c_in[off + i];
+      }
+      __syncthreads();
+    } else {
+      int nmd = nrcs_dev(nrc + md_off, npc, m);
+      if (nmd > nn3) nmd = nn3;
+      const double lch = p_ln[0];
+      for (int i = threadIdx.x; i < nn3; i += blockDim.x) {
+        A[i] = p_in[off + i];
+        C[i] = c_in[off + i];
+      }
+      __syncthreads();
+      for (int i = threadIdx.x; i < nn3; i += blockDim.x) {
+        double2 v = A[i];
+        if (i < nmd) {
+          v = band_row(A, i, nmd, 0, m, ne, d2c, xxc, akk);
+          if (c00) {
+            if (i == 0) v.x += 4.0 / 3.0 / ell2 / exp(lognorm[0]) * lch;
+            if (i == 1) v.x -= 2.0 / 1.0 / ell2 / exp(lognorm[1]) * lch;
+            if (i == 2) v.x += 2.0 / 3.0 / ell2 / exp(lognorm[2]) * lch;
+          }
+        }
+        P[i] = make_double2(-v.x, -v.y);
+      }
+      __syncthreads();
+    }
+    for (int i = threadIdx.x; i < nrdim; i += blockDim.x) {
+      double2 vr = zero, vp = zero, vz = zero;
+      if (i < nn0) {
+        double2 xc = band_row(C, i, nn3, 2, m, ne, d2c, xxc, 0.0);
+        double2 xp = band_row(P, i, nn3, 2, m, ne, d2c, xxc, 0.0);
+        double2 hc = band_row(C, i, nn3, 1, m, ne, d2c, xxc, 0.0);
+        if (c00) {
+          if (i < 2) {
+            xc.x += 1.0 / exp(lognorm[i]) * lnC;
+            xp.x += 1.0 / exp(lognorm[i]) * lnP;
+          }
+          if (i == 0) hc.x += 4.0 / 3.0 / ell2 / exp(lognorm[0]) * lnC;
+          if (i == 1) hc.x -= 2.0 / 1.0 / ell2 / exp(lognorm[1]) * lnC;
+          if (i == 2) hc.x += 2.0 / 3.0 / ell2 / exp(lognorm[2]) * lnC;
+        }
+        vr = make_double2(-mv1 * P[i].y - kv1 * xc.y, mv1 * P[i].x + kv1 * xc.x);
+        vp = make_double2(-xp.x - mv1 * kv1 * C[i].x, -xp.y - mv1 * kv1 * C[i].y);
+        vz = make_double2(-hc.x, -hc.y);
+      }
+      ur[off + i] = vr;
+      up[off + i] = vp;
+      uz[off + i] = vz;
+    }
+    return;
+  }
   if (mode == 0) {
     lnP = p_ln[0];
     for (int i = threadIdx.x; i < nn3; i += blockDim.x) P[i] = p_in[off + i];
@@ -789,9 +849,9 @@ void k_tp2vec_front(mlx_ctx *c, double2 *ur, double2 *up, double2 *uz, const dou
   if (k.nrchop + 3 > k.nrdim || k.nrchop + 3 > k.ne) MLX_THROW(MLX_E_ARG, "chop_index: chopping in r too large");
   if (md_off < 0 || md_off > 3 || k.nrchop + md_off > k.nrdim)
     MLX_THROW(MLX_E_ARG, "curl_vector_reconstruction: chopping offset of chi out of range");
-  dim3 grid(k.npdim, k.nzdim);
+  dim3 grid(k.nzdim, k.npdim);
   size_t lds = 4 * sizeof(double2) * (size_t)k.nrdim;
-  hipLaunchKernelGGL(k_tp2vec_front_, grid, dim3(64), lds, c->stream, ur, up, uz, p_in, c_in, p_ln, c_ln, mode,
+  hipLaunchKernelGGL(k_tp2vec_front_, grid, dim3(256), lds, c->stream, ur, up, uz, p_in, c_in, p_ln, c_ln, mode,
                      md_off, k.d2c, k.xxc, k.lognorm, k.ak, k.nrdim, k.npdim, k.nzdim, k.ne, k.nrchop, k.npchop, k.chopzl,
                      k.chopzu, k.ell);
   HIPCHK(hipGetLastError());

@@ -116,3 +116,13 @@ def test_transform_256_fp32_vs_oracle():
         _say('  prec %d backward %.2e round trip %.2e' % (prec, eb, ef))
         assert eb < tol and ef < tol
     tfm.close()
+
+
+def test_split_bf16_is_rejected():
+    """MLX_PREC_BF16X3 is reserved: this build has no split-BF16 kernels and says so."""
+    import mlegs_amd as M
+    p = C.params(C.SMALL)
+    tfm = M.Kit(p, device=0)
+    with pytest.raises(M.MlxError):
+        tfm.set_precision(M.PREC_BF16X3)
+    tfm.close()

@@ -390,3 +390,37 @@ def test_svv_filter_properties():
     s.e = e0.copy()
     assert so.svv_filter(s, kit, 0.3, p.replace(is_svv=False)) == 0.3
     assert np.array_equal(s.e, e0)
+
+
+def test_calcat_values_at_origin_and_infinity():
+    """calcat0 / calcat1 (:237-309) give f(r=0) and f(r=inf) of the m = 0 column:
+    e^{-r^2} -> (1, 0); r^2/(r^2+L^2) = (1+x)/2 -> (0, 1)."""
+    kit = _kit(nr=64, np=4, nz=1, nrchop=64, npchop=2, nzchop=1, ell=ELL, zlen=1.0)
+    for f, at0, at1, tol in ((lambda r, p, z: np.exp(-r ** 2), 1.0, 0.0, 1e-12),
+                             (lambda r, p, z: r ** 2 / (r ** 2 + ELL ** 2), 0.0, 1.0, 1e-13)):
+        s = so.Scalar(kit, 'PPP')
+        s.e = _sample(kit, f)
+        so.trans(s, 'FFF', kit)
+        c0, c1 = so.calcat0(s, kit), so.calcat1(s, kit)
+        assert abs(c0[0] - at0) < tol and abs(c1[0] - at1) < tol
+    # zeroat1 removes the value at infinity through the n = 0 coefficient
+    so.zeroat1(s, kit)
+    assert abs(so.calcat1(s, kit)[0]) < 1e-14
+
+
+def test_svv_gain_on_a_two_entry_field():
+    """Tail ratio b^2/(a^2+b^2) of a field with one entry below and one at the
+    top of the radial range; gain and damping by hand (:72-155)."""
+    kit = _kit(nr=16, np=4, nz=1, nrchop=16, npchop=2, nzchop=1)
+    p = Params(nr=16, np=4, nz=1, nrchop=16, npchop=2, nzchop=1, is_svv=True,
+               svv_cutoff=0.75, svv_target=6e-3, svv_strength=0.4, svv_relax=0.25)
+    s = so.Scalar(kit, 'FFF')
+    s.e[0, 0, 0] = 1.0
+    s.e[15, 0, 0] = 0.1j          # q = 15/15 = 1
+    ratio = 0.01 / 1.01
+    feedback = min(max(ratio / 6e-3 - 1.0, 0.0), 1.0)
+    gain = 0.75 * 0.2 + 0.25 * feedback
+    g = so.svv_filter(s, kit, 0.2, p)
+    assert abs(g - gain) < 1e-15
+    assert s.e[0, 0, 0] == 1.0
+    assert abs(s.e[15, 0, 0] - 0.1j * math.exp(-0.4 * gain)) < 1e-16
