@@ -27,7 +27,9 @@ typedef float floatx4 __attribute__((ext_vector_type(4)));
 constexpr int KC = 32;        // contraction chunk
 constexpr int KT = 32;        // axial slices per workgroup
 constexpr int BJ = 2 * KT;    // real columns per workgroup
-constexpr int BJP = BJ + 4;   // padded LDS row (floats)
+// LDS row pitches are 16 (mod 64) floats: the four rows an MFMA operand read
+// touches (lane / 16) then fall in disjoint banks
+constexpr int BJP = BJ + 16;  // padded LDS row (floats)
 
 __device__ inline floatx4 mfma(float a, float b, floatx4 c) {
   return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
@@ -43,7 +45,7 @@ __global__ void __launch_bounds__(256) k_leg_fwd_f32(const double2 *__restrict__
                                                      int nrdim, int npdim, int nz, int nr, int nrh, int nrc,
                                                      int npc, int nyt) {
   constexpr int AP = 4 * AT * 16;
-  constexpr int APP = AP + 4;
+  constexpr int APP = AP + 16;
   __shared__ float Bs[2][KC][BJP];
   __shared__ float As[2][KC][APP];
   int m, yt;
@@ -74,10 +76,12 @@ __global__ void __launch_bounds__(256) k_leg_fwd_f32(const double2 *__restrict__
 
   for (int x0 = 0; x0 < nrh; x0 += KC) {
     // table chunk: As[p][x][a] = pfg[m][p][x0+x][a]
-    for (int idx = tid; idx < 2 * KC * AP; idx += 256) {
-      int a = idx % AP, x = (idx / AP) % KC, p = idx / (AP * KC);
-      int xg = x0 + x;
-      As[p][x][a] = (xg < nrh) ? T[((size_t)p * nrh + xg) * AP + a] : 0.f;
+    for (int idx = tid; idx < 2 * KC * (AP / 4); idx += 256) {
+      const int a = 4 * (idx % (AP / 4)), x = (idx / (AP / 4)) % KC, p = idx / ((AP / 4) * KC);
+      const int xg = x0 + x;
+      float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+      if (xg < nrh) v = *reinterpret_cast<const float4 *>(T + ((size_t)p * nrh + xg) * AP + a);
+      *reinterpret_cast<float4 *>(&As[p][x][a]) = v;
     }
     // field chunk, folded with the quadrature weights
     for (int idx = tid; idx < KC * KT; idx += 256) {
@@ -158,7 +162,7 @@ __global__ void __launch_bounds__(256) k_leg_bwd_f32(const double2 *__restrict__
                                                      const float *__restrict__ pf32, int nrdim, int npdim, int nz,
                                                      int nr, int nrh, int ne, int nrc, int npc, int nyt) {
   constexpr int XP = 4 * XT * 16;
-  constexpr int XPP = XP + 4;
+  constexpr int XPP = XP + 16;
   __shared__ float Es[2][KC][BJP];
   __shared__ float Ps[2][KC][XPP];
   int m, yt;
@@ -188,10 +192,21 @@ __global__ void __launch_bounds__(256) k_leg_bwd_f32(const double2 *__restrict__
       for (int p = 0; p < 2; ++p) acc[s][t][p] = floatx4{0.f, 0.f, 0.f, 0.f};
 
   for (int a0 = 0; a0 < Me; a0 += KC) {
-    for (int idx = tid; idx < 2 * KC * XP; idx += 256) {
-      int x = idx % XP, a = (idx / XP) % KC, p = idx / (XP * KC);
-      int n = 2 * (a0 + a) + p;
-      Ps[p][a][x] = (x < nrh && n < nn) ? P[(size_t)n * nrh + x] : 0.f;
+    if ((nrh & 3) == 0) {
+      // four consecutive x per thread: 16-byte loads and LDS writes
+      for (int idx = tid; idx < 2 * KC * (XP / 4); idx += 256) {
+        const int x = 4 * (idx % (XP / 4)), a = (idx / (XP / 4)) % KC, p = idx / ((XP / 4) * KC);
+        const int n = 2 * (a0 + a) + p;
+        float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (x < nrh && n < nn) v = *reinterpret_cast<const float4 *>(P + (size_t)n * nrh + x);
+        *reinterpret_cast<float4 *>(&Ps[p][a][x]) = v;
+      }
+    } else {
+      for (int idx = tid; idx < 2 * KC * XP; idx += 256) {
+        int x = idx % XP, a = (idx / XP) % KC, p = idx / (XP * KC);
+        int n = 2 * (a0 + a) + p;
+        Ps[p][a][x] = (x < nrh && n < nn) ? P[(size_t)n * nrh + x] : 0.f;
+      }
     }
     for (int idx = tid; idx < 2 * KC * KT; idx += 256) {
       int r = idx % (2 * KC), kl = idx / (2 * KC);
@@ -264,7 +279,7 @@ __global__ void __launch_bounds__(256) k_v2t_f32(const double2 *__restrict__ ur,
                                                  const double *__restrict__ ak, int nrdim, int npdim, int nz, int nr,
                                                  int nrh, int nrc0, int npc, int nzc, int nzcu, int nyt) {
   constexpr int AP = 4 * AT * 16;
-  constexpr int APP = AP + 4;
+  constexpr int APP = AP + 16;
   constexpr int KV = 16;  // contraction chunk for this kernel
   __shared__ float X[5][KV][BJP];
   __shared__ float Tb[3][KV][APP];
@@ -291,11 +306,13 @@ __global__ void __launch_bounds__(256) k_v2t_f32(const double2 *__restrict__ ur,
     for (int t = 0; t < 4; ++t) aps[s][t] = ach[s][t] = floatx4{0.f, 0.f, 0.f, 0.f};
 
   for (int x0 = 0; x0 < nrh; x0 += KV) {
-    for (int idx = tid; idx < 3 * KV * AP; idx += 256) {
-      int a = idx % AP, x = (idx / AP) % KV, q = idx / (AP * KV);
-      int xg = x0 + x;
+    for (int idx = tid; idx < 3 * KV * (AP / 4); idx += 256) {
+      const int a = 4 * (idx % (AP / 4)), x = (idx / (AP / 4)) % KV, q = idx / ((AP / 4) * KV);
+      const int xg = x0 + x;
       const float *src = (q == 0) ? tgV : (q == 1) ? tgD : tgT;
-      Tb[q][x][a] = (xg < nrh) ? src[tb + (size_t)xg * AP + a] : 0.f;
+      float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+      if (xg < nrh) v = *reinterpret_cast<const float4 *>(src + tb + (size_t)xg * AP + a);
+      *reinterpret_cast<float4 *>(&Tb[q][x][a]) = v;
     }
     for (int idx = tid; idx < KV * KT; idx += 256) {
       int x = idx % KV, kl = idx / KV;
