@@ -24,7 +24,7 @@ namespace {
 
 typedef float floatx4 __attribute__((ext_vector_type(4)));
 
-constexpr int KC = 16;        // contraction chunk
+constexpr int KC = 8;         // contraction chunk
 constexpr int KT = 32;        // axial slices per workgroup
 constexpr int BJ = 2 * KT;    // real columns per workgroup
 // LDS row pitches are 16 (mod 64) floats: the four rows an MFMA operand read
