@@ -279,8 +279,12 @@ void phi_forward_from(mlx_ctx *c, mlx_field *dst, const mlx_field *src, int rsca
 }
 
 // vec2tp (vector_projection, mlegs_scalar_ops.f90:1308-1453)
+// pre_dealias: the caller de-aliases psi and chi afterwards (advection_rhs).
+// idel2 and chop act on each (m, kappa) column separately and zeroat1 leaves a
+// zero slice at zero, so de-aliasing right after the projection gives the same
+// result and lets the banded solve skip the de-aliased (all-zero) systems.
 void vec2tp(mlx_ctx *c, const mlx_field *vr, const mlx_field *vp, const mlx_field *vz, mlx_field *psi,
-            mlx_field *chi) {
+            mlx_field *chi, bool pre_dealias = false) {
   need_kit(c);
   for (const mlx_field *f : {vr, vp, vz})
     if (!is_ppp(f)) MLX_THROW(MLX_E_LAYOUT, "vector_projection: vr, vp, vz must be in PPP");
@@ -305,6 +309,10 @@ void vec2tp(mlx_ctx *c, const mlx_field *vr, const mlx_field *vp, const mlx_fiel
   vec2tp_project(c, ur->e, up->e, uz->e, psi->e, chi->e, 3);
   set_space(psi, "FFF");
   set_space(chi, "FFF");
+  if (pre_dealias) {
+    dealias(c, psi);
+    dealias(c, chi);
+  }
   idel2_proln(c, chi);
   psi->off[0] = 0;
   chi->off[0] = 0;
@@ -327,7 +335,7 @@ void advection_rhs(mlx_ctx *c, mlx_field *psi, mlx_field *chi, mlx_field *nlpsi,
   k_axpby(c, vz->e, vz->sc, 1.0, uz->e, nullptr, 1.0);
   tp2curlvec(c, psi, chi, wr, wp, wz);
   vecprod(c, vr, vp, vz, wr, wp, wz);
-  vec2tp(c, vr, vp, vz, nlpsi, nlchi);
+  vec2tp(c, vr, vp, vz, nlpsi, nlchi, true);
   k_set_scalar(c, nlpsi->sc, 0.0);
   k_set_scalar(c, nlchi->sc, 0.0);
   dealias(c, nlpsi);

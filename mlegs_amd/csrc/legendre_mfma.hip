@@ -280,7 +280,7 @@ __global__ void __launch_bounds__(256) k_v2t_f32(const double2 *__restrict__ ur,
                                                  int nrh, int nrc0, int npc, int nzc, int nzcu, int nyt) {
   constexpr int AP = 4 * AT * 16;
   constexpr int APP = AP + 16;
-  constexpr int KV = 16;  // contraction chunk for this kernel
+  constexpr int KV = 8;   // contraction chunk for this kernel
   __shared__ float X[5][KV][BJP];
   __shared__ float Tb[3][KV][APP];
   int u, yt;
