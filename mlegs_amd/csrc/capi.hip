@@ -200,11 +200,8 @@ void abcn(mlx_ctx *c, mlx_field *s, mlx_field *s_p, mlx_field *nl, mlx_field *nl
           bool swap_nl) {
   check_step_inputs(c, {s, nl, s_p, nl_p});
   if (c->p.visc < 5e-14) MLX_THROW(MLX_E_ARG, "abcn: inviscid case and no linear term in rhs. semi-implicit time adv is impossible");
-  mlx_field *svis = tmp(c, 10);
-  fcopy(c, svis, s);
-  k_bandmul(c, svis->e, svis->sc, 0, svis->off[0]);
   double a = -2.0 / (dt * c->p.visc);
-  k_abcn_rhs(c, s->e, s->sc, s->e, s->sc, nl->e, nl->sc, nl_p->e, nl_p->sc, svis->e, dt, a, 1);
+  k_abcn_rhs_d2(c, s->e, s->sc, nl->e, nl->sc, nl_p->e, nl_p->sc, dt, a, s->off[0]);
   ihelm(c, s, a);
   if (copy_prev) fcopy(c, s_p, s);
   if (swap_nl) fmove(nl_p, nl);
@@ -322,24 +319,28 @@ void vec2tp(mlx_ctx *c, const mlx_field *vr, const mlx_field *vp, const mlx_fiel
   zeroat1(c, chi);
 }
 
-// advection_rhs (vortical_flow_3d.f90:353-395)
+// advection_rhs (vortical_flow_3d.f90:353-395).  dealiased: psi and chi are
+// already de-aliased (the stepper de-aliases them before svv_filter and
+// zeroat1, which keep a zero mode zero), so the opening dealias is a no-op.
 void advection_rhs(mlx_ctx *c, mlx_field *psi, mlx_field *chi, mlx_field *nlpsi, mlx_field *nlchi,
-                   const mlx_field *uz) {
+                   const mlx_field *uz, bool dealiased = false) {
   if (!is_fff(psi) || !is_fff(chi)) MLX_THROW(MLX_E_LAYOUT, "advection_rhs: psi and chi must be in FFF");
   if (!is_ppp(uz)) MLX_THROW(MLX_E_LAYOUT, "advection_rhs: uz must be in PPP");
-  dealias(c, psi);
-  dealias(c, chi);
+  if (!dealiased) {
+    dealias(c, psi);
+    dealias(c, chi);
+  }
   mlx_field *vr = tmp(c, 4), *vp = tmp(c, 5), *vz = tmp(c, 6);
   mlx_field *wr = tmp(c, 7), *wp = tmp(c, 8), *wz = tmp(c, 9);
   tp2vec(c, psi, chi, vr, vp, vz);
   k_axpby(c, vz->e, vz->sc, 1.0, uz->e, nullptr, 1.0);
   tp2curlvec(c, psi, chi, wr, wp, wz);
   vecprod(c, vr, vp, vz, wr, wp, wz);
+  // de-aliased inside vec2tp before idel2; chop and zeroat1 keep zero modes
+  // zero, so the closing dealias of the reference is already satisfied
   vec2tp(c, vr, vp, vz, nlpsi, nlchi, true);
   k_set_scalar(c, nlpsi->sc, 0.0);
   k_set_scalar(c, nlchi->sc, 0.0);
-  dealias(c, nlpsi);
-  dealias(c, nlchi);
 }
 
 void svv(mlx_ctx *c, mlx_field *f, double *gain_dev) {
@@ -822,7 +823,7 @@ int mlx_vortex_bootstrap(mlx_ctx *c, mlx_vortex *v) {
     svv(c, chi, v->gains + 1);
     zeroat1(c, psi);
     zeroat1(c, chi);
-    advection_rhs(c, psi, chi, nlpsi, nlchi, uz);
+    advection_rhs(c, psi, chi, nlpsi, nlchi, uz, true);
     v->prev_stale = false;
   });
 }
@@ -844,7 +845,7 @@ int mlx_vortex_step(mlx_ctx *c, mlx_vortex *v, int32_t nsteps, double dt, int32_
       svv(c, chi, v->gains + 1);
       zeroat1(c, psi);
       zeroat1(c, chi);
-      advection_rhs(c, psi, chi, nlpsi, nlchi, uz);
+      advection_rhs(c, psi, chi, nlpsi, nlchi, uz, true);
       check_finite(c, psi, chi, v->flag);
     }
     if (check) {

@@ -403,6 +403,55 @@ __global__ void k_abcn_rhs_(double2 *sh, double *shln, const double2 *s, const d
   }
 }
 
+__device__ inline void band_pow(double2 *y, const double2 *x, double2 *tmp, int n, int cnt, int op, int m, int ne,
+                                const double *d2c, const double *xxc, double akk);
+
+// AB2/CN right-hand side with the viscous term formed in place, one (m, kappa)
+// column per workgroup (abcn, mlegs_scalar_ops.f90:1218-1240):
+//   sh = a (s + dt (1.5 nl - 0.5 nl_p) + dt/2 visc svis),  svis = del2(s)
+// where del2 acts on the rows of s's radial offset (applied as often as the
+// kk-loops visit the slice, log-term correction of (0,0) after) and leaves
+// every other entry equal to s, as the reference's copy-then-del2 does.
+__global__ void __launch_bounds__(256) k_abcn_rhs_d2_(double2 *e, const double *sln, const double2 *nl,
+                                                      const double2 *nlp, const double *d2c, const double *lognorm,
+                                                      const double *ak, int nrdim, int npdim, int ne, int nrc,
+                                                      int npc, int nzc, int nzcu, double ell, double dt, double a,
+                                                      double visc) {
+  extern __shared__ double2 lds[];
+  double2 *S = lds, *V = lds + nrdim, *T = lds + 2 * nrdim;
+  const int m = blockIdx.x, k = blockIdx.y;
+  const size_t off = (size_t)k * nrdim * npdim + (size_t)nrdim * m;
+  const int cnt = z_count(k, nzc, nzcu);
+  const int nn = (cnt > 0) ? nrcs_dev(nrc, npc, m) : 0;
+  for (int i = threadIdx.x; i < nrdim; i += blockDim.x) S[i] = e[off + i];
+  __syncthreads();
+  if (nn > 0) {
+    band_pow(V, S, T, nn, cnt, 0, m, ne, d2c, nullptr, ak[k] * ak[k]);
+    if (m == 0 && k == 0 && threadIdx.x == 0) {
+      const double l = sln[0], ell2 = ell * ell;
+      V[0].x += 4.0 / 3.0 / ell2 / exp(lognorm[0]) * l;
+      if (nn > 1) V[1].x -= 2.0 / 1.0 / ell2 / exp(lognorm[1]) * l;
+      if (nn > 2) V[2].x += 2.0 / 3.0 / ell2 / exp(lognorm[2]) * l;
+    }
+    __syncthreads();
+  }
+  const double hdt = dt / 2.0;
+  for (int i = threadIdx.x; i < nrdim; i += blockDim.x) {
+    const double2 x = S[i], y = nl[off + i], z = nlp[off + i];
+    const double2 v = (i < nn) ? V[i] : x;
+    const double tx = x.x + dt * (1.5 * y.x - 0.5 * z.x);
+    const double ty = x.y + dt * (1.5 * y.y - 0.5 * z.y);
+    const double vx = v.x * visc, vy = v.y * visc;
+    e[off + i] = make_double2(a * (tx + hdt * vx), a * (ty + hdt * vy));
+  }
+}
+
+// sh.ln = a (s.ln + dt (1.5 nl.ln - 0.5 nl_p.ln))  (svis has no log term)
+__global__ void k_abcn_ln_(double *shln, const double *sln, const double *nlln, const double *nlpln, double dt,
+                           double a) {
+  shln[0] = a * (sln[0] + dt * (1.5 * nlln[0] - 0.5 * nlpln[0]));
+}
+
 __global__ void k_scale_rows_(double2 *e, const double *r, int nr, int nrdim, int npdim, int nz, int inverse) {
   size_t n = (size_t)nrdim * npdim * nz;
   for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (size_t)gridDim.x * blockDim.x) {
@@ -829,6 +878,20 @@ void k_abcn_rhs(mlx_ctx *c, double2 *sh, double *shln, const double2 *s, const d
                 double a, int ab2) {
   hipLaunchKernelGGL(k_abcn_rhs_, dim3(grid_for(c->nelem)), dim3(kT), 0, c->stream, sh, shln, s, sln, nl, nlln, nlp,
                      nlpln, svis, dt, a, c->p.visc, ab2, c->nelem);
+}
+
+void k_abcn_rhs_d2(mlx_ctx *c, double2 *e, double *ln, const double2 *nl, const double *nlln, const double2 *nlp,
+                   const double *nlpln, double dt, double a, int nroff) {
+  const KitDev &k = c->kit;
+  const int nrc = k.nrchop + nroff;
+  if (nrc > k.nrdim) MLX_THROW(MLX_E_ARG, "chop_index: chopping in r too large");
+  dim3 grid(k.npdim, k.nzdim);
+  size_t lds = 3 * sizeof(double2) * (size_t)k.nrdim;
+  // the log coefficient first: the data kernel reads s's ln for the (0,0) correction
+  hipLaunchKernelGGL(k_abcn_rhs_d2_, grid, dim3(256), lds, c->stream, e, ln, nl, nlp, k.d2c, k.lognorm, k.ak, k.nrdim,
+                     k.npdim, k.ne, nrc, k.npchop, k.chopzl, k.chopzu, k.ell, dt, a, c->p.visc);
+  hipLaunchKernelGGL(k_abcn_ln_, dim3(1), dim3(1), 0, c->stream, ln, ln, nlln, nlpln, dt, a);
+  HIPCHK(hipGetLastError());
 }
 
 void k_scale_rows_r(mlx_ctx *c, double2 *e, int inverse) {
