@@ -74,35 +74,66 @@ __global__ void __launch_bounds__(256) k_leg_fwd_f32(const double2 *__restrict__
 #pragma unroll
       for (int p = 0; p < 2; ++p) acc[s][t][p] = floatx4{0.f, 0.f, 0.f, 0.f};
 
-  for (int x0 = 0; x0 < nrh; x0 += KC) {
-    // table chunk: As[p][x][a] = pfg[m][p][x0+x][a]
-    for (int idx = tid; idx < 2 * KC * (AP / 4); idx += 256) {
-      const int a = 4 * (idx % (AP / 4)), x = (idx / (AP / 4)) % KC, p = idx / ((AP / 4) * KC);
-      const int xg = x0 + x;
+  // Software pipeline: the next chunk's table and field loads are issued
+  // into registers before the current chunk's MFMA work and stored to LDS
+  // after it.  Table: 2 KC AP / 4 float4 per chunk; field: KC KT points,
+  // each the pair x, nr-1-x and the weight, folded when stored.
+  constexpr int NTA = (2 * KC * (AP / 4) + 255) / 256;
+  constexpr int NFA = (KC * KT + 255) / 256;
+  float4 rt[NTA];
+  double2 ra[NFA], rb[NFA];
+  double rw[NFA];
+  auto fetch = [&](int x0) {
+#pragma unroll
+    for (int q = 0; q < NTA; ++q) {
+      const int idx = tid + 256 * q;
       float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
-      if (xg < nrh) v = *reinterpret_cast<const float4 *>(T + ((size_t)p * nrh + xg) * AP + a);
-      *reinterpret_cast<float4 *>(&As[p][x][a]) = v;
-    }
-    // field chunk, folded with the quadrature weights
-    for (int idx = tid; idx < KC * KT; idx += 256) {
-      int x = idx % KC, kl = idx / KC;
-      int xg = x0 + x, kg = k0 + kl;
-      float e0 = 0.f, e1 = 0.f, o0 = 0.f, o1 = 0.f;
-      if (xg < nrh && kg < nz) {
-        double2 a = icol[(size_t)kg * zs + xg];
-        double2 b = icol[(size_t)kg * zs + (nr - 1 - xg)];
-        double wx = w[xg];
-        e0 = (float)((a.x + b.x) * wx);
-        e1 = (float)((a.y + b.y) * wx);
-        o0 = (float)((a.x - b.x) * wx);
-        o1 = (float)((a.y - b.y) * wx);
+      if (idx < 2 * KC * (AP / 4)) {
+        const int a = 4 * (idx % (AP / 4)), x = (idx / (AP / 4)) % KC, p = idx / ((AP / 4) * KC);
+        const int xg = x0 + x;
+        if (xg < nrh) v = *reinterpret_cast<const float4 *>(T + ((size_t)p * nrh + xg) * AP + a);
       }
-      Bs[0][x][2 * kl] = e0;
-      Bs[0][x][2 * kl + 1] = e1;
-      Bs[1][x][2 * kl] = o0;
-      Bs[1][x][2 * kl + 1] = o1;
+      rt[q] = v;
+    }
+#pragma unroll
+    for (int q = 0; q < NFA; ++q) {
+      const int idx = tid + 256 * q;
+      const int x = idx % KC, kl = idx / KC;
+      const int xg = x0 + x, kg = k0 + kl;
+      ra[q] = rb[q] = make_double2(0.0, 0.0);
+      rw[q] = 0.0;
+      if (idx < KC * KT && xg < nrh && kg < nz) {
+        ra[q] = icol[(size_t)kg * zs + xg];
+        rb[q] = icol[(size_t)kg * zs + (nr - 1 - xg)];
+        rw[q] = w[xg];
+      }
+    }
+  };
+  fetch(0);
+  for (int x0 = 0; x0 < nrh; x0 += KC) {
+#pragma unroll
+    for (int q = 0; q < NTA; ++q) {
+      const int idx = tid + 256 * q;
+      if (idx < 2 * KC * (AP / 4)) {
+        const int a = 4 * (idx % (AP / 4)), x = (idx / (AP / 4)) % KC, p = idx / ((AP / 4) * KC);
+        *reinterpret_cast<float4 *>(&As[p][x][a]) = rt[q];
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < NFA; ++q) {
+      const int idx = tid + 256 * q;
+      if (idx < KC * KT) {
+        const int x = idx % KC, kl = idx / KC;
+        const double2 a = ra[q], b = rb[q];
+        const double wx = rw[q];
+        Bs[0][x][2 * kl] = (float)((a.x + b.x) * wx);
+        Bs[0][x][2 * kl + 1] = (float)((a.y + b.y) * wx);
+        Bs[1][x][2 * kl] = (float)((a.x - b.x) * wx);
+        Bs[1][x][2 * kl + 1] = (float)((a.y - b.y) * wx);
+      }
     }
     __syncthreads();
+    if (x0 + KC < nrh) fetch(x0 + KC);
 #pragma unroll 2
     for (int kk = 0; kk < KC; kk += 4) {
       const int xr = kk + lane / 16;
@@ -191,32 +222,8 @@ __global__ void __launch_bounds__(256) k_leg_bwd_f32(const double2 *__restrict__
 #pragma unroll
       for (int p = 0; p < 2; ++p) acc[s][t][p] = floatx4{0.f, 0.f, 0.f, 0.f};
 
-  for (int a0 = 0; a0 < Me; a0 += KC) {
-    if ((nrh & 3) == 0) {
-      // four consecutive x per thread: 16-byte loads and LDS writes
-      for (int idx = tid; idx < 2 * KC * (XP / 4); idx += 256) {
-        const int x = 4 * (idx % (XP / 4)), a = (idx / (XP / 4)) % KC, p = idx / ((XP / 4) * KC);
-        const int n = 2 * (a0 + a) + p;
-        float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
-        if (x < nrh && n < nn) v = *reinterpret_cast<const float4 *>(P + (size_t)n * nrh + x);
-        *reinterpret_cast<float4 *>(&Ps[p][a][x]) = v;
-      }
-    } else {
-      for (int idx = tid; idx < 2 * KC * XP; idx += 256) {
-        int x = idx % XP, a = (idx / XP) % KC, p = idx / (XP * KC);
-        int n = 2 * (a0 + a) + p;
-        Ps[p][a][x] = (x < nrh && n < nn) ? P[(size_t)n * nrh + x] : 0.f;
-      }
-    }
-    for (int idx = tid; idx < 2 * KC * KT; idx += 256) {
-      int r = idx % (2 * KC), kl = idx / (2 * KC);
-      int n = 2 * a0 + r, kg = k0 + kl;
-      double2 v = make_double2(0.0, 0.0);
-      if (n < nn && kg < nz) v = icol[(size_t)kg * zs + n];
-      Es[r & 1][r >> 1][2 * kl] = (float)v.x;
-      Es[r & 1][r >> 1][2 * kl + 1] = (float)v.y;
-    }
-    __syncthreads();
+  // MFMA work of one staged chunk (both parities)
+  auto mfma_chunk = [&]() {
 #pragma unroll 2
     for (int kk = 0; kk < KC; kk += 4) {
       const int ar = kk + lane / 16;
@@ -235,7 +242,75 @@ __global__ void __launch_bounds__(256) k_leg_bwd_f32(const double2 *__restrict__
         }
       }
     }
-    __syncthreads();
+  };
+  // field chunk rows 2 a0 .. 2 a0 + 2 KC - 1 of 32 slices (FP64 -> FP32)
+  auto field_at = [&](int a0, int idx) {
+    const int r = idx % (2 * KC), kl = idx / (2 * KC);
+    const int n = 2 * a0 + r, kg = k0 + kl;
+    return (n < nn && kg < nz) ? icol[(size_t)kg * zs + n] : make_double2(0.0, 0.0);
+  };
+  auto field_put = [&](int idx, double2 v) {
+    const int r = idx % (2 * KC), kl = idx / (2 * KC);
+    Es[r & 1][r >> 1][2 * kl] = (float)v.x;
+    Es[r & 1][r >> 1][2 * kl + 1] = (float)v.y;
+  };
+  if ((nrh & 3) == 0) {
+    // software pipeline: the next chunk's global loads are in flight while
+    // the current chunk runs on MFMA; four consecutive x per 16-byte load
+    constexpr int NTB = (2 * KC * (XP / 4) + 255) / 256;
+    constexpr int NFB = (2 * KC * KT + 255) / 256;
+    float4 rt[NTB];
+    double2 rf[NFB];
+    auto fetch = [&](int a0) {
+#pragma unroll
+      for (int q = 0; q < NTB; ++q) {
+        const int idx = tid + 256 * q;
+        float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (idx < 2 * KC * (XP / 4)) {
+          const int x = 4 * (idx % (XP / 4)), a = (idx / (XP / 4)) % KC, p = idx / ((XP / 4) * KC);
+          const int n = 2 * (a0 + a) + p;
+          if (x < nrh && n < nn) v = *reinterpret_cast<const float4 *>(P + (size_t)n * nrh + x);
+        }
+        rt[q] = v;
+      }
+#pragma unroll
+      for (int q = 0; q < NFB; ++q) {
+        const int idx = tid + 256 * q;
+        rf[q] = (idx < 2 * KC * KT) ? field_at(a0, idx) : make_double2(0.0, 0.0);
+      }
+    };
+    fetch(0);
+    for (int a0 = 0; a0 < Me; a0 += KC) {
+#pragma unroll
+      for (int q = 0; q < NTB; ++q) {
+        const int idx = tid + 256 * q;
+        if (idx < 2 * KC * (XP / 4)) {
+          const int x = 4 * (idx % (XP / 4)), a = (idx / (XP / 4)) % KC, p = idx / ((XP / 4) * KC);
+          *reinterpret_cast<float4 *>(&Ps[p][a][x]) = rt[q];
+        }
+      }
+#pragma unroll
+      for (int q = 0; q < NFB; ++q) {
+        const int idx = tid + 256 * q;
+        if (idx < 2 * KC * KT) field_put(idx, rf[q]);
+      }
+      __syncthreads();
+      if (a0 + KC < Me) fetch(a0 + KC);
+      mfma_chunk();
+      __syncthreads();
+    }
+  } else {
+    for (int a0 = 0; a0 < Me; a0 += KC) {
+      for (int idx = tid; idx < 2 * KC * XP; idx += 256) {
+        int x = idx % XP, a = (idx / XP) % KC, p = idx / (XP * KC);
+        int n = 2 * (a0 + a) + p;
+        Ps[p][a][x] = (x < nrh && n < nn) ? P[(size_t)n * nrh + x] : 0.f;
+      }
+      for (int idx = tid; idx < 2 * KC * KT; idx += 256) field_put(idx, field_at(a0, idx));
+      __syncthreads();
+      mfma_chunk();
+      __syncthreads();
+    }
   }
 #pragma unroll
   for (int s = 0; s < XT; ++s) {
