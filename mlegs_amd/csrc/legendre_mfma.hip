@@ -380,45 +380,74 @@ __global__ void __launch_bounds__(256) k_v2t_f32(const double2 *__restrict__ ur,
 #pragma unroll
     for (int t = 0; t < 4; ++t) aps[s][t] = ach[s][t] = floatx4{0.f, 0.f, 0.f, 0.f};
 
-  for (int x0 = 0; x0 < nrh; x0 += KV) {
-    for (int idx = tid; idx < 3 * KV * (AP / 4); idx += 256) {
-      const int a = 4 * (idx % (AP / 4)), x = (idx / (AP / 4)) % KV, q = idx / ((AP / 4) * KV);
-      const int xg = x0 + x;
-      const float *src = (q == 0) ? tgV : (q == 1) ? tgD : tgT;
-      float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
-      if (xg < nrh) v = *reinterpret_cast<const float4 *>(src + tb + (size_t)xg * AP + a);
-      *reinterpret_cast<float4 *>(&Tb[q][x][a]) = v;
-    }
-    for (int idx = tid; idx < KV * KT; idx += 256) {
-      int x = idx % KV, kl = idx / KV;
-      int xg = x0 + x, kg = k0 + kl;
-      double2 x1 = make_double2(0, 0), x2 = x1, x3 = x1, x4 = x1, x5 = x1;
-      if (xg < nrh && kg < nz) {
-        size_t oa = (size_t)kg * zs + xg, ob = (size_t)kg * zs + (nr - 1 - xg);
-        double kv = ak[kg];
-        double2 a = cr[oa], b = cr[ob];
-        double2 reo = make_double2(a.x + seo * b.x, a.y + seo * b.y);  // eo(ur)
-        double2 roe = make_double2(a.x - seo * b.x, a.y - seo * b.y);  // oe(ur)
-        a = cp[oa];
-        b = cp[ob];
-        double2 peo = make_double2(a.x + seo * b.x, a.y + seo * b.y);
-        double2 poe = make_double2(a.x - seo * b.x, a.y - seo * b.y);
-        a = cz[oa];
-        b = cz[ob];
-        double2 zeo = make_double2(a.x + seo * b.x, a.y + seo * b.y);
-        x1 = make_double2(mv * reo.y, -mv * reo.x);          // -i m eo(ur)
-        x2 = make_double2(-poe.x, -poe.y);                   // -oe(up)
-        x3 = make_double2(-kv * roe.y, kv * roe.x);          // i kv oe(ur)
-        x4 = make_double2(mv * kv * peo.x, mv * kv * peo.y); // m kv eo(up)
-        x5 = make_double2(-zeo.x, -zeo.y);                   // -eo(uz)
+  // Software pipeline as in the forward kernel: raw loads of the next
+  // chunk (three tables; ur, up, uz at x and nr-1-x) are held in registers
+  // during the current chunk's MFMA work and combined when stored.  The
+  // loop starts one chunk early so the fetch code appears once.
+  static_assert(KV * KT == 256, "one field point per thread and chunk");
+  constexpr int NTV = (3 * KV * (AP / 4) + 255) / 256;
+  float4 rt[NTV];
+  double2 rra, rrb, rpa, rpb, rza, rzb;
+  double rkv = 0.0;
+  const int fx = tid % KV, fkl = tid / KV;
+  for (int x0 = -KV; x0 < nrh; x0 += KV) {
+    if (x0 >= 0) {
+#pragma unroll
+      for (int j = 0; j < NTV; ++j) {
+        const int idx = tid + 256 * j;
+        if (idx < 3 * KV * (AP / 4)) {
+          const int a = 4 * (idx % (AP / 4)), x = (idx / (AP / 4)) % KV, q = idx / ((AP / 4) * KV);
+          *reinterpret_cast<float4 *>(&Tb[q][x][a]) = rt[j];
+        }
       }
-      X[0][x][2 * kl] = (float)x1.x; X[0][x][2 * kl + 1] = (float)x1.y;
-      X[1][x][2 * kl] = (float)x2.x; X[1][x][2 * kl + 1] = (float)x2.y;
-      X[2][x][2 * kl] = (float)x3.x; X[2][x][2 * kl + 1] = (float)x3.y;
-      X[3][x][2 * kl] = (float)x4.x; X[3][x][2 * kl + 1] = (float)x4.y;
-      X[4][x][2 * kl] = (float)x5.x; X[4][x][2 * kl + 1] = (float)x5.y;
+      const double kv = rkv;
+      const double2 reo = make_double2(rra.x + seo * rrb.x, rra.y + seo * rrb.y);  // eo(ur)
+      const double2 roe = make_double2(rra.x - seo * rrb.x, rra.y - seo * rrb.y);  // oe(ur)
+      const double2 peo = make_double2(rpa.x + seo * rpb.x, rpa.y + seo * rpb.y);
+      const double2 poe = make_double2(rpa.x - seo * rpb.x, rpa.y - seo * rpb.y);
+      const double2 zeo = make_double2(rza.x + seo * rzb.x, rza.y + seo * rzb.y);
+      const double2 x1 = make_double2(mv * reo.y, -mv * reo.x);           // -i m eo(ur)
+      const double2 x2 = make_double2(-poe.x, -poe.y);                    // -oe(up)
+      const double2 x3 = make_double2(-kv * roe.y, kv * roe.x);           // i kv oe(ur)
+      const double2 x4 = make_double2(mv * kv * peo.x, mv * kv * peo.y);  // m kv eo(up)
+      const double2 x5 = make_double2(-zeo.x, -zeo.y);                    // -eo(uz)
+      X[0][fx][2 * fkl] = (float)x1.x; X[0][fx][2 * fkl + 1] = (float)x1.y;
+      X[1][fx][2 * fkl] = (float)x2.x; X[1][fx][2 * fkl + 1] = (float)x2.y;
+      X[2][fx][2 * fkl] = (float)x3.x; X[2][fx][2 * fkl + 1] = (float)x3.y;
+      X[3][fx][2 * fkl] = (float)x4.x; X[3][fx][2 * fkl + 1] = (float)x4.y;
+      X[4][fx][2 * fkl] = (float)x5.x; X[4][fx][2 * fkl + 1] = (float)x5.y;
+      __syncthreads();
     }
-    __syncthreads();
+    const int xn = x0 + KV;  // chunk fetched now, stored next iteration
+    if (xn < nrh) {
+#pragma unroll
+      for (int j = 0; j < NTV; ++j) {
+        const int idx = tid + 256 * j;
+        float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (idx < 3 * KV * (AP / 4)) {
+          const int a = 4 * (idx % (AP / 4)), x = (idx / (AP / 4)) % KV, q = idx / ((AP / 4) * KV);
+          const int xg = xn + x;
+          const float *src = (q == 0) ? tgV : (q == 1) ? tgD : tgT;
+          if (xg < nrh) v = *reinterpret_cast<const float4 *>(src + tb + (size_t)xg * AP + a);
+        }
+        rt[j] = v;
+      }
+      const int xg = xn + fx, kg = k0 + fkl;
+      const double2 z2 = make_double2(0.0, 0.0);
+      rra = rrb = rpa = rpb = rza = rzb = z2;
+      rkv = 0.0;
+      if (xg < nrh && kg < nz) {
+        const size_t oa = (size_t)kg * zs + xg, ob = (size_t)kg * zs + (nr - 1 - xg);
+        rra = cr[oa];
+        rrb = cr[ob];
+        rpa = cp[oa];
+        rpb = cp[ob];
+        rza = cz[oa];
+        rzb = cz[ob];
+        rkv = ak[kg];
+      }
+    }
+    if (x0 < 0) continue;
 #pragma unroll
     for (int kk = 0; kk < KV; kk += 4) {
       const int xr = kk + lane / 16;
