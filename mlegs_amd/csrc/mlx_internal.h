@@ -81,9 +81,12 @@ struct LuCache {
   double alpha;
   int kind;   // 0: del2 + alpha; 1: idel2 bordered (alpha = 0); 2: idel2, (0,0) excluded
   int nroff;  // radial chop offset the systems were built for
-  double *sol = nullptr;     // factor rows
-  uint16_t *pvw = nullptr;   // row-interchange words
-  int *anyp = nullptr;       // per 64-system group: any interchange
+  int seg = 0;               // rows per lane segment of the solve
+  double *sol = nullptr;     // factor rows (segment layout)
+  uint64_t *pvw = nullptr;   // row-interchange words, one per lane segment
+  int *anyp = nullptr;       // per system: any interchange
+  double fac_bytes = 0.0;    // factor bytes one solve reads
+  int64_t nsys = 0, npiv = 0; // systems, systems with interchanges
   uint64_t stamp = 0;
 };
 

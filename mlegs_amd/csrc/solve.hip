@@ -9,36 +9,48 @@
 // The matrix depends on k only through ak(k)^2, which is bit-identical for
 // k and nz - k, so factors are stored once per |kappa|.
 //
-// Solve layout: systems grouped by 64 consecutive m (one wavefront; its 64
-// columns of the field lie in one z-plane).  Each sweep streams its own dense
-// array, row j of lane l at index ((kk*G + g)*nrc + j)*64 + l
-// (kk = |kappa| index, m = 64g + l):
-//   Ff (double2)  L(j+1,j), L(j+2,j)                 -- forward sweep
-//   Fb (double4)  U(j,j), U(j,j+1), U(j,j+2), 0       -- back substitution
-//   Fx (double2)  U(j,j+3), U(j,j+4)                 -- fill-in, pivoting groups
-// Row interchanges are 2 bits per row in one 16-bit word per 8-row chunk.
-// A group without any interchange has no fill-in (U(j,j+3) = U(j,j+4) = 0)
-// and takes the short path.  Waves for k and nz - k are launched next to
-// each other so the second read of their shared factors hits the caches.
+// Solve: the recurrences of the two triangular sweeps are split over the
+// lanes.  A system's rows are cut into 32 segments of S rows, one per lane,
+// and a wavefront holds the systems of k and nz - k (same factors, lanes
+// 0-31 and 32-63).  Each sweep is an affine map of a small carried state
+// (forward: the three rows w0..w2 being eliminated; back: the four solved
+// rows x(i+1..i+4)); every lane builds its segment's map (state matrix from
+// unit states, offset from its right-hand side), a five-step shuffle scan
+// composes the maps of the preceding segments, and each lane then reruns
+// its segment from the true incoming state with the sequential arithmetic.
+// All of a lane's rows and factors are loaded at once, so a wavefront makes
+// one round trip to memory instead of one per chunk of rows.
+//
+// Factor layout, per system sys = kk * npc + m and row j = S s + u (lane s):
+//   FL (double2) L(j+1,j), L(j+2,j)        at (sys S + u) 32 + s
+//   FU (double2) U(j,j+1), U(j,j+2)        same index
+//   FD (double)  U(j,j)                    same index
+//   FX (double2) U(j,j+3), U(j,j+4)        fill-in, read by pivoting systems
+//   PW (uint64)  2 bits per row interchange, one word per (sys, s)
+// Consecutive lanes read consecutive factor entries.
 #include <cmath>
+#include <vector>
 
 #include "mlx_internal.h"
 
 namespace {
 
 constexpr int KL = 2, KU = 2, KV = 4, LDAB = 7;
-constexpr int CH = 8;  // rows per chunk (one pivot word)
+constexpr int NSEG = 32;  // row segments per system (lanes per system)
 
 struct FacPtrs {
-  double2 *ff;
-  double4 *fb;
+  double2 *fl;
+  double2 *fu;
+  double *fd;
   double2 *fx;
 };
+// nrows = systems x S x 32 padded rows
 __host__ __device__ inline FacPtrs fac_ptrs(double *base, size_t nrows) {
   FacPtrs p;
-  p.ff = reinterpret_cast<double2 *>(base);
-  p.fb = reinterpret_cast<double4 *>(base + 2 * nrows);
-  p.fx = reinterpret_cast<double2 *>(base + 6 * nrows);
+  p.fl = reinterpret_cast<double2 *>(base);
+  p.fu = reinterpret_cast<double2 *>(base + 2 * nrows);
+  p.fd = base + 4 * nrows;
+  p.fx = reinterpret_cast<double2 *>(base + 5 * nrows);
   return p;
 }
 
@@ -50,20 +62,6 @@ __device__ inline int nrcs_dev(int nrc, int npc, int m) {
 __device__ inline int z_count(int k, int nzc, int nzcu) {
   return ((k + 1 <= nzc) ? 1 : 0) + ((k + 1 >= nzcu) ? 1 : 0);
 }
-// launch position p -> (k, kk): k = 0; then kk, nz-kk pairs; then nz/2
-__device__ inline void pos_to_k(int p, int nz, int &k, int &kk) {
-  if (nz == 1 || p == 0) {
-    k = 0;
-    kk = 0;
-  } else if (p == nz - 1) {
-    k = nz / 2;
-    kk = nz / 2;
-  } else {
-    kk = (p + 1) / 2;
-    k = (p & 1) ? kk : nz - kk;
-  }
-}
-
 // (del2 + alpha)(i, j) in the normalised basis; bordered (m=0,k=0 of
 // idel2_proln): column 0 gains the 4/3, -2, 2/3 log-term entries
 // (mlegs_scalar_ops.f90:705-711)
@@ -146,178 +144,381 @@ __global__ void k_lu_factor(double *fac, int8_t *piv, const double *d2c, const d
   }
 }
 
-__global__ void k_lu_repack(const double *fac, const int8_t *piv, FacPtrs F, uint16_t *pvw, int *anyp, int nrdim,
-                            int nkk, int nrc, int npc, int kind) {
+// factors of one system into the segment layout; rows n.. of the padded
+// system are identity rows (U(j,j) = 1, no multipliers)
+__global__ void k_lu_repack(const double *fac, const int8_t *piv, FacPtrs F, uint64_t *pw, int *anyp, int nrdim,
+                            int nkk, int nrc, int npc, int kind, int S) {
   int kk = blockIdx.x * blockDim.x + threadIdx.x;
   int m = blockIdx.y;
   if (kk >= nkk || m >= npc) return;
   if (kind == 2 && m == 0 && kk == 0) return;
   int n = nrcs_dev(nrc, npc, m);
-  const int G = (npc + 63) / 64, g = m / 64, l = m % 64;
-  const int NC = (nrc + CH - 1) / CH;
-  size_t sys = (size_t)m * nkk + kk;
-  const double *ab = fac + sys * (size_t)nrdim * LDAB;
-  const int8_t *ip = piv + sys * (size_t)nrdim;
-  const size_t r0 = ((size_t)kk * G + g) * (size_t)nrc * 64 + l;
-  uint16_t *P = pvw + ((size_t)kk * G + g) * NC * 64 + l;
+  const size_t sysk = (size_t)kk * npc + m;  // segment-layout system index
+  const size_t sysf = (size_t)m * nkk + kk;  // factor-kernel system index
+  const double *ab = fac + sysf * (size_t)nrdim * LDAB;
+  const int8_t *ip = piv + sysf * (size_t)nrdim;
   int any = 0;
-  uint16_t word = 0;
-  for (int j = 0; j < n; ++j) {
-    double u[5];
-    for (int d = 0; d <= 4; ++d) u[d] = (j + d < n) ? ab[(KV - d) + LDAB * (j + d)] : 0.0;
-    const size_t o = r0 + (size_t)j * 64;
-    F.ff[o] = make_double2((j + 1 < n) ? ab[KV + 1 + LDAB * j] : 0.0, (j + 2 < n) ? ab[KV + 2 + LDAB * j] : 0.0);
-    double4 fb;
-    fb.x = u[0];
-    fb.y = u[1];
-    fb.z = u[2];
-    fb.w = 0.0;
-    F.fb[o] = fb;
-    F.fx[o] = make_double2(u[3], u[4]);
-    int p = ip[j];
-    any |= p;
-    word |= (uint16_t)(p << (2 * (j % CH)));
-    if (j % CH == CH - 1 || j == n - 1) {
-      P[(size_t)(j / CH) * 64] = word;
-      word = 0;
+  for (int sg = 0; sg < NSEG; ++sg) {
+    uint64_t word = 0;
+    for (int u = 0; u < S; ++u) {
+      const int j = sg * S + u;
+      const size_t o = (sysk * S + u) * NSEG + sg;
+      if (j < n) {
+        double uu[5];
+        for (int d = 0; d <= 4; ++d) uu[d] = (j + d < n) ? ab[(KV - d) + LDAB * (j + d)] : 0.0;
+        F.fl[o] = make_double2((j + 1 < n) ? ab[KV + 1 + LDAB * j] : 0.0, (j + 2 < n) ? ab[KV + 2 + LDAB * j] : 0.0);
+        F.fu[o] = make_double2(uu[1], uu[2]);
+        F.fd[o] = uu[0];
+        F.fx[o] = make_double2(uu[3], uu[4]);
+        const int p = ip[j];
+        any |= p;
+        word |= (uint64_t)p << (2 * u);
+      } else {
+        F.fl[o] = make_double2(0.0, 0.0);
+        F.fu[o] = make_double2(0.0, 0.0);
+        F.fd[o] = 1.0;
+        F.fx[o] = make_double2(0.0, 0.0);
+      }
     }
+    pw[sysk * NSEG + sg] = word;
   }
-  if (any) atomicOr(&anyp[(size_t)kk * G + g], 1);
+  anyp[sysk] = any;
 }
 
-template <bool PIV>
-__device__ __forceinline__ void solve_column(double2 *b, const double2 *__restrict__ Ff,
-                                             const double4 *__restrict__ Fb, const double2 *__restrict__ Fx,
-                                             const uint16_t *P, int n) {
+__device__ __forceinline__ double2 shup2(double2 v, int d) {
+  return make_double2(__shfl_up(v.x, d, NSEG), __shfl_up(v.y, d, NSEG));
+}
+__device__ __forceinline__ double2 shdn2(double2 v, int d) {
+  return make_double2(__shfl_down(v.x, d, NSEG), __shfl_down(v.y, d, NSEG));
+}
+__device__ __forceinline__ void axpy2(double2 &y, double a, double2 x) {
+  y.x -= a * x.x;
+  y.y -= a * x.y;
+}
+__device__ __forceinline__ void fma2(double2 &y, double a, double2 x) {
+  y.x += a * x.x;
+  y.y += a * x.y;
+}
+// affine maps of a two-row state: (t, p) <- (t, p) o (o, q) = (t o, t q + p)
+__device__ __forceinline__ void compose2(double (&t)[2][2], double2 (&p)[2], const double (&o)[2][2],
+                                         const double2 (&q)[2]) {
+  double2 n0 = p[0], n1 = p[1];
+  fma2(n0, t[0][0], q[0]);
+  fma2(n0, t[0][1], q[1]);
+  fma2(n1, t[1][0], q[0]);
+  fma2(n1, t[1][1], q[1]);
+  p[0] = n0;
+  p[1] = n1;
+  const double a00 = t[0][0] * o[0][0] + t[0][1] * o[1][0], a01 = t[0][0] * o[0][1] + t[0][1] * o[1][1];
+  const double a10 = t[1][0] * o[0][0] + t[1][1] * o[1][0], a11 = t[1][0] * o[0][1] + t[1][1] * o[1][1];
+  t[0][0] = a00;
+  t[0][1] = a01;
+  t[1][0] = a10;
+  t[1][1] = a11;
+}
+// inclusive scan of two-row maps over the 32 lanes of a half: prefix
+// (UP: lane s composes lanes 0..s, later maps applied last) or suffix
+template <bool UP>
+__device__ __forceinline__ void scan2(double (&t)[2][2], double2 (&p)[2], int s) {
+#pragma unroll
+  for (int d = 1; d < NSEG; d *= 2) {
+    double o[2][2];
+    double2 q[2];
+#pragma unroll
+    for (int r = 0; r < 2; ++r) {
+#pragma unroll
+      for (int c = 0; c < 2; ++c) o[r][c] = UP ? __shfl_up(t[r][c], d, NSEG) : __shfl_down(t[r][c], d, NSEG);
+      q[r] = UP ? shup2(p[r], d) : shdn2(p[r], d);
+    }
+    if (UP ? (s >= d) : (s + d < NSEG)) compose2(t, p, o, q);
+  }
+}
+// LDS of one wavefront for the four-row chain of pivoting systems: segment
+// maps (state matrix t, offset p) and the incoming states handed back
+struct SegLds {
+  double t[64][16];
+  double2 p[64][4];
+  double2 in[64][4];
+};
+
+// Forward sweep L y = P b over the segments.  b[u] holds row s S + u on
+// entry and y on exit.  Row j (j < n - 1): interchange w0 with w1 / w2,
+// w1 -= L(j+1,j) w0, w2 -= L(j+2,j) w0, emit w0, shift in b(j+3).
+// Entering a segment, w2 is always the untouched b(j0 + 2) (shifted in
+// after the previous row's interchange), so the carried state is (w0, w1).
+template <int S, bool PIV>
+__device__ __forceinline__ void fwd_sweep(double2 (&b)[S], const double2 (&fl)[S], uint64_t pw, int s, int n) {
   const double2 z2 = make_double2(0.0, 0.0);
-  // ---- forward elimination, L y = P b ----
-  double2 w0 = b[0];
-  double2 w1 = (n > 1) ? b[1] : z2;
-  double2 w2 = (n > 2) ? b[2] : z2;
-  for (int j0 = 0; j0 < n - 1; j0 += CH) {
-    double l1[CH], l2[CH];
-    double2 nb[CH];
-    unsigned pw = PIV ? (unsigned)P[(size_t)(j0 / CH) * 64] : 0u;
+  const int j0 = s * S;
+  // rows j0 + S .. j0 + S + 2 belong to the next lane
+  double2 nb[3];
 #pragma unroll
-    for (int u = 0; u < CH; ++u) {
-      const int j = j0 + u;
-      const double2 f = Ff[(size_t)(j < n ? j : 0) * 64];
-      l1[u] = f.x;
-      l2[u] = f.y;
-      nb[u] = (j + 3 < n) ? b[j + 3] : z2;
-    }
+  for (int q = 0; q < 3; ++q) {
+    nb[q] = shdn2(b[q], 1);
+    if (s == NSEG - 1) nb[q] = z2;
+  }
+  auto inj = [&](int u) -> double2 { return (u + 3 < S) ? b[u + 3] : nb[u + 3 - S]; };
+  // segment map (w0, w1)' = T (w0, w1) + p; lane 0 starts from the true
+  // state (b0, b1, b2) and has T = 0
+  double2 w0 = (s == 0) ? b[0] : z2, w1 = (s == 0) ? b[1] : z2, w2 = b[2];
+  double v[3][2];  // state rows for unit inputs in w0 / w1
 #pragma unroll
-    for (int u = 0; u < CH; ++u) {
-      const int j = j0 + u;
-      if (j < n - 1) {
-        if (PIV) {
-          const unsigned pv = (pw >> (2 * u)) & 3u;
-          if (pv == 1) { double2 t = w0; w0 = w1; w1 = t; }
-          else if (pv == 2) { double2 t = w0; w0 = w2; w2 = t; }
+  for (int r = 0; r < 3; ++r)
+#pragma unroll
+    for (int c = 0; c < 2; ++c) v[r][c] = (s != 0 && r == c) ? 1.0 : 0.0;
+#pragma unroll
+  for (int u = 0; u < S; ++u) {
+    const int j = j0 + u;
+    if (j < n - 1) {
+      if (PIV) {
+        const unsigned pv = (unsigned)(pw >> (2 * u)) & 3u;
+        if (pv == 1) {
+          double2 x = w0; w0 = w1; w1 = x;
+#pragma unroll
+          for (int c = 0; c < 2; ++c) { double y = v[0][c]; v[0][c] = v[1][c]; v[1][c] = y; }
+        } else if (pv == 2) {
+          double2 x = w0; w0 = w2; w2 = x;
+#pragma unroll
+          for (int c = 0; c < 2; ++c) { double y = v[0][c]; v[0][c] = v[2][c]; v[2][c] = y; }
         }
-        w1.x -= l1[u] * w0.x;
-        w1.y -= l1[u] * w0.y;
-        if (j + 2 < n) {
-          w2.x -= l2[u] * w0.x;
-          w2.y -= l2[u] * w0.y;
-        }
-        b[j] = w0;
-        w0 = w1;
-        w1 = w2;
-        w2 = nb[u];
+      }
+      axpy2(w1, fl[u].x, w0);
+      axpy2(w2, fl[u].y, w0);
+      w0 = w1;
+      w1 = w2;
+      w2 = inj(u);
+#pragma unroll
+      for (int c = 0; c < 2; ++c) {
+        const double a0 = v[0][c];
+        v[0][c] = v[1][c] - fl[u].x * a0;
+        v[1][c] = v[2][c] - fl[u].y * a0;
+        v[2][c] = 0.0;
       }
     }
   }
-  b[n - 1] = w0;
-  // ---- back substitution, U x = y ----
-  double2 x1 = z2, x2 = z2, x3 = z2, x4 = z2;
-  for (int i0 = n - 1; i0 >= 0; i0 -= CH) {
-    constexpr int NU = PIV ? 5 : 3;
-    double uu[CH][NU];
-    double2 bb[CH];
+  double t[2][2] = {{v[0][0], v[0][1]}, {v[1][0], v[1][1]}};
+  double2 p[2] = {w0, w1};
+  scan2<true>(t, p, s);
+  // incoming state: the previous lane's outgoing state
+  {
+    const double2 q0 = shup2(p[0], 1), q1 = shup2(p[1], 1);
+    w0 = (s == 0) ? b[0] : q0;
+    w1 = (s == 0) ? b[1] : q1;
+    w2 = b[2];
+  }
 #pragma unroll
-    for (int u = 0; u < CH; ++u) {
-      const int i = i0 - u;
-      const size_t o = (size_t)(i >= 0 ? i : 0) * 64;
-      const double4 f = Fb[o];
-      uu[u][0] = f.x;
-      uu[u][1] = f.y;
-      uu[u][2] = f.z;
+  for (int u = 0; u < S; ++u) {
+    const int j = j0 + u;
+    if (j < n - 1) {
       if (PIV) {
-        const double2 fx = Fx[o];
-        uu[u][NU - 2] = fx.x;
-        uu[u][NU - 1] = fx.y;
+        const unsigned pv = (unsigned)(pw >> (2 * u)) & 3u;
+        if (pv == 1) { double2 x = w0; w0 = w1; w1 = x; }
+        else if (pv == 2) { double2 x = w0; w0 = w2; w2 = x; }
       }
-      bb[u] = (i >= 0) ? b[i] : z2;
-    }
-#pragma unroll
-    for (int u = 0; u < CH; ++u) {
-      const int i = i0 - u;
-      if (i >= 0) {
-        double2 acc = bb[u];
-        if (PIV) {
-          if (i + 4 < n) { acc.x -= x4.x * uu[u][NU - 1]; acc.y -= x4.y * uu[u][NU - 1]; }
-          if (i + 3 < n) { acc.x -= x3.x * uu[u][NU - 2]; acc.y -= x3.y * uu[u][NU - 2]; }
-        }
-        if (i + 2 < n) { acc.x -= x2.x * uu[u][2]; acc.y -= x2.y * uu[u][2]; }
-        if (i + 1 < n) { acc.x -= x1.x * uu[u][1]; acc.y -= x1.y * uu[u][1]; }
-        const double d = uu[u][0];
-        double2 xi = make_double2(acc.x / d, acc.y / d);
-        b[i] = xi;
-        x4 = x3;
-        x3 = x2;
-        x2 = x1;
-        x1 = xi;
-      }
+      axpy2(w1, fl[u].x, w0);
+      axpy2(w2, fl[u].y, w0);
+      const double2 nx = inj(u);  // read before b[u] is overwritten (u + 3 > u)
+      b[u] = w0;
+      w0 = w1;
+      w1 = w2;
+      w2 = nx;
+    } else if (j == n - 1) {
+      b[u] = w0;
     }
   }
 }
 
-__global__ void __launch_bounds__(64) k_lu_solve(double2 *e, FacPtrs Fp, const uint16_t *__restrict__ pvw,
+// Back substitution U x = y over the segments, rows in descending order:
+// x(i) = (y(i) - U(i,i+4) x(i+4) - U(i,i+3) x(i+3) - U(i,i+2) x(i+2)
+//         - U(i,i+1) x(i+1)) / U(i,i)
+// The carried state is x(i+1..i+D): D = 2 without interchanges (U has two
+// superdiagonals), D = 4 with them (fill-in).  Two-row maps are scanned
+// across the lanes; four-row maps are chained in order by one lane per half.
+template <int S, bool PIV>
+__device__ __forceinline__ void back_sweep(double2 (&b)[S], const double (&fd)[S], const double2 (&fu)[S],
+                                           const double2 (&fx)[S], int s, int n, SegLds &sc) {
+  constexpr int D = PIV ? 4 : 2;
+  const double2 z2 = make_double2(0.0, 0.0);
+  const int j0 = s * S;
+  double2 x[4] = {z2, z2, z2, z2};
+  double t[D][D];
+#pragma unroll
+  for (int r = 0; r < D; ++r)
+#pragma unroll
+    for (int c = 0; c < D; ++c) t[r][c] = (r == c) ? 1.0 : 0.0;
+#pragma unroll
+  for (int u = S - 1; u >= 0; --u) {
+    const int i = j0 + u;
+    if (i < n) {
+      double2 acc = b[u];
+      if (PIV) {
+        axpy2(acc, fx[u].y, x[3]);
+        axpy2(acc, fx[u].x, x[2]);
+      }
+      axpy2(acc, fu[u].y, x[1]);
+      axpy2(acc, fu[u].x, x[0]);
+      const double d = fd[u];
+      const double2 xi = make_double2(acc.x / d, acc.y / d);
+      x[3] = x[2];
+      x[2] = x[1];
+      x[1] = x[0];
+      x[0] = xi;
+      const double rd = 1.0 / d;
+#pragma unroll
+      for (int c = 0; c < D; ++c) {
+        double a = -fu[u].x * t[0][c] - fu[u].y * t[1][c];
+        if constexpr (D == 4) a -= fx[u].x * t[2][c] + fx[u].y * t[3][c];
+#pragma unroll
+        for (int r = D - 1; r > 0; --r) t[r][c] = t[r - 1][c];
+        t[0][c] = a * rd;
+      }
+    }
+  }
+  if constexpr (D == 2) {
+    double2 p[2] = {x[0], x[1]};
+    scan2<false>(t, p, s);
+    const double2 q0 = shdn2(p[0], 1), q1 = shdn2(p[1], 1);
+    const bool last = (s == NSEG - 1);
+    x[0] = last ? z2 : q0;
+    x[1] = last ? z2 : q1;
+  } else {
+    // chain over the segments in descending order
+    double *ts = sc.t[threadIdx.x];
+    double2 *ps = sc.p[threadIdx.x];
+#pragma unroll
+    for (int r = 0; r < 4; ++r)
+#pragma unroll
+      for (int c = 0; c < 4; ++c) ts[4 * r + c] = t[r][c];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) ps[r] = x[r];
+    __syncthreads();
+    if (s == 0) {
+      const int h0 = threadIdx.x;
+      double2 a[4] = {z2, z2, z2, z2};
+#pragma unroll 4
+      for (int q = NSEG - 1; q >= 0; --q) {
+        const double *tq = sc.t[h0 + q];
+        const double2 *pq = sc.p[h0 + q];
+        double2 c[4];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          sc.in[h0 + q][r] = a[r];
+          c[r] = pq[r];
+#pragma unroll
+          for (int k = 0; k < 4; ++k) fma2(c[r], tq[4 * r + k], a[k]);
+        }
+#pragma unroll
+        for (int r = 0; r < 4; ++r) a[r] = c[r];
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < 4; ++r) x[r] = sc.in[threadIdx.x][r];
+    __syncthreads();
+  }
+#pragma unroll
+  for (int u = S - 1; u >= 0; --u) {
+    const int i = j0 + u;
+    if (i < n) {
+      double2 acc = b[u];
+      if (PIV) {
+        axpy2(acc, fx[u].y, x[3]);
+        axpy2(acc, fx[u].x, x[2]);
+      }
+      axpy2(acc, fu[u].y, x[1]);
+      axpy2(acc, fu[u].x, x[0]);
+      const double d = fd[u];
+      const double2 xi = make_double2(acc.x / d, acc.y / d);
+      b[u] = xi;
+      x[3] = x[2];
+      x[2] = x[1];
+      x[1] = x[0];
+      x[0] = xi;
+    }
+  }
+}
+
+template <int S, bool PIV>
+__device__ __forceinline__ void solve_rows(double2 (&b)[S], const FacPtrs &F, const uint64_t *__restrict__ pw,
+                                           size_t sys, int s, int n, SegLds &sc) {
+  {
+    double2 fl[S];
+#pragma unroll
+    for (int u = 0; u < S; ++u) fl[u] = F.fl[(sys * S + u) * NSEG + s];
+    const uint64_t w = PIV ? pw[sys * NSEG + s] : 0;
+    fwd_sweep<S, PIV>(b, fl, w, s, n);
+  }
+  double2 fu[S], fx[S];
+  double fd[S];
+#pragma unroll
+  for (int u = 0; u < S; ++u) {
+    const size_t o = (sys * S + u) * NSEG + s;
+    fu[u] = F.fu[o];
+    fd[u] = F.fd[o];
+    fx[u] = PIV ? F.fx[o] : make_double2(0.0, 0.0);
+  }
+  back_sweep<S, PIV>(b, fd, fu, fx, s, n, sc);
+}
+
+// one wavefront per (m, |kappa|): lanes 0-31 solve slice k = kk, lanes 32-63
+// slice nz - kk (same factors); lane s of a half owns rows s S .. s S + S - 1.
+// Systems with and without row interchanges are solved by separate
+// instantiations (PIV), each returning at once for the other kind.
+template <int S, bool PIV>
+__global__ void __launch_bounds__(64) k_lu_solve(double2 *e, FacPtrs F, const uint64_t *__restrict__ pw,
                                                  const int *__restrict__ anyp, int nrdim, int npdim, int nz, int nrc,
                                                  int npc, int nzc, int nzcu, int skip00) {
-  const int g = blockIdx.x, l = threadIdx.x;
-  int k, kk;
-  pos_to_k(blockIdx.y, nz, k, kk);
-  const int m = 64 * g + l;
-  const int G = (npc + 63) / 64;
-  const int NC = (nrc + CH - 1) / CH;
-  if (k >= nz || m >= npc) return;
-  int cnt = z_count(k, nzc, nzcu);
-  const int n = nrc - m > 0 ? nrc - m : 0;
+  const int m = blockIdx.x, kk = blockIdx.y;
+  const int lane = threadIdx.x, s = lane % NSEG;
+  const bool twin = lane >= NSEG;
+  if (m >= npc) return;
+  const int n = nrc - m;
+  if (n <= 0) return;
+  const size_t sys = (size_t)kk * npc + m;
+  if ((anyp[sys] != 0) != PIV) return;
+  const int k = twin ? nz - kk : kk;
+  const bool valid = !twin || (kk != 0 && 2 * kk != nz);
+  int cnt = valid ? z_count(k, nzc, nzcu) : 0;
   if (skip00 && m == 0 && k == 0) cnt = 0;
-  if (cnt == 0 || n <= 0) return;
-  const size_t r0 = ((size_t)kk * G + g) * (size_t)nrc * 64 + l;
-  const uint16_t *P = pvw + ((size_t)kk * G + g) * NC * 64 + l;
-  double2 *b = e + (size_t)k * nrdim * npdim + (size_t)nrdim * m;
-  // An exactly zero right-hand side has the zero solution.  De-aliased fields
-  // are zero on a third of the kappa slices and above m = np/3, so a wave whose
-  // columns are all zero returns here; a non-zero column usually stops the
-  // scan at its first rows.
+  double2 *col = e + (size_t)(valid ? k : kk) * nrdim * npdim + (size_t)nrdim * m;
+  const int j0 = s * S;
+  double2 b[S];
   bool nonzero = false;
-  for (int i0 = 0; i0 < n && !nonzero; i0 += CH) {
 #pragma unroll
-    for (int u = 0; u < CH; ++u) {
-      const int i = i0 + u;
-      if (i < n) {
-        const double2 v = b[i];
-        nonzero |= (v.x != 0.0) || (v.y != 0.0);
-      }
-    }
+  for (int u = 0; u < S; ++u) {
+    const int j = j0 + u;
+    b[u] = (cnt > 0 && j < n) ? col[j] : make_double2(0.0, 0.0);
+    nonzero |= (b[u].x != 0.0) || (b[u].y != 0.0);
   }
+  // an exactly zero right-hand side has the zero solution: de-aliased fields
+  // are zero on a third of the kappa slices and above m = np/3
   if (!__any(nonzero)) return;
-  const bool piv = anyp[(size_t)kk * G + g] != 0;
-  for (int rep = 0; rep < cnt; ++rep) {
-    if (piv) solve_column<true>(b, Fp.ff + r0, Fp.fb + r0, Fp.fx + r0, P, n);
-    else solve_column<false>(b, Fp.ff + r0, Fp.fb + r0, Fp.fx + r0, P, n);
+  __shared__ SegLds sc;
+  // the duplicated Nyquist slice (z_count 2) is solved twice, as by the
+  // reference's loop over both chopped ranges
+  for (int rep = 0; rep < 2; ++rep) {
+    if (!__any(rep < cnt)) break;
+    solve_rows<S, PIV>(b, F, pw, sys, s, n, sc);
+    if (rep < cnt) {
+#pragma unroll
+      for (int u = 0; u < S; ++u)
+        if (j0 + u < n) col[j0 + u] = b[u];
+    }
   }
 }
 
-// factor rows of all systems: (|kappa| values) x (64-system groups) x rows x 64 lanes
-size_t fac_rows(const KitDev &k, int nroff) {
+// rows per segment: the smallest instantiated S with 32 S >= nrc
+int seg_rows(int nrc) {
+  static const int opts[] = {4, 8, 9, 12, 16, 17};
+  for (int S : opts)
+    if (NSEG * S >= nrc) return S;
+  return 0;
+}
+
+// factor rows of all systems: (|kappa| values) x npchop systems x S x 32
+size_t fac_rows(const KitDev &k, int S) {
   const int nkk = (k.nzdim > 1) ? k.nzdim / 2 + 1 : 1;
-  const int G = (k.npchop + 63) / 64;
-  return (size_t)nkk * G * (k.nrchop + nroff) * 64 + 64;
+  return (size_t)nkk * k.npchop * S * NSEG;
 }
 
 LuCache &lu_get(mlx_ctx *c, double alpha, int kind, int nroff) {
@@ -329,6 +530,8 @@ LuCache &lu_get(mlx_ctx *c, double alpha, int kind, int nroff) {
   const KitDev &k = c->kit;
   const int nrc = k.nrchop + nroff;
   if (nrc > k.nrdim) MLX_THROW(MLX_E_ARG, "chop_index: chopping in r too large");
+  const int S = seg_rows(nrc);
+  if (S == 0) MLX_THROW(MLX_E_ARG, "ihelm: more than 544 radial rows per system are not supported");
   if (c->lu.size() >= 4) {  // evict the least recently used entry
     size_t victim = 0;
     for (size_t i = 1; i < c->lu.size(); ++i)
@@ -343,11 +546,10 @@ LuCache &lu_get(mlx_ctx *c, double alpha, int kind, int nroff) {
   e.alpha = alpha;
   e.kind = kind;
   e.nroff = nroff;
+  e.seg = S;
   const int nkk = (k.nzdim > 1) ? k.nzdim / 2 + 1 : 1;
-  const int G = (k.npchop + 63) / 64;
-  const int NC = (nrc + CH - 1) / CH;
-  size_t nsys = (size_t)k.npchop * nkk;
-  const size_t nrows = fac_rows(k, nroff);
+  const size_t nsys = (size_t)k.npchop * nkk;
+  const size_t nrows = fac_rows(k, S);
   double *fac = nullptr;
   int8_t *piv = nullptr;
   auto cleanup = [&]() {
@@ -359,25 +561,27 @@ LuCache &lu_get(mlx_ctx *c, double alpha, int kind, int nroff) {
   };
   if (hipMalloc(&fac, sizeof(double) * nsys * k.nrdim * LDAB) != hipSuccess ||
       hipMalloc(&piv, nsys * k.nrdim) != hipSuccess ||
-      hipMalloc(&e.sol, sizeof(double) * 8 * nrows) != hipSuccess ||
-      hipMalloc(&e.pvw, sizeof(uint16_t) * ((size_t)nkk * G * NC * 64 + 1)) != hipSuccess ||
-      hipMalloc(&e.anyp, sizeof(int) * ((size_t)nkk * G + 1)) != hipSuccess) {
+      hipMalloc(&e.sol, sizeof(double) * 7 * nrows) != hipSuccess ||
+      hipMalloc(&e.pvw, sizeof(uint64_t) * nsys * NSEG) != hipSuccess ||
+      hipMalloc(&e.anyp, sizeof(int) * nsys) != hipSuccess) {
     cleanup();
     MLX_THROW(MLX_E_ALLOC, "ihelm: cannot allocate LU factors");
   }
-  HIPCHK(hipMemsetAsync(e.anyp, 0, sizeof(int) * ((size_t)nkk * G + 1), c->stream));
-  HIPCHK(hipMemsetAsync(e.pvw, 0, sizeof(uint16_t) * ((size_t)nkk * G * NC * 64 + 1), c->stream));
+  // systems the factor kernel skips ((0,0) of kind 2) keep zero words
+  HIPCHK(hipMemsetAsync(e.anyp, 0, sizeof(int) * nsys, c->stream));
+  HIPCHK(hipMemsetAsync(e.pvw, 0, sizeof(uint64_t) * nsys * NSEG, c->stream));
   int *dinfo = reinterpret_cast<int *>(c->dsmall + 4000);
   HIPCHK(hipMemsetAsync(dinfo, 0, sizeof(int), c->stream));
   dim3 grid((nkk + 63) / 64, k.npchop);
   hipLaunchKernelGGL(k_lu_factor, grid, dim3(64), 0, c->stream, fac, piv, k.d2c, k.lognorm, k.ak, k.nrdim, nkk, k.ne,
                      nrc, k.npchop, alpha, kind, k.ell, dinfo);
   hipLaunchKernelGGL(k_lu_repack, grid, dim3(64), 0, c->stream, fac, piv, fac_ptrs(e.sol, nrows), e.pvw, e.anyp,
-                     k.nrdim, nkk, nrc,
-                     k.npchop, kind);
+                     k.nrdim, nkk, nrc, k.npchop, kind, S);
   HIPCHK(hipGetLastError());
   int info = 0;
+  std::vector<int> anyp(nsys);
   HIPCHK(hipMemcpyAsync(&info, dinfo, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipMemcpyAsync(anyp.data(), e.anyp, sizeof(int) * nsys, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
   (void)hipFree(fac);
   (void)hipFree(piv);
@@ -387,6 +591,18 @@ LuCache &lu_get(mlx_ctx *c, double alpha, int kind, int nroff) {
     cleanup();
     MLX_THROW(MLX_E_ARG, "lurc: lu factorization resulted in failure");
   }
+  // factor bytes one solve must read: 40 per row, 16 more per row and the
+  // interchange words for systems with interchanges
+  e.fac_bytes = 0.0;
+  e.npiv = 0;
+  for (int kk = 0; kk < nkk; ++kk)
+    for (int m = 0; m < k.npchop; ++m) {
+      const int n = nrc - m > 0 ? nrc - m : 0;
+      const bool p = anyp[(size_t)kk * k.npchop + m] != 0;
+      e.fac_bytes += (double)n * (p ? 56.0 : 40.0) + (p ? 8.0 * NSEG : 0.0);
+      e.npiv += p ? 1 : 0;
+    }
+  e.nsys = (int64_t)nsys;
   e.stamp = ++c->lu_clock;
   c->lu.push_back(e);
   return c->lu.back();
@@ -399,13 +615,32 @@ namespace mlx {
 void k_helm_solve(mlx_ctx *c, double2 *e, double alpha, int kind, int nroff) {
   const KitDev &k = c->kit;
   LuCache &lu = lu_get(c, alpha, kind, nroff);
-  dim3 grid((k.npchop + 63) / 64, k.nzdim);
+  const int nkk = (k.nzdim > 1) ? k.nzdim / 2 + 1 : 1;
+  dim3 grid(k.npchop, nkk);
   // right-hand sides read and written once, factors of every |kappa| read once
   const double S = active_pairs(k, nroff);
-  const int nkk = (k.nzdim > 1) ? k.nzdim / 2 + 1 : 1;
-  ProbeScope probe(c, MLX_PROBE_LU_SOLVE, 32.0 * S * k.nzdim + 48.0 * S * nkk, 2.0 * 4.0 * 5.0 * S * k.nzdim);
-  hipLaunchKernelGGL(k_lu_solve, grid, dim3(64), 0, c->stream, e, fac_ptrs(lu.sol, fac_rows(k, nroff)), lu.pvw, lu.anyp, k.nrdim, k.npdim, k.nzdim,
-                     k.nrchop + nroff, k.npchop, k.chopzl, k.chopzu, kind == 2 ? 1 : 0);
+  ProbeScope probe(c, MLX_PROBE_LU_SOLVE, 32.0 * S * k.nzdim + lu.fac_bytes, 2.0 * 4.0 * 5.0 * S * k.nzdim);
+  const FacPtrs F = fac_ptrs(lu.sol, fac_rows(k, lu.seg));
+  const int nrc = k.nrchop + nroff, sk = kind == 2 ? 1 : 0;
+#define MLX_LU_LAUNCH(SV, PV)                                                                              \
+  hipLaunchKernelGGL((k_lu_solve<SV, PV>), grid, dim3(64), 0, c->stream, e, F, lu.pvw, lu.anyp, k.nrdim, k.npdim, \
+                     k.nzdim, nrc, k.npchop, k.chopzl, k.chopzu, sk)
+#define MLX_LU_BOTH(SV)                               \
+  do {                                                \
+    if (lu.npiv < lu.nsys) MLX_LU_LAUNCH(SV, false); \
+    if (lu.npiv > 0) MLX_LU_LAUNCH(SV, true);         \
+  } while (0)
+  switch (lu.seg) {
+    case 4: MLX_LU_BOTH(4); break;
+    case 8: MLX_LU_BOTH(8); break;
+    case 9: MLX_LU_BOTH(9); break;
+    case 12: MLX_LU_BOTH(12); break;
+    case 16: MLX_LU_BOTH(16); break;
+    case 17: MLX_LU_BOTH(17); break;
+    default: MLX_THROW(MLX_E_ARG, "ihelm: bad segment size");
+  }
+#undef MLX_LU_BOTH
+#undef MLX_LU_LAUNCH
   HIPCHK(hipGetLastError());
 }
 
