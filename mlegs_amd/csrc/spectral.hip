@@ -308,59 +308,71 @@ __global__ void k_idel2_post_(const double2 *e, double *ln, const double *lognor
 
 // idel2 with a prescribed log coefficient, (m,k)=(0,0) system only
 // (mlegs_scalar_ops.f90:603-625): rows shifted down, first row pins the
-// log coefficient; solved with partial pivoting on a dense copy.
+// log coefficient; banded LU with partial pivoting on one thread.
 __global__ void k_idel2_preln00_(double2 *e, double *ln, const double *d2c, const double *lognorm, int ne, int n,
                                  double preln, double ell, double *work) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
-  // work: dense n x n (column-major) + 2n rhs
-  double *A = work;
-  double *br = work + (size_t)n * n;
+  // The system: row 0 prescribes the log coefficient, rows 1.. are the
+  // Laplacian's rows 0..n-2 (bandwidths 2, 2) shifted down one, and column 0
+  // carries the log term's Laplacian: lower bandwidth 3, upper 1.  Banded LU
+  // with partial pivoting (kl = 3, ku = 2 as the reference's gbsv call), band
+  // storage with fill-in rows: element (i, j) at ab[KV + i - j + LD j].
+  constexpr int KLP = 3, KUP = 2, KVP = KLP + KUP, LD = 2 * KLP + KUP + 1;
+  double *ab = work;
+  double *br = work + (size_t)LD * n;
   double *bi = br + n;
   double ell2 = ell * ell;
-  for (int j = 0; j < n; ++j)
-    for (int i = 0; i < n; ++i) A[i + (size_t)n * j] = 0.0;
+  for (size_t t = 0; t < (size_t)LD * n; ++t) ab[t] = 0.0;
+#define AB(i, j) ab[(KVP + (i) - (j)) + (size_t)LD * (j)]
   for (int i = 0; i < n - 1; ++i)
     for (int d = 0; d < 5; ++d) {
       int j = i + d - 2;
       if (j < 0 || j >= n) continue;
-      A[(i + 1) + (size_t)n * j] = d2c[((size_t)0 * ne + i) * 5 + d];
+      AB(i + 1, j) = d2c[((size_t)0 * ne + i) * 5 + d];
     }
-  A[0] = 1.0;
-  if (n > 1) A[1] += 4.0 / 3.0 / ell2;
-  if (n > 2) A[2] -= 2.0 / 1.0 / ell2 * exp(lognorm[0] - lognorm[1]);
-  if (n > 3) A[3] += 2.0 / 3.0 / ell2 * exp(lognorm[0] - lognorm[2]);
+  AB(0, 0) = 1.0;
+  if (n > 1) AB(1, 0) += 4.0 / 3.0 / ell2;
+  if (n > 2) AB(2, 0) -= 2.0 / 1.0 / ell2 * exp(lognorm[0] - lognorm[1]);
+  if (n > 3) AB(3, 0) += 2.0 / 3.0 / ell2 * exp(lognorm[0] - lognorm[2]);
   for (int i = n - 1; i >= 1; --i) { br[i] = e[i - 1].x; bi[i] = e[i - 1].y; }
   br[0] = preln / exp(lognorm[0]);
   bi[0] = 0.0;
-  // Gaussian elimination with partial pivoting
+  // factor and forward-eliminate the right-hand side together
+  int ju = 0;
   for (int j = 0; j < n; ++j) {
-    int p = j;
-    double best = fabs(A[j + (size_t)n * j]);
-    for (int i = j + 1; i < n; ++i)
-      if (fabs(A[i + (size_t)n * j]) > best) { best = fabs(A[i + (size_t)n * j]); p = i; }
-    if (p != j) {
-      for (int c = 0; c < n; ++c) { double t = A[j + (size_t)n * c]; A[j + (size_t)n * c] = A[p + (size_t)n * c]; A[p + (size_t)n * c] = t; }
-      double t = br[j]; br[j] = br[p]; br[p] = t;
-      t = bi[j]; bi[j] = bi[p]; bi[p] = t;
+    const int km = (KLP < n - 1 - j) ? KLP : n - 1 - j;
+    int p = 0;
+    double best = fabs(AB(j, j));
+    for (int r = 1; r <= km; ++r)
+      if (fabs(AB(j + r, j)) > best) { best = fabs(AB(j + r, j)); p = r; }
+    int jn = j + KUP + p;
+    if (jn > n - 1) jn = n - 1;
+    if (jn > ju) ju = jn;
+    if (p != 0) {
+      for (int c = j; c <= ju; ++c) { double t = AB(j, c); AB(j, c) = AB(j + p, c); AB(j + p, c) = t; }
+      double t = br[j]; br[j] = br[j + p]; br[j + p] = t;
+      t = bi[j]; bi[j] = bi[j + p]; bi[j + p] = t;
     }
-    double rp = 1.0 / A[j + (size_t)n * j];
-    for (int i = j + 1; i < n; ++i) {
-      double l = A[i + (size_t)n * j] * rp;
+    const double rp = 1.0 / AB(j, j);
+    for (int r = 1; r <= km; ++r) {
+      const double l = AB(j + r, j) * rp;
       if (l == 0.0) continue;
-      for (int c = j + 1; c < n; ++c) A[i + (size_t)n * c] -= l * A[j + (size_t)n * c];
-      br[i] -= l * br[j];
-      bi[i] -= l * bi[j];
+      for (int c = j + 1; c <= ju; ++c) AB(j + r, c) -= l * AB(j, c);
+      br[j + r] -= l * br[j];
+      bi[j + r] -= l * bi[j];
     }
   }
+  // back substitution, U has KL + KU superdiagonals
   for (int j = n - 1; j >= 0; --j) {
-    double d = A[j + (size_t)n * j];
+    double d = AB(j, j);
     br[j] /= d;
     bi[j] /= d;
-    for (int i = 0; i < j; ++i) {
-      br[i] -= br[j] * A[i + (size_t)n * j];
-      bi[i] -= bi[j] * A[i + (size_t)n * j];
+    for (int i = (j - KVP > 0 ? j - KVP : 0); i < j; ++i) {
+      br[i] -= br[j] * AB(i, j);
+      bi[i] -= bi[j] * AB(i, j);
     }
   }
+#undef AB
   for (int i = 0; i < n; ++i) e[i] = make_double2(br[i], bi[i]);
   ln[0] = br[0] * exp(lognorm[0]);
 }
@@ -856,10 +868,10 @@ void k_idel2_preln00(mlx_ctx *c, double2 *e, double *ln, double preln, int nroff
   const KitDev &k = c->kit;
   int n = nrcs_of(k, 0, nroff);
   if (n <= 0) return;
-  // dense n x n scratch: the second field-sized scratch buffer, which no
-  // caller of this function holds (n*n*8 bytes <= nrdim*npdim*nzdim*16)
+  // band storage (9 n) and the right-hand side (2 n) in the second
+  // field-sized scratch buffer, which no caller of this function holds
   double *work = reinterpret_cast<double *>(c->scr[1]);
-  if ((size_t)n * n + 2 * (size_t)n > 2 * c->nelem) MLX_THROW(MLX_E_ARG, "idel2: field too small for dense solve");
+  if (11 * (size_t)n > 2 * c->nelem) MLX_THROW(MLX_E_ARG, "idel2: field too small for the (0,0) solve");
   hipLaunchKernelGGL(k_idel2_preln00_, dim3(1), dim3(64), 0, c->stream, e, ln, k.d2c, k.lognorm, k.ne, n, preln,
                      k.ell, work);
 }
