@@ -36,7 +36,7 @@ sys.path.insert(0, ROOT)
 METRIC = 'steps/s @256³ mixed-prec vs FP64 at ≤1e-6 rel.err; adjoint/forward time ratio'
 PEAK_TFLOPS = {'fp64': 78.6, 'fp32': 157.3}
 HBM_GBPS = 8000.0
-HBM_BOUND = ('fft_phi', 'fft_z', 'lu_solve')
+HBM_BOUND = ('fft_phi', 'fft_z', 'lu_solve', 'phi_cross')
 TRAFFIC_FILE = os.path.join(ROOT, 'profiles', 'traffic_256.json')
 
 
@@ -208,7 +208,7 @@ def kernel_sample(tfm, p, prec, iters):
     e = inputs.transform_field(tfm.dims, (p.nrchop, p.npchop, p.nzchop, p.np, p.nz), field_id=3)
     out = {}
     for w in ('copy', 'fft_phi_fwd', 'fft_phi_bwd', 'fft_z_fwd', 'fft_z_bwd', 'leg_fwd', 'leg_bwd', 'v2t',
-              'lu_solve'):
+              'lu_solve', 'phi_cross'):
         g = M.Scalar(tfm).upload(e, space='FFF')
         out[w] = round(1e3 * tfm.time_kernel(g, w, iters), 2)
         log(w, out[w], 'us')

@@ -58,7 +58,8 @@ extern "C" {
 #define MLX_PROBE_LEG_BWD   3  /* backward Legendre contraction            */
 #define MLX_PROBE_V2T       4  /* vec2tp projection (5 contractions)       */
 #define MLX_PROBE_LU_SOLVE  5  /* banded LU solve of ihelm / idel2         */
-#define MLX_PROBE_COUNT     6
+#define MLX_PROBE_PHI_CROSS 6  /* v x w formed inside three phi transforms */
+#define MLX_PROBE_COUNT     7
 
 typedef struct mlx_ctx mlx_ctx;
 typedef struct mlx_field mlx_field;
@@ -236,8 +237,10 @@ int mlx_energy(mlx_ctx* ctx, const mlx_field* vr, const mlx_field* vp, const mlx
  * events (after one warm-up launch) and return the mean duration in ms.
  * which: "leg_fwd", "leg_bwd", "fft_phi_fwd", "fft_phi_bwd", "fft_z_fwd",
  * "fft_z_bwd", "v2t" (the vec2tp projection with the field as all three
- * components), "lu_solve" (the ihelm solve of the AB2/CN step), "copy" (a
- * plain copy of the field, 32 bytes per complex entry of HBM traffic).  The
+ * components), "lu_solve" (the ihelm solve of the AB2/CN step), "phi_cross"
+ * (v x w formed inside the three forward phi transforms, the field as all
+ * seven physical inputs), "copy" (a plain copy of the field, 32 bytes per
+ * complex entry of HBM traffic).  The
  * field is used as operand and left in an unspecified state of the same
  * space. */
 int mlx_time_kernel(mlx_ctx* ctx, mlx_field* f, const char* which, int32_t iters, double* ms);

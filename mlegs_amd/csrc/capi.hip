@@ -280,18 +280,10 @@ void phi_forward_from(mlx_ctx *c, mlx_field *dst, const mlx_field *src, int rsca
 // idel2 and chop act on each (m, kappa) column separately and zeroat1 leaves a
 // zero slice at zero, so de-aliasing right after the projection gives the same
 // result and lets the banded solve skip the de-aliased (all-zero) systems.
-void vec2tp(mlx_ctx *c, const mlx_field *vr, const mlx_field *vp, const mlx_field *vz, mlx_field *psi,
-            mlx_field *chi, bool pre_dealias = false) {
-  need_kit(c);
-  for (const mlx_field *f : {vr, vp, vz})
-    if (!is_ppp(f)) MLX_THROW(MLX_E_LAYOUT, "vector_projection: vr, vp, vz must be in PPP");
+// the projection from ur = r vr, up = r vp, uz = vz already in PFP
+void vec2tp_pfp(mlx_ctx *c, mlx_field *ur, mlx_field *up, mlx_field *uz, mlx_field *psi, mlx_field *chi,
+                bool pre_dealias) {
   const KitDev &k = c->kit;
-  mlx_field *ur = tmp(c, 0), *up = tmp(c, 1), *uz = tmp(c, 2);
-  // ur = r vr, up = r vp, uz = vz, each taken to PFP by an out-of-place phi
-  // transform that applies the r weight on its input
-  phi_forward_from(c, ur, vr, 1);
-  phi_forward_from(c, up, vp, 1);
-  phi_forward_from(c, uz, vz, 0);
   k_set_scalar(c, psi->sc, 0.0);
   k_set_scalar(c, chi->sc, 0.0);
   // the projection reads columns m < npchop only
@@ -319,6 +311,20 @@ void vec2tp(mlx_ctx *c, const mlx_field *vr, const mlx_field *vp, const mlx_fiel
   zeroat1(c, chi);
 }
 
+void vec2tp(mlx_ctx *c, const mlx_field *vr, const mlx_field *vp, const mlx_field *vz, mlx_field *psi,
+            mlx_field *chi, bool pre_dealias = false) {
+  need_kit(c);
+  for (const mlx_field *f : {vr, vp, vz})
+    if (!is_ppp(f)) MLX_THROW(MLX_E_LAYOUT, "vector_projection: vr, vp, vz must be in PPP");
+  mlx_field *ur = tmp(c, 0), *up = tmp(c, 1), *uz = tmp(c, 2);
+  // ur = r vr, up = r vp, uz = vz, each taken to PFP by an out-of-place phi
+  // transform that applies the r weight on its input
+  phi_forward_from(c, ur, vr, 1);
+  phi_forward_from(c, up, vp, 1);
+  phi_forward_from(c, uz, vz, 0);
+  vec2tp_pfp(c, ur, up, uz, psi, chi, pre_dealias);
+}
+
 // advection_rhs (vortical_flow_3d.f90:353-395).  dealiased: psi and chi are
 // already de-aliased (the stepper de-aliases them before svv_filter and
 // zeroat1, which keep a zero mode zero), so the opening dealias is a no-op.
@@ -333,12 +339,24 @@ void advection_rhs(mlx_ctx *c, mlx_field *psi, mlx_field *chi, mlx_field *nlpsi,
   mlx_field *vr = tmp(c, 4), *vp = tmp(c, 5), *vz = tmp(c, 6);
   mlx_field *wr = tmp(c, 7), *wp = tmp(c, 8), *wz = tmp(c, 9);
   tp2vec(c, psi, chi, vr, vp, vz);
-  k_axpby(c, vz->e, vz->sc, 1.0, uz->e, nullptr, 1.0);
   tp2curlvec(c, psi, chi, wr, wp, wz);
-  vecprod(c, vr, vp, vz, wr, wp, wz);
   // de-aliased inside vec2tp before idel2; chop and zeroat1 keep zero modes
   // zero, so the closing dealias of the reference is already satisfied
-  vec2tp(c, vr, vp, vz, nlpsi, nlchi, true);
+  mlx_field *ur = tmp(c, 0), *up = tmp(c, 1), *uzp = tmp(c, 2);
+  if (fft_phi_cross(c, ur->e, up->e, uzp->e, vr->e, vp->e, vz->e, wr->e, wp->e, wz->e, uz->e)) {
+    // vz + uz, v x w, and the phi transforms of r (v x w)_r, r (v x w)_phi,
+    // (v x w)_z in one pass: log terms zero, no offsets, as vecprod leaves them
+    for (mlx_field *f : {ur, up, uzp}) {
+      k_set_scalar(c, f->sc, 0.0);
+      f->off[0] = f->off[1] = f->off[2] = 0;
+      set_space(f, "PFP");
+    }
+    vec2tp_pfp(c, ur, up, uzp, nlpsi, nlchi, true);
+  } else {
+    k_axpby(c, vz->e, vz->sc, 1.0, uz->e, nullptr, 1.0);
+    vecprod(c, vr, vp, vz, wr, wp, wz);
+    vec2tp(c, vr, vp, vz, nlpsi, nlchi, true);
+  }
   k_set_scalar(c, nlpsi->sc, 0.0);
   k_set_scalar(c, nlchi->sc, 0.0);
 }
@@ -905,6 +923,10 @@ int mlx_time_kernel(mlx_ctx *c, mlx_field *f, const char *which, int32_t iters, 
       else if (w == "fft_z_bwd") fft_z(c, f->e, +1, active_pairs(c->kit, 0));
       else if (w == "v2t") vec2tp_project(c, f->e, f->e, f->e, c->scr[0], c->scr[1], 3);
       else if (w == "lu_solve") k_helm_solve(c, f->e, -2.0 / (c->p.dt * c->p.visc), 0, 0);
+      else if (w == "phi_cross") {
+        if (!fft_phi_cross(c, tmp(c, 0)->e, tmp(c, 1)->e, tmp(c, 2)->e, f->e, f->e, f->e, f->e, f->e, f->e, f->e))
+          MLX_THROW(MLX_E_ARG, "mlx_time_kernel: phi_cross is not built for this np");
+      }
       else if (w == "copy") k_copy_field(c, c->scr[0], f->e);
       else MLX_THROW(MLX_E_ARG, "mlx_time_kernel: unknown kernel " + w);
     };

@@ -451,6 +451,135 @@ __global__ void __launch_bounds__(kThreads) k_fft4(double2 *out, const double2 *
   }
 }
 
+// Forward phi transform of the three components of v x w, straight from the
+// six physical (PPP, packed real) fields: vector_product
+// (mlegs_scalar_ops.f90:1561-1566, here with the background axial flow ub
+// added to v_z first, vortical_flow_3d.f90:381) followed by the first step
+// of vector_projection (:1320-1336: r v_r, r v_phi and v_z to PFP).  Stage 1
+// of the four-step transform forms the products while loading, so each
+// input is read once and no physical product field is written.  Rows
+// i >= nr give zero products, as vector_product leaves them.
+template <int N1, int N2, int TI>
+__global__ void __launch_bounds__(kThreads) k_fft4_cross(double2 *o0, double2 *o1, double2 *o2, const double2 *v0,
+                                                         const double2 *v1, const double2 *v2, const double2 *w0,
+                                                         const double2 *w1, const double2 *w2, const double2 *ub,
+                                                         size_t es, size_t gs, int nrow, const double2 *tw,
+                                                         const double2 *twh, double scale, const double *r, int nr) {
+  constexpr int N = N1 * N2;
+  constexpr int T1 = (TI * N2 + kThreads - 1) / kThreads;
+  constexpr int T2 = (TI * N1 + kThreads - 1) / kThreads;
+  const double sg = -1.0;
+  __shared__ double2 buf[TI * N];
+  const int i0 = blockIdx.x * TI;
+  const size_t base = (size_t)blockIdx.y * gs;
+  if (i0 >= nrow) return;
+  const int tid = threadIdx.x;
+
+  // stage 1, all three components
+  double2 y1[3][T1][N1];
+#pragma unroll
+  for (int t = 0; t < T1; ++t) {
+    const int b = tid + t * kThreads;
+    if (b < TI * N2) {
+      const int l = b % TI, n2 = b / TI, i = i0 + l;
+      double2 c0[N1], c1[N1], c2[N1];
+      const double rr = (i < nr) ? r[i] : 0.0;
+#pragma unroll
+      for (int n1 = 0; n1 < N1; ++n1) {
+        const int j = n2 + N2 * n1;
+        const double2 z2 = make_double2(0.0, 0.0);
+        double2 a1 = z2, a2 = z2, a3 = z2, b1 = z2, b2 = z2, b3 = z2;
+        if (i < nr) {
+          const size_t o = base + (size_t)j * es + i;
+          a1 = v0[o];
+          a2 = v1[o];
+          a3 = v2[o];
+          b1 = w0[o];
+          b2 = w1[o];
+          b3 = w2[o];
+          if (ub) {
+            const double2 u = ub[o];
+            a3 = make_double2(a3.x + u.x, a3.y + u.y);
+          }
+        }
+        const double2 pr = make_double2(a2.x * b3.x - a3.x * b2.x, a2.y * b3.y - a3.y * b2.y);
+        const double2 pp = make_double2(a3.x * b1.x - a1.x * b3.x, a3.y * b1.y - a1.y * b3.y);
+        c2[n1] = make_double2(a1.x * b2.x - a2.x * b1.x, a1.y * b2.y - a2.y * b1.y);
+        c0[n1] = make_double2(pr.x * rr, pr.y * rr);
+        c1[n1] = make_double2(pp.x * rr, pp.y * rr);
+      }
+      dft_reg<N1>(c0, sg);
+      dft_reg<N1>(c1, sg);
+      dft_reg<N1>(c2, sg);
+#pragma unroll
+      for (int k1 = 1; k1 < N1; ++k1) {
+        const int q = (n2 * k1) & (N - 1);
+        if (q) {
+          const double2 w = twid(tw, q, sg);
+          c0[k1] = cmul(c0[k1], w);
+          c1[k1] = cmul(c1[k1], w);
+          c2[k1] = cmul(c2[k1], w);
+        }
+      }
+#pragma unroll
+      for (int k1 = 0; k1 < N1; ++k1) {
+        y1[0][t][k1] = c0[k1];
+        y1[1][t][k1] = c1[k1];
+        y1[2][t][k1] = c2[k1];
+      }
+    }
+  }
+#pragma unroll
+  for (int comp = 0; comp < 3; ++comp) {
+    double2 *out = comp == 0 ? o0 : comp == 1 ? o1 : o2;
+    if (comp > 0) __syncthreads();  // the previous component's untangling has read buf
+#pragma unroll
+    for (int t = 0; t < T1; ++t) {
+      const int b = tid + t * kThreads;
+      if (b < TI * N2) {
+        const int l = b % TI, n2 = b / TI;
+#pragma unroll
+        for (int k1 = 0; k1 < N1; ++k1) buf[(k1 * N2 + n2) * TI + l] = y1[comp][t][k1];
+      }
+    }
+    __syncthreads();
+    double2 y2[T2][N2];
+#pragma unroll
+    for (int t = 0; t < T2; ++t) {
+      const int b = tid + t * kThreads;
+      if (b < TI * N1) {
+        const int l = b % TI, k1 = b / TI;
+#pragma unroll
+        for (int n2 = 0; n2 < N2; ++n2) y2[t][n2] = buf[(k1 * N2 + n2) * TI + l];
+        dft_reg<N2>(y2[t], sg);
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int t = 0; t < T2; ++t) {
+      const int b = tid + t * kThreads;
+      if (b < TI * N1) {
+        const int l = b % TI, k1 = b / TI;
+#pragma unroll
+        for (int k2 = 0; k2 < N2; ++k2) buf[(k1 + N1 * k2) * TI + l] = y2[t][k2];
+      }
+    }
+    __syncthreads();
+    // C_m = E + W^m O, m = 0..N, scaled 1/np
+    for (int b = tid; b < TI * (N + 1); b += kThreads) {
+      const int l = b % TI, m = b / TI, i = i0 + l;
+      const double2 zm = buf[(m % N) * TI + l];
+      double2 zc = buf[((N - m) % N) * TI + l];
+      zc.y = -zc.y;
+      const double2 E = make_double2(0.5 * (zm.x + zc.x), 0.5 * (zm.y + zc.y));
+      const double2 D = make_double2(0.5 * (zm.x - zc.x), 0.5 * (zm.y - zc.y));
+      const double2 O = make_double2(D.y, -D.x);  // -i * D
+      const double2 C = cadd(E, cmul(twh[m], O));
+      if (i < nrow) out[base + (size_t)m * es + i] = make_double2(C.x * scale, C.y * scale);
+    }
+  }
+}
+
 template <int MODE>
 bool launch_fft4(mlx_ctx *c, int N, dim3 grid_y, double2 *out, const double2 *in, size_t es, size_t gs, int nrow,
                  int act_nrc, const double2 *tw, const double2 *twh, double sg, double scale, int rscale) {
@@ -508,6 +637,34 @@ void fft_phi(mlx_ctx *c, double2 *out, const double2 *in, int dir, int rscale) {
   hipLaunchKernelGGL(k_fft_phi, grid, dim3(kThreads), 0, c->stream, out, in, k.nrdim, k.npdim, k.nzdim, TI,
                      k.plan_nph, k.tw_nph, k.tw_half, dir, k.r, k.nr, rscale);
   HIPCHK(hipGetLastError());
+}
+
+bool fft_phi_cross(mlx_ctx *c, double2 *o0, double2 *o1, double2 *o2, const double2 *v0, const double2 *v1,
+                   const double2 *v2, const double2 *w0, const double2 *w1, const double2 *w2, const double2 *ub) {
+  const KitDev &k = c->kit;
+  const int N = k.nph;
+  if (k.np <= 1 || !(N == 16 || N == 32 || N == 64 || N == 128 || N == 256)) return false;
+  const size_t plane = (size_t)k.nrdim * k.npdim;
+  // seven physical fields read (rows < nr), three PFP fields written
+  ProbeScope probe(c, MLX_PROBE_PHI_CROSS, 16.0 * k.nr * k.nz * (7.0 * N + 3.0 * (N + 1)),
+                   3.0 * 2.5 * k.np * std::log2((double)k.np) * k.nr * k.nz + 18.0 * k.nr * k.np * k.nz);
+#define MLX_FFTX(N1_, N2_, TI_)                                                                                 \
+  do {                                                                                                          \
+    dim3 grid((k.nrdim + TI_ - 1) / TI_, k.nzdim);                                                              \
+    hipLaunchKernelGGL((k_fft4_cross<N1_, N2_, TI_>), grid, dim3(kThreads), 0, c->stream, o0, o1, o2, v0, v1, v2, \
+                       w0, w1, w2, ub, (size_t)k.nrdim, plane, k.nrdim, k.tw_nph, k.tw_half, 1.0 / k.np, k.r,   \
+                       k.nr);                                                                                   \
+  } while (0)
+  switch (N) {
+    case 16: MLX_FFTX(4, 4, 64); break;
+    case 32: MLX_FFTX(4, 8, 64); break;
+    case 64: MLX_FFTX(8, 8, 32); break;
+    case 128: MLX_FFTX(8, 16, 16); break;
+    default: MLX_FFTX(16, 16, 8); break;
+  }
+#undef MLX_FFTX
+  HIPCHK(hipGetLastError());
+  return true;
 }
 
 void fft_z(mlx_ctx *c, double2 *e, int dir, double ncol, int act_nrc, int ncols) {

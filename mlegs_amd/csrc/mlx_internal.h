@@ -172,6 +172,11 @@ void kit_free(KitDev &k);
 // dir -1 forward (PPP->PFP), +1 backward; out may equal in; rscale: +1 multiply
 // input rows i < nr by r_i (forward), -1 divide output rows by r_i (backward)
 void fft_phi(mlx_ctx *c, double2 *out, const double2 *in, int dir, int rscale = 0);
+// v x w (v_z + ub) to PFP with the r weights of vector_projection; false when
+// the fused form is not built for this np (the caller then runs vecprod and
+// three phi transforms)
+bool fft_phi_cross(mlx_ctx *c, double2 *o0, double2 *o1, double2 *o2, const double2 *v0, const double2 *v1,
+                   const double2 *v2, const double2 *w0, const double2 *w1, const double2 *w2, const double2 *ub);
 // dir -1 forward (/nz), +1 backward.  ncol: the (row, column) lines the caller
 // needs transformed (the launch's algorithmic work).  act_nrc > 0: only rows
 // i < act_nrc - j of column j need transforming; ncols > 0: only columns
