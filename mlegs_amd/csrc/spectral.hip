@@ -547,14 +547,15 @@ __global__ void __launch_bounds__(256) k_tp2vec_front_(double2 *ur, double2 *up,
                                                       double ell) {
   extern __shared__ double2 lds[];
   double2 *P = lds, *C = lds + nrdim, *A = lds + 2 * nrdim, *B = lds + 3 * nrdim;
-  // kappa fastest: consecutive workgroups share m and its band coefficients
-  const int k = blockIdx.x, m = blockIdx.y;
+  // m fastest: consecutive workgroups take adjacent columns of one slice
+  const int m = blockIdx.x, k = blockIdx.y;
   const size_t off = (size_t)k * nrdim * npdim + (size_t)nrdim * m;
   const int cnt = z_count(k, nzc, nzcu);
   const int nn0 = nrcs_dev(nrc, npc, m);
   const double2 zero = make_double2(0.0, 0.0);
   if (cnt == 0 || nn0 == 0) {
-    for (int i = threadIdx.x; i < nrdim; i += blockDim.x) ur[off + i] = up[off + i] = uz[off + i] = zero;
+    // a chopped kappa slice of an active column is input to the z transform
+    for (int i = threadIdx.x; i < nn0; i += blockDim.x) ur[off + i] = up[off + i] = uz[off + i] = zero;
     return;
   }
   int nn3 = nrcs_dev(nrc + 3, npc, m);
@@ -566,7 +567,22 @@ __global__ void __launch_bounds__(256) k_tp2vec_front_(double2 *ur, double2 *up,
   const double lnC = c_ln[0];
   if (cnt == 1) {
     // one application of each band product: every output row comes straight
-    // from the staged columns
+    // from the staged columns.  A thread's rows i = tid, tid + 256, ... (at
+    // most three): their band coefficients are loaded before the barrier so
+    // they arrive with the staged columns.
+    constexpr int RT = 3;
+    double cx[RT][2], cd[RT][5];
+#pragma unroll
+    for (int q = 0; q < RT; ++q) {
+      const int i = threadIdx.x + q * blockDim.x;
+      const bool in = i < nn0;
+      const double *bx = xxc + ((size_t)m * ne + (in ? i : 0)) * 2;
+      const double *bd = d2c + ((size_t)m * ne + (in ? i : 0)) * 5;
+      cx[q][0] = in ? bx[0] : 0.0;
+      cx[q][1] = in ? bx[1] : 0.0;
+#pragma unroll
+      for (int d = 0; d < 5; ++d) cd[q][d] = in ? bd[d] : 0.0;
+    }
     const double mv1 = (double)m, kv1 = ak[k];
     if (mode == 0) {
       lnP = p_ln[0];
@@ -598,28 +614,42 @@ __global__ void __launch_bounds__(256) k_tp2vec_front_(double2 *ur, double2 *up,
       }
       __syncthreads();
     }
-    for (int i = threadIdx.x; i < nrdim; i += blockDim.x) {
-      double2 vr = zero, vp = zero, vz = zero;
-      if (i < nn0) {
-        double2 xc = band_row(C, i, nn3, 2, m, ne, d2c, xxc, 0.0);
-        double2 xp = band_row(P, i, nn3, 2, m, ne, d2c, xxc, 0.0);
-        double2 hc = band_row(C, i, nn3, 1, m, ne, d2c, xxc, 0.0);
-        if (c00) {
-          if (i < 2) {
-            xc.x += 1.0 / exp(lognorm[i]) * lnC;
-            xp.x += 1.0 / exp(lognorm[i]) * lnP;
-          }
-          if (i == 0) hc.x += 4.0 / 3.0 / ell2 / exp(lognorm[0]) * lnC;
-          if (i == 1) hc.x -= 2.0 / 1.0 / ell2 / exp(lognorm[1]) * lnC;
-          if (i == 2) hc.x += 2.0 / 3.0 / ell2 / exp(lognorm[2]) * lnC;
-        }
-        vr = make_double2(-mv1 * P[i].y - kv1 * xc.y, mv1 * P[i].x + kv1 * xc.x);
-        vp = make_double2(-xp.x - mv1 * kv1 * C[i].x, -xp.y - mv1 * kv1 * C[i].y);
-        vz = make_double2(-hc.x, -hc.y);
+    // Rows i >= nn0 are not written: the transforms that consume these
+    // outputs (z transform restricted to the active lines, then the radial
+    // transform) read rows i < nrchop - m of columns m < npchop only.
+#pragma unroll
+    for (int q = 0; q < RT; ++q) {
+      const int i = threadIdx.x + q * blockDim.x;
+      if (i >= nn0) break;
+      // xxdx: coefficients of rows i - 1 and i + 1; del2h: rows i - 2 .. i + 2
+      double2 xc = make_double2(0.0, 0.0), xp = xc, hc = xc;
+      if (i >= 1) {
+        xc.x += cx[q][0] * C[i - 1].x; xc.y += cx[q][0] * C[i - 1].y;
+        xp.x += cx[q][0] * P[i - 1].x; xp.y += cx[q][0] * P[i - 1].y;
       }
-      ur[off + i] = vr;
-      up[off + i] = vp;
-      uz[off + i] = vz;
+      if (i + 1 < nn3) {
+        xc.x += cx[q][1] * C[i + 1].x; xc.y += cx[q][1] * C[i + 1].y;
+        xp.x += cx[q][1] * P[i + 1].x; xp.y += cx[q][1] * P[i + 1].y;
+      }
+#pragma unroll
+      for (int d = 0; d < 5; ++d) {
+        const int j = i + d - 2;
+        if (j < 0 || j >= nn3) continue;
+        hc.x += cd[q][d] * C[j].x;
+        hc.y += cd[q][d] * C[j].y;
+      }
+      if (c00) {
+        if (i < 2) {
+          xc.x += 1.0 / exp(lognorm[i]) * lnC;
+          xp.x += 1.0 / exp(lognorm[i]) * lnP;
+        }
+        if (i == 0) hc.x += 4.0 / 3.0 / ell2 / exp(lognorm[0]) * lnC;
+        if (i == 1) hc.x -= 2.0 / 1.0 / ell2 / exp(lognorm[1]) * lnC;
+        if (i == 2) hc.x += 2.0 / 3.0 / ell2 / exp(lognorm[2]) * lnC;
+      }
+      ur[off + i] = make_double2(-mv1 * P[i].y - kv1 * xc.y, mv1 * P[i].x + kv1 * xc.x);
+      up[off + i] = make_double2(-xp.x - mv1 * kv1 * C[i].x, -xp.y - mv1 * kv1 * C[i].y);
+      uz[off + i] = make_double2(-hc.x, -hc.y);
     }
     return;
   }
@@ -922,9 +952,10 @@ void k_tp2vec_front(mlx_ctx *c, double2 *ur, double2 *up, double2 *uz, const dou
                     const double *p_ln, const double *c_ln, int mode, int md_off) {
   const KitDev &k = c->kit;
   if (k.nrchop + 3 > k.nrdim || k.nrchop + 3 > k.ne) MLX_THROW(MLX_E_ARG, "chop_index: chopping in r too large");
+  if (k.nrchop > 768) MLX_THROW(MLX_E_ARG, "vector_reconstruction: nrchop above 768 is not supported");
   if (md_off < 0 || md_off > 3 || k.nrchop + md_off > k.nrdim)
     MLX_THROW(MLX_E_ARG, "curl_vector_reconstruction: chopping offset of chi out of range");
-  dim3 grid(k.nzdim, k.npdim);
+  dim3 grid(k.npdim, k.nzdim);
   size_t lds = 4 * sizeof(double2) * (size_t)k.nrdim;
   hipLaunchKernelGGL(k_tp2vec_front_, grid, dim3(256), lds, c->stream, ur, up, uz, p_in, c_in, p_ln, c_ln, mode,
                      md_off, k.d2c, k.xxc, k.lognorm, k.ak, k.nrdim, k.npdim, k.nzdim, k.ne, k.nrchop, k.npchop, k.chopzl,
