@@ -210,15 +210,22 @@ void abcn(mlx_ctx *c, mlx_field *s, mlx_field *s_p, mlx_field *nl, mlx_field *nl
 
 // Back half of tp2vec: the three FFF components (offset 0, no log term) to
 // PPP; ur and up are divided by r in the last phi transform.
+// The front kernel has zeroed the log terms, so trans's log-column step
+// (adding 0 * ln x) is left out; the other steps are trans's FFF -> PPP.
 void tp2vec_back(mlx_ctx *c, mlx_field *ur, mlx_field *up, mlx_field *uz) {
+  const KitDev &k = c->kit;
+  const int rs[3] = {-1, -1, 0};
+  int q = 0;
   for (mlx_field *f : {ur, up, uz}) {
-    set_space(f, "FFF");
     f->off[0] = f->off[1] = f->off[2] = 0;
-    k_set_scalar(c, f->sc, 0.0);
+    if (k.nz > 1) fft_z(c, f->e, +1, active_pairs(k, 0), k.nrchop, k.npchop);
+    leg_backward(c, f->e, c->scr[0], 0);
+    swap_scr(c, f);
+    if (k.np > 1) fft_phi(c, f->e, f->e, +1, rs[q]);
+    else if (rs[q] < 0) k_scale_rows_r(c, f->e, 1);
+    set_space(f, "PPP");
+    ++q;
   }
-  trans(c, ur, "PPP", -1);
-  trans(c, up, "PPP", -1);
-  trans(c, uz, "PPP");
 }
 
 // tp2vec (vector_reconstruction, mlegs_scalar_ops.f90:1455-1545): the +3-offset
@@ -227,7 +234,7 @@ void tp2vec(mlx_ctx *c, const mlx_field *psi, const mlx_field *chi, mlx_field *v
   need_kit(c);
   if (!is_fff(psi) || !is_fff(chi)) MLX_THROW(MLX_E_LAYOUT, "vector_projection: psi and chi must be in FFF");
   mlx_field *ur = tmp(c, 0), *up = tmp(c, 1), *uz = tmp(c, 2);
-  k_tp2vec_front(c, ur->e, up->e, uz->e, psi->e, chi->e, psi->sc, chi->sc, 0, 0);
+  k_tp2vec_front(c, ur->e, up->e, uz->e, psi->e, chi->e, psi->sc, chi->sc, 0, 0, ur->sc, up->sc, uz->sc);
   tp2vec_back(c, ur, up, uz);
   fmove(vr, ur);
   fmove(vp, up);
@@ -242,7 +249,8 @@ void tp2curlvec(mlx_ctx *c, const mlx_field *psi, const mlx_field *chi, mlx_fiel
   if (!is_fff(psi) || !is_fff(chi)) MLX_THROW(MLX_E_LAYOUT, "vector_projection: psi and chi must be in FFF");
   mlx_field *ur = tmp(c, 0), *up = tmp(c, 1), *uz = tmp(c, 2);
   // del2 of md = chi runs over the rows of chi's own radial offset
-  k_tp2vec_front(c, ur->e, up->e, uz->e, chi->e, psi->e, chi->sc, psi->sc, 1, chi->off[0]);
+  k_tp2vec_front(c, ur->e, up->e, uz->e, chi->e, psi->e, chi->sc, psi->sc, 1, chi->off[0], ur->sc, up->sc,
+                 uz->sc);
   tp2vec_back(c, ur, up, uz);
   fmove(wr, ur);
   fmove(wp, up);
@@ -284,8 +292,7 @@ void phi_forward_from(mlx_ctx *c, mlx_field *dst, const mlx_field *src, int rsca
 void vec2tp_pfp(mlx_ctx *c, mlx_field *ur, mlx_field *up, mlx_field *uz, mlx_field *psi, mlx_field *chi,
                 bool pre_dealias) {
   const KitDev &k = c->kit;
-  k_set_scalar(c, psi->sc, 0.0);
-  k_set_scalar(c, chi->sc, 0.0);
+  // psi's log term is set by k_vec2tp_up_ln and chi's by idel2 below
   // the projection reads columns m < npchop only
   if (k.nz > 1) fft_z(c, ur->e, -1, (double)k.nr * k.npchop, 0, k.npchop);
   trans(c, up, "FFF");
@@ -343,11 +350,12 @@ void advection_rhs(mlx_ctx *c, mlx_field *psi, mlx_field *chi, mlx_field *nlpsi,
   // de-aliased inside vec2tp before idel2; chop and zeroat1 keep zero modes
   // zero, so the closing dealias of the reference is already satisfied
   mlx_field *ur = tmp(c, 0), *up = tmp(c, 1), *uzp = tmp(c, 2);
-  if (fft_phi_cross(c, ur->e, up->e, uzp->e, vr->e, vp->e, vz->e, wr->e, wp->e, wz->e, uz->e)) {
+  if (fft_phi_cross(c, ur->e, up->e, uzp->e, vr->e, vp->e, vz->e, wr->e, wp->e, wz->e, uz->e, ur->sc, up->sc,
+                    uzp->sc)) {
     // vz + uz, v x w, and the phi transforms of r (v x w)_r, r (v x w)_phi,
-    // (v x w)_z in one pass: log terms zero, no offsets, as vecprod leaves them
+    // (v x w)_z in one pass: log terms zero (set by the kernel), no offsets,
+    // as vecprod leaves them
     for (mlx_field *f : {ur, up, uzp}) {
-      k_set_scalar(c, f->sc, 0.0);
       f->off[0] = f->off[1] = f->off[2] = 0;
       set_space(f, "PFP");
     }
@@ -361,11 +369,12 @@ void advection_rhs(mlx_ctx *c, mlx_field *psi, mlx_field *chi, mlx_field *nlpsi,
   k_set_scalar(c, nlchi->sc, 0.0);
 }
 
-void svv(mlx_ctx *c, mlx_field *f, double *gain_dev) {
+// flag_dev (optional): the non-finite check of the field, on the same pass
+void svv(mlx_ctx *c, mlx_field *f, double *gain_dev, int *flag_dev = nullptr) {
   if (!c->p.is_svv) return;
   if (!is_fff(f)) MLX_THROW(MLX_E_LAYOUT, "svv_filter: scalar must be in FFF space");
   need_kit(c);
-  k_svv(c, f->e, gain_dev);
+  k_svv(c, f->e, gain_dev, flag_dev);
 }
 
 void check_finite(mlx_ctx *c, const mlx_field *a, const mlx_field *b, int *flag_dev) {
@@ -859,12 +868,16 @@ int mlx_vortex_step(mlx_ctx *c, mlx_vortex *v, int32_t nsteps, double dt, int32_
       v->prev_stale = true;
       dealias(c, psi);
       dealias(c, chi);
-      svv(c, psi, v->gains);
-      svv(c, chi, v->gains + 1);
+      // with SVV on, its reduction pass also checks psi and chi for
+      // non-finite entries: the filter and zeroat1 keep a finite field
+      // finite and a non-finite one non-finite, so the flag is the one the
+      // check after the step would raise
+      svv(c, psi, v->gains, v->flag);
+      svv(c, chi, v->gains + 1, v->flag);
       zeroat1(c, psi);
       zeroat1(c, chi);
       advection_rhs(c, psi, chi, nlpsi, nlchi, uz, true);
-      check_finite(c, psi, chi, v->flag);
+      if (!c->p.is_svv) check_finite(c, psi, chi, v->flag);
     }
     if (check) {
       int h = 0;

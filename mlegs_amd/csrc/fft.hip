@@ -464,7 +464,8 @@ __global__ void __launch_bounds__(kThreads) k_fft4_cross(double2 *o0, double2 *o
                                                          const double2 *v1, const double2 *v2, const double2 *w0,
                                                          const double2 *w1, const double2 *w2, const double2 *ub,
                                                          size_t es, size_t gs, int nrow, const double2 *tw,
-                                                         const double2 *twh, double scale, const double *r, int nr) {
+                                                         const double2 *twh, double scale, const double *r, int nr,
+                                                         double *ln0, double *ln1, double *ln2) {
   constexpr int N = N1 * N2;
   constexpr int T1 = (TI * N2 + kThreads - 1) / kThreads;
   constexpr int T2 = (TI * N1 + kThreads - 1) / kThreads;
@@ -472,6 +473,12 @@ __global__ void __launch_bounds__(kThreads) k_fft4_cross(double2 *o0, double2 *o
   __shared__ double2 buf[TI * N];
   const int i0 = blockIdx.x * TI;
   const size_t base = (size_t)blockIdx.y * gs;
+  // the products carry no log term
+  if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) {
+    if (ln0) ln0[0] = 0.0;
+    if (ln1) ln1[0] = 0.0;
+    if (ln2) ln2[0] = 0.0;
+  }
   if (i0 >= nrow) return;
   const int tid = threadIdx.x;
 
@@ -640,7 +647,8 @@ void fft_phi(mlx_ctx *c, double2 *out, const double2 *in, int dir, int rscale) {
 }
 
 bool fft_phi_cross(mlx_ctx *c, double2 *o0, double2 *o1, double2 *o2, const double2 *v0, const double2 *v1,
-                   const double2 *v2, const double2 *w0, const double2 *w1, const double2 *w2, const double2 *ub) {
+                   const double2 *v2, const double2 *w0, const double2 *w1, const double2 *w2, const double2 *ub,
+                   double *ln0, double *ln1, double *ln2) {
   const KitDev &k = c->kit;
   const int N = k.nph;
   if (k.np <= 1 || !(N == 16 || N == 32 || N == 64 || N == 128 || N == 256)) return false;
@@ -653,7 +661,7 @@ bool fft_phi_cross(mlx_ctx *c, double2 *o0, double2 *o1, double2 *o2, const doub
     dim3 grid((k.nrdim + TI_ - 1) / TI_, k.nzdim);                                                              \
     hipLaunchKernelGGL((k_fft4_cross<N1_, N2_, TI_>), grid, dim3(kThreads), 0, c->stream, o0, o1, o2, v0, v1, v2, \
                        w0, w1, w2, ub, (size_t)k.nrdim, plane, k.nrdim, k.tw_nph, k.tw_half, 1.0 / k.np, k.r,   \
-                       k.nr);                                                                                   \
+                       k.nr, ln0, ln1, ln2);                                                                    \
   } while (0)
   switch (N) {
     case 16: MLX_FFTX(4, 4, 64); break;

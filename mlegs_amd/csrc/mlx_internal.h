@@ -176,7 +176,8 @@ void fft_phi(mlx_ctx *c, double2 *out, const double2 *in, int dir, int rscale = 
 // the fused form is not built for this np (the caller then runs vecprod and
 // three phi transforms)
 bool fft_phi_cross(mlx_ctx *c, double2 *o0, double2 *o1, double2 *o2, const double2 *v0, const double2 *v1,
-                   const double2 *v2, const double2 *w0, const double2 *w1, const double2 *w2, const double2 *ub);
+                   const double2 *v2, const double2 *w0, const double2 *w1, const double2 *w2, const double2 *ub,
+                   double *ln0 = nullptr, double *ln1 = nullptr, double *ln2 = nullptr);
 // dir -1 forward (/nz), +1 backward.  ncol: the (row, column) lines the caller
 // needs transformed (the launch's algorithmic work).  act_nrc > 0: only rows
 // i < act_nrc - j of column j need transforming; ncols > 0: only columns
@@ -201,7 +202,7 @@ void vec2tp_project(mlx_ctx *c, const double2 *ur, const double2 *up, const doub
 void k_ln_column(mlx_ctx *c, double2 *e, const double *ln, double sign);
 void k_chop(mlx_ctx *c, double2 *e, const char *space, int nroff);
 void k_dealias(mlx_ctx *c, double2 *e, const char *space);
-void k_svv(mlx_ctx *c, double2 *e, double *gain_dev);
+void k_svv(mlx_ctx *c, double2 *e, double *gain_dev, int *flag = nullptr);
 void k_calcat(mlx_ctx *c, const double2 *e, const double *tbl, int nrc, double2 *out);
 void k_zeroat1(mlx_ctx *c, double2 *e, int nrc);
 void k_delsqp(mlx_ctx *c, double2 *e, double *ln, int nroff, int inverse);
@@ -224,7 +225,8 @@ void k_tp2vec_combine(mlx_ctx *c, double2 *ur, double2 *up, const double2 *psi, 
 // mode 0: (P, C) = (psi, chi); mode 1: P = -del2(p_in = chi) over the rows of
 // chi's radial offset md_off, C = c_in = psi
 void k_tp2vec_front(mlx_ctx *c, double2 *ur, double2 *up, double2 *uz, const double2 *p_in, const double2 *c_in,
-                    const double *p_ln, const double *c_ln, int mode, int md_off);
+                    const double *p_ln, const double *c_ln, int mode, int md_off, double *ln_r = nullptr,
+                    double *ln_p = nullptr, double *ln_z = nullptr);
 void k_vecprod(mlx_ctx *c, double2 *vr, double2 *vp, double2 *vz, const double2 *ur, const double2 *up,
                const double2 *uz);
 void k_vec2tp_up_ln(mlx_ctx *c, double2 *up, const double2 *inf, double *psiln);

@@ -91,20 +91,25 @@ __device__ inline double svv_q(int i, int j, int k, double inv_r, double inv_p, 
   return fmin(1.0, fmax(fmax(qr, qp), qz));
 }
 
+// flag (optional): set to 1 when any entry is not finite, the stability
+// check of the time loop taken on the same pass over the field
 __global__ void k_svv_reduce(const double2 *e, int nrdim, int npdim, int nz, double inv_r, double inv_p,
-                             const double *ak, double kmax, double cutoff, double *part) {
+                             const double *ak, double kmax, double cutoff, double *part, int *flag) {
   __shared__ double sh[16];
   size_t n = (size_t)nrdim * npdim * nz;
   double tot = 0.0, tail = 0.0;
+  int bad = 0;
   for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (size_t)gridDim.x * blockDim.x) {
     int i = (int)(t % nrdim);
     int j = (int)((t / nrdim) % npdim);
     int k = (int)(t / ((size_t)nrdim * npdim));
     double2 v = e[t];
+    if (!isfinite(v.x) || !isfinite(v.y)) bad = 1;
     double a2 = v.x * v.x + v.y * v.y;
     tot += a2;
     if (svv_q(i, j, k, inv_r, inv_p, ak, kmax) >= cutoff) tail += a2;
   }
+  if (flag && bad) atomicOr(flag, 1);
   double a = block_sum(tot, sh);
   double b = block_sum(tail, sh);
   if (threadIdx.x == 0) {
@@ -544,8 +549,16 @@ __global__ void __launch_bounds__(256) k_tp2vec_front_(double2 *ur, double2 *up,
                                                       int mode, int md_off, const double *d2c, const double *xxc,
                                                       const double *lognorm, const double *ak, int nrdim, int npdim,
                                                       int nz, int ne, int nrc, int npc, int nzc, int nzcu,
-                                                      double ell) {
+                                                      double ell, double *ln_r, double *ln_p, double *ln_z) {
   extern __shared__ double2 lds[];
+  // the three outputs carry no log term (their ln slots are distinct from
+  // the inputs' p_ln and c_ln)
+  if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) {
+    if (ln_r) ln_r[0] = 0.0;
+    if (ln_p) ln_p[0] = 0.0;
+    if (ln_z) ln_z[0] = 0.0;
+  }
+
   double2 *P = lds, *C = lds + nrdim, *A = lds + 2 * nrdim, *B = lds + 3 * nrdim;
   // m fastest: consecutive workgroups take adjacent columns of one slice
   const int m = blockIdx.x, k = blockIdx.y;
@@ -828,7 +841,7 @@ void k_dealias(mlx_ctx *c, double2 *e, const char *space) {
                      pcut, zcut, zupper);
 }
 
-void k_svv(mlx_ctx *c, double2 *e, double *gain_dev) {
+void k_svv(mlx_ctx *c, double2 *e, double *gain_dev, int *flag) {
   const KitDev &k = c->kit;
   const mlx_params &p = c->p;
   double inv_r = 1.0 / fmax((double)(k.nrchop - 1), 1.0);
@@ -843,7 +856,7 @@ void k_svv(mlx_ctx *c, double2 *e, double *gain_dev) {
   double *part = c->dsmall;
   double *out = c->dsmall + 2 * nb;
   hipLaunchKernelGGL(k_svv_reduce, dim3(nb), dim3(kT), 0, c->stream, e, k.nrdim, k.npdim, k.nzdim, inv_r, inv_p,
-                     k.ak, kmax, cutoff, part);
+                     k.ak, kmax, cutoff, part, flag);
   hipLaunchKernelGGL(k_svv_gain, dim3(1), dim3(1024), 0, c->stream, part, nb, gain_dev, out, target, relax, st0);
   hipLaunchKernelGGL(k_svv_apply, dim3(grid_for(c->nelem)), dim3(kT), 0, c->stream, e, k.nrdim, k.npdim, k.nzdim,
                      inv_r, inv_p, k.ak, kmax, cutoff, out);
@@ -949,7 +962,8 @@ void k_tp2vec_combine(mlx_ctx *c, double2 *ur, double2 *up, const double2 *psi, 
 }
 
 void k_tp2vec_front(mlx_ctx *c, double2 *ur, double2 *up, double2 *uz, const double2 *p_in, const double2 *c_in,
-                    const double *p_ln, const double *c_ln, int mode, int md_off) {
+                    const double *p_ln, const double *c_ln, int mode, int md_off, double *ln_r, double *ln_p,
+                    double *ln_z) {
   const KitDev &k = c->kit;
   if (k.nrchop + 3 > k.nrdim || k.nrchop + 3 > k.ne) MLX_THROW(MLX_E_ARG, "chop_index: chopping in r too large");
   if (k.nrchop > 768) MLX_THROW(MLX_E_ARG, "vector_reconstruction: nrchop above 768 is not supported");
@@ -959,7 +973,7 @@ void k_tp2vec_front(mlx_ctx *c, double2 *ur, double2 *up, double2 *uz, const dou
   size_t lds = 4 * sizeof(double2) * (size_t)k.nrdim;
   hipLaunchKernelGGL(k_tp2vec_front_, grid, dim3(256), lds, c->stream, ur, up, uz, p_in, c_in, p_ln, c_ln, mode,
                      md_off, k.d2c, k.xxc, k.lognorm, k.ak, k.nrdim, k.npdim, k.nzdim, k.ne, k.nrchop, k.npchop, k.chopzl,
-                     k.chopzu, k.ell);
+                     k.chopzu, k.ell, ln_r, ln_p, ln_z);
   HIPCHK(hipGetLastError());
 }
 
