@@ -208,18 +208,44 @@ void abcn(mlx_ctx *c, mlx_field *s, mlx_field *s_p, mlx_field *nl, mlx_field *nl
   else fcopy(c, nl_p, nl);
 }
 
+// The modes de-aliasing zeroes (dealias, mlegs_scalar_ops.f90:305-352):
+// columns m >= pcut and kappa slices zlo..zhi.  When the advection term's
+// inputs are de-aliased and its result is de-aliased anyway, the MFMA path
+// transforms columns m < pcut only and the projection skips the zlo..zhi
+// slices (on = false: every column and slice, as without de-aliasing).
+struct DealiasLimits {
+  bool on = false;
+  int pcut = -1, zlo = 1, zhi = 0;
+};
+DealiasLimits dealias_limits(mlx_ctx *c) {
+  const KitDev &k = c->kit;
+  DealiasLimits d;
+  if (c->prec == MLX_PREC_FP64 || !mfma_ok(k, 0) || k.np <= 1) return d;
+  d.on = true;
+  d.pcut = k.np / 3 + 1 > 1 ? k.np / 3 + 1 : 1;
+  if (d.pcut > k.npchop) d.pcut = k.npchop;
+  if (k.nz > 1) {
+    const int zcut = (k.nz / 3 + 1 > 1) ? k.nz / 3 + 1 : 1;
+    d.zlo = zcut;
+    d.zhi = k.nz - zcut;  // zupper - 2 with zupper = nz - zcut + 2
+  }
+  return d;
+}
+
 // Back half of tp2vec: the three FFF components (offset 0, no log term) to
-// PPP; ur and up are divided by r in the last phi transform.
+// PPP; ur and up are divided by r in the last phi transform.  ncols > 0: the
+// components are zero in columns m >= ncols, which the transforms skip.
 // The front kernel has zeroed the log terms, so trans's log-column step
 // (adding 0 * ln x) is left out; the other steps are trans's FFF -> PPP.
-void tp2vec_back(mlx_ctx *c, mlx_field *ur, mlx_field *up, mlx_field *uz) {
+void tp2vec_back(mlx_ctx *c, mlx_field *ur, mlx_field *up, mlx_field *uz, int ncols = -1) {
   const KitDev &k = c->kit;
   const int rs[3] = {-1, -1, 0};
+  const int nc = (ncols > 0 && ncols < k.npchop) ? ncols : k.npchop;
   int q = 0;
   for (mlx_field *f : {ur, up, uz}) {
     f->off[0] = f->off[1] = f->off[2] = 0;
-    if (k.nz > 1) fft_z(c, f->e, +1, active_pairs(k, 0), k.nrchop, k.npchop);
-    leg_backward(c, f->e, c->scr[0], 0);
+    if (k.nz > 1) fft_z(c, f->e, +1, active_pairs(k, 0, nc), k.nrchop, nc);
+    leg_backward(c, f->e, c->scr[0], 0, true, nc);
     swap_scr(c, f);
     if (k.np > 1) fft_phi(c, f->e, f->e, +1, rs[q]);
     else if (rs[q] < 0) k_scale_rows_r(c, f->e, 1);
@@ -230,12 +256,14 @@ void tp2vec_back(mlx_ctx *c, mlx_field *ur, mlx_field *up, mlx_field *uz) {
 
 // tp2vec (vector_reconstruction, mlegs_scalar_ops.f90:1455-1545): the +3-offset
 // band products, the combination and the chop run in one column pass
-void tp2vec(mlx_ctx *c, const mlx_field *psi, const mlx_field *chi, mlx_field *vr, mlx_field *vp, mlx_field *vz) {
+// ncols > 0: psi and chi are zero in columns m >= ncols (de-aliased)
+void tp2vec(mlx_ctx *c, const mlx_field *psi, const mlx_field *chi, mlx_field *vr, mlx_field *vp, mlx_field *vz,
+            int ncols = -1) {
   need_kit(c);
   if (!is_fff(psi) || !is_fff(chi)) MLX_THROW(MLX_E_LAYOUT, "vector_projection: psi and chi must be in FFF");
   mlx_field *ur = tmp(c, 0), *up = tmp(c, 1), *uz = tmp(c, 2);
-  k_tp2vec_front(c, ur->e, up->e, uz->e, psi->e, chi->e, psi->sc, chi->sc, 0, 0, ur->sc, up->sc, uz->sc);
-  tp2vec_back(c, ur, up, uz);
+  k_tp2vec_front(c, ur->e, up->e, uz->e, psi->e, chi->e, psi->sc, chi->sc, 0, 0, ur->sc, up->sc, uz->sc, ncols);
+  tp2vec_back(c, ur, up, uz, ncols);
   fmove(vr, ur);
   fmove(vp, up);
   fmove(vz, uz);
@@ -244,14 +272,14 @@ void tp2vec(mlx_ctx *c, const mlx_field *psi, const mlx_field *chi, mlx_field *v
 // tp2curlvec (curl_vector_reconstruction, :1547-1560): curl V = tp2vec(-del2 chi, psi),
 // with -del2 chi formed inside the same column pass
 void tp2curlvec(mlx_ctx *c, const mlx_field *psi, const mlx_field *chi, mlx_field *wr, mlx_field *wp,
-                mlx_field *wz) {
+                mlx_field *wz, int ncols = -1) {
   need_kit(c);
   if (!is_fff(psi) || !is_fff(chi)) MLX_THROW(MLX_E_LAYOUT, "vector_projection: psi and chi must be in FFF");
   mlx_field *ur = tmp(c, 0), *up = tmp(c, 1), *uz = tmp(c, 2);
   // del2 of md = chi runs over the rows of chi's own radial offset
   k_tp2vec_front(c, ur->e, up->e, uz->e, chi->e, psi->e, chi->sc, psi->sc, 1, chi->off[0], ur->sc, up->sc,
-                 uz->sc);
-  tp2vec_back(c, ur, up, uz);
+                 uz->sc, ncols);
+  tp2vec_back(c, ur, up, uz, ncols);
   fmove(wr, ur);
   fmove(wp, up);
   fmove(wz, uz);
@@ -289,23 +317,33 @@ void phi_forward_from(mlx_ctx *c, mlx_field *dst, const mlx_field *src, int rsca
 // zero slice at zero, so de-aliasing right after the projection gives the same
 // result and lets the banded solve skip the de-aliased (all-zero) systems.
 // the projection from ur = r vr, up = r vp, uz = vz already in PFP
+// dl.on (with pre_dealias): the result is de-aliased by the projection
+// itself, which computes columns m < pcut and slices outside zlo..zhi only;
+// the transforms before it then skip the columns it does not read.
 void vec2tp_pfp(mlx_ctx *c, mlx_field *ur, mlx_field *up, mlx_field *uz, mlx_field *psi, mlx_field *chi,
-                bool pre_dealias) {
+                bool pre_dealias, DealiasLimits dl = DealiasLimits()) {
   const KitDev &k = c->kit;
+  if (!pre_dealias) dl = DealiasLimits();
+  const int nc = dl.on ? dl.pcut : k.npchop;
   // psi's log term is set by k_vec2tp_up_ln and chi's by idel2 below
-  // the projection reads columns m < npchop only
-  if (k.nz > 1) fft_z(c, ur->e, -1, (double)k.nr * k.npchop, 0, k.npchop);
-  trans(c, up, "FFF");
+  // the projection reads columns m < npchop (nc) only
+  if (k.nz > 1) fft_z(c, ur->e, -1, (double)k.nr * nc, 0, nc);
+  // up to FFF as trans does it (PFP -> FFP -> FFF), over columns m < nc
+  k_ln_column(c, up->e, up->sc, -1.0);
+  leg_forward(c, up->e, c->scr[0], up->off[0], true, nc);
+  swap_scr(c, up);
+  if (k.nz > 1) fft_z(c, up->e, -1, active_pairs(k, up->off[0], nc), k.nrchop + up->off[0], nc);
+  set_space(up, "FFF");
   double2 *inf = reinterpret_cast<double2 *>(c->dsmall + 2048);
   k_calcat(c, up->e, k.at1, k.nrchop + up->off[0], inf);
-  leg_backward(c, up->e, c->scr[0], up->off[0]);
+  leg_backward(c, up->e, c->scr[0], up->off[0], true, nc);
   swap_scr(c, up);
   k_vec2tp_up_ln(c, up->e, inf, psi->sc);
-  if (k.nz > 1) fft_z(c, uz->e, -1, (double)k.nr * k.npchop, 0, k.npchop);
-  vec2tp_project(c, ur->e, up->e, uz->e, psi->e, chi->e, 3);
+  if (k.nz > 1) fft_z(c, uz->e, -1, (double)k.nr * nc, 0, nc);
+  vec2tp_project(c, ur->e, up->e, uz->e, psi->e, chi->e, 3, nc, dl.on ? dl.zlo : 1, dl.on ? dl.zhi : 0);
   set_space(psi, "FFF");
   set_space(chi, "FFF");
-  if (pre_dealias) {
+  if (pre_dealias && !dl.on) {
     dealias(c, psi);
     dealias(c, chi);
   }
@@ -345,13 +383,16 @@ void advection_rhs(mlx_ctx *c, mlx_field *psi, mlx_field *chi, mlx_field *nlpsi,
   }
   mlx_field *vr = tmp(c, 4), *vp = tmp(c, 5), *vz = tmp(c, 6);
   mlx_field *wr = tmp(c, 7), *wp = tmp(c, 8), *wz = tmp(c, 9);
-  tp2vec(c, psi, chi, vr, vp, vz);
-  tp2curlvec(c, psi, chi, wr, wp, wz);
+  // psi and chi are de-aliased here and so is the result: the MFMA path
+  // skips the columns and slices de-aliasing zeroes
+  const DealiasLimits dl = dealias_limits(c);
+  tp2vec(c, psi, chi, vr, vp, vz, dl.on ? dl.pcut : -1);
+  tp2curlvec(c, psi, chi, wr, wp, wz, dl.on ? dl.pcut : -1);
   // de-aliased inside vec2tp before idel2; chop and zeroat1 keep zero modes
   // zero, so the closing dealias of the reference is already satisfied
   mlx_field *ur = tmp(c, 0), *up = tmp(c, 1), *uzp = tmp(c, 2);
   if (fft_phi_cross(c, ur->e, up->e, uzp->e, vr->e, vp->e, vz->e, wr->e, wp->e, wz->e, uz->e, ur->sc, up->sc,
-                    uzp->sc)) {
+                    uzp->sc, dl.on ? dl.pcut : -1)) {
     // vz + uz, v x w, and the phi transforms of r (v x w)_r, r (v x w)_phi,
     // (v x w)_z in one pass: log terms zero (set by the kernel), no offsets,
     // as vecprod leaves them
@@ -359,7 +400,7 @@ void advection_rhs(mlx_ctx *c, mlx_field *psi, mlx_field *chi, mlx_field *nlpsi,
       f->off[0] = f->off[1] = f->off[2] = 0;
       set_space(f, "PFP");
     }
-    vec2tp_pfp(c, ur, up, uzp, nlpsi, nlchi, true);
+    vec2tp_pfp(c, ur, up, uzp, nlpsi, nlchi, true, dl);
   } else {
     k_axpby(c, vz->e, vz->sc, 1.0, uz->e, nullptr, 1.0);
     vecprod(c, vr, vp, vz, wr, wp, wz);

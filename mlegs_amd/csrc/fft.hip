@@ -465,7 +465,7 @@ __global__ void __launch_bounds__(kThreads) k_fft4_cross(double2 *o0, double2 *o
                                                          const double2 *w1, const double2 *w2, const double2 *ub,
                                                          size_t es, size_t gs, int nrow, const double2 *tw,
                                                          const double2 *twh, double scale, const double *r, int nr,
-                                                         double *ln0, double *ln1, double *ln2) {
+                                                         double *ln0, double *ln1, double *ln2, int mmax) {
   constexpr int N = N1 * N2;
   constexpr int T1 = (TI * N2 + kThreads - 1) / kThreads;
   constexpr int T2 = (TI * N1 + kThreads - 1) / kThreads;
@@ -582,7 +582,7 @@ __global__ void __launch_bounds__(kThreads) k_fft4_cross(double2 *o0, double2 *o
       const double2 D = make_double2(0.5 * (zm.x - zc.x), 0.5 * (zm.y - zc.y));
       const double2 O = make_double2(D.y, -D.x);  // -i * D
       const double2 C = cadd(E, cmul(twh[m], O));
-      if (i < nrow) out[base + (size_t)m * es + i] = make_double2(C.x * scale, C.y * scale);
+      if (i < nrow && m < mmax) out[base + (size_t)m * es + i] = make_double2(C.x * scale, C.y * scale);
     }
   }
 }
@@ -648,20 +648,21 @@ void fft_phi(mlx_ctx *c, double2 *out, const double2 *in, int dir, int rscale) {
 
 bool fft_phi_cross(mlx_ctx *c, double2 *o0, double2 *o1, double2 *o2, const double2 *v0, const double2 *v1,
                    const double2 *v2, const double2 *w0, const double2 *w1, const double2 *w2, const double2 *ub,
-                   double *ln0, double *ln1, double *ln2) {
+                   double *ln0, double *ln1, double *ln2, int mmax) {
   const KitDev &k = c->kit;
   const int N = k.nph;
+  if (mmax <= 0 || mmax > N + 1) mmax = N + 1;
   if (k.np <= 1 || !(N == 16 || N == 32 || N == 64 || N == 128 || N == 256)) return false;
   const size_t plane = (size_t)k.nrdim * k.npdim;
   // seven physical fields read (rows < nr), three PFP fields written
-  ProbeScope probe(c, MLX_PROBE_PHI_CROSS, 16.0 * k.nr * k.nz * (7.0 * N + 3.0 * (N + 1)),
+  ProbeScope probe(c, MLX_PROBE_PHI_CROSS, 16.0 * k.nr * k.nz * (7.0 * N + 3.0 * mmax),
                    3.0 * 2.5 * k.np * std::log2((double)k.np) * k.nr * k.nz + 18.0 * k.nr * k.np * k.nz);
 #define MLX_FFTX(N1_, N2_, TI_)                                                                                 \
   do {                                                                                                          \
     dim3 grid((k.nrdim + TI_ - 1) / TI_, k.nzdim);                                                              \
     hipLaunchKernelGGL((k_fft4_cross<N1_, N2_, TI_>), grid, dim3(kThreads), 0, c->stream, o0, o1, o2, v0, v1, v2, \
                        w0, w1, w2, ub, (size_t)k.nrdim, plane, k.nrdim, k.tw_nph, k.tw_half, 1.0 / k.np, k.r,   \
-                       k.nr, ln0, ln1, ln2);                                                                    \
+                       k.nr, ln0, ln1, ln2, mmax);                                                              \
   } while (0)
   switch (N) {
     case 16: MLX_FFTX(4, 4, 64); break;

@@ -316,11 +316,11 @@ const WorkTable &get_work(mlx_ctx *c, int kind, int nroff) {
 
 namespace mlx {
 
-void leg_forward(mlx_ctx *c, const double2 *in, double2 *out, int nroff, bool zero) {
+void leg_forward(mlx_ctx *c, const double2 *in, double2 *out, int nroff, bool zero, int ncols) {
   const KitDev &k = c->kit;
   if (k.nrchop + nroff > k.nrdim) MLX_THROW(MLX_E_ARG, "rtrans_forward: chopping in r too large");
   if (c->prec != MLX_PREC_FP64 && mfma_ok(k, nroff)) {
-    leg_forward_f32(c, in, out);
+    leg_forward_f32(c, in, out, ncols);
     return;
   }
   if (zero) HIPCHK(hipMemsetAsync(out, 0, sizeof(double2) * c->nelem, c->stream));
@@ -333,11 +333,11 @@ void leg_forward(mlx_ctx *c, const double2 *in, double2 *out, int nroff, bool ze
   HIPCHK(hipGetLastError());
 }
 
-void leg_backward(mlx_ctx *c, const double2 *in, double2 *out, int nroff, bool zero) {
+void leg_backward(mlx_ctx *c, const double2 *in, double2 *out, int nroff, bool zero, int ncols) {
   const KitDev &k = c->kit;
   if (k.nrchop + nroff > k.nrdim) MLX_THROW(MLX_E_ARG, "rtrans_backward: chopping in r too large");
   if (c->prec != MLX_PREC_FP64 && mfma_ok(k, nroff)) {
-    leg_backward_f32(c, in, out);
+    leg_backward_f32(c, in, out, ncols);
     return;
   }
   if (zero) HIPCHK(hipMemsetAsync(out, 0, sizeof(double2) * c->nelem, c->stream));
@@ -350,11 +350,13 @@ void leg_backward(mlx_ctx *c, const double2 *in, double2 *out, int nroff, bool z
   HIPCHK(hipGetLastError());
 }
 
+// ncols, zlo, zhi: as vec2tp_project_f32 on the MFMA path; the FP64 path
+// computes every column and slice
 void vec2tp_project(mlx_ctx *c, const double2 *ur, const double2 *up, const double2 *uz, double2 *psi,
-                    double2 *chi, int nroff) {
+                    double2 *chi, int nroff, int ncols, int zlo, int zhi) {
   const KitDev &k = c->kit;
   if (c->prec != MLX_PREC_FP64 && mfma_ok(k, 0)) {
-    vec2tp_project_f32(c, ur, up, uz, psi, chi);
+    vec2tp_project_f32(c, ur, up, uz, psi, chi, ncols, zlo, zhi);
     return;
   }
   HIPCHK(hipMemsetAsync(psi, 0, sizeof(double2) * c->nelem, c->stream));

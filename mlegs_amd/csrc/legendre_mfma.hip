@@ -352,7 +352,8 @@ __global__ void __launch_bounds__(256) k_v2t_f32(const double2 *__restrict__ ur,
                                                  double2 *__restrict__ chi, const float *__restrict__ tgV,
                                                  const float *__restrict__ tgD, const float *__restrict__ tgT,
                                                  const double *__restrict__ ak, int nrdim, int npdim, int nz, int nr,
-                                                 int nrh, int nrc0, int npc, int nzc, int nzcu, int nyt) {
+                                                 int nrh, int nrc0, int npc, int nzc, int nzcu, int nyt, int zlo,
+                                                 int zhi) {
   constexpr int AP = 4 * AT * 16;
   constexpr int APP = AP + 16;
   constexpr int KV = 8;   // contraction chunk for this kernel
@@ -361,13 +362,17 @@ __global__ void __launch_bounds__(256) k_v2t_f32(const double2 *__restrict__ ur,
   int u, yt;
   xcd_map(blockIdx.x, nyt, u, yt);
   const int m = u >> 1, p = u & 1;
+  if (m >= npdim) return;
   const int k0 = yt * KT;
   const int tid = threadIdx.x, lane = tid % 64, wv = tid / 64;
   const size_t zs = (size_t)nrdim * npdim;
-  if (m >= npc) return;
-  const int nn = nrc0 - m > 0 ? nrc0 - m : 0;
+  // every (m, parity, kappa tile) writes all its rows: computed values on
+  // the active set, zeros elsewhere (columns m >= npc, rows past the
+  // chopped range, kappa slices outside the chopped set or in zlo..zhi)
+  const int nn = (m < npc && nrc0 - m > 0) ? nrc0 - m : 0;
   const int M = (nn - p + 1) / 2;
-  if (M <= 0) return;
+  const bool band_tile = (k0 >= zlo && k0 + KT - 1 <= zhi);
+  const bool active = M > 0 && !band_tile;
   const double mv = (double)m;
   const double2 *cr = ur + (size_t)nrdim * m;
   const double2 *cp = up + (size_t)nrdim * m;
@@ -390,107 +395,120 @@ __global__ void __launch_bounds__(256) k_v2t_f32(const double2 *__restrict__ ur,
   double2 rra, rrb, rpa, rpb, rza, rzb;
   double rkv = 0.0;
   const int fx = tid % KV, fkl = tid / KV;
-  for (int x0 = -KV; x0 < nrh; x0 += KV) {
-    if (x0 >= 0) {
-#pragma unroll
-      for (int j = 0; j < NTV; ++j) {
-        const int idx = tid + 256 * j;
-        if (idx < 3 * KV * (AP / 4)) {
-          const int a = 4 * (idx % (AP / 4)), x = (idx / (AP / 4)) % KV, q = idx / ((AP / 4) * KV);
-          *reinterpret_cast<float4 *>(&Tb[q][x][a]) = rt[j];
+  if (active) {
+    for (int x0 = -KV; x0 < nrh; x0 += KV) {
+      if (x0 >= 0) {
+  #pragma unroll
+        for (int j = 0; j < NTV; ++j) {
+          const int idx = tid + 256 * j;
+          if (idx < 3 * KV * (AP / 4)) {
+            const int a = 4 * (idx % (AP / 4)), x = (idx / (AP / 4)) % KV, q = idx / ((AP / 4) * KV);
+            *reinterpret_cast<float4 *>(&Tb[q][x][a]) = rt[j];
+          }
+        }
+        const double kv = rkv;
+        const double2 reo = make_double2(rra.x + seo * rrb.x, rra.y + seo * rrb.y);  // eo(ur)
+        const double2 roe = make_double2(rra.x - seo * rrb.x, rra.y - seo * rrb.y);  // oe(ur)
+        const double2 peo = make_double2(rpa.x + seo * rpb.x, rpa.y + seo * rpb.y);
+        const double2 poe = make_double2(rpa.x - seo * rpb.x, rpa.y - seo * rpb.y);
+        const double2 zeo = make_double2(rza.x + seo * rzb.x, rza.y + seo * rzb.y);
+        const double2 x1 = make_double2(mv * reo.y, -mv * reo.x);           // -i m eo(ur)
+        const double2 x2 = make_double2(-poe.x, -poe.y);                    // -oe(up)
+        const double2 x3 = make_double2(-kv * roe.y, kv * roe.x);           // i kv oe(ur)
+        const double2 x4 = make_double2(mv * kv * peo.x, mv * kv * peo.y);  // m kv eo(up)
+        const double2 x5 = make_double2(-zeo.x, -zeo.y);                    // -eo(uz)
+        X[0][fx][2 * fkl] = (float)x1.x; X[0][fx][2 * fkl + 1] = (float)x1.y;
+        X[1][fx][2 * fkl] = (float)x2.x; X[1][fx][2 * fkl + 1] = (float)x2.y;
+        X[2][fx][2 * fkl] = (float)x3.x; X[2][fx][2 * fkl + 1] = (float)x3.y;
+        X[3][fx][2 * fkl] = (float)x4.x; X[3][fx][2 * fkl + 1] = (float)x4.y;
+        X[4][fx][2 * fkl] = (float)x5.x; X[4][fx][2 * fkl + 1] = (float)x5.y;
+        __syncthreads();
+      }
+      const int xn = x0 + KV;  // chunk fetched now, stored next iteration
+      if (xn < nrh) {
+  #pragma unroll
+        for (int j = 0; j < NTV; ++j) {
+          const int idx = tid + 256 * j;
+          float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+          if (idx < 3 * KV * (AP / 4)) {
+            const int a = 4 * (idx % (AP / 4)), x = (idx / (AP / 4)) % KV, q = idx / ((AP / 4) * KV);
+            const int xg = xn + x;
+            const float *src = (q == 0) ? tgV : (q == 1) ? tgD : tgT;
+            if (xg < nrh) v = *reinterpret_cast<const float4 *>(src + tb + (size_t)xg * AP + a);
+          }
+          rt[j] = v;
+        }
+        const int xg = xn + fx, kg = k0 + fkl;
+        const double2 z2 = make_double2(0.0, 0.0);
+        rra = rrb = rpa = rpb = rza = rzb = z2;
+        rkv = 0.0;
+        if (xg < nrh && kg < nz) {
+          const size_t oa = (size_t)kg * zs + xg, ob = (size_t)kg * zs + (nr - 1 - xg);
+          rra = cr[oa];
+          rrb = cr[ob];
+          rpa = cp[oa];
+          rpb = cp[ob];
+          rza = cz[oa];
+          rzb = cz[ob];
+          rkv = ak[kg];
         }
       }
-      const double kv = rkv;
-      const double2 reo = make_double2(rra.x + seo * rrb.x, rra.y + seo * rrb.y);  // eo(ur)
-      const double2 roe = make_double2(rra.x - seo * rrb.x, rra.y - seo * rrb.y);  // oe(ur)
-      const double2 peo = make_double2(rpa.x + seo * rpb.x, rpa.y + seo * rpb.y);
-      const double2 poe = make_double2(rpa.x - seo * rpb.x, rpa.y - seo * rpb.y);
-      const double2 zeo = make_double2(rza.x + seo * rzb.x, rza.y + seo * rzb.y);
-      const double2 x1 = make_double2(mv * reo.y, -mv * reo.x);           // -i m eo(ur)
-      const double2 x2 = make_double2(-poe.x, -poe.y);                    // -oe(up)
-      const double2 x3 = make_double2(-kv * roe.y, kv * roe.x);           // i kv oe(ur)
-      const double2 x4 = make_double2(mv * kv * peo.x, mv * kv * peo.y);  // m kv eo(up)
-      const double2 x5 = make_double2(-zeo.x, -zeo.y);                    // -eo(uz)
-      X[0][fx][2 * fkl] = (float)x1.x; X[0][fx][2 * fkl + 1] = (float)x1.y;
-      X[1][fx][2 * fkl] = (float)x2.x; X[1][fx][2 * fkl + 1] = (float)x2.y;
-      X[2][fx][2 * fkl] = (float)x3.x; X[2][fx][2 * fkl + 1] = (float)x3.y;
-      X[3][fx][2 * fkl] = (float)x4.x; X[3][fx][2 * fkl + 1] = (float)x4.y;
-      X[4][fx][2 * fkl] = (float)x5.x; X[4][fx][2 * fkl + 1] = (float)x5.y;
+      if (x0 < 0) continue;
+  #pragma unroll
+      for (int kk = 0; kk < KV; kk += 4) {
+        const int xr = kk + lane / 16;
+        float f[5][4];
+  #pragma unroll
+        for (int q = 0; q < 5; ++q)
+  #pragma unroll
+          for (int t = 0; t < 4; ++t) f[q][t] = X[q][xr][16 * t + lane % 16];
+  #pragma unroll
+        for (int s = 0; s < AT; ++s) {
+          const int at = wv + 4 * s;
+          if (at * 16 >= M) continue;
+          float bv = Tb[0][xr][at * 16 + lane % 16];
+          float bd = Tb[1][xr][at * 16 + lane % 16];
+          float bt = Tb[2][xr][at * 16 + lane % 16];
+  #pragma unroll
+          for (int t = 0; t < 4; ++t) {
+            aps[s][t] = mfma(f[0][t], bv, aps[s][t]);
+            aps[s][t] = mfma(f[1][t], bd, aps[s][t]);
+            ach[s][t] = mfma(f[2][t], bd, ach[s][t]);
+            ach[s][t] = mfma(f[3][t], bv, ach[s][t]);
+            ach[s][t] = mfma(f[4][t], bt, ach[s][t]);
+          }
+        }
+      }
       __syncthreads();
     }
-    const int xn = x0 + KV;  // chunk fetched now, stored next iteration
-    if (xn < nrh) {
-#pragma unroll
-      for (int j = 0; j < NTV; ++j) {
-        const int idx = tid + 256 * j;
-        float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
-        if (idx < 3 * KV * (AP / 4)) {
-          const int a = 4 * (idx % (AP / 4)), x = (idx / (AP / 4)) % KV, q = idx / ((AP / 4) * KV);
-          const int xg = xn + x;
-          const float *src = (q == 0) ? tgV : (q == 1) ? tgD : tgT;
-          if (xg < nrh) v = *reinterpret_cast<const float4 *>(src + tb + (size_t)xg * AP + a);
-        }
-        rt[j] = v;
-      }
-      const int xg = xn + fx, kg = k0 + fkl;
-      const double2 z2 = make_double2(0.0, 0.0);
-      rra = rrb = rpa = rpb = rza = rzb = z2;
-      rkv = 0.0;
-      if (xg < nrh && kg < nz) {
-        const size_t oa = (size_t)kg * zs + xg, ob = (size_t)kg * zs + (nr - 1 - xg);
-        rra = cr[oa];
-        rrb = cr[ob];
-        rpa = cp[oa];
-        rpb = cp[ob];
-        rza = cz[oa];
-        rzb = cz[ob];
-        rkv = ak[kg];
-      }
-    }
-    if (x0 < 0) continue;
-#pragma unroll
-    for (int kk = 0; kk < KV; kk += 4) {
-      const int xr = kk + lane / 16;
-      float f[5][4];
-#pragma unroll
-      for (int q = 0; q < 5; ++q)
-#pragma unroll
-        for (int t = 0; t < 4; ++t) f[q][t] = X[q][xr][16 * t + lane % 16];
-#pragma unroll
-      for (int s = 0; s < AT; ++s) {
-        const int at = wv + 4 * s;
-        if (at * 16 >= M) continue;
-        float bv = Tb[0][xr][at * 16 + lane % 16];
-        float bd = Tb[1][xr][at * 16 + lane % 16];
-        float bt = Tb[2][xr][at * 16 + lane % 16];
-#pragma unroll
-        for (int t = 0; t < 4; ++t) {
-          aps[s][t] = mfma(f[0][t], bv, aps[s][t]);
-          aps[s][t] = mfma(f[1][t], bd, aps[s][t]);
-          ach[s][t] = mfma(f[2][t], bd, ach[s][t]);
-          ach[s][t] = mfma(f[3][t], bv, ach[s][t]);
-          ach[s][t] = mfma(f[4][t], bt, ach[s][t]);
-        }
-      }
-    }
-    __syncthreads();
   }
 #pragma unroll
   for (int s = 0; s < AT; ++s) {
     const int a = (wv + 4 * s) * 16 + lane % 16;
-    if (a >= M) continue;
     const int n = 2 * a + p;
+    if (n >= nrdim) continue;
 #pragma unroll
     for (int t = 0; t < 4; ++t) {
 #pragma unroll
       for (int h = 0; h < 2; ++h) {
         const int kg = k0 + 8 * t + 2 * (lane / 16) + h;
         if (kg >= nz) continue;
-        if (!((kg + 1 <= nzc) || (kg + 1 >= nzcu))) continue;
+        const bool keep = active && a < M && ((kg + 1 <= nzc) || (kg + 1 >= nzcu)) && !(kg >= zlo && kg <= zhi);
         size_t o = (size_t)kg * zs + (size_t)nrdim * m + n;
-        psi[o] = make_double2(aps[s][t][2 * h], aps[s][t][2 * h + 1]);
-        chi[o] = make_double2(ach[s][t][2 * h], ach[s][t][2 * h + 1]);
+        psi[o] = keep ? make_double2(aps[s][t][2 * h], aps[s][t][2 * h + 1]) : make_double2(0.0, 0.0);
+        chi[o] = keep ? make_double2(ach[s][t][2 * h], ach[s][t][2 * h + 1]) : make_double2(0.0, 0.0);
       }
+    }
+  }
+  // rows of this parity past the tiles: zero
+  constexpr int A_END = 4 * AT * 16;
+  const int nrest = (nrdim - p - 2 * A_END + 1) / 2;
+  for (int idx = tid; idx < (nrest > 0 ? nrest : 0) * KT; idx += 256) {
+    const int n = 2 * (A_END + idx % nrest) + p, kg = k0 + idx / nrest;
+    if (kg < nz && n < nrdim) {
+      const size_t o = (size_t)kg * zs + (size_t)nrdim * m + n;
+      psi[o] = make_double2(0.0, 0.0);
+      chi[o] = make_double2(0.0, 0.0);
     }
   }
 }
@@ -547,51 +565,60 @@ void mfma_tables(mlx_ctx *c) {
   HIPCHK(hipGetLastError());
 }
 
-void leg_forward_f32(mlx_ctx *c, const double2 *in, double2 *out) {
+// ncols > 0: columns m >= ncols are written as zero without being read
+void leg_forward_f32(mlx_ctx *c, const double2 *in, double2 *out, int ncols) {
   const KitDev &k = c->kit;
   const int nyt = (k.nzdim + KT - 1) / KT;
+  const int npc = (ncols > 0 && ncols < k.npchop) ? ncols : k.npchop;
   dim3 grid = xcd_grid(k.npdim, nyt);
   int at = mfma_tiles(k);
-  ProbeScope probe(c, MLX_PROBE_LEG_FWD, leg_bytes(k, 0, 4), leg_flops(k, 0));
+  ProbeScope probe(c, MLX_PROBE_LEG_FWD, leg_bytes(k, 0, 4, npc), leg_flops(k, 0, npc));
   if (at == 1)
     hipLaunchKernelGGL(k_leg_fwd_f32<1>, grid, dim3(256), 0, c->stream, in, out, k.pfg32, k.w, k.nrdim, k.npdim,
-                       k.nzdim, k.nr, k.nrh, k.nrchop, k.npchop, nyt);
+                       k.nzdim, k.nr, k.nrh, k.nrchop, npc, nyt);
   else
     hipLaunchKernelGGL(k_leg_fwd_f32<2>, grid, dim3(256), 0, c->stream, in, out, k.pfg32, k.w, k.nrdim, k.npdim,
-                       k.nzdim, k.nr, k.nrh, k.nrchop, k.npchop, nyt);
+                       k.nzdim, k.nr, k.nrh, k.nrchop, npc, nyt);
   HIPCHK(hipGetLastError());
 }
 
-void leg_backward_f32(mlx_ctx *c, const double2 *in, double2 *out) {
+void leg_backward_f32(mlx_ctx *c, const double2 *in, double2 *out, int ncols) {
   const KitDev &k = c->kit;
   const int nyt = (k.nzdim + KT - 1) / KT;
+  const int npc = (ncols > 0 && ncols < k.npchop) ? ncols : k.npchop;
   dim3 grid = xcd_grid(k.npdim, nyt);
   int xt = (k.nrh + 63) / 64;
-  ProbeScope probe(c, MLX_PROBE_LEG_BWD, leg_bytes(k, 0, 4), leg_flops(k, 0));
+  ProbeScope probe(c, MLX_PROBE_LEG_BWD, leg_bytes(k, 0, 4, npc), leg_flops(k, 0, npc));
   if (xt == 1)
     hipLaunchKernelGGL(k_leg_bwd_f32<1>, grid, dim3(256), 0, c->stream, in, out, k.pf32, k.nrdim, k.npdim, k.nzdim,
-                       k.nr, k.nrh, k.ne, k.nrchop, k.npchop, nyt);
+                       k.nr, k.nrh, k.ne, k.nrchop, npc, nyt);
   else
     hipLaunchKernelGGL(k_leg_bwd_f32<2>, grid, dim3(256), 0, c->stream, in, out, k.pf32, k.nrdim, k.npdim, k.nzdim,
-                       k.nr, k.nrh, k.ne, k.nrchop, k.npchop, nyt);
+                       k.nr, k.nrh, k.ne, k.nrchop, npc, nyt);
   HIPCHK(hipGetLastError());
 }
 
+// Every entry of psi and chi is written (zeros outside the computed set).
+// ncols > 0: columns m >= ncols are zero; kappa slices zlo..zhi are zero
+// (zlo > zhi: none), for a caller that de-aliases the result.
 void vec2tp_project_f32(mlx_ctx *c, const double2 *ur, const double2 *up, const double2 *uz, double2 *psi,
-                        double2 *chi) {
+                        double2 *chi, int ncols, int zlo, int zhi) {
   const KitDev &k = c->kit;
-  HIPCHK(hipMemsetAsync(psi, 0, sizeof(double2) * c->nelem, c->stream));
-  HIPCHK(hipMemsetAsync(chi, 0, sizeof(double2) * c->nelem, c->stream));
   const int nyt = (k.nzdim + KT - 1) / KT;
-  dim3 grid = xcd_grid(2 * k.npchop, nyt);
+  const int npc = (ncols > 0 && ncols < k.npchop) ? ncols : k.npchop;
+  int nzk = k.nzdim;
+  for (int kk = 0; kk < k.nzdim; ++kk)
+    if (kk >= zlo && kk <= zhi) --nzk;
+  dim3 grid = xcd_grid(2 * k.npdim, nyt);
   int at = mfma_tiles(k);
-  ProbeScope probe(c, MLX_PROBE_V2T, v2t_bytes(k, 0, 4), 5.0 * leg_flops(k, 0));
+  ProbeScope probe(c, MLX_PROBE_V2T, v2t_bytes(k, 0, 4, npc, nzk, true),
+                   5.0 * leg_flops(k, 0, npc) * nzk / (double)k.nzdim);
   if (at == 1)
     hipLaunchKernelGGL(k_v2t_f32<1>, grid, dim3(256), 0, c->stream, ur, up, uz, psi, chi, k.tVg32, k.tDg32, k.tTg32,
-                       k.ak, k.nrdim, k.npdim, k.nzdim, k.nr, k.nrh, k.nrchop, k.npchop, k.chopzl, k.chopzu, nyt);
+                       k.ak, k.nrdim, k.npdim, k.nzdim, k.nr, k.nrh, k.nrchop, npc, k.chopzl, k.chopzu, nyt, zlo, zhi);
   else
     hipLaunchKernelGGL(k_v2t_f32<2>, grid, dim3(256), 0, c->stream, ur, up, uz, psi, chi, k.tVg32, k.tDg32, k.tTg32,
-                       k.ak, k.nrdim, k.npdim, k.nzdim, k.nr, k.nrh, k.nrchop, k.npchop, k.chopzl, k.chopzu, nyt);
+                       k.ak, k.nrdim, k.npdim, k.nzdim, k.nr, k.nrh, k.nrchop, npc, k.chopzl, k.chopzu, nyt, zlo, zhi);
   HIPCHK(hipGetLastError());
 }
 

@@ -150,9 +150,10 @@ struct ProbeScope {
 };
 
 // active (n, m) pairs for a radial chop offset: sum over m < npchop of nrcs(m)
-static inline double active_pairs(const KitDev &k, int nroff) {
+static inline double active_pairs(const KitDev &k, int nroff, int ncols = -1) {
   double s = 0;
-  for (int m = 0; m < k.npchop; ++m) {
+  const int nc = (ncols > 0 && ncols < k.npchop) ? ncols : k.npchop;
+  for (int m = 0; m < nc; ++m) {
     int v = k.nrchop + nroff - m;
     s += v > 0 ? v : 0;
   }
@@ -172,12 +173,13 @@ void kit_free(KitDev &k);
 // dir -1 forward (PPP->PFP), +1 backward; out may equal in; rscale: +1 multiply
 // input rows i < nr by r_i (forward), -1 divide output rows by r_i (backward)
 void fft_phi(mlx_ctx *c, double2 *out, const double2 *in, int dir, int rscale = 0);
-// v x w (v_z + ub) to PFP with the r weights of vector_projection; false when
+// v x w (v_z + ub) to PFP with the r weights of vector_projection (modes
+// m < mmax written; mmax <= 0: all); false when
 // the fused form is not built for this np (the caller then runs vecprod and
 // three phi transforms)
 bool fft_phi_cross(mlx_ctx *c, double2 *o0, double2 *o1, double2 *o2, const double2 *v0, const double2 *v1,
                    const double2 *v2, const double2 *w0, const double2 *w1, const double2 *w2, const double2 *ub,
-                   double *ln0 = nullptr, double *ln1 = nullptr, double *ln2 = nullptr);
+                   double *ln0 = nullptr, double *ln1 = nullptr, double *ln2 = nullptr, int mmax = -1);
 // dir -1 forward (/nz), +1 backward.  ncol: the (row, column) lines the caller
 // needs transformed (the launch's algorithmic work).  act_nrc > 0: only rows
 // i < act_nrc - j of column j need transforming; ncols > 0: only columns
@@ -186,17 +188,21 @@ bool fft_phi_cross(mlx_ctx *c, double2 *o0, double2 *o1, double2 *o2, const doub
 void fft_z(mlx_ctx *c, double2 *e, int dir, double ncol, int act_nrc = 0, int ncols = 0);
 
 // legendre.hip
-void leg_forward(mlx_ctx *c, const double2 *in, double2 *out, int nroff, bool zero = true);
-void leg_backward(mlx_ctx *c, const double2 *in, double2 *out, int nroff, bool zero = true);
+// ncols > 0 (MFMA path): only columns m < ncols are transformed, the others
+// are written as zero (their input is zero or their output unused)
+void leg_forward(mlx_ctx *c, const double2 *in, double2 *out, int nroff, bool zero = true, int ncols = -1);
+void leg_backward(mlx_ctx *c, const double2 *in, double2 *out, int nroff, bool zero = true, int ncols = -1);
 // legendre_mfma.hip (FP32 MFMA; valid when mfma_ok)
 bool mfma_ok(const KitDev &k, int nroff);
 void mfma_tables(mlx_ctx *c);
-void leg_forward_f32(mlx_ctx *c, const double2 *in, double2 *out);
-void leg_backward_f32(mlx_ctx *c, const double2 *in, double2 *out);
+void leg_forward_f32(mlx_ctx *c, const double2 *in, double2 *out, int ncols = -1);
+void leg_backward_f32(mlx_ctx *c, const double2 *in, double2 *out, int ncols = -1);
 void vec2tp_project_f32(mlx_ctx *c, const double2 *ur, const double2 *up, const double2 *uz, double2 *psi,
-                        double2 *chi);
+                        double2 *chi,
+                        int ncols = -1, int zlo = 1, int zhi = 0);
 void vec2tp_project(mlx_ctx *c, const double2 *ur, const double2 *up, const double2 *uz,
-                    double2 *psi, double2 *chi, int nroff);
+                    double2 *psi, double2 *chi, int nroff, int ncols = -1,
+                    int zlo = 1, int zhi = 0);
 
 // spectral.hip
 void k_ln_column(mlx_ctx *c, double2 *e, const double *ln, double sign);
@@ -226,7 +232,7 @@ void k_tp2vec_combine(mlx_ctx *c, double2 *ur, double2 *up, const double2 *psi, 
 // chi's radial offset md_off, C = c_in = psi
 void k_tp2vec_front(mlx_ctx *c, double2 *ur, double2 *up, double2 *uz, const double2 *p_in, const double2 *c_in,
                     const double *p_ln, const double *c_ln, int mode, int md_off, double *ln_r = nullptr,
-                    double *ln_p = nullptr, double *ln_z = nullptr);
+                    double *ln_p = nullptr, double *ln_z = nullptr, int ncols = -1);
 void k_vecprod(mlx_ctx *c, double2 *vr, double2 *vp, double2 *vz, const double2 *ur, const double2 *up,
                const double2 *uz);
 void k_vec2tp_up_ln(mlx_ctx *c, double2 *up, const double2 *inf, double *psiln);
@@ -242,18 +248,26 @@ void k_copy_scalar(mlx_ctx *c, double *dst, const double *src, double scale);
 // 4 (nr/2) nz S flops; bytes = physical columns m < npchop read (or written)
 // once, spectral coefficients written (or read) once, table read once at
 // tbytes per entry.
-static inline double leg_flops(const KitDev &k, int nroff) {
-  return 4.0 * k.nrh * k.nzdim * active_pairs(k, nroff);
+// ncols > 0: only columns m < ncols are transformed
+static inline double leg_flops(const KitDev &k, int nroff, int ncols = -1) {
+  return 4.0 * k.nrh * k.nzdim * active_pairs(k, nroff, ncols);
 }
-static inline double leg_bytes(const KitDev &k, int nroff, int tbytes) {
-  double S = active_pairs(k, nroff);
-  return 16.0 * k.nr * k.npchop * k.nzdim + 16.0 * S * k.nzdim + (double)tbytes * k.nrh * S;
+static inline double leg_bytes(const KitDev &k, int nroff, int tbytes, int ncols = -1) {
+  const double S = active_pairs(k, nroff, ncols);
+  const int nc = (ncols > 0 && ncols < k.npchop) ? ncols : k.npchop;
+  return 16.0 * k.nr * nc * k.nzdim + 16.0 * S * k.nzdim + (double)tbytes * k.nrh * S;
 }
 // vec2tp projection: three physical fields in, two spectral fields out,
 // tables V, D, T
-static inline double v2t_bytes(const KitDev &k, int nroff, int tbytes) {
-  double S = active_pairs(k, nroff);
-  return 48.0 * k.nr * k.npchop * k.nzdim + 32.0 * S * k.nzdim + 3.0 * tbytes * k.nrh * S;
+// ncols > 0: columns m < ncols only; nzk: kappa slices computed (default all);
+// full: both output fields written in full (zeros outside the computed set)
+static inline double v2t_bytes(const KitDev &k, int nroff, int tbytes, int ncols = -1, int nzk = -1,
+                               bool full = false) {
+  const double S = active_pairs(k, nroff, ncols);
+  const int nc = (ncols > 0 && ncols < k.npchop) ? ncols : k.npchop;
+  const double zk = (nzk >= 0) ? nzk : k.nzdim;
+  const double out = full ? 32.0 * k.nrdim * k.npdim * k.nzdim : 32.0 * S * zk;
+  return 48.0 * k.nr * nc * zk + out + 3.0 * tbytes * k.nrh * S;
 }
 
 // XCD-aware work mapping.  Workgroups are dispatched round-robin over the 8

@@ -963,13 +963,15 @@ void k_tp2vec_combine(mlx_ctx *c, double2 *ur, double2 *up, const double2 *psi, 
 
 void k_tp2vec_front(mlx_ctx *c, double2 *ur, double2 *up, double2 *uz, const double2 *p_in, const double2 *c_in,
                     const double *p_ln, const double *c_ln, int mode, int md_off, double *ln_r, double *ln_p,
-                    double *ln_z) {
+                    double *ln_z, int ncols) {
   const KitDev &k = c->kit;
   if (k.nrchop + 3 > k.nrdim || k.nrchop + 3 > k.ne) MLX_THROW(MLX_E_ARG, "chop_index: chopping in r too large");
   if (k.nrchop > 768) MLX_THROW(MLX_E_ARG, "vector_reconstruction: nrchop above 768 is not supported");
   if (md_off < 0 || md_off > 3 || k.nrchop + md_off > k.nrdim)
     MLX_THROW(MLX_E_ARG, "curl_vector_reconstruction: chopping offset of chi out of range");
-  dim3 grid(k.npdim, k.nzdim);
+  // ncols > 0: columns m >= ncols are neither read nor written (their
+  // inputs are zero and the caller's transforms skip them)
+  dim3 grid((ncols > 0 && ncols < k.npdim) ? ncols : k.npdim, k.nzdim);
   size_t lds = 4 * sizeof(double2) * (size_t)k.nrdim;
   hipLaunchKernelGGL(k_tp2vec_front_, grid, dim3(256), lds, c->stream, ur, up, uz, p_in, c_in, p_ln, c_ln, mode,
                      md_off, k.d2c, k.xxc, k.lognorm, k.ak, k.nrdim, k.npdim, k.nzdim, k.ne, k.nrchop, k.npchop, k.chopzl,

@@ -19,7 +19,7 @@ def _say(*a):
     print('[case]', *a, flush=True)
 
 
-@pytest.fixture(scope='module', params=['C1', 'ODD'])
+@pytest.fixture(scope='module', params=['C1', 'ODD', 'POW2'])
 def setup(request):
     import mlegs_amd as M
     from oracle import kit as K
