@@ -193,21 +193,6 @@ void febe(mlx_ctx *c, mlx_field *s, const mlx_field *nl, double dt) {
   ihelm(c, s, a);
 }
 
-// abcn (mlegs_scalar_ops.f90:1200-1262); copy_prev = false skips s_p <- s
-// (s_p is never read by the stepper) and exchanges nl_p <- nl by buffer swap
-// when nl may be overwritten afterwards (swap_nl).
-void abcn(mlx_ctx *c, mlx_field *s, mlx_field *s_p, mlx_field *nl, mlx_field *nl_p, double dt, bool copy_prev,
-          bool swap_nl) {
-  check_step_inputs(c, {s, nl, s_p, nl_p});
-  if (c->p.visc < 5e-14) MLX_THROW(MLX_E_ARG, "abcn: inviscid case and no linear term in rhs. semi-implicit time adv is impossible");
-  double a = -2.0 / (dt * c->p.visc);
-  k_abcn_rhs_d2(c, s->e, s->sc, nl->e, nl->sc, nl_p->e, nl_p->sc, dt, a, s->off[0]);
-  ihelm(c, s, a);
-  if (copy_prev) fcopy(c, s_p, s);
-  if (swap_nl) fmove(nl_p, nl);
-  else fcopy(c, nl_p, nl);
-}
-
 // The modes de-aliasing zeroes (dealias, mlegs_scalar_ops.f90:305-352):
 // columns m >= pcut and kappa slices zlo..zhi.  When the advection term's
 // inputs are de-aliased and its result is de-aliased anyway, the MFMA path
@@ -232,6 +217,33 @@ DealiasLimits dealias_limits(mlx_ctx *c) {
   return d;
 }
 
+// abcn (mlegs_scalar_ops.f90:1200-1262); copy_prev = false skips s_p <- s
+// (s_p is never read by the stepper) and exchanges nl_p <- nl by buffer swap
+// when nl may be overwritten afterwards (swap_nl).
+// dl (the time loop, which de-aliases s right after): the modes de-aliasing
+// zeroes are neither formed nor solved
+void abcn(mlx_ctx *c, mlx_field *s, mlx_field *s_p, mlx_field *nl, mlx_field *nl_p, double dt, bool copy_prev,
+          bool swap_nl, const DealiasLimits *dl = nullptr) {
+  check_step_inputs(c, {s, nl, s_p, nl_p});
+  if (c->p.visc < 5e-14) MLX_THROW(MLX_E_ARG, "abcn: inviscid case and no linear term in rhs. semi-implicit time adv is impossible");
+  double a = -2.0 / (dt * c->p.visc);
+  const bool lim = dl && dl->on;
+  k_abcn_rhs_d2(c, s->e, s->sc, nl->e, nl->sc, nl_p->e, nl_p->sc, dt, a, s->off[0], lim ? dl->pcut : -1,
+                lim ? dl->zlo : 1, lim ? dl->zhi : 0);
+  if (lim) {
+    need_kit(c);
+    to_fff(c, s);
+    if (std::fabs(a) < 5e-14) MLX_THROW(MLX_E_ARG, "ihelm: alpha equals to zero. Inversion of 0*identity is impossible");
+    k_helm_pre(c, s->e, s->sc, s->sc, a);
+    k_helm_solve(c, s->e, a, 0, s->off[0], dl->pcut, dl->zlo, dl->zhi);
+  } else {
+    ihelm(c, s, a);
+  }
+  if (copy_prev) fcopy(c, s_p, s);
+  if (swap_nl) fmove(nl_p, nl);
+  else fcopy(c, nl_p, nl);
+}
+
 // Back half of tp2vec: the three FFF components (offset 0, no log term) to
 // PPP; ur and up are divided by r in the last phi transform.  ncols > 0: the
 // components are zero in columns m >= ncols, which the transforms skip.
@@ -241,13 +253,16 @@ void tp2vec_back(mlx_ctx *c, mlx_field *ur, mlx_field *up, mlx_field *uz, int nc
   const KitDev &k = c->kit;
   const int rs[3] = {-1, -1, 0};
   const int nc = (ncols > 0 && ncols < k.npchop) ? ncols : k.npchop;
+  // limited columns and the four-step phi kernels: the radial transform
+  // leaves columns m >= nc unwritten and the phi transform takes them as zero
+  const bool lean = nc < k.npchop && fft_phi_fast(k);
   int q = 0;
   for (mlx_field *f : {ur, up, uz}) {
     f->off[0] = f->off[1] = f->off[2] = 0;
     if (k.nz > 1) fft_z(c, f->e, +1, active_pairs(k, 0, nc), k.nrchop, nc);
-    leg_backward(c, f->e, c->scr[0], 0, true, nc);
+    leg_backward(c, f->e, c->scr[0], 0, true, nc, !lean);
     swap_scr(c, f);
-    if (k.np > 1) fft_phi(c, f->e, f->e, +1, rs[q]);
+    if (k.np > 1) fft_phi(c, f->e, f->e, +1, rs[q], lean ? nc : -1);
     else if (rs[q] < 0) k_scale_rows_r(c, f->e, 1);
     set_space(f, "PPP");
     ++q;
@@ -903,9 +918,10 @@ int mlx_vortex_step(mlx_ctx *c, mlx_vortex *v, int32_t nsteps, double dt, int32_
     mlx_field *psi = v->f[0], *chi = v->f[1], *nlpsi = v->f[2], *nlchi = v->f[3];
     mlx_field *psi_prev = v->f[4], *chi_prev = v->f[5], *nlpsi_prev = v->f[6], *nlchi_prev = v->f[7];
     mlx_field *uz = v->f[8];
+    const DealiasLimits dl = dealias_limits(c);
     for (int it = 0; it < nsteps; ++it) {
-      abcn(c, psi, psi_prev, nlpsi, nlpsi_prev, dt, false, true);
-      abcn(c, chi, chi_prev, nlchi, nlchi_prev, dt, false, true);
+      abcn(c, psi, psi_prev, nlpsi, nlpsi_prev, dt, false, true, &dl);
+      abcn(c, chi, chi_prev, nlchi, nlchi_prev, dt, false, true, &dl);
       v->prev_stale = true;
       dealias(c, psi);
       dealias(c, chi);

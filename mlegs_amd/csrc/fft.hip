@@ -310,10 +310,11 @@ __device__ __forceinline__ double2 twid(const double2 *tw, int idx, double sg) {
 }
 
 // MODE 0: complex transform (z); MODE 1: real forward (phi); MODE 2: real backward (phi)
+// MODE 2: modes m >= mlim are taken as zero without being read
 template <int N1, int N2, int TI, int MODE>
 __global__ void __launch_bounds__(kThreads) k_fft4(double2 *out, const double2 *in, size_t es, size_t gs, int nrow,
                                                    int act_nrc, const double2 *tw, const double2 *twh, double sg,
-                                                   double scale, const double *r, int nr, int rscale) {
+                                                   double scale, const double *r, int nr, int rscale, int mlim) {
   constexpr int N = N1 * N2;
   constexpr int T1 = (TI * N2 + kThreads - 1) / kThreads;
   constexpr int T2 = (TI * N1 + kThreads - 1) / kThreads;
@@ -333,7 +334,7 @@ __global__ void __launch_bounds__(kThreads) k_fft4(double2 *out, const double2 *
     // stage 0: C_m, m = 0..N, into LDS (C_N apart)
     for (int b = tid; b < TI * (N + 1); b += kThreads) {
       const int l = b % TI, m = b / TI, i = i0 + l;
-      const double2 v = (i < rlim) ? in[base + (size_t)m * es + i] : make_double2(0.0, 0.0);
+      const double2 v = (i < rlim && m < mlim) ? in[base + (size_t)m * es + i] : make_double2(0.0, 0.0);
       if (m < N) buf[m * TI + l] = v;
       else nyq[l] = v;
     }
@@ -589,13 +590,14 @@ __global__ void __launch_bounds__(kThreads) k_fft4_cross(double2 *o0, double2 *o
 
 template <int MODE>
 bool launch_fft4(mlx_ctx *c, int N, dim3 grid_y, double2 *out, const double2 *in, size_t es, size_t gs, int nrow,
-                 int act_nrc, const double2 *tw, const double2 *twh, double sg, double scale, int rscale) {
+                 int act_nrc, const double2 *tw, const double2 *twh, double sg, double scale, int rscale,
+                 int mlim = 1 << 30) {
   const KitDev &k = c->kit;
 #define MLX_FFT4(N1_, N2_, TI_)                                                                               \
   do {                                                                                                        \
     dim3 grid((nrow + TI_ - 1) / TI_, grid_y.y);                                                              \
     hipLaunchKernelGGL((k_fft4<N1_, N2_, TI_, MODE>), grid, dim3(kThreads), 0, c->stream, out, in, es, gs, nrow, \
-                       act_nrc, tw, twh, sg, scale, k.r, k.nr, rscale);                                      \
+                       act_nrc, tw, twh, sg, scale, k.r, k.nr, rscale, mlim);                               \
     return true;                                                                                              \
   } while (0)
   switch (N) {
@@ -623,20 +625,27 @@ int tile_rows(int n_line, int nrdim) {
 
 namespace mlx {
 
-void fft_phi(mlx_ctx *c, double2 *out, const double2 *in, int dir, int rscale) {
+bool fft_phi_fast(const KitDev &k) {
+  const int N = k.nph;
+  return k.np > 1 && (N == 16 || N == 32 || N == 64 || N == 128 || N == 256);
+}
+
+void fft_phi(mlx_ctx *c, double2 *out, const double2 *in, int dir, int rscale, int mlim) {
   const KitDev &k = c->kit;
   if (k.np <= 1) return;
+  if (mlim > 0 && dir > 0 && !fft_phi_fast(k)) MLX_THROW(MLX_E_ARG, "fft_phi: mode limit without the fast path");
   if (k.nph > 512) MLX_THROW(MLX_E_ARG, "fft_phi: np/2 above 512 is not supported");
   int TI = tile_rows(k.nph, k.nrdim);
   dim3 grid((k.nrdim + TI - 1) / TI, k.nzdim);
   // packed reals in, nph+1 modes out (or back): nr rows x nz slices
-  ProbeScope probe(c, MLX_PROBE_FFT_PHI, 16.0 * k.nr * k.nz * (2.0 * k.nph + 1),
+  const double nread = (mlim > 0 && mlim < k.nph + 1) ? mlim : k.nph + 1;
+  ProbeScope probe(c, MLX_PROBE_FFT_PHI, 16.0 * k.nr * k.nz * (dir > 0 ? nread + k.nph : 2.0 * k.nph + 1),
                    2.5 * k.np * std::log2((double)k.np) * k.nr * k.nz);
   const size_t plane = (size_t)k.nrdim * k.npdim;
   bool fast = (dir < 0) ? launch_fft4<1>(c, k.nph, dim3(1, k.nzdim), out, in, k.nrdim, plane, k.nrdim, 0, k.tw_nph,
                                          k.tw_half, -1.0, 1.0 / k.np, rscale)
                         : launch_fft4<2>(c, k.nph, dim3(1, k.nzdim), out, in, k.nrdim, plane, k.nrdim, 0, k.tw_nph,
-                                         k.tw_half, +1.0, 1.0, rscale);
+                                         k.tw_half, +1.0, 1.0, rscale, mlim > 0 ? mlim : 1 << 30);
   if (fast) {
     HIPCHK(hipGetLastError());
     return;

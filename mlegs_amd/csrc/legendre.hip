@@ -333,11 +333,11 @@ void leg_forward(mlx_ctx *c, const double2 *in, double2 *out, int nroff, bool ze
   HIPCHK(hipGetLastError());
 }
 
-void leg_backward(mlx_ctx *c, const double2 *in, double2 *out, int nroff, bool zero, int ncols) {
+void leg_backward(mlx_ctx *c, const double2 *in, double2 *out, int nroff, bool zero, int ncols, bool ztail) {
   const KitDev &k = c->kit;
   if (k.nrchop + nroff > k.nrdim) MLX_THROW(MLX_E_ARG, "rtrans_backward: chopping in r too large");
   if (c->prec != MLX_PREC_FP64 && mfma_ok(k, nroff)) {
-    leg_backward_f32(c, in, out, ncols);
+    leg_backward_f32(c, in, out, ncols, ztail);
     return;
   }
   if (zero) HIPCHK(hipMemsetAsync(out, 0, sizeof(double2) * c->nelem, c->stream));

@@ -582,11 +582,12 @@ void leg_forward_f32(mlx_ctx *c, const double2 *in, double2 *out, int ncols) {
   HIPCHK(hipGetLastError());
 }
 
-void leg_backward_f32(mlx_ctx *c, const double2 *in, double2 *out, int ncols) {
+void leg_backward_f32(mlx_ctx *c, const double2 *in, double2 *out, int ncols, bool ztail) {
   const KitDev &k = c->kit;
   const int nyt = (k.nzdim + KT - 1) / KT;
   const int npc = (ncols > 0 && ncols < k.npchop) ? ncols : k.npchop;
-  dim3 grid = xcd_grid(k.npdim, nyt);
+  // without the zero tail only the computed columns get workgroups
+  dim3 grid = xcd_grid(ztail ? k.npdim : npc, nyt);
   int xt = (k.nrh + 63) / 64;
   ProbeScope probe(c, MLX_PROBE_LEG_BWD, leg_bytes(k, 0, 4, npc), leg_flops(k, 0, npc));
   if (xt == 1)

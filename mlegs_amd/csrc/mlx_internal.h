@@ -87,6 +87,7 @@ struct LuCache {
   int *anyp = nullptr;       // per system: any interchange
   double fac_bytes = 0.0;    // factor bytes one solve reads
   int64_t nsys = 0, npiv = 0; // systems, systems with interchanges
+  std::vector<uint8_t> pivh;   // per system (kk * npchop + m): interchanges
   uint64_t stamp = 0;
 };
 
@@ -172,7 +173,10 @@ void kit_free(KitDev &k);
 // fft.hip
 // dir -1 forward (PPP->PFP), +1 backward; out may equal in; rscale: +1 multiply
 // input rows i < nr by r_i (forward), -1 divide output rows by r_i (backward)
-void fft_phi(mlx_ctx *c, double2 *out, const double2 *in, int dir, int rscale = 0);
+// mlim > 0 (backward, fast path only): modes m >= mlim are taken as zero
+// without being read
+void fft_phi(mlx_ctx *c, double2 *out, const double2 *in, int dir, int rscale = 0, int mlim = -1);
+bool fft_phi_fast(const KitDev &k);  // the four-step phi kernels cover np/2
 // v x w (v_z + ub) to PFP with the r weights of vector_projection (modes
 // m < mmax written; mmax <= 0: all); false when
 // the fused form is not built for this np (the caller then runs vecprod and
@@ -191,12 +195,14 @@ void fft_z(mlx_ctx *c, double2 *e, int dir, double ncol, int act_nrc = 0, int nc
 // ncols > 0 (MFMA path): only columns m < ncols are transformed, the others
 // are written as zero (their input is zero or their output unused)
 void leg_forward(mlx_ctx *c, const double2 *in, double2 *out, int nroff, bool zero = true, int ncols = -1);
-void leg_backward(mlx_ctx *c, const double2 *in, double2 *out, int nroff, bool zero = true, int ncols = -1);
+void leg_backward(mlx_ctx *c, const double2 *in, double2 *out, int nroff, bool zero = true, int ncols = -1,
+                  bool ztail = true);
 // legendre_mfma.hip (FP32 MFMA; valid when mfma_ok)
 bool mfma_ok(const KitDev &k, int nroff);
 void mfma_tables(mlx_ctx *c);
 void leg_forward_f32(mlx_ctx *c, const double2 *in, double2 *out, int ncols = -1);
-void leg_backward_f32(mlx_ctx *c, const double2 *in, double2 *out, int ncols = -1);
+// ztail = false: columns m >= ncols are left unwritten
+void leg_backward_f32(mlx_ctx *c, const double2 *in, double2 *out, int ncols = -1, bool ztail = true);
 void vec2tp_project_f32(mlx_ctx *c, const double2 *ur, const double2 *up, const double2 *uz, double2 *psi,
                         double2 *chi,
                         int ncols = -1, int zlo = 1, int zhi = 0);
@@ -214,7 +220,10 @@ void k_zeroat1(mlx_ctx *c, double2 *e, int nrc);
 void k_delsqp(mlx_ctx *c, double2 *e, double *ln, int nroff, int inverse);
 void k_bandmul(mlx_ctx *c, double2 *e, double *ln, int op, int nroff);  // op 0 del2, 1 del2h, 2 xxdx
 void k_helm_pre(mlx_ctx *c, double2 *e, double *ln_out, const double *ln_in, double alpha);
-void k_helm_solve(mlx_ctx *c, double2 *e, double alpha, int kind, int nroff);
+// pcut > 0: columns m >= pcut and kappa slices zlo..zhi are left unsolved
+// (for a caller that de-aliases the result)
+void k_helm_solve(mlx_ctx *c, double2 *e, double alpha, int kind, int nroff, int pcut = -1, int zlo = 1,
+                  int zhi = 0);
 void lu_cache_free(mlx_ctx *c);  // solve.hip
 void k_idel2_post(mlx_ctx *c, const double2 *e, double *ln);
 void k_idel2_preln00(mlx_ctx *c, double2 *e, double *ln, double preln, int nroff);
@@ -225,7 +234,8 @@ void k_abcn_rhs(mlx_ctx *c, double2 *sh, double *shln, const double2 *s, const d
                 const double2 *svis, double dt, double a, int ab2);
 // abcn's right-hand side in place on e (= s), with svis = del2(s) formed per column
 void k_abcn_rhs_d2(mlx_ctx *c, double2 *e, double *ln, const double2 *nl, const double *nlln, const double2 *nlp,
-                   const double *nlpln, double dt, double a, int nroff);
+                   const double *nlpln, double dt, double a, int nroff, int pcut = -1,
+                   int zlo = 1, int zhi = 0);
 void k_scale_rows_r(mlx_ctx *c, double2 *e, int inverse);
 void k_tp2vec_combine(mlx_ctx *c, double2 *ur, double2 *up, const double2 *psi, const double2 *uz, int nroff);
 // mode 0: (P, C) = (psi, chi); mode 1: P = -del2(p_in = chi) over the rows of

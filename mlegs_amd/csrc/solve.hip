@@ -467,8 +467,10 @@ __device__ __forceinline__ void solve_rows(double2 (&b)[S], const FacPtrs &F, co
 template <int S, bool PIV>
 __global__ void __launch_bounds__(64) k_lu_solve(double2 *e, FacPtrs F, const uint64_t *__restrict__ pw,
                                                  const int *__restrict__ anyp, int nrdim, int npdim, int nz, int nrc,
-                                                 int npc, int nzc, int nzcu, int skip00) {
+                                                 int npc, int nzc, int nzcu, int skip00, int zlo, int zhi) {
   const int m = blockIdx.x, kk = blockIdx.y;
+  // slices zlo..zhi left unsolved (both kk and nz - kk in the range)
+  if (kk >= zlo && kk <= zhi && (kk == 0 || 2 * kk == nz || (nz - kk >= zlo && nz - kk <= zhi))) return;
   const int lane = threadIdx.x, s = lane % NSEG;
   const bool twin = lane >= NSEG;
   if (m >= npc) return;
@@ -595,12 +597,14 @@ LuCache &lu_get(mlx_ctx *c, double alpha, int kind, int nroff) {
   // interchange words for systems with interchanges
   e.fac_bytes = 0.0;
   e.npiv = 0;
+  e.pivh.assign(nsys, 0);
   for (int kk = 0; kk < nkk; ++kk)
     for (int m = 0; m < k.npchop; ++m) {
       const int n = nrc - m > 0 ? nrc - m : 0;
       const bool p = anyp[(size_t)kk * k.npchop + m] != 0;
       e.fac_bytes += (double)n * (p ? 56.0 : 40.0) + (p ? 8.0 * NSEG : 0.0);
       e.npiv += p ? 1 : 0;
+      e.pivh[(size_t)kk * k.npchop + m] = p ? 1 : 0;
     }
   e.nsys = (int64_t)nsys;
   e.stamp = ++c->lu_clock;
@@ -612,19 +616,35 @@ LuCache &lu_get(mlx_ctx *c, double alpha, int kind, int nroff) {
 
 namespace mlx {
 
-void k_helm_solve(mlx_ctx *c, double2 *e, double alpha, int kind, int nroff) {
+void k_helm_solve(mlx_ctx *c, double2 *e, double alpha, int kind, int nroff, int pcut, int zlo, int zhi) {
   const KitDev &k = c->kit;
   LuCache &lu = lu_get(c, alpha, kind, nroff);
   const int nkk = (k.nzdim > 1) ? k.nzdim / 2 + 1 : 1;
-  dim3 grid(k.npchop, nkk);
-  // right-hand sides read and written once, factors of every |kappa| read once
-  const double S = active_pairs(k, nroff);
-  ProbeScope probe(c, MLX_PROBE_LU_SOLVE, 32.0 * S * k.nzdim + lu.fac_bytes, 2.0 * 4.0 * 5.0 * S * k.nzdim);
+  const int npc_grid = (pcut > 0 && pcut < k.npchop) ? pcut : k.npchop;
+  dim3 grid(npc_grid, nkk);
+  // right-hand sides of the solved systems read and written once, their
+  // factors (one set per |kappa|) read once
+  const int nrc0 = k.nrchop + nroff;
+  auto in_band = [&](int kz) { return kz >= zlo && kz <= zhi; };
+  double rows = 0.0, fbytes = 0.0;
+  for (int kk = 0; kk < nkk; ++kk) {
+    int nk = 0;  // slices kk and nz - kk that are solved
+    if (!in_band(kk)) ++nk;
+    if (kk != 0 && 2 * kk != k.nzdim && !in_band(k.nzdim - kk)) ++nk;
+    if (nk == 0) continue;
+    for (int m = 0; m < npc_grid; ++m) {
+      const int n = nrc0 - m > 0 ? nrc0 - m : 0;
+      const bool pv = lu.pivh[(size_t)kk * k.npchop + m] != 0;
+      rows += (double)n * nk;
+      fbytes += (double)n * (pv ? 56.0 : 40.0) + (pv ? 8.0 * NSEG : 0.0);
+    }
+  }
+  ProbeScope probe(c, MLX_PROBE_LU_SOLVE, 32.0 * rows + fbytes, 2.0 * 4.0 * 5.0 * rows);
   const FacPtrs F = fac_ptrs(lu.sol, fac_rows(k, lu.seg));
   const int nrc = k.nrchop + nroff, sk = kind == 2 ? 1 : 0;
 #define MLX_LU_LAUNCH(SV, PV)                                                                              \
   hipLaunchKernelGGL((k_lu_solve<SV, PV>), grid, dim3(64), 0, c->stream, e, F, lu.pvw, lu.anyp, k.nrdim, k.npdim, \
-                     k.nzdim, nrc, k.npchop, k.chopzl, k.chopzu, sk)
+                     k.nzdim, nrc, k.npchop, k.chopzl, k.chopzu, sk, zlo, zhi)
 #define MLX_LU_BOTH(SV)                               \
   do {                                                \
     if (lu.npiv < lu.nsys) MLX_LU_LAUNCH(SV, false); \

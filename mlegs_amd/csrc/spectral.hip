@@ -433,10 +433,13 @@ __global__ void __launch_bounds__(256) k_abcn_rhs_d2_(double2 *e, const double *
                                                       const double2 *nlp, const double *d2c, const double *lognorm,
                                                       const double *ak, int nrdim, int npdim, int ne, int nrc,
                                                       int npc, int nzc, int nzcu, double ell, double dt, double a,
-                                                      double visc) {
+                                                      double visc, int pcut, int zlo, int zhi) {
   extern __shared__ double2 lds[];
   double2 *S = lds, *V = lds + nrdim, *T = lds + 2 * nrdim;
   const int m = blockIdx.x, k = blockIdx.y;
+  // columns m >= pcut and slices zlo..zhi: left as they are (the caller
+  // de-aliases the result)
+  if (m >= pcut || (k >= zlo && k <= zhi)) return;
   const size_t off = (size_t)k * nrdim * npdim + (size_t)nrdim * m;
   const int cnt = z_count(k, nzc, nzcu);
   const int nn = (cnt > 0) ? nrcs_dev(nrc, npc, m) : 0;
@@ -936,15 +939,16 @@ void k_abcn_rhs(mlx_ctx *c, double2 *sh, double *shln, const double2 *s, const d
 }
 
 void k_abcn_rhs_d2(mlx_ctx *c, double2 *e, double *ln, const double2 *nl, const double *nlln, const double2 *nlp,
-                   const double *nlpln, double dt, double a, int nroff) {
+                   const double *nlpln, double dt, double a, int nroff, int pcut, int zlo, int zhi) {
   const KitDev &k = c->kit;
   const int nrc = k.nrchop + nroff;
   if (nrc > k.nrdim) MLX_THROW(MLX_E_ARG, "chop_index: chopping in r too large");
-  dim3 grid(k.npdim, k.nzdim);
+  if (pcut <= 0 || pcut > k.npdim) pcut = k.npdim;
+  dim3 grid(pcut, k.nzdim);
   size_t lds = 3 * sizeof(double2) * (size_t)k.nrdim;
   // the log coefficient first: the data kernel reads s's ln for the (0,0) correction
   hipLaunchKernelGGL(k_abcn_rhs_d2_, grid, dim3(256), lds, c->stream, e, ln, nl, nlp, k.d2c, k.lognorm, k.ak, k.nrdim,
-                     k.npdim, k.ne, nrc, k.npchop, k.chopzl, k.chopzu, k.ell, dt, a, c->p.visc);
+                     k.npdim, k.ne, nrc, k.npchop, k.chopzl, k.chopzu, k.ell, dt, a, c->p.visc, pcut, zlo, zhi);
   hipLaunchKernelGGL(k_abcn_ln_, dim3(1), dim3(1), 0, c->stream, ln, ln, nlln, nlpln, dt, a);
   HIPCHK(hipGetLastError());
 }
