@@ -249,7 +249,8 @@ void abcn(mlx_ctx *c, mlx_field *s, mlx_field *s_p, mlx_field *nl, mlx_field *nl
 // components are zero in columns m >= ncols, which the transforms skip.
 // The front kernel has zeroed the log terms, so trans's log-column step
 // (adding 0 * ln x) is left out; the other steps are trans's FFF -> PPP.
-void tp2vec_back(mlx_ctx *c, mlx_field *ur, mlx_field *up, mlx_field *uz, int ncols = -1) {
+void tp2vec_back(mlx_ctx *c, mlx_field *ur, mlx_field *up, mlx_field *uz, int ncols = -1, int zlo = 1,
+                 int zhi = 0) {
   const KitDev &k = c->kit;
   const int rs[3] = {-1, -1, 0};
   const int nc = (ncols > 0 && ncols < k.npchop) ? ncols : k.npchop;
@@ -259,7 +260,8 @@ void tp2vec_back(mlx_ctx *c, mlx_field *ur, mlx_field *up, mlx_field *uz, int nc
   int q = 0;
   for (mlx_field *f : {ur, up, uz}) {
     f->off[0] = f->off[1] = f->off[2] = 0;
-    if (k.nz > 1) fft_z(c, f->e, +1, active_pairs(k, 0, nc), k.nrchop, nc);
+    // the de-aliased kappa slices zlo..zhi of the input are zero
+    if (k.nz > 1) fft_z(c, f->e, +1, active_pairs(k, 0, nc), k.nrchop, nc, zlo, zhi);
     leg_backward(c, f->e, c->scr[0], 0, true, nc, !lean);
     swap_scr(c, f);
     if (k.np > 1) fft_phi(c, f->e, f->e, +1, rs[q], lean ? nc : -1);
@@ -273,12 +275,12 @@ void tp2vec_back(mlx_ctx *c, mlx_field *ur, mlx_field *up, mlx_field *uz, int nc
 // band products, the combination and the chop run in one column pass
 // ncols > 0: psi and chi are zero in columns m >= ncols (de-aliased)
 void tp2vec(mlx_ctx *c, const mlx_field *psi, const mlx_field *chi, mlx_field *vr, mlx_field *vp, mlx_field *vz,
-            int ncols = -1) {
+            int ncols = -1, int zlo = 1, int zhi = 0) {
   need_kit(c);
   if (!is_fff(psi) || !is_fff(chi)) MLX_THROW(MLX_E_LAYOUT, "vector_projection: psi and chi must be in FFF");
   mlx_field *ur = tmp(c, 0), *up = tmp(c, 1), *uz = tmp(c, 2);
   k_tp2vec_front(c, ur->e, up->e, uz->e, psi->e, chi->e, psi->sc, chi->sc, 0, 0, ur->sc, up->sc, uz->sc, ncols);
-  tp2vec_back(c, ur, up, uz, ncols);
+  tp2vec_back(c, ur, up, uz, ncols, zlo, zhi);
   fmove(vr, ur);
   fmove(vp, up);
   fmove(vz, uz);
@@ -287,14 +289,14 @@ void tp2vec(mlx_ctx *c, const mlx_field *psi, const mlx_field *chi, mlx_field *v
 // tp2curlvec (curl_vector_reconstruction, :1547-1560): curl V = tp2vec(-del2 chi, psi),
 // with -del2 chi formed inside the same column pass
 void tp2curlvec(mlx_ctx *c, const mlx_field *psi, const mlx_field *chi, mlx_field *wr, mlx_field *wp,
-                mlx_field *wz, int ncols = -1) {
+                mlx_field *wz, int ncols = -1, int zlo = 1, int zhi = 0) {
   need_kit(c);
   if (!is_fff(psi) || !is_fff(chi)) MLX_THROW(MLX_E_LAYOUT, "vector_projection: psi and chi must be in FFF");
   mlx_field *ur = tmp(c, 0), *up = tmp(c, 1), *uz = tmp(c, 2);
   // del2 of md = chi runs over the rows of chi's own radial offset
   k_tp2vec_front(c, ur->e, up->e, uz->e, chi->e, psi->e, chi->sc, psi->sc, 1, chi->off[0], ur->sc, up->sc,
                  uz->sc, ncols);
-  tp2vec_back(c, ur, up, uz, ncols);
+  tp2vec_back(c, ur, up, uz, ncols, zlo, zhi);
   fmove(wr, ur);
   fmove(wp, up);
   fmove(wz, uz);
@@ -342,20 +344,23 @@ void vec2tp_pfp(mlx_ctx *c, mlx_field *ur, mlx_field *up, mlx_field *uz, mlx_fie
   const int nc = dl.on ? dl.pcut : k.npchop;
   // psi's log term is set by k_vec2tp_up_ln and chi's by idel2 below
   // the projection reads columns m < npchop (nc) only
-  if (k.nz > 1) fft_z(c, ur->e, -1, (double)k.nr * nc, 0, nc);
+  // outputs in the kappa slices the projection skips are not written
+  const int zlo = dl.on ? dl.zlo : 1, zhi = dl.on ? dl.zhi : 0;
+  if (k.nz > 1) fft_z(c, ur->e, -1, (double)k.nr * nc, 0, nc, 1, 0, zlo, zhi);
   // up to FFF as trans does it (PFP -> FFP -> FFF), over columns m < nc
   k_ln_column(c, up->e, up->sc, -1.0);
   leg_forward(c, up->e, c->scr[0], up->off[0], true, nc);
   swap_scr(c, up);
-  if (k.nz > 1) fft_z(c, up->e, -1, active_pairs(k, up->off[0], nc), k.nrchop + up->off[0], nc);
+  if (k.nz > 1)
+    fft_z(c, up->e, -1, active_pairs(k, up->off[0], nc), k.nrchop + up->off[0], nc, 1, 0, zlo, zhi);
   set_space(up, "FFF");
   double2 *inf = reinterpret_cast<double2 *>(c->dsmall + 2048);
   k_calcat(c, up->e, k.at1, k.nrchop + up->off[0], inf);
   leg_backward(c, up->e, c->scr[0], up->off[0], true, nc);
   swap_scr(c, up);
   k_vec2tp_up_ln(c, up->e, inf, psi->sc);
-  if (k.nz > 1) fft_z(c, uz->e, -1, (double)k.nr * nc, 0, nc);
-  vec2tp_project(c, ur->e, up->e, uz->e, psi->e, chi->e, 3, nc, dl.on ? dl.zlo : 1, dl.on ? dl.zhi : 0);
+  if (k.nz > 1) fft_z(c, uz->e, -1, (double)k.nr * nc, 0, nc, 1, 0, zlo, zhi);
+  vec2tp_project(c, ur->e, up->e, uz->e, psi->e, chi->e, 3, nc, zlo, zhi);
   set_space(psi, "FFF");
   set_space(chi, "FFF");
   if (pre_dealias && !dl.on) {
@@ -401,8 +406,8 @@ void advection_rhs(mlx_ctx *c, mlx_field *psi, mlx_field *chi, mlx_field *nlpsi,
   // psi and chi are de-aliased here and so is the result: the MFMA path
   // skips the columns and slices de-aliasing zeroes
   const DealiasLimits dl = dealias_limits(c);
-  tp2vec(c, psi, chi, vr, vp, vz, dl.on ? dl.pcut : -1);
-  tp2curlvec(c, psi, chi, wr, wp, wz, dl.on ? dl.pcut : -1);
+  tp2vec(c, psi, chi, vr, vp, vz, dl.on ? dl.pcut : -1, dl.zlo, dl.zhi);
+  tp2curlvec(c, psi, chi, wr, wp, wz, dl.on ? dl.pcut : -1, dl.zlo, dl.zhi);
   // de-aliased inside vec2tp before idel2; chop and zeroat1 keep zero modes
   // zero, so the closing dealias of the reference is already satisfied
   mlx_field *ur = tmp(c, 0), *up = tmp(c, 1), *uzp = tmp(c, 2);

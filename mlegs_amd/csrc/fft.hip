@@ -310,11 +310,14 @@ __device__ __forceinline__ double2 twid(const double2 *tw, int idx, double sg) {
 }
 
 // MODE 0: complex transform (z); MODE 1: real forward (phi); MODE 2: real backward (phi)
-// MODE 2: modes m >= mlim are taken as zero without being read
+// MODE 2: modes m >= mlim are taken as zero without being read.  MODE 0:
+// line entries zin_lo..zin_hi are taken as zero without being read and
+// outputs zout_lo..zout_hi are not written (their consumer does not read them)
 template <int N1, int N2, int TI, int MODE>
 __global__ void __launch_bounds__(kThreads) k_fft4(double2 *out, const double2 *in, size_t es, size_t gs, int nrow,
                                                    int act_nrc, const double2 *tw, const double2 *twh, double sg,
-                                                   double scale, const double *r, int nr, int rscale, int mlim) {
+                                                   double scale, const double *r, int nr, int rscale, int mlim,
+                                                   int zin_lo, int zin_hi, int zout_lo, int zout_hi) {
   constexpr int N = N1 * N2;
   constexpr int T1 = (TI * N2 + kThreads - 1) / kThreads;
   constexpr int T2 = (TI * N1 + kThreads - 1) / kThreads;
@@ -363,7 +366,8 @@ __global__ void __launch_bounds__(kThreads) k_fft4(double2 *out, const double2 *
           const double2 O = cmul(csub(cm, cc), w);
           v[n1] = make_double2(E.x - O.y, E.y + O.x);
         } else {
-          double2 x = (i < rlim) ? in[base + (size_t)j * es + i] : make_double2(0.0, 0.0);
+          const bool zin = MODE == 0 && j >= zin_lo && j <= zin_hi;
+          double2 x = (i < rlim && !zin) ? in[base + (size_t)j * es + i] : make_double2(0.0, 0.0);
           if (MODE == 1 && rscale > 0 && i < nr) {
             const double rr = r[i];
             x = make_double2(x.x * rr, x.y * rr);
@@ -416,10 +420,12 @@ __global__ void __launch_bounds__(kThreads) k_fft4(double2 *out, const double2 *
           if (MODE == 2 && rscale < 0 && i < nr) dv = r[i];
 #pragma unroll
           for (int k2 = 0; k2 < N2; ++k2) {
+            const int ko = k1 + N1 * k2;
+            if (MODE == 0 && ko >= zout_lo && ko <= zout_hi) continue;
             double2 x = y2[t][k2];
             if (MODE == 0) x = make_double2(x.x * scale, x.y * scale);
             else if (dv != 1.0) x = make_double2(x.x / dv, x.y / dv);
-            out[base + (size_t)(k1 + N1 * k2) * es + i] = x;
+            out[base + (size_t)ko * es + i] = x;
           }
           if (MODE == 2 && k1 == 0) out[base + (size_t)N * es + i] = make_double2(0.0, 0.0);
         }
@@ -591,13 +597,14 @@ __global__ void __launch_bounds__(kThreads) k_fft4_cross(double2 *o0, double2 *o
 template <int MODE>
 bool launch_fft4(mlx_ctx *c, int N, dim3 grid_y, double2 *out, const double2 *in, size_t es, size_t gs, int nrow,
                  int act_nrc, const double2 *tw, const double2 *twh, double sg, double scale, int rscale,
-                 int mlim = 1 << 30) {
+                 int mlim = 1 << 30, int zin_lo = 1, int zin_hi = 0, int zout_lo = 1, int zout_hi = 0) {
   const KitDev &k = c->kit;
 #define MLX_FFT4(N1_, N2_, TI_)                                                                               \
   do {                                                                                                        \
     dim3 grid((nrow + TI_ - 1) / TI_, grid_y.y);                                                              \
     hipLaunchKernelGGL((k_fft4<N1_, N2_, TI_, MODE>), grid, dim3(kThreads), 0, c->stream, out, in, es, gs, nrow, \
-                       act_nrc, tw, twh, sg, scale, k.r, k.nr, rscale, mlim);                               \
+                       act_nrc, tw, twh, sg, scale, k.r, k.nr, rscale, mlim, zin_lo, zin_hi, zout_lo,     \
+                       zout_hi);                                                                             \
     return true;                                                                                              \
   } while (0)
   switch (N) {
@@ -685,14 +692,23 @@ bool fft_phi_cross(mlx_ctx *c, double2 *o0, double2 *o1, double2 *o2, const doub
   return true;
 }
 
-void fft_z(mlx_ctx *c, double2 *e, int dir, double ncol, int act_nrc, int ncols) {
+void fft_z(mlx_ctx *c, double2 *e, int dir, double ncol, int act_nrc, int ncols, int zin_lo, int zin_hi,
+           int zout_lo, int zout_hi) {
   const KitDev &k = c->kit;
   if (k.nz <= 1) return;
   if (k.nz > 512) MLX_THROW(MLX_E_ARG, "fft_z: nz above 512 is not supported");
   if (ncols <= 0 || ncols > k.npdim) ncols = k.npdim;
-  ProbeScope probe(c, MLX_PROBE_FFT_Z, 32.0 * ncol * k.nz, 5.0 * k.nz * std::log2((double)k.nz) * ncol);
+  auto span = [&](int lo, int hi) {
+    lo = lo < 0 ? 0 : lo;
+    hi = hi > k.nz - 1 ? k.nz - 1 : hi;
+    return hi >= lo ? hi - lo + 1 : 0;
+  };
+  // entries read and written per line (the skipped ranges neither)
+  ProbeScope probe(c, MLX_PROBE_FFT_Z, 16.0 * ncol * (2.0 * k.nz - span(zin_lo, zin_hi) - span(zout_lo, zout_hi)),
+                   5.0 * k.nz * std::log2((double)k.nz) * ncol);
   if (launch_fft4<0>(c, k.nz, dim3(1, ncols), e, e, (size_t)k.nrdim * k.npdim, k.nrdim, k.nrdim, act_nrc, k.tw_nz,
-                     nullptr, dir < 0 ? -1.0 : 1.0, dir < 0 ? 1.0 / k.nz : 1.0, 0)) {
+                     nullptr, dir < 0 ? -1.0 : 1.0, dir < 0 ? 1.0 / k.nz : 1.0, 0, 1 << 30, zin_lo, zin_hi, zout_lo,
+                     zout_hi)) {
     HIPCHK(hipGetLastError());
     return;
   }

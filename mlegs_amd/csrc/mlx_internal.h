@@ -189,7 +189,11 @@ bool fft_phi_cross(mlx_ctx *c, double2 *o0, double2 *o1, double2 *o2, const doub
 // i < act_nrc - j of column j need transforming; ncols > 0: only columns
 // j < ncols.  Lines outside are left as they are (the caller guarantees they
 // are zero or never read); the generic kernel transforms every line.
-void fft_z(mlx_ctx *c, double2 *e, int dir, double ncol, int act_nrc = 0, int ncols = 0);
+// zin_lo..zin_hi: line entries known to be zero (not read); zout_lo..zout_hi:
+// outputs the caller does not use (not written) -- four-step kernels only,
+// the generic kernel reads and writes everything
+void fft_z(mlx_ctx *c, double2 *e, int dir, double ncol, int act_nrc = 0, int ncols = 0, int zin_lo = 1,
+           int zin_hi = 0, int zout_lo = 1, int zout_hi = 0);
 
 // legendre.hip
 // ncols > 0 (MFMA path): only columns m < ncols are transformed, the others
