@@ -228,13 +228,12 @@ void abcn(mlx_ctx *c, mlx_field *s, mlx_field *s_p, mlx_field *nl, mlx_field *nl
   if (c->p.visc < 5e-14) MLX_THROW(MLX_E_ARG, "abcn: inviscid case and no linear term in rhs. semi-implicit time adv is impossible");
   double a = -2.0 / (dt * c->p.visc);
   const bool lim = dl && dl->on;
+  if (lim && std::fabs(a) < 5e-14)
+    MLX_THROW(MLX_E_ARG, "ihelm: alpha equals to zero. Inversion of 0*identity is impossible");
+  // lim: ihelm's log-term step rides on the log-coefficient update
   k_abcn_rhs_d2(c, s->e, s->sc, nl->e, nl->sc, nl_p->e, nl_p->sc, dt, a, s->off[0], lim ? dl->pcut : -1,
-                lim ? dl->zlo : 1, lim ? dl->zhi : 0);
+                lim ? dl->zlo : 1, lim ? dl->zhi : 0, lim);
   if (lim) {
-    need_kit(c);
-    to_fff(c, s);
-    if (std::fabs(a) < 5e-14) MLX_THROW(MLX_E_ARG, "ihelm: alpha equals to zero. Inversion of 0*identity is impossible");
-    k_helm_pre(c, s->e, s->sc, s->sc, a);
     k_helm_solve(c, s->e, a, 0, s->off[0], dl->pcut, dl->zlo, dl->zhi);
   } else {
     ihelm(c, s, a);
@@ -367,11 +366,16 @@ void vec2tp_pfp(mlx_ctx *c, mlx_field *ur, mlx_field *up, mlx_field *uz, mlx_fie
     dealias(c, psi);
     dealias(c, chi);
   }
+  // with dl on, the projection has written zeros outside the chopped set, and
+  // a solve over offset-0 systems keeps them: chop would change nothing
+  const bool chopped = dl.on && chi->off[0] == 0;
   idel2_proln(c, chi);
   psi->off[0] = 0;
   chi->off[0] = 0;
-  chop(c, psi);
-  chop(c, chi);
+  if (!chopped) {
+    chop(c, psi);
+    chop(c, chi);
+  }
   zeroat1(c, psi);
   zeroat1(c, chi);
 }

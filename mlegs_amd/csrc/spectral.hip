@@ -472,6 +472,21 @@ __global__ void k_abcn_ln_(double *shln, const double *sln, const double *nlln, 
   shln[0] = a * (sln[0] + dt * (1.5 * nlln[0] - 0.5 * nlpln[0]));
 }
 
+// k_abcn_ln_ followed by ihelm's log-term step (k_helm_pre_ with alpha = a)
+// in one launch: ln <- a (ln + dt (1.5 nl.ln - 0.5 nl_p.ln)) / a, and the
+// (0,0) column loses the log term's Laplacian
+__global__ void k_abcn_ln_pre_(double2 *e, double *ln, const double *nlln, const double *nlpln, double dt, double a,
+                               const double *lognorm, double ell) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  const double sh = a * (ln[0] + dt * (1.5 * nlln[0] - 0.5 * nlpln[0]));
+  const double l = sh / a;
+  ln[0] = l;
+  const double ell2 = ell * ell;
+  e[0].x -= 4.0 / 3.0 / ell2 / exp(lognorm[0]) * l;
+  e[1].x += 2.0 / 1.0 / ell2 / exp(lognorm[1]) * l;
+  e[2].x -= 2.0 / 3.0 / ell2 / exp(lognorm[2]) * l;
+}
+
 __global__ void k_scale_rows_(double2 *e, const double *r, int nr, int nrdim, int npdim, int nz, int inverse) {
   size_t n = (size_t)nrdim * npdim * nz;
   for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (size_t)gridDim.x * blockDim.x) {
@@ -939,7 +954,7 @@ void k_abcn_rhs(mlx_ctx *c, double2 *sh, double *shln, const double2 *s, const d
 }
 
 void k_abcn_rhs_d2(mlx_ctx *c, double2 *e, double *ln, const double2 *nl, const double *nlln, const double2 *nlp,
-                   const double *nlpln, double dt, double a, int nroff, int pcut, int zlo, int zhi) {
+                   const double *nlpln, double dt, double a, int nroff, int pcut, int zlo, int zhi, bool helm_pre) {
   const KitDev &k = c->kit;
   const int nrc = k.nrchop + nroff;
   if (nrc > k.nrdim) MLX_THROW(MLX_E_ARG, "chop_index: chopping in r too large");
@@ -949,7 +964,10 @@ void k_abcn_rhs_d2(mlx_ctx *c, double2 *e, double *ln, const double2 *nl, const 
   // the log coefficient first: the data kernel reads s's ln for the (0,0) correction
   hipLaunchKernelGGL(k_abcn_rhs_d2_, grid, dim3(256), lds, c->stream, e, ln, nl, nlp, k.d2c, k.lognorm, k.ak, k.nrdim,
                      k.npdim, k.ne, nrc, k.npchop, k.chopzl, k.chopzu, k.ell, dt, a, c->p.visc, pcut, zlo, zhi);
-  hipLaunchKernelGGL(k_abcn_ln_, dim3(1), dim3(1), 0, c->stream, ln, ln, nlln, nlpln, dt, a);
+  if (helm_pre)
+    hipLaunchKernelGGL(k_abcn_ln_pre_, dim3(1), dim3(64), 0, c->stream, e, ln, nlln, nlpln, dt, a, k.lognorm, k.ell);
+  else
+    hipLaunchKernelGGL(k_abcn_ln_, dim3(1), dim3(1), 0, c->stream, ln, ln, nlln, nlpln, dt, a);
   HIPCHK(hipGetLastError());
 }
 
