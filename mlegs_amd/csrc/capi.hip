@@ -278,7 +278,10 @@ void tp2vec(mlx_ctx *c, const mlx_field *psi, const mlx_field *chi, mlx_field *v
   need_kit(c);
   if (!is_fff(psi) || !is_fff(chi)) MLX_THROW(MLX_E_LAYOUT, "vector_projection: psi and chi must be in FFF");
   mlx_field *ur = tmp(c, 0), *up = tmp(c, 1), *uz = tmp(c, 2);
-  k_tp2vec_front(c, ur->e, up->e, uz->e, psi->e, chi->e, psi->sc, chi->sc, 0, 0, ur->sc, up->sc, uz->sc, ncols);
+  // the z transform of tp2vec_back takes zlo..zhi as zero (four-step kernels)
+  const bool zskip = fft_z_fast(c->kit);
+  k_tp2vec_front(c, ur->e, up->e, uz->e, psi->e, chi->e, psi->sc, chi->sc, 0, 0, ur->sc, up->sc, uz->sc, ncols,
+                 zskip ? zlo : 1, zskip ? zhi : 0);
   tp2vec_back(c, ur, up, uz, ncols, zlo, zhi);
   fmove(vr, ur);
   fmove(vp, up);
@@ -293,8 +296,9 @@ void tp2curlvec(mlx_ctx *c, const mlx_field *psi, const mlx_field *chi, mlx_fiel
   if (!is_fff(psi) || !is_fff(chi)) MLX_THROW(MLX_E_LAYOUT, "vector_projection: psi and chi must be in FFF");
   mlx_field *ur = tmp(c, 0), *up = tmp(c, 1), *uz = tmp(c, 2);
   // del2 of md = chi runs over the rows of chi's own radial offset
+  const bool zskip = fft_z_fast(c->kit);
   k_tp2vec_front(c, ur->e, up->e, uz->e, chi->e, psi->e, chi->sc, psi->sc, 1, chi->off[0], ur->sc, up->sc,
-                 uz->sc, ncols);
+                 uz->sc, ncols, zskip ? zlo : 1, zskip ? zhi : 0);
   tp2vec_back(c, ur, up, uz, ncols, zlo, zhi);
   fmove(wr, ur);
   fmove(wp, up);
@@ -435,11 +439,14 @@ void advection_rhs(mlx_ctx *c, mlx_field *psi, mlx_field *chi, mlx_field *nlpsi,
 }
 
 // flag_dev (optional): the non-finite check of the field, on the same pass
-void svv(mlx_ctx *c, mlx_field *f, double *gain_dev, int *flag_dev = nullptr) {
+// dl (the time loop, right after de-aliasing f): the reduction skips the
+// de-aliased modes, which are zero
+void svv(mlx_ctx *c, mlx_field *f, double *gain_dev, int *flag_dev = nullptr, const DealiasLimits *dl = nullptr) {
   if (!c->p.is_svv) return;
   if (!is_fff(f)) MLX_THROW(MLX_E_LAYOUT, "svv_filter: scalar must be in FFF space");
   need_kit(c);
-  k_svv(c, f->e, gain_dev, flag_dev);
+  const bool lim = dl && dl->on;
+  k_svv(c, f->e, gain_dev, flag_dev, lim ? dl->pcut : -1, lim ? dl->zlo : 1, lim ? dl->zhi : 0);
 }
 
 void check_finite(mlx_ctx *c, const mlx_field *a, const mlx_field *b, int *flag_dev) {
@@ -938,8 +945,8 @@ int mlx_vortex_step(mlx_ctx *c, mlx_vortex *v, int32_t nsteps, double dt, int32_
       // non-finite entries: the filter and zeroat1 keep a finite field
       // finite and a non-finite one non-finite, so the flag is the one the
       // check after the step would raise
-      svv(c, psi, v->gains, v->flag);
-      svv(c, chi, v->gains + 1, v->flag);
+      svv(c, psi, v->gains, v->flag, &dl);
+      svv(c, chi, v->gains + 1, v->flag, &dl);
       zeroat1(c, psi);
       zeroat1(c, chi);
       advection_rhs(c, psi, chi, nlpsi, nlchi, uz, true);

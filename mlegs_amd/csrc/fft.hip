@@ -632,6 +632,11 @@ int tile_rows(int n_line, int nrdim) {
 
 namespace mlx {
 
+bool fft_z_fast(const KitDev &k) {
+  const int N = k.nz;
+  return k.nz > 1 && (N == 16 || N == 32 || N == 64 || N == 128 || N == 256);
+}
+
 bool fft_phi_fast(const KitDev &k) {
   const int N = k.nph;
   return k.np > 1 && (N == 16 || N == 32 || N == 64 || N == 128 || N == 256);

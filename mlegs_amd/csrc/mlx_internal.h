@@ -177,6 +177,7 @@ void kit_free(KitDev &k);
 // without being read
 void fft_phi(mlx_ctx *c, double2 *out, const double2 *in, int dir, int rscale = 0, int mlim = -1);
 bool fft_phi_fast(const KitDev &k);  // the four-step phi kernels cover np/2
+bool fft_z_fast(const KitDev &k);    // the four-step z kernels cover nz
 // v x w (v_z + ub) to PFP with the r weights of vector_projection (modes
 // m < mmax written; mmax <= 0: all); false when
 // the fused form is not built for this np (the caller then runs vecprod and
@@ -218,7 +219,8 @@ void vec2tp_project(mlx_ctx *c, const double2 *ur, const double2 *up, const doub
 void k_ln_column(mlx_ctx *c, double2 *e, const double *ln, double sign);
 void k_chop(mlx_ctx *c, double2 *e, const char *space, int nroff);
 void k_dealias(mlx_ctx *c, double2 *e, const char *space);
-void k_svv(mlx_ctx *c, double2 *e, double *gain_dev, int *flag = nullptr);
+void k_svv(mlx_ctx *c, double2 *e, double *gain_dev, int *flag = nullptr, int pcut = -1, int zlo = 1,
+           int zhi = 0);
 void k_calcat(mlx_ctx *c, const double2 *e, const double *tbl, int nrc, double2 *out);
 void k_zeroat1(mlx_ctx *c, double2 *e, int nrc);
 void k_delsqp(mlx_ctx *c, double2 *e, double *ln, int nroff, int inverse);
@@ -247,7 +249,8 @@ void k_tp2vec_combine(mlx_ctx *c, double2 *ur, double2 *up, const double2 *psi, 
 // chi's radial offset md_off, C = c_in = psi
 void k_tp2vec_front(mlx_ctx *c, double2 *ur, double2 *up, double2 *uz, const double2 *p_in, const double2 *c_in,
                     const double *p_ln, const double *c_ln, int mode, int md_off, double *ln_r = nullptr,
-                    double *ln_p = nullptr, double *ln_z = nullptr, int ncols = -1);
+                    double *ln_p = nullptr, double *ln_z = nullptr, int ncols = -1, int zlo = 1,
+                    int zhi = 0);
 void k_vecprod(mlx_ctx *c, double2 *vr, double2 *vp, double2 *vz, const double2 *ur, const double2 *up,
                const double2 *uz);
 void k_vec2tp_up_ln(mlx_ctx *c, double2 *up, const double2 *inf, double *psiln);

@@ -93,8 +93,11 @@ __device__ inline double svv_q(int i, int j, int k, double inv_r, double inv_p, 
 
 // flag (optional): set to 1 when any entry is not finite, the stability
 // check of the time loop taken on the same pass over the field
+// Entries of columns j >= pcut and slices zlo..zhi are skipped (the caller
+// has just de-aliased the field: they are zero and add nothing).
 __global__ void k_svv_reduce(const double2 *e, int nrdim, int npdim, int nz, double inv_r, double inv_p,
-                             const double *ak, double kmax, double cutoff, double *part, int *flag) {
+                             const double *ak, double kmax, double cutoff, double *part, int *flag, int pcut, int zlo,
+                             int zhi) {
   __shared__ double sh[16];
   size_t n = (size_t)nrdim * npdim * nz;
   double tot = 0.0, tail = 0.0;
@@ -103,6 +106,7 @@ __global__ void k_svv_reduce(const double2 *e, int nrdim, int npdim, int nz, dou
     int i = (int)(t % nrdim);
     int j = (int)((t / nrdim) % npdim);
     int k = (int)(t / ((size_t)nrdim * npdim));
+    if (j >= pcut || (k >= zlo && k <= zhi)) continue;
     double2 v = e[t];
     if (!isfinite(v.x) || !isfinite(v.y)) bad = 1;
     double a2 = v.x * v.x + v.y * v.y;
@@ -567,7 +571,8 @@ __global__ void __launch_bounds__(256) k_tp2vec_front_(double2 *ur, double2 *up,
                                                       int mode, int md_off, const double *d2c, const double *xxc,
                                                       const double *lognorm, const double *ak, int nrdim, int npdim,
                                                       int nz, int ne, int nrc, int npc, int nzc, int nzcu,
-                                                      double ell, double *ln_r, double *ln_p, double *ln_z) {
+                                                      double ell, double *ln_r, double *ln_p, double *ln_z, int zlo,
+                                                      int zhi) {
   extern __shared__ double2 lds[];
   // the three outputs carry no log term (their ln slots are distinct from
   // the inputs' p_ln and c_ln)
@@ -580,6 +585,9 @@ __global__ void __launch_bounds__(256) k_tp2vec_front_(double2 *ur, double2 *up,
   double2 *P = lds, *C = lds + nrdim, *A = lds + 2 * nrdim, *B = lds + 3 * nrdim;
   // m fastest: consecutive workgroups take adjacent columns of one slice
   const int m = blockIdx.x, k = blockIdx.y;
+  // slices zlo..zhi: zero in the inputs, and the z transform that follows
+  // takes them as zero without reading them
+  if (k >= zlo && k <= zhi) return;
   const size_t off = (size_t)k * nrdim * npdim + (size_t)nrdim * m;
   const int cnt = z_count(k, nzc, nzcu);
   const int nn0 = nrcs_dev(nrc, npc, m);
@@ -859,7 +867,7 @@ void k_dealias(mlx_ctx *c, double2 *e, const char *space) {
                      pcut, zcut, zupper);
 }
 
-void k_svv(mlx_ctx *c, double2 *e, double *gain_dev, int *flag) {
+void k_svv(mlx_ctx *c, double2 *e, double *gain_dev, int *flag, int pcut, int zlo, int zhi) {
   const KitDev &k = c->kit;
   const mlx_params &p = c->p;
   double inv_r = 1.0 / fmax((double)(k.nrchop - 1), 1.0);
@@ -874,7 +882,7 @@ void k_svv(mlx_ctx *c, double2 *e, double *gain_dev, int *flag) {
   double *part = c->dsmall;
   double *out = c->dsmall + 2 * nb;
   hipLaunchKernelGGL(k_svv_reduce, dim3(nb), dim3(kT), 0, c->stream, e, k.nrdim, k.npdim, k.nzdim, inv_r, inv_p,
-                     k.ak, kmax, cutoff, part, flag);
+                     k.ak, kmax, cutoff, part, flag, pcut > 0 ? pcut : k.npdim, zlo, zhi);
   hipLaunchKernelGGL(k_svv_gain, dim3(1), dim3(1024), 0, c->stream, part, nb, gain_dev, out, target, relax, st0);
   hipLaunchKernelGGL(k_svv_apply, dim3(grid_for(c->nelem)), dim3(kT), 0, c->stream, e, k.nrdim, k.npdim, k.nzdim,
                      inv_r, inv_p, k.ak, kmax, cutoff, out);
@@ -985,7 +993,7 @@ void k_tp2vec_combine(mlx_ctx *c, double2 *ur, double2 *up, const double2 *psi, 
 
 void k_tp2vec_front(mlx_ctx *c, double2 *ur, double2 *up, double2 *uz, const double2 *p_in, const double2 *c_in,
                     const double *p_ln, const double *c_ln, int mode, int md_off, double *ln_r, double *ln_p,
-                    double *ln_z, int ncols) {
+                    double *ln_z, int ncols, int zlo, int zhi) {
   const KitDev &k = c->kit;
   if (k.nrchop + 3 > k.nrdim || k.nrchop + 3 > k.ne) MLX_THROW(MLX_E_ARG, "chop_index: chopping in r too large");
   if (k.nrchop > 768) MLX_THROW(MLX_E_ARG, "vector_reconstruction: nrchop above 768 is not supported");
@@ -997,7 +1005,7 @@ void k_tp2vec_front(mlx_ctx *c, double2 *ur, double2 *up, double2 *uz, const dou
   size_t lds = 4 * sizeof(double2) * (size_t)k.nrdim;
   hipLaunchKernelGGL(k_tp2vec_front_, grid, dim3(256), lds, c->stream, ur, up, uz, p_in, c_in, p_ln, c_ln, mode,
                      md_off, k.d2c, k.xxc, k.lognorm, k.ak, k.nrdim, k.npdim, k.nzdim, k.ne, k.nrchop, k.npchop, k.chopzl,
-                     k.chopzu, k.ell, ln_r, ln_p, ln_z);
+                     k.chopzu, k.ell, ln_r, ln_p, ln_z, zlo, zhi);
   HIPCHK(hipGetLastError());
 }
 
