@@ -373,7 +373,15 @@ void vec2tp_pfp(mlx_ctx *c, mlx_field *ur, mlx_field *up, mlx_field *uz, mlx_fie
   // with dl on, the projection has written zeros outside the chopped set, and
   // a solve over offset-0 systems keeps them: chop would change nothing
   const bool chopped = dl.on && chi->off[0] == 0;
-  idel2_proln(c, chi);
+  if (dl.on) {
+    // the projection left the de-aliased modes zero: their systems (zero
+    // right-hand sides) are not visited
+    to_fff(c, chi);
+    k_helm_solve(c, chi->e, 0.0, 1, chi->off[0], dl.pcut, dl.zlo, dl.zhi);
+    k_idel2_post(c, chi->e, chi->sc);
+  } else {
+    idel2_proln(c, chi);
+  }
   psi->off[0] = 0;
   chi->off[0] = 0;
   if (!chopped) {
