@@ -359,9 +359,12 @@ __global__ void __launch_bounds__(256) k_v2t_f32(const double2 *__restrict__ ur,
   constexpr int KV = 8;   // contraction chunk for this kernel
   __shared__ float X[5][KV][BJP];
   __shared__ float Tb[3][KV][APP];
-  int u, yt;
-  xcd_map(blockIdx.x, nyt, u, yt);
-  const int m = u >> 1, p = u & 1;
+  // both parities and every kappa tile of a column m on one XCD: the two
+  // parity workgroups read the same input columns and write interleaved
+  // rows of the same output lines, which then meet in that XCD's L2
+  int m, y2;
+  xcd_map(blockIdx.x, 2 * nyt, m, y2);
+  const int yt = y2 >> 1, p = y2 & 1;
   if (m >= npdim) return;
   const int k0 = yt * KT;
   const int tid = threadIdx.x, lane = tid % 64, wv = tid / 64;
@@ -610,7 +613,7 @@ void vec2tp_project_f32(mlx_ctx *c, const double2 *ur, const double2 *up, const 
   int nzk = k.nzdim;
   for (int kk = 0; kk < k.nzdim; ++kk)
     if (kk >= zlo && kk <= zhi) --nzk;
-  dim3 grid = xcd_grid(2 * k.npdim, nyt);
+  dim3 grid = xcd_grid(k.npdim, 2 * nyt);
   int at = mfma_tiles(k);
   ProbeScope probe(c, MLX_PROBE_V2T, v2t_bytes(k, 0, 4, npc, nzk, true),
                    5.0 * leg_flops(k, 0, npc) * nzk / (double)k.nzdim);
