@@ -580,7 +580,7 @@ int mlx_kit_dims(mlx_ctx *c, int32_t *o) {
   return guarded(c, [&] {
     need_kit(c);
     const KitDev &k = c->kit;
-    int32_t v[9] = {k.nrdim, k.npdim, k.nzdim, k.nrh, k.ne, k.chopzl, k.chopzu, k.nph, k.npchop};
+    int32_t v[9] = {k.nrdim_ref, k.npdim, k.nzdim, k.nrh, k.ne, k.chopzl, k.chopzu, k.nph, k.npchop};
     std::memcpy(o, v, sizeof(v));
   });
 }
@@ -637,7 +637,7 @@ int mlx_field_copy(mlx_ctx *c, mlx_field *d, const mlx_field *s) {
 
 static void check_dims(mlx_ctx *c, int64_t a, int64_t b, int64_t d) {
   const KitDev &k = c->kit;
-  if (a != k.nrdim || b != k.npdim || d != k.nzdim)
+  if (a != k.nrdim_ref || b != k.npdim || d != k.nzdim)
     MLX_THROW(MLX_E_LAYOUT, "scalar dimension incompatible with the dimension defined in the kit");
 }
 
@@ -647,7 +647,13 @@ int mlx_field_upload(mlx_ctx *c, mlx_field *f, const double *host, int64_t nrdim
     need_kit(c);
     check_dims(c, nrdim, npdim, nzdim);
     if (space_id(space) < 0) MLX_THROW(MLX_E_ARG, "upload: space must be PPP, PFP, FFP or FFF");
-    HIPCHK(hipMemcpyAsync(f->e, host, sizeof(double2) * c->nelem, hipMemcpyHostToDevice, c->stream));
+    // host columns of nrdim_ref rows into device columns of nrdim rows (the
+    // padding rows stay zero)
+    const KitDev &k = c->kit;
+    if (k.nrdim != k.nrdim_ref) HIPCHK(hipMemsetAsync(f->e, 0, sizeof(double2) * c->nelem, c->stream));
+    HIPCHK(hipMemcpy2DAsync(f->e, sizeof(double2) * k.nrdim, host, sizeof(double2) * k.nrdim_ref,
+                            sizeof(double2) * k.nrdim_ref, (size_t)k.npdim * k.nzdim, hipMemcpyHostToDevice,
+                            c->stream));
     c->hsmall[0] = ln;
     HIPCHK(hipMemcpyAsync(f->sc, c->hsmall, sizeof(double), hipMemcpyHostToDevice, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
@@ -661,7 +667,11 @@ int mlx_field_download(mlx_ctx *c, const mlx_field *f, double *host, int64_t nrd
   return guarded(c, [&] {
     need_kit(c);
     check_dims(c, nrdim, npdim, nzdim);
-    if (host) HIPCHK(hipMemcpyAsync(host, f->e, sizeof(double2) * c->nelem, hipMemcpyDeviceToHost, c->stream));
+    const KitDev &k = c->kit;
+    if (host)
+      HIPCHK(hipMemcpy2DAsync(host, sizeof(double2) * k.nrdim_ref, f->e, sizeof(double2) * k.nrdim,
+                              sizeof(double2) * k.nrdim_ref, (size_t)k.npdim * k.nzdim, hipMemcpyDeviceToHost,
+                              c->stream));
     HIPCHK(hipMemcpyAsync(c->hsmall, f->sc, sizeof(double), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
     if (ln) *ln = c->hsmall[0];
@@ -682,7 +692,7 @@ int mlx_field_set_meta(mlx_ctx *c, mlx_field *f, const char *space, double ln, c
 }
 
 int mlx_field_get_meta(mlx_ctx *c, const mlx_field *f, char *space4, double *ln, int32_t *offsets) {
-  return mlx_field_download(c, f, nullptr, c->kit.nrdim, c->kit.npdim, c->kit.nzdim, space4, ln, offsets);
+  return mlx_field_download(c, f, nullptr, c->kit.nrdim_ref, c->kit.npdim, c->kit.nzdim, space4, ln, offsets);
 }
 
 int mlx_chop_offset(mlx_ctx *c, mlx_field *f, int32_t a, int32_t b, int32_t d) {

@@ -245,7 +245,8 @@ void kit_build(mlx_ctx *c) {
   k.hyperpow = p.hyperpow;
   k.ell = p.ell; k.zlen = p.zlen;
   k.nrh = p.nr / 2;
-  k.nrdim = p.nr + (p.hyperpow > 3 ? p.hyperpow : 3);
+  k.nrdim_ref = p.nr + (p.hyperpow > 3 ? p.hyperpow : 3);
+  k.nrdim = (k.nrdim_ref + 7) / 8 * 8;
   k.npdim = p.np / 2 + 1;
   k.nzdim = p.nz;
   k.nph = p.np / 2;

@@ -318,7 +318,7 @@ namespace mlx {
 
 void leg_forward(mlx_ctx *c, const double2 *in, double2 *out, int nroff, bool zero, int ncols) {
   const KitDev &k = c->kit;
-  if (k.nrchop + nroff > k.nrdim) MLX_THROW(MLX_E_ARG, "rtrans_forward: chopping in r too large");
+  if (k.nrchop + nroff > k.nrdim_ref) MLX_THROW(MLX_E_ARG, "rtrans_forward: chopping in r too large");
   if (c->prec != MLX_PREC_FP64 && mfma_ok(k, nroff)) {
     leg_forward_f32(c, in, out, ncols);
     return;
@@ -335,7 +335,7 @@ void leg_forward(mlx_ctx *c, const double2 *in, double2 *out, int nroff, bool ze
 
 void leg_backward(mlx_ctx *c, const double2 *in, double2 *out, int nroff, bool zero, int ncols, bool ztail) {
   const KitDev &k = c->kit;
-  if (k.nrchop + nroff > k.nrdim) MLX_THROW(MLX_E_ARG, "rtrans_backward: chopping in r too large");
+  if (k.nrchop + nroff > k.nrdim_ref) MLX_THROW(MLX_E_ARG, "rtrans_backward: chopping in r too large");
   if (c->prec != MLX_PREC_FP64 && mfma_ok(k, nroff)) {
     leg_backward_f32(c, in, out, ncols, ztail);
     return;

@@ -45,6 +45,11 @@ struct KitDev {
   int nr, np, nz, nrchop, npchop, nzchop, hyperpow;
   int nrh, nrdim, npdim, nzdim, nph, ne, chopzl, chopzu;
   int nt;  // rows per m of the projection tables (nrchop + 3)
+  // nrdim_ref = nr + max(hyperpow, 3), the reference's first dimension and the
+  // one the C ABI's host arrays use; nrdim (the device column length) is it
+  // rounded up to 8 rows, so every (m, kappa) column starts 128-byte aligned.
+  // Rows nrdim_ref .. nrdim-1 are zero padding.
+  int nrdim_ref;
   double ell, zlen;
   // device arrays
   double *x = nullptr, *w = nullptr, *lnx = nullptr, *r = nullptr, *ak = nullptr;

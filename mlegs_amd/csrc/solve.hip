@@ -531,7 +531,7 @@ LuCache &lu_get(mlx_ctx *c, double alpha, int kind, int nroff) {
     }
   const KitDev &k = c->kit;
   const int nrc = k.nrchop + nroff;
-  if (nrc > k.nrdim) MLX_THROW(MLX_E_ARG, "chop_index: chopping in r too large");
+  if (nrc > k.nrdim_ref) MLX_THROW(MLX_E_ARG, "chop_index: chopping in r too large");
   const int S = seg_rows(nrc);
   if (S == 0) MLX_THROW(MLX_E_ARG, "ihelm: more than 544 radial rows per system are not supported");
   if (c->lu.size() >= 4) {  // evict the least recently used entry

@@ -914,7 +914,7 @@ void k_delsqp(mlx_ctx *c, double2 *e, double *ln, int nroff, int inverse) {
 void k_bandmul(mlx_ctx *c, double2 *e, double *ln, int op, int nroff) {
   const KitDev &k = c->kit;
   int nrc = k.nrchop + nroff;
-  if (nrc > k.nrdim) MLX_THROW(MLX_E_ARG, "chop_index: chopping in r too large");
+  if (nrc > k.nrdim_ref) MLX_THROW(MLX_E_ARG, "chop_index: chopping in r too large");
   if (nrc > 8 * 64) MLX_THROW(MLX_E_ARG, "banded operators support at most 512 radial modes");
   if (op == 2 && nrc > k.ne) MLX_THROW(MLX_E_ARG, "xxdx: table too small");
   dim3 grid(k.npdim, k.nzdim);
@@ -965,7 +965,7 @@ void k_abcn_rhs_d2(mlx_ctx *c, double2 *e, double *ln, const double2 *nl, const 
                    const double *nlpln, double dt, double a, int nroff, int pcut, int zlo, int zhi, bool helm_pre) {
   const KitDev &k = c->kit;
   const int nrc = k.nrchop + nroff;
-  if (nrc > k.nrdim) MLX_THROW(MLX_E_ARG, "chop_index: chopping in r too large");
+  if (nrc > k.nrdim_ref) MLX_THROW(MLX_E_ARG, "chop_index: chopping in r too large");
   if (pcut <= 0 || pcut > k.npdim) pcut = k.npdim;
   dim3 grid(pcut, k.nzdim);
   size_t lds = 3 * sizeof(double2) * (size_t)k.nrdim;
@@ -995,9 +995,9 @@ void k_tp2vec_front(mlx_ctx *c, double2 *ur, double2 *up, double2 *uz, const dou
                     const double *p_ln, const double *c_ln, int mode, int md_off, double *ln_r, double *ln_p,
                     double *ln_z, int ncols, int zlo, int zhi) {
   const KitDev &k = c->kit;
-  if (k.nrchop + 3 > k.nrdim || k.nrchop + 3 > k.ne) MLX_THROW(MLX_E_ARG, "chop_index: chopping in r too large");
+  if (k.nrchop + 3 > k.nrdim_ref || k.nrchop + 3 > k.ne) MLX_THROW(MLX_E_ARG, "chop_index: chopping in r too large");
   if (k.nrchop > 768) MLX_THROW(MLX_E_ARG, "vector_reconstruction: nrchop above 768 is not supported");
-  if (md_off < 0 || md_off > 3 || k.nrchop + md_off > k.nrdim)
+  if (md_off < 0 || md_off > 3 || k.nrchop + md_off > k.nrdim_ref)
     MLX_THROW(MLX_E_ARG, "curl_vector_reconstruction: chopping offset of chi out of range");
   // ncols > 0: columns m >= ncols are neither read nor written (their
   // inputs are zero and the caller's transforms skip them)
