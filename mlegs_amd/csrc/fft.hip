@@ -595,15 +595,15 @@ __global__ void __launch_bounds__(kThreads) k_fft4_cross(double2 *o0, double2 *o
 }
 
 template <int MODE>
-bool launch_fft4(mlx_ctx *c, int N, dim3 grid_y, double2 *out, const double2 *in, size_t es, size_t gs, int nrow,
-                 int act_nrc, const double2 *tw, const double2 *twh, double sg, double scale, int rscale,
-                 int mlim = 1 << 30, int zin_lo = 1, int zin_hi = 0, int zout_lo = 1, int zout_hi = 0) {
+bool launch_fft4(mlx_ctx *c, ProbeScope &ps, int N, dim3 grid_y, double2 *out, const double2 *in, size_t es,
+                 size_t gs, int nrow, int act_nrc, const double2 *tw, const double2 *twh, double sg, double scale,
+                 int rscale, int mlim = 1 << 30, int zin_lo = 1, int zin_hi = 0, int zout_lo = 1, int zout_hi = 0) {
   const KitDev &k = c->kit;
 #define MLX_FFT4(N1_, N2_, TI_)                                                                               \
   do {                                                                                                        \
     dim3 grid((nrow + TI_ - 1) / TI_, grid_y.y);                                                              \
-    hipLaunchKernelGGL((k_fft4<N1_, N2_, TI_, MODE>), grid, dim3(kThreads), 0, c->stream, out, in, es, gs, nrow, \
-                       act_nrc, tw, twh, sg, scale, k.r, k.nr, rscale, mlim, zin_lo, zin_hi, zout_lo,     \
+    probed_launch(ps, k_fft4<N1_, N2_, TI_, MODE>, grid, dim3(kThreads), 0, c->stream, out, in, es, gs, nrow,   \
+                  act_nrc, tw, twh, sg, scale, k.r, k.nr, rscale, mlim, zin_lo, zin_hi, zout_lo,     \
                        zout_hi);                                                                             \
     return true;                                                                                              \
   } while (0)
@@ -654,15 +654,15 @@ void fft_phi(mlx_ctx *c, double2 *out, const double2 *in, int dir, int rscale, i
   ProbeScope probe(c, MLX_PROBE_FFT_PHI, 16.0 * k.nr * k.nz * (dir > 0 ? nread + k.nph : 2.0 * k.nph + 1),
                    2.5 * k.np * std::log2((double)k.np) * k.nr * k.nz);
   const size_t plane = (size_t)k.nrdim * k.npdim;
-  bool fast = (dir < 0) ? launch_fft4<1>(c, k.nph, dim3(1, k.nzdim), out, in, k.nrdim, plane, k.nrdim, 0, k.tw_nph,
-                                         k.tw_half, -1.0, 1.0 / k.np, rscale)
-                        : launch_fft4<2>(c, k.nph, dim3(1, k.nzdim), out, in, k.nrdim, plane, k.nrdim, 0, k.tw_nph,
-                                         k.tw_half, +1.0, 1.0, rscale, mlim > 0 ? mlim : 1 << 30);
+  bool fast = (dir < 0) ? launch_fft4<1>(c, probe, k.nph, dim3(1, k.nzdim), out, in, k.nrdim, plane, k.nrdim, 0,
+                                         k.tw_nph, k.tw_half, -1.0, 1.0 / k.np, rscale)
+                        : launch_fft4<2>(c, probe, k.nph, dim3(1, k.nzdim), out, in, k.nrdim, plane, k.nrdim, 0,
+                                         k.tw_nph, k.tw_half, +1.0, 1.0, rscale, mlim > 0 ? mlim : 1 << 30);
   if (fast) {
     HIPCHK(hipGetLastError());
     return;
   }
-  hipLaunchKernelGGL(k_fft_phi, grid, dim3(kThreads), 0, c->stream, out, in, k.nrdim, k.npdim, k.nzdim, TI,
+  probed_launch(probe, k_fft_phi, grid, dim3(kThreads), 0, c->stream, out, in, k.nrdim, k.npdim, k.nzdim, TI,
                      k.plan_nph, k.tw_nph, k.tw_half, dir, k.r, k.nr, rscale);
   HIPCHK(hipGetLastError());
 }
@@ -681,7 +681,7 @@ bool fft_phi_cross(mlx_ctx *c, double2 *o0, double2 *o1, double2 *o2, const doub
 #define MLX_FFTX(N1_, N2_, TI_)                                                                                 \
   do {                                                                                                          \
     dim3 grid((k.nrdim + TI_ - 1) / TI_, k.nzdim);                                                              \
-    hipLaunchKernelGGL((k_fft4_cross<N1_, N2_, TI_>), grid, dim3(kThreads), 0, c->stream, o0, o1, o2, v0, v1, v2, \
+    probed_launch(probe, k_fft4_cross<N1_, N2_, TI_>, grid, dim3(kThreads), 0, c->stream, o0, o1, o2, v0, v1, v2, \
                        w0, w1, w2, ub, (size_t)k.nrdim, plane, k.nrdim, k.tw_nph, k.tw_half, 1.0 / k.np, k.r,   \
                        k.nr, ln0, ln1, ln2, mmax);                                                              \
   } while (0)
@@ -711,15 +711,15 @@ void fft_z(mlx_ctx *c, double2 *e, int dir, double ncol, int act_nrc, int ncols,
   // entries read and written per line (the skipped ranges neither)
   ProbeScope probe(c, MLX_PROBE_FFT_Z, 16.0 * ncol * (2.0 * k.nz - span(zin_lo, zin_hi) - span(zout_lo, zout_hi)),
                    5.0 * k.nz * std::log2((double)k.nz) * ncol);
-  if (launch_fft4<0>(c, k.nz, dim3(1, ncols), e, e, (size_t)k.nrdim * k.npdim, k.nrdim, k.nrdim, act_nrc, k.tw_nz,
-                     nullptr, dir < 0 ? -1.0 : 1.0, dir < 0 ? 1.0 / k.nz : 1.0, 0, 1 << 30, zin_lo, zin_hi, zout_lo,
-                     zout_hi)) {
+  if (launch_fft4<0>(c, probe, k.nz, dim3(1, ncols), e, e, (size_t)k.nrdim * k.npdim, k.nrdim, k.nrdim, act_nrc,
+                     k.tw_nz, nullptr, dir < 0 ? -1.0 : 1.0, dir < 0 ? 1.0 / k.nz : 1.0, 0, 1 << 30, zin_lo, zin_hi,
+                     zout_lo, zout_hi)) {
     HIPCHK(hipGetLastError());
     return;
   }
   int TI = tile_rows(k.nz, k.nrdim);
   dim3 grid((k.nrdim + TI - 1) / TI, k.npdim);
-  hipLaunchKernelGGL(k_fft_z, grid, dim3(kThreads), 0, c->stream, e, k.nrdim, k.npdim, k.nzdim, TI, k.plan_nz,
+  probed_launch(probe, k_fft_z, grid, dim3(kThreads), 0, c->stream, e, k.nrdim, k.npdim, k.nzdim, TI, k.plan_nz,
                      k.tw_nz, dir);
   HIPCHK(hipGetLastError());
 }

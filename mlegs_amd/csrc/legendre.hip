@@ -328,7 +328,7 @@ void leg_forward(mlx_ctx *c, const double2 *in, double2 *out, int nroff, bool ze
   if (wt.n == 0) return;
   dim3 grid(wt.n, (k.nzdim + BN - 1) / BN);
   ProbeScope probe(c, MLX_PROBE_LEG_FWD, leg_bytes(k, nroff, 8), leg_flops(k, nroff));
-  hipLaunchKernelGGL(k_leg_fwd_f64, grid, dim3(256), 0, c->stream, in, out, k.pf, k.w, wt.d, k.nrdim, k.npdim,
+  probed_launch(probe, k_leg_fwd_f64, grid, dim3(256), 0, c->stream, in, out, k.pf, k.w, wt.d, k.nrdim, k.npdim,
                      k.nzdim, k.nr, k.nrh, k.ne);
   HIPCHK(hipGetLastError());
 }
@@ -345,7 +345,7 @@ void leg_backward(mlx_ctx *c, const double2 *in, double2 *out, int nroff, bool z
   if (wt.n == 0) return;
   dim3 grid(wt.n, (k.nzdim + BN - 1) / BN);
   ProbeScope probe(c, MLX_PROBE_LEG_BWD, leg_bytes(k, nroff, 8), leg_flops(k, nroff));
-  hipLaunchKernelGGL(k_leg_bwd_f64, grid, dim3(256), 0, c->stream, in, out, k.pf, wt.d, k.nrdim, k.npdim, k.nzdim,
+  probed_launch(probe, k_leg_bwd_f64, grid, dim3(256), 0, c->stream, in, out, k.pf, wt.d, k.nrdim, k.npdim, k.nzdim,
                      k.nr, k.nrh, k.ne);
   HIPCHK(hipGetLastError());
 }
@@ -365,7 +365,7 @@ void vec2tp_project(mlx_ctx *c, const double2 *ur, const double2 *up, const doub
   if (wt.n == 0) return;
   dim3 grid(wt.n, (k.nzdim + VBN - 1) / VBN);
   ProbeScope probe(c, MLX_PROBE_V2T, v2t_bytes(k, 0, 8), 5.0 * leg_flops(k, 0));
-  hipLaunchKernelGGL(k_v2t_f64, grid, dim3(256), 0, c->stream, ur, up, uz, psi, chi, k.tV, k.tD, k.tT, k.ak, wt.d,
+  probed_launch(probe, k_v2t_f64, grid, dim3(256), 0, c->stream, ur, up, uz, psi, chi, k.tV, k.tD, k.tT, k.ak, wt.d,
                      k.nrdim, k.npdim, k.nzdim, k.nr, k.nrh, k.nt, k.chopzl, k.chopzu);
   HIPCHK(hipGetLastError());
 }

@@ -577,10 +577,10 @@ void leg_forward_f32(mlx_ctx *c, const double2 *in, double2 *out, int ncols) {
   int at = mfma_tiles(k);
   ProbeScope probe(c, MLX_PROBE_LEG_FWD, leg_bytes(k, 0, 4, npc), leg_flops(k, 0, npc));
   if (at == 1)
-    hipLaunchKernelGGL(k_leg_fwd_f32<1>, grid, dim3(256), 0, c->stream, in, out, k.pfg32, k.w, k.nrdim, k.npdim,
+    probed_launch(probe, k_leg_fwd_f32<1>, grid, dim3(256), 0, c->stream, in, out, k.pfg32, k.w, k.nrdim, k.npdim,
                        k.nzdim, k.nr, k.nrh, k.nrchop, npc, nyt);
   else
-    hipLaunchKernelGGL(k_leg_fwd_f32<2>, grid, dim3(256), 0, c->stream, in, out, k.pfg32, k.w, k.nrdim, k.npdim,
+    probed_launch(probe, k_leg_fwd_f32<2>, grid, dim3(256), 0, c->stream, in, out, k.pfg32, k.w, k.nrdim, k.npdim,
                        k.nzdim, k.nr, k.nrh, k.nrchop, npc, nyt);
   HIPCHK(hipGetLastError());
 }
@@ -594,10 +594,10 @@ void leg_backward_f32(mlx_ctx *c, const double2 *in, double2 *out, int ncols, bo
   int xt = (k.nrh + 63) / 64;
   ProbeScope probe(c, MLX_PROBE_LEG_BWD, leg_bytes(k, 0, 4, npc), leg_flops(k, 0, npc));
   if (xt == 1)
-    hipLaunchKernelGGL(k_leg_bwd_f32<1>, grid, dim3(256), 0, c->stream, in, out, k.pf32, k.nrdim, k.npdim, k.nzdim,
+    probed_launch(probe, k_leg_bwd_f32<1>, grid, dim3(256), 0, c->stream, in, out, k.pf32, k.nrdim, k.npdim, k.nzdim,
                        k.nr, k.nrh, k.ne, k.nrchop, npc, nyt);
   else
-    hipLaunchKernelGGL(k_leg_bwd_f32<2>, grid, dim3(256), 0, c->stream, in, out, k.pf32, k.nrdim, k.npdim, k.nzdim,
+    probed_launch(probe, k_leg_bwd_f32<2>, grid, dim3(256), 0, c->stream, in, out, k.pf32, k.nrdim, k.npdim, k.nzdim,
                        k.nr, k.nrh, k.ne, k.nrchop, npc, nyt);
   HIPCHK(hipGetLastError());
 }
@@ -618,10 +618,10 @@ void vec2tp_project_f32(mlx_ctx *c, const double2 *ur, const double2 *up, const 
   ProbeScope probe(c, MLX_PROBE_V2T, v2t_bytes(k, 0, 4, npc, nzk, true),
                    5.0 * leg_flops(k, 0, npc) * nzk / (double)k.nzdim);
   if (at == 1)
-    hipLaunchKernelGGL(k_v2t_f32<1>, grid, dim3(256), 0, c->stream, ur, up, uz, psi, chi, k.tVg32, k.tDg32, k.tTg32,
+    probed_launch(probe, k_v2t_f32<1>, grid, dim3(256), 0, c->stream, ur, up, uz, psi, chi, k.tVg32, k.tDg32, k.tTg32,
                        k.ak, k.nrdim, k.npdim, k.nzdim, k.nr, k.nrh, k.nrchop, npc, k.chopzl, k.chopzu, nyt, zlo, zhi);
   else
-    hipLaunchKernelGGL(k_v2t_f32<2>, grid, dim3(256), 0, c->stream, ur, up, uz, psi, chi, k.tVg32, k.tDg32, k.tTg32,
+    probed_launch(probe, k_v2t_f32<2>, grid, dim3(256), 0, c->stream, ur, up, uz, psi, chi, k.tVg32, k.tDg32, k.tTg32,
                        k.ak, k.nrdim, k.npdim, k.nzdim, k.nr, k.nrh, k.nrchop, npc, k.chopzl, k.chopzu, nyt, zlo, zhi);
   HIPCHK(hipGetLastError());
 }

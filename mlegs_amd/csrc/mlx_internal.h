@@ -9,6 +9,7 @@
 //          band rows of del2 / xxdx per (m, n); projection tables of vec2tp
 //          V, D, T [m][n][x].
 #pragma once
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
@@ -138,22 +139,41 @@ struct mlx_ctx {
 
 // Records the opening event on construction and the closing one on
 // destruction when probing is on and the probe has room left.
+// The launches of a probed region go through probed_launch, which has the
+// dispatch itself stamp the events (hipExtLaunchKernel): the first launch
+// sets the start, every launch the stop, so the interval is the first
+// kernel's start to the last one's end, with no event packets in between.
 struct ProbeScope {
   mlx_ctx *c;
   int id;
   int slot = -1;
+  bool started = false;
   ProbeScope(mlx_ctx *c_, int id_, double bytes, double flops) : c(c_), id(id_) {
     ProbeSet &p = c->probe;
     if (!p.on || p.used[id] >= p.cap) return;
     slot = p.used[id]++;
     p.bytes[id] += bytes;
     p.flops[id] += flops;
-    (void)hipEventRecord(p.ev[id][2 * slot], c->stream);
   }
+  hipEvent_t start_event() {
+    if (slot < 0 || started) return nullptr;
+    started = true;
+    return c->probe.ev[id][2 * slot];
+  }
+  hipEvent_t stop_event() const { return slot >= 0 ? c->probe.ev[id][2 * slot + 1] : nullptr; }
   ~ProbeScope() {
-    if (slot >= 0) (void)hipEventRecord(c->probe.ev[id][2 * slot + 1], c->stream);
+    if (slot >= 0 && !started) {  // nothing launched: an empty interval
+      (void)hipEventRecord(c->probe.ev[id][2 * slot], c->stream);
+      (void)hipEventRecord(c->probe.ev[id][2 * slot + 1], c->stream);
+    }
   }
 };
+
+template <typename... Args, typename F>
+inline void probed_launch(ProbeScope &ps, F kernel, dim3 grid, dim3 block, unsigned shm, hipStream_t s,
+                          Args... args) {
+  hipExtLaunchKernelGGL(kernel, grid, block, shm, s, ps.start_event(), ps.stop_event(), 0u, args...);
+}
 
 // active (n, m) pairs for a radial chop offset: sum over m < npchop of nrcs(m)
 static inline double active_pairs(const KitDev &k, int nroff, int ncols = -1) {

@@ -643,7 +643,7 @@ void k_helm_solve(mlx_ctx *c, double2 *e, double alpha, int kind, int nroff, int
   const FacPtrs F = fac_ptrs(lu.sol, fac_rows(k, lu.seg));
   const int nrc = k.nrchop + nroff, sk = kind == 2 ? 1 : 0;
 #define MLX_LU_LAUNCH(SV, PV)                                                                              \
-  hipLaunchKernelGGL((k_lu_solve<SV, PV>), grid, dim3(64), 0, c->stream, e, F, lu.pvw, lu.anyp, k.nrdim, k.npdim, \
+  probed_launch(probe, k_lu_solve<SV, PV>, grid, dim3(64), 0, c->stream, e, F, lu.pvw, lu.anyp, k.nrdim, k.npdim, \
                      k.nzdim, nrc, k.npchop, k.chopzl, k.chopzu, sk, zlo, zhi)
 #define MLX_LU_BOTH(SV)                               \
   do {                                                \
