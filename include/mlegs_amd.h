@@ -257,8 +257,9 @@ int mlx_seed_spectral(double* host, int64_t nrdim, int64_t npdim, int64_t nzdim,
                       uint64_t seed, int32_t field_id, double amp, double scale, int32_t kind);
 
 /* Measurement probes, no counterpart in the reference.  After
- * mlx_probe_start, every launch of a probed kernel (MLX_PROBE_*) is bracketed
- * by HIP events on the context stream, up to max_launches per probe, and its
+ * mlx_probe_start, every launch of a probed kernel (MLX_PROBE_*) is timed by
+ * HIP events that the dispatch itself stamps (its start and its end, on the
+ * context stream), up to max_launches per probe, and its
  * algorithmic bytes and flops are summed.  mlx_probe_stop synchronises,
  * switches probing off and writes, per probe i, out[4i..4i+3] = launches
  * recorded, total kernel time in ms, algorithmic bytes, algorithmic flops. */
