@@ -850,6 +850,21 @@ int mlx_tp2curlvec(mlx_ctx *c, const mlx_field *psi, const mlx_field *chi, mlx_f
   });
 }
 
+int mlx_vorticity_magnitude(mlx_ctx *c, const mlx_field *psi, const mlx_field *chi, mlx_field *out) {
+  return guarded(c, [&] {
+    need_kit(c);
+    if (!is_fff(psi) || !is_fff(chi)) MLX_THROW(MLX_E_LAYOUT, "save_vort_mag: psi and chi must be in FFF space");
+    mlx_field *wr = tmp(c, 7), *wp = tmp(c, 8), *wz = tmp(c, 9);
+    if (out == psi || out == chi || out == wr || out == wp || out == wz)
+      MLX_THROW(MLX_E_ARG, "vorticity_magnitude: the output must be distinct from psi and chi");
+    tp2curlvec(c, psi, chi, wr, wp, wz);
+    k_vormag(c, out->e, wr->e, wp->e, wz->e);
+    k_set_scalar(c, out->sc, 0.0);
+    set_space(out, "PPP");
+    for (int i = 0; i < 3; ++i) out->off[i] = 0;
+  });
+}
+
 int mlx_advection_rhs(mlx_ctx *c, mlx_field *psi, mlx_field *chi, mlx_field *nlpsi, mlx_field *nlchi,
                       const mlx_field *uz) {
   return guarded(c, [&] {
@@ -957,8 +972,15 @@ int mlx_vortex_step(mlx_ctx *c, mlx_vortex *v, int32_t nsteps, double dt, int32_
       abcn(c, psi, psi_prev, nlpsi, nlpsi_prev, dt, false, true, &dl);
       abcn(c, chi, chi_prev, nlchi, nlchi_prev, dt, false, true, &dl);
       v->prev_stale = true;
-      dealias(c, psi);
-      dealias(c, chi);
+      // With the de-aliasing restrictions the right-hand side and the solve
+      // leave the de-aliased modes untouched, and the step's other inputs
+      // (the nonlinear terms) are zero there, so after the first step of a
+      // call psi and chi stay de-aliased and the dealias pass is a no-op.
+      // The first step keeps it: psi and chi come from the caller.
+      if (!dl.on || it == 0) {
+        dealias(c, psi);
+        dealias(c, chi);
+      }
       // with SVV on, its reduction pass also checks psi and chi for
       // non-finite entries: the filter and zeroat1 keep a finite field
       // finite and a non-finite one non-finite, so the flag is the one the
