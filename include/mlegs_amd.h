@@ -41,7 +41,7 @@ extern "C" {
 #define MLX_E_LAYOUT    201  /* size / layout / space mismatch            */
 #define MLX_E_ALLOC     202  /* device allocation failure                  */
 #define MLX_E_ARG       203  /* invalid argument                           */
-#define MLX_E_FILE      204  /* reserved (file I/O is not on this path)    */
+#define MLX_E_FILE      204  /* field file open / read / write failure     */
 #define MLX_E_NONFINITE 205  /* non-finite state (check_stability)         */
 #define MLX_E_HIP       206  /* HIP runtime failure                        */
 
@@ -224,6 +224,39 @@ int mlx_vortex_step(mlx_ctx* ctx, mlx_vortex* v, int32_t nsteps, double dt, int3
  * 5: chi_prev, 6: nlpsi_prev, 7: nlchi_prev, 8: uz) and read the SVV gains. */
 int mlx_vortex_field(mlx_ctx* ctx, mlx_vortex* v, int32_t which, mlx_field** out);
 int mlx_vortex_gains(mlx_ctx* ctx, mlx_vortex* v, double* gain_psi, double* gain_chi);
+
+/* save_vort_mag's field (vortical_flow_3d.f90:411-446): out (PPP, ln 0) =
+ * |curl u| of the flow (psi, chi) (FFF) at each point, computed on the real
+ * and imaginary parts separately (the two azimuthal points of a packed PPP
+ * entry).  Write it with mlx_field_save as the reference's msave does. */
+int mlx_vorticity_magnitude(mlx_ctx* ctx, const mlx_field* psi, const mlx_field* chi, mlx_field* out);
+
+/* ---- field files: msave / mload (src/submodules/mlegs_scalar_io.f90) ----- */
+
+/* msave(s, fn, is_binary, is_global) (mlegs_scalar_io.f90:6-101,
+ * mlegs_scalar.f90:431-439).  is_global != 0: one file `path` with the global
+ * array (nr+3, npdim, nzdim), the reference's layout; is_global == 0: the
+ * per-process file "<path>_0" with the global size, block size and block
+ * start (0,0,0) in front (one process holds the whole field).  is_binary: a
+ * Fortran stream file (int32 sizes, complex128 data, real64 ln, int32 chop
+ * offsets, 3-character space); else ASCII: a "(3(1X,I10))" size record, one
+ * record of 1PE24.15E3 (re, im) pairs per row, a blank record between z
+ * slices, then "ln offsets" and "space".  Errors: MLX_E_FILE. */
+int mlx_field_save(mlx_ctx* ctx, const mlx_field* f, const char* path, int32_t is_binary, int32_t is_global);
+/* mload(fn, s, is_binary, is_global) (mlegs_scalar_io.f90:103-248,
+ * mlegs_scalar.f90:442-450): reads a file of the format above into f, with
+ * its space, ln and chop offsets.  The sizes in the file must equal the
+ * kit's (MLX_E_FILE "mloadc: size inconsistency ..." otherwise, where the
+ * reference stops); only radial offsets are accepted (MLX_E_ARG). */
+int mlx_field_load(mlx_ctx* ctx, mlx_field* f, const char* path, int32_t is_binary, int32_t is_global);
+/* The same file formats for a host array e(n1, n2, n3) (complex128,
+ * column-major, interleaved re/im), with no context and no GPU.  The
+ * message of the last failure on the calling thread is mlx_file_last_error(). */
+int mlx_file_write(const char* path, const double* host, int64_t n1, int64_t n2, int64_t n3,
+                   const char* space, double ln, const int32_t* offsets, int32_t is_binary, int32_t is_global);
+int mlx_file_read(const char* path, double* host, int64_t n1, int64_t n2, int64_t n3,
+                  char* space4, double* ln, int32_t* offsets, int32_t is_binary, int32_t is_global);
+const char* mlx_file_last_error(void);
 
 /* ---- diagnostics (no reference counterpart; FP64 reductions) ------------- */
 

@@ -9,6 +9,7 @@ goes through the C ABI of ``libmlegs_amd.so`` to hand-written HIP kernels;
 errors raise ``MlxError`` (the reference stops the program).
 """
 import ctypes
+import os
 
 import numpy as np
 
@@ -315,6 +316,55 @@ def energy(vr, vp, vz, tfm):
     out = ctypes.c_double()
     _c(tfm, tfm.lib.mlx_energy(tfm.ctx, vr.h, vp.h, vz.h, ctypes.byref(out)))
     return out.value
+
+
+def vorticity_magnitude(psi, chi, out, tfm):
+    """save_vort_mag's field (vortical_flow_3d.f90:411-446): out (PPP) =
+    |curl u| at each point, on the real and imaginary parts separately."""
+    _c(tfm, tfm.lib.mlx_vorticity_magnitude(tfm.ctx, psi.h, chi.h, out.h))
+
+
+def msave(s, fn, is_binary=False, is_global=True):
+    """msave(s, fn, is_binary, is_global) (mlegs_scalar_io.f90:6-101): the
+    field as a global file `fn` or the per-process file `fn`_0."""
+    tfm = s.tfm
+    _c(tfm, tfm.lib.mlx_field_save(tfm.ctx, s.h, os.fsencode(fn), int(bool(is_binary)), int(bool(is_global))))
+
+
+def mload(fn, s, is_binary=False, is_global=True):
+    """mload(fn, s, is_binary, is_global) (mlegs_scalar_io.f90:103-248): data,
+    space, ln and chop offsets of s from a file msave wrote."""
+    tfm = s.tfm
+    _c(tfm, tfm.lib.mlx_field_load(tfm.ctx, s.h, os.fsencode(fn), int(bool(is_binary)), int(bool(is_global))))
+    return s
+
+
+def file_write(fn, e, space='FFF', ln=0.0, offsets=(0, 0, 0), is_binary=False, is_global=True):
+    """The msave file formats for a host array e(n1, n2, n3) (no GPU)."""
+    lib = _lib.load()
+    a = np.asfortranarray(e, dtype=np.complex128)
+    if a.ndim != 3:
+        raise ValueError('file_write: a 3D array is expected')
+    off = (ctypes.c_int32 * 3)(*offsets)
+    rc = lib.mlx_file_write(os.fsencode(fn), a.ctypes.data_as(_lib._DP), *a.shape, space.encode(), float(ln), off,
+                            int(bool(is_binary)), int(bool(is_global)))
+    if rc:
+        raise _lib.MlxError(rc, lib.mlx_file_last_error().decode())
+
+
+def file_read(fn, shape, is_binary=False, is_global=True):
+    """Read a file of the msave formats into a host array of `shape`;
+    returns (e, space, ln, offsets)."""
+    lib = _lib.load()
+    out = np.zeros(shape, dtype=np.complex128, order='F')
+    sp = ctypes.create_string_buffer(4)
+    ln = ctypes.c_double()
+    off = (ctypes.c_int32 * 3)()
+    rc = lib.mlx_file_read(os.fsencode(fn), out.ctypes.data_as(_lib._DP), *shape, sp, ctypes.byref(ln), off,
+                           int(bool(is_binary)), int(bool(is_global)))
+    if rc:
+        raise _lib.MlxError(rc, lib.mlx_file_last_error().decode())
+    return out, sp.value.decode(), ln.value, tuple(off)
 
 
 class VortexRun:

@@ -93,6 +93,12 @@ SIGNATURES = {
     'mlx_vortex_field': ([_P, _P, _I, ctypes.POINTER(_P)], ctypes.c_int),
     'mlx_vortex_gains': ([_P, _P, _DP, _DP], ctypes.c_int),
     'mlx_energy': ([_P, _P, _P, _P, _DP], ctypes.c_int),
+    'mlx_vorticity_magnitude': ([_P, _P, _P, _P], ctypes.c_int),
+    'mlx_field_save': ([_P, _P, _S, _I, _I], ctypes.c_int),
+    'mlx_field_load': ([_P, _P, _S, _I, _I], ctypes.c_int),
+    'mlx_file_write': ([_S, _DP, _I64, _I64, _I64, _S, _D, _IP, _I, _I], ctypes.c_int),
+    'mlx_file_read': ([_S, _DP, _I64, _I64, _I64, ctypes.c_char_p, _DP, _IP, _I, _I], ctypes.c_int),
+    'mlx_file_last_error': ([], ctypes.c_char_p),
     'mlx_time_kernel': ([_P, _P, _S, _I, _DP], ctypes.c_int),
     'mlx_probe_start': ([_P, _I], ctypes.c_int),
     'mlx_seed_spectral': ([_DP, _I64, _I64, _I64, _IP, ctypes.c_uint64, _I, ctypes.c_double, ctypes.c_double, _I],

@@ -281,6 +281,7 @@ void k_vecprod(mlx_ctx *c, double2 *vr, double2 *vp, double2 *vz, const double2 
 void k_vec2tp_up_ln(mlx_ctx *c, double2 *up, const double2 *inf, double *psiln);
 void k_nonfinite(mlx_ctx *c, const double2 *a, const double2 *b, int *flag);
 void k_energy(mlx_ctx *c, const double2 *vr, const double2 *vp, const double2 *vz, double *out);
+void k_vormag(mlx_ctx *c, double2 *out, const double2 *wr, const double2 *wp, const double2 *wz);
 void k_set_scalar(mlx_ctx *c, double *dst, double v);
 void k_copy_scalar(mlx_ctx *c, double *dst, const double *src, double scale);
 

@@ -822,6 +822,17 @@ __global__ void k_copy_field_(double2 *__restrict__ d, const double2 *__restrict
     d[i] = s[i];
 }
 
+// |w| at each point, on the real and the imaginary parts separately (the two
+// azimuthal points a packed PPP entry holds): save_vort_mag
+// (vortical_flow_3d.f90:433-443)
+__global__ void k_vormag_(double2 *__restrict__ o, const double2 *__restrict__ a, const double2 *__restrict__ b,
+                          const double2 *__restrict__ d, size_t n) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+    const double2 x = a[i], y = b[i], z = d[i];
+    o[i] = make_double2(sqrt(x.x * x.x + y.x * y.x + z.x * z.x), sqrt(x.y * x.y + y.y * y.y + z.y * z.y));
+  }
+}
+
 __global__ void k_set_scalar_(double *d, double v) {
   if (threadIdx.x == 0 && blockIdx.x == 0) d[0] = v;
 }
@@ -1027,13 +1038,15 @@ void k_nonfinite(mlx_ctx *c, const double2 *a, const double2 *b, int *flag) {
 void k_energy(mlx_ctx *c, const double2 *vr, const double2 *vp, const double2 *vz, double *out) {
   const KitDev &k = c->kit;
   const int nb = 1024;
-  double *part = c->dsmall + 3000;  // 1024 partials... keep clear of svv scratch
-  (void)part;
-  double *parts = c->dsmall;
+  double *parts = c->dsmall;  // 1024 partial sums
   double cell = (2.0 * 3.14159265358979323846 / k.np) * (k.zlen / k.nz);
   hipLaunchKernelGGL(k_energy_, dim3(nb), dim3(kT), 0, c->stream, vr, vp, vz, k.x, k.w, k.nr, k.nph, k.nrdim,
                      k.npdim, k.nzdim, k.ell * k.ell, cell, parts);
   hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(1024), 0, c->stream, parts, nb, out);
+}
+
+void k_vormag(mlx_ctx *c, double2 *out, const double2 *wr, const double2 *wp, const double2 *wz) {
+  hipLaunchKernelGGL(k_vormag_, dim3(grid_for(c->nelem)), dim3(kT), 0, c->stream, out, wr, wp, wz, c->nelem);
 }
 
 void k_set_scalar(mlx_ctx *c, double *dst, double v) {

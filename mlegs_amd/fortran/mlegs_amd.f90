@@ -103,9 +103,16 @@ module mlegs_amd
   public :: trans, chop, dealias, svv_filter, calcat0, calcat1, zeroat1
   public :: delsqp, idelsqp, xxdx, del2h, del2, idel2, ihelm, febe, abcn
   public :: vecprod, vec2tp, tp2vec, tp2curlvec, advection_rhs, check_stability, energy
+  public :: msave, mload, vorticity_magnitude
 
   interface assignment(=)
     module procedure scalar_copy
+  end interface
+  interface msave
+    module procedure msave_scalar
+  end interface
+  interface mload
+    module procedure mload_scalar
   end interface
   interface trans
     module procedure scalar_trans
@@ -315,6 +322,22 @@ module mlegs_amd
       import :: c_int, c_ptr, c_double
       type(c_ptr), value :: ctx, vr, vp, vz
       real(c_double), intent(out) :: out
+    end function
+    integer(c_int) function mlx_vorticity_magnitude(ctx, psi, chi, out) bind(C, name='mlx_vorticity_magnitude')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx, psi, chi, out
+    end function
+    integer(c_int) function mlx_field_save(ctx, f, path, is_binary, is_global) bind(C, name='mlx_field_save')
+      import :: c_int, c_ptr, c_char, c_int32_t
+      type(c_ptr), value :: ctx, f
+      character(kind=c_char), intent(in) :: path(*)
+      integer(c_int32_t), value :: is_binary, is_global
+    end function
+    integer(c_int) function mlx_field_load(ctx, f, path, is_binary, is_global) bind(C, name='mlx_field_load')
+      import :: c_int, c_ptr, c_char, c_int32_t
+      type(c_ptr), value :: ctx, f
+      character(kind=c_char), intent(in) :: path(*)
+      integer(c_int32_t), value :: is_binary, is_global
     end function
   end interface
 
@@ -904,5 +927,48 @@ contains
     type(scalar), intent(in) :: vr, vp, vz
     call check(tfm%ctx, mlx_energy(tfm%ctx, vr%h, vp%h, vz%h, e), 'energy')
   end function
+
+  !> |curl u| of (psi, chi) at each point, real and imaginary parts apart:
+  !> the field of save_vort_mag (vortical_flow_3d.f90:411-446), on the device
+  subroutine vorticity_magnitude(psi, chi, vormag)
+    type(scalar), intent(in) :: psi, chi
+    type(scalar), intent(inout) :: vormag
+    call need_set(tfm, 'vorticity_magnitude')
+    call check(tfm%ctx, mlx_vorticity_magnitude(tfm%ctx, psi%h, chi%h, vormag%h), 'vorticity_magnitude')
+  end subroutine
+
+  !> msave(s, fn, is_binary, is_global) (mlegs_scalar_io.f90:6-101): ASCII
+  !> unless is_binary; one global file unless is_global is .false. (then
+  !> fn_0, this process's block).  Defaults as the reference documents them.
+  subroutine msave_scalar(s, fn, is_binary, is_global)
+    type(scalar), intent(in) :: s
+    character(len=*), intent(in) :: fn
+    logical, intent(in), optional :: is_binary, is_global
+    integer(c_int32_t) :: b, g
+    b = 0; g = 1
+    if (present(is_binary)) then
+      if (is_binary) b = 1
+    endif
+    if (present(is_global)) then
+      if (.not. is_global) g = 0
+    endif
+    call check(s%ctx, mlx_field_save(s%ctx, s%h, trim(fn) // c_null_char, b, g), 'msave')
+  end subroutine
+
+  !> mload(fn, s, is_binary, is_global) (mlegs_scalar_io.f90:103-248)
+  subroutine mload_scalar(fn, s, is_binary, is_global)
+    character(len=*), intent(in) :: fn
+    type(scalar), intent(inout) :: s
+    logical, intent(in), optional :: is_binary, is_global
+    integer(c_int32_t) :: b, g
+    b = 0; g = 1
+    if (present(is_binary)) then
+      if (is_binary) b = 1
+    endif
+    if (present(is_global)) then
+      if (.not. is_global) g = 0
+    endif
+    call check(s%ctx, mlx_field_load(s%ctx, s%h, trim(fn) // c_null_char, b, g), 'mload')
+  end subroutine
 
 end module mlegs_amd

@@ -79,6 +79,7 @@ program vortical_flow_3d
   call qvort_dist_tp(psi, chi, q=1.0_p8, ran_noise=1.0e-10_p8)
   call uniform_z_fld(uz, b=-0.5_p8)
   write(*, *) 'Swirling Flow Field Declaration & Initialization'
+  if (isfldsav) call save_vort_mag(psi, chi)
 
   !> Richardson extrapolation bootstrapping (vortical_flow_3d.f90:116-147)
   call advection_rhs(psi, chi, nlpsi, nlchi, uz)
@@ -111,6 +112,7 @@ program vortical_flow_3d
   curr_n = 1; curr_t = dt
   call tfm%sync()
   write(*, *) 'Richardson Extrapolation (time stepping # 1)'
+  if (isfldsav) call save_vort_mag(psi, chi)
 
   !> main time stepping (vortical_flow_3d.f90:160-206)
   call system_clock(c0)
@@ -130,6 +132,9 @@ program vortical_flow_3d
     call zeroat1(psi, tfm); call zeroat1(chi, tfm)
     call advection_rhs(psi, chi, nlpsi, nlchi, uz)
     call check_stability(psi, chi)
+    if (isfldsav) then
+      if (mod(curr_n, fldsavintvl) .eq. 0) call save_vort_mag(psi, chi)
+    endif
     if (mod(curr_n, stepping_notice) .eq. 0) then
       call tfm%sync()
       call system_clock(c1)
@@ -241,6 +246,25 @@ contains
     glb(1:nr, 1:np/2, 1:nz) = cmplx(b, b, p8)
     call uz%disassemble(glb)
     deallocate(glb)
+  end subroutine
+
+  !> the vorticity magnitude field at time step curr_n, written as
+  !> <flddir>vormagfld<curr_n, 6 digits>_PPP in the msave ASCII format
+  !> (vortical_flow_3d.f90:411-459)
+  subroutine save_vort_mag(psi, chi)
+    type(scalar), intent(inout) :: psi, chi
+    type(scalar) :: vormag
+    character(len=16) :: tag
+    if (psi%get_space() .ne. 'FFF') error stop 'save_vort_mag: psi must be in FFF'
+    if (chi%get_space() .ne. 'FFF') error stop 'save_vort_mag: chi must be in FFF'
+    call vormag%init(glb_sz, [1, 0, 2])
+    call vormag%set_space('PPP')
+    call vorticity_magnitude(psi, chi, vormag)
+    write(tag, '(i0.6)') curr_n
+    call msave(vormag, trim(adjustl(flddir)) // 'vormagfld' // trim(tag) // '_' // vormag%get_space(), &
+               is_binary=.false., is_global=.true.)
+    write(*, *) 'Field Data Saved at Time Step #' // trim(tag)
+    call vormag%dealloc()
   end subroutine
 
   subroutine dump(s, fn)

@@ -2,7 +2,8 @@
 
 Restates the stepping of ``src/apps/vortical_flow_3d.f90``: advection_rhs
 (:353-395), the FE-BE + Richardson bootstrap (:116-147), the AB2/CN loop body
-(:160-181) and check_stability (:397-409).
+(:160-181), check_stability (:397-409) and the vorticity-magnitude field of
+save_vort_mag (:411-446).
 """
 import numpy as np
 
@@ -32,6 +33,21 @@ def advection_rhs(psi, chi, uz, kit):
     so.dealias(nlpsi, kit)
     so.dealias(nlchi, kit)
     return nlpsi, nlchi
+
+
+def vorticity_magnitude(psi, chi, kit):
+    """The field save_vort_mag writes (:411-446): |curl V| from tp2curlvec,
+    on the real and imaginary parts of each packed PPP entry separately (the
+    azimuthal points 2j-1 and 2j).  Returns a PPP scalar with ln = 0."""
+    for s in (psi, chi):
+        if s.space != 'FFF':
+            raise ValueError('save_vort_mag: psi and chi must be in FFF')
+    wr, wp, wz = so.tp2curlvec(psi, chi, kit)
+    out = so.Scalar(kit, 'PPP')
+    re = np.sqrt(wr.e.real ** 2 + wp.e.real ** 2 + wz.e.real ** 2)
+    im = np.sqrt(wr.e.imag ** 2 + wp.e.imag ** 2 + wz.e.imag ** 2)
+    out.e = re + 1j * im
+    return out
 
 
 def check_stability(psi, chi):

@@ -154,3 +154,41 @@ def test_fortran_driver_rejects_bad_input(tmp_path):
     r = subprocess.run([DRIVER, str(pfile)], cwd=str(tmp_path), capture_output=True, text=True, timeout=120)
     assert r.returncode != 0
     assert 'hyperpow' in (r.stdout + r.stderr)
+
+
+def test_fortran_driver_saves_vorticity_magnitude(tmp_path):
+    """ISFLDSAV = T: the driver writes save_vort_mag's field at the start, after
+    the bootstrap and every FLDSAVINTVL steps (vortical_flow_3d.f90:112, 157,
+    194-197) as <FLDDIR>vormagfld<n, 6 digits>_PPP in the msave ASCII format;
+    the last one equals |curl u| of the final psi, chi it dumps."""
+    import mlegs_amd as M
+    from mlegs_amd.params import read_input, timestep_set
+    print('[case] fortran vortical_flow_3d 48x32x16, field saves', flush=True)
+    assert os.path.exists(DRIVER)
+    pfile = tmp_path / 'input.params'
+    pfile.write_text(PARAMS.replace('              F            100', '              T              3'))
+    r = subprocess.run([DRIVER, str(pfile), str(tmp_path / 'dump')], cwd=str(tmp_path), capture_output=True,
+                       text=True, timeout=120)
+    print(r.stdout[-2000:], r.stderr[-2000:], flush=True)
+    assert r.returncode == 0
+    fld = tmp_path / 'out' / 'fld'
+    names = sorted(os.listdir(fld))
+    assert names == ['vormagfld000000_PPP', 'vormagfld000001_PPP', 'vormagfld000003_PPP']
+    assert 'Field Data Saved at Time Step #000003' in r.stdout
+    p = read_input(str(pfile), make_dirs=False)
+    timestep_set(p)
+    tfm = M.Kit(p, device=0)
+    fpsi, fpsi_ln = _load_dump(tmp_path / 'dump_psi.bin')
+    fchi, fchi_ln = _load_dump(tmp_path / 'dump_chi.bin')
+    psi = M.Scalar(tfm).upload(fpsi, space='FFF', ln=fpsi_ln)
+    chi = M.Scalar(tfm).upload(fchi, space='FFF', ln=fchi_ln)
+    vm = M.Scalar(tfm)
+    M.vorticity_magnitude(psi, chi, vm, tfm)
+    want = vm.e
+    got, sp, ln, off = M.file_read(str(fld / 'vormagfld000003_PPP'), tfm.dims)
+    assert sp == 'PPP' and ln == 0.0 and off == (0, 0, 0)
+    assert rel_err(got, want) < 1e-15
+    # the tutorials' reader takes the first z slice
+    d = np.loadtxt(str(fld / 'vormagfld000003_PPP'), skiprows=1, max_rows=tfm.dims[0])
+    assert d.shape == (tfm.dims[0], 2 * tfm.dims[1])
+    tfm.close()

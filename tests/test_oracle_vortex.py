@@ -3,6 +3,8 @@
   tp2vec / tp2curlvec (mlegs_scalar_ops.f90:1455-1560)  the q-vortex
       u_phi = (1 - e^{-r^2})/r, u_z = e^{-r^2}/q, w_z = 2 e^{-r^2},
       w_phi = 2 r e^{-r^2}/q  (docs/tutorial/vector_field.md:92-106)
+  save_vort_mag's |w|        2 e^{-r^2} sqrt(1 + r^2/q^2) of the same vortex
+  (vortical_flow_3d.f90:411-446)
   vec2tp (:1308-1453)        inverse of tp2vec on smooth fields; a gradient
                              field projects to zero
   vecprod (:1264-1306)       numpy's cross product on both packed phi points
@@ -66,6 +68,25 @@ def test_qvortex_velocity_and_vorticity():
     assert np.abs(_radial(wr, kit)).max() < 1e-14
     assert np.abs(_radial(wp, kit) - c * 2 * r * np.exp(-r ** 2) / Q).max() < 5e-12
     assert np.abs(_radial(wz, kit) - c * 2 * np.exp(-r ** 2)).max() < 5e-12
+
+
+def test_vorticity_magnitude_qvortex():
+    """save_vort_mag's field (vortical_flow_3d.f90:411-446) of the q-vortex:
+    |w| = 2 e^{-r^2} sqrt(1 + r^2/q^2) at both packed phi points (real and
+    imaginary parts), from the closed forms above."""
+    kit, p = _kit()
+    psi, chi = _qvortex(kit, p)
+    for s in (psi, chi):
+        so.trans(s, 'FFF', kit)
+    vm = V.vorticity_magnitude(psi, chi, kit)
+    assert vm.space == 'PPP' and vm.ln == 0.0
+    r = kit.r[:, None, None]
+    want = 2 * np.exp(-r ** 2) * np.sqrt(1 + r ** 2 / Q ** 2)
+    got = _radial(vm, kit)
+    assert np.abs(got.real - want).max() < 5e-12
+    assert np.abs(got.imag - want).max() < 5e-12
+    with pytest.raises(ValueError):
+        V.vorticity_magnitude(vm, chi, kit)
 
 
 def test_vec2tp_inverts_tp2vec_on_smooth_fields():
