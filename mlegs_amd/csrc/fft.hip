@@ -337,7 +337,7 @@ __global__ void __launch_bounds__(kThreads) k_fft4(double2 *out, const double2 *
     // stage 0: C_m, m = 0..N, into LDS (C_N apart)
     for (int b = tid; b < TI * (N + 1); b += kThreads) {
       const int l = b % TI, m = b / TI, i = i0 + l;
-      const double2 v = (i < rlim && m < mlim) ? in[base + (size_t)m * es + i] : make_double2(0.0, 0.0);
+      const double2 v = (i < rlim && m < mlim) ? ld_nt(&in[base + (size_t)m * es + i]) : make_double2(0.0, 0.0);
       if (m < N) buf[m * TI + l] = v;
       else nyq[l] = v;
     }
@@ -367,7 +367,7 @@ __global__ void __launch_bounds__(kThreads) k_fft4(double2 *out, const double2 *
           v[n1] = make_double2(E.x - O.y, E.y + O.x);
         } else {
           const bool zin = MODE == 0 && j >= zin_lo && j <= zin_hi;
-          double2 x = (i < rlim && !zin) ? in[base + (size_t)j * es + i] : make_double2(0.0, 0.0);
+          double2 x = (i < rlim && !zin) ? ld_nt(&in[base + (size_t)j * es + i]) : make_double2(0.0, 0.0);
           if (MODE == 1 && rscale > 0 && i < nr) {
             const double rr = r[i];
             x = make_double2(x.x * rr, x.y * rr);
@@ -425,9 +425,9 @@ __global__ void __launch_bounds__(kThreads) k_fft4(double2 *out, const double2 *
             double2 x = y2[t][k2];
             if (MODE == 0) x = make_double2(x.x * scale, x.y * scale);
             else if (dv != 1.0) x = make_double2(x.x / dv, x.y / dv);
-            out[base + (size_t)ko * es + i] = x;
+            st_nt(&out[base + (size_t)ko * es + i], x);
           }
-          if (MODE == 2 && k1 == 0) out[base + (size_t)N * es + i] = make_double2(0.0, 0.0);
+          if (MODE == 2 && k1 == 0) st_nt(&out[base + (size_t)N * es + i], make_double2(0.0, 0.0));
         }
       }
     }
@@ -454,7 +454,7 @@ __global__ void __launch_bounds__(kThreads) k_fft4(double2 *out, const double2 *
     const double2 D = make_double2(0.5 * (zm.x - zc.x), 0.5 * (zm.y - zc.y));
     const double2 O = make_double2(D.y, -D.x);  // -i * D
     const double2 C = cadd(E, cmul(twh[m], O));
-    if (i < rlim) out[base + (size_t)m * es + i] = make_double2(C.x * scale, C.y * scale);
+    if (i < rlim) st_nt(&out[base + (size_t)m * es + i], make_double2(C.x * scale, C.y * scale));
   }
 }
 
@@ -505,14 +505,14 @@ __global__ void __launch_bounds__(kThreads) k_fft4_cross(double2 *o0, double2 *o
         double2 a1 = z2, a2 = z2, a3 = z2, b1 = z2, b2 = z2, b3 = z2;
         if (i < nr) {
           const size_t o = base + (size_t)j * es + i;
-          a1 = v0[o];
-          a2 = v1[o];
-          a3 = v2[o];
-          b1 = w0[o];
-          b2 = w1[o];
-          b3 = w2[o];
+          a1 = ld_nt(&v0[o]);
+          a2 = ld_nt(&v1[o]);
+          a3 = ld_nt(&v2[o]);
+          b1 = ld_nt(&w0[o]);
+          b2 = ld_nt(&w1[o]);
+          b3 = ld_nt(&w2[o]);
           if (ub) {
-            const double2 u = ub[o];
+            const double2 u = ld_nt(&ub[o]);
             a3 = make_double2(a3.x + u.x, a3.y + u.y);
           }
         }
@@ -589,7 +589,7 @@ __global__ void __launch_bounds__(kThreads) k_fft4_cross(double2 *o0, double2 *o
       const double2 D = make_double2(0.5 * (zm.x - zc.x), 0.5 * (zm.y - zc.y));
       const double2 O = make_double2(D.y, -D.x);  // -i * D
       const double2 C = cadd(E, cmul(twh[m], O));
-      if (i < nrow && m < mmax) out[base + (size_t)m * es + i] = make_double2(C.x * scale, C.y * scale);
+      if (i < nrow && m < mmax) st_nt(&out[base + (size_t)m * es + i], make_double2(C.x * scale, C.y * scale));
     }
   }
 }

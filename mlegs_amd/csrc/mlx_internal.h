@@ -326,6 +326,19 @@ __device__ inline void xcd_map(int L, int nyt, int &u, int &y) {
 }
 inline dim3 xcd_grid(int nu, int nyt) { return dim3(((nu + kXcd - 1) / kXcd) * kXcd * nyt); }
 
+// Streaming (non-temporal) access to field data that a kernel reads or writes
+// once: the lines bypass the normal cache retention, so a pass over a 140 MB
+// field does not evict what the next kernels reuse (tables, twiddles).
+typedef double mlx_dv2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ double2 ld_nt(const double2 *p) {
+  const mlx_dv2 v = __builtin_nontemporal_load(reinterpret_cast<const mlx_dv2 *>(p));
+  return make_double2(v.x, v.y);
+}
+__device__ __forceinline__ void st_nt(double2 *p, double2 x) {
+  const mlx_dv2 v = {x.x, x.y};
+  __builtin_nontemporal_store(v, reinterpret_cast<mlx_dv2 *>(p));
+}
+
 // helpers shared by host code
 static inline int nrcs_of(const KitDev &k, int m, int nroff) {
   int nrc = k.nrchop + nroff;
