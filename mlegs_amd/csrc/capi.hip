@@ -449,12 +449,15 @@ void advection_rhs(mlx_ctx *c, mlx_field *psi, mlx_field *chi, mlx_field *nlpsi,
 // flag_dev (optional): the non-finite check of the field, on the same pass
 // dl (the time loop, right after de-aliasing f): the reduction skips the
 // de-aliased modes, which are zero
-void svv(mlx_ctx *c, mlx_field *f, double *gain_dev, int *flag_dev = nullptr, const DealiasLimits *dl = nullptr) {
+// f2 (optional): a second field filtered in the same launches, gain gain2
+void svv(mlx_ctx *c, mlx_field *f, double *gain_dev, int *flag_dev = nullptr, const DealiasLimits *dl = nullptr,
+         mlx_field *f2 = nullptr, double *gain2 = nullptr) {
   if (!c->p.is_svv) return;
-  if (!is_fff(f)) MLX_THROW(MLX_E_LAYOUT, "svv_filter: scalar must be in FFF space");
+  if (!is_fff(f) || (f2 && !is_fff(f2))) MLX_THROW(MLX_E_LAYOUT, "svv_filter: scalar must be in FFF space");
   need_kit(c);
   const bool lim = dl && dl->on;
-  k_svv(c, f->e, gain_dev, flag_dev, lim ? dl->pcut : -1, lim ? dl->zlo : 1, lim ? dl->zhi : 0);
+  k_svv(c, f->e, gain_dev, f2 ? f2->e : nullptr, gain2, flag_dev, lim ? dl->pcut : -1, lim ? dl->zlo : 1,
+        lim ? dl->zhi : 0);
 }
 
 void check_finite(mlx_ctx *c, const mlx_field *a, const mlx_field *b, int *flag_dev) {
@@ -505,6 +508,10 @@ int mlx_ctx_create(const mlx_params *p, int device, void *stream, mlx_ctx **out)
       c->own_stream = true;
     }
     HIPCHK(hipMalloc(&c->dsmall, sizeof(double) * 4096));
+    // zero: the SVV reduction's completion counter starts at 0 (its last
+    // workgroup resets it after each use)
+    HIPCHK(hipMemsetAsync(c->dsmall, 0, sizeof(double) * 4096, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
     HIPCHK(hipHostMalloc(&c->hsmall, sizeof(double) * 4096));
   });
   if (rc != MLX_OK) {
@@ -985,8 +992,7 @@ int mlx_vortex_step(mlx_ctx *c, mlx_vortex *v, int32_t nsteps, double dt, int32_
       // non-finite entries: the filter and zeroat1 keep a finite field
       // finite and a non-finite one non-finite, so the flag is the one the
       // check after the step would raise
-      svv(c, psi, v->gains, v->flag, &dl);
-      svv(c, chi, v->gains + 1, v->flag, &dl);
+      svv(c, psi, v->gains, v->flag, &dl, chi, v->gains + 1);
       zeroat1(c, psi);
       zeroat1(c, chi);
       advection_rhs(c, psi, chi, nlpsi, nlchi, uz, true);

@@ -244,8 +244,10 @@ void vec2tp_project(mlx_ctx *c, const double2 *ur, const double2 *up, const doub
 void k_ln_column(mlx_ctx *c, double2 *e, const double *ln, double sign);
 void k_chop(mlx_ctx *c, double2 *e, const char *space, int nroff);
 void k_dealias(mlx_ctx *c, double2 *e, const char *space);
-void k_svv(mlx_ctx *c, double2 *e, double *gain_dev, int *flag = nullptr, int pcut = -1, int zlo = 1,
-           int zhi = 0);
+// svv_filter on e0 (gain g0) and, when e1 is given, on e1 (gain g1) in the
+// same two launches
+void k_svv(mlx_ctx *c, double2 *e0, double *g0, double2 *e1 = nullptr, double *g1 = nullptr, int *flag = nullptr,
+           int pcut = -1, int zlo = 1, int zhi = 0);
 void k_calcat(mlx_ctx *c, const double2 *e, const double *tbl, int nrc, double2 *out);
 void k_zeroat1(mlx_ctx *c, double2 *e, int nrc);
 void k_delsqp(mlx_ctx *c, double2 *e, double *ln, int nroff, int inverse);

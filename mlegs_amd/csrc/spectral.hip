@@ -91,66 +91,92 @@ __device__ inline double svv_q(int i, int j, int k, double inv_r, double inv_p, 
   return fmin(1.0, fmax(fmax(qr, qp), qz));
 }
 
-// flag (optional): set to 1 when any entry is not finite, the stability
-// check of the time loop taken on the same pass over the field
-// Entries of columns j >= pcut and slices zlo..zhi are skipped (the caller
-// has just de-aliased the field: they are zero and add nothing).
-__global__ void k_svv_reduce(const double2 *e, int nrdim, int npdim, int nz, double inv_r, double inv_p,
-                             const double *ak, double kmax, double cutoff, double *part, int *flag, int pcut, int zlo,
-                             int zhi) {
+// SVV on one or two fields (blockIdx.z / blockIdx.y selects the field).
+// Reduction: workgroup (bx, k) sums |c|^2 and the tail (q >= cutoff) over
+// slice k of columns j < pcut; slices zlo..zhi are skipped (the caller has
+// just de-aliased the field: they are zero and add nothing).  flag
+// (optional): set to 1 when an entry is not finite, the stability check of
+// the time loop taken on the same pass.  The last workgroup to finish (a
+// device counter, reset by that workgroup) adds the partial sums in a fixed
+// order and updates each field's gain and strength
+// (mlegs_scalar_ops.f90:72-126): out[3 f] = strength, out[3 f + 1] = total,
+// out[3 f + 2] = tail.
+struct SvvFields {
+  double2 *e[2];
+  double *gain[2];
+};
+
+__global__ void k_svv_reduce(SvvFields F, int nrdim, int npdim, double inv_r, double inv_p, const double *ak,
+                             double kmax, double cutoff, double *part, unsigned *count, double *out, int *flag,
+                             int pcut, int zlo, int zhi, double target, double relax, double strength0) {
   __shared__ double sh[16];
-  size_t n = (size_t)nrdim * npdim * nz;
+  __shared__ bool last;
+  const int f = blockIdx.z, k = blockIdx.y, nbx = gridDim.x;
   double tot = 0.0, tail = 0.0;
   int bad = 0;
-  for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (size_t)gridDim.x * blockDim.x) {
-    int i = (int)(t % nrdim);
-    int j = (int)((t / nrdim) % npdim);
-    int k = (int)(t / ((size_t)nrdim * npdim));
-    if (j >= pcut || (k >= zlo && k <= zhi)) continue;
-    double2 v = e[t];
-    if (!isfinite(v.x) || !isfinite(v.y)) bad = 1;
-    double a2 = v.x * v.x + v.y * v.y;
-    tot += a2;
-    if (svv_q(i, j, k, inv_r, inv_p, ak, kmax) >= cutoff) tail += a2;
+  if (k < zlo || k > zhi) {
+    const double qz = fmin(1.0, fabs(ak[k]) / kmax);
+    const int ncol = pcut < npdim ? pcut : npdim;
+    const int plane = nrdim * ncol;
+    const double2 *ep = F.e[f] + (size_t)k * nrdim * npdim;
+    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < plane; t += nbx * blockDim.x) {
+      const int j = t / nrdim, i = t - j * nrdim;
+      const double2 v = ld_nt(&ep[t]);
+      if (!isfinite(v.x) || !isfinite(v.y)) bad = 1;
+      const double a2 = v.x * v.x + v.y * v.y;
+      tot += a2;
+      const double q = fmin(1.0, fmax(fmax(fmin(1.0, (double)i * inv_r), fmin(1.0, (double)j * inv_p)), qz));
+      if (q >= cutoff) tail += a2;
+    }
   }
   if (flag && bad) atomicOr(flag, 1);
-  double a = block_sum(tot, sh);
-  double b = block_sum(tail, sh);
+  const double a = block_sum(tot, sh);
+  const double b = block_sum(tail, sh);
+  const int nb = nbx * gridDim.y;
+  double *pf = part + (size_t)f * 2 * nb;
+  const int slot = k * nbx + blockIdx.x;
   if (threadIdx.x == 0) {
-    part[2 * blockIdx.x] = a;
-    part[2 * blockIdx.x + 1] = b;
+    pf[2 * slot] = a;
+    pf[2 * slot + 1] = b;
+    __threadfence();
+    const unsigned total = (unsigned)(nb * gridDim.z);
+    last = (atomicAdd(count, 1u) == total - 1);
   }
+  __syncthreads();
+  if (!last) return;
+  __threadfence();
+  for (int g = 0; g < (int)gridDim.z; ++g) {
+    // the other workgroups' partial sums: read past this CU's cache
+    const volatile double *pg = part + (size_t)g * 2 * nb;
+    double x = 0.0, y = 0.0;
+    for (int i = threadIdx.x; i < nb; i += blockDim.x) {
+      x += pg[2 * i];
+      y += pg[2 * i + 1];
+    }
+    const double T = block_sum(x, sh), L = block_sum(y, sh);
+    if (threadIdx.x == 0) {
+      double *o = out + 3 * g;
+      o[1] = T;
+      o[2] = L;
+      if (T <= 2.2250738585072014e-308) {
+        o[0] = 0.0;
+      } else {
+        const double feedback = fmin(fmax((L / T) / target - 1.0, 0.0), 1.0);
+        const double gn = (1.0 - relax) * F.gain[g][0] + relax * feedback;
+        F.gain[g][0] = gn;
+        o[0] = fmin(strength0 * gn, 1.0);
+      }
+    }
+  }
+  if (threadIdx.x == 0) count[0] = 0u;
 }
 
-// gain/strength update (mlegs_scalar_ops.f90:117-126); out[0] = strength
-__global__ void k_svv_gain(const double *part, int nb, double *gain, double *out, double target, double relax,
-                           double strength0) {
-  __shared__ double sh[16];
-  double a = 0.0, b = 0.0;
-  for (int i = threadIdx.x; i < nb; i += blockDim.x) {
-    a += part[2 * i];
-    b += part[2 * i + 1];
-  }
-  double tot = block_sum(a, sh);
-  double tail = block_sum(b, sh);
-  if (threadIdx.x != 0) return;
-  out[1] = tot;
-  out[2] = tail;
-  if (tot <= 2.2250738585072014e-308) {
-    out[0] = 0.0;
-    return;
-  }
-  double ratio = tail / tot;
-  double feedback = fmin(fmax(ratio / target - 1.0, 0.0), 1.0);
-  double g = (1.0 - relax) * gain[0] + relax * feedback;
-  gain[0] = g;
-  out[0] = fmin(strength0 * g, 1.0);
-}
-
-__global__ void k_svv_apply(double2 *e, int nrdim, int npdim, int nz, double inv_r, double inv_p,
-                            const double *ak, double kmax, double cutoff, const double *strength) {
-  double st = strength[0];
+__global__ void k_svv_apply(SvvFields F, int nrdim, int npdim, int nz, double inv_r, double inv_p,
+                            const double *ak, double kmax, double cutoff, const double *out) {
+  const int f = blockIdx.y;
+  const double st = out[3 * f];
   if (!(st > 0.0)) return;
+  double2 *e = F.e[f];
   size_t n = (size_t)nrdim * npdim * nz;
   for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (size_t)gridDim.x * blockDim.x) {
     int i = (int)(t % nrdim);
@@ -160,9 +186,9 @@ __global__ void k_svv_apply(double2 *e, int nrdim, int npdim, int nz, double inv
     if (q > cutoff) {
       double s = (q - cutoff) / (1.0 - cutoff);
       double s2 = s * s, s4 = s2 * s2;
-      double f = exp(-st * (s4 * s4));
+      double f2 = exp(-st * (s4 * s4));
       double2 v = e[t];
-      e[t] = make_double2(f * v.x, f * v.y);
+      e[t] = make_double2(f2 * v.x, f2 * v.y);
     }
   }
 }
@@ -461,7 +487,7 @@ __global__ void __launch_bounds__(256) k_abcn_rhs_d2_(double2 *e, const double *
   }
   const double hdt = dt / 2.0;
   for (int i = threadIdx.x; i < nrdim; i += blockDim.x) {
-    const double2 x = S[i], y = nl[off + i], z = nlp[off + i];
+    const double2 x = S[i], y = ld_nt(&nl[off + i]), z = ld_nt(&nlp[off + i]);
     const double2 v = (i < nn) ? V[i] : x;
     const double tx = x.x + dt * (1.5 * y.x - 0.5 * z.x);
     const double ty = x.y + dt * (1.5 * y.y - 0.5 * z.y);
@@ -878,7 +904,7 @@ void k_dealias(mlx_ctx *c, double2 *e, const char *space) {
                      pcut, zcut, zupper);
 }
 
-void k_svv(mlx_ctx *c, double2 *e, double *gain_dev, int *flag, int pcut, int zlo, int zhi) {
+void k_svv(mlx_ctx *c, double2 *e0, double *g0, double2 *e1, double *g1, int *flag, int pcut, int zlo, int zhi) {
   const KitDev &k = c->kit;
   const mlx_params &p = c->p;
   double inv_r = 1.0 / fmax((double)(k.nrchop - 1), 1.0);
@@ -889,14 +915,22 @@ void k_svv(mlx_ctx *c, double2 *e, double *gain_dev, int *flag, int pcut, int zl
   double target = fmax(p.svv_target, 1e-12);
   double relax = fmin(fmax(p.svv_relax, 0.0), 1.0);
   double st0 = fmax(p.svv_strength, 0.0);
-  const int nb = 1024;
+  const int nf = e1 ? 2 : 1;
+  SvvFields F{{e0, e1 ? e1 : e0}, {g0, g1 ? g1 : g0}};
+  // partial sums: at most 512 workgroups per field (dsmall[0, 2048))
+  const int ncol = (pcut > 0 && pcut < k.npdim) ? pcut : k.npdim;
+  const int per_slice = (k.nrdim * ncol + kT - 1) / kT;
+  int nbx = 512 / k.nzdim;
+  if (nbx > per_slice) nbx = per_slice;
+  if (nbx < 1) nbx = 1;
+  if ((size_t)nbx * k.nzdim > 512) MLX_THROW(MLX_E_ARG, "svv_filter: nz above 512 is not supported");
   double *part = c->dsmall;
-  double *out = c->dsmall + 2 * nb;
-  hipLaunchKernelGGL(k_svv_reduce, dim3(nb), dim3(kT), 0, c->stream, e, k.nrdim, k.npdim, k.nzdim, inv_r, inv_p,
-                     k.ak, kmax, cutoff, part, flag, pcut > 0 ? pcut : k.npdim, zlo, zhi);
-  hipLaunchKernelGGL(k_svv_gain, dim3(1), dim3(1024), 0, c->stream, part, nb, gain_dev, out, target, relax, st0);
-  hipLaunchKernelGGL(k_svv_apply, dim3(grid_for(c->nelem)), dim3(kT), 0, c->stream, e, k.nrdim, k.npdim, k.nzdim,
-                     inv_r, inv_p, k.ak, kmax, cutoff, out);
+  double *out = c->dsmall + 2048;
+  unsigned *count = reinterpret_cast<unsigned *>(c->dsmall + 4000 + 40);
+  hipLaunchKernelGGL(k_svv_reduce, dim3(nbx, k.nzdim, nf), dim3(kT), 0, c->stream, F, k.nrdim, k.npdim, inv_r,
+                     inv_p, k.ak, kmax, cutoff, part, count, out, flag, ncol, zlo, zhi, target, relax, st0);
+  hipLaunchKernelGGL(k_svv_apply, dim3(grid_for(c->nelem), nf), dim3(kT), 0, c->stream, F, k.nrdim, k.npdim,
+                     k.nzdim, inv_r, inv_p, k.ak, kmax, cutoff, out);
 }
 
 void k_calcat(mlx_ctx *c, const double2 *e, const double *tbl, int nrc, double2 *out) {
