@@ -157,6 +157,15 @@ void zeroat1(mlx_ctx *c, mlx_field *f) {
   k_zeroat1(c, f->e, c->kit.nrchop + f->off[0]);
 }
 
+// zeroat1 on two fields in one launch; ln_zero: their log coefficients are
+// then set to zero by the same launch
+void zeroat1_pair(mlx_ctx *c, mlx_field *a, mlx_field *b, bool ln_zero = false) {
+  to_fff(c, a);
+  to_fff(c, b);
+  k_zeroat1(c, a->e, c->kit.nrchop + a->off[0], b->e, c->kit.nrchop + b->off[0], ln_zero ? a->sc : nullptr,
+            ln_zero ? b->sc : nullptr);
+}
+
 void bandmul(mlx_ctx *c, mlx_field *f, int op) {
   need_kit(c);
   to_fff(c, f);
@@ -340,8 +349,10 @@ void phi_forward_from(mlx_ctx *c, mlx_field *dst, const mlx_field *src, int rsca
 // dl.on (with pre_dealias): the result is de-aliased by the projection
 // itself, which computes columns m < pcut and slices outside zlo..zhi only;
 // the transforms before it then skip the columns it does not read.
+// ln_zero: psi.ln and chi.ln are set to zero at the end (advection_rhs's
+// nl.ln = 0), by the closing zeroat1 launch
 void vec2tp_pfp(mlx_ctx *c, mlx_field *ur, mlx_field *up, mlx_field *uz, mlx_field *psi, mlx_field *chi,
-                bool pre_dealias, DealiasLimits dl = DealiasLimits()) {
+                bool pre_dealias, DealiasLimits dl = DealiasLimits(), bool ln_zero = false) {
   const KitDev &k = c->kit;
   if (!pre_dealias) dl = DealiasLimits();
   const int nc = dl.on ? dl.pcut : k.npchop;
@@ -388,8 +399,7 @@ void vec2tp_pfp(mlx_ctx *c, mlx_field *ur, mlx_field *up, mlx_field *uz, mlx_fie
     chop(c, psi);
     chop(c, chi);
   }
-  zeroat1(c, psi);
-  zeroat1(c, chi);
+  zeroat1_pair(c, psi, chi, ln_zero);
 }
 
 void vec2tp(mlx_ctx *c, const mlx_field *vr, const mlx_field *vp, const mlx_field *vz, mlx_field *psi,
@@ -436,14 +446,14 @@ void advection_rhs(mlx_ctx *c, mlx_field *psi, mlx_field *chi, mlx_field *nlpsi,
       f->off[0] = f->off[1] = f->off[2] = 0;
       set_space(f, "PFP");
     }
-    vec2tp_pfp(c, ur, up, uzp, nlpsi, nlchi, true, dl);
+    vec2tp_pfp(c, ur, up, uzp, nlpsi, nlchi, true, dl, true);
   } else {
     k_axpby(c, vz->e, vz->sc, 1.0, uz->e, nullptr, 1.0);
     vecprod(c, vr, vp, vz, wr, wp, wz);
     vec2tp(c, vr, vp, vz, nlpsi, nlchi, true);
+    k_set_scalar(c, nlpsi->sc, 0.0);
+    k_set_scalar(c, nlchi->sc, 0.0);
   }
-  k_set_scalar(c, nlpsi->sc, 0.0);
-  k_set_scalar(c, nlchi->sc, 0.0);
 }
 
 // flag_dev (optional): the non-finite check of the field, on the same pass
@@ -993,8 +1003,7 @@ int mlx_vortex_step(mlx_ctx *c, mlx_vortex *v, int32_t nsteps, double dt, int32_
       // finite and a non-finite one non-finite, so the flag is the one the
       // check after the step would raise
       svv(c, psi, v->gains, v->flag, &dl, chi, v->gains + 1);
-      zeroat1(c, psi);
-      zeroat1(c, chi);
+      zeroat1_pair(c, psi, chi);
       advection_rhs(c, psi, chi, nlpsi, nlchi, uz, true);
       if (!c->p.is_svv) check_finite(c, psi, chi, v->flag);
     }

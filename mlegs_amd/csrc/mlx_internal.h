@@ -249,7 +249,10 @@ void k_dealias(mlx_ctx *c, double2 *e, const char *space);
 void k_svv(mlx_ctx *c, double2 *e0, double *g0, double2 *e1 = nullptr, double *g1 = nullptr, int *flag = nullptr,
            int pcut = -1, int zlo = 1, int zhi = 0);
 void k_calcat(mlx_ctx *c, const double2 *e, const double *tbl, int nrc, double2 *out);
-void k_zeroat1(mlx_ctx *c, double2 *e, int nrc);
+// zeroat1 on e (rows < nrc) and, when e2 is given, on e2 in the same launch;
+// lnz / lnz2 (optional): log coefficients set to zero by that launch
+void k_zeroat1(mlx_ctx *c, double2 *e, int nrc, double2 *e2 = nullptr, int nrc2 = 0, double *lnz = nullptr,
+               double *lnz2 = nullptr);
 void k_delsqp(mlx_ctx *c, double2 *e, double *ln, int nroff, int inverse);
 void k_bandmul(mlx_ctx *c, double2 *e, double *ln, int op, int nroff);  // op 0 del2, 1 del2h, 2 xxdx
 void k_helm_pre(mlx_ctx *c, double2 *e, double *ln_out, const double *ln_in, double alpha);

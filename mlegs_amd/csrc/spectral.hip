@@ -119,14 +119,26 @@ __global__ void k_svv_reduce(SvvFields F, int nrdim, int npdim, double inv_r, do
     const int ncol = pcut < npdim ? pcut : npdim;
     const int plane = nrdim * ncol;
     const double2 *ep = F.e[f] + (size_t)k * nrdim * npdim;
-    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < plane; t += nbx * blockDim.x) {
-      const int j = t / nrdim, i = t - j * nrdim;
-      const double2 v = ld_nt(&ep[t]);
-      if (!isfinite(v.x) || !isfinite(v.y)) bad = 1;
-      const double a2 = v.x * v.x + v.y * v.y;
-      tot += a2;
-      const double q = fmin(1.0, fmax(fmax(fmin(1.0, (double)i * inv_r), fmin(1.0, (double)j * inv_p)), qz));
-      if (q >= cutoff) tail += a2;
+    // U entries per thread and round, their loads issued together
+    constexpr int U = 8;
+    const int stride = nbx * blockDim.x;
+    for (int t0 = blockIdx.x * blockDim.x + threadIdx.x; t0 < plane; t0 += U * stride) {
+      double2 v[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int t = t0 + u * stride;
+        v[u] = t < plane ? ld_nt(&ep[t]) : make_double2(0.0, 0.0);
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int t = t0 + u * stride;
+        const int j = t / nrdim, i = t - j * nrdim;
+        if (!isfinite(v[u].x) || !isfinite(v[u].y)) bad = 1;
+        const double a2 = v[u].x * v[u].x + v[u].y * v[u].y;
+        tot += a2;
+        const double q = fmin(1.0, fmax(fmax(fmin(1.0, (double)i * inv_r), fmin(1.0, (double)j * inv_p)), qz));
+        if (t < plane && q >= cutoff) tail += a2;
+      }
     }
   }
   if (flag && bad) atomicOr(flag, 1);
@@ -208,10 +220,17 @@ __global__ void k_calcat_(const double2 *e, const double *tbl, int n, int nrdim,
   if (threadIdx.x == 0) out[k] = make_double2(a, b);
 }
 
-__global__ void k_zeroat1_(double2 *e, const double *at1, int n, int nrdim, int npdim) {
+// zeroat1 on one or two fields (blockIdx.y selects the field); lnz[f]
+// (optional): the field's log coefficient, set to zero by the same launch
+struct Zat1Fields {
+  double2 *e[2];
+  int n[2];
+  double *lnz[2];
+};
+__global__ void k_zeroat1_(Zat1Fields F, const double *at1, int nrdim, int npdim) {
   __shared__ double sh[16];
-  int k = blockIdx.x;
-  double2 *c = e + (size_t)k * nrdim * npdim;
+  const int k = blockIdx.x, f = blockIdx.y, n = F.n[f];
+  double2 *c = F.e[f] + (size_t)k * nrdim * npdim;
   double sr = 0.0, si = 0.0;
   for (int i = threadIdx.x; i < n; i += blockDim.x) {
     sr += c[i].x * at1[i];
@@ -222,6 +241,7 @@ __global__ void k_zeroat1_(double2 *e, const double *at1, int n, int nrdim, int 
   if (threadIdx.x == 0) {
     double d = at1[0];
     c[0] = make_double2(c[0].x - a / d, c[0].y - b / d);
+    if (k == 0 && F.lnz[f]) F.lnz[f][0] = 0.0;
   }
 }
 
@@ -459,11 +479,16 @@ __device__ inline void band_pow(double2 *y, const double2 *x, double2 *tmp, int 
 // where del2 acts on the rows of s's radial offset (applied as often as the
 // kk-loops visit the slice, log-term correction of (0,0) after) and leaves
 // every other entry equal to s, as the reference's copy-then-del2 does.
-__global__ void __launch_bounds__(256) k_abcn_rhs_d2_(double2 *e, const double *sln, const double2 *nl,
+// pre (nlln, nlpln given): k_abcn_ln_ followed by ihelm's log-term step
+// (k_helm_pre_ with alpha = a), in the (0,0) workgroup: ln <- a (ln + dt
+// (1.5 nl.ln - 0.5 nl_p.ln)) / a, and rows 0..2 of the (0,0) column lose the
+// log term's Laplacian.  No other workgroup reads ln.
+__global__ void __launch_bounds__(256) k_abcn_rhs_d2_(double2 *e, double *sln, const double2 *nl,
                                                       const double2 *nlp, const double *d2c, const double *lognorm,
                                                       const double *ak, int nrdim, int npdim, int ne, int nrc,
                                                       int npc, int nzc, int nzcu, double ell, double dt, double a,
-                                                      double visc, int pcut, int zlo, int zhi) {
+                                                      double visc, int pcut, int zlo, int zhi, const double *nlln,
+                                                      const double *nlpln) {
   extern __shared__ double2 lds[];
   double2 *S = lds, *V = lds + nrdim, *T = lds + 2 * nrdim;
   const int m = blockIdx.x, k = blockIdx.y;
@@ -486,13 +511,30 @@ __global__ void __launch_bounds__(256) k_abcn_rhs_d2_(double2 *e, const double *
     __syncthreads();
   }
   const double hdt = dt / 2.0;
+  const bool pre = nlln && m == 0 && k == 0;
+  double l = 0.0;
+  if (pre) {
+    const double sh = a * (sln[0] + dt * (1.5 * nlln[0] - 0.5 * nlpln[0]));
+    l = sh / a;
+  }
+  const double ell2 = ell * ell;
   for (int i = threadIdx.x; i < nrdim; i += blockDim.x) {
     const double2 x = S[i], y = ld_nt(&nl[off + i]), z = ld_nt(&nlp[off + i]);
     const double2 v = (i < nn) ? V[i] : x;
     const double tx = x.x + dt * (1.5 * y.x - 0.5 * z.x);
     const double ty = x.y + dt * (1.5 * y.y - 0.5 * z.y);
     const double vx = v.x * visc, vy = v.y * visc;
-    e[off + i] = make_double2(a * (tx + hdt * vx), a * (ty + hdt * vy));
+    double2 o = make_double2(a * (tx + hdt * vx), a * (ty + hdt * vy));
+    if (pre) {
+      if (i == 0) o.x -= 4.0 / 3.0 / ell2 / exp(lognorm[0]) * l;
+      if (i == 1) o.x += 2.0 / 1.0 / ell2 / exp(lognorm[1]) * l;
+      if (i == 2) o.x -= 2.0 / 3.0 / ell2 / exp(lognorm[2]) * l;
+    }
+    e[off + i] = o;
+  }
+  if (pre) {
+    __syncthreads();  // every read of sln above is done
+    if (threadIdx.x == 0) sln[0] = l;
   }
 }
 
@@ -500,21 +542,6 @@ __global__ void __launch_bounds__(256) k_abcn_rhs_d2_(double2 *e, const double *
 __global__ void k_abcn_ln_(double *shln, const double *sln, const double *nlln, const double *nlpln, double dt,
                            double a) {
   shln[0] = a * (sln[0] + dt * (1.5 * nlln[0] - 0.5 * nlpln[0]));
-}
-
-// k_abcn_ln_ followed by ihelm's log-term step (k_helm_pre_ with alpha = a)
-// in one launch: ln <- a (ln + dt (1.5 nl.ln - 0.5 nl_p.ln)) / a, and the
-// (0,0) column loses the log term's Laplacian
-__global__ void k_abcn_ln_pre_(double2 *e, double *ln, const double *nlln, const double *nlpln, double dt, double a,
-                               const double *lognorm, double ell) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
-  const double sh = a * (ln[0] + dt * (1.5 * nlln[0] - 0.5 * nlpln[0]));
-  const double l = sh / a;
-  ln[0] = l;
-  const double ell2 = ell * ell;
-  e[0].x -= 4.0 / 3.0 / ell2 / exp(lognorm[0]) * l;
-  e[1].x += 2.0 / 1.0 / ell2 / exp(lognorm[1]) * l;
-  e[2].x -= 2.0 / 3.0 / ell2 / exp(lognorm[2]) * l;
 }
 
 __global__ void k_scale_rows_(double2 *e, const double *r, int nr, int nrdim, int npdim, int nz, int inverse) {
@@ -940,11 +967,15 @@ void k_calcat(mlx_ctx *c, const double2 *e, const double *tbl, int nrc, double2 
   hipLaunchKernelGGL(k_calcat_, dim3(k.nzdim), dim3(kT), 0, c->stream, e, tbl, n, k.nrdim, k.npdim, out);
 }
 
-void k_zeroat1(mlx_ctx *c, double2 *e, int nrc) {
+void k_zeroat1(mlx_ctx *c, double2 *e, int nrc, double2 *e2, int nrc2, double *lnz, double *lnz2) {
   const KitDev &k = c->kit;
-  int n = nrc < k.nrchop ? nrc : k.nrchop;
-  if (n > k.nrdim) n = k.nrdim;
-  hipLaunchKernelGGL(k_zeroat1_, dim3(k.nzdim), dim3(kT), 0, c->stream, e, k.at1, n, k.nrdim, k.npdim);
+  auto rows = [&](int nrc_) {
+    int n = nrc_ < k.nrchop ? nrc_ : k.nrchop;
+    return n > k.nrdim ? k.nrdim : n;
+  };
+  const int nf = e2 ? 2 : 1;
+  Zat1Fields F{{e, e2 ? e2 : e}, {rows(nrc), rows(e2 ? nrc2 : nrc)}, {lnz, e2 ? lnz2 : nullptr}};
+  hipLaunchKernelGGL(k_zeroat1_, dim3(k.nzdim, nf), dim3(kT), 0, c->stream, F, k.at1, k.nrdim, k.npdim);
 }
 
 void k_delsqp(mlx_ctx *c, double2 *e, double *ln, int nroff, int inverse) {
@@ -1012,15 +1043,15 @@ void k_abcn_rhs_d2(mlx_ctx *c, double2 *e, double *ln, const double2 *nl, const 
   const int nrc = k.nrchop + nroff;
   if (nrc > k.nrdim_ref) MLX_THROW(MLX_E_ARG, "chop_index: chopping in r too large");
   if (pcut <= 0 || pcut > k.npdim) pcut = k.npdim;
+  if (helm_pre && zlo <= 0 && zhi >= 0) MLX_THROW(MLX_E_ARG, "abcn: the (0,0) column must be in the solved range");
   dim3 grid(pcut, k.nzdim);
   size_t lds = 3 * sizeof(double2) * (size_t)k.nrdim;
   // the log coefficient first: the data kernel reads s's ln for the (0,0) correction
+  // with helm_pre the (0,0) workgroup also takes ihelm's log-term step
   hipLaunchKernelGGL(k_abcn_rhs_d2_, grid, dim3(256), lds, c->stream, e, ln, nl, nlp, k.d2c, k.lognorm, k.ak, k.nrdim,
-                     k.npdim, k.ne, nrc, k.npchop, k.chopzl, k.chopzu, k.ell, dt, a, c->p.visc, pcut, zlo, zhi);
-  if (helm_pre)
-    hipLaunchKernelGGL(k_abcn_ln_pre_, dim3(1), dim3(64), 0, c->stream, e, ln, nlln, nlpln, dt, a, k.lognorm, k.ell);
-  else
-    hipLaunchKernelGGL(k_abcn_ln_, dim3(1), dim3(1), 0, c->stream, ln, ln, nlln, nlpln, dt, a);
+                     k.npdim, k.ne, nrc, k.npchop, k.chopzl, k.chopzu, k.ell, dt, a, c->p.visc, pcut, zlo, zhi,
+                     helm_pre ? nlln : nullptr, helm_pre ? nlpln : nullptr);
+  if (!helm_pre) hipLaunchKernelGGL(k_abcn_ln_, dim3(1), dim3(1), 0, c->stream, ln, ln, nlln, nlpln, dt, a);
   HIPCHK(hipGetLastError());
 }
 
