@@ -70,6 +70,44 @@ def test_vec2tp_fp32(setup):
     assert ep < TOL_MIXED and ec < TOL_MIXED
 
 
+def test_advection_rhs_fp32(setup):
+    """The step's nonlinear term on the FP32-MFMA path (de-aliasing
+    restrictions, fused cross-product transform) against the same operators
+    composed from the public entry points in the same precision, and against
+    the oracle.  N_chi is a small difference of large products, so its FP32
+    error relative to its own norm is of order 1e-2 (as in the time step);
+    N_psi carries the FP32 contraction error (~1e-6)."""
+    import mlegs_amd as M
+    from oracle import scalar_ops as so, vortex as V
+    name, p, tfm, okit = setup
+    _say('advection_rhs fp32', name)
+    # a perturbation of order one: products without cancellation
+    (gpsi, opsi), (gchi, ochi) = C.qvortex(tfm, okit, p, amp=0.5)
+    ub = C.physical(tfm.dims, p, 31)
+    guz = M.Scalar(tfm).upload(ub, space='PPP')
+    ouz = so.Scalar(okit, 'PPP')
+    ouz.e = ub.copy()
+    gnp, gnc = M.Scalar(tfm), M.Scalar(tfm)
+    M.advection_rhs(gpsi, gchi, gnp, gnc, guz, tfm)
+    onp, onc = V.advection_rhs(opsi, ochi, ouz, okit)
+    ep, ec = rel_err(gnp.e, onp.e), rel_err(gnc.e, onc.e)
+    _say('  vs oracle: nlpsi %.2e nlchi %.2e' % (ep, ec))
+    assert ep < 1e-5 and ec < 1e-1
+    vr, vp, vz = M.Scalar(tfm), M.Scalar(tfm), M.Scalar(tfm)
+    wr, wp, wz = M.Scalar(tfm), M.Scalar(tfm), M.Scalar(tfm)
+    M.tp2vec(gpsi, gchi, vr, vp, vz, tfm)
+    M.tp2curlvec(gpsi, gchi, wr, wp, wz, tfm)
+    vz.axpby(1.0, guz, 1.0)
+    M.vecprod(vr, vp, vz, wr, wp, wz, tfm)
+    cp, cc = M.Scalar(tfm), M.Scalar(tfm)
+    M.vec2tp(vr, vp, vz, cp, cc, tfm)
+    M.dealias(cp, tfm)
+    M.dealias(cc, tfm)
+    fp, fc = rel_err(gnp.e, cp.e), rel_err(gnc.e, cc.e)
+    _say('  vs composed (fp32): nlpsi %.2e nlchi %.2e' % (fp, fc))
+    assert fp < 1e-12 and fc < 1e-10
+
+
 def test_vortex_fp32(setup):
     import mlegs_amd as M
     from oracle import scalar_ops as so, vortex as V

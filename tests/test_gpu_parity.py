@@ -223,6 +223,92 @@ def test_tp_operators_unchopped_with_offsets(setup):
         assert rel_err(phys(a.e, nr, nph), phys(b.e, nr, nph)) < 1e-11
 
 
+def test_advection_rhs(setup):
+    """N = P_TP[(U + V) x curl V] with a non-zero background field U_z
+    (the cross product fused into the forward phi transforms where np/2 is a
+    power of two) against the oracle, and against the same operators composed
+    from the public entry points."""
+    import mlegs_amd as M
+    from oracle import scalar_ops as so, vortex as V
+    name, p, tfm, okit = setup
+    _say('advection_rhs', name)
+    # a perturbation of order one: products without cancellation
+    (gpsi, opsi), (gchi, ochi) = C.qvortex(tfm, okit, p, amp=0.5)
+    ub = C.physical(tfm.dims, p, 31)
+    guz = M.Scalar(tfm).upload(ub, space='PPP')
+    ouz = so.Scalar(okit, 'PPP')
+    ouz.e = ub.copy()
+    gnp, gnc = M.Scalar(tfm), M.Scalar(tfm)
+    M.advection_rhs(gpsi, gchi, gnp, gnc, guz, tfm)
+    onp, onc = V.advection_rhs(opsi, ochi, ouz, okit)
+    ep, ec = rel_err(gnp.e, onp.e), rel_err(gnc.e, onc.e)
+    _say('  vs oracle: nlpsi %.2e nlchi %.2e' % (ep, ec))
+    assert ep < 1e-10 and ec < 1e-10
+    assert gnp.ln == 0.0 and gnc.ln == 0.0
+    # the unfused composition of the same operators
+    vr, vp, vz = M.Scalar(tfm), M.Scalar(tfm), M.Scalar(tfm)
+    wr, wp, wz = M.Scalar(tfm), M.Scalar(tfm), M.Scalar(tfm)
+    M.tp2vec(gpsi, gchi, vr, vp, vz, tfm)
+    M.tp2curlvec(gpsi, gchi, wr, wp, wz, tfm)
+    vz.axpby(1.0, guz, 1.0)
+    M.vecprod(vr, vp, vz, wr, wp, wz, tfm)
+    cp, cc = M.Scalar(tfm), M.Scalar(tfm)
+    M.vec2tp(vr, vp, vz, cp, cc, tfm)
+    M.dealias(cp, tfm)
+    M.dealias(cc, tfm)
+    fp, fc = rel_err(gnp.e, cp.e), rel_err(gnc.e, cc.e)
+    _say('  vs composed: nlpsi %.2e nlchi %.2e' % (fp, fc))
+    assert fp < 1e-12 and fc < 1e-12
+
+
+PHI_SIZES = [dict(nr=24, np=32, nz=8, nrchop=24, npchop=17, nzchop=5),      # np/2 = 16
+             dict(nr=20, np=128, nz=8, nrchop=18, npchop=65, nzchop=5),     # np/2 = 64
+             dict(nr=16, np=256, nz=6, nrchop=16, npchop=129, nzchop=4)]    # np/2 = 128
+
+
+@pytest.mark.parametrize('prec', ['fp64', 'fp32'])
+@pytest.mark.parametrize('cfg', PHI_SIZES, ids=['np%d' % c['np'] for c in PHI_SIZES])
+def test_advection_rhs_phi_sizes(cfg, prec):
+    """advection_rhs at the four-step phi kernels' sizes np/2 = 16, 64, 128
+    (its fused cross-product transform included), in FP64 and on the
+    FP32-MFMA path with the de-aliasing restrictions, against the composed
+    operators and, in FP64, against the oracle."""
+    import math
+    import mlegs_amd as M
+    from oracle import kit as K, scalar_ops as so, vortex as V
+    d = dict(cfg, ell=1.3, zlen=2 * math.pi, visc=1e-3, dt=1e-3, hyperpow=0, hypervisc=0.0, is_svv=False)
+    p = C.params(d)
+    _say('advection phi sizes', cfg['np'], prec)
+    tfm = M.Kit(p, device=0, precision=M.PREC_FP64 if prec == 'fp64' else M.PREC_FP32)
+    okit = K.Kit3D(p)
+    (gpsi, opsi), (gchi, ochi) = C.qvortex(tfm, okit, p, amp=0.5)
+    ub = C.physical(tfm.dims, p, 33)
+    guz = M.Scalar(tfm).upload(ub, space='PPP')
+    gnp, gnc = M.Scalar(tfm), M.Scalar(tfm)
+    M.advection_rhs(gpsi, gchi, gnp, gnc, guz, tfm)
+    vr, vp, vz = M.Scalar(tfm), M.Scalar(tfm), M.Scalar(tfm)
+    wr, wp, wz = M.Scalar(tfm), M.Scalar(tfm), M.Scalar(tfm)
+    M.tp2vec(gpsi, gchi, vr, vp, vz, tfm)
+    M.tp2curlvec(gpsi, gchi, wr, wp, wz, tfm)
+    vz.axpby(1.0, guz, 1.0)
+    M.vecprod(vr, vp, vz, wr, wp, wz, tfm)
+    cp, cc = M.Scalar(tfm), M.Scalar(tfm)
+    M.vec2tp(vr, vp, vz, cp, cc, tfm)
+    M.dealias(cp, tfm)
+    M.dealias(cc, tfm)
+    fp, fc = rel_err(gnp.e, cp.e), rel_err(gnc.e, cc.e)
+    _say('  vs composed: nlpsi %.2e nlchi %.2e' % (fp, fc))
+    assert fp < 1e-12 and fc < 1e-10
+    if prec == 'fp64':
+        ouz = so.Scalar(okit, 'PPP')
+        ouz.e = ub.copy()
+        onp, onc = V.advection_rhs(opsi, ochi, ouz, okit)
+        ep, ec = rel_err(gnp.e, onp.e), rel_err(gnc.e, onc.e)
+        _say('  vs oracle: nlpsi %.2e nlchi %.2e' % (ep, ec))
+        assert ep < 1e-10 and ec < 1e-10
+    tfm.close()
+
+
 def test_vortex_steps(setup):
     import mlegs_amd as M
     from oracle import scalar_ops as so, vortex as V
