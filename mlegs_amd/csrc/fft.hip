@@ -321,7 +321,8 @@ __global__ void __launch_bounds__(kThreads) k_fft4(double2 *out, const double2 *
   constexpr int N = N1 * N2;
   constexpr int T1 = (TI * N2 + kThreads - 1) / kThreads;
   constexpr int T2 = (TI * N1 + kThreads - 1) / kThreads;
-  __shared__ double2 buf[TI * N];
+  constexpr bool HALF = (MODE == 0 && N == 256);
+  __shared__ double2 buf[HALF ? TI * N / 2 : TI * N];
   __shared__ double2 nyq[MODE == 2 ? TI : 1];
   const int i0 = blockIdx.x * TI;
   const size_t base = (size_t)blockIdx.y * gs;
@@ -386,27 +387,62 @@ __global__ void __launch_bounds__(kThreads) k_fft4(double2 *out, const double2 *
     }
   }
   if (MODE == 2) __syncthreads();  // stage 0 data fully consumed
-#pragma unroll
-  for (int t = 0; t < T1; ++t) {
-    const int b = tid + t * kThreads;
-    if (b < TI * N2) {
-      const int l = b % TI, n2 = b / TI;
-#pragma unroll
-      for (int k1 = 0; k1 < N1; ++k1) buf[(k1 * N2 + n2) * TI + l] = y1[t][k1];
-    }
-  }
-  __syncthreads();
-
-  // stage 2
   double2 y2[T2][N2];
+  if (HALF) {
+    // the exchange in two halves through TI N doubles: real parts, then
+    // imaginary parts
+    double *bd = reinterpret_cast<double *>(buf);
 #pragma unroll
-  for (int t = 0; t < T2; ++t) {
-    const int b = tid + t * kThreads;
-    if (b < TI * N1) {
-      const int l = b % TI, k1 = b / TI;
+    for (int h = 0; h < 2; ++h) {
+      if (h) __syncthreads();
 #pragma unroll
-      for (int n2 = 0; n2 < N2; ++n2) y2[t][n2] = buf[(k1 * N2 + n2) * TI + l];
-      dft_reg<N2>(y2[t], sg);
+      for (int t = 0; t < T1; ++t) {
+        const int b = tid + t * kThreads;
+        if (b < TI * N2) {
+          const int l = b % TI, n2 = b / TI;
+#pragma unroll
+          for (int k1 = 0; k1 < N1; ++k1) bd[(k1 * N2 + n2) * TI + l] = h ? y1[t][k1].y : y1[t][k1].x;
+        }
+      }
+      __syncthreads();
+#pragma unroll
+      for (int t = 0; t < T2; ++t) {
+        const int b = tid + t * kThreads;
+        if (b < TI * N1) {
+          const int l = b % TI, k1 = b / TI;
+#pragma unroll
+          for (int n2 = 0; n2 < N2; ++n2) {
+            if (h) y2[t][n2].y = bd[(k1 * N2 + n2) * TI + l];
+            else y2[t][n2].x = bd[(k1 * N2 + n2) * TI + l];
+          }
+        }
+      }
+    }
+#pragma unroll
+    for (int t = 0; t < T2; ++t)
+      if (tid + t * kThreads < TI * N1) dft_reg<N2>(y2[t], sg);
+  } else {
+#pragma unroll
+    for (int t = 0; t < T1; ++t) {
+      const int b = tid + t * kThreads;
+      if (b < TI * N2) {
+        const int l = b % TI, n2 = b / TI;
+#pragma unroll
+        for (int k1 = 0; k1 < N1; ++k1) buf[(k1 * N2 + n2) * TI + l] = y1[t][k1];
+      }
+    }
+    __syncthreads();
+
+    // stage 2
+#pragma unroll
+    for (int t = 0; t < T2; ++t) {
+      const int b = tid + t * kThreads;
+      if (b < TI * N1) {
+        const int l = b % TI, k1 = b / TI;
+#pragma unroll
+        for (int n2 = 0; n2 < N2; ++n2) y2[t][n2] = buf[(k1 * N2 + n2) * TI + l];
+        dft_reg<N2>(y2[t], sg);
+      }
     }
   }
   if (MODE == 0 || MODE == 2) {
