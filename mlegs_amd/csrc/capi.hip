@@ -252,6 +252,29 @@ void abcn(mlx_ctx *c, mlx_field *s, mlx_field *s_p, mlx_field *nl, mlx_field *nl
   else fcopy(c, nl_p, nl);
 }
 
+// abcn on psi and chi with the de-aliasing restrictions (the time loop): the
+// two right-hand sides, then both Helmholtz solves in one launch (same alpha,
+// same factors, read once for both fields)
+void abcn_pair(mlx_ctx *c, mlx_field *s1, mlx_field *s1p, mlx_field *nl1, mlx_field *nl1p, mlx_field *s2,
+               mlx_field *s2p, mlx_field *nl2, mlx_field *nl2p, double dt, const DealiasLimits &dl) {
+  if (!dl.on || s1->off[0] != s2->off[0]) {
+    abcn(c, s1, s1p, nl1, nl1p, dt, false, true, &dl);
+    abcn(c, s2, s2p, nl2, nl2p, dt, false, true, &dl);
+    return;
+  }
+  check_step_inputs(c, {s1, nl1, s1p, nl1p, s2, nl2, s2p, nl2p});
+  if (c->p.visc < 5e-14) MLX_THROW(MLX_E_ARG, "abcn: inviscid case and no linear term in rhs. semi-implicit time adv is impossible");
+  const double a = -2.0 / (dt * c->p.visc);
+  if (std::fabs(a) < 5e-14) MLX_THROW(MLX_E_ARG, "ihelm: alpha equals to zero. Inversion of 0*identity is impossible");
+  k_abcn_rhs_d2(c, s1->e, s1->sc, nl1->e, nl1->sc, nl1p->e, nl1p->sc, dt, a, s1->off[0], dl.pcut, dl.zlo, dl.zhi,
+                true);
+  k_abcn_rhs_d2(c, s2->e, s2->sc, nl2->e, nl2->sc, nl2p->e, nl2p->sc, dt, a, s2->off[0], dl.pcut, dl.zlo, dl.zhi,
+                true);
+  k_helm_solve(c, s1->e, a, 0, s1->off[0], dl.pcut, dl.zlo, dl.zhi, s2->e);
+  fmove(nl1p, nl1);
+  fmove(nl2p, nl2);
+}
+
 // Back half of tp2vec: the three FFF components (offset 0, no log term) to
 // PPP; ur and up are divided by r in the last phi transform.  ncols > 0: the
 // components are zero in columns m >= ncols, which the transforms skip.
@@ -432,8 +455,16 @@ void advection_rhs(mlx_ctx *c, mlx_field *psi, mlx_field *chi, mlx_field *nlpsi,
   // psi and chi are de-aliased here and so is the result: the MFMA path
   // skips the columns and slices de-aliasing zeroes
   const DealiasLimits dl = dealias_limits(c);
-  tp2vec(c, psi, chi, vr, vp, vz, dl.on ? dl.pcut : -1, dl.zlo, dl.zhi);
-  tp2curlvec(c, psi, chi, wr, wp, wz, dl.on ? dl.pcut : -1, dl.zlo, dl.zhi);
+  {
+    // tp2vec and tp2curlvec: both front passes in one launch, then the six
+    // backward transforms
+    const int ncols = dl.on ? dl.pcut : -1;
+    const bool zskip = fft_z_fast(c->kit);
+    k_tp2vec_front2(c, vr->e, vp->e, vz->e, vr->sc, vp->sc, vz->sc, wr->e, wp->e, wz->e, wr->sc, wp->sc, wz->sc,
+                    psi->e, chi->e, psi->sc, chi->sc, chi->off[0], ncols, zskip ? dl.zlo : 1, zskip ? dl.zhi : 0);
+    tp2vec_back(c, vr, vp, vz, ncols, dl.zlo, dl.zhi);
+    tp2vec_back(c, wr, wp, wz, ncols, dl.zlo, dl.zhi);
+  }
   // de-aliased inside vec2tp before idel2; chop and zeroat1 keep zero modes
   // zero, so the closing dealias of the reference is already satisfied
   mlx_field *ur = tmp(c, 0), *up = tmp(c, 1), *uzp = tmp(c, 2);
@@ -986,8 +1017,7 @@ int mlx_vortex_step(mlx_ctx *c, mlx_vortex *v, int32_t nsteps, double dt, int32_
     mlx_field *uz = v->f[8];
     const DealiasLimits dl = dealias_limits(c);
     for (int it = 0; it < nsteps; ++it) {
-      abcn(c, psi, psi_prev, nlpsi, nlpsi_prev, dt, false, true, &dl);
-      abcn(c, chi, chi_prev, nlchi, nlchi_prev, dt, false, true, &dl);
+      abcn_pair(c, psi, psi_prev, nlpsi, nlpsi_prev, chi, chi_prev, nlchi, nlchi_prev, dt, dl);
       v->prev_stale = true;
       // With the de-aliasing restrictions the right-hand side and the solve
       // leave the de-aliased modes untouched, and the step's other inputs

@@ -258,8 +258,9 @@ void k_bandmul(mlx_ctx *c, double2 *e, double *ln, int op, int nroff);  // op 0 
 void k_helm_pre(mlx_ctx *c, double2 *e, double *ln_out, const double *ln_in, double alpha);
 // pcut > 0: columns m >= pcut and kappa slices zlo..zhi are left unsolved
 // (for a caller that de-aliases the result)
+// e2 (optional): a second field solved with the same factors in the same launch
 void k_helm_solve(mlx_ctx *c, double2 *e, double alpha, int kind, int nroff, int pcut = -1, int zlo = 1,
-                  int zhi = 0);
+                  int zhi = 0, double2 *e2 = nullptr);
 void lu_cache_free(mlx_ctx *c);  // solve.hip
 void k_idel2_post(mlx_ctx *c, const double2 *e, double *ln);
 void k_idel2_preln00(mlx_ctx *c, double2 *e, double *ln, double preln, int nroff);
@@ -281,6 +282,12 @@ void k_tp2vec_front(mlx_ctx *c, double2 *ur, double2 *up, double2 *uz, const dou
                     const double *p_ln, const double *c_ln, int mode, int md_off, double *ln_r = nullptr,
                     double *ln_p = nullptr, double *ln_z = nullptr, int ncols = -1, int zlo = 1,
                     int zhi = 0);
+// mode 0 into (vr, vp, vz) and mode 1 (chi's offset md_off) into (wr, wp, wz)
+// in one launch
+void k_tp2vec_front2(mlx_ctx *c, double2 *vr, double2 *vp, double2 *vz, double *lvr, double *lvp, double *lvz,
+                     double2 *wr, double2 *wp, double2 *wz, double *lwr, double *lwp, double *lwz,
+                     const double2 *psi, const double2 *chi, const double *psi_ln, const double *chi_ln,
+                     int md_off, int ncols, int zlo, int zhi);
 void k_vecprod(mlx_ctx *c, double2 *vr, double2 *vp, double2 *vz, const double2 *ur, const double2 *up,
                const double2 *uz);
 void k_vec2tp_up_ln(mlx_ctx *c, double2 *up, const double2 *inf, double *psiln);

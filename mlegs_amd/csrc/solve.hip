@@ -464,11 +464,14 @@ __device__ __forceinline__ void solve_rows(double2 (&b)[S], const FacPtrs &F, co
 // slice nz - kk (same factors); lane s of a half owns rows s S .. s S + S - 1.
 // Systems with and without row interchanges are solved by separate
 // instantiations (PIV), each returning at once for the other kind.
+// blockIdx.z selects the right-hand sides: e (0) or e2 (1), solved with the
+// same factors
 template <int S, bool PIV>
-__global__ void __launch_bounds__(64) k_lu_solve(double2 *e, FacPtrs F, const uint64_t *__restrict__ pw,
+__global__ void __launch_bounds__(64) k_lu_solve(double2 *e0, double2 *e2, FacPtrs F, const uint64_t *__restrict__ pw,
                                                  const int *__restrict__ anyp, int nrdim, int npdim, int nz, int nrc,
                                                  int npc, int nzc, int nzcu, int skip00, int zlo, int zhi) {
   const int m = blockIdx.x, kk = blockIdx.y;
+  double2 *e = blockIdx.z ? e2 : e0;
   // slices zlo..zhi left unsolved (both kk and nz - kk in the range)
   if (kk >= zlo && kk <= zhi && (kk == 0 || 2 * kk == nz || (nz - kk >= zlo && nz - kk <= zhi))) return;
   const int lane = threadIdx.x, s = lane % NSEG;
@@ -616,12 +619,14 @@ LuCache &lu_get(mlx_ctx *c, double alpha, int kind, int nroff) {
 
 namespace mlx {
 
-void k_helm_solve(mlx_ctx *c, double2 *e, double alpha, int kind, int nroff, int pcut, int zlo, int zhi) {
+void k_helm_solve(mlx_ctx *c, double2 *e, double alpha, int kind, int nroff, int pcut, int zlo, int zhi,
+                  double2 *e2) {
   const KitDev &k = c->kit;
   LuCache &lu = lu_get(c, alpha, kind, nroff);
   const int nkk = (k.nzdim > 1) ? k.nzdim / 2 + 1 : 1;
   const int npc_grid = (pcut > 0 && pcut < k.npchop) ? pcut : k.npchop;
-  dim3 grid(npc_grid, nkk);
+  const int nf = e2 ? 2 : 1;
+  dim3 grid(npc_grid, nkk, nf);
   // right-hand sides of the solved systems read and written once, their
   // factors (one set per |kappa|) read once
   const int nrc0 = k.nrchop + nroff;
@@ -639,12 +644,12 @@ void k_helm_solve(mlx_ctx *c, double2 *e, double alpha, int kind, int nroff, int
       fbytes += (double)n * (pv ? 56.0 : 40.0) + (pv ? 8.0 * NSEG : 0.0);
     }
   }
-  ProbeScope probe(c, MLX_PROBE_LU_SOLVE, 32.0 * rows + fbytes, 2.0 * 4.0 * 5.0 * rows);
+  ProbeScope probe(c, MLX_PROBE_LU_SOLVE, 32.0 * rows * nf + fbytes, 2.0 * 4.0 * 5.0 * rows * nf);
   const FacPtrs F = fac_ptrs(lu.sol, fac_rows(k, lu.seg));
   const int nrc = k.nrchop + nroff, sk = kind == 2 ? 1 : 0;
 #define MLX_LU_LAUNCH(SV, PV)                                                                              \
-  probed_launch(probe, k_lu_solve<SV, PV>, grid, dim3(64), 0, c->stream, e, F, lu.pvw, lu.anyp, k.nrdim, k.npdim, \
-                     k.nzdim, nrc, k.npchop, k.chopzl, k.chopzu, sk, zlo, zhi)
+  probed_launch(probe, k_lu_solve<SV, PV>, grid, dim3(64), 0, c->stream, e, e2 ? e2 : e, F, lu.pvw, lu.anyp,       \
+                k.nrdim, k.npdim, k.nzdim, nrc, k.npchop, k.chopzl, k.chopzu, sk, zlo, zhi)
 #define MLX_LU_BOTH(SV)                               \
   do {                                                \
     if (lu.npiv < lu.nsys) MLX_LU_LAUNCH(SV, false); \

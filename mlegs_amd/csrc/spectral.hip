@@ -619,13 +619,12 @@ __device__ inline void band_pow(double2 *y, const double2 *x, double2 *tmp, int 
   }
 }
 
-__global__ void __launch_bounds__(256) k_tp2vec_front_(double2 *ur, double2 *up, double2 *uz, const double2 *p_in,
-                                                      const double2 *c_in, const double *p_ln, const double *c_ln,
-                                                      int mode, int md_off, const double *d2c, const double *xxc,
-                                                      const double *lognorm, const double *ak, int nrdim, int npdim,
-                                                      int nz, int ne, int nrc, int npc, int nzc, int nzcu,
-                                                      double ell, double *ln_r, double *ln_p, double *ln_z, int zlo,
-                                                      int zhi) {
+__device__ __forceinline__ void tp2vec_front_body(double2 *ur, double2 *up, double2 *uz, const double2 *p_in,
+                                                  const double2 *c_in, const double *p_ln, const double *c_ln,
+                                                  int mode, int md_off, const double *d2c, const double *xxc,
+                                                  const double *lognorm, const double *ak, int nrdim, int npdim,
+                                                  int nz, int ne, int nrc, int npc, int nzc, int nzcu, double ell,
+                                                  double *ln_r, double *ln_p, double *ln_z, int zlo, int zhi) {
   extern __shared__ double2 lds[];
   // the three outputs carry no log term (their ln slots are distinct from
   // the inputs' p_ln and c_ln)
@@ -807,6 +806,36 @@ __global__ void __launch_bounds__(256) k_tp2vec_front_(double2 *ur, double2 *up,
     if (i < nn0) v = make_double2(-A[i].x, -A[i].y);
     uz[off + i] = v;
   }
+}
+
+__global__ void __launch_bounds__(256) k_tp2vec_front_(double2 *ur, double2 *up, double2 *uz, const double2 *p_in,
+                                                      const double2 *c_in, const double *p_ln, const double *c_ln,
+                                                      int mode, int md_off, const double *d2c, const double *xxc,
+                                                      const double *lognorm, const double *ak, int nrdim, int npdim,
+                                                      int nz, int ne, int nrc, int npc, int nzc, int nzcu,
+                                                      double ell, double *ln_r, double *ln_p, double *ln_z, int zlo,
+                                                      int zhi) {
+  tp2vec_front_body(ur, up, uz, p_in, c_in, p_ln, c_ln, mode, md_off, d2c, xxc, lognorm, ak, nrdim, npdim, nz, ne,
+                    nrc, npc, nzc, nzcu, ell, ln_r, ln_p, ln_z, zlo, zhi);
+}
+
+// tp2vec's and tp2curlvec's front passes over the same column in one
+// workgroup: the second pass finds psi and chi in the cache
+struct FrontOut {
+  double2 *r, *p, *z;
+  double *lr, *lp, *lz;
+};
+__global__ void __launch_bounds__(256) k_tp2vec_front2_(FrontOut v, FrontOut w, const double2 *psi,
+                                                       const double2 *chi, const double *psi_ln,
+                                                       const double *chi_ln, int md_off, const double *d2c,
+                                                       const double *xxc, const double *lognorm, const double *ak,
+                                                       int nrdim, int npdim, int nz, int ne, int nrc, int npc,
+                                                       int nzc, int nzcu, double ell, int zlo, int zhi) {
+  tp2vec_front_body(v.r, v.p, v.z, psi, chi, psi_ln, chi_ln, 0, 0, d2c, xxc, lognorm, ak, nrdim, npdim, nz, ne, nrc,
+                    npc, nzc, nzcu, ell, v.lr, v.lp, v.lz, zlo, zhi);
+  __syncthreads();  // the staging buffers are reused
+  tp2vec_front_body(w.r, w.p, w.z, chi, psi, chi_ln, psi_ln, 1, md_off, d2c, xxc, lognorm, ak, nrdim, npdim, nz, ne,
+                    nrc, npc, nzc, nzcu, ell, w.lr, w.lp, w.lz, zlo, zhi);
 }
 
 // v <- v x u on the two packed phi points (mlegs_scalar_ops.f90:1284-1296)
@@ -1082,6 +1111,24 @@ void k_tp2vec_front(mlx_ctx *c, double2 *ur, double2 *up, double2 *uz, const dou
   hipLaunchKernelGGL(k_tp2vec_front_, grid, dim3(256), lds, c->stream, ur, up, uz, p_in, c_in, p_ln, c_ln, mode,
                      md_off, k.d2c, k.xxc, k.lognorm, k.ak, k.nrdim, k.npdim, k.nzdim, k.ne, k.nrchop, k.npchop, k.chopzl,
                      k.chopzu, k.ell, ln_r, ln_p, ln_z, zlo, zhi);
+  HIPCHK(hipGetLastError());
+}
+
+void k_tp2vec_front2(mlx_ctx *c, double2 *vr, double2 *vp, double2 *vz, double *lvr, double *lvp, double *lvz,
+                     double2 *wr, double2 *wp, double2 *wz, double *lwr, double *lwp, double *lwz,
+                     const double2 *psi, const double2 *chi, const double *psi_ln, const double *chi_ln,
+                     int md_off, int ncols, int zlo, int zhi) {
+  const KitDev &k = c->kit;
+  if (k.nrchop + 3 > k.nrdim_ref || k.nrchop + 3 > k.ne) MLX_THROW(MLX_E_ARG, "chop_index: chopping in r too large");
+  if (k.nrchop > 768) MLX_THROW(MLX_E_ARG, "vector_reconstruction: nrchop above 768 is not supported");
+  if (md_off < 0 || md_off > 3 || k.nrchop + md_off > k.nrdim_ref)
+    MLX_THROW(MLX_E_ARG, "curl_vector_reconstruction: chopping offset of chi out of range");
+  dim3 grid((ncols > 0 && ncols < k.npdim) ? ncols : k.npdim, k.nzdim);
+  size_t lds = 4 * sizeof(double2) * (size_t)k.nrdim;
+  const FrontOut v{vr, vp, vz, lvr, lvp, lvz}, w{wr, wp, wz, lwr, lwp, lwz};
+  hipLaunchKernelGGL(k_tp2vec_front2_, grid, dim3(256), lds, c->stream, v, w, psi, chi, psi_ln, chi_ln, md_off,
+                     k.d2c, k.xxc, k.lognorm, k.ak, k.nrdim, k.npdim, k.nzdim, k.ne, k.nrchop, k.npchop, k.chopzl,
+                     k.chopzu, k.ell, zlo, zhi);
   HIPCHK(hipGetLastError());
 }
 
