@@ -549,10 +549,9 @@ int mlx_ctx_create(const mlx_params *p, int device, void *stream, mlx_ctx **out)
       c->own_stream = true;
     }
     HIPCHK(hipMalloc(&c->dsmall, sizeof(double) * 4096));
-    // zero: the SVV reduction's completion counter starts at 0 (its last
-    // workgroup resets it after each use)
     HIPCHK(hipMemsetAsync(c->dsmall, 0, sizeof(double) * 4096, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
+    HIPCHK(hipMalloc(&c->dpart, sizeof(double) * kPartMax));
     HIPCHK(hipHostMalloc(&c->hsmall, sizeof(double) * 4096));
   });
   if (rc != MLX_OK) {
@@ -578,6 +577,7 @@ int mlx_ctx_destroy(mlx_ctx *c) {
     if (s) (void)hipFree(s);
   kit_free(c->kit);
   if (c->dsmall) (void)hipFree(c->dsmall);
+  if (c->dpart) (void)hipFree(c->dpart);
   if (c->hsmall) (void)hipHostFree(c->hsmall);
   if (c->own_stream && c->stream) (void)hipStreamDestroy(c->stream);
   delete c;

@@ -117,6 +117,7 @@ struct WorkTable {
   int n = 0;
 };
 
+constexpr int kPartMax = 2 * 2 * 4096;  // two fields, (total, tail), up to 4096 workgroups each
 struct mlx_ctx {
   std::map<long, WorkTable> work;
   mlx_params p;
@@ -130,6 +131,7 @@ struct mlx_ctx {
   // scratch buffers of one field each, and small device scratch
   double2 *scr[2] = {nullptr, nullptr};
   double *dsmall = nullptr;     // 4096 doubles of reduction scratch
+  double *dpart = nullptr;      // kPartMax doubles: partial sums of the SVV reduction
   double *hsmall = nullptr;     // pinned host mirror (4096 doubles)
   std::vector<mlx_field *> pool;  // temporaries (allocated on demand)
   std::vector<LuCache> lu;

@@ -92,13 +92,12 @@ __device__ inline double svv_q(int i, int j, int k, double inv_r, double inv_p, 
 }
 
 // SVV on one or two fields (blockIdx.z / blockIdx.y selects the field).
-// Reduction: workgroup (bx, k) sums |c|^2 and the tail (q >= cutoff) over
-// slice k of columns j < pcut; slices zlo..zhi are skipped (the caller has
-// just de-aliased the field: they are zero and add nothing).  flag
-// (optional): set to 1 when an entry is not finite, the stability check of
-// the time loop taken on the same pass.  The last workgroup to finish (a
-// device counter, reset by that workgroup) adds the partial sums in a fixed
-// order and updates each field's gain and strength
+// Reduction: workgroup (bx, k) sums |c|^2 and the tail (q >= cutoff) over a
+// range of columns j < pcut of slice k; slices zlo..zhi are skipped (the
+// caller has just de-aliased the field: they are zero and add nothing).
+// flag (optional): set to 1 when an entry is not finite, the stability check
+// of the time loop taken on the same pass.  k_svv_gain then adds the partial
+// sums in a fixed order and updates each field's gain and strength
 // (mlegs_scalar_ops.f90:72-126): out[3 f] = strength, out[3 f + 1] = total,
 // out[3 f + 2] = tail.
 struct SvvFields {
@@ -107,80 +106,78 @@ struct SvvFields {
 };
 
 __global__ void k_svv_reduce(SvvFields F, int nrdim, int npdim, double inv_r, double inv_p, const double *ak,
-                             double kmax, double cutoff, double *part, unsigned *count, double *out, int *flag,
-                             int pcut, int zlo, int zhi, double target, double relax, double strength0) {
+                             double kmax, double cutoff, double *part, int *flag, int pcut, int zlo, int zhi) {
   __shared__ double sh[16];
-  __shared__ bool last;
   const int f = blockIdx.z, k = blockIdx.y, nbx = gridDim.x;
   double tot = 0.0, tail = 0.0;
   int bad = 0;
   if (k < zlo || k > zhi) {
-    const double qz = fmin(1.0, fabs(ak[k]) / kmax);
+    // q >= cutoff (cutoff < 1) when any of the row, column and slice ratios
+    // reaches the cutoff
+    const bool tz = fmin(1.0, fabs(ak[k]) / kmax) >= cutoff;
     const int ncol = pcut < npdim ? pcut : npdim;
-    const int plane = nrdim * ncol;
-    const double2 *ep = F.e[f] + (size_t)k * nrdim * npdim;
-    // U entries per thread and round, their loads issued together
-    constexpr int U = 8;
-    const int stride = nbx * blockDim.x;
-    for (int t0 = blockIdx.x * blockDim.x + threadIdx.x; t0 < plane; t0 += U * stride) {
+    const int per = (ncol + nbx - 1) / nbx, j0 = blockIdx.x * per;
+    const int j1 = j0 + per < ncol ? j0 + per : ncol;
+    // columns j0..j1-1 are contiguous: item t is row t % nrdim of column
+    // j0 + t / nrdim
+    const double2 *ep = F.e[f] + (size_t)k * nrdim * npdim + (size_t)j0 * nrdim;
+    const int cnt = j1 > j0 ? (j1 - j0) * nrdim : 0;
+    constexpr int U = 4;
+    for (int t0 = threadIdx.x; t0 < cnt; t0 += U * blockDim.x) {
       double2 v[U];
 #pragma unroll
       for (int u = 0; u < U; ++u) {
-        const int t = t0 + u * stride;
-        v[u] = t < plane ? ld_nt(&ep[t]) : make_double2(0.0, 0.0);
+        const int t = t0 + u * blockDim.x;
+        v[u] = t < cnt ? ld_nt(&ep[t]) : make_double2(0.0, 0.0);
       }
 #pragma unroll
       for (int u = 0; u < U; ++u) {
-        const int t = t0 + u * stride;
-        const int j = t / nrdim, i = t - j * nrdim;
+        const int t = t0 + u * blockDim.x;
+        const int jj = t / nrdim, i = t - jj * nrdim;
         if (!isfinite(v[u].x) || !isfinite(v[u].y)) bad = 1;
         const double a2 = v[u].x * v[u].x + v[u].y * v[u].y;
         tot += a2;
-        const double q = fmin(1.0, fmax(fmax(fmin(1.0, (double)i * inv_r), fmin(1.0, (double)j * inv_p)), qz));
-        if (t < plane && q >= cutoff) tail += a2;
+        const bool tq = tz || fmin(1.0, (double)i * inv_r) >= cutoff || fmin(1.0, (double)(j0 + jj) * inv_p) >= cutoff;
+        if (t < cnt && tq) tail += a2;
       }
     }
   }
   if (flag && bad) atomicOr(flag, 1);
   const double a = block_sum(tot, sh);
   const double b = block_sum(tail, sh);
-  const int nb = nbx * gridDim.y;
-  double *pf = part + (size_t)f * 2 * nb;
-  const int slot = k * nbx + blockIdx.x;
   if (threadIdx.x == 0) {
+    const int nb = nbx * gridDim.y;
+    double *pf = part + (size_t)f * 2 * nb;
+    const int slot = k * nbx + blockIdx.x;
     pf[2 * slot] = a;
     pf[2 * slot + 1] = b;
-    __threadfence();
-    const unsigned total = (unsigned)(nb * gridDim.z);
-    last = (atomicAdd(count, 1u) == total - 1);
   }
-  __syncthreads();
-  if (!last) return;
-  __threadfence();
-  for (int g = 0; g < (int)gridDim.z; ++g) {
-    // the other workgroups' partial sums: read past this CU's cache
-    const volatile double *pg = part + (size_t)g * 2 * nb;
-    double x = 0.0, y = 0.0;
-    for (int i = threadIdx.x; i < nb; i += blockDim.x) {
-      x += pg[2 * i];
-      y += pg[2 * i + 1];
-    }
-    const double T = block_sum(x, sh), L = block_sum(y, sh);
-    if (threadIdx.x == 0) {
-      double *o = out + 3 * g;
-      o[1] = T;
-      o[2] = L;
-      if (T <= 2.2250738585072014e-308) {
-        o[0] = 0.0;
-      } else {
-        const double feedback = fmin(fmax((L / T) / target - 1.0, 0.0), 1.0);
-        const double gn = (1.0 - relax) * F.gain[g][0] + relax * feedback;
-        F.gain[g][0] = gn;
-        o[0] = fmin(strength0 * gn, 1.0);
-      }
-    }
+}
+
+// one workgroup: the partial sums of field blockIdx.x, its gain and strength
+__global__ void k_svv_gain(SvvFields F, const double *part, int nb, double *out, double target, double relax,
+                           double strength0) {
+  __shared__ double sh[16];
+  const int g = blockIdx.x;
+  const double *pg = part + (size_t)g * 2 * nb;
+  double x = 0.0, y = 0.0;
+  for (int i = threadIdx.x; i < nb; i += blockDim.x) {
+    x += pg[2 * i];
+    y += pg[2 * i + 1];
   }
-  if (threadIdx.x == 0) count[0] = 0u;
+  const double T = block_sum(x, sh), L = block_sum(y, sh);
+  if (threadIdx.x != 0) return;
+  double *o = out + 3 * g;
+  o[1] = T;
+  o[2] = L;
+  if (T <= 2.2250738585072014e-308) {
+    o[0] = 0.0;
+    return;
+  }
+  const double feedback = fmin(fmax((L / T) / target - 1.0, 0.0), 1.0);
+  const double gn = (1.0 - relax) * F.gain[g][0] + relax * feedback;
+  F.gain[g][0] = gn;
+  o[0] = fmin(strength0 * gn, 1.0);
 }
 
 __global__ void k_svv_apply(SvvFields F, int nrdim, int npdim, int nz, double inv_r, double inv_p,
@@ -973,18 +970,20 @@ void k_svv(mlx_ctx *c, double2 *e0, double *g0, double2 *e1, double *g1, int *fl
   double st0 = fmax(p.svv_strength, 0.0);
   const int nf = e1 ? 2 : 1;
   SvvFields F{{e0, e1 ? e1 : e0}, {g0, g1 ? g1 : g0}};
-  // partial sums: at most 512 workgroups per field (dsmall[0, 2048))
+  // workgroups per slice, each over a range of about eight columns (a few
+  // loads per thread: the pass is latency-bound otherwise); partial sums of
+  // at most 4096 workgroups per field (c->dpart)
   const int ncol = (pcut > 0 && pcut < k.npdim) ? pcut : k.npdim;
-  const int per_slice = (k.nrdim * ncol + kT - 1) / kT;
-  int nbx = 512 / k.nzdim;
-  if (nbx > per_slice) nbx = per_slice;
+  int nbx = (ncol + 7) / 8;
+  if (nbx * k.nzdim > 4096) nbx = 4096 / k.nzdim;
   if (nbx < 1) nbx = 1;
-  if ((size_t)nbx * k.nzdim > 512) MLX_THROW(MLX_E_ARG, "svv_filter: nz above 512 is not supported");
-  double *part = c->dsmall;
+  if ((size_t)nbx * k.nzdim > 4096) MLX_THROW(MLX_E_ARG, "svv_filter: nz above 4096 is not supported");
+  double *part = c->dpart;
   double *out = c->dsmall + 2048;
-  unsigned *count = reinterpret_cast<unsigned *>(c->dsmall + 4000 + 40);
   hipLaunchKernelGGL(k_svv_reduce, dim3(nbx, k.nzdim, nf), dim3(kT), 0, c->stream, F, k.nrdim, k.npdim, inv_r,
-                     inv_p, k.ak, kmax, cutoff, part, count, out, flag, ncol, zlo, zhi, target, relax, st0);
+                     inv_p, k.ak, kmax, cutoff, part, flag, ncol, zlo, zhi);
+  hipLaunchKernelGGL(k_svv_gain, dim3(nf), dim3(1024), 0, c->stream, F, part, nbx * k.nzdim, out, target, relax,
+                     st0);
   hipLaunchKernelGGL(k_svv_apply, dim3(grid_for(c->nelem), nf), dim3(kT), 0, c->stream, F, k.nrdim, k.npdim,
                      k.nzdim, inv_r, inv_p, k.ak, kmax, cutoff, out);
 }
