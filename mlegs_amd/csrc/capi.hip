@@ -280,8 +280,10 @@ void abcn_pair(mlx_ctx *c, mlx_field *s1, mlx_field *s1p, mlx_field *nl1, mlx_fi
 // components are zero in columns m >= ncols, which the transforms skip.
 // The front kernel has zeroed the log terms, so trans's log-column step
 // (adding 0 * ln x) is left out; the other steps are trans's FFF -> PPP.
+// phys_rows: rows nr.. of the outputs are not needed (the caller's cross
+// product takes them as zero) and are not synthesized
 void tp2vec_back(mlx_ctx *c, mlx_field *ur, mlx_field *up, mlx_field *uz, int ncols = -1, int zlo = 1,
-                 int zhi = 0) {
+                 int zhi = 0, bool phys_rows = false) {
   const KitDev &k = c->kit;
   const int rs[3] = {-1, -1, 0};
   const int nc = (ncols > 0 && ncols < k.npchop) ? ncols : k.npchop;
@@ -295,7 +297,7 @@ void tp2vec_back(mlx_ctx *c, mlx_field *ur, mlx_field *up, mlx_field *uz, int nc
     if (k.nz > 1) fft_z(c, f->e, +1, active_pairs(k, 0, nc), k.nrchop, nc, zlo, zhi);
     leg_backward(c, f->e, c->scr[0], 0, true, nc, !lean);
     swap_scr(c, f);
-    if (k.np > 1) fft_phi(c, f->e, f->e, +1, rs[q], lean ? nc : -1);
+    if (k.np > 1) fft_phi(c, f->e, f->e, +1, rs[q], lean ? nc : -1, phys_rows && fft_phi_fast(k));
     else if (rs[q] < 0) k_scale_rows_r(c, f->e, 1);
     set_space(f, "PPP");
     ++q;
@@ -462,8 +464,8 @@ void advection_rhs(mlx_ctx *c, mlx_field *psi, mlx_field *chi, mlx_field *nlpsi,
     const bool zskip = fft_z_fast(c->kit);
     k_tp2vec_front2(c, vr->e, vp->e, vz->e, vr->sc, vp->sc, vz->sc, wr->e, wp->e, wz->e, wr->sc, wp->sc, wz->sc,
                     psi->e, chi->e, psi->sc, chi->sc, chi->off[0], ncols, zskip ? dl.zlo : 1, zskip ? dl.zhi : 0);
-    tp2vec_back(c, vr, vp, vz, ncols, dl.zlo, dl.zhi);
-    tp2vec_back(c, wr, wp, wz, ncols, dl.zlo, dl.zhi);
+    tp2vec_back(c, vr, vp, vz, ncols, dl.zlo, dl.zhi, true);
+    tp2vec_back(c, wr, wp, wz, ncols, dl.zlo, dl.zhi, true);
   }
   // de-aliased inside vec2tp before idel2; chop and zeroat1 keep zero modes
   // zero, so the closing dealias of the reference is already satisfied

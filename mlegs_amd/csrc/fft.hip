@@ -678,7 +678,7 @@ bool fft_phi_fast(const KitDev &k) {
   return k.np > 1 && (N == 16 || N == 32 || N == 64 || N == 128 || N == 256);
 }
 
-void fft_phi(mlx_ctx *c, double2 *out, const double2 *in, int dir, int rscale, int mlim) {
+void fft_phi(mlx_ctx *c, double2 *out, const double2 *in, int dir, int rscale, int mlim, bool phys_rows) {
   const KitDev &k = c->kit;
   if (k.np <= 1) return;
   if (mlim > 0 && dir > 0 && !fft_phi_fast(k)) MLX_THROW(MLX_E_ARG, "fft_phi: mode limit without the fast path");
@@ -692,8 +692,9 @@ void fft_phi(mlx_ctx *c, double2 *out, const double2 *in, int dir, int rscale, i
   const size_t plane = (size_t)k.nrdim * k.npdim;
   bool fast = (dir < 0) ? launch_fft4<1>(c, probe, k.nph, dim3(1, k.nzdim), out, in, k.nrdim, plane, k.nrdim, 0,
                                          k.tw_nph, k.tw_half, -1.0, 1.0 / k.np, rscale)
-                        : launch_fft4<2>(c, probe, k.nph, dim3(1, k.nzdim), out, in, k.nrdim, plane, k.nrdim, 0,
-                                         k.tw_nph, k.tw_half, +1.0, 1.0, rscale, mlim > 0 ? mlim : 1 << 30);
+                        : launch_fft4<2>(c, probe, k.nph, dim3(1, k.nzdim), out, in, k.nrdim, plane,
+                                         phys_rows ? k.nr : k.nrdim, 0, k.tw_nph, k.tw_half, +1.0, 1.0, rscale,
+                                         mlim > 0 ? mlim : 1 << 30);
   if (fast) {
     HIPCHK(hipGetLastError());
     return;

@@ -202,7 +202,10 @@ void kit_free(KitDev &k);
 // input rows i < nr by r_i (forward), -1 divide output rows by r_i (backward)
 // mlim > 0 (backward, fast path only): modes m >= mlim are taken as zero
 // without being read
-void fft_phi(mlx_ctx *c, double2 *out, const double2 *in, int dir, int rscale = 0, int mlim = -1);
+// phys_rows (backward, fast path): only rows i < nr are synthesized and
+// written (rows nr.. of the output are left as they are)
+void fft_phi(mlx_ctx *c, double2 *out, const double2 *in, int dir, int rscale = 0, int mlim = -1,
+             bool phys_rows = false);
 bool fft_phi_fast(const KitDev &k);  // the four-step phi kernels cover np/2
 bool fft_z_fast(const KitDev &k);    // the four-step z kernels cover nz
 // v x w (v_z + ub) to PFP with the r weights of vector_projection (modes
