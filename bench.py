@@ -142,11 +142,18 @@ def transform_error(tfm, p, prec):
 
 
 def roofline(probes, prec):
-    """The probe with the most kernel time; achieved = algorithmic work / kernel time."""
+    """The probe with the most kernel time; achieved = algorithmic work / kernel time.
+
+    The bound is the resource whose peak time is the longer for the launch's
+    algorithmic bytes and flops (arithmetic intensity against the ridge
+    point peak flops / peak bandwidth); the FFTs and the solve count as
+    HBM-bound whatever their flop count."""
     name = max(probes, key=lambda k: probes[k][1])
     n, ms, nbytes, flops = probes[name]
     per_launch_s = ms * 1e-3 / n
-    if name in HBM_BOUND:
+    t_hbm = nbytes / n / (HBM_GBPS * 1e9)
+    t_mfma = flops / n / (PEAK_TFLOPS[prec] * 1e12)
+    if name in HBM_BOUND or t_hbm >= t_mfma:
         achieved = nbytes / n / per_launch_s / 1e9
         peak, unit, bound = HBM_GBPS, 'GB/s', 'hbm'
     else:
