@@ -290,11 +290,23 @@ void tp2vec_back(mlx_ctx *c, mlx_field *ur, mlx_field *up, mlx_field *uz, int nc
   // limited columns and the four-step phi kernels: the radial transform
   // leaves columns m >= nc unwritten and the phi transform takes them as zero
   const bool lean = nc < k.npchop && fft_phi_fast(k);
+  // physical rows only, on the MFMA path: the radial transform hands the
+  // phi transform its FP32 even/odd sums (half the bytes of the FP64 rows,
+  // which the phi transform forms from them exactly)
+  const bool pairs = lean && phys_rows && c->prec != MLX_PREC_FP64 && mfma_ok(k, 0);
   int q = 0;
   for (mlx_field *f : {ur, up, uz}) {
     f->off[0] = f->off[1] = f->off[2] = 0;
     // the de-aliased kappa slices zlo..zhi of the input are zero
     if (k.nz > 1) fft_z(c, f->e, +1, active_pairs(k, 0, nc), k.nrchop, nc, zlo, zhi);
+    if (pairs) {
+      float4 *pr = reinterpret_cast<float4 *>(c->scr[0]);
+      leg_backward_pairs_f32(c, f->e, pr, nc);
+      fft_phi_pairs(c, f->e, pr, rs[q], nc);
+      set_space(f, "PPP");
+      ++q;
+      continue;
+    }
     leg_backward(c, f->e, c->scr[0], 0, true, nc, !lean);
     swap_scr(c, f);
     if (k.np > 1) fft_phi(c, f->e, f->e, +1, rs[q], lean ? nc : -1, phys_rows && fft_phi_fast(k));

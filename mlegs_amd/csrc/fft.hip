@@ -313,18 +313,25 @@ __device__ __forceinline__ double2 twid(const double2 *tw, int idx, double sg) {
 // MODE 2: modes m >= mlim are taken as zero without being read.  MODE 0:
 // line entries zin_lo..zin_hi are taken as zero without being read and
 // outputs zout_lo..zout_hi are not written (their consumer does not read them)
-template <int N1, int N2, int TI, int MODE>
+// PAIR (MODE 2): the input is the backward radial contraction's float4
+// (be, bo) per radial pair x < nrow at ((z npdim + m) pstride + x), npdim =
+// gs / es (each line of pairs starts on a 256-byte boundary); a tile holds
+// the pairs x0 .. x0 + TI/2 - 1 and synthesizes rows x = be + bo and
+// nr - 1 - x = be - bo (formed in FP64 on load), rows < nr only.
+template <int N1, int N2, int TI, int MODE, bool PAIR = false>
 __global__ void __launch_bounds__(kThreads) k_fft4(double2 *out, const double2 *in, size_t es, size_t gs, int nrow,
                                                    int act_nrc, const double2 *tw, const double2 *twh, double sg,
                                                    double scale, const double *r, int nr, int rscale, int mlim,
-                                                   int zin_lo, int zin_hi, int zout_lo, int zout_hi) {
+                                                   int zin_lo, int zin_hi, int zout_lo, int zout_hi,
+                                                   int pstride) {
   constexpr int N = N1 * N2;
   constexpr int T1 = (TI * N2 + kThreads - 1) / kThreads;
   constexpr int T2 = (TI * N1 + kThreads - 1) / kThreads;
   constexpr bool HALF = (MODE == 0 && N == 256);
   __shared__ double2 buf[HALF ? TI * N / 2 : TI * N];
   __shared__ double2 nyq[MODE == 2 ? TI : 1];
-  const int i0 = blockIdx.x * TI;
+  constexpr int TH = PAIR ? TI / 2 : TI;  // input rows (or pairs) per tile
+  const int i0 = blockIdx.x * TH;
   const size_t base = (size_t)blockIdx.y * gs;
   int rlim = nrow;
   if (act_nrc > 0) {
@@ -334,7 +341,26 @@ __global__ void __launch_bounds__(kThreads) k_fft4(double2 *out, const double2 *
   if (i0 >= rlim) return;  // the whole tile lies outside the active rows
   const int tid = threadIdx.x;
 
-  if (MODE == 2) {
+  if (MODE == 2 && PAIR) {
+    // stage 0 from radial pairs: tile rows l (x) and TH + l (nr - 1 - x)
+    const float4 *pin = reinterpret_cast<const float4 *>(in);
+    for (int b = tid; b < TH * (N + 1); b += kThreads) {
+      const int l = b % TH, m = b / TH, x = i0 + l;
+      float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+      // an ordinary load: the pairs were just written and are read from the cache
+      if (x < rlim && m < mlim) v = pin[((size_t)blockIdx.y * (gs / es) + m) * pstride + x];
+      const double ex = v.x, ey = v.y, ox = v.z, oy = v.w;
+      const double2 hi = make_double2(ex + ox, ey + oy), lo = make_double2(ex - ox, ey - oy);
+      if (m < N) {
+        buf[m * TI + l] = hi;
+        buf[m * TI + TH + l] = lo;
+      } else {
+        nyq[l] = hi;
+        nyq[TH + l] = lo;
+      }
+    }
+    __syncthreads();
+  } else if (MODE == 2) {
     // stage 0: C_m, m = 0..N, into LDS (C_N apart)
     for (int b = tid; b < TI * (N + 1); b += kThreads) {
       const int l = b % TI, m = b / TI, i = i0 + l;
@@ -450,8 +476,11 @@ __global__ void __launch_bounds__(kThreads) k_fft4(double2 *out, const double2 *
     for (int t = 0; t < T2; ++t) {
       const int b = tid + t * kThreads;
       if (b < TI * N1) {
-        const int l = b % TI, k1 = b / TI, i = i0 + l;
-        if (i < rlim) {
+        const int l = b % TI, k1 = b / TI;
+        // the output row of tile row l (PAIR: pair x = i0 + l % TH)
+        const int lx = PAIR ? l % TH : l;
+        const int i = (PAIR && l >= TH) ? nr - 1 - (i0 + lx) : i0 + lx;
+        if (i0 + lx < rlim) {
           double dv = 1.0;
           if (MODE == 2 && rscale < 0 && i < nr) dv = r[i];
 #pragma unroll
@@ -630,17 +659,19 @@ __global__ void __launch_bounds__(kThreads) k_fft4_cross(double2 *o0, double2 *o
   }
 }
 
-template <int MODE>
+template <int MODE, bool PAIR = false>
 bool launch_fft4(mlx_ctx *c, ProbeScope &ps, int N, dim3 grid_y, double2 *out, const double2 *in, size_t es,
                  size_t gs, int nrow, int act_nrc, const double2 *tw, const double2 *twh, double sg, double scale,
-                 int rscale, int mlim = 1 << 30, int zin_lo = 1, int zin_hi = 0, int zout_lo = 1, int zout_hi = 0) {
+                 int rscale, int mlim = 1 << 30, int zin_lo = 1, int zin_hi = 0, int zout_lo = 1, int zout_hi = 0,
+                 int pstride = 0) {
   const KitDev &k = c->kit;
 #define MLX_FFT4(N1_, N2_, TI_)                                                                               \
   do {                                                                                                        \
-    dim3 grid((nrow + TI_ - 1) / TI_, grid_y.y);                                                              \
-    probed_launch(ps, k_fft4<N1_, N2_, TI_, MODE>, grid, dim3(kThreads), 0, c->stream, out, in, es, gs, nrow,   \
+    constexpr int TH_ = PAIR ? TI_ / 2 : TI_;                                                                 \
+    dim3 grid((nrow + TH_ - 1) / TH_, grid_y.y);                                                              \
+    probed_launch(ps, k_fft4<N1_, N2_, TI_, MODE, PAIR>, grid, dim3(kThreads), 0, c->stream, out, in, es, gs, nrow,   \
                   act_nrc, tw, twh, sg, scale, k.r, k.nr, rscale, mlim, zin_lo, zin_hi, zout_lo,     \
-                       zout_hi);                                                                             \
+                       zout_hi, pstride);                                                                    \
     return true;                                                                                              \
   } while (0)
   switch (N) {
@@ -701,6 +732,22 @@ void fft_phi(mlx_ctx *c, double2 *out, const double2 *in, int dir, int rscale, i
   }
   probed_launch(probe, k_fft_phi, grid, dim3(kThreads), 0, c->stream, out, in, k.nrdim, k.npdim, k.nzdim, TI,
                      k.plan_nph, k.tw_nph, k.tw_half, dir, k.r, k.nr, rscale);
+  HIPCHK(hipGetLastError());
+}
+
+// The backward phi transform of the radial pairs that leg_backward_pairs_f32
+// writes: rows x and nr - 1 - x (x < nr / 2) of the PPP output, rows >= nr
+// left as they are.  mlim: modes m >= mlim are taken as zero.
+void fft_phi_pairs(mlx_ctx *c, double2 *out, const float4 *pairs, int rscale, int mlim) {
+  const KitDev &k = c->kit;
+  if (!fft_phi_fast(k) || (k.nr & 1)) MLX_THROW(MLX_E_ARG, "fft_phi: radial pairs need the four-step kernels");
+  const double nread = (mlim > 0 && mlim < k.nph + 1) ? mlim : k.nph + 1;
+  ProbeScope probe(c, MLX_PROBE_FFT_PHI, 8.0 * k.nr * k.nz * nread + 16.0 * k.nr * k.nz * k.nph,
+                   2.5 * k.np * std::log2((double)k.np) * k.nr * k.nz);
+  const size_t plane = (size_t)k.nrdim * k.npdim;
+  launch_fft4<2, true>(c, probe, k.nph, dim3(1, k.nzdim), out, reinterpret_cast<const double2 *>(pairs), k.nrdim,
+                       plane, k.nrh, 0, k.tw_nph, k.tw_half, +1.0, 1.0, rscale, mlim > 0 ? mlim : 1 << 30, 1, 0, 1,
+                       0, pair_stride(k));
   HIPCHK(hipGetLastError());
 }
 

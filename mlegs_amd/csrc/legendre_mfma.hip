@@ -17,7 +17,7 @@
 //
 // Work decomposition: one workgroup (4 wavefronts) per (m, 32 axial slices);
 // wave w owns 16-row tiles w, w+4, ... of the radial index.  Contraction in
-// chunks of KC = 32 staged in LDS.
+// chunks of KC = 8 staged in LDS.
 #include "mlx_internal.h"
 
 namespace {
@@ -187,11 +187,16 @@ __global__ void __launch_bounds__(256) k_leg_fwd_f32(const double2 *__restrict__
 // ---------------------------------------------------------------------------
 // backward: f(x) = be + bo, f(-x) = be - bo, b_p(x, k) = sum_a P(x, 2a+p) e(2a+p, k)
 //   table pf32[m][n][x]
+// PAIRS: the FP32 sums themselves are written, (be, bo) of radial node x as
+// one float4 at pairs[(k npdim + m) pstride + x], x < nr / 2: half the bytes
+// of the two FP64 rows, which a consumer forms exactly as be + bo and be - bo
+// in FP64.  Only columns m < npc are written.
 // ---------------------------------------------------------------------------
-template <int XT>  // 16-column tiles of x per wave: XP = 4 * XT * 16 >= nrh
+template <int XT, bool PAIRS>  // 16-column tiles of x per wave: XP = 4 * XT * 16 >= nrh
 __global__ void __launch_bounds__(256) k_leg_bwd_f32(const double2 *__restrict__ in, double2 *__restrict__ out,
-                                                     const float *__restrict__ pf32, int nrdim, int npdim, int nz,
-                                                     int nr, int nrh, int ne, int nrc, int npc, int nyt) {
+                                                     float4 *__restrict__ pairs, const float *__restrict__ pf32,
+                                                     int nrdim, int npdim, int nz, int nr, int nrh, int ne, int nrc,
+                                                     int npc, int nyt, int pstride) {
   constexpr int XP = 4 * XT * 16;
   constexpr int XPP = XP + 16;
   __shared__ float Es[2][KC][BJP];
@@ -204,6 +209,7 @@ __global__ void __launch_bounds__(256) k_leg_bwd_f32(const double2 *__restrict__
   const size_t zs = (size_t)nrdim * npdim;
   double2 *ocol = out + (size_t)nrdim * m;
   if (m >= npc) {
+    if (PAIRS) return;  // columns m >= npc are not written
     for (int idx = tid; idx < nrdim * KT; idx += 256) {
       int i = idx % nrdim, kl = idx / nrdim;
       if (k0 + kl < nz) ocol[(size_t)(k0 + kl) * zs + i] = make_double2(0.0, 0.0);
@@ -322,6 +328,12 @@ __global__ void __launch_bounds__(256) k_leg_bwd_f32(const double2 *__restrict__
       for (int h = 0; h < 2; ++h) {
         const int kg = k0 + 8 * t + 2 * (lane / 16) + h;
         if (kg >= nz) continue;
+        if (PAIRS) {
+          pairs[((size_t)kg * npdim + m) * pstride + x] =
+              make_float4(acc[s][t][0][2 * h], acc[s][t][0][2 * h + 1], acc[s][t][1][2 * h],
+                          acc[s][t][1][2 * h + 1]);
+          continue;
+        }
         double ber = acc[s][t][0][2 * h], bei = acc[s][t][0][2 * h + 1];
         double bor = acc[s][t][1][2 * h], boi = acc[s][t][1][2 * h + 1];
         double2 *o = ocol + (size_t)kg * zs;
@@ -330,6 +342,7 @@ __global__ void __launch_bounds__(256) k_leg_bwd_f32(const double2 *__restrict__
       }
     }
   }
+  if (PAIRS) return;
   for (int idx = tid; idx < (nrdim - nr) * KT; idx += 256) {
     int span = nrdim - nr;
     int i = nr + idx % span, kl = idx / span;
@@ -594,11 +607,30 @@ void leg_backward_f32(mlx_ctx *c, const double2 *in, double2 *out, int ncols, bo
   int xt = (k.nrh + 63) / 64;
   ProbeScope probe(c, MLX_PROBE_LEG_BWD, leg_bytes(k, 0, 4, npc), leg_flops(k, 0, npc));
   if (xt == 1)
-    probed_launch(probe, k_leg_bwd_f32<1>, grid, dim3(256), 0, c->stream, in, out, k.pf32, k.nrdim, k.npdim, k.nzdim,
-                       k.nr, k.nrh, k.ne, k.nrchop, npc, nyt);
+    probed_launch(probe, k_leg_bwd_f32<1, false>, grid, dim3(256), 0, c->stream, in, out, nullptr, k.pf32, k.nrdim,
+                  k.npdim, k.nzdim, k.nr, k.nrh, k.ne, k.nrchop, npc, nyt, 0);
   else
-    probed_launch(probe, k_leg_bwd_f32<2>, grid, dim3(256), 0, c->stream, in, out, k.pf32, k.nrdim, k.npdim, k.nzdim,
-                       k.nr, k.nrh, k.ne, k.nrchop, npc, nyt);
+    probed_launch(probe, k_leg_bwd_f32<2, false>, grid, dim3(256), 0, c->stream, in, out, nullptr, k.pf32, k.nrdim,
+                  k.npdim, k.nzdim, k.nr, k.nrh, k.ne, k.nrchop, npc, nyt, 0);
+  HIPCHK(hipGetLastError());
+}
+
+void leg_backward_pairs_f32(mlx_ctx *c, const double2 *in, float4 *pairs, int ncols) {
+  const KitDev &k = c->kit;
+  if ((k.nr & 1) || pair_stride(k) > k.nrdim) MLX_THROW(MLX_E_ARG, "rtrans_backward: radial pairs need an even nr");
+  const int nyt = (k.nzdim + KT - 1) / KT;
+  const int npc = (ncols > 0 && ncols < k.npchop) ? ncols : k.npchop;
+  dim3 grid = xcd_grid(npc, nyt);
+  int xt = (k.nrh + 63) / 64;
+  // input: the active rows; output: one float4 per radial pair
+  const double bytes = leg_bytes(k, 0, 4, npc) - 8.0 * k.nr * npc * k.nzdim;
+  ProbeScope probe(c, MLX_PROBE_LEG_BWD, bytes, leg_flops(k, 0, npc));
+  if (xt == 1)
+    probed_launch(probe, k_leg_bwd_f32<1, true>, grid, dim3(256), 0, c->stream, in, nullptr, pairs, k.pf32, k.nrdim,
+                  k.npdim, k.nzdim, k.nr, k.nrh, k.ne, k.nrchop, npc, nyt, pair_stride(k));
+  else
+    probed_launch(probe, k_leg_bwd_f32<2, true>, grid, dim3(256), 0, c->stream, in, nullptr, pairs, k.pf32, k.nrdim,
+                  k.npdim, k.nzdim, k.nr, k.nrh, k.ne, k.nrchop, npc, nyt, pair_stride(k));
   HIPCHK(hipGetLastError());
 }
 

@@ -206,6 +206,9 @@ void kit_free(KitDev &k);
 // written (rows nr.. of the output are left as they are)
 void fft_phi(mlx_ctx *c, double2 *out, const double2 *in, int dir, int rscale = 0, int mlim = -1,
              bool phys_rows = false);
+// backward phi transform of leg_backward_pairs_f32's radial pairs into rows
+// < nr of out (fast path only)
+void fft_phi_pairs(mlx_ctx *c, double2 *out, const float4 *pairs, int rscale, int mlim);
 bool fft_phi_fast(const KitDev &k);  // the four-step phi kernels cover np/2
 bool fft_z_fast(const KitDev &k);    // the four-step z kernels cover nz
 // v x w (v_z + ub) to PFP with the r weights of vector_projection (modes
@@ -238,6 +241,11 @@ void mfma_tables(mlx_ctx *c);
 void leg_forward_f32(mlx_ctx *c, const double2 *in, double2 *out, int ncols = -1);
 // ztail = false: columns m >= ncols are left unwritten
 void leg_backward_f32(mlx_ctx *c, const double2 *in, double2 *out, int ncols = -1, bool ztail = true);
+// the FP32 even/odd sums (be, bo) of columns m < ncols as float4 radial pairs
+// (layout at k_leg_bwd_f32), for fft_phi_pairs; a line of pairs (one z slice
+// and column) takes pair_stride float4 (256-byte aligned lines)
+inline int pair_stride(const KitDev &k) { return (k.nrh + 15) / 16 * 16; }
+void leg_backward_pairs_f32(mlx_ctx *c, const double2 *in, float4 *pairs, int ncols);
 void vec2tp_project_f32(mlx_ctx *c, const double2 *ur, const double2 *up, const double2 *uz, double2 *psi,
                         double2 *chi,
                         int ncols = -1, int zlo = 1, int zhi = 0);
