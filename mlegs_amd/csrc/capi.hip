@@ -398,16 +398,20 @@ void vec2tp_pfp(mlx_ctx *c, mlx_field *ur, mlx_field *up, mlx_field *uz, mlx_fie
   // outputs in the kappa slices the projection skips are not written
   const int zlo = dl.on ? dl.zlo : 1, zhi = dl.on ? dl.zhi : 0;
   if (k.nz > 1) fft_z(c, ur->e, -1, (double)k.nr * nc, 0, nc, 1, 0, zlo, zhi);
-  // up to FFF as trans does it (PFP -> FFP -> FFF), over columns m < nc
+  // up to FFF as trans does it (PFP -> FFP -> FFF), over columns m < nc.
+  // With the de-aliasing limits the columns m >= nc of up are read by
+  // nothing below (the z transforms, the radial transforms and the
+  // projection take columns m < nc, calcat and the log step column 0), so
+  // the radial transforms leave them unwritten.
   k_ln_column(c, up->e, up->sc, -1.0);
-  leg_forward(c, up->e, c->scr[0], up->off[0], true, nc);
+  leg_forward(c, up->e, c->scr[0], up->off[0], true, nc, !dl.on);
   swap_scr(c, up);
   if (k.nz > 1)
     fft_z(c, up->e, -1, active_pairs(k, up->off[0], nc), k.nrchop + up->off[0], nc, 1, 0, zlo, zhi);
   set_space(up, "FFF");
   double2 *inf = reinterpret_cast<double2 *>(c->dsmall + 2048);
   k_calcat(c, up->e, k.at1, k.nrchop + up->off[0], inf);
-  leg_backward(c, up->e, c->scr[0], up->off[0], true, nc);
+  leg_backward(c, up->e, c->scr[0], up->off[0], true, nc, !dl.on);
   swap_scr(c, up);
   k_vec2tp_up_ln(c, up->e, inf, psi->sc);
   if (k.nz > 1) fft_z(c, uz->e, -1, (double)k.nr * nc, 0, nc, 1, 0, zlo, zhi);

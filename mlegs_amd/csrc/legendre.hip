@@ -316,11 +316,11 @@ const WorkTable &get_work(mlx_ctx *c, int kind, int nroff) {
 
 namespace mlx {
 
-void leg_forward(mlx_ctx *c, const double2 *in, double2 *out, int nroff, bool zero, int ncols) {
+void leg_forward(mlx_ctx *c, const double2 *in, double2 *out, int nroff, bool zero, int ncols, bool ztail) {
   const KitDev &k = c->kit;
   if (k.nrchop + nroff > k.nrdim_ref) MLX_THROW(MLX_E_ARG, "rtrans_forward: chopping in r too large");
   if (c->prec != MLX_PREC_FP64 && mfma_ok(k, nroff)) {
-    leg_forward_f32(c, in, out, ncols);
+    leg_forward_f32(c, in, out, ncols, ztail);
     return;
   }
   if (zero) HIPCHK(hipMemsetAsync(out, 0, sizeof(double2) * c->nelem, c->stream));

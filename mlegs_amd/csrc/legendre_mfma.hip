@@ -582,11 +582,12 @@ void mfma_tables(mlx_ctx *c) {
 }
 
 // ncols > 0: columns m >= ncols are written as zero without being read
-void leg_forward_f32(mlx_ctx *c, const double2 *in, double2 *out, int ncols) {
+// (ztail = false: they get no workgroups and are left as they are)
+void leg_forward_f32(mlx_ctx *c, const double2 *in, double2 *out, int ncols, bool ztail) {
   const KitDev &k = c->kit;
   const int nyt = (k.nzdim + KT - 1) / KT;
   const int npc = (ncols > 0 && ncols < k.npchop) ? ncols : k.npchop;
-  dim3 grid = xcd_grid(k.npdim, nyt);
+  dim3 grid = xcd_grid(ztail ? k.npdim : npc, nyt);
   int at = mfma_tiles(k);
   ProbeScope probe(c, MLX_PROBE_LEG_FWD, leg_bytes(k, 0, 4, npc), leg_flops(k, 0, npc));
   if (at == 1)

@@ -232,13 +232,15 @@ void fft_z(mlx_ctx *c, double2 *e, int dir, double ncol, int act_nrc = 0, int nc
 // legendre.hip
 // ncols > 0 (MFMA path): only columns m < ncols are transformed, the others
 // are written as zero (their input is zero or their output unused)
-void leg_forward(mlx_ctx *c, const double2 *in, double2 *out, int nroff, bool zero = true, int ncols = -1);
+// ztail = false (MFMA path): columns m >= ncols are left unwritten
+void leg_forward(mlx_ctx *c, const double2 *in, double2 *out, int nroff, bool zero = true, int ncols = -1,
+                 bool ztail = true);
 void leg_backward(mlx_ctx *c, const double2 *in, double2 *out, int nroff, bool zero = true, int ncols = -1,
                   bool ztail = true);
 // legendre_mfma.hip (FP32 MFMA; valid when mfma_ok)
 bool mfma_ok(const KitDev &k, int nroff);
 void mfma_tables(mlx_ctx *c);
-void leg_forward_f32(mlx_ctx *c, const double2 *in, double2 *out, int ncols = -1);
+void leg_forward_f32(mlx_ctx *c, const double2 *in, double2 *out, int ncols = -1, bool ztail = true);
 // ztail = false: columns m >= ncols are left unwritten
 void leg_backward_f32(mlx_ctx *c, const double2 *in, double2 *out, int ncols = -1, bool ztail = true);
 // the FP32 even/odd sums (be, bo) of columns m < ncols as float4 radial pairs

@@ -621,7 +621,8 @@ __device__ __forceinline__ void tp2vec_front_body(double2 *ur, double2 *up, doub
                                                   int mode, int md_off, const double *d2c, const double *xxc,
                                                   const double *lognorm, const double *ak, int nrdim, int npdim,
                                                   int nz, int ne, int nrc, int npc, int nzc, int nzcu, double ell,
-                                                  double *ln_r, double *ln_p, double *ln_z, int zlo, int zhi) {
+                                                  double *ln_r, double *ln_p, double *ln_z, int zlo, int zhi,
+                                                  bool staged = false) {
   extern __shared__ double2 lds[];
   // the three outputs carry no log term (their ln slots are distinct from
   // the inputs' p_ln and c_ln)
@@ -654,6 +655,14 @@ __device__ __forceinline__ void tp2vec_front_body(double2 *ur, double2 *up, doub
   double lnP = 0.0;
   const double lnC = c_ln[0];
   if (cnt == 1) {
+    // staged (mode 1 right after the mode 0 pass of the same column, which
+    // left p_in in LDS slot 1 and c_in in slot 0): the columns are taken
+    // from there and -del2(p_in) goes to slot 2
+    if (staged) {
+      P = lds + 2 * nrdim;
+      C = lds;
+      A = lds + nrdim;
+    }
     // one application of each band product: every output row comes straight
     // from the staged columns.  A thread's rows i = tid, tid + 256, ... (at
     // most three): their band coefficients are loaded before the barrier so
@@ -683,11 +692,13 @@ __device__ __forceinline__ void tp2vec_front_body(double2 *ur, double2 *up, doub
       int nmd = nrcs_dev(nrc + md_off, npc, m);
       if (nmd > nn3) nmd = nn3;
       const double lch = p_ln[0];
-      for (int i = threadIdx.x; i < nn3; i += blockDim.x) {
-        A[i] = p_in[off + i];
-        C[i] = c_in[off + i];
+      if (!staged) {
+        for (int i = threadIdx.x; i < nn3; i += blockDim.x) {
+          A[i] = p_in[off + i];
+          C[i] = c_in[off + i];
+        }
+        __syncthreads();
       }
-      __syncthreads();
       for (int i = threadIdx.x; i < nn3; i += blockDim.x) {
         double2 v = A[i];
         if (i < nmd) {
@@ -831,8 +842,10 @@ __global__ void __launch_bounds__(256) k_tp2vec_front2_(FrontOut v, FrontOut w, 
   tp2vec_front_body(v.r, v.p, v.z, psi, chi, psi_ln, chi_ln, 0, 0, d2c, xxc, lognorm, ak, nrdim, npdim, nz, ne, nrc,
                     npc, nzc, nzcu, ell, v.lr, v.lp, v.lz, zlo, zhi);
   __syncthreads();  // the staging buffers are reused
+  // where the first pass staged psi and chi (one band application), the
+  // second takes them from LDS
   tp2vec_front_body(w.r, w.p, w.z, chi, psi, chi_ln, psi_ln, 1, md_off, d2c, xxc, lognorm, ak, nrdim, npdim, nz, ne,
-                    nrc, npc, nzc, nzcu, ell, w.lr, w.lp, w.lz, zlo, zhi);
+                    nrc, npc, nzc, nzcu, ell, w.lr, w.lp, w.lz, zlo, zhi, true);
 }
 
 // v <- v x u on the two packed phi points (mlegs_scalar_ops.f90:1284-1296)
