@@ -527,6 +527,79 @@ def ihelm(s, alpha, kit):
     s.assign(so)
 
 
+def _check_power(name, power):
+    if power % 2 != 0 or power < 4:
+        raise ValueError('%s: even power greater than or equal to 4' % name)
+    if power > 8:
+        raise ValueError('%s: power must be less than or equal to 8 (supported power = 4, 6 or 8)' % name)
+
+
+def helmp(s, power, alpha, beta, kit):
+    """del2^(power/2) + beta del2 + alpha (mlegs_scalar_ops.f90:856-898).
+
+    As the reference: so%e = sp%e + beta*s2%e + alpha*s%e over every entry,
+    with s2 = del2(s) and sp = del2^(power/2)(s) (del2's copy semantics:
+    entries del2 does not visit keep the value of s), so%ln = alpha*s%ln.
+    """
+    _check_power('helmp', power)
+    if s.space != 'FFF':
+        trans(s, 'FFF', kit)
+    s2 = s.copy()
+    del2(s2, kit)
+    sp = s2.copy()
+    for _ in range(power // 2 - 1):
+        del2(sp, kit)
+    so = s.copy()
+    so.e = sp.e + beta * s2.e + alpha * s.e
+    so.ln = alpha * s.ln
+    s.assign(so)
+
+
+def helmp_band(kit, mval, akval, nn, power, alpha, beta):
+    """ihelmp's system matrix of one (m, kappa) (mlegs_scalar_ops.f90:950-958):
+    leg_del2^(power/2) + beta leg_del2 (products of the nn x nn truncations),
+    banded with kl = ku = power, plus alpha on the diagonal."""
+    D = kit.leg_del2(mval, akval, nn)
+    H = D.copy()
+    for _ in range(power // 2 - 1):
+        H = D @ H
+    W = H + beta * D
+    W[np.abs(np.subtract.outer(np.arange(nn), np.arange(nn))) > power] = 0.0
+    W[np.arange(nn), np.arange(nn)] += alpha
+    return W
+
+
+def ihelmp(s, power, alpha, beta, kit):
+    """Solve (del2^(power/2) + beta del2 + alpha) x = s per (m, k)
+    (mlegs_scalar_ops.f90:900-1000)."""
+    _check_power('ihelmp', power)
+    nrc, npc, nzc, nzcu, nrcs = chop_index(s, kit)
+    if s.space != 'FFF':
+        trans(s, 'FFF', kit)
+    if abs(alpha) < 5e-14:
+        raise ValueError('ihelmp: alpha equals to zero. Inversion of 0*identity is impossible')
+    so = s.copy()
+    so.ln = s.ln / alpha
+    ell2 = kit.ell ** 2
+    bl2 = np.zeros(nrc)
+    bl2[0] = 4.0 / 3.0 / ell2 / np.exp(kit.lognorm[0, 0]) * so.ln
+    bl2[1] = -2.0 / ell2 / np.exp(kit.lognorm[1, 0]) * so.ln
+    bl2[2] = 2.0 / 3.0 / ell2 / np.exp(kit.lognorm[2, 0]) * so.ln
+    bl = bl2.copy()
+    for _ in range(power // 2 - 1):
+        bl = kit.leg_del2(kit.m[0], kit.ak[0], nrc) @ bl
+    so.e[:nrc, 0, 0] = so.e[:nrc, 0, 0] - bl - beta * bl2
+    ks = zlist(kit, nzc, nzcu)
+    for mm in range(npc):
+        nn = nrcs[mm]
+        if nn == 0:
+            continue
+        for k in ks:
+            W = helmp_band(kit, kit.m[mm], kit.ak[k], nn, power, alpha, beta)
+            so.e[:nn, mm, k] = _band_solve(W, so.e[:nn, mm, k], power, power)
+    s.assign(so)
+
+
 def idel2_proln(s, kit):
     """Inverse Laplacian, log coefficient from the (m,k)=(0,0) solve (:671-760)."""
     nrc, npc, nzc, nzcu, nrcs = chop_index(s, kit)

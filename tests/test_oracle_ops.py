@@ -8,6 +8,8 @@ operator identities (no reference code is run).
   del2 / del2h / xxdx / delsqp (:327-573)    analytic derivatives (sympy) at the
                                              collocation points; operation.md:150
   ihelm / idel2 (:575-854)                   operator residuals
+  helmp / ihelmp (:856-1000)                 helmp against the dense band products
+                                             ihelmp assembles; helmp(ihelmp(s)) = s
   chop / dealias (:6-70)                     index rules enumerated by hand
 """
 import cmath
@@ -283,6 +285,51 @@ def test_ihelm_rejects_zero_alpha():
     kit = _solvekit()
     with pytest.raises(ValueError):
         so.ihelm(_field(kit, 1), 0.0, kit)
+
+
+@pytest.mark.parametrize('power', [4, 6, 8])
+def test_helmp_matches_band_products(power):
+    """helmp (repeated del2 on the field) equals the dense product of the
+    nn x nn truncations that ihelmp assembles (leg_del2^(p/2) + beta leg_del2
+    + alpha), column by column, for a chopped field without log term."""
+    kit = _solvekit()
+    s = _field(kit, 21)
+    alpha, beta = 3.7, -0.45
+    x = s.copy()
+    so.helmp(x, power, alpha, beta, kit)
+    nrc, npc, nzc, nzcu, nrcs = so.chop_index(s, kit)
+    for mm in range(npc):
+        nn = nrcs[mm]
+        for k in so.zlist(kit, nzc, nzcu):
+            W = so.helmp_band(kit, kit.m[mm], kit.ak[k], nn, power, alpha, beta)
+            want = W @ s.e[:nn, mm, k]
+            assert np.allclose(x.e[:nn, mm, k], want, rtol=1e-12, atol=1e-12 * np.abs(want).max())
+    assert x.ln == 0.0
+
+
+@pytest.mark.parametrize('power,alpha,beta', [(4, -2.0e3, 0.8), (6, 55.0, -1.5), (8, -1.0e6, 0.0)])
+def test_ihelmp_residual(power, alpha, beta):
+    """helmp(ihelmp(s)) = s, the log term and the (0, 0) correction included."""
+    kit = _solvekit()
+    s = _field(kit, 22, ln=0.3)
+    x = s.copy()
+    so.ihelmp(x, power, alpha, beta, kit)
+    assert abs(alpha * x.ln - s.ln) < 1e-14
+    y = x.copy()
+    so.helmp(y, power, alpha, beta, kit)
+    assert _rel(y.e, s.e) < 1e-10
+    assert abs(y.ln - s.ln) < 1e-14
+
+
+def test_helmp_power_checks():
+    kit = _solvekit()
+    for pw in (2, 5, 10):
+        with pytest.raises(ValueError):
+            so.helmp(_field(kit, 1), pw, 1.0, 0.0, kit)
+        with pytest.raises(ValueError):
+            so.ihelmp(_field(kit, 1), pw, 1.0, 0.0, kit)
+    with pytest.raises(ValueError):
+        so.ihelmp(_field(kit, 1), 4, 0.0, 1.0, kit)
 
 
 def test_idel2_proln_residual():
