@@ -171,6 +171,15 @@ int mlx_idel2(mlx_ctx* ctx, mlx_field* f);
 int mlx_idel2_preln(mlx_ctx* ctx, mlx_field* f, double preln);
 /* ihelm(s, alpha, tfm) (mlegs_scalar.f90:277-285; mlegs_scalar_ops.f90:791-854). */
 int mlx_ihelm(mlx_ctx* ctx, mlx_field* f, double alpha);
+/* helmp(s, power, alpha, beta, tfm) (mlegs_scalar.f90:288-297;
+ * mlegs_scalar_ops.f90:856-898): s <- del2^(power/2) s + beta del2 s + alpha s,
+ * ln <- alpha ln; power 4, 6 or 8. */
+int mlx_helmp(mlx_ctx* ctx, mlx_field* f, int power, double alpha, double beta);
+/* ihelmp(s, power, alpha, beta, tfm) (mlegs_scalar.f90:300-309;
+ * mlegs_scalar_ops.f90:900-1000): the inverse, per (m, kappa) a banded solve
+ * with kl = ku = power and partial pivoting; ln <- ln / alpha.  A column whose
+ * factorization meets a zero pivot is left unsolved (the reference stops). */
+int mlx_ihelmp(mlx_ctx* ctx, mlx_field* f, int power, double alpha, double beta);
 
 /* ---- time steppers (hyperpow = 0) ---------------------------------------- */
 

@@ -18,7 +18,7 @@ from ._lib import MlxError, PREC_FP64, PREC_FP32, PREC_BF16X3  # noqa: F401
 from .params import Params, read_input, timestep_set, InputError  # noqa: F401
 
 __all__ = ['Kit', 'Scalar', 'trans', 'chop', 'dealias', 'svv_filter', 'calcat0', 'calcat1', 'zeroat1',
-           'delsqp', 'idelsqp', 'xxdx', 'del2h', 'del2', 'idel2', 'ihelm', 'febe', 'abcn', 'vecprod',
+           'delsqp', 'idelsqp', 'xxdx', 'del2h', 'del2', 'idel2', 'ihelm', 'helmp', 'ihelmp', 'febe', 'abcn', 'vecprod',
            'vec2tp', 'tp2vec', 'tp2curlvec', 'advection_rhs', 'check_stability', 'VortexRun', 'energy',
            'Params', 'read_input', 'timestep_set', 'MlxError', 'InputError']
 
@@ -278,6 +278,16 @@ def idel2(s, tfm, preln=None):
 
 def ihelm(s, alpha, tfm):
     _c(tfm, tfm.lib.mlx_ihelm(tfm.ctx, s.h, float(alpha)))
+
+
+def helmp(s, power, alpha, beta, tfm):
+    """del2^(power/2) + beta del2 + alpha (mlegs_scalar_ops.f90:856-898)."""
+    _c(tfm, tfm.lib.mlx_helmp(tfm.ctx, s.h, int(power), float(alpha), float(beta)))
+
+
+def ihelmp(s, power, alpha, beta, tfm):
+    """Inverse of helmp by banded solves (mlegs_scalar_ops.f90:900-1000)."""
+    _c(tfm, tfm.lib.mlx_ihelmp(tfm.ctx, s.h, int(power), float(alpha), float(beta)))
 
 
 def febe(s, nl, dt, tfm):

@@ -78,6 +78,8 @@ SIGNATURES = {
     'mlx_idel2': ([_P, _P], ctypes.c_int),
     'mlx_idel2_preln': ([_P, _P, _D], ctypes.c_int),
     'mlx_ihelm': ([_P, _P, _D], ctypes.c_int),
+    'mlx_helmp': ([_P, _P, ctypes.c_int, _D, _D], ctypes.c_int),
+    'mlx_ihelmp': ([_P, _P, ctypes.c_int, _D, _D], ctypes.c_int),
     'mlx_febe': ([_P, _P, _P, _D], ctypes.c_int),
     'mlx_abcn': ([_P, _P, _P, _P, _P, _D], ctypes.c_int),
     'mlx_vecprod': ([_P, _P, _P, _P, _P, _P, _P], ctypes.c_int),

@@ -180,6 +180,39 @@ void ihelm(mlx_ctx *c, mlx_field *f, double alpha) {
   k_helm_solve(c, f->e, alpha, 0, f->off[0]);
 }
 
+// helmp / ihelmp (mlegs_scalar_ops.f90:856-1000): the powered Helmholtz
+// operator del2^(power/2) + beta del2 + alpha and its inverse
+void check_power(const char *name, int power) {
+  if (power % 2 != 0 || power < 4)
+    MLX_THROW(MLX_E_ARG, std::string(name) + ": even power greater than or equal to 4");
+  if (power > 8)
+    MLX_THROW(MLX_E_ARG, std::string(name) + ": power must be less than or equal to 8 (supported power = 4, 6 or 8)");
+}
+
+void helmp(mlx_ctx *c, mlx_field *f, int power, double alpha, double beta) {
+  need_kit(c);
+  check_power("helmp", power);
+  to_fff(c, f);
+  // s2 = del2(s), sp = del2^(power/2)(s); s <- sp + beta s2 + alpha s,
+  // ln <- alpha ln (del2's results carry no log term)
+  mlx_field *s2 = tmp(c, 7), *sp = tmp(c, 8);
+  fcopy(c, s2, f);
+  bandmul(c, s2, 0);
+  fcopy(c, sp, s2);
+  for (int n = 1; n < power / 2; ++n) bandmul(c, sp, 0);
+  k_axpby(c, f->e, f->sc, alpha, s2->e, s2->sc, beta);
+  k_axpby(c, f->e, nullptr, 1.0, sp->e, nullptr, 1.0);
+}
+
+void ihelmp(mlx_ctx *c, mlx_field *f, int power, double alpha, double beta) {
+  need_kit(c);
+  check_power("ihelmp", power);
+  to_fff(c, f);
+  if (std::fabs(alpha) < 5e-14)
+    MLX_THROW(MLX_E_ARG, "ihelmp: alpha equals to zero. Inversion of 0*identity is impossible");
+  k_ihelmp(c, f->e, f->sc, power, alpha, beta, f->off[0]);
+}
+
 void idel2_proln(mlx_ctx *c, mlx_field *f) {
   need_kit(c);
   to_fff(c, f);
@@ -864,6 +897,13 @@ int mlx_idel2_preln(mlx_ctx *c, mlx_field *f, double preln) {
 }
 int mlx_ihelm(mlx_ctx *c, mlx_field *f, double alpha) {
   return guarded(c, [&] { ihelm(c, f, alpha); });
+}
+
+int mlx_helmp(mlx_ctx *c, mlx_field *f, int power, double alpha, double beta) {
+  return guarded(c, [&] { helmp(c, f, power, alpha, beta); });
+}
+int mlx_ihelmp(mlx_ctx *c, mlx_field *f, int power, double alpha, double beta) {
+  return guarded(c, [&] { ihelmp(c, f, power, alpha, beta); });
 }
 
 int mlx_febe(mlx_ctx *c, mlx_field *s, const mlx_field *nl, double dt) {

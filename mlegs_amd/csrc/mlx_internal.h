@@ -257,6 +257,8 @@ void vec2tp_project(mlx_ctx *c, const double2 *ur, const double2 *up, const doub
 
 // spectral.hip
 void k_ln_column(mlx_ctx *c, double2 *e, const double *ln, double sign);
+// ihelmp's banded solves (power 4, 6 or 8) and its log-term step, FP64
+void k_ihelmp(mlx_ctx *c, double2 *e, double *sln, int power, double alpha, double beta, int nroff);
 void k_chop(mlx_ctx *c, double2 *e, const char *space, int nroff);
 void k_dealias(mlx_ctx *c, double2 *e, const char *space);
 // svv_filter on e0 (gain g0) and, when e1 is given, on e1 (gain g1) in the

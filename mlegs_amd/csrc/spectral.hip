@@ -867,6 +867,192 @@ __global__ void k_vecprod_(double2 *vr, double2 *vp, double2 *vz, const double2 
 }
 
 // psi.ln = -Re(f_inf(kappa=0))/2 ; up(:nr, m=0, k=0) += psi.ln (1 + x)  (:1366-1373)
+
+// ---------------------------------------------------------------------------
+// ihelmp (mlegs_scalar_ops.f90:900-1000): per (m, kappa) column the system
+// (leg_del2^(P/2) + beta leg_del2 + alpha) x = b of the nn x nn truncations,
+// banded with kl = ku = P, factored with partial pivoting as LAPACK gbtf2
+// (first maximum as pivot, multipliers scaled by the pivot's reciprocal) and
+// solved as gbtrs, once per visit of the slice.  One workgroup per column;
+// the band storage AB (2P + P + 1 rows, A(i, j) at AB[2P + i - j][j]), the
+// right-hand side and the pivots sit in LDS.  Workgroup (0, 0) first removes
+// the log term's contribution (bl, bl2 of the reference) and sets ln/alpha.
+// ---------------------------------------------------------------------------
+template <int P>
+__global__ void __launch_bounds__(256) k_ihelmp_(double2 *e, double *sln, const double *d2c, const double *ak,
+                                                 const double *lognorm, double alpha, double beta, int nrdim,
+                                                 int npdim, int ne, int nrc, int npc, int nzc, int nzcu,
+                                                 double ell) {
+  constexpr int Q = P / 2, KV = 2 * P, LDAB = 3 * P + 1, W = 2 * P + 1;
+  extern __shared__ double hsm[];
+  const int m = blockIdx.x, k = blockIdx.y, tid = threadIdx.x;
+  const int cnt = z_count(k, nzc, nzcu);
+  const int nn = nrcs_dev(nrc, npc, m);
+  if (cnt == 0 || nn <= 0) return;
+  double *AB = hsm;                                            // LDAB * nn
+  double2 *b = reinterpret_cast<double2 *>(AB + (size_t)LDAB * nn);  // nn
+  double *v = reinterpret_cast<double *>(b + nn);              // nn
+  double *t = v + nn;                                          // nn
+  int *piv = reinterpret_cast<int *>(t + nn);                  // nn
+  __shared__ int s_jp, s_ju;
+  const double akk = ak[k] * ak[k];
+  const double *dm = d2c + (size_t)m * ne * 5;
+  auto dcoef = [&](int r, int d) { return dm[(size_t)r * 5 + d] - (d == 2 ? akk : 0.0); };
+  for (int q = tid; q < LDAB * nn; q += blockDim.x) AB[q] = 0.0;
+  __syncthreads();
+  // row i of W: e_i^T D^Q (support i - P .. i + P) + beta D(i, :) + alpha
+  for (int i = tid; i < nn; i += blockDim.x) {
+    double r[W];
+#pragma unroll
+    for (int o = 0; o < W; ++o) r[o] = (o == P) ? 1.0 : 0.0;
+#pragma unroll
+    for (int st = 0; st < Q; ++st) {
+      double x[W];
+#pragma unroll
+      for (int o = 0; o < W; ++o) x[o] = 0.0;
+#pragma unroll
+      for (int o = 0; o < W; ++o) {
+        const int kr = i + o - P;
+        if (kr < 0 || kr >= nn || r[o] == 0.0) continue;
+#pragma unroll
+        for (int d = 0; d < 5; ++d) {
+          const int oo = o + d - 2, j = kr + d - 2;
+          if (oo < 0 || oo >= W || j < 0 || j >= nn) continue;
+          x[oo] += r[o] * dcoef(kr, d);
+        }
+      }
+#pragma unroll
+      for (int o = 0; o < W; ++o) r[o] = x[o];
+    }
+#pragma unroll
+    for (int d = 0; d < 5; ++d) {
+      const int j = i + d - 2;
+      if (j >= 0 && j < nn) r[P + d - 2] += beta * dcoef(i, d);
+    }
+    r[P] += alpha;
+#pragma unroll
+    for (int o = 0; o < W; ++o) {
+      const int j = i + o - P;
+      if (j >= 0 && j < nn) AB[(size_t)j * LDAB + (KV + i - j)] = r[o];
+    }
+  }
+  const size_t off = (size_t)k * nrdim * npdim + (size_t)nrdim * m;
+  const bool c00 = (m == 0 && k == 0);
+  double lnp = 0.0;
+  if (c00) {
+    // bl2: the log term's Laplacian rows; bl = leg_del2(0, ak_0)^(Q-1) bl2
+    lnp = sln[0] / alpha;
+    const double ell2 = ell * ell;
+    const double b0 = 4.0 / 3.0 / ell2 / exp(lognorm[0]) * lnp, b1 = -2.0 / ell2 / exp(lognorm[1]) * lnp,
+                 b2 = 2.0 / 3.0 / ell2 / exp(lognorm[2]) * lnp;
+    for (int i = tid; i < nn; i += blockDim.x) v[i] = (i == 0) ? b0 : (i == 1) ? b1 : (i == 2) ? b2 : 0.0;
+    __syncthreads();
+    for (int st = 0; st < Q - 1; ++st) {
+      for (int i = tid; i < nn; i += blockDim.x) {
+        double a = 0.0;
+        for (int d = 0; d < 5; ++d) {
+          const int j = i + d - 2;
+          if (j >= 0 && j < nn) a += dcoef(i, d) * v[j];
+        }
+        t[i] = a;
+      }
+      __syncthreads();
+      for (int i = tid; i < nn; i += blockDim.x) v[i] = t[i];
+      __syncthreads();
+    }
+    for (int i = tid; i < nn; i += blockDim.x) {
+      const double bl2 = (i == 0) ? b0 : (i == 1) ? b1 : (i == 2) ? b2 : 0.0;
+      double2 x = e[off + i];
+      x.x = x.x - v[i] - beta * bl2;
+      e[off + i] = x;
+    }
+    __syncthreads();
+  }
+  for (int i = tid; i < nn; i += blockDim.x) b[i] = e[off + i];
+  if (tid == 0) s_ju = 0;
+  __syncthreads();
+  // gbtf2
+  bool ok = true;
+  for (int j = 0; j < nn; ++j) {
+    const int km = (P < nn - 1 - j) ? P : nn - 1 - j;
+    double *cj = AB + (size_t)j * LDAB;
+    if (tid == 0) {
+      int jp = 0;
+      double best = fabs(cj[KV]);
+      for (int q = 1; q <= km; ++q)
+        if (fabs(cj[KV + q]) > best) { best = fabs(cj[KV + q]); jp = q; }
+      piv[j] = j + jp;
+      s_jp = jp;
+      if (cj[KV + jp] != 0.0) {
+        const int cand = (j + P + jp < nn - 1) ? j + P + jp : nn - 1;
+        if (cand > s_ju) s_ju = cand;
+      }
+    }
+    __syncthreads();
+    const int jp = s_jp, ju = s_ju;
+    if (cj[KV + jp] == 0.0) {
+      ok = false;
+      continue;
+    }
+    if (jp != 0)
+      for (int cc = j + tid; cc <= ju; cc += blockDim.x) {
+        double *col = AB + (size_t)cc * LDAB;
+        const double x = col[KV + j - cc];
+        col[KV + j - cc] = col[KV + j + jp - cc];
+        col[KV + j + jp - cc] = x;
+      }
+    __syncthreads();
+    const double rp = 1.0 / cj[KV];
+    for (int q = 1 + tid; q <= km; q += blockDim.x) cj[KV + q] *= rp;
+    __syncthreads();
+    const int ncol = ju - j;
+    for (int w = tid; w < km * ncol; w += blockDim.x) {
+      const int q = 1 + w % km, cc = j + 1 + w / km;
+      double *col = AB + (size_t)cc * LDAB;
+      col[KV + j + q - cc] -= cj[KV + q] * col[KV + j - cc];
+    }
+    __syncthreads();
+  }
+  if (!ok) return;  // singular: left unsolved (the reference stops)
+  // gbtrs, once per visit of the slice
+  for (int rep = 0; rep < cnt; ++rep) {
+    for (int j = 0; j < nn - 1; ++j) {
+      const int km = (P < nn - 1 - j) ? P : nn - 1 - j;
+      if (tid == 0) {
+        const int l = piv[j];
+        if (l != j) {
+          const double2 x = b[l];
+          b[l] = b[j];
+          b[j] = x;
+        }
+      }
+      __syncthreads();
+      const double2 bj = b[j];
+      const double *cj = AB + (size_t)j * LDAB;
+      for (int q = 1 + tid; q <= km; q += blockDim.x) {
+        b[j + q].x -= cj[KV + q] * bj.x;
+        b[j + q].y -= cj[KV + q] * bj.y;
+      }
+      __syncthreads();
+    }
+    for (int j = nn - 1; j >= 0; --j) {
+      const double *cj = AB + (size_t)j * LDAB;
+      if (tid == 0) b[j] = make_double2(b[j].x / cj[KV], b[j].y / cj[KV]);
+      __syncthreads();
+      const double2 xj = b[j];
+      const int i0 = (j - KV > 0) ? j - KV : 0;
+      for (int i = i0 + tid; i < j; i += blockDim.x) {
+        const double a = cj[KV + i - j];
+        b[i].x -= a * xj.x;
+        b[i].y -= a * xj.y;
+      }
+      __syncthreads();
+    }
+  }
+  for (int i = tid; i < nn; i += blockDim.x) e[off + i] = b[i];
+  if (c00 && tid == 0) sln[0] = lnp;
+}
+
 __global__ void k_vec2tp_up_ln_(double2 *up, const double2 *inf, double *psiln, const double *x, int nr) {
   double l = -1.0 / 2.0 * inf[0].x;
   if (blockIdx.x == 0 && threadIdx.x == 0) psiln[0] = l;
@@ -1060,6 +1246,27 @@ void k_idel2_preln00(mlx_ctx *c, double2 *e, double *ln, double preln, int nroff
   if (11 * (size_t)n > 2 * c->nelem) MLX_THROW(MLX_E_ARG, "idel2: field too small for the (0,0) solve");
   hipLaunchKernelGGL(k_idel2_preln00_, dim3(1), dim3(64), 0, c->stream, e, ln, k.d2c, k.lognorm, k.ne, n, preln,
                      k.ell, work);
+}
+
+void k_ihelmp(mlx_ctx *c, double2 *e, double *sln, int power, double alpha, double beta, int nroff) {
+  const KitDev &k = c->kit;
+  const int nrc = k.nrchop + nroff;
+  if (nrc > k.ne) MLX_THROW(MLX_E_ARG, "ihelmp: chopping in r too large");
+  const size_t lds = (size_t)nrc * ((3 * power + 1) * sizeof(double) + sizeof(double2) + 2 * sizeof(double) +
+                                    sizeof(int));
+  if (lds > 65536) MLX_THROW(MLX_E_ARG, "ihelmp: the banded system of one column exceeds the workgroup's LDS");
+  dim3 grid(k.npchop, k.nzdim);
+#define MLX_IHP(P_)                                                                                             \
+  hipLaunchKernelGGL(k_ihelmp_<P_>, grid, dim3(256), lds, c->stream, e, sln, k.d2c, k.ak, k.lognorm, alpha, beta, \
+                     k.nrdim, k.npdim, k.ne, nrc, k.npchop, k.chopzl, k.chopzu, k.ell)
+  switch (power) {
+    case 4: MLX_IHP(4); break;
+    case 6: MLX_IHP(6); break;
+    case 8: MLX_IHP(8); break;
+    default: MLX_THROW(MLX_E_ARG, "ihelmp: power must be 4, 6 or 8");
+  }
+#undef MLX_IHP
+  HIPCHK(hipGetLastError());
 }
 
 void k_axpby(mlx_ctx *c, double2 *dst, double *dln, double a, const double2 *src, const double *sln, double b) {

@@ -124,6 +124,55 @@ def test_spectral_ops(setup, op):
     assert abs(g.ln - o.ln) <= (1e-10 if op.startswith('idel2') else 1e-12) * max(1.0, abs(o.ln))
 
 
+# the powered Helmholtz operators with the signs abcn's hyperviscous branch
+# gives them (mlegs_scalar_ops.f90:1243-1249): definite systems
+HYPER = [(4, 2.0e3, -0.8), (6, -55.0, 1.5), (8, 1.0e6, -0.5)]
+
+
+@pytest.mark.parametrize('power,alpha,beta', HYPER, ids=['p4', 'p6', 'p8'])
+def test_hyperviscous_ops(setup, power, alpha, beta):
+    """helmp and ihelmp (mlegs_scalar_ops.f90:856-1000) against the oracle.
+    helmp is a product of band operators (FP64 rounding only).  ihelmp's
+    systems have condition numbers up to ~1e9 here (lambda_max^(power/2) /
+    alpha), so the solution is compared at 1e-6 and, tightly, through the
+    residual: the oracle's helmp of the GPU solution gives back the input
+    (to 1e-8: its own solution does to 1e-9 on C1 at power 8)."""
+    import mlegs_amd as M
+    from oracle import scalar_ops as so
+    name, p, tfm, okit = setup
+    _say('hyper', name, power)
+    e = C.spectral(tfm.dims, p, 23, admissible=True)
+    g, o = C.pair(tfm, okit, e, 'FFF', ln=0.37)
+    M.helmp(g, power, alpha, beta, tfm)
+    so.helmp(o, power, alpha, beta, okit)
+    assert rel_err(g.e, o.e) < TOL
+    assert abs(g.ln - o.ln) <= 1e-12 * max(1.0, abs(o.ln))
+    g, o = C.pair(tfm, okit, e, 'FFF', ln=0.37)
+    s0 = o.copy()
+    M.ihelmp(g, power, alpha, beta, tfm)
+    so.ihelmp(o, power, alpha, beta, okit)
+    assert rel_err(g.e, o.e) < 1e-6
+    assert abs(g.ln - o.ln) <= 1e-14 * max(1.0, abs(o.ln))
+    back = o.copy()
+    back.e = g.e.copy()
+    back.ln = g.ln
+    so.helmp(back, power, alpha, beta, okit)
+    assert rel_err(back.e, s0.e) < 1e-8  # the oracle's own residual reaches 1e-9 on C1, power 8
+
+
+def test_hyperviscous_errors(setup):
+    import mlegs_amd as M
+    name, p, tfm, okit = setup
+    g = M.Scalar(tfm).upload(C.spectral(tfm.dims, p, 24, admissible=True), space='FFF')
+    for bad in (2, 5, 10):
+        with pytest.raises(M.MlxError):
+            M.helmp(g, bad, 1.0, 0.0, tfm)
+        with pytest.raises(M.MlxError):
+            M.ihelmp(g, bad, 1.0, 0.0, tfm)
+    with pytest.raises(M.MlxError):
+        M.ihelmp(g, 4, 0.0, 1.0, tfm)
+
+
 def test_calcat_svv(setup):
     import mlegs_amd as M
     from oracle import scalar_ops as so
