@@ -223,12 +223,38 @@ void idel2_proln(mlx_ctx *c, mlx_field *f) {
 void check_step_inputs(mlx_ctx *c, std::initializer_list<const mlx_field *> fs) {
   for (const mlx_field *f : fs)
     if (!is_fff(f)) MLX_THROW(MLX_E_LAYOUT, "fefe: all input scalars must be in FFF for time stepping");
-  if (c->p.hyperpow != 0) MLX_THROW(MLX_E_ARG, "hyperviscous time stepping (helmp/ihelmp) is not on this path");
+}
+
+// The hyperviscous branches of febe and abcn (mlegs_scalar_ops.f90:1181-1195,
+// 1226-1230 and 1241-1255): the diffusive operator visc del2 +
+// hypervisc del^p as g (del^p + b del2), g = hypervisc (-1)^(p/2+1),
+// b = visc / g, inverted by ihelmp with alpha = -1/(dt g) (febe) or
+// -2/(dt g) (abcn, whose right-hand side adds dt/2 g helmp(s, p, 0, b))
+void step_hyper(mlx_ctx *c, mlx_field *s, const mlx_field *nl, const mlx_field *nl_p, double dt) {
+  const int pw = c->p.hyperpow;
+  const double g = c->p.hypervisc * (((pw / 2 + 1) % 2) ? -1.0 : 1.0);
+  const double b = c->p.visc / g;
+  if (nl_p) {
+    mlx_field *sv = tmp(c, 14);
+    fcopy(c, sv, s);
+    helmp(c, sv, pw, 0.0, b);
+    const double a = -2.0 / (dt * g);
+    k_abcn_rhs_g(c, s->e, s->sc, s->e, s->sc, nl->e, nl->sc, nl_p->e, nl_p->sc, sv->e, dt, a, 1, g);
+    ihelmp(c, s, pw, a, b);
+  } else {
+    const double a = -1.0 / (dt * g);
+    k_abcn_rhs_g(c, s->e, s->sc, s->e, s->sc, nl->e, nl->sc, nullptr, nullptr, nullptr, dt, a, 0, g);
+    ihelmp(c, s, pw, a, b);
+  }
 }
 
 // febe (mlegs_scalar_ops.f90:1157-1198)
 void febe(mlx_ctx *c, mlx_field *s, const mlx_field *nl, double dt) {
   check_step_inputs(c, {s, nl});
+  if (c->p.hyperpow != 0) {
+    step_hyper(c, s, nl, nullptr, dt);
+    return;
+  }
   if (c->p.visc < 5e-14) MLX_THROW(MLX_E_ARG, "febe: inviscid case and no linear term in rhs. semi-implicit time adv is impossible");
   double a = -1.0 / (dt * c->p.visc);
   k_abcn_rhs(c, s->e, s->sc, s->e, s->sc, nl->e, nl->sc, nullptr, nullptr, nullptr, dt, a, 0);
@@ -267,6 +293,13 @@ DealiasLimits dealias_limits(mlx_ctx *c) {
 void abcn(mlx_ctx *c, mlx_field *s, mlx_field *s_p, mlx_field *nl, mlx_field *nl_p, double dt, bool copy_prev,
           bool swap_nl, const DealiasLimits *dl = nullptr) {
   check_step_inputs(c, {s, nl, s_p, nl_p});
+  if (c->p.hyperpow != 0) {
+    step_hyper(c, s, nl, nl_p, dt);
+    if (copy_prev) fcopy(c, s_p, s);
+    if (swap_nl) fmove(nl_p, nl);
+    else fcopy(c, nl_p, nl);
+    return;
+  }
   if (c->p.visc < 5e-14) MLX_THROW(MLX_E_ARG, "abcn: inviscid case and no linear term in rhs. semi-implicit time adv is impossible");
   double a = -2.0 / (dt * c->p.visc);
   const bool lim = dl && dl->on;
@@ -290,7 +323,7 @@ void abcn(mlx_ctx *c, mlx_field *s, mlx_field *s_p, mlx_field *nl, mlx_field *nl
 // same factors, read once for both fields)
 void abcn_pair(mlx_ctx *c, mlx_field *s1, mlx_field *s1p, mlx_field *nl1, mlx_field *nl1p, mlx_field *s2,
                mlx_field *s2p, mlx_field *nl2, mlx_field *nl2p, double dt, const DealiasLimits &dl) {
-  if (!dl.on || s1->off[0] != s2->off[0]) {
+  if (!dl.on || s1->off[0] != s2->off[0] || c->p.hyperpow != 0) {
     abcn(c, s1, s1p, nl1, nl1p, dt, false, true, &dl);
     abcn(c, s2, s2p, nl2, nl2p, dt, false, true, &dl);
     return;

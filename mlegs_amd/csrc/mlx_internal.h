@@ -257,6 +257,9 @@ void vec2tp_project(mlx_ctx *c, const double2 *ur, const double2 *up, const doub
 
 // spectral.hip
 void k_ln_column(mlx_ctx *c, double2 *e, const double *ln, double sign);
+void k_abcn_rhs_g(mlx_ctx *c, double2 *sh, double *shln, const double2 *s, const double *sln, const double2 *nl,
+                  const double *nlln, const double2 *nlp, const double *nlpln, const double2 *svis, double dt,
+                  double a, int ab2, double g);
 // ihelmp's banded solves (power 4, 6 or 8) and its log-term step, FP64
 void k_ihelmp(mlx_ctx *c, double2 *e, double *sln, int power, double alpha, double beta, int nroff);
 void k_chop(mlx_ctx *c, double2 *e, const char *space, int nroff);

@@ -1285,6 +1285,15 @@ void k_abcn_rhs(mlx_ctx *c, double2 *sh, double *shln, const double2 *s, const d
                      nlpln, svis, dt, a, c->p.visc, ab2, c->nelem);
 }
 
+// the same with the diffusive term's factor given (hyperviscous abcn: the
+// powered operator's factor hypervisc (-1)^(power/2+1) in place of visc)
+void k_abcn_rhs_g(mlx_ctx *c, double2 *sh, double *shln, const double2 *s, const double *sln, const double2 *nl,
+                  const double *nlln, const double2 *nlp, const double *nlpln, const double2 *svis, double dt,
+                  double a, int ab2, double g) {
+  hipLaunchKernelGGL(k_abcn_rhs_, dim3(grid_for(c->nelem)), dim3(kT), 0, c->stream, sh, shln, s, sln, nl, nlln, nlp,
+                     nlpln, svis, dt, a, g, ab2, c->nelem);
+}
+
 void k_abcn_rhs_d2(mlx_ctx *c, double2 *e, double *ln, const double2 *nl, const double *nlln, const double2 *nlp,
                    const double *nlpln, double dt, double a, int nroff, int pcut, int zlo, int zhi, bool helm_pre) {
   const KitDev &k = c->kit;

@@ -665,7 +665,7 @@ def idel2_preln(s, kit, preln):
 
 
 # ----------------------------------------------------------------------------
-# time steppers (hyperpow = 0 branch)
+# time steppers
 # ----------------------------------------------------------------------------
 def _need_fff(*ss):
     for s in ss:
@@ -677,7 +677,16 @@ def febe(s, nl, dt, kit, p):
     """Forward/backward Euler (mlegs_scalar_ops.f90:1157-1198)."""
     _need_fff(s, nl)
     if p.hyperpow != 0:
-        raise NotImplementedError('febe: hyperviscous branch is not on this path')
+        # (:1181-1195) ihelmp with alpha = -1/(dt g), beta = visc/g,
+        # g = hypervisc (-1)^(hyperpow/2+1)
+        g = p.hypervisc * (-1.0) ** (p.hyperpow // 2 + 1)
+        a, b = -1.0 / (dt * g), p.visc / g
+        sh = s.copy()
+        sh.e = a * (s.e + dt * nl.e)
+        sh.ln = a * (s.ln + dt * nl.ln)
+        ihelmp(sh, p.hyperpow, a, b, kit)
+        s.assign(sh)
+        return
     if p.visc < 5e-14:
         raise ValueError('febe: inviscid case and no linear term in rhs. semi-implicit time adv is impossible')
     sh = s.copy()
@@ -693,12 +702,24 @@ def febe(s, nl, dt, kit, p):
 def abcn(s, s_p, nl, nl_p, dt, kit, p):
     """Adams-Bashforth / Crank-Nicolson (mlegs_scalar_ops.f90:1200-1262)."""
     _need_fff(s, nl, s_p, nl_p)
-    if p.hyperpow != 0:
-        raise NotImplementedError('abcn: hyperviscous branch is not on this path')
     sh = s.copy()
     sh.e = s.e + dt * (1.5 * nl.e - 0.5 * nl_p.e)
     sh.ln = s.ln + dt * (1.5 * nl.ln - 0.5 * nl_p.ln)
     svis = s.copy()
+    if p.hyperpow != 0:
+        # (:1226-1230, 1241-1255) svis = g helmp(s, p, 0, b), then ihelmp
+        g = p.hypervisc * (-1.0) ** (p.hyperpow // 2 + 1)
+        b = p.visc / g
+        helmp(svis, p.hyperpow, 0.0, b, kit)
+        svis.e = svis.e * g
+        a = -2.0 / (dt * g)
+        sh.ln = a * (sh.ln + dt / 2.0 * svis.ln)
+        sh.e = a * (sh.e + dt / 2.0 * svis.e)
+        ihelmp(sh, p.hyperpow, a, b, kit)
+        s.assign(sh)
+        s_p.assign(s)
+        nl_p.assign(nl)
+        return
     if p.visc < 5e-14:
         raise ValueError('abcn: inviscid case and no linear term in rhs. semi-implicit time adv is impossible')
     del2(svis, kit)

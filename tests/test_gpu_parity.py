@@ -218,6 +218,41 @@ def test_steppers(setup):
     assert abs(gs.ln - os_.ln) < 1e-12
 
 
+@pytest.mark.parametrize('power', [4, 8])
+def test_steppers_hyperviscous(power):
+    """febe and abcn with HYPERPOW = power (mlegs_scalar_ops.f90:1181-1195,
+    1226-1255): helmp on the right-hand side, ihelmp as the implicit solve.
+    The solves' condition numbers (up to ~1e8 at nr = 64) bound the
+    comparison with the oracle at 1e-7."""
+    import mlegs_amd as M
+    from oracle import kit as K, scalar_ops as so
+    cfg = dict(C.C1, hyperpow=power, hypervisc=1e-7)
+    p = C.params(cfg)
+    _say('febe/abcn hyper', power)
+    tfm = M.Kit(p, device=0)
+    try:
+        okit = K.Kit3D(p)
+        s_e = C.spectral(tfm.dims, p, 51)
+        nl_e = C.spectral(tfm.dims, p, 52)
+        nlp_e = C.spectral(tfm.dims, p, 53)
+        gs, os_ = C.pair(tfm, okit, s_e, 'FFF', ln=0.2)
+        gn, on = C.pair(tfm, okit, nl_e, 'FFF', ln=0.01)
+        M.febe(gs, gn, p.dt, tfm)
+        so.febe(os_, on, p.dt, okit, p)
+        assert rel_err(gs.e, os_.e) < 1e-7
+        assert abs(gs.ln - os_.ln) < 1e-12
+        gp, op_ = C.pair(tfm, okit, s_e, 'FFF')
+        gnp, onp = C.pair(tfm, okit, nlp_e, 'FFF', ln=-0.02)
+        M.abcn(gs, gp, gn, gnp, p.dt, tfm)
+        so.abcn(os_, op_, on, onp, p.dt, okit, p)
+        assert rel_err(gs.e, os_.e) < 1e-7
+        assert rel_err(gp.e, op_.e) < 1e-7
+        assert rel_err(gnp.e, onp.e) == 0.0
+        assert abs(gs.ln - os_.ln) < 1e-12
+    finally:
+        tfm.close()
+
+
 def test_tp_operators(setup):
     import mlegs_amd as M
     from oracle import scalar_ops as so
