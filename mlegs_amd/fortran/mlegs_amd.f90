@@ -8,7 +8,7 @@
 !>   mlegs_scalar  type(scalar) with init/dealloc/chop_offset/assemble/disassemble,
 !>                 assignment(=) and the generic operators trans, chop, dealias,
 !>                 svv_filter, calcat0/1, zeroat1, delsqp, idelsqp, xxdx, del2h,
-!>                 del2, idel2, ihelm, febe, abcn, vecprod, vec2tp, tp2vec,
+!>                 del2, idel2, ihelm, helmp, ihelmp, febe, abcn, vecprod, vec2tp, tp2vec,
 !>                 tp2curlvec (src/modules/mlegs_scalar.f90:15-428)
 !> Every operator calls the C ABI of include/mlegs_amd.h through iso_c_binding;
 !> the data of a scalar stays in GPU memory.  Differences from the reference
@@ -101,7 +101,7 @@ module mlegs_amd
   public :: read_input, timestep_set
   public :: assignment(=)
   public :: trans, chop, dealias, svv_filter, calcat0, calcat1, zeroat1
-  public :: delsqp, idelsqp, xxdx, del2h, del2, idel2, ihelm, febe, abcn
+  public :: delsqp, idelsqp, xxdx, del2h, del2, idel2, ihelm, helmp, ihelmp, febe, abcn
   public :: vecprod, vec2tp, tp2vec, tp2curlvec, advection_rhs, check_stability, energy
   public :: msave, mload, vorticity_magnitude
 
@@ -155,6 +155,12 @@ module mlegs_amd
   end interface
   interface ihelm
     module procedure scalar_ihelm
+  end interface
+  interface helmp
+    module procedure scalar_helmp
+  end interface
+  interface ihelmp
+    module procedure scalar_ihelmp
   end interface
   interface febe
     module procedure scalar_febe
@@ -303,6 +309,18 @@ module mlegs_amd
       import :: c_int, c_ptr, c_double
       type(c_ptr), value :: ctx, f
       real(c_double), value :: alpha
+    end function
+    integer(c_int) function mlx_helmp(ctx, f, power, alpha, beta) bind(C, name='mlx_helmp')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx, f
+      integer(c_int), value :: power
+      real(c_double), value :: alpha, beta
+    end function
+    integer(c_int) function mlx_ihelmp(ctx, f, power, alpha, beta) bind(C, name='mlx_ihelmp')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx, f
+      integer(c_int), value :: power
+      real(c_double), value :: alpha, beta
     end function
     integer(c_int) function mlx_febe(ctx, s, nl, dt) bind(C, name='mlx_febe')
       import :: c_int, c_ptr, c_double
@@ -849,6 +867,26 @@ contains
     class(tfm_kit_3d) :: tfm
     call need_set(tfm, 'ihelm')
     call check(tfm%ctx, mlx_ihelm(tfm%ctx, s%h, alpha), 'ihelm')
+  end subroutine
+
+  ! helmp(s, power, alpha, beta, tfm) (mlegs_scalar.f90:288-297)
+  subroutine scalar_helmp(s, power, alpha, beta, tfm)
+    class(scalar), intent(inout) :: s
+    integer(i4) :: power
+    real(p8) :: alpha, beta
+    class(tfm_kit_3d) :: tfm
+    call need_set(tfm, 'helmp')
+    call check(tfm%ctx, mlx_helmp(tfm%ctx, s%h, int(power, c_int), alpha, beta), 'helmp')
+  end subroutine
+
+  ! ihelmp(s, power, alpha, beta, tfm) (mlegs_scalar.f90:300-309)
+  subroutine scalar_ihelmp(s, power, alpha, beta, tfm)
+    class(scalar), intent(inout) :: s
+    integer(i4) :: power
+    real(p8) :: alpha, beta
+    class(tfm_kit_3d) :: tfm
+    call need_set(tfm, 'ihelmp')
+    call check(tfm%ctx, mlx_ihelmp(tfm%ctx, s%h, int(power, c_int), alpha, beta), 'ihelmp')
   end subroutine
 
   subroutine scalar_febe(s, s_rhs_nonlin, dt, tfm)
