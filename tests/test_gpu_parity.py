@@ -393,6 +393,34 @@ def test_advection_rhs_phi_sizes(cfg, prec):
     tfm.close()
 
 
+def test_vortex_steps_hyperviscous():
+    """The vortex time loop (bootstrap + 3 AB2/CN steps) with HYPERPOW = 8,
+    the shipped input.params' setting (input.params:15), against the
+    oracle's loop.  The implicit solves' conditioning (see
+    test_steppers_hyperviscous) bounds the comparison at 1e-7."""
+    import mlegs_amd as M
+    from oracle import kit as K, scalar_ops as so, vortex as V
+    p = C.params(dict(C.SMALL, hyperpow=8, hypervisc=1e-7))
+    _say('vortex hyper', p.nr, p.np, p.nz)
+    tfm = M.Kit(p, device=0)
+    try:
+        okit = K.Kit3D(p)
+        (gpsi, opsi), (gchi, ochi) = C.qvortex(tfm, okit, p)
+        run = M.VortexRun(tfm, gpsi, gchi, M.Scalar(tfm))
+        orun = V.VortexRun(okit, p, opsi, ochi, so.Scalar(okit, 'PPP'))
+        run.bootstrap()
+        orun.bootstrap()
+        for it in range(3):
+            run.step(1)
+            orun.step()
+        for nm, ob in (('psi', orun.psi), ('chi', orun.chi)):
+            err = rel_err(run.field(nm).e, ob.e)
+            _say('  field', nm, err)
+            assert err < 1e-7, nm
+    finally:
+        tfm.close()
+
+
 def test_vortex_steps(setup):
     import mlegs_amd as M
     from oracle import scalar_ops as so, vortex as V
