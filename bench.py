@@ -8,14 +8,26 @@ initial state = centred q = 4 vortex + seeded 1e-4 perturbation (synthetic,
 SplitMix64/Box-Muller).  The FE-BE + Richardson bootstrap runs untimed, then
 --warmup steps, then --steps timed steps between device synchronisations.
 
+Precisions (--prec) of the Legendre contractions; everything else is FP64:
+  mixed  (default) FP32 MFMA, except vec2tp's projection in FP64 MFMA
+  fp32   FP32 MFMA everywhere
+  fp64m  FP64 MFMA everywhere
+  fp64   the plain FP64 path: FP64 vector-FMA contractions over every column,
+         no de-aliasing restrictions (the reference's operation set)
+
 Printed: one JSON line with
-  value            steps/s of the selected precision (--prec, default fp32:
-                   Legendre contractions on FP32 MFMA, everything else FP64)
-  fp64_steps_per_s the same-GPU FP64 path over the same steps
-  rel_err_vs_fp64  spectral error of the selected precision against FP64:
-                   one step from identical FP64 states (per field), after the
-                   whole run (trajectory), and per transform direction of one
-                   field (the transform-only workload's input)
+  value            steps/s of the selected precision
+  fp64_steps_per_s the same-GPU plain FP64 path over the same steps
+  gate             the accuracy gate of the metric: the selected precision's
+                   psi and chi after a --traj-steps trajectory against the
+                   plain FP64 path's, norm-wise relative, <= 1e-6
+  rel_err_vs_fp64  that trajectory error, the error of one step from
+                   identical FP64 states (per field), and per transform
+                   direction of one field (the transform-only input)
+  ke_history       FP64 kinetic energy of the perturbation, 1/2 int |u - U|^2
+                   (U: the same run without perturbation, plain FP64), along
+                   the trajectory for the selected precision and plain FP64
+  other_precisions steps/s and trajectory error of the other precisions
   roofline         the kernel family with the most kernel time, from a second
                    timed region: --steps more steps of the same run in which
                    every launch of the probed kernel families is timed by HIP
@@ -79,81 +91,143 @@ def rel(a, b):
     return float(np.linalg.norm((a - b).ravel()) / np.linalg.norm(b.ravel()))
 
 
-def timed_run(tfm, prec, psi0, chi0, warmup, steps, probe=False):
-    """Bootstrap + warmup untimed, then `steps` timed steps; returns (run, seconds, probes)."""
+PRESETS = ('mixed', 'fp32', 'fp64m', 'fp64')
+DTYPES = {'fp64': 'f64', 'fp64m': 'f64-mfma', 'fp32': 'f32-mfma+f64', 'mixed': 'f32-mfma+f64-mfma'}
+GATE_TOL = 1e-6
+
+
+def configure(tfm, name):
+    """Contraction precisions of a preset (context precision + site overrides)."""
     import mlegs_amd as M
-    tfm.set_precision(prec)
+    base = {'fp64': M.PREC_FP64, 'fp32': M.PREC_FP32, 'fp64m': M.PREC_FP64_MFMA, 'mixed': M.PREC_FP32}[name]
+    tfm.set_precision(base)
+    for st in (M.STAGE_SYNTH, M.STAGE_UP, M.STAGE_PROJ):
+        tfm.set_stage_precision(st, -1)
+    if name == 'mixed':
+        tfm.set_stage_precision(M.STAGE_PROJ, M.PREC_FP64_MFMA)
+
+
+def timed_run(tfm, name, psi0, chi0, warmup, steps):
+    """Bootstrap + warmup untimed, then `steps` timed steps; returns (run, seconds)."""
+    import mlegs_amd as M
+    configure(tfm, name)
     run = M.VortexRun(tfm, psi0, chi0, M.Scalar(tfm))
     run.bootstrap()
     if warmup:
         run.step(warmup, check=True)
     tfm.sync()
-    if probe:
-        tfm.probe_start(64 * max(steps, 1))
     t0 = time.perf_counter()
     run.step(steps, check=False)
     tfm.sync()
     t1 = time.perf_counter()
-    probes = tfm.probe_stop() if probe else None
     run.step(0, check=True)  # raises on a non-finite state
-    return run, t1 - t0, probes
+    return run, t1 - t0
 
 
-def step_errors(tfm, prec, psi0, chi0, presteps=2):
-    """From the same FP64 state, steps in `prec` against steps in FP64.
+def perturbation_energy(tfm, run, base):
+    """1/2 int |u - U|^2 in FP64: tp2vec of (psi - psi_base, chi - chi_base)."""
+    import mlegs_amd as M
+    dp = run.field('psi').copy()
+    dc = run.field('chi').copy()
+    dp.axpby(1.0, base[0], -1.0)
+    dc.axpby(1.0, base[1], -1.0)
+    vr, vp, vz = M.Scalar(tfm), M.Scalar(tfm), M.Scalar(tfm)
+    M.tp2vec(dp, dc, vr, vp, vz, tfm)
+    return M.energy(vr, vp, vz, tfm)
+
+
+def trajectory(tfm, name, psi0, chi0, nsteps, every, base_states=None):
+    """Bootstrap + nsteps steps in `name`; returns (psi, chi) on the host and
+    the perturbation energy at steps 0, every, ... (when base_states holds
+    the unperturbed run's (psi, chi) at those steps)."""
+    import mlegs_amd as M
+    configure(tfm, name)
+    run = M.VortexRun(tfm, psi0, chi0, M.Scalar(tfm))
+    ke = []
+    states = []
+    done = 0
+    run.bootstrap()
+    marks = list(range(0, nsteps + 1, every))
+    if marks[-1] != nsteps:
+        marks.append(nsteps)
+    for mk in marks:
+        if mk > done:
+            configure(tfm, name)
+            run.step(mk - done, check=True)
+            done = mk
+        if base_states is not None:
+            configure(tfm, 'fp64')
+            ke.append(perturbation_energy(tfm, run, base_states[len(ke)]))
+        else:
+            states.append((run.field('psi').copy(), run.field('chi').copy()))
+    out = (run.field('psi').e, run.field('chi').e)
+    del run
+    return out, ke, states, marks
+
+
+def step_errors(tfm, name, psi0, chi0, presteps=2):
+    """From the same FP64 state, one step in `name` against one in FP64.
 
     Inside one step the transforms only produce the nonlinear terms N for the
-    next step, so the error of one step is the error of N (nlpsi, nlchi); two
-    steps carry it into psi and chi."""
+    next step, so the error of one step is the error of N (nlpsi, nlchi)."""
     import mlegs_amd as M
     runs = []
     for _ in range(2):
-        tfm.set_precision(M.PREC_FP64)
+        configure(tfm, 'fp64')
         r = M.VortexRun(tfm, psi0, chi0, M.Scalar(tfm))
         r.bootstrap()
         r.step(presteps, check=True)
         runs.append(r)
-    out = {}
-    for n, names in ((1, ('nlpsi', 'nlchi')), (2, ('psi', 'chi'))):
-        tfm.set_precision(M.PREC_FP64)
-        runs[0].step(1)
-        tfm.set_precision(prec)
-        runs[1].step(1)
-        out['%d_step' % n] = {nm: rel(runs[1].field(nm).e, runs[0].field(nm).e) for nm in names}
+    configure(tfm, 'fp64')
+    runs[0].step(1)
+    configure(tfm, name)
+    runs[1].step(1)
+    out = {nm: rel(runs[1].field(nm).e, runs[0].field(nm).e) for nm in ('nlpsi', 'nlchi')}
     del runs
     return out
 
 
-def transform_error(tfm, p, prec):
-    """Transform-only input (|c| ~ N(0,1) exp(-n/32)): backward and round-trip error vs FP64."""
+def transform_error(tfm, p, name):
+    """Transform-only input (|c| ~ N(0,1) exp(-n/32)): backward and round-trip
+    error of trans at the preset's context precision vs plain FP64."""
     import mlegs_amd as M
     from mlegs_amd import inputs
     e = inputs.transform_field(tfm.dims, (p.nrchop, p.npchop, p.nzchop, p.np, p.nz), field_id=3)
     res = {}
-    for pr in (M.PREC_FP64, prec):
-        tfm.set_precision(pr)
+    for nm in ('fp64', name):
+        configure(tfm, nm)
         g = M.Scalar(tfm).upload(e, space='FFF')
         M.trans(g, 'PPP', tfm)
         b = g.e[:p.nr, :p.np // 2].copy()
         M.trans(g, 'FFF', tfm)
-        res[pr] = (b, g.e.copy())
-    (b64, f64), (bx, fx) = res[M.PREC_FP64], res[prec]
+        res[nm] = (b, g.e.copy())
+    (b64, f64), (bx, fx) = res['fp64'], res[name]
     return {'backward': rel(bx, b64), 'round_trip': rel(fx, f64)}
 
 
-def roofline(probes, prec):
+# arithmetic type of each probed family's contractions under a preset
+def family_precision(name, family):
+    if name in ('fp64', 'fp64m'):
+        return 'fp64'
+    if name == 'mixed' and family == 'v2t':
+        return 'fp64'
+    return 'fp32' if family in ('leg_fwd', 'leg_bwd', 'v2t') else 'fp64'
+
+
+def roofline(probes, name):
     """The probe with the most kernel time; achieved = algorithmic work / kernel time.
 
     The bound is the resource whose peak time is the longer for the launch's
     algorithmic bytes and flops (arithmetic intensity against the ridge
     point peak flops / peak bandwidth); the FFTs and the solve count as
     HBM-bound whatever their flop count."""
-    name = max(probes, key=lambda k: probes[k][1])
-    n, ms, nbytes, flops = probes[name]
+    fam = max(probes, key=lambda k: probes[k][1])
+    n, ms, nbytes, flops = probes[fam]
+    prec = family_precision(name, fam)
     per_launch_s = ms * 1e-3 / n
     t_hbm = nbytes / n / (HBM_GBPS * 1e9)
     t_mfma = flops / n / (PEAK_TFLOPS[prec] * 1e12)
-    if name in HBM_BOUND or t_hbm >= t_mfma:
+    if fam in HBM_BOUND or t_hbm >= t_mfma:
         achieved = nbytes / n / per_launch_s / 1e9
         peak, unit, bound = HBM_GBPS, 'GB/s', 'hbm'
     else:
@@ -163,10 +237,10 @@ def roofline(probes, prec):
     if os.path.exists(TRAFFIC_FILE):
         with open(TRAFFIC_FILE) as f:
             t = json.load(f)
-        entry = t.get('kernels', {}).get(name)
+        entry = t.get('kernels', {}).get(fam)
         if entry and entry.get('precision') in (prec, 'any'):
             traffic = entry.get('hbm_bytes_per_launch')
-    return {'bound': bound, 'kernel': name, 'achieved': achieved, 'peak': peak, 'unit': unit,
+    return {'bound': bound, 'kernel': fam, 'precision': prec, 'achieved': achieved, 'peak': peak, 'unit': unit,
             'frac': achieved / peak, 'traffic': traffic,
             'launches': n, 'avg_launch_us': 1e3 * ms / n,
             'algorithmic_bytes_per_launch': nbytes / n, 'algorithmic_flops_per_launch': flops / n}
@@ -211,11 +285,11 @@ def cpu_baseline(p):
                       'NR=NP=NZ=%d, one step timed: %.2f s' % (threads, p.nr, t)}
 
 
-def kernel_sample(tfm, p, prec, iters):
-    """Each probed kernel family on one field of the workload's size (plus the copy)."""
+def kernel_sample(tfm, p, name, iters):
+    """Each probed kernel family on one field of the workload's size (plus the
+    copy), at the precision configure() has set."""
     import mlegs_amd as M
     from mlegs_amd import inputs
-    tfm.set_precision(prec)
     e = inputs.transform_field(tfm.dims, (p.nrchop, p.npchop, p.nzchop, p.np, p.nz), field_id=3)
     out = {}
     for w in ('copy', 'fft_phi_fwd', 'fft_phi_bwd', 'fft_z_fwd', 'fft_z_bwd', 'leg_fwd', 'leg_bwd', 'v2t',
@@ -223,7 +297,7 @@ def kernel_sample(tfm, p, prec, iters):
         g = M.Scalar(tfm).upload(e, space='FFF')
         out[w] = round(1e3 * tfm.time_kernel(g, w, iters), 2)
         log(w, out[w], 'us')
-    print(json.dumps({'kernel_sample_us': out, 'iters': iters, 'precision': prec}), flush=True)
+    print(json.dumps({'kernel_sample_us': out, 'iters': iters, 'precision': name}), flush=True)
 
 
 def main():
@@ -231,12 +305,15 @@ def main():
     ap.add_argument('--gpus', type=int, default=1)
     ap.add_argument('--steps', type=int, default=20)
     ap.add_argument('--warmup', type=int, default=3)
-    ap.add_argument('--prec', default='fp32', choices=['fp64', 'fp32'])
+    ap.add_argument('--prec', default='mixed', choices=list(PRESETS))
     ap.add_argument('--size', type=int, default=256)
     ap.add_argument('--cpu-baseline', type=int, default=1)
-    ap.add_argument('--errors', type=int, default=1, help='measure the errors against FP64')
+    ap.add_argument('--errors', type=int, default=1, help='measure the errors against FP64 (the accuracy gate)')
+    ap.add_argument('--traj-steps', type=int, default=500, help='length of the gate trajectory')
+    ap.add_argument('--ke-every', type=int, default=50, help='perturbation-energy sampling interval')
+    ap.add_argument('--others', type=int, default=1, help='also time the other precisions')
     ap.add_argument('--compare-fp64', type=int, default=1,
-                    help='also time the FP64 path (0: the FP32 steps only, e.g. under a profiler)')
+                    help='also time the plain FP64 path (0: the selected precision only, e.g. under a profiler)')
     ap.add_argument('--kernel-sample', type=int, default=0,
                     help='only launch each probed kernel family (and a plain field copy) this many times, '
                          'for hardware-counter runs; prints their mean durations')
@@ -249,38 +326,77 @@ def main():
     import mlegs_amd as M
 
     p = workload(args.size)
-    precs = {'fp64': M.PREC_FP64, 'fp32': M.PREC_FP32}
-    prec = precs[args.prec]
+    name = args.prec
     log('kit', p.nr, p.np, p.nz)
     tfm = M.Kit(p, device=0, precision=M.PREC_FP64)
     if args.kernel_sample:
-        kernel_sample(tfm, p, prec, args.kernel_sample)
+        configure(tfm, name)
+        kernel_sample(tfm, p, name, args.kernel_sample)
         return
     psi0, chi0 = initial_state(tfm, p)
 
-    log(args.prec, 'run: warmup', args.warmup, 'steps', args.steps)
-    run, t, _ = timed_run(tfm, prec, psi0, chi0, args.warmup, args.steps)
+    log(name, 'run: warmup', args.warmup, 'steps', args.steps)
+    run, t = timed_run(tfm, name, psi0, chi0, args.warmup, args.steps)
     rate, ms = args.steps / t, 1e3 * t / args.steps
     log('%.3f ms/step' % ms)
-
-    err = None
-    if args.prec == 'fp64':
+    fp64_rate = None
+    if name == 'fp64':
         fp64_rate = rate
-    elif not args.compare_fp64:
-        fp64_rate = None
-    else:
-        log('fp64 run')
-        run64, t64, _ = timed_run(tfm, M.PREC_FP64, psi0, chi0, args.warmup, args.steps)
+    elif args.compare_fp64:
+        log('plain fp64 run')
+        r64, t64 = timed_run(tfm, 'fp64', psi0, chi0, args.warmup, args.steps)
+        del r64
         fp64_rate = args.steps / t64
-        if args.errors:
-            err = {'trajectory_%d_steps' % (args.warmup + args.steps):
-                   {nm: rel(run.field(nm).e, run64.field(nm).e) for nm in ('psi', 'chi')}}
-            del run64
-            err.update(step_errors(tfm, prec, psi0, chi0))
-            err['transform'] = transform_error(tfm, p, prec)
-            log('errors', err)
+
+    err, gate, ke, others = None, None, None, None
+    if args.errors:
+        n = args.traj_steps
+        log('gate trajectories: %d steps' % n)
+        base0 = initial_state(tfm, p, amp=0.0)
+        _, _, base_states, marks = trajectory(tfm, 'fp64', base0[0], base0[1], n, args.ke_every)
+        (p64, c64), ke64, _, _ = trajectory(tfm, 'fp64', psi0, chi0, n, args.ke_every, base_states)
+        trajs = {}
+        for nm in ([name] + ([x for x in PRESETS if x not in (name, 'fp64')] if args.others else [])):
+            if nm == 'fp64':
+                continue
+            (ps, cs), kx, _, _ = trajectory(tfm, nm, psi0, chi0, n, args.ke_every, base_states)
+            trajs[nm] = ({'psi': rel(ps, p64), 'chi': rel(cs, c64)}, kx)
+            log(nm, 'trajectory error', trajs[nm][0])
+        del base_states
+        if name == 'fp64':
+            tr = {'psi': 0.0, 'chi': 0.0}
+            kx = ke64
+        else:
+            tr, kx = trajs[name]
+        err = {'trajectory_%d_steps' % n: tr}
+        gate = {'tol': GATE_TOL, 'steps': n, 'fields': ['psi', 'chi'],
+                'pass': bool(max(tr.values()) <= GATE_TOL)}
+        if name != 'fp64':
+            err['1_step'] = step_errors(tfm, name, psi0, chi0)
+        err['transform'] = transform_error(tfm, p, name)
+        ke = {'steps': marks, 'fp64': ke64, name: kx,
+              'max_rel_diff': max(abs(a - b) / abs(b) for a, b in zip(kx, ke64) if b != 0.0)}
+        if args.others:
+            others = {}
+            for nm in PRESETS:
+                if nm in (name,):
+                    continue
+                if nm == 'fp64':
+                    others[nm] = {'steps_per_s': fp64_rate, 'trajectory_err': {'psi': 0.0, 'chi': 0.0},
+                                  'gate_pass': True}
+                    continue
+                ro, to = timed_run(tfm, nm, psi0, chi0, args.warmup, args.steps)
+                del ro
+                te = trajs[nm][0]
+                others[nm] = {'steps_per_s': args.steps / to, 'trajectory_err': te,
+                              'gate_pass': bool(max(te.values()) <= GATE_TOL), 'dtype': DTYPES[nm]}
+                log(nm, others[nm])
+        log('errors', err, 'gate', gate)
+        if not gate['pass']:
+            log('WARNING: %s misses the accuracy gate (trajectory error above %.0e)' % (name, GATE_TOL))
+
     # the probed region: as many steps again, continuing the run
-    tfm.set_precision(prec)
+    configure(tfm, name)
     tfm.sync()
     tfm.probe_start(64 * max(args.steps, 1))
     t0 = time.perf_counter()
@@ -290,7 +406,7 @@ def main():
     probes = tfm.probe_stop()
     run.step(0, check=True)
     probed_rate = args.steps / (t1 - t0)
-    roof = roofline(probes, args.prec)
+    roof = roofline(probes, name)
     log('probed: %.3f ms/step; dominant kernel %s %.1f us/launch' % (1e3 * (t1 - t0) / args.steps, roof['kernel'],
                                                                      roof['avg_launch_us']))
     del run
@@ -301,17 +417,18 @@ def main():
         log('cpu baseline (oracle step)')
         cpu = cpu_baseline(p)
 
-    dtype = {'fp64': 'f64', 'fp32': 'f32-mfma+f64'}[args.prec]
     out = {
         'metric': METRIC, 'value': rate, 'unit': 'steps/s', 'n_gpus': 1, 'steps': args.steps,
         'warmup': args.warmup, 'ms_per_step': ms, 'higher_is_better': True, 'scaling': 'weak',
-        'vs_baseline': None, 'dtype': dtype,
+        'vs_baseline': None, 'dtype': DTYPES[name],
         'data': 'synthetic: centred q=4 vortex + seeded 1e-4 perturbation (SplitMix64/Box-Muller, seed 20261202)',
         'config': {'workload': '3D vortex AB2/CN time step, NR=NP=NZ=%d, NRCHOP=%d, NPCHOP=NZCHOP=%d, L=1, '
                                'ZLEN=2pi, VISC=5e-4, dt=5e-4, SVV on' % (p.nr, p.nrchop, p.npchop),
-                   'precision': args.prec},
+                   'precision': name},
+        'gate': gate,
         'fp64_steps_per_s': fp64_rate, 'speedup_vs_fp64': rate / fp64_rate if fp64_rate else None,
-        'rel_err_vs_fp64': err, 'adjoint_forward_ratio': None, 'probed_steps_per_s': probed_rate,
+        'rel_err_vs_fp64': err, 'ke_history': ke, 'other_precisions': others,
+        'adjoint_forward_ratio': None, 'probed_steps_per_s': probed_rate,
         'kernels': probe_table(probes),
         'roofline': roof, 'cpu_baseline': cpu,
     }

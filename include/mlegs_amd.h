@@ -24,8 +24,10 @@
  *   - One context per host thread; contexts share no mutable state.
  *   - Only the radial chopping offset (nrchop_offset) is supported; the
  *     reference's path never sets the azimuthal or axial offsets.
- *   - Hyperviscosity (hyperpow != 0) is accepted by the kit (it sizes nrdim)
- *     but the steppers reject it: helmp/ihelmp are not on this path.
+ *   - Hyperviscosity: with hyperpow = 4, 6 or 8 the kit sizes nrdim as
+ *     nr + hyperpow and febe / abcn / the vortex step take the reference's
+ *     hyperviscous branches through helmp / ihelmp
+ *     (src/submodules/mlegs_scalar_ops.f90:1181-1195, 1226-1255).
  */
 #ifndef MLEGS_AMD_H
 #define MLEGS_AMD_H
@@ -46,10 +48,22 @@ extern "C" {
 #define MLX_E_HIP       206  /* HIP runtime failure                        */
 
 /* precision of the Legendre contractions (trans and vec2tp) */
-#define MLX_PREC_FP64   0    /* FP64 operands, FP64 accumulation            */
+#define MLX_PREC_FP64   0    /* FP64 operands, FP64 accumulation (vector
+                                FMA), every column computed: the plain path */
 #define MLX_PREC_FP32   1    /* FP32 operands, FP32 MFMA accumulation       */
 #define MLX_PREC_BF16X3 2    /* 3-term split-BF16 on MFMA: reserved; rejected
                                 (MLX_E_ARG) by this build                   */
+#define MLX_PREC_FP64_MFMA 3 /* FP64 operands, FP64 MFMA accumulation
+                                (v_mfma_f64_16x16x4_f64)                    */
+
+/* contraction sites whose precision can be set apart from the context's
+ * (mlx_set_stage_precision) */
+#define MLX_STAGE_SYNTH 0    /* backward radial transforms of tp2vec and
+                                tp2curlvec (the advection term's six)      */
+#define MLX_STAGE_UP    1    /* vec2tp's radial transforms of r u_phi (to
+                                FFF for its value at r = inf, and back)    */
+#define MLX_STAGE_PROJ  2    /* vec2tp's five projection contractions      */
+#define MLX_STAGE_COUNT 3
 
 /* kernel probes (measurement): see mlx_probe_start */
 #define MLX_PROBE_FFT_PHI   0  /* phi FFT, one launch per direction       */
@@ -105,8 +119,14 @@ int mlx_kit_dims(mlx_ctx* ctx, int32_t* out9);
 int mlx_kit_get(mlx_ctx* ctx, const char* which, double* host, int64_t n);
 
 /* Legendre-contraction precision used by mlx_trans and mlx_vec2tp:
- * MLX_PREC_FP64 or MLX_PREC_FP32 (MLX_PREC_BF16X3 returns MLX_E_ARG). */
+ * MLX_PREC_FP64, MLX_PREC_FP32 or MLX_PREC_FP64_MFMA (MLX_PREC_BF16X3
+ * returns MLX_E_ARG).  Every precision but MLX_PREC_FP64 runs the MFMA
+ * kernels, and the vortex step then skips the modes de-aliasing zeroes. */
 int mlx_set_precision(mlx_ctx* ctx, int prec);
+/* Precision of one contraction site (MLX_STAGE_*) apart from the context's;
+ * prec = -1 makes the site follow mlx_set_precision again.  No counterpart
+ * in the reference (its contractions are all FP64). */
+int mlx_set_stage_precision(mlx_ctx* ctx, int stage, int prec);
 
 /* ---- fields --------------------------------------------------------------- */
 
@@ -178,10 +198,12 @@ int mlx_helmp(mlx_ctx* ctx, mlx_field* f, int power, double alpha, double beta);
 /* ihelmp(s, power, alpha, beta, tfm) (mlegs_scalar.f90:300-309;
  * mlegs_scalar_ops.f90:900-1000): the inverse, per (m, kappa) a banded solve
  * with kl = ku = power and partial pivoting; ln <- ln / alpha.  A column whose
- * factorization meets a zero pivot is left unsolved (the reference stops). */
+ * factorization meets a zero pivot is left unsolved and the call returns
+ * MLX_E_ARG (the reference stops); febe, abcn and the vortex calls report it
+ * the same way when hyperpow != 0. */
 int mlx_ihelmp(mlx_ctx* ctx, mlx_field* f, int power, double alpha, double beta);
 
-/* ---- time steppers (hyperpow = 0) ---------------------------------------- */
+/* ---- time steppers (hyperpow = 0, or 4, 6, 8 through helmp / ihelmp) ---- */
 
 /* febe(s, nl, dt, tfm) (mlegs_scalar.f90:350-359; mlegs_scalar_ops.f90:1157-1198). */
 int mlx_febe(mlx_ctx* ctx, mlx_field* s, const mlx_field* nl, double dt);

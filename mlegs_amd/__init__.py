@@ -14,7 +14,8 @@ import os
 import numpy as np
 
 from . import _lib
-from ._lib import MlxError, PREC_FP64, PREC_FP32, PREC_BF16X3  # noqa: F401
+from ._lib import MlxError, PREC_FP64, PREC_FP32, PREC_BF16X3, PREC_FP64_MFMA  # noqa: F401
+from ._lib import STAGE_SYNTH, STAGE_UP, STAGE_PROJ  # noqa: F401
 from .params import Params, read_input, timestep_set, InputError  # noqa: F401
 
 __all__ = ['Kit', 'Scalar', 'trans', 'chop', 'dealias', 'svv_filter', 'calcat0', 'calcat1', 'zeroat1',
@@ -67,6 +68,10 @@ class Kit:
     def set_precision(self, prec):
         self._check(self.lib.mlx_set_precision(self._ctx, int(prec)))
         self.precision = prec
+
+    def set_stage_precision(self, stage, prec):
+        """Precision of one contraction site (STAGE_*); prec = -1 follows set_precision."""
+        self._check(self.lib.mlx_set_stage_precision(self._ctx, int(stage), int(prec)))
 
     def get(self, which, n):
         buf = np.zeros(n, dtype=np.float64)

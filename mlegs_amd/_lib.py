@@ -17,7 +17,8 @@ MLX_E_FILE = 204
 MLX_E_NONFINITE = 205
 MLX_E_HIP = 206
 
-PREC_FP64, PREC_FP32, PREC_BF16X3 = 0, 1, 2
+PREC_FP64, PREC_FP32, PREC_BF16X3, PREC_FP64_MFMA = 0, 1, 2, 3
+STAGE_SYNTH, STAGE_UP, STAGE_PROJ = 0, 1, 2
 
 
 class MlxParams(ctypes.Structure):
@@ -54,6 +55,7 @@ SIGNATURES = {
     'mlx_kit_dims': ([_P, _IP], ctypes.c_int),
     'mlx_kit_get': ([_P, _S, _DP, _I64], ctypes.c_int),
     'mlx_set_precision': ([_P, ctypes.c_int], ctypes.c_int),
+    'mlx_set_stage_precision': ([_P, ctypes.c_int, ctypes.c_int], ctypes.c_int),
     'mlx_field_create': ([_P, ctypes.POINTER(_P)], ctypes.c_int),
     'mlx_field_destroy': ([_P, _P], ctypes.c_int),
     'mlx_field_copy': ([_P, _P, _P], ctypes.c_int),

@@ -204,13 +204,24 @@ void helmp(mlx_ctx *c, mlx_field *f, int power, double alpha, double beta) {
   k_axpby(c, f->e, nullptr, 1.0, sp->e, nullptr, 1.0);
 }
 
+// ihelmp's singular-system flag (device int): set by a column whose
+// factorization meets a zero pivot, cleared and read by the entry points
+int *sing_flag(mlx_ctx *c) { return reinterpret_cast<int *>(c->dsmall + 4000 + 40); }
+void clear_sing(mlx_ctx *c) { HIPCHK(hipMemsetAsync(sing_flag(c), 0, sizeof(int), c->stream)); }
+void check_sing(mlx_ctx *c) {
+  int h = 0;
+  HIPCHK(hipMemcpyAsync(&h, sing_flag(c), sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  if (h) MLX_THROW(MLX_E_ARG, "ihelmp: zero pivot in a banded system (singular); the column is left unsolved");
+}
+
 void ihelmp(mlx_ctx *c, mlx_field *f, int power, double alpha, double beta) {
   need_kit(c);
   check_power("ihelmp", power);
   to_fff(c, f);
   if (std::fabs(alpha) < 5e-14)
     MLX_THROW(MLX_E_ARG, "ihelmp: alpha equals to zero. Inversion of 0*identity is impossible");
-  k_ihelmp(c, f->e, f->sc, power, alpha, beta, f->off[0]);
+  k_ihelmp(c, f->e, f->sc, power, alpha, beta, f->off[0], sing_flag(c));
 }
 
 void idel2_proln(mlx_ctx *c, mlx_field *f) {
@@ -273,7 +284,11 @@ struct DealiasLimits {
 DealiasLimits dealias_limits(mlx_ctx *c) {
   const KitDev &k = c->kit;
   DealiasLimits d;
-  if (c->prec == MLX_PREC_FP64 || !mfma_ok(k, 0) || k.np <= 1) return d;
+  // the FP64 contraction kernels compute every column and slice: the
+  // restrictions hold only when every site of the advection term runs on MFMA
+  for (int s = 0; s < MLX_STAGE_COUNT; ++s)
+    if (stage_precision(c, s) == MLX_PREC_FP64) return d;
+  if (!mfma_ok(k, 0) || k.np <= 1) return d;
   d.on = true;
   d.pcut = k.np / 3 + 1 > 1 ? k.np / 3 + 1 : 1;
   if (d.pcut > k.npchop) d.pcut = k.npchop;
@@ -351,6 +366,7 @@ void abcn_pair(mlx_ctx *c, mlx_field *s1, mlx_field *s1p, mlx_field *nl1, mlx_fi
 void tp2vec_back(mlx_ctx *c, mlx_field *ur, mlx_field *up, mlx_field *uz, int ncols = -1, int zlo = 1,
                  int zhi = 0, bool phys_rows = false) {
   const KitDev &k = c->kit;
+  PrecScope ps(c, MLX_STAGE_SYNTH);
   const int rs[3] = {-1, -1, 0};
   const int nc = (ncols > 0 && ncols < k.npchop) ? ncols : k.npchop;
   // limited columns and the four-step phi kernels: the radial transform
@@ -359,7 +375,7 @@ void tp2vec_back(mlx_ctx *c, mlx_field *ur, mlx_field *up, mlx_field *uz, int nc
   // physical rows only, on the MFMA path: the radial transform hands the
   // phi transform its FP32 even/odd sums (half the bytes of the FP64 rows,
   // which the phi transform forms from them exactly)
-  const bool pairs = lean && phys_rows && c->prec != MLX_PREC_FP64 && mfma_ok(k, 0);
+  const bool pairs = lean && phys_rows && c->prec == MLX_PREC_FP32 && mfma_ok(k, 0);
   int q = 0;
   for (mlx_field *f : {ur, up, uz}) {
     f->off[0] = f->off[1] = f->off[2] = 0;
@@ -470,6 +486,7 @@ void vec2tp_pfp(mlx_ctx *c, mlx_field *ur, mlx_field *up, mlx_field *uz, mlx_fie
   // projection take columns m < nc, calcat and the log step column 0), so
   // the radial transforms leave them unwritten.
   k_ln_column(c, up->e, up->sc, -1.0);
+  PrecScope ps_up(c, MLX_STAGE_UP);
   leg_forward(c, up->e, c->scr[0], up->off[0], true, nc, !dl.on);
   swap_scr(c, up);
   if (k.nz > 1)
@@ -481,7 +498,10 @@ void vec2tp_pfp(mlx_ctx *c, mlx_field *ur, mlx_field *up, mlx_field *uz, mlx_fie
   swap_scr(c, up);
   k_vec2tp_up_ln(c, up->e, inf, psi->sc);
   if (k.nz > 1) fft_z(c, uz->e, -1, (double)k.nr * nc, 0, nc, 1, 0, zlo, zhi);
-  vec2tp_project(c, ur->e, up->e, uz->e, psi->e, chi->e, 3, nc, zlo, zhi);
+  {
+    PrecScope ps_proj(c, MLX_STAGE_PROJ);
+    vec2tp_project(c, ur->e, up->e, uz->e, psi->e, chi->e, 3, nc, zlo, zhi);
+  }
   set_space(psi, "FFF");
   set_space(chi, "FFF");
   if (pre_dealias && !dl.on) {
@@ -744,8 +764,20 @@ int mlx_set_precision(mlx_ctx *c, int prec) {
   return guarded(c, [&] {
     if (prec == MLX_PREC_BF16X3)
       MLX_THROW(MLX_E_ARG, "mlx_set_precision: the split-BF16 contractions are not part of this build");
-    if (prec != MLX_PREC_FP64 && prec != MLX_PREC_FP32) MLX_THROW(MLX_E_ARG, "mlx_set_precision: unknown precision");
-    c->prec = prec;
+    if (prec != MLX_PREC_FP64 && prec != MLX_PREC_FP32 && prec != MLX_PREC_FP64_MFMA)
+      MLX_THROW(MLX_E_ARG, "mlx_set_precision: unknown precision");
+    c->prec = c->prec_ctx = prec;
+  });
+}
+
+int mlx_set_stage_precision(mlx_ctx *c, int stage, int prec) {
+  return guarded(c, [&] {
+    if (stage < 0 || stage >= MLX_STAGE_COUNT) MLX_THROW(MLX_E_ARG, "mlx_set_stage_precision: unknown stage");
+    if (prec == MLX_PREC_BF16X3)
+      MLX_THROW(MLX_E_ARG, "mlx_set_stage_precision: the split-BF16 contractions are not part of this build");
+    if (prec != -1 && prec != MLX_PREC_FP64 && prec != MLX_PREC_FP32 && prec != MLX_PREC_FP64_MFMA)
+      MLX_THROW(MLX_E_ARG, "mlx_set_stage_precision: unknown precision");
+    c->stage_prec[stage] = prec;
   });
 }
 
@@ -765,6 +797,13 @@ int mlx_field_destroy(mlx_ctx *c, mlx_field *f) {
 
 int mlx_field_copy(mlx_ctx *c, mlx_field *d, const mlx_field *s) {
   return guarded(c, [&] { fcopy(c, d, s); });
+}
+
+// chop_index's radial check (mlegs_scalar_ops.f90:2053), made when an offset
+// is set: nrchop + offset rows must fit the field's column
+static void check_nroff(mlx_ctx *c, int32_t a) {
+  need_kit(c);
+  if (c->kit.nrchop + a > c->kit.nrdim_ref) MLX_THROW(MLX_E_ARG, "chop_index: chopping in r too large");
 }
 
 static void check_dims(mlx_ctx *c, int64_t a, int64_t b, int64_t d) {
@@ -789,6 +828,9 @@ int mlx_field_upload(mlx_ctx *c, mlx_field *f, const double *host, int64_t nrdim
     c->hsmall[0] = ln;
     HIPCHK(hipMemcpyAsync(f->sc, c->hsmall, sizeof(double), hipMemcpyHostToDevice, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
+    if (offsets && (offsets[1] != 0 || offsets[2] != 0))
+      MLX_THROW(MLX_E_ARG, "upload: only radial chopping offsets are supported on this path");
+    if (offsets) check_nroff(c, offsets[0]);
     set_space(f, space);
     for (int i = 0; i < 3; ++i) f->off[i] = offsets ? offsets[i] : 0;
   });
@@ -816,6 +858,9 @@ int mlx_field_download(mlx_ctx *c, const mlx_field *f, double *host, int64_t nrd
 int mlx_field_set_meta(mlx_ctx *c, mlx_field *f, const char *space, double ln, const int32_t *offsets) {
   return guarded(c, [&] {
     if (space_id(space) < 0) MLX_THROW(MLX_E_ARG, "set_meta: space must be PPP, PFP, FFP or FFF");
+    if (offsets && (offsets[1] != 0 || offsets[2] != 0))
+      MLX_THROW(MLX_E_ARG, "set_meta: only radial chopping offsets are supported on this path");
+    if (offsets) check_nroff(c, offsets[0]);
     set_space(f, space);
     k_set_scalar(c, f->sc, ln);
     if (offsets)
@@ -830,6 +875,7 @@ int mlx_field_get_meta(mlx_ctx *c, const mlx_field *f, char *space4, double *ln,
 int mlx_chop_offset(mlx_ctx *c, mlx_field *f, int32_t a, int32_t b, int32_t d) {
   return guarded(c, [&] {
     if (b != 0 || d != 0) MLX_THROW(MLX_E_ARG, "chop_offset: only radial offsets are supported on this path");
+    check_nroff(c, a);
     f->off[0] = a;
     f->off[1] = b;
     f->off[2] = d;
@@ -936,19 +982,27 @@ int mlx_helmp(mlx_ctx *c, mlx_field *f, int power, double alpha, double beta) {
   return guarded(c, [&] { helmp(c, f, power, alpha, beta); });
 }
 int mlx_ihelmp(mlx_ctx *c, mlx_field *f, int power, double alpha, double beta) {
-  return guarded(c, [&] { ihelmp(c, f, power, alpha, beta); });
+  return guarded(c, [&] {
+    clear_sing(c);
+    ihelmp(c, f, power, alpha, beta);
+    check_sing(c);
+  });
 }
 
 int mlx_febe(mlx_ctx *c, mlx_field *s, const mlx_field *nl, double dt) {
   return guarded(c, [&] {
     need_kit(c);
+    if (c->p.hyperpow != 0) clear_sing(c);
     febe(c, s, nl, dt);
+    if (c->p.hyperpow != 0) check_sing(c);
   });
 }
 int mlx_abcn(mlx_ctx *c, mlx_field *s, mlx_field *s_p, const mlx_field *nl, mlx_field *nl_p, double dt) {
   return guarded(c, [&] {
     need_kit(c);
+    if (c->p.hyperpow != 0) clear_sing(c);
     abcn(c, s, s_p, const_cast<mlx_field *>(nl), nl_p, dt, true, false);
+    if (c->p.hyperpow != 0) check_sing(c);
   });
 }
 
@@ -1069,6 +1123,7 @@ int mlx_vortex_bootstrap(mlx_ctx *c, mlx_vortex *v) {
     mlx_field *psi_prev = v->f[4], *chi_prev = v->f[5], *nlpsi_prev = v->f[6], *nlchi_prev = v->f[7];
     mlx_field *uz = v->f[8];
     double dt = c->p.dt;
+    if (c->p.hyperpow != 0) clear_sing(c);
     advection_rhs(c, psi, chi, nlpsi, nlchi, uz);
     fcopy(c, psi_prev, psi);
     fcopy(c, nlpsi_prev, nlpsi);
@@ -1096,6 +1151,7 @@ int mlx_vortex_bootstrap(mlx_ctx *c, mlx_vortex *v) {
     zeroat1(c, chi);
     advection_rhs(c, psi, chi, nlpsi, nlchi, uz, true);
     v->prev_stale = false;
+    if (c->p.hyperpow != 0) check_sing(c);
   });
 }
 
@@ -1107,6 +1163,7 @@ int mlx_vortex_step(mlx_ctx *c, mlx_vortex *v, int32_t nsteps, double dt, int32_
     mlx_field *psi_prev = v->f[4], *chi_prev = v->f[5], *nlpsi_prev = v->f[6], *nlchi_prev = v->f[7];
     mlx_field *uz = v->f[8];
     const DealiasLimits dl = dealias_limits(c);
+    if (c->p.hyperpow != 0 && nsteps > 0) clear_sing(c);
     for (int it = 0; it < nsteps; ++it) {
       abcn_pair(c, psi, psi_prev, nlpsi, nlpsi_prev, chi, chi_prev, nlchi, nlchi_prev, dt, dl);
       v->prev_stale = true;
@@ -1133,6 +1190,7 @@ int mlx_vortex_step(mlx_ctx *c, mlx_vortex *v, int32_t nsteps, double dt, int32_
       HIPCHK(hipMemcpyAsync(&h, v->flag, sizeof(int), hipMemcpyDeviceToHost, c->stream));
       HIPCHK(hipStreamSynchronize(c->stream));
       if (h) MLX_THROW(MLX_E_NONFINITE, "ERROR: non-finite vortex state");
+      if (c->p.hyperpow != 0 && nsteps > 0) check_sing(c);
     }
   });
 }

@@ -223,13 +223,14 @@ namespace mlx {
 void kit_free(KitDev &k) {
   void *ptrs[] = {k.x, k.w, k.lnx, k.r, k.ak, k.lognorm, k.pf, k.pf32, k.at0, k.at1, k.d2c,
                   k.xxc, k.tV, k.tD, k.tT, k.tV32, k.tD32, k.tT32, k.tw_nph, k.tw_nz, k.tw_half,
-                  k.pfg32, k.tVg32, k.tDg32, k.tTg32};
+                  k.pfg32, k.tVg32, k.tDg32, k.tTg32, k.pfg64, k.tVg64, k.tDg64, k.tTg64};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   k.x = k.w = k.lnx = k.r = k.ak = k.lognorm = k.pf = k.at0 = k.at1 = k.d2c = k.xxc = nullptr;
   k.tV = k.tD = k.tT = nullptr;
   k.pf32 = k.tV32 = k.tD32 = k.tT32 = nullptr;
   k.pfg32 = k.tVg32 = k.tDg32 = k.tTg32 = nullptr;
+  k.pfg64 = k.tVg64 = k.tDg64 = k.tTg64 = nullptr;
   k.mfma_ap = 0;
   k.tw_nph = k.tw_nz = k.tw_half = nullptr;
   k.ready = false;

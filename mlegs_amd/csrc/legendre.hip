@@ -320,7 +320,7 @@ void leg_forward(mlx_ctx *c, const double2 *in, double2 *out, int nroff, bool ze
   const KitDev &k = c->kit;
   if (k.nrchop + nroff > k.nrdim_ref) MLX_THROW(MLX_E_ARG, "rtrans_forward: chopping in r too large");
   if (c->prec != MLX_PREC_FP64 && mfma_ok(k, nroff)) {
-    leg_forward_f32(c, in, out, ncols, ztail);
+    leg_forward_mfma(c, in, out, ncols, ztail);
     return;
   }
   if (zero) HIPCHK(hipMemsetAsync(out, 0, sizeof(double2) * c->nelem, c->stream));
@@ -337,7 +337,7 @@ void leg_backward(mlx_ctx *c, const double2 *in, double2 *out, int nroff, bool z
   const KitDev &k = c->kit;
   if (k.nrchop + nroff > k.nrdim_ref) MLX_THROW(MLX_E_ARG, "rtrans_backward: chopping in r too large");
   if (c->prec != MLX_PREC_FP64 && mfma_ok(k, nroff)) {
-    leg_backward_f32(c, in, out, ncols, ztail);
+    leg_backward_mfma(c, in, out, ncols, ztail);
     return;
   }
   if (zero) HIPCHK(hipMemsetAsync(out, 0, sizeof(double2) * c->nelem, c->stream));
@@ -356,7 +356,7 @@ void vec2tp_project(mlx_ctx *c, const double2 *ur, const double2 *up, const doub
                     double2 *chi, int nroff, int ncols, int zlo, int zhi) {
   const KitDev &k = c->kit;
   if (c->prec != MLX_PREC_FP64 && mfma_ok(k, 0)) {
-    vec2tp_project_f32(c, ur, up, uz, psi, chi, ncols, zlo, zhi);
+    vec2tp_project_mfma(c, ur, up, uz, psi, chi, ncols, zlo, zhi);
     return;
   }
   HIPCHK(hipMemsetAsync(psi, 0, sizeof(double2) * c->nelem, c->stream));

@@ -1,19 +1,24 @@
-// Mixed-precision Legendre contractions on FP32 MFMA (v_mfma_f32_16x16x4_f32).
+// Legendre contractions on MFMA, in FP32 (v_mfma_f32_16x16x4_f32) or FP64
+// (v_mfma_f64_16x16x4_f64).
 //
 // Same contractions as legendre.hip (rtrans_forward / rtrans_backward /
 // vector_projection, src/submodules/mlegs_scalar_ops.f90:1852-2008,
-// 1387-1436) with FP32 operands and FP32 accumulation; fields stay FP64 in
-// HBM and are rounded to FP32 when staged into LDS.  These kernels also write
-// every entry of their output (zeros outside the active set), so no separate
-// zero-fill pass is needed.
+// 1387-1436).  Fields stay FP64 in HBM; the FP32 kernels round them to FP32
+// when staging into LDS and accumulate in FP32, the FP64 kernels stage and
+// accumulate in FP64 (the same kernels, templated on the operand type).
+// These kernels also write every entry of their output (zeros outside the
+// active set), so no separate zero-fill pass is needed.
 //
 // GEMM orientation.  Both directions are computed transposed: the MFMA "A"
 // operand is the field (rows = real columns j = 2k + re/im, contraction
 // dimension in LDS), the "B" operand is the table (columns = radial index).
-// The 16x16 accumulator layout (lane l holds rows 4(l/16)+r, column l%16)
-// then gives each lane the (re, im) pairs of two axial slices for one radial
-// index, so rows 2a and 2a+1 (both parities) or x and nr-1-x are written as
-// contiguous FP64 pairs.
+// The FP32 instruction's 16x16 accumulator holds rows 4(l/16)+r, column l%16
+// in element r of lane l; the FP64 instruction's holds rows l/16 + 4r
+// (measured on gfx950).  The FP64 kernels feed A row i the field column
+// 4(i%4) + i/4, so in both types element r of lane l holds field column
+// 4(l/16)+r: each lane has the (re, im) pairs of two axial slices for one
+// radial index, and rows 2a and 2a+1 (both parities) or x and nr-1-x are
+// written as contiguous FP64 pairs.
 //
 // Work decomposition: one workgroup (4 wavefronts) per (m, 32 axial slices);
 // wave w owns 16-row tiles w, w+4, ... of the radial index.  Contraction in
@@ -23,31 +28,63 @@
 namespace {
 
 typedef float floatx4 __attribute__((ext_vector_type(4)));
+typedef double doublex4 __attribute__((ext_vector_type(4)));
 
 constexpr int KC = 8;         // contraction chunk
 constexpr int KT = 32;        // axial slices per workgroup
 constexpr int BJ = 2 * KT;    // real columns per workgroup
-// LDS row pitches are 16 (mod 64) floats: the four rows an MFMA operand read
-// touches (lane / 16) then fall in disjoint banks
-constexpr int BJP = BJ + 16;  // padded LDS row (floats)
+// LDS row pitches are 16 (mod 64) elements: the four rows an MFMA operand
+// read touches (lane / 16) then fall in disjoint banks
+constexpr int BJP = BJ + 16;  // padded LDS row (elements)
 
-__device__ inline floatx4 mfma(float a, float b, floatx4 c) {
-  return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
-}
+// the MFMA of one operand type: 16x16 accumulator (4 per lane), 16-byte
+// table loads of VE elements
+template <typename T>
+struct Mma;
+template <>
+struct Mma<float> {
+  typedef floatx4 acc;
+  typedef float4 vec;
+  static constexpr int VE = 4;
+  __device__ static inline acc op(float a, float b, acc c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+  __device__ static inline acc zero() { return acc{0.f, 0.f, 0.f, 0.f}; }
+  __device__ static inline vec vzero() { return make_float4(0.f, 0.f, 0.f, 0.f); }
+  // field column that A row i carries (accumulator element r of lane l is row 4(l/16)+r)
+  __device__ static inline int arow(int i) { return i; }
+};
+template <>
+struct Mma<double> {
+  typedef doublex4 acc;
+  typedef double2 vec;
+  static constexpr int VE = 2;
+  __device__ static inline acc op(double a, double b, acc c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+  __device__ static inline acc zero() { return acc{0.0, 0.0, 0.0, 0.0}; }
+  __device__ static inline vec vzero() { return make_double2(0.0, 0.0); }
+  // accumulator element r of lane l is row l/16 + 4r: A row i carries field
+  // column 4(i%4) + i/4, which puts column 4(l/16)+r in that element
+  __device__ static inline int arow(int i) { return 4 * (i & 3) + (i >> 2); }
+};
 
 // ---------------------------------------------------------------------------
 // forward: out(2a+p, m, k) = sum_x P_p(a, x) fold_p(x, k)
 //   table pfg[m][p][x][a] (a padded to AP), fold from FP64 input
 // ---------------------------------------------------------------------------
-template <int AT>  // 16-row tiles per wave: AP = 4 * AT * 16
-__global__ void __launch_bounds__(256) k_leg_fwd_f32(const double2 *__restrict__ in, double2 *__restrict__ out,
-                                                     const float *__restrict__ pfg, const double *__restrict__ w,
+template <typename T, int AT>  // 16-row tiles per wave: AP = 4 * AT * 16
+__global__ void __launch_bounds__(256) k_leg_fwd_mf(const double2 *__restrict__ in, double2 *__restrict__ out,
+                                                    const T *__restrict__ pfg, const double *__restrict__ w,
                                                      int nrdim, int npdim, int nz, int nr, int nrh, int nrc,
                                                      int npc, int nyt) {
   constexpr int AP = 4 * AT * 16;
   constexpr int APP = AP + 16;
-  __shared__ float Bs[2][KC][BJP];
-  __shared__ float As[2][KC][APP];
+  typedef Mma<T> M;
+  typedef typename M::vec vec;
+  constexpr int VE = M::VE;
+  __shared__ T Bs[2][KC][BJP];
+  __shared__ T As[2][KC][APP];
   int m, yt;
   xcd_map(blockIdx.x, nyt, m, yt);
   if (m >= npdim) return;
@@ -65,33 +102,33 @@ __global__ void __launch_bounds__(256) k_leg_fwd_f32(const double2 *__restrict__
   const int nn = nrc - m > 0 ? nrc - m : 0;
   const int Mmax = (nn + 1) / 2;
   const double2 *icol = in + (size_t)nrdim * m;
-  const float *T = pfg + (size_t)m * 2 * nrh * AP;
-  floatx4 acc[AT][4][2];
+  const T *Tg = pfg + (size_t)m * 2 * nrh * AP;
+  typename M::acc acc[AT][4][2];
 #pragma unroll
   for (int s = 0; s < AT; ++s)
 #pragma unroll
     for (int t = 0; t < 4; ++t)
 #pragma unroll
-      for (int p = 0; p < 2; ++p) acc[s][t][p] = floatx4{0.f, 0.f, 0.f, 0.f};
+      for (int p = 0; p < 2; ++p) acc[s][t][p] = M::zero();
 
   // Software pipeline: the next chunk's table and field loads are issued
   // into registers before the current chunk's MFMA work and stored to LDS
-  // after it.  Table: 2 KC AP / 4 float4 per chunk; field: KC KT points,
+  // after it.  Table: 2 KC AP / VE 16-byte loads per chunk; field: KC KT points,
   // each the pair x, nr-1-x and the weight, folded when stored.
-  constexpr int NTA = (2 * KC * (AP / 4) + 255) / 256;
+  constexpr int NTA = (2 * KC * (AP / VE) + 255) / 256;
   constexpr int NFA = (KC * KT + 255) / 256;
-  float4 rt[NTA];
+  vec rt[NTA];
   double2 ra[NFA], rb[NFA];
   double rw[NFA];
   auto fetch = [&](int x0) {
 #pragma unroll
     for (int q = 0; q < NTA; ++q) {
       const int idx = tid + 256 * q;
-      float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
-      if (idx < 2 * KC * (AP / 4)) {
-        const int a = 4 * (idx % (AP / 4)), x = (idx / (AP / 4)) % KC, p = idx / ((AP / 4) * KC);
+      vec v = M::vzero();
+      if (idx < 2 * KC * (AP / VE)) {
+        const int a = VE * (idx % (AP / VE)), x = (idx / (AP / VE)) % KC, p = idx / ((AP / VE) * KC);
         const int xg = x0 + x;
-        if (xg < nrh) v = *reinterpret_cast<const float4 *>(T + ((size_t)p * nrh + xg) * AP + a);
+        if (xg < nrh) v = *reinterpret_cast<const vec *>(Tg + ((size_t)p * nrh + xg) * AP + a);
       }
       rt[q] = v;
     }
@@ -114,9 +151,9 @@ __global__ void __launch_bounds__(256) k_leg_fwd_f32(const double2 *__restrict__
 #pragma unroll
     for (int q = 0; q < NTA; ++q) {
       const int idx = tid + 256 * q;
-      if (idx < 2 * KC * (AP / 4)) {
-        const int a = 4 * (idx % (AP / 4)), x = (idx / (AP / 4)) % KC, p = idx / ((AP / 4) * KC);
-        *reinterpret_cast<float4 *>(&As[p][x][a]) = rt[q];
+      if (idx < 2 * KC * (AP / VE)) {
+        const int a = VE * (idx % (AP / VE)), x = (idx / (AP / VE)) % KC, p = idx / ((AP / VE) * KC);
+        *reinterpret_cast<vec *>(&As[p][x][a]) = rt[q];
       }
     }
 #pragma unroll
@@ -126,10 +163,10 @@ __global__ void __launch_bounds__(256) k_leg_fwd_f32(const double2 *__restrict__
         const int x = idx % KC, kl = idx / KC;
         const double2 a = ra[q], b = rb[q];
         const double wx = rw[q];
-        Bs[0][x][2 * kl] = (float)((a.x + b.x) * wx);
-        Bs[0][x][2 * kl + 1] = (float)((a.y + b.y) * wx);
-        Bs[1][x][2 * kl] = (float)((a.x - b.x) * wx);
-        Bs[1][x][2 * kl + 1] = (float)((a.y - b.y) * wx);
+        Bs[0][x][2 * kl] = (T)((a.x + b.x) * wx);
+        Bs[0][x][2 * kl + 1] = (T)((a.y + b.y) * wx);
+        Bs[1][x][2 * kl] = (T)((a.x - b.x) * wx);
+        Bs[1][x][2 * kl + 1] = (T)((a.y - b.y) * wx);
       }
     }
     __syncthreads();
@@ -139,16 +176,16 @@ __global__ void __launch_bounds__(256) k_leg_fwd_f32(const double2 *__restrict__
       const int xr = kk + lane / 16;
 #pragma unroll
       for (int p = 0; p < 2; ++p) {
-        float fa[4];
+        T fa[4];
 #pragma unroll
-        for (int t = 0; t < 4; ++t) fa[t] = Bs[p][xr][16 * t + lane % 16];
+        for (int t = 0; t < 4; ++t) fa[t] = Bs[p][xr][16 * t + M::arow(lane % 16)];
 #pragma unroll
         for (int s = 0; s < AT; ++s) {
           const int at = wv + 4 * s;
           if (at * 16 >= Mmax) continue;
-          float fb = As[p][xr][at * 16 + lane % 16];
+          T fb = As[p][xr][at * 16 + lane % 16];
 #pragma unroll
-          for (int t = 0; t < 4; ++t) acc[s][t][p] = mfma(fa[t], fb, acc[s][t][p]);
+          for (int t = 0; t < 4; ++t) acc[s][t][p] = M::op(fa[t], fb, acc[s][t][p]);
         }
       }
     }
@@ -192,15 +229,19 @@ __global__ void __launch_bounds__(256) k_leg_fwd_f32(const double2 *__restrict__
 // of the two FP64 rows, which a consumer forms exactly as be + bo and be - bo
 // in FP64.  Only columns m < npc are written.
 // ---------------------------------------------------------------------------
-template <int XT, bool PAIRS>  // 16-column tiles of x per wave: XP = 4 * XT * 16 >= nrh
-__global__ void __launch_bounds__(256) k_leg_bwd_f32(const double2 *__restrict__ in, double2 *__restrict__ out,
-                                                     float4 *__restrict__ pairs, const float *__restrict__ pf32,
+template <typename T, int XT, bool PAIRS>  // 16-column tiles of x per wave: XP = 4 * XT * 16 >= nrh
+__global__ void __launch_bounds__(256) k_leg_bwd_mf(const double2 *__restrict__ in, double2 *__restrict__ out,
+                                                    float4 *__restrict__ pairs, const T *__restrict__ ptab,
                                                      int nrdim, int npdim, int nz, int nr, int nrh, int ne, int nrc,
                                                      int npc, int nyt, int pstride) {
   constexpr int XP = 4 * XT * 16;
   constexpr int XPP = XP + 16;
-  __shared__ float Es[2][KC][BJP];
-  __shared__ float Ps[2][KC][XPP];
+  typedef Mma<T> M;
+  typedef typename M::vec vec;
+  constexpr int VE = M::VE;
+  static_assert(!PAIRS || sizeof(T) == 4, "radial pairs are FP32 sums");
+  __shared__ T Es[2][KC][BJP];
+  __shared__ T Ps[2][KC][XPP];
   int m, yt;
   xcd_map(blockIdx.x, nyt, m, yt);
   if (m >= npdim) return;
@@ -219,14 +260,14 @@ __global__ void __launch_bounds__(256) k_leg_bwd_f32(const double2 *__restrict__
   const int nn = nrc - m > 0 ? nrc - m : 0;
   const int Me = (nn + 1) / 2;
   const double2 *icol = in + (size_t)nrdim * m;
-  const float *P = pf32 + (size_t)m * ne * nrh;
-  floatx4 acc[XT][4][2];
+  const T *P = ptab + (size_t)m * ne * nrh;
+  typename M::acc acc[XT][4][2];
 #pragma unroll
   for (int s = 0; s < XT; ++s)
 #pragma unroll
     for (int t = 0; t < 4; ++t)
 #pragma unroll
-      for (int p = 0; p < 2; ++p) acc[s][t][p] = floatx4{0.f, 0.f, 0.f, 0.f};
+      for (int p = 0; p < 2; ++p) acc[s][t][p] = M::zero();
 
   // MFMA work of one staged chunk (both parities)
   auto mfma_chunk = [&]() {
@@ -235,21 +276,21 @@ __global__ void __launch_bounds__(256) k_leg_bwd_f32(const double2 *__restrict__
       const int ar = kk + lane / 16;
 #pragma unroll
       for (int p = 0; p < 2; ++p) {
-        float fa[4];
+        T fa[4];
 #pragma unroll
-        for (int t = 0; t < 4; ++t) fa[t] = Es[p][ar][16 * t + lane % 16];
+        for (int t = 0; t < 4; ++t) fa[t] = Es[p][ar][16 * t + M::arow(lane % 16)];
 #pragma unroll
         for (int s = 0; s < XT; ++s) {
           const int xt = wv + 4 * s;
           if (xt * 16 >= nrh) continue;
-          float fb = Ps[p][ar][xt * 16 + lane % 16];
+          T fb = Ps[p][ar][xt * 16 + lane % 16];
 #pragma unroll
-          for (int t = 0; t < 4; ++t) acc[s][t][p] = mfma(fa[t], fb, acc[s][t][p]);
+          for (int t = 0; t < 4; ++t) acc[s][t][p] = M::op(fa[t], fb, acc[s][t][p]);
         }
       }
     }
   };
-  // field chunk rows 2 a0 .. 2 a0 + 2 KC - 1 of 32 slices (FP64 -> FP32)
+  // field chunk rows 2 a0 .. 2 a0 + 2 KC - 1 of 32 slices (FP64 -> T)
   auto field_at = [&](int a0, int idx) {
     const int r = idx % (2 * KC), kl = idx / (2 * KC);
     const int n = 2 * a0 + r, kg = k0 + kl;
@@ -257,25 +298,25 @@ __global__ void __launch_bounds__(256) k_leg_bwd_f32(const double2 *__restrict__
   };
   auto field_put = [&](int idx, double2 v) {
     const int r = idx % (2 * KC), kl = idx / (2 * KC);
-    Es[r & 1][r >> 1][2 * kl] = (float)v.x;
-    Es[r & 1][r >> 1][2 * kl + 1] = (float)v.y;
+    Es[r & 1][r >> 1][2 * kl] = (T)v.x;
+    Es[r & 1][r >> 1][2 * kl + 1] = (T)v.y;
   };
-  if ((nrh & 3) == 0) {
+  if ((nrh % VE) == 0) {
     // software pipeline: the next chunk's global loads are in flight while
-    // the current chunk runs on MFMA; four consecutive x per 16-byte load
-    constexpr int NTB = (2 * KC * (XP / 4) + 255) / 256;
+    // the current chunk runs on MFMA; VE consecutive x per 16-byte load
+    constexpr int NTB = (2 * KC * (XP / VE) + 255) / 256;
     constexpr int NFB = (2 * KC * KT + 255) / 256;
-    float4 rt[NTB];
+    vec rt[NTB];
     double2 rf[NFB];
     auto fetch = [&](int a0) {
 #pragma unroll
       for (int q = 0; q < NTB; ++q) {
         const int idx = tid + 256 * q;
-        float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
-        if (idx < 2 * KC * (XP / 4)) {
-          const int x = 4 * (idx % (XP / 4)), a = (idx / (XP / 4)) % KC, p = idx / ((XP / 4) * KC);
+        vec v = M::vzero();
+        if (idx < 2 * KC * (XP / VE)) {
+          const int x = VE * (idx % (XP / VE)), a = (idx / (XP / VE)) % KC, p = idx / ((XP / VE) * KC);
           const int n = 2 * (a0 + a) + p;
-          if (x < nrh && n < nn) v = *reinterpret_cast<const float4 *>(P + (size_t)n * nrh + x);
+          if (x < nrh && n < nn) v = *reinterpret_cast<const vec *>(P + (size_t)n * nrh + x);
         }
         rt[q] = v;
       }
@@ -290,9 +331,9 @@ __global__ void __launch_bounds__(256) k_leg_bwd_f32(const double2 *__restrict__
 #pragma unroll
       for (int q = 0; q < NTB; ++q) {
         const int idx = tid + 256 * q;
-        if (idx < 2 * KC * (XP / 4)) {
-          const int x = 4 * (idx % (XP / 4)), a = (idx / (XP / 4)) % KC, p = idx / ((XP / 4) * KC);
-          *reinterpret_cast<float4 *>(&Ps[p][a][x]) = rt[q];
+        if (idx < 2 * KC * (XP / VE)) {
+          const int x = VE * (idx % (XP / VE)), a = (idx / (XP / VE)) % KC, p = idx / ((XP / VE) * KC);
+          *reinterpret_cast<vec *>(&Ps[p][a][x]) = rt[q];
         }
       }
 #pragma unroll
@@ -310,7 +351,7 @@ __global__ void __launch_bounds__(256) k_leg_bwd_f32(const double2 *__restrict__
       for (int idx = tid; idx < 2 * KC * XP; idx += 256) {
         int x = idx % XP, a = (idx / XP) % KC, p = idx / (XP * KC);
         int n = 2 * (a0 + a) + p;
-        Ps[p][a][x] = (x < nrh && n < nn) ? P[(size_t)n * nrh + x] : 0.f;
+        Ps[p][a][x] = (x < nrh && n < nn) ? P[(size_t)n * nrh + x] : (T)0;
       }
       for (int idx = tid; idx < 2 * KC * KT; idx += 256) field_put(idx, field_at(a0, idx));
       __syncthreads();
@@ -330,8 +371,8 @@ __global__ void __launch_bounds__(256) k_leg_bwd_f32(const double2 *__restrict__
         if (kg >= nz) continue;
         if (PAIRS) {
           pairs[((size_t)kg * npdim + m) * pstride + x] =
-              make_float4(acc[s][t][0][2 * h], acc[s][t][0][2 * h + 1], acc[s][t][1][2 * h],
-                          acc[s][t][1][2 * h + 1]);
+              make_float4((float)acc[s][t][0][2 * h], (float)acc[s][t][0][2 * h + 1], (float)acc[s][t][1][2 * h],
+                          (float)acc[s][t][1][2 * h + 1]);
           continue;
         }
         double ber = acc[s][t][0][2 * h], bei = acc[s][t][0][2 * h + 1];
@@ -359,19 +400,22 @@ __global__ void __launch_bounds__(256) k_leg_bwd_f32(const double2 *__restrict__
 // computed: the three rows of the radial offset are chopped afterwards by
 // the reference.  Tables tg[m][p][x][a] (V, D, T).
 // ---------------------------------------------------------------------------
-template <int AT>
-__global__ void __launch_bounds__(256) k_v2t_f32(const double2 *__restrict__ ur, const double2 *__restrict__ up,
-                                                 const double2 *__restrict__ uz, double2 *__restrict__ psi,
-                                                 double2 *__restrict__ chi, const float *__restrict__ tgV,
-                                                 const float *__restrict__ tgD, const float *__restrict__ tgT,
+template <typename T, int AT>
+__global__ void __launch_bounds__(256) k_v2t_mf(const double2 *__restrict__ ur, const double2 *__restrict__ up,
+                                                const double2 *__restrict__ uz, double2 *__restrict__ psi,
+                                                double2 *__restrict__ chi, const T *__restrict__ tgV,
+                                                const T *__restrict__ tgD, const T *__restrict__ tgT,
                                                  const double *__restrict__ ak, int nrdim, int npdim, int nz, int nr,
                                                  int nrh, int nrc0, int npc, int nzc, int nzcu, int nyt, int zlo,
                                                  int zhi) {
   constexpr int AP = 4 * AT * 16;
   constexpr int APP = AP + 16;
   constexpr int KV = 8;   // contraction chunk for this kernel
-  __shared__ float X[5][KV][BJP];
-  __shared__ float Tb[3][KV][APP];
+  typedef Mma<T> MM;
+  typedef typename MM::vec vec;
+  constexpr int VE = MM::VE;
+  __shared__ T X[5][KV][BJP];
+  __shared__ T Tb[3][KV][APP];
   // both parities and every kappa tile of a column m on one XCD: the two
   // parity workgroups read the same input columns and write interleaved
   // rows of the same output lines, which then meet in that XCD's L2
@@ -395,19 +439,19 @@ __global__ void __launch_bounds__(256) k_v2t_f32(const double2 *__restrict__ ur,
   const double2 *cz = uz + (size_t)nrdim * m;
   const size_t tb = ((size_t)m * 2 + p) * nrh * AP;
   const double seo = (p == 0) ? 1.0 : -1.0;
-  floatx4 aps[AT][4], ach[AT][4];
+  typename MM::acc aps[AT][4], ach[AT][4];
 #pragma unroll
   for (int s = 0; s < AT; ++s)
 #pragma unroll
-    for (int t = 0; t < 4; ++t) aps[s][t] = ach[s][t] = floatx4{0.f, 0.f, 0.f, 0.f};
+    for (int t = 0; t < 4; ++t) aps[s][t] = ach[s][t] = MM::zero();
 
   // Software pipeline as in the forward kernel: raw loads of the next
   // chunk (three tables; ur, up, uz at x and nr-1-x) are held in registers
   // during the current chunk's MFMA work and combined when stored.  The
   // loop starts one chunk early so the fetch code appears once.
   static_assert(KV * KT == 256, "one field point per thread and chunk");
-  constexpr int NTV = (3 * KV * (AP / 4) + 255) / 256;
-  float4 rt[NTV];
+  constexpr int NTV = (3 * KV * (AP / VE) + 255) / 256;
+  vec rt[NTV];
   double2 rra, rrb, rpa, rpb, rza, rzb;
   double rkv = 0.0;
   const int fx = tid % KV, fkl = tid / KV;
@@ -417,9 +461,9 @@ __global__ void __launch_bounds__(256) k_v2t_f32(const double2 *__restrict__ ur,
   #pragma unroll
         for (int j = 0; j < NTV; ++j) {
           const int idx = tid + 256 * j;
-          if (idx < 3 * KV * (AP / 4)) {
-            const int a = 4 * (idx % (AP / 4)), x = (idx / (AP / 4)) % KV, q = idx / ((AP / 4) * KV);
-            *reinterpret_cast<float4 *>(&Tb[q][x][a]) = rt[j];
+          if (idx < 3 * KV * (AP / VE)) {
+            const int a = VE * (idx % (AP / VE)), x = (idx / (AP / VE)) % KV, q = idx / ((AP / VE) * KV);
+            *reinterpret_cast<vec *>(&Tb[q][x][a]) = rt[j];
           }
         }
         const double kv = rkv;
@@ -433,11 +477,11 @@ __global__ void __launch_bounds__(256) k_v2t_f32(const double2 *__restrict__ ur,
         const double2 x3 = make_double2(-kv * roe.y, kv * roe.x);           // i kv oe(ur)
         const double2 x4 = make_double2(mv * kv * peo.x, mv * kv * peo.y);  // m kv eo(up)
         const double2 x5 = make_double2(-zeo.x, -zeo.y);                    // -eo(uz)
-        X[0][fx][2 * fkl] = (float)x1.x; X[0][fx][2 * fkl + 1] = (float)x1.y;
-        X[1][fx][2 * fkl] = (float)x2.x; X[1][fx][2 * fkl + 1] = (float)x2.y;
-        X[2][fx][2 * fkl] = (float)x3.x; X[2][fx][2 * fkl + 1] = (float)x3.y;
-        X[3][fx][2 * fkl] = (float)x4.x; X[3][fx][2 * fkl + 1] = (float)x4.y;
-        X[4][fx][2 * fkl] = (float)x5.x; X[4][fx][2 * fkl + 1] = (float)x5.y;
+        X[0][fx][2 * fkl] = (T)x1.x; X[0][fx][2 * fkl + 1] = (T)x1.y;
+        X[1][fx][2 * fkl] = (T)x2.x; X[1][fx][2 * fkl + 1] = (T)x2.y;
+        X[2][fx][2 * fkl] = (T)x3.x; X[2][fx][2 * fkl + 1] = (T)x3.y;
+        X[3][fx][2 * fkl] = (T)x4.x; X[3][fx][2 * fkl + 1] = (T)x4.y;
+        X[4][fx][2 * fkl] = (T)x5.x; X[4][fx][2 * fkl + 1] = (T)x5.y;
         __syncthreads();
       }
       const int xn = x0 + KV;  // chunk fetched now, stored next iteration
@@ -445,12 +489,12 @@ __global__ void __launch_bounds__(256) k_v2t_f32(const double2 *__restrict__ ur,
   #pragma unroll
         for (int j = 0; j < NTV; ++j) {
           const int idx = tid + 256 * j;
-          float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
-          if (idx < 3 * KV * (AP / 4)) {
-            const int a = 4 * (idx % (AP / 4)), x = (idx / (AP / 4)) % KV, q = idx / ((AP / 4) * KV);
+          vec v = MM::vzero();
+          if (idx < 3 * KV * (AP / VE)) {
+            const int a = VE * (idx % (AP / VE)), x = (idx / (AP / VE)) % KV, q = idx / ((AP / VE) * KV);
             const int xg = xn + x;
-            const float *src = (q == 0) ? tgV : (q == 1) ? tgD : tgT;
-            if (xg < nrh) v = *reinterpret_cast<const float4 *>(src + tb + (size_t)xg * AP + a);
+            const T *src = (q == 0) ? tgV : (q == 1) ? tgD : tgT;
+            if (xg < nrh) v = *reinterpret_cast<const vec *>(src + tb + (size_t)xg * AP + a);
           }
           rt[j] = v;
         }
@@ -473,25 +517,25 @@ __global__ void __launch_bounds__(256) k_v2t_f32(const double2 *__restrict__ ur,
   #pragma unroll
       for (int kk = 0; kk < KV; kk += 4) {
         const int xr = kk + lane / 16;
-        float f[5][4];
+        T f[5][4];
   #pragma unroll
         for (int q = 0; q < 5; ++q)
   #pragma unroll
-          for (int t = 0; t < 4; ++t) f[q][t] = X[q][xr][16 * t + lane % 16];
+          for (int t = 0; t < 4; ++t) f[q][t] = X[q][xr][16 * t + MM::arow(lane % 16)];
   #pragma unroll
         for (int s = 0; s < AT; ++s) {
           const int at = wv + 4 * s;
           if (at * 16 >= M) continue;
-          float bv = Tb[0][xr][at * 16 + lane % 16];
-          float bd = Tb[1][xr][at * 16 + lane % 16];
-          float bt = Tb[2][xr][at * 16 + lane % 16];
+          T bv = Tb[0][xr][at * 16 + lane % 16];
+          T bd = Tb[1][xr][at * 16 + lane % 16];
+          T bt = Tb[2][xr][at * 16 + lane % 16];
   #pragma unroll
           for (int t = 0; t < 4; ++t) {
-            aps[s][t] = mfma(f[0][t], bv, aps[s][t]);
-            aps[s][t] = mfma(f[1][t], bd, aps[s][t]);
-            ach[s][t] = mfma(f[2][t], bd, ach[s][t]);
-            ach[s][t] = mfma(f[3][t], bv, ach[s][t]);
-            ach[s][t] = mfma(f[4][t], bt, ach[s][t]);
+            aps[s][t] = MM::op(f[0][t], bv, aps[s][t]);
+            aps[s][t] = MM::op(f[1][t], bd, aps[s][t]);
+            ach[s][t] = MM::op(f[2][t], bd, ach[s][t]);
+            ach[s][t] = MM::op(f[3][t], bv, ach[s][t]);
+            ach[s][t] = MM::op(f[4][t], bt, ach[s][t]);
           }
         }
       }
@@ -530,7 +574,8 @@ __global__ void __launch_bounds__(256) k_v2t_f32(const double2 *__restrict__ ur,
 }
 
 // table re-layouts for the forward and projection kernels: dst[m][p][x][a]
-__global__ void k_tbl_pxa(const double *src, float *dst, int nrh, int rows_per_m, int nrows, int AP, int npchop) {
+template <typename T>
+__global__ void k_tbl_pxa(const double *src, T *dst, int nrh, int rows_per_m, int nrows, int AP, int npchop) {
   size_t n = (size_t)npchop * 2 * nrh * AP;
   for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (size_t)gridDim.x * blockDim.x) {
     int a = (int)(t % AP);
@@ -538,7 +583,7 @@ __global__ void k_tbl_pxa(const double *src, float *dst, int nrh, int rows_per_m
     int p = (int)((t / ((size_t)AP * nrh)) % 2);
     int m = (int)(t / ((size_t)AP * nrh * 2));
     int r = 2 * a + p;
-    dst[t] = (r < nrows) ? (float)src[((size_t)m * rows_per_m + r) * nrh + x] : 0.f;
+    dst[t] = (r < nrows) ? (T)src[((size_t)m * rows_per_m + r) * nrh + x] : (T)0;
   }
 }
 
@@ -565,54 +610,86 @@ void mfma_tables(mlx_ctx *c) {
   int AP = 4 * mfma_tiles(k) * 16;
   size_t n = (size_t)k.npchop * 2 * k.nrh * AP;
   k.mfma_ap = AP;
-  auto alloc = [&](float *&p) {
+  auto alloc = [&](auto *&p) {
     if (p) (void)hipFree(p);
-    if (hipMalloc(&p, sizeof(float) * n) != hipSuccess) MLX_THROW(MLX_E_ALLOC, "kit: table allocation failed");
+    if (hipMalloc(&p, sizeof(*p) * n) != hipSuccess) MLX_THROW(MLX_E_ALLOC, "kit: table allocation failed");
   };
   alloc(k.pfg32);
   alloc(k.tVg32);
   alloc(k.tDg32);
   alloc(k.tTg32);
+  alloc(k.pfg64);
+  alloc(k.tVg64);
+  alloc(k.tDg64);
+  alloc(k.tTg64);
   int nb = (int)((n + 255) / 256 > 8192 ? 8192 : (n + 255) / 256);
-  hipLaunchKernelGGL(k_tbl_pxa, dim3(nb), dim3(256), 0, c->stream, k.pf, k.pfg32, k.nrh, k.ne, k.ne, AP, k.npchop);
-  hipLaunchKernelGGL(k_tbl_pxa, dim3(nb), dim3(256), 0, c->stream, k.tV, k.tVg32, k.nrh, k.nt, k.nt, AP, k.npchop);
-  hipLaunchKernelGGL(k_tbl_pxa, dim3(nb), dim3(256), 0, c->stream, k.tD, k.tDg32, k.nrh, k.nt, k.nt, AP, k.npchop);
-  hipLaunchKernelGGL(k_tbl_pxa, dim3(nb), dim3(256), 0, c->stream, k.tT, k.tTg32, k.nrh, k.nt, k.nt, AP, k.npchop);
+  auto relayout = [&](auto *dst, const double *src, int rows) {
+    hipLaunchKernelGGL(k_tbl_pxa, dim3(nb), dim3(256), 0, c->stream, src, dst, k.nrh, rows, rows, AP, k.npchop);
+  };
+  relayout(k.pfg32, k.pf, k.ne);
+  relayout(k.tVg32, k.tV, k.nt);
+  relayout(k.tDg32, k.tD, k.nt);
+  relayout(k.tTg32, k.tT, k.nt);
+  relayout(k.pfg64, k.pf, k.ne);
+  relayout(k.tVg64, k.tV, k.nt);
+  relayout(k.tDg64, k.tD, k.nt);
+  relayout(k.tTg64, k.tT, k.nt);
   HIPCHK(hipGetLastError());
 }
 
+namespace {
+// the operand type of the MFMA path at the context's current precision
+bool mfma_fp64(const mlx_ctx *c) {
+  if (c->prec == MLX_PREC_FP64_MFMA) return true;
+  if (c->prec == MLX_PREC_FP32) return false;
+  MLX_THROW(MLX_E_ARG, "MFMA contractions: no kernels for this precision");
+}
+}  // namespace
+
 // ncols > 0: columns m >= ncols are written as zero without being read
 // (ztail = false: they get no workgroups and are left as they are)
-void leg_forward_f32(mlx_ctx *c, const double2 *in, double2 *out, int ncols, bool ztail) {
+void leg_forward_mfma(mlx_ctx *c, const double2 *in, double2 *out, int ncols, bool ztail) {
   const KitDev &k = c->kit;
   const int nyt = (k.nzdim + KT - 1) / KT;
   const int npc = (ncols > 0 && ncols < k.npchop) ? ncols : k.npchop;
   dim3 grid = xcd_grid(ztail ? k.npdim : npc, nyt);
-  int at = mfma_tiles(k);
-  ProbeScope probe(c, MLX_PROBE_LEG_FWD, leg_bytes(k, 0, 4, npc), leg_flops(k, 0, npc));
-  if (at == 1)
-    probed_launch(probe, k_leg_fwd_f32<1>, grid, dim3(256), 0, c->stream, in, out, k.pfg32, k.w, k.nrdim, k.npdim,
-                       k.nzdim, k.nr, k.nrh, k.nrchop, npc, nyt);
-  else
-    probed_launch(probe, k_leg_fwd_f32<2>, grid, dim3(256), 0, c->stream, in, out, k.pfg32, k.w, k.nrdim, k.npdim,
-                       k.nzdim, k.nr, k.nrh, k.nrchop, npc, nyt);
+  const int at = mfma_tiles(k);
+  const bool f64 = mfma_fp64(c);
+  ProbeScope probe(c, MLX_PROBE_LEG_FWD, leg_bytes(k, 0, f64 ? 8 : 4, npc), leg_flops(k, 0, npc));
+#define MLX_FWD(T_, AT_, TB_)                                                                                  \
+  probed_launch(probe, k_leg_fwd_mf<T_, AT_>, grid, dim3(256), 0, c->stream, in, out, TB_, k.w, k.nrdim, k.npdim, \
+                k.nzdim, k.nr, k.nrh, k.nrchop, npc, nyt)
+  if (f64) {
+    if (at == 1) MLX_FWD(double, 1, k.pfg64);
+    else MLX_FWD(double, 2, k.pfg64);
+  } else {
+    if (at == 1) MLX_FWD(float, 1, k.pfg32);
+    else MLX_FWD(float, 2, k.pfg32);
+  }
+#undef MLX_FWD
   HIPCHK(hipGetLastError());
 }
 
-void leg_backward_f32(mlx_ctx *c, const double2 *in, double2 *out, int ncols, bool ztail) {
+void leg_backward_mfma(mlx_ctx *c, const double2 *in, double2 *out, int ncols, bool ztail) {
   const KitDev &k = c->kit;
   const int nyt = (k.nzdim + KT - 1) / KT;
   const int npc = (ncols > 0 && ncols < k.npchop) ? ncols : k.npchop;
   // without the zero tail only the computed columns get workgroups
   dim3 grid = xcd_grid(ztail ? k.npdim : npc, nyt);
-  int xt = (k.nrh + 63) / 64;
-  ProbeScope probe(c, MLX_PROBE_LEG_BWD, leg_bytes(k, 0, 4, npc), leg_flops(k, 0, npc));
-  if (xt == 1)
-    probed_launch(probe, k_leg_bwd_f32<1, false>, grid, dim3(256), 0, c->stream, in, out, nullptr, k.pf32, k.nrdim,
-                  k.npdim, k.nzdim, k.nr, k.nrh, k.ne, k.nrchop, npc, nyt, 0);
-  else
-    probed_launch(probe, k_leg_bwd_f32<2, false>, grid, dim3(256), 0, c->stream, in, out, nullptr, k.pf32, k.nrdim,
-                  k.npdim, k.nzdim, k.nr, k.nrh, k.ne, k.nrchop, npc, nyt, 0);
+  const int xt = (k.nrh + 63) / 64;
+  const bool f64 = mfma_fp64(c);
+  ProbeScope probe(c, MLX_PROBE_LEG_BWD, leg_bytes(k, 0, f64 ? 8 : 4, npc), leg_flops(k, 0, npc));
+#define MLX_BWD(T_, XT_, TB_)                                                                                    \
+  probed_launch(probe, k_leg_bwd_mf<T_, XT_, false>, grid, dim3(256), 0, c->stream, in, out, nullptr, TB_, k.nrdim, \
+                k.npdim, k.nzdim, k.nr, k.nrh, k.ne, k.nrchop, npc, nyt, 0)
+  if (f64) {
+    if (xt == 1) MLX_BWD(double, 1, k.pf);
+    else MLX_BWD(double, 2, k.pf);
+  } else {
+    if (xt == 1) MLX_BWD(float, 1, k.pf32);
+    else MLX_BWD(float, 2, k.pf32);
+  }
+#undef MLX_BWD
   HIPCHK(hipGetLastError());
 }
 
@@ -627,19 +704,19 @@ void leg_backward_pairs_f32(mlx_ctx *c, const double2 *in, float4 *pairs, int nc
   const double bytes = leg_bytes(k, 0, 4, npc) - 8.0 * k.nr * npc * k.nzdim;
   ProbeScope probe(c, MLX_PROBE_LEG_BWD, bytes, leg_flops(k, 0, npc));
   if (xt == 1)
-    probed_launch(probe, k_leg_bwd_f32<1, true>, grid, dim3(256), 0, c->stream, in, nullptr, pairs, k.pf32, k.nrdim,
-                  k.npdim, k.nzdim, k.nr, k.nrh, k.ne, k.nrchop, npc, nyt, pair_stride(k));
+    probed_launch(probe, k_leg_bwd_mf<float, 1, true>, grid, dim3(256), 0, c->stream, in, nullptr, pairs, k.pf32,
+                  k.nrdim, k.npdim, k.nzdim, k.nr, k.nrh, k.ne, k.nrchop, npc, nyt, pair_stride(k));
   else
-    probed_launch(probe, k_leg_bwd_f32<2, true>, grid, dim3(256), 0, c->stream, in, nullptr, pairs, k.pf32, k.nrdim,
-                  k.npdim, k.nzdim, k.nr, k.nrh, k.ne, k.nrchop, npc, nyt, pair_stride(k));
+    probed_launch(probe, k_leg_bwd_mf<float, 2, true>, grid, dim3(256), 0, c->stream, in, nullptr, pairs, k.pf32,
+                  k.nrdim, k.npdim, k.nzdim, k.nr, k.nrh, k.ne, k.nrchop, npc, nyt, pair_stride(k));
   HIPCHK(hipGetLastError());
 }
 
 // Every entry of psi and chi is written (zeros outside the computed set).
 // ncols > 0: columns m >= ncols are zero; kappa slices zlo..zhi are zero
 // (zlo > zhi: none), for a caller that de-aliases the result.
-void vec2tp_project_f32(mlx_ctx *c, const double2 *ur, const double2 *up, const double2 *uz, double2 *psi,
-                        double2 *chi, int ncols, int zlo, int zhi) {
+void vec2tp_project_mfma(mlx_ctx *c, const double2 *ur, const double2 *up, const double2 *uz, double2 *psi,
+                         double2 *chi, int ncols, int zlo, int zhi) {
   const KitDev &k = c->kit;
   const int nyt = (k.nzdim + KT - 1) / KT;
   const int npc = (ncols > 0 && ncols < k.npchop) ? ncols : k.npchop;
@@ -647,15 +724,21 @@ void vec2tp_project_f32(mlx_ctx *c, const double2 *ur, const double2 *up, const 
   for (int kk = 0; kk < k.nzdim; ++kk)
     if (kk >= zlo && kk <= zhi) --nzk;
   dim3 grid = xcd_grid(k.npdim, 2 * nyt);
-  int at = mfma_tiles(k);
-  ProbeScope probe(c, MLX_PROBE_V2T, v2t_bytes(k, 0, 4, npc, nzk, true),
+  const int at = mfma_tiles(k);
+  const bool f64 = mfma_fp64(c);
+  ProbeScope probe(c, MLX_PROBE_V2T, v2t_bytes(k, 0, f64 ? 8 : 4, npc, nzk, true),
                    5.0 * leg_flops(k, 0, npc) * nzk / (double)k.nzdim);
-  if (at == 1)
-    probed_launch(probe, k_v2t_f32<1>, grid, dim3(256), 0, c->stream, ur, up, uz, psi, chi, k.tVg32, k.tDg32, k.tTg32,
-                       k.ak, k.nrdim, k.npdim, k.nzdim, k.nr, k.nrh, k.nrchop, npc, k.chopzl, k.chopzu, nyt, zlo, zhi);
-  else
-    probed_launch(probe, k_v2t_f32<2>, grid, dim3(256), 0, c->stream, ur, up, uz, psi, chi, k.tVg32, k.tDg32, k.tTg32,
-                       k.ak, k.nrdim, k.npdim, k.nzdim, k.nr, k.nrh, k.nrchop, npc, k.chopzl, k.chopzu, nyt, zlo, zhi);
+#define MLX_V2T(T_, AT_, V_, D_, T3_)                                                                            \
+  probed_launch(probe, k_v2t_mf<T_, AT_>, grid, dim3(256), 0, c->stream, ur, up, uz, psi, chi, V_, D_, T3_, k.ak,   \
+                k.nrdim, k.npdim, k.nzdim, k.nr, k.nrh, k.nrchop, npc, k.chopzl, k.chopzu, nyt, zlo, zhi)
+  if (f64) {
+    if (at == 1) MLX_V2T(double, 1, k.tVg64, k.tDg64, k.tTg64);
+    else MLX_V2T(double, 2, k.tVg64, k.tDg64, k.tTg64);
+  } else {
+    if (at == 1) MLX_V2T(float, 1, k.tVg32, k.tDg32, k.tTg32);
+    else MLX_V2T(float, 2, k.tVg32, k.tDg32, k.tTg32);
+  }
+#undef MLX_V2T
   HIPCHK(hipGetLastError());
 }
 

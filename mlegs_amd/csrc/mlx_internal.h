@@ -64,6 +64,8 @@ struct KitDev {
   float *tV32 = nullptr, *tD32 = nullptr, *tT32 = nullptr;
   // MFMA-friendly FP32 tables [m][parity][x][a] (a padded to mfma_ap)
   float *pfg32 = nullptr, *tVg32 = nullptr, *tDg32 = nullptr, *tTg32 = nullptr;
+  // the same re-layouts in FP64 for the FP64 MFMA kernels
+  double *pfg64 = nullptr, *tVg64 = nullptr, *tDg64 = nullptr, *tTg64 = nullptr;
   int mfma_ap = 0;
   double2 *tw_nph = nullptr, *tw_nz = nullptr, *tw_half = nullptr;
   FftPlan plan_nph, plan_nz;
@@ -125,7 +127,10 @@ struct mlx_ctx {
   hipStream_t stream = nullptr;
   bool own_stream = false;
   KitDev kit;
-  int prec = MLX_PREC_FP64;
+  int prec = MLX_PREC_FP64;  // precision of the contraction being issued
+  int prec_ctx = MLX_PREC_FP64;  // the context's precision (mlx_set_precision)
+  // per contraction site (MLX_STAGE_*): -1 follows prec_ctx
+  int stage_prec[MLX_STAGE_COUNT] = {-1, -1, -1};
   std::string err;
   size_t nelem = 0;
   // scratch buffers of one field each, and small device scratch
@@ -137,6 +142,19 @@ struct mlx_ctx {
   std::vector<LuCache> lu;
   uint64_t lu_clock = 0;
   ProbeSet probe;
+};
+
+// The precision of a contraction site: its own setting, else the context's.
+static inline int stage_precision(const mlx_ctx *c, int stage) {
+  return c->stage_prec[stage] >= 0 ? c->stage_prec[stage] : c->prec_ctx;
+}
+// Issues the contractions of one site at its precision (c->prec) and
+// restores the previous precision afterwards.
+struct PrecScope {
+  mlx_ctx *c;
+  int saved;
+  PrecScope(mlx_ctx *c_, int stage) : c(c_), saved(c_->prec) { c->prec = stage_precision(c, stage); }
+  ~PrecScope() { c->prec = saved; }
 };
 
 // Records the opening event on construction and the closing one on
@@ -237,20 +255,19 @@ void leg_forward(mlx_ctx *c, const double2 *in, double2 *out, int nroff, bool ze
                  bool ztail = true);
 void leg_backward(mlx_ctx *c, const double2 *in, double2 *out, int nroff, bool zero = true, int ncols = -1,
                   bool ztail = true);
-// legendre_mfma.hip (FP32 MFMA; valid when mfma_ok)
+// legendre_mfma.hip (MFMA in FP32 or FP64 by c->prec; valid when mfma_ok)
 bool mfma_ok(const KitDev &k, int nroff);
 void mfma_tables(mlx_ctx *c);
-void leg_forward_f32(mlx_ctx *c, const double2 *in, double2 *out, int ncols = -1, bool ztail = true);
+void leg_forward_mfma(mlx_ctx *c, const double2 *in, double2 *out, int ncols = -1, bool ztail = true);
 // ztail = false: columns m >= ncols are left unwritten
-void leg_backward_f32(mlx_ctx *c, const double2 *in, double2 *out, int ncols = -1, bool ztail = true);
+void leg_backward_mfma(mlx_ctx *c, const double2 *in, double2 *out, int ncols = -1, bool ztail = true);
 // the FP32 even/odd sums (be, bo) of columns m < ncols as float4 radial pairs
 // (layout at k_leg_bwd_f32), for fft_phi_pairs; a line of pairs (one z slice
 // and column) takes pair_stride float4 (256-byte aligned lines)
 inline int pair_stride(const KitDev &k) { return (k.nrh + 15) / 16 * 16; }
 void leg_backward_pairs_f32(mlx_ctx *c, const double2 *in, float4 *pairs, int ncols);
-void vec2tp_project_f32(mlx_ctx *c, const double2 *ur, const double2 *up, const double2 *uz, double2 *psi,
-                        double2 *chi,
-                        int ncols = -1, int zlo = 1, int zhi = 0);
+void vec2tp_project_mfma(mlx_ctx *c, const double2 *ur, const double2 *up, const double2 *uz, double2 *psi,
+                         double2 *chi, int ncols = -1, int zlo = 1, int zhi = 0);
 void vec2tp_project(mlx_ctx *c, const double2 *ur, const double2 *up, const double2 *uz,
                     double2 *psi, double2 *chi, int nroff, int ncols = -1,
                     int zlo = 1, int zhi = 0);
@@ -260,8 +277,10 @@ void k_ln_column(mlx_ctx *c, double2 *e, const double *ln, double sign);
 void k_abcn_rhs_g(mlx_ctx *c, double2 *sh, double *shln, const double2 *s, const double *sln, const double2 *nl,
                   const double *nlln, const double2 *nlp, const double *nlpln, const double2 *svis, double dt,
                   double a, int ab2, double g);
-// ihelmp's banded solves (power 4, 6 or 8) and its log-term step, FP64
-void k_ihelmp(mlx_ctx *c, double2 *e, double *sln, int power, double alpha, double beta, int nroff);
+// ihelmp's banded solves (power 4, 6 or 8) and its log-term step, FP64; a
+// column whose factorization meets a zero pivot is left unsolved and sets
+// *sing (device int) to 1
+void k_ihelmp(mlx_ctx *c, double2 *e, double *sln, int power, double alpha, double beta, int nroff, int *sing);
 void k_chop(mlx_ctx *c, double2 *e, const char *space, int nroff);
 void k_dealias(mlx_ctx *c, double2 *e, const char *space);
 // svv_filter on e0 (gain g0) and, when e1 is given, on e1 (gain g1) in the

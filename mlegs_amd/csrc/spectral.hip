@@ -882,19 +882,19 @@ template <int P>
 __global__ void __launch_bounds__(256) k_ihelmp_(double2 *e, double *sln, const double *d2c, const double *ak,
                                                  const double *lognorm, double alpha, double beta, int nrdim,
                                                  int npdim, int ne, int nrc, int npc, int nzc, int nzcu,
-                                                 double ell) {
+                                                 double ell, int *sing) {
   constexpr int Q = P / 2, KV = 2 * P, LDAB = 3 * P + 1, W = 2 * P + 1;
   extern __shared__ double hsm[];
   const int m = blockIdx.x, k = blockIdx.y, tid = threadIdx.x;
   const int cnt = z_count(k, nzc, nzcu);
   const int nn = nrcs_dev(nrc, npc, m);
   if (cnt == 0 || nn <= 0) return;
-  double *AB = hsm;                                            // LDAB * nn
-  double2 *b = reinterpret_cast<double2 *>(AB + (size_t)LDAB * nn);  // nn
+  double *AB = hsm;  // LDAB * nn, rounded up to an even count: b is 16-byte aligned
+  double2 *b = reinterpret_cast<double2 *>(AB + (((size_t)LDAB * nn + 1) & ~(size_t)1));  // nn
   double *v = reinterpret_cast<double *>(b + nn);              // nn
   double *t = v + nn;                                          // nn
   int *piv = reinterpret_cast<int *>(t + nn);                  // nn
-  __shared__ int s_jp, s_ju;
+  __shared__ int s_jp, s_ju, s_sing;
   const double akk = ak[k] * ak[k];
   const double *dm = d2c + (size_t)m * ne * 5;
   auto dcoef = [&](int r, int d) { return dm[(size_t)r * 5 + d] - (d == 2 ? akk : 0.0); };
@@ -972,7 +972,6 @@ __global__ void __launch_bounds__(256) k_ihelmp_(double2 *e, double *sln, const 
   if (tid == 0) s_ju = 0;
   __syncthreads();
   // gbtf2
-  bool ok = true;
   for (int j = 0; j < nn; ++j) {
     const int km = (P < nn - 1 - j) ? P : nn - 1 - j;
     double *cj = AB + (size_t)j * LDAB;
@@ -983,16 +982,19 @@ __global__ void __launch_bounds__(256) k_ihelmp_(double2 *e, double *sln, const 
         if (fabs(cj[KV + q]) > best) { best = fabs(cj[KV + q]); jp = q; }
       piv[j] = j + jp;
       s_jp = jp;
-      if (cj[KV + jp] != 0.0) {
+      // the pivot test is made once, here, and read by every thread from
+      // LDS: after the barrier tid 0 may already be swapping cj[KV + jp]
+      s_sing = (cj[KV + jp] == 0.0);
+      if (!s_sing) {
         const int cand = (j + P + jp < nn - 1) ? j + P + jp : nn - 1;
         if (cand > s_ju) s_ju = cand;
       }
     }
     __syncthreads();
     const int jp = s_jp, ju = s_ju;
-    if (cj[KV + jp] == 0.0) {
-      ok = false;
-      continue;
+    if (s_sing) {  // uniform over the workgroup: every thread leaves here
+      if (tid == 0) atomicOr(sing, 1);
+      return;  // singular: left unsolved and reported (the reference stops)
     }
     if (jp != 0)
       for (int cc = j + tid; cc <= ju; cc += blockDim.x) {
@@ -1013,7 +1015,6 @@ __global__ void __launch_bounds__(256) k_ihelmp_(double2 *e, double *sln, const 
     }
     __syncthreads();
   }
-  if (!ok) return;  // singular: left unsolved (the reference stops)
   // gbtrs, once per visit of the slice
   for (int rep = 0; rep < cnt; ++rep) {
     for (int j = 0; j < nn - 1; ++j) {
@@ -1248,17 +1249,19 @@ void k_idel2_preln00(mlx_ctx *c, double2 *e, double *ln, double preln, int nroff
                      k.ell, work);
 }
 
-void k_ihelmp(mlx_ctx *c, double2 *e, double *sln, int power, double alpha, double beta, int nroff) {
+void k_ihelmp(mlx_ctx *c, double2 *e, double *sln, int power, double alpha, double beta, int nroff, int *sing) {
   const KitDev &k = c->kit;
   const int nrc = k.nrchop + nroff;
-  if (nrc > k.ne) MLX_THROW(MLX_E_ARG, "ihelmp: chopping in r too large");
+  // chop_index's check (mlegs_scalar_ops.f90:2053): the rows of a system lie
+  // inside the field's column
+  if (nrc > k.nrdim_ref || nrc > k.ne) MLX_THROW(MLX_E_ARG, "chop_index: chopping in r too large");
   const size_t lds = (size_t)nrc * ((3 * power + 1) * sizeof(double) + sizeof(double2) + 2 * sizeof(double) +
-                                    sizeof(int));
+                                    sizeof(int)) + sizeof(double);
   if (lds > 65536) MLX_THROW(MLX_E_ARG, "ihelmp: the banded system of one column exceeds the workgroup's LDS");
   dim3 grid(k.npchop, k.nzdim);
 #define MLX_IHP(P_)                                                                                             \
   hipLaunchKernelGGL(k_ihelmp_<P_>, grid, dim3(256), lds, c->stream, e, sln, k.d2c, k.ak, k.lognorm, alpha, beta, \
-                     k.nrdim, k.npdim, k.ne, nrc, k.npchop, k.chopzl, k.chopzu, k.ell)
+                     k.nrdim, k.npdim, k.ne, nrc, k.npchop, k.chopzl, k.chopzu, k.ell, sing)
   switch (power) {
     case 4: MLX_IHP(4); break;
     case 6: MLX_IHP(6); break;

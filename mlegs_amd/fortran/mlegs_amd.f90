@@ -33,6 +33,9 @@ module mlegs_amd
   real(p8), parameter, public :: pi = 3.14159265358979323846264338327950288_p8
 
   integer(c_int), parameter, public :: MLX_PREC_FP64 = 0, MLX_PREC_FP32 = 1, MLX_PREC_BF16X3 = 2
+  integer(c_int), parameter, public :: MLX_PREC_FP64_MFMA = 3
+  !> contraction sites with their own precision (kit%set_stage_precision)
+  integer(c_int), parameter, public :: MLX_STAGE_SYNTH = 0, MLX_STAGE_UP = 1, MLX_STAGE_PROJ = 2
 
   !> run parameters (mlegs_base.f90:7-53) and their defaults
   integer(i4), public :: nr = 256, np = 128, nz = 128
@@ -76,6 +79,7 @@ module mlegs_amd
     procedure :: init => kit_init
     procedure :: dealloc => kit_dealloc
     procedure :: set_precision => kit_set_precision
+    procedure :: set_stage_precision => kit_set_stage_precision
     procedure :: sync => kit_sync
   end type
   type(tfm_kit_3d), public, target :: tfm
@@ -222,6 +226,11 @@ module mlegs_amd
       import :: c_int, c_ptr
       type(c_ptr), value :: ctx
       integer(c_int), value :: prec
+    end function
+    integer(c_int) function mlx_set_stage_precision(ctx, stage, prec) bind(C, name='mlx_set_stage_precision')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: stage, prec
     end function
     integer(c_int) function mlx_field_create(ctx, out) bind(C, name='mlx_field_create')
       import :: c_int, c_ptr
@@ -610,6 +619,15 @@ contains
     integer(i4), intent(in) :: prec
     call need_set(this, 'set_precision')
     call check(this%ctx, mlx_set_precision(this%ctx, int(prec, c_int)), 'set_precision')
+  end subroutine
+
+  !> precision of one contraction site (MLX_STAGE_*); prec = -1 follows set_precision
+  subroutine kit_set_stage_precision(this, stage, prec)
+    class(tfm_kit_3d), intent(inout) :: this
+    integer(i4), intent(in) :: stage, prec
+    call need_set(this, 'set_stage_precision')
+    call check(this%ctx, mlx_set_stage_precision(this%ctx, int(stage, c_int), int(prec, c_int)), &
+               'set_stage_precision')
   end subroutine
 
   subroutine kit_sync(this)

@@ -50,6 +50,22 @@ def vorticity_magnitude(psi, chi, kit):
     return out
 
 
+def energy(vr, vp, vz, kit):
+    """Kinetic energy 1/2 int |u|^2 r dr dphi dz of a PPP velocity by physical
+    quadrature: r dr = L^2 dx / (1 - x)^2 on the Gauss-Legendre nodes, the
+    packed phi points (real and imaginary parts) at 2 pi / np, the z points
+    at zlen / nz.  No counterpart in the reference (a diagnostic it lacks,
+    docs/release_notes_v1.1.x.md:48-50)."""
+    for s in (vr, vp, vz):
+        if s.space != 'PPP':
+            raise ValueError('energy: velocity components must be in PPP')
+    nr, nph = kit.nr, kit.np // 2
+    wq = kit.w * kit.ell ** 2 / (1.0 - kit.x) ** 2
+    q = sum(np.abs(s.e[:nr, :nph, :]) ** 2 for s in (vr, vp, vz))
+    cell = (2.0 * np.pi / kit.np) * (kit.zlen / kit.nz)
+    return 0.5 * cell * float(np.einsum('i,ijk->', wq, q))
+
+
 def check_stability(psi, chi):
     """Non-finite check (:397-409); raises instead of MPI_abort(205)."""
     if not (np.all(np.isfinite(psi.e)) and np.all(np.isfinite(chi.e))):

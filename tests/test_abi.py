@@ -54,8 +54,10 @@ def test_ctypes_table_matches_header():
     d = _defines()
     for name in ('MLX_OK', 'MLX_E_LAYOUT', 'MLX_E_ALLOC', 'MLX_E_ARG', 'MLX_E_FILE', 'MLX_E_NONFINITE', 'MLX_E_HIP'):
         assert getattr(_lib, name) == d[name]
-    assert (_lib.PREC_FP64, _lib.PREC_FP32, _lib.PREC_BF16X3) == (d['MLX_PREC_FP64'], d['MLX_PREC_FP32'],
-                                                                    d['MLX_PREC_BF16X3'])
+    assert (_lib.PREC_FP64, _lib.PREC_FP32, _lib.PREC_BF16X3, _lib.PREC_FP64_MFMA) == (
+        d['MLX_PREC_FP64'], d['MLX_PREC_FP32'], d['MLX_PREC_BF16X3'], d['MLX_PREC_FP64_MFMA'])
+    assert (_lib.STAGE_SYNTH, _lib.STAGE_UP, _lib.STAGE_PROJ) == (d['MLX_STAGE_SYNTH'], d['MLX_STAGE_UP'],
+                                                                   d['MLX_STAGE_PROJ'])
 
 
 def test_params_struct_layout(tmp_path):

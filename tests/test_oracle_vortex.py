@@ -220,3 +220,24 @@ def test_steppers_reject_inviscid_and_physical_input():
         so.febe(s, nl, 1e-3, kit, p.replace(visc=0.0))
     with pytest.raises(ValueError):
         so.abcn(s, s.copy(), so.Scalar(kit, 'PPP'), nl.copy(), 1e-3, kit, p)
+
+
+def test_energy_closed_forms():
+    """1/2 int |u|^2 over the domain: u_r = e^{-r^2} gives
+    1/2 * 1/4 * 2 pi * zlen (int_0^inf e^{-2 r^2} r dr = 1/4); the q-vortex
+    axial flow e^{-r^2}/q gives the same over q^2; a cos(phi) cos(z)
+    modulation halves it twice."""
+    kit, p = _kit(nr=64, nrchop=64, np=8, npchop=5, nz=4, nzchop=3, zlen=3.0)
+    zero = so.Scalar(kit, 'PPP')
+    u = so.Scalar(kit, 'PPP')
+    u.e = _sample(kit, lambda r, ph, z: np.exp(-r ** 2) + 0 * ph + 0 * z)
+    want = 0.5 * 0.25 * 2 * math.pi * 3.0
+    assert abs(V.energy(u, zero, zero, kit) - want) < 1e-13 * want
+    uz = so.Scalar(kit, 'PPP')
+    uz.e = _sample(kit, lambda r, ph, z: np.exp(-r ** 2) / Q + 0 * ph + 0 * z)
+    assert abs(V.energy(zero, zero, uz, kit) - want / Q ** 2) < 1e-13 * want
+    m = so.Scalar(kit, 'PPP')
+    m.e = _sample(kit, lambda r, ph, z: np.exp(-r ** 2) * np.cos(ph) * np.cos(2 * math.pi * z / 3.0))
+    assert abs(V.energy(zero, m, zero, kit) - want / 4) < 1e-13 * want
+    with pytest.raises(ValueError):
+        V.energy(so.Scalar(kit, 'FFF'), zero, zero, kit)
