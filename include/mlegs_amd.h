@@ -169,6 +169,15 @@ int mlx_dealias(mlx_ctx* ctx, mlx_field* f);
 /* svv_filter(s, tfm, gain) (mlegs_scalar.f90:136-144; mlegs_scalar_ops.f90:72-155);
  * *gain is read and updated. */
 int mlx_svv_filter(mlx_ctx* ctx, mlx_field* f, double* gain);
+/* svv_filter without device allocation: the same filter and gain update as
+ * mlx_svv_filter, with the reduction's partial sums, results and gain in a
+ * static device array of 8200 doubles (64 KiB) that each GPU holds until
+ * the process ends, instead of buffers of the context.  No counterpart in
+ * the reference; it is for callers that cannot allocate device memory
+ * (code under graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Limit: nz <= 4096 (MLX_E_ARG beyond).  Calls that overlap on
+ * one GPU (other threads or streams) share the array and race. */
+int mlx_svv_filter_noalloc(mlx_ctx* ctx, mlx_field* f, double* gain);
 /* calcat0 / calcat1: values at r=0 / r=inf of m=0 per z slice; out holds
  * 2*nz doubles (mlegs_scalar_ops.f90:237-309). */
 int mlx_calcat0(mlx_ctx* ctx, const mlx_field* f, double* out);

@@ -69,6 +69,7 @@ SIGNATURES = {
     'mlx_chop': ([_P, _P], ctypes.c_int),
     'mlx_dealias': ([_P, _P], ctypes.c_int),
     'mlx_svv_filter': ([_P, _P, _DP], ctypes.c_int),
+    'mlx_svv_filter_noalloc': ([_P, _P, _DP], ctypes.c_int),
     'mlx_calcat0': ([_P, _P, _DP], ctypes.c_int),
     'mlx_calcat1': ([_P, _P, _DP], ctypes.c_int),
     'mlx_zeroat1': ([_P, _P], ctypes.c_int),

@@ -646,6 +646,9 @@ int mlx_ctx_create(const mlx_params *p, int device, void *stream, mlx_ctx **out)
   c->device = device;
   int rc = guarded(c, [&] {
     HIPCHK(hipSetDevice(device));
+    // this device's address of the static scratch of the no-allocation
+    // entry point (looked up once per device)
+    noalloc_setup(device);
     if (stream) {
       c->stream = reinterpret_cast<hipStream_t>(stream);
     } else {

@@ -285,8 +285,15 @@ void k_chop(mlx_ctx *c, double2 *e, const char *space, int nroff);
 void k_dealias(mlx_ctx *c, double2 *e, const char *space);
 // svv_filter on e0 (gain g0) and, when e1 is given, on e1 (gain g1) in the
 // same two launches
+// part_ws, out_ws (optional): the partial sums (2 per workgroup and field,
+// at most kSvvParts workgroups per field) and the per-field results (3 per
+// field) in memory the caller provides instead of the context's
+constexpr int kSvvParts = 4096;
 void k_svv(mlx_ctx *c, double2 *e0, double *g0, double2 *e1 = nullptr, double *g1 = nullptr, int *flag = nullptr,
-           int pcut = -1, int zlo = 1, int zhi = 0);
+           int pcut = -1, int zlo = 1, int zhi = 0, double *part_ws = nullptr, double *out_ws = nullptr);
+// noalloc.hip: looks up this device's static scratch array (once per device;
+// the device must be current)
+void noalloc_setup(int device);
 void k_calcat(mlx_ctx *c, const double2 *e, const double *tbl, int nrc, double2 *out);
 // zeroat1 on e (rows < nrc) and, when e2 is given, on e2 in the same launch;
 // lnz / lnz2 (optional): log coefficients set to zero by that launch

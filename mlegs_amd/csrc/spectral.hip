@@ -1157,7 +1157,8 @@ void k_dealias(mlx_ctx *c, double2 *e, const char *space) {
                      pcut, zcut, zupper);
 }
 
-void k_svv(mlx_ctx *c, double2 *e0, double *g0, double2 *e1, double *g1, int *flag, int pcut, int zlo, int zhi) {
+void k_svv(mlx_ctx *c, double2 *e0, double *g0, double2 *e1, double *g1, int *flag, int pcut, int zlo, int zhi,
+           double *part_ws, double *out_ws) {
   const KitDev &k = c->kit;
   const mlx_params &p = c->p;
   double inv_r = 1.0 / fmax((double)(k.nrchop - 1), 1.0);
@@ -1172,14 +1173,14 @@ void k_svv(mlx_ctx *c, double2 *e0, double *g0, double2 *e1, double *g1, int *fl
   SvvFields F{{e0, e1 ? e1 : e0}, {g0, g1 ? g1 : g0}};
   // workgroups per slice, each over a range of about eight columns (a few
   // loads per thread: the pass is latency-bound otherwise); partial sums of
-  // at most 4096 workgroups per field (c->dpart)
+  // at most kSvvParts workgroups per field (c->dpart or part_ws)
   const int ncol = (pcut > 0 && pcut < k.npdim) ? pcut : k.npdim;
   int nbx = (ncol + 7) / 8;
-  if (nbx * k.nzdim > 4096) nbx = 4096 / k.nzdim;
+  if (nbx * k.nzdim > kSvvParts) nbx = kSvvParts / k.nzdim;
   if (nbx < 1) nbx = 1;
-  if ((size_t)nbx * k.nzdim > 4096) MLX_THROW(MLX_E_ARG, "svv_filter: nz above 4096 is not supported");
-  double *part = c->dpart;
-  double *out = c->dsmall + 2048;
+  if ((size_t)nbx * k.nzdim > kSvvParts) MLX_THROW(MLX_E_ARG, "svv_filter: nz above 4096 is not supported");
+  double *part = part_ws ? part_ws : c->dpart;
+  double *out = out_ws ? out_ws : c->dsmall + 2048;
   hipLaunchKernelGGL(k_svv_reduce, dim3(nbx, k.nzdim, nf), dim3(kT), 0, c->stream, F, k.nrdim, k.npdim, inv_r,
                      inv_p, k.ak, kmax, cutoff, part, flag, ncol, zlo, zhi);
   hipLaunchKernelGGL(k_svv_gain, dim3(nf), dim3(1024), 0, c->stream, F, part, nbx * k.nzdim, out, target, relax,
