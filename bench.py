@@ -9,7 +9,9 @@ SplitMix64/Box-Muller).  The FE-BE + Richardson bootstrap runs untimed, then
 --warmup steps, then --steps timed steps between device synchronisations.
 
 Precisions (--prec) of the Legendre contractions; everything else is FP64:
-  mixed  (default) FP32 MFMA, except vec2tp's projection in FP64 MFMA
+  mixed  (default) FP32 MFMA, except the m = 0 column of vec2tp's projection
+         in FP64 MFMA (on a second stream, concurrent with the rest)
+  mixedp FP32 MFMA, except vec2tp's whole projection in FP64 MFMA
   fp32   FP32 MFMA everywhere
   fp64m  FP64 MFMA everywhere
   fp64   the plain FP64 path: FP64 vector-FMA contractions over every column,
@@ -91,19 +93,23 @@ def rel(a, b):
     return float(np.linalg.norm((a - b).ravel()) / np.linalg.norm(b.ravel()))
 
 
-PRESETS = ('mixed', 'fp32', 'fp64m', 'fp64')
-DTYPES = {'fp64': 'f64', 'fp64m': 'f64-mfma', 'fp32': 'f32-mfma+f64', 'mixed': 'f32-mfma+f64-mfma'}
+PRESETS = ('mixed', 'mixedp', 'fp32', 'fp64m', 'fp64')
+DTYPES = {'fp64': 'f64', 'fp64m': 'f64-mfma', 'fp32': 'f32-mfma+f64', 'mixed': 'f32-mfma+f64-mfma',
+          'mixedp': 'f32-mfma+f64-mfma'}
 GATE_TOL = 1e-6
 
 
 def configure(tfm, name):
     """Contraction precisions of a preset (context precision + site overrides)."""
     import mlegs_amd as M
-    base = {'fp64': M.PREC_FP64, 'fp32': M.PREC_FP32, 'fp64m': M.PREC_FP64_MFMA, 'mixed': M.PREC_FP32}[name]
+    base = {'fp64': M.PREC_FP64, 'fp32': M.PREC_FP32, 'fp64m': M.PREC_FP64_MFMA, 'mixed': M.PREC_FP32,
+            'mixedp': M.PREC_FP32}[name]
     tfm.set_precision(base)
-    for st in (M.STAGE_SYNTH, M.STAGE_UP, M.STAGE_PROJ):
+    for st in (M.STAGE_SYNTH, M.STAGE_UP, M.STAGE_PROJ, M.STAGE_PROJ0):
         tfm.set_stage_precision(st, -1)
-    if name == 'mixed':
+    if name in ('mixed', 'mixedp'):
+        tfm.set_stage_precision(M.STAGE_PROJ0, M.PREC_FP64_MFMA)
+    if name == 'mixedp':
         tfm.set_stage_precision(M.STAGE_PROJ, M.PREC_FP64_MFMA)
 
 
@@ -209,7 +215,7 @@ def transform_error(tfm, p, name):
 def family_precision(name, family):
     if name in ('fp64', 'fp64m'):
         return 'fp64'
-    if name == 'mixed' and family == 'v2t':
+    if name == 'mixedp' and family == 'v2t':
         return 'fp64'
     return 'fp32' if family in ('leg_fwd', 'leg_bwd', 'v2t') else 'fp64'
 

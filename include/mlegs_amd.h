@@ -62,8 +62,16 @@ extern "C" {
                                 tp2curlvec (the advection term's six)      */
 #define MLX_STAGE_UP    1    /* vec2tp's radial transforms of r u_phi (to
                                 FFF for its value at r = inf, and back)    */
-#define MLX_STAGE_PROJ  2    /* vec2tp's five projection contractions      */
-#define MLX_STAGE_COUNT 3
+#define MLX_STAGE_PROJ  2    /* vec2tp's five projection contractions,
+                                columns m >= 1                              */
+#define MLX_STAGE_PROJ0 3    /* the same for column m = 0, where idel2's
+                                kappa = 0 Laplacian systems amplify the
+                                contraction's rounding; a setting that
+                                differs from MLX_STAGE_PROJ's runs the column
+                                concurrently on a second stream (an MFMA
+                                precision: plain FP64 there means FP64 MFMA
+                                while the other columns run on MFMA)        */
+#define MLX_STAGE_COUNT 4
 
 /* kernel probes (measurement): see mlx_probe_start */
 #define MLX_PROBE_FFT_PHI   0  /* phi FFT, one launch per direction       */

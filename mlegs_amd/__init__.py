@@ -15,7 +15,7 @@ import numpy as np
 
 from . import _lib
 from ._lib import MlxError, PREC_FP64, PREC_FP32, PREC_BF16X3, PREC_FP64_MFMA  # noqa: F401
-from ._lib import STAGE_SYNTH, STAGE_UP, STAGE_PROJ  # noqa: F401
+from ._lib import STAGE_SYNTH, STAGE_UP, STAGE_PROJ, STAGE_PROJ0  # noqa: F401
 from .params import Params, read_input, timestep_set, InputError  # noqa: F401
 
 __all__ = ['Kit', 'Scalar', 'trans', 'chop', 'dealias', 'svv_filter', 'calcat0', 'calcat1', 'zeroat1',

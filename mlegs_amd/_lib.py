@@ -18,7 +18,7 @@ MLX_E_NONFINITE = 205
 MLX_E_HIP = 206
 
 PREC_FP64, PREC_FP32, PREC_BF16X3, PREC_FP64_MFMA = 0, 1, 2, 3
-STAGE_SYNTH, STAGE_UP, STAGE_PROJ = 0, 1, 2
+STAGE_SYNTH, STAGE_UP, STAGE_PROJ, STAGE_PROJ0 = 0, 1, 2, 3
 
 
 class MlxParams(ctypes.Structure):

@@ -499,8 +499,9 @@ void vec2tp_pfp(mlx_ctx *c, mlx_field *ur, mlx_field *up, mlx_field *uz, mlx_fie
   k_vec2tp_up_ln(c, up->e, inf, psi->sc);
   if (k.nz > 1) fft_z(c, uz->e, -1, (double)k.nr * nc, 0, nc, 1, 0, zlo, zhi);
   {
+    const int prec_m0 = stage_precision(c, MLX_STAGE_PROJ0);
     PrecScope ps_proj(c, MLX_STAGE_PROJ);
-    vec2tp_project(c, ur->e, up->e, uz->e, psi->e, chi->e, 3, nc, zlo, zhi);
+    vec2tp_project(c, ur->e, up->e, uz->e, psi->e, chi->e, 3, nc, zlo, zhi, prec_m0);
   }
   set_space(psi, "FFF");
   set_space(chi, "FFF");
@@ -680,6 +681,10 @@ int mlx_ctx_destroy(mlx_ctx *c) {
     for (hipEvent_t e : v) (void)hipEventDestroy(e);
   for (auto &kv : c->work)
     if (kv.second.d) (void)hipFree(kv.second.d);
+  if (c->aux) (void)hipStreamSynchronize(c->aux);
+  if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
+  if (c->ev_join) (void)hipEventDestroy(c->ev_join);
+  if (c->aux) (void)hipStreamDestroy(c->aux);
   for (double2 *s : c->scr)
     if (s) (void)hipFree(s);
   kit_free(c->kit);

@@ -353,10 +353,10 @@ void leg_backward(mlx_ctx *c, const double2 *in, double2 *out, int nroff, bool z
 // ncols, zlo, zhi: as vec2tp_project_f32 on the MFMA path; the FP64 path
 // computes every column and slice
 void vec2tp_project(mlx_ctx *c, const double2 *ur, const double2 *up, const double2 *uz, double2 *psi,
-                    double2 *chi, int nroff, int ncols, int zlo, int zhi) {
+                    double2 *chi, int nroff, int ncols, int zlo, int zhi, int prec_m0) {
   const KitDev &k = c->kit;
   if (c->prec != MLX_PREC_FP64 && mfma_ok(k, 0)) {
-    vec2tp_project_mfma(c, ur, up, uz, psi, chi, ncols, zlo, zhi);
+    vec2tp_project_mfma(c, ur, up, uz, psi, chi, ncols, zlo, zhi, prec_m0);
     return;
   }
   HIPCHK(hipMemsetAsync(psi, 0, sizeof(double2) * c->nelem, c->stream));

@@ -407,7 +407,9 @@ __global__ void __launch_bounds__(256) k_v2t_mf(const double2 *__restrict__ ur, 
                                                 const T *__restrict__ tgD, const T *__restrict__ tgT,
                                                  const double *__restrict__ ak, int nrdim, int npdim, int nz, int nr,
                                                  int nrh, int nrc0, int npc, int nzc, int nzcu, int nyt, int zlo,
-                                                 int zhi) {
+                                                 int zhi, int mbeg, int mend, int asplit, int tpitch) {
+  // AP: the table columns (radial rows a of one parity) this workgroup
+  // stages; tpitch: the tables' row pitch (4 16 tiles of the widest kernel)
   constexpr int AP = 4 * AT * 16;
   constexpr int APP = AP + 16;
   constexpr int KV = 8;   // contraction chunk for this kernel
@@ -419,10 +421,14 @@ __global__ void __launch_bounds__(256) k_v2t_mf(const double2 *__restrict__ ur, 
   // both parities and every kappa tile of a column m on one XCD: the two
   // parity workgroups read the same input columns and write interleaved
   // rows of the same output lines, which then meet in that XCD's L2
+  // this launch's columns: mbeg .. mend-1.  asplit workgroups share the
+  // rows of one (m, parity, kappa tile): workgroup sp takes the 16-row tiles
+  // wv + 4 (sp AT + s), s < AT, i.e. table columns acol0 .. acol0 + AP - 1
   int m, y2;
-  xcd_map(blockIdx.x, 2 * nyt, m, y2);
-  const int yt = y2 >> 1, p = y2 & 1;
-  if (m >= npdim) return;
+  xcd_map(blockIdx.x, 2 * nyt * asplit, m, y2);
+  m += mbeg;
+  const int p = y2 & 1, yt = (y2 >> 1) % nyt, sp = (y2 >> 1) / nyt;
+  if (m >= mend) return;
   const int k0 = yt * KT;
   const int tid = threadIdx.x, lane = tid % 64, wv = tid / 64;
   const size_t zs = (size_t)nrdim * npdim;
@@ -437,7 +443,8 @@ __global__ void __launch_bounds__(256) k_v2t_mf(const double2 *__restrict__ ur, 
   const double2 *cr = ur + (size_t)nrdim * m;
   const double2 *cp = up + (size_t)nrdim * m;
   const double2 *cz = uz + (size_t)nrdim * m;
-  const size_t tb = ((size_t)m * 2 + p) * nrh * AP;
+  const int acol0 = AP * sp;
+  const size_t tb = ((size_t)m * 2 + p) * nrh * tpitch + acol0;
   const double seo = (p == 0) ? 1.0 : -1.0;
   typename MM::acc aps[AT][4], ach[AT][4];
 #pragma unroll
@@ -494,7 +501,7 @@ __global__ void __launch_bounds__(256) k_v2t_mf(const double2 *__restrict__ ur, 
             const int a = VE * (idx % (AP / VE)), x = (idx / (AP / VE)) % KV, q = idx / ((AP / VE) * KV);
             const int xg = xn + x;
             const T *src = (q == 0) ? tgV : (q == 1) ? tgD : tgT;
-            if (xg < nrh) v = *reinterpret_cast<const vec *>(src + tb + (size_t)xg * AP + a);
+            if (xg < nrh) v = *reinterpret_cast<const vec *>(src + tb + (size_t)xg * tpitch + a);
           }
           rt[j] = v;
         }
@@ -524,11 +531,12 @@ __global__ void __launch_bounds__(256) k_v2t_mf(const double2 *__restrict__ ur, 
           for (int t = 0; t < 4; ++t) f[q][t] = X[q][xr][16 * t + MM::arow(lane % 16)];
   #pragma unroll
         for (int s = 0; s < AT; ++s) {
-          const int at = wv + 4 * s;
+          const int at = wv + 4 * (sp * AT + s);
           if (at * 16 >= M) continue;
-          T bv = Tb[0][xr][at * 16 + lane % 16];
-          T bd = Tb[1][xr][at * 16 + lane % 16];
-          T bt = Tb[2][xr][at * 16 + lane % 16];
+          const int al = (wv + 4 * s) * 16 + lane % 16;  // local table column
+          T bv = Tb[0][xr][al];
+          T bd = Tb[1][xr][al];
+          T bt = Tb[2][xr][al];
   #pragma unroll
           for (int t = 0; t < 4; ++t) {
             aps[s][t] = MM::op(f[0][t], bv, aps[s][t]);
@@ -544,7 +552,7 @@ __global__ void __launch_bounds__(256) k_v2t_mf(const double2 *__restrict__ ur, 
   }
 #pragma unroll
   for (int s = 0; s < AT; ++s) {
-    const int a = (wv + 4 * s) * 16 + lane % 16;
+    const int a = (wv + 4 * (sp * AT + s)) * 16 + lane % 16;
     const int n = 2 * a + p;
     if (n >= nrdim) continue;
 #pragma unroll
@@ -561,7 +569,8 @@ __global__ void __launch_bounds__(256) k_v2t_mf(const double2 *__restrict__ ur, 
     }
   }
   // rows of this parity past the tiles: zero
-  constexpr int A_END = 4 * AT * 16;
+  if (sp != 0) return;  // the first of the row-sharing workgroups zeroes the rest
+  const int A_END = 4 * AT * asplit * 16;
   const int nrest = (nrdim - p - 2 * A_END + 1) / 2;
   for (int idx = tid; idx < (nrest > 0 ? nrest : 0) * KT; idx += 256) {
     const int n = 2 * (A_END + idx % nrest) + p, kg = k0 + idx / nrest;
@@ -715,31 +724,66 @@ void leg_backward_pairs_f32(mlx_ctx *c, const double2 *in, float4 *pairs, int nc
 // Every entry of psi and chi is written (zeros outside the computed set).
 // ncols > 0: columns m >= ncols are zero; kappa slices zlo..zhi are zero
 // (zlo > zhi: none), for a caller that de-aliases the result.
+// prec_m0: precision of column m = 0 (an MFMA precision; -1: c->prec).  When
+// it differs from c->prec, column 0 runs as a launch of its own on the
+// context's second stream, concurrently with the other columns' launch.
 void vec2tp_project_mfma(mlx_ctx *c, const double2 *ur, const double2 *up, const double2 *uz, double2 *psi,
-                         double2 *chi, int ncols, int zlo, int zhi) {
+                         double2 *chi, int ncols, int zlo, int zhi, int prec_m0) {
   const KitDev &k = c->kit;
   const int nyt = (k.nzdim + KT - 1) / KT;
   const int npc = (ncols > 0 && ncols < k.npchop) ? ncols : k.npchop;
   int nzk = k.nzdim;
   for (int kk = 0; kk < k.nzdim; ++kk)
     if (kk >= zlo && kk <= zhi) --nzk;
-  dim3 grid = xcd_grid(k.npdim, 2 * nyt);
   const int at = mfma_tiles(k);
   const bool f64 = mfma_fp64(c);
-  ProbeScope probe(c, MLX_PROBE_V2T, v2t_bytes(k, 0, f64 ? 8 : 4, npc, nzk, true),
-                   5.0 * leg_flops(k, 0, npc) * nzk / (double)k.nzdim);
-#define MLX_V2T(T_, AT_, V_, D_, T3_)                                                                            \
-  probed_launch(probe, k_v2t_mf<T_, AT_>, grid, dim3(256), 0, c->stream, ur, up, uz, psi, chi, V_, D_, T3_, k.ak,   \
-                k.nrdim, k.npdim, k.nzdim, k.nr, k.nrh, k.nrchop, npc, k.chopzl, k.chopzu, nyt, zlo, zhi)
-  if (f64) {
-    if (at == 1) MLX_V2T(double, 1, k.tVg64, k.tDg64, k.tTg64);
-    else MLX_V2T(double, 2, k.tVg64, k.tDg64, k.tTg64);
-  } else {
-    if (at == 1) MLX_V2T(float, 1, k.tVg32, k.tDg32, k.tTg32);
-    else MLX_V2T(float, 2, k.tVg32, k.tDg32, k.tTg32);
+  if (prec_m0 == MLX_PREC_FP64) prec_m0 = MLX_PREC_FP64_MFMA;  // column 0 stays on MFMA
+  const bool split = prec_m0 >= 0 && prec_m0 != c->prec;
+  bool f64_0 = f64;
+  if (split) {
+    if (prec_m0 != MLX_PREC_FP32 && prec_m0 != MLX_PREC_FP64_MFMA)
+      MLX_THROW(MLX_E_ARG, "vec2tp: no projection kernels for the m = 0 column's precision");
+    f64_0 = prec_m0 == MLX_PREC_FP64_MFMA;
   }
+  // algorithmic bytes: column 0's three table slabs at its own width
+  const double tb = f64 ? 8.0 : 4.0, tb0 = f64_0 ? 8.0 : 4.0;
+  const double bytes = v2t_bytes(k, 0, (int)tb, npc, nzk, true) + 3.0 * k.nrh * nrcs_of(k, 0, 0) * (tb0 - tb);
+  ProbeScope probe(c, MLX_PROBE_V2T, bytes, 5.0 * leg_flops(k, 0, npc) * nzk / (double)k.nzdim);
+  // rs: row splits; the launch of column 0 alone spreads its rows over
+  // `at` workgroups of one tile per wave (more workgroups for its long
+  // column, run beside the other columns' launch)
+  auto launch = [&](bool d, int mb, int me, hipStream_t s, int rs) {
+    dim3 grid = xcd_grid(me - mb, 2 * nyt * rs);
+#define MLX_V2T(T_, AT_, V_, D_, T3_)                                                                    \
+  hipLaunchKernelGGL((k_v2t_mf<T_, AT_>), grid, dim3(256), 0, s, ur, up, uz, psi, chi, V_, D_, T3_, k.ak, k.nrdim, \
+                     k.npdim, k.nzdim, k.nr, k.nrh, k.nrchop, npc, k.chopzl, k.chopzu, nyt, zlo, zhi, mb, me, rs, \
+                     k.mfma_ap)
+    const int at_l = rs > 1 ? 1 : at;
+    if (d) {
+      if (at_l == 1) MLX_V2T(double, 1, k.tVg64, k.tDg64, k.tTg64);
+      else MLX_V2T(double, 2, k.tVg64, k.tDg64, k.tTg64);
+    } else {
+      if (at_l == 1) MLX_V2T(float, 1, k.tVg32, k.tDg32, k.tTg32);
+      else MLX_V2T(float, 2, k.tVg32, k.tDg32, k.tTg32);
+    }
 #undef MLX_V2T
-  HIPCHK(hipGetLastError());
+    HIPCHK(hipGetLastError());
+  };
+  probe.mark_start(c->stream);
+  if (!split) {
+    launch(f64, 0, k.npdim, c->stream, 1);
+  } else {
+    // fork: column 0 on the second stream once the inputs are ready, the
+    // rest on the context stream; join before anything reads psi or chi
+    aux_stream_setup(c);
+    HIPCHK(hipEventRecord(c->ev_fork, c->stream));
+    HIPCHK(hipStreamWaitEvent(c->aux, c->ev_fork, 0));
+    launch(f64_0, 0, 1, c->aux, at);
+    launch(f64, 1, k.npdim, c->stream, 1);
+    HIPCHK(hipEventRecord(c->ev_join, c->aux));
+    HIPCHK(hipStreamWaitEvent(c->stream, c->ev_join, 0));
+  }
+  probe.mark_stop(c->stream);
 }
 
 }  // namespace mlx

@@ -130,7 +130,7 @@ struct mlx_ctx {
   int prec = MLX_PREC_FP64;  // precision of the contraction being issued
   int prec_ctx = MLX_PREC_FP64;  // the context's precision (mlx_set_precision)
   // per contraction site (MLX_STAGE_*): -1 follows prec_ctx
-  int stage_prec[MLX_STAGE_COUNT] = {-1, -1, -1};
+  int stage_prec[MLX_STAGE_COUNT] = {-1, -1, -1, -1};
   std::string err;
   size_t nelem = 0;
   // scratch buffers of one field each, and small device scratch
@@ -142,7 +142,19 @@ struct mlx_ctx {
   std::vector<LuCache> lu;
   uint64_t lu_clock = 0;
   ProbeSet probe;
+  // second stream (default priority) for work that overlaps the context
+  // stream inside one operation, with its fork and join events; created on
+  // first use
+  hipStream_t aux = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
 };
+
+inline void aux_stream_setup(mlx_ctx *c) {
+  if (c->aux) return;
+  HIPCHK(hipStreamCreateWithFlags(&c->aux, hipStreamNonBlocking));
+  HIPCHK(hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming));
+  HIPCHK(hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming));
+}
 
 // The precision of a contraction site: its own setting, else the context's.
 static inline int stage_precision(const mlx_ctx *c, int stage) {
@@ -181,6 +193,17 @@ struct ProbeScope {
     return c->probe.ev[id][2 * slot];
   }
   hipEvent_t stop_event() const { return slot >= 0 ? c->probe.ev[id][2 * slot + 1] : nullptr; }
+  // explicit interval on stream s around launches that are not probed_launch
+  // (several streams inside): the start before the first, the stop after the
+  // join
+  void mark_start(hipStream_t s) {
+    if (slot < 0 || started) return;
+    started = true;
+    (void)hipEventRecord(c->probe.ev[id][2 * slot], s);
+  }
+  void mark_stop(hipStream_t s) const {
+    if (slot >= 0) (void)hipEventRecord(c->probe.ev[id][2 * slot + 1], s);
+  }
   ~ProbeScope() {
     if (slot >= 0 && !started) {  // nothing launched: an empty interval
       (void)hipEventRecord(c->probe.ev[id][2 * slot], c->stream);
@@ -266,11 +289,12 @@ void leg_backward_mfma(mlx_ctx *c, const double2 *in, double2 *out, int ncols = 
 // and column) takes pair_stride float4 (256-byte aligned lines)
 inline int pair_stride(const KitDev &k) { return (k.nrh + 15) / 16 * 16; }
 void leg_backward_pairs_f32(mlx_ctx *c, const double2 *in, float4 *pairs, int ncols);
+// prec_m0: precision of column m = 0 (-1: the context's current one)
 void vec2tp_project_mfma(mlx_ctx *c, const double2 *ur, const double2 *up, const double2 *uz, double2 *psi,
-                         double2 *chi, int ncols = -1, int zlo = 1, int zhi = 0);
+                         double2 *chi, int ncols = -1, int zlo = 1, int zhi = 0, int prec_m0 = -1);
 void vec2tp_project(mlx_ctx *c, const double2 *ur, const double2 *up, const double2 *uz,
                     double2 *psi, double2 *chi, int nroff, int ncols = -1,
-                    int zlo = 1, int zhi = 0);
+                    int zlo = 1, int zhi = 0, int prec_m0 = -1);
 
 // spectral.hip
 void k_ln_column(mlx_ctx *c, double2 *e, const double *ln, double sign);

@@ -35,7 +35,8 @@ module mlegs_amd
   integer(c_int), parameter, public :: MLX_PREC_FP64 = 0, MLX_PREC_FP32 = 1, MLX_PREC_BF16X3 = 2
   integer(c_int), parameter, public :: MLX_PREC_FP64_MFMA = 3
   !> contraction sites with their own precision (kit%set_stage_precision)
-  integer(c_int), parameter, public :: MLX_STAGE_SYNTH = 0, MLX_STAGE_UP = 1, MLX_STAGE_PROJ = 2
+  integer(c_int), parameter, public :: MLX_STAGE_SYNTH = 0, MLX_STAGE_UP = 1, MLX_STAGE_PROJ = 2, &
+                                        MLX_STAGE_PROJ0 = 3
 
   !> run parameters (mlegs_base.f90:7-53) and their defaults
   integer(i4), public :: nr = 256, np = 128, nz = 128

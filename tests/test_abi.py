@@ -56,8 +56,8 @@ def test_ctypes_table_matches_header():
         assert getattr(_lib, name) == d[name]
     assert (_lib.PREC_FP64, _lib.PREC_FP32, _lib.PREC_BF16X3, _lib.PREC_FP64_MFMA) == (
         d['MLX_PREC_FP64'], d['MLX_PREC_FP32'], d['MLX_PREC_BF16X3'], d['MLX_PREC_FP64_MFMA'])
-    assert (_lib.STAGE_SYNTH, _lib.STAGE_UP, _lib.STAGE_PROJ) == (d['MLX_STAGE_SYNTH'], d['MLX_STAGE_UP'],
-                                                                   d['MLX_STAGE_PROJ'])
+    assert (_lib.STAGE_SYNTH, _lib.STAGE_UP, _lib.STAGE_PROJ, _lib.STAGE_PROJ0) == (
+        d['MLX_STAGE_SYNTH'], d['MLX_STAGE_UP'], d['MLX_STAGE_PROJ'], d['MLX_STAGE_PROJ0'])
 
 
 def test_params_struct_layout(tmp_path):

@@ -78,11 +78,11 @@ def test_c3_nonlinear_term_mixed(c3):
     import mlegs_amd as M
     p, tfm, okit, (gpsi, gchi), _, (onp, onc) = c3
     tfm.set_precision(M.PREC_FP32)
-    tfm.set_stage_precision(M.STAGE_PROJ, M.PREC_FP64_MFMA)
+    tfm.set_stage_precision(M.STAGE_PROJ0, M.PREC_FP64_MFMA)
     try:
         _, _, gnp, gnc = _gpu_nl(tfm, gpsi, gchi)
     finally:
-        tfm.set_stage_precision(M.STAGE_PROJ, -1)
+        tfm.set_stage_precision(M.STAGE_PROJ0, -1)
         tfm.set_precision(M.PREC_FP64)
     ep, ec = rel_err(gnp.e, onp.e), rel_err(gnc.e, onc.e)
     _say('mixed nonlinear term: psi %.2e chi %.2e' % (ep, ec))

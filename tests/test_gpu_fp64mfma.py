@@ -141,17 +141,21 @@ def test_stage_precision_projection_fp64(setup):
     import mlegs_amd as M
     name, p, tfm, okit = setup
     _say('stage precision', name)
+    F = M.PREC_FP64_MFMA
     ep32, ec32 = _nl_errors(tfm, okit, p, {})
-    epm, ecm = _nl_errors(tfm, okit, p, {M.STAGE_PROJ: M.PREC_FP64_MFMA})
-    ep64, ec64 = _nl_errors(tfm, okit, p, {M.STAGE_PROJ: M.PREC_FP64_MFMA, M.STAGE_SYNTH: M.PREC_FP64_MFMA,
-                                             M.STAGE_UP: M.PREC_FP64_MFMA})
-    _say('  fp32: %.2e %.2e; proj fp64: %.2e %.2e; all fp64: %.2e %.2e' % (ep32, ec32, epm, ecm, ep64, ec64))
+    ep0, ec0 = _nl_errors(tfm, okit, p, {M.STAGE_PROJ0: F})
+    epm, ecm = _nl_errors(tfm, okit, p, {M.STAGE_PROJ: F, M.STAGE_PROJ0: F})
+    ep64, ec64 = _nl_errors(tfm, okit, p, {M.STAGE_PROJ: F, M.STAGE_PROJ0: F, M.STAGE_SYNTH: F, M.STAGE_UP: F})
+    epx, ecx = _nl_errors(tfm, okit, p, {M.STAGE_PROJ: F, M.STAGE_SYNTH: F, M.STAGE_UP: F})
+    _say('  fp32: %.2e %.2e; m=0 fp64: %.2e %.2e; proj fp64: %.2e %.2e; all fp64: %.2e %.2e; '
+         'all but m=0 fp64: %.2e %.2e' % (ep32, ec32, ep0, ec0, epm, ecm, ep64, ec64, epx, ecx))
     # every site in FP64 through the policy is the FP64 MFMA path
     assert ep64 < 1e-10 and ec64 < 1e-10
-    # FP32 rounding is visible, and the projection's site setting takes effect
+    # FP32 rounding is visible, and the projection's site settings take effect
     assert ec32 > 1e-9
-    assert ecm != ec32
-    assert epm < 1e-5 and ecm < 1e-3
+    assert ec0 != ec32 and ecm != ec0 and ecx > 1e-9
+    for a, b in ((ep0, ec0), (epm, ecm)):
+        assert a < 1e-5 and b < 1e-3
 
 
 def test_stage_precision_rejects_bad_arguments():
