@@ -363,8 +363,11 @@ void abcn_pair(mlx_ctx *c, mlx_field *s1, mlx_field *s1p, mlx_field *nl1, mlx_fi
 // (adding 0 * ln x) is left out; the other steps are trans's FFF -> PPP.
 // phys_rows: rows nr.. of the outputs are not needed (the caller's cross
 // product takes them as zero) and are not synthesized
-void tp2vec_back(mlx_ctx *c, mlx_field *ur, mlx_field *up, mlx_field *uz, int ncols = -1, int zlo = 1,
-                 int zhi = 0, bool phys_rows = false) {
+// f32_ok: the caller reads the outputs through fft_phi_cross's FP32 input;
+// when the radial pairs path runs (FP32 sums) the outputs are then stored in
+// FP32.  Returns whether they were.
+bool tp2vec_back(mlx_ctx *c, mlx_field *ur, mlx_field *up, mlx_field *uz, int ncols = -1, int zlo = 1,
+                 int zhi = 0, bool phys_rows = false, bool f32_ok = false) {
   const KitDev &k = c->kit;
   PrecScope ps(c, MLX_STAGE_SYNTH);
   const int rs[3] = {-1, -1, 0};
@@ -384,7 +387,7 @@ void tp2vec_back(mlx_ctx *c, mlx_field *ur, mlx_field *up, mlx_field *uz, int nc
     if (pairs) {
       float4 *pr = reinterpret_cast<float4 *>(c->scr[0]);
       leg_backward_pairs_f32(c, f->e, pr, nc);
-      fft_phi_pairs(c, f->e, pr, rs[q], nc);
+      fft_phi_pairs(c, f->e, pr, rs[q], nc, f32_ok);
       set_space(f, "PPP");
       ++q;
       continue;
@@ -396,6 +399,7 @@ void tp2vec_back(mlx_ctx *c, mlx_field *ur, mlx_field *up, mlx_field *uz, int nc
     set_space(f, "PPP");
     ++q;
   }
+  return pairs && f32_ok;
 }
 
 // tp2vec (vector_reconstruction, mlegs_scalar_ops.f90:1455-1545): the +3-offset
@@ -560,6 +564,7 @@ void advection_rhs(mlx_ctx *c, mlx_field *psi, mlx_field *chi, mlx_field *nlpsi,
   // psi and chi are de-aliased here and so is the result: the MFMA path
   // skips the columns and slices de-aliasing zeroes
   const DealiasLimits dl = dealias_limits(c);
+  bool f32phys = false;
   {
     // tp2vec and tp2curlvec: both front passes in one launch, then the six
     // backward transforms
@@ -567,14 +572,18 @@ void advection_rhs(mlx_ctx *c, mlx_field *psi, mlx_field *chi, mlx_field *nlpsi,
     const bool zskip = fft_z_fast(c->kit);
     k_tp2vec_front2(c, vr->e, vp->e, vz->e, vr->sc, vp->sc, vz->sc, wr->e, wp->e, wz->e, wr->sc, wp->sc, wz->sc,
                     psi->e, chi->e, psi->sc, chi->sc, chi->off[0], ncols, zskip ? dl.zlo : 1, zskip ? dl.zhi : 0);
-    tp2vec_back(c, vr, vp, vz, ncols, dl.zlo, dl.zhi, true);
-    tp2vec_back(c, wr, wp, wz, ncols, dl.zlo, dl.zhi, true);
+    // the six physical fields are read only by the fused cross-product
+    // transform below: with FP32 radial sums they are stored in FP32
+    const bool cross = fft_phi_cross_ok(c->kit);
+    f32phys = tp2vec_back(c, vr, vp, vz, ncols, dl.zlo, dl.zhi, true, cross);
+    if (tp2vec_back(c, wr, wp, wz, ncols, dl.zlo, dl.zhi, true, cross) != f32phys)
+      MLX_THROW(MLX_E_ARG, "advection_rhs: velocity and vorticity stored at different precisions");
   }
   // de-aliased inside vec2tp before idel2; chop and zeroat1 keep zero modes
   // zero, so the closing dealias of the reference is already satisfied
   mlx_field *ur = tmp(c, 0), *up = tmp(c, 1), *uzp = tmp(c, 2);
   if (fft_phi_cross(c, ur->e, up->e, uzp->e, vr->e, vp->e, vz->e, wr->e, wp->e, wz->e, uz->e, ur->sc, up->sc,
-                    uzp->sc, dl.on ? dl.pcut : -1)) {
+                    uzp->sc, dl.on ? dl.pcut : -1, f32phys)) {
     // vz + uz, v x w, and the phi transforms of r (v x w)_r, r (v x w)_phi,
     // (v x w)_z in one pass: log terms zero (set by the kernel), no offsets,
     // as vecprod leaves them

@@ -275,6 +275,17 @@ __host__ __device__ constexpr int bitrev(int i, int n) {
 }
 
 // in-register radix-2 DIT DFT of size N (N <= 16), sign sg of the exponent
+// FP32 storage of physical fields (two floats per packed entry)
+typedef float mlx_fv2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ void st_nt_f32(double2 *base, size_t idx, double2 x) {
+  const mlx_fv2 v = {(float)x.x, (float)x.y};
+  __builtin_nontemporal_store(v, reinterpret_cast<mlx_fv2 *>(base) + idx);
+}
+__device__ __forceinline__ double2 ld_nt_f32(const double2 *base, size_t idx) {
+  const mlx_fv2 v = __builtin_nontemporal_load(reinterpret_cast<const mlx_fv2 *>(base) + idx);
+  return make_double2((double)v.x, (double)v.y);
+}
+
 template <int N>
 __device__ __forceinline__ void dft_reg(double2 *v, double sg) {
   double2 t[N];
@@ -318,7 +329,9 @@ __device__ __forceinline__ double2 twid(const double2 *tw, int idx, double sg) {
 // gs / es (each line of pairs starts on a 256-byte boundary); a tile holds
 // the pairs x0 .. x0 + TI/2 - 1 and synthesizes rows x = be + bo and
 // nr - 1 - x = be - bo (formed in FP64 on load), rows < nr only.
-template <int N1, int N2, int TI, int MODE, bool PAIR = false>
+// F32OUT (MODE 2): the output is stored in FP32, entry e at float pair e of
+// out's buffer (same indexing, half the bytes).
+template <int N1, int N2, int TI, int MODE, bool PAIR = false, bool F32OUT = false>
 __global__ void __launch_bounds__(kThreads) k_fft4(double2 *out, const double2 *in, size_t es, size_t gs, int nrow,
                                                    int act_nrc, const double2 *tw, const double2 *twh, double sg,
                                                    double scale, const double *r, int nr, int rscale, int mlim,
@@ -490,9 +503,13 @@ __global__ void __launch_bounds__(kThreads) k_fft4(double2 *out, const double2 *
             double2 x = y2[t][k2];
             if (MODE == 0) x = make_double2(x.x * scale, x.y * scale);
             else if (dv != 1.0) x = make_double2(x.x / dv, x.y / dv);
-            st_nt(&out[base + (size_t)ko * es + i], x);
+            if (F32OUT) st_nt_f32(out, base + (size_t)ko * es + i, x);
+            else st_nt(&out[base + (size_t)ko * es + i], x);
           }
-          if (MODE == 2 && k1 == 0) st_nt(&out[base + (size_t)N * es + i], make_double2(0.0, 0.0));
+          if (MODE == 2 && k1 == 0) {
+            if (F32OUT) st_nt_f32(out, base + (size_t)N * es + i, make_double2(0.0, 0.0));
+            else st_nt(&out[base + (size_t)N * es + i], make_double2(0.0, 0.0));
+          }
         }
       }
     }
@@ -531,7 +548,9 @@ __global__ void __launch_bounds__(kThreads) k_fft4(double2 *out, const double2 *
 // of the four-step transform forms the products while loading, so each
 // input is read once and no physical product field is written.  Rows
 // i >= nr give zero products, as vector_product leaves them.
-template <int N1, int N2, int TI>
+// F32IN: the six velocity and vorticity fields are stored in FP32 (ub, the
+// background flow, stays FP64).
+template <int N1, int N2, int TI, bool F32IN = false>
 __global__ void __launch_bounds__(kThreads) k_fft4_cross(double2 *o0, double2 *o1, double2 *o2, const double2 *v0,
                                                          const double2 *v1, const double2 *v2, const double2 *w0,
                                                          const double2 *w1, const double2 *w2, const double2 *ub,
@@ -570,12 +589,21 @@ __global__ void __launch_bounds__(kThreads) k_fft4_cross(double2 *o0, double2 *o
         double2 a1 = z2, a2 = z2, a3 = z2, b1 = z2, b2 = z2, b3 = z2;
         if (i < nr) {
           const size_t o = base + (size_t)j * es + i;
-          a1 = ld_nt(&v0[o]);
-          a2 = ld_nt(&v1[o]);
-          a3 = ld_nt(&v2[o]);
-          b1 = ld_nt(&w0[o]);
-          b2 = ld_nt(&w1[o]);
-          b3 = ld_nt(&w2[o]);
+          if (F32IN) {
+            a1 = ld_nt_f32(v0, o);
+            a2 = ld_nt_f32(v1, o);
+            a3 = ld_nt_f32(v2, o);
+            b1 = ld_nt_f32(w0, o);
+            b2 = ld_nt_f32(w1, o);
+            b3 = ld_nt_f32(w2, o);
+          } else {
+            a1 = ld_nt(&v0[o]);
+            a2 = ld_nt(&v1[o]);
+            a3 = ld_nt(&v2[o]);
+            b1 = ld_nt(&w0[o]);
+            b2 = ld_nt(&w1[o]);
+            b3 = ld_nt(&w2[o]);
+          }
           if (ub) {
             const double2 u = ld_nt(&ub[o]);
             a3 = make_double2(a3.x + u.x, a3.y + u.y);
@@ -659,7 +687,7 @@ __global__ void __launch_bounds__(kThreads) k_fft4_cross(double2 *o0, double2 *o
   }
 }
 
-template <int MODE, bool PAIR = false>
+template <int MODE, bool PAIR = false, bool F32OUT = false>
 bool launch_fft4(mlx_ctx *c, ProbeScope &ps, int N, dim3 grid_y, double2 *out, const double2 *in, size_t es,
                  size_t gs, int nrow, int act_nrc, const double2 *tw, const double2 *twh, double sg, double scale,
                  int rscale, int mlim = 1 << 30, int zin_lo = 1, int zin_hi = 0, int zout_lo = 1, int zout_hi = 0,
@@ -669,7 +697,8 @@ bool launch_fft4(mlx_ctx *c, ProbeScope &ps, int N, dim3 grid_y, double2 *out, c
   do {                                                                                                        \
     constexpr int TH_ = PAIR ? TI_ / 2 : TI_;                                                                 \
     dim3 grid((nrow + TH_ - 1) / TH_, grid_y.y);                                                              \
-    probed_launch(ps, k_fft4<N1_, N2_, TI_, MODE, PAIR>, grid, dim3(kThreads), 0, c->stream, out, in, es, gs, nrow,   \
+    probed_launch(ps, k_fft4<N1_, N2_, TI_, MODE, PAIR, F32OUT>, grid, dim3(kThreads), 0, c->stream, out, in, es, gs, \
+                  nrow,                                                                                       \
                   act_nrc, tw, twh, sg, scale, k.r, k.nr, rscale, mlim, zin_lo, zin_hi, zout_lo,     \
                        zout_hi, pstride);                                                                    \
     return true;                                                                                              \
@@ -738,36 +767,52 @@ void fft_phi(mlx_ctx *c, double2 *out, const double2 *in, int dir, int rscale, i
 // The backward phi transform of the radial pairs that leg_backward_pairs_f32
 // writes: rows x and nr - 1 - x (x < nr / 2) of the PPP output, rows >= nr
 // left as they are.  mlim: modes m >= mlim are taken as zero.
-void fft_phi_pairs(mlx_ctx *c, double2 *out, const float4 *pairs, int rscale, int mlim) {
+void fft_phi_pairs(mlx_ctx *c, double2 *out, const float4 *pairs, int rscale, int mlim, bool f32out) {
   const KitDev &k = c->kit;
   if (!fft_phi_fast(k) || (k.nr & 1)) MLX_THROW(MLX_E_ARG, "fft_phi: radial pairs need the four-step kernels");
   const double nread = (mlim > 0 && mlim < k.nph + 1) ? mlim : k.nph + 1;
-  ProbeScope probe(c, MLX_PROBE_FFT_PHI, 8.0 * k.nr * k.nz * nread + 16.0 * k.nr * k.nz * k.nph,
+  ProbeScope probe(c, MLX_PROBE_FFT_PHI, 8.0 * k.nr * k.nz * nread + (f32out ? 8.0 : 16.0) * k.nr * k.nz * k.nph,
                    2.5 * k.np * std::log2((double)k.np) * k.nr * k.nz);
   const size_t plane = (size_t)k.nrdim * k.npdim;
-  launch_fft4<2, true>(c, probe, k.nph, dim3(1, k.nzdim), out, reinterpret_cast<const double2 *>(pairs), k.nrdim,
-                       plane, k.nrh, 0, k.tw_nph, k.tw_half, +1.0, 1.0, rscale, mlim > 0 ? mlim : 1 << 30, 1, 0, 1,
-                       0, pair_stride(k));
+  const double2 *pin = reinterpret_cast<const double2 *>(pairs);
+  const int ml = mlim > 0 ? mlim : 1 << 30;
+  if (f32out)
+    launch_fft4<2, true, true>(c, probe, k.nph, dim3(1, k.nzdim), out, pin, k.nrdim, plane, k.nrh, 0, k.tw_nph,
+                               k.tw_half, +1.0, 1.0, rscale, ml, 1, 0, 1, 0, pair_stride(k));
+  else
+    launch_fft4<2, true>(c, probe, k.nph, dim3(1, k.nzdim), out, pin, k.nrdim, plane, k.nrh, 0, k.tw_nph, k.tw_half,
+                         +1.0, 1.0, rscale, ml, 1, 0, 1, 0, pair_stride(k));
   HIPCHK(hipGetLastError());
+}
+
+bool fft_phi_cross_ok(const KitDev &k) {
+  const int N = k.nph;
+  return k.np > 1 && (N == 16 || N == 32 || N == 64 || N == 128 || N == 256);
 }
 
 bool fft_phi_cross(mlx_ctx *c, double2 *o0, double2 *o1, double2 *o2, const double2 *v0, const double2 *v1,
                    const double2 *v2, const double2 *w0, const double2 *w1, const double2 *w2, const double2 *ub,
-                   double *ln0, double *ln1, double *ln2, int mmax) {
+                   double *ln0, double *ln1, double *ln2, int mmax, bool f32in) {
   const KitDev &k = c->kit;
   const int N = k.nph;
   if (mmax <= 0 || mmax > N + 1) mmax = N + 1;
-  if (k.np <= 1 || !(N == 16 || N == 32 || N == 64 || N == 128 || N == 256)) return false;
+  if (!fft_phi_cross_ok(k)) return false;
   const size_t plane = (size_t)k.nrdim * k.npdim;
-  // seven physical fields read (rows < nr), three PFP fields written
-  ProbeScope probe(c, MLX_PROBE_PHI_CROSS, 16.0 * k.nr * k.nz * (7.0 * N + 3.0 * mmax),
+  // six physical fields (FP32 or FP64) and ub read (rows < nr), three PFP
+  // fields written
+  ProbeScope probe(c, MLX_PROBE_PHI_CROSS, k.nr * k.nz * ((f32in ? 48.0 : 96.0) * N + 16.0 * N + 48.0 * mmax),
                    3.0 * 2.5 * k.np * std::log2((double)k.np) * k.nr * k.nz + 18.0 * k.nr * k.np * k.nz);
 #define MLX_FFTX(N1_, N2_, TI_)                                                                                 \
   do {                                                                                                          \
     dim3 grid((k.nrdim + TI_ - 1) / TI_, k.nzdim);                                                              \
-    probed_launch(probe, k_fft4_cross<N1_, N2_, TI_>, grid, dim3(kThreads), 0, c->stream, o0, o1, o2, v0, v1, v2, \
-                       w0, w1, w2, ub, (size_t)k.nrdim, plane, k.nrdim, k.tw_nph, k.tw_half, 1.0 / k.np, k.r,   \
-                       k.nr, ln0, ln1, ln2, mmax);                                                              \
+    if (f32in)                                                                                                  \
+      probed_launch(probe, k_fft4_cross<N1_, N2_, TI_, true>, grid, dim3(kThreads), 0, c->stream, o0, o1, o2, v0,  \
+                    v1, v2, w0, w1, w2, ub, (size_t)k.nrdim, plane, k.nrdim, k.tw_nph, k.tw_half, 1.0 / k.np, k.r, \
+                    k.nr, ln0, ln1, ln2, mmax);                                                                 \
+    else                                                                                                        \
+      probed_launch(probe, k_fft4_cross<N1_, N2_, TI_, false>, grid, dim3(kThreads), 0, c->stream, o0, o1, o2, v0, \
+                    v1, v2, w0, w1, w2, ub, (size_t)k.nrdim, plane, k.nrdim, k.tw_nph, k.tw_half, 1.0 / k.np, k.r, \
+                    k.nr, ln0, ln1, ln2, mmax);                                                                 \
   } while (0)
   switch (N) {
     case 16: MLX_FFTX(4, 4, 64); break;

@@ -249,16 +249,21 @@ void fft_phi(mlx_ctx *c, double2 *out, const double2 *in, int dir, int rscale = 
              bool phys_rows = false);
 // backward phi transform of leg_backward_pairs_f32's radial pairs into rows
 // < nr of out (fast path only)
-void fft_phi_pairs(mlx_ctx *c, double2 *out, const float4 *pairs, int rscale, int mlim);
+// f32out: the rows are stored in FP32 (float pair e of out's buffer for
+// entry e), for fft_phi_cross's FP32 input
+void fft_phi_pairs(mlx_ctx *c, double2 *out, const float4 *pairs, int rscale, int mlim, bool f32out = false);
 bool fft_phi_fast(const KitDev &k);  // the four-step phi kernels cover np/2
 bool fft_z_fast(const KitDev &k);    // the four-step z kernels cover nz
 // v x w (v_z + ub) to PFP with the r weights of vector_projection (modes
 // m < mmax written; mmax <= 0: all); false when
 // the fused form is not built for this np (the caller then runs vecprod and
 // three phi transforms)
+// f32in: v0..w2 are stored in FP32 (as fft_phi_pairs's f32out writes them)
 bool fft_phi_cross(mlx_ctx *c, double2 *o0, double2 *o1, double2 *o2, const double2 *v0, const double2 *v1,
                    const double2 *v2, const double2 *w0, const double2 *w1, const double2 *w2, const double2 *ub,
-                   double *ln0 = nullptr, double *ln1 = nullptr, double *ln2 = nullptr, int mmax = -1);
+                   double *ln0 = nullptr, double *ln1 = nullptr, double *ln2 = nullptr, int mmax = -1,
+                   bool f32in = false);
+bool fft_phi_cross_ok(const KitDev &k);  // the fused kernel is built for this np
 // dir -1 forward (/nz), +1 backward.  ncol: the (row, column) lines the caller
 // needs transformed (the launch's algorithmic work).  act_nrc > 0: only rows
 // i < act_nrc - j of column j need transforming; ncols > 0: only columns

@@ -97,6 +97,35 @@ def test_advection_rhs_fp64_mfma(setup):
     assert ep < 1e-10 and ec < 1e-10
 
 
+def test_advection_rhs_fused_matches_composed(setup):
+    """The fused path of the step's nonlinear term (both tp2vec front passes
+    in one kernel, the de-aliasing restrictions, v x w inside the forward phi
+    transforms) against the same operators composed from the public entry
+    points (tp2vec, tp2curlvec, vecprod, vec2tp, dealias), both on FP64 MFMA:
+    the same arithmetic, so agreement to rounding."""
+    import mlegs_amd as M
+    from oracle import scalar_ops as so
+    name, p, tfm, okit = setup
+    _say('fused vs composed fp64 mfma', name)
+    (gpsi, _), (gchi, _) = C.qvortex(tfm, okit, p, amp=0.5)
+    guz = M.Scalar(tfm).upload(C.physical(tfm.dims, p, 31), space='PPP')
+    gnp, gnc = M.Scalar(tfm), M.Scalar(tfm)
+    M.advection_rhs(gpsi, gchi, gnp, gnc, guz, tfm)
+    vr, vp, vz = M.Scalar(tfm), M.Scalar(tfm), M.Scalar(tfm)
+    wr, wp, wz = M.Scalar(tfm), M.Scalar(tfm), M.Scalar(tfm)
+    M.tp2vec(gpsi, gchi, vr, vp, vz, tfm)
+    M.tp2curlvec(gpsi, gchi, wr, wp, wz, tfm)
+    vz.axpby(1.0, guz, 1.0)
+    M.vecprod(vr, vp, vz, wr, wp, wz, tfm)
+    cp, cc = M.Scalar(tfm), M.Scalar(tfm)
+    M.vec2tp(vr, vp, vz, cp, cc, tfm)
+    M.dealias(cp, tfm)
+    M.dealias(cc, tfm)
+    fp, fc = rel_err(gnp.e, cp.e), rel_err(gnc.e, cc.e)
+    _say('  nlpsi %.2e nlchi %.2e' % (fp, fc))
+    assert fp < 1e-12 and fc < 1e-10
+
+
 def test_vortex_fp64_mfma(setup):
     import mlegs_amd as M
     from oracle import scalar_ops as so, vortex as V
