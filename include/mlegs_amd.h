@@ -220,6 +220,13 @@ int mlx_helmp(mlx_ctx* ctx, mlx_field* f, int power, double alpha, double beta);
  * the same way when hyperpow != 0. */
 int mlx_ihelmp(mlx_ctx* ctx, mlx_field* f, int power, double alpha, double beta);
 
+/* fftreat(s, tfm) (mlegs_scalar.f90:190-197; mlegs_scalar_ops.f90:1002-1063):
+ * far-field smoothing.  delsqp, radial backward transform, (1-x)^2
+ * weighting, rows >= 3 nr / 4 + 4 of columns m >= 1 zeroed, five passes of
+ * smooth (:2149-2175) over rows >= 3 nr / 4, weighting removed, radial
+ * forward transform, idelsqp; ln is kept, zeroat1 applied. */
+int mlx_fftreat(mlx_ctx* ctx, mlx_field* f);
+
 /* ---- time steppers (hyperpow = 0, or 4, 6, 8 through helmp / ihelmp) ---- */
 
 /* febe(s, nl, dt, tfm) (mlegs_scalar.f90:350-359; mlegs_scalar_ops.f90:1157-1198). */

@@ -18,7 +18,7 @@ from ._lib import MlxError, PREC_FP64, PREC_FP32, PREC_BF16X3, PREC_FP64_MFMA  #
 from ._lib import STAGE_SYNTH, STAGE_UP, STAGE_PROJ, STAGE_PROJ0  # noqa: F401
 from .params import Params, read_input, timestep_set, InputError  # noqa: F401
 
-__all__ = ['Kit', 'Scalar', 'trans', 'chop', 'dealias', 'svv_filter', 'calcat0', 'calcat1', 'zeroat1',
+__all__ = ['Kit', 'Scalar', 'trans', 'chop', 'dealias', 'svv_filter', 'calcat0', 'calcat1', 'zeroat1', 'fftreat',
            'delsqp', 'idelsqp', 'xxdx', 'del2h', 'del2', 'idel2', 'ihelm', 'helmp', 'ihelmp', 'febe', 'abcn', 'vecprod',
            'vec2tp', 'tp2vec', 'tp2curlvec', 'advection_rhs', 'check_stability', 'VortexRun', 'energy',
            'Params', 'read_input', 'timestep_set', 'MlxError', 'InputError']
@@ -279,6 +279,10 @@ def idel2(s, tfm, preln=None):
         _c(tfm, tfm.lib.mlx_idel2(tfm.ctx, s.h))
     else:
         _c(tfm, tfm.lib.mlx_idel2_preln(tfm.ctx, s.h, float(preln)))
+
+
+def fftreat(s, tfm):
+    _c(tfm, tfm.lib.mlx_fftreat(tfm.ctx, s.h))
 
 
 def ihelm(s, alpha, tfm):

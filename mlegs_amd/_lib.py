@@ -79,6 +79,7 @@ SIGNATURES = {
     'mlx_del2h': ([_P, _P], ctypes.c_int),
     'mlx_del2': ([_P, _P], ctypes.c_int),
     'mlx_idel2': ([_P, _P], ctypes.c_int),
+    'mlx_fftreat': ([_P, _P], ctypes.c_int),
     'mlx_idel2_preln': ([_P, _P, _D], ctypes.c_int),
     'mlx_ihelm': ([_P, _P, _D], ctypes.c_int),
     'mlx_helmp': ([_P, _P, ctypes.c_int, _D, _D], ctypes.c_int),

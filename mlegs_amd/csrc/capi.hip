@@ -224,6 +224,31 @@ void ihelmp(mlx_ctx *c, mlx_field *f, int power, double alpha, double beta) {
   k_ihelmp(c, f->e, f->sc, power, alpha, beta, f->off[0], sing_flag(c));
 }
 
+// fftreat (mlegs_scalar_ops.f90:1002-1063): far-field smoothing.  delsqp
+// (log term left out), rtrans_backward, the physical-space weighting,
+// zeroing and smoothing, rtrans_forward, idelsqp; then the original log
+// coefficient is restored and zeroat1 applied.
+void fftreat(mlx_ctx *c, mlx_field *f) {
+  need_kit(c);
+  to_fff(c, f);
+  const KitDev &k = c->kit;
+  if (k.nrchop + f->off[0] > k.nrdim_ref) MLX_THROW(MLX_E_ARG, "chop_index: chopping in r too large");
+  mlx_field *so = tmp(c, 15);
+  fcopy(c, so, f);
+  k_set_scalar(c, so->sc, 0.0);
+  k_delsqp(c, so->e, so->sc, so->off[0], 0);
+  leg_backward(c, so->e, c->scr[0], so->off[0], true);
+  swap_scr(c, so);
+  k_fftreat_mid(c, so->e, k.npchop);
+  leg_forward(c, so->e, c->scr[0], so->off[0], true);
+  swap_scr(c, so);
+  k_delsqp(c, so->e, so->sc, so->off[0], 1);
+  k_copy_scalar(c, so->sc, f->sc, 1.0);
+  k_zeroat1(c, so->e, k.nrchop + so->off[0]);
+  set_space(so, "FFF");
+  fmove(f, so);
+}
+
 void idel2_proln(mlx_ctx *c, mlx_field *f) {
   need_kit(c);
   to_fff(c, f);
@@ -979,6 +1004,9 @@ int mlx_del2h(mlx_ctx *c, mlx_field *f) {
 }
 int mlx_del2(mlx_ctx *c, mlx_field *f) {
   return guarded(c, [&] { bandmul(c, f, 0); });
+}
+int mlx_fftreat(mlx_ctx *c, mlx_field *f) {
+  return guarded(c, [&] { fftreat(c, f); });
 }
 int mlx_idel2(mlx_ctx *c, mlx_field *f) {
   return guarded(c, [&] { idel2_proln(c, f); });

@@ -310,6 +310,8 @@ void k_abcn_rhs_g(mlx_ctx *c, double2 *sh, double *shln, const double2 *s, const
 // column whose factorization meets a zero pivot is left unsolved and sets
 // *sing (device int) to 1
 void k_ihelmp(mlx_ctx *c, double2 *e, double *sln, int power, double alpha, double beta, int nroff, int *sing);
+// fftreat's physical-space step on the PFF lines of columns m < npc
+void k_fftreat_mid(mlx_ctx *c, double2 *e, int npc);
 void k_chop(mlx_ctx *c, double2 *e, const char *space, int nroff);
 void k_dealias(mlx_ctx *c, double2 *e, const char *space);
 // svv_filter on e0 (gain g0) and, when e1 is given, on e1 (gain g1) in the

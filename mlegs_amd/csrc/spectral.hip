@@ -1054,6 +1054,75 @@ __global__ void __launch_bounds__(256) k_ihelmp_(double2 *e, double *sln, const 
   if (c00 && tid == 0) sln[0] = lnp;
 }
 
+// ---------------------------------------------------------------------------
+// fftreat (mlegs_scalar_ops.f90:1002-1063), the physical-space part: on the
+// radial lines of the field after delsqp and rtrans_backward, per column
+// m < npc and kappa slice, (1 - x)^2 weighting of rows < nr, rows
+// >= ns0 - 1 zeroed in columns m >= 1 (every slice), five passes of smooth
+// (:2149-2175) over rows ns - 1 .. nrow - 1 on the chopped slices, and the
+// weighting removed again.  nrow is the reference's column length nr +
+// max(3, hyperpow): the smoothing reaches into the rows past nr.
+// smooth(a), a of length ni: out(1) = a(1); a(ni) goes to out(ni-2:ni) with
+// weights 0.2, 0.3, 0.5; each a(i), 1 < i < ni, to out(i-1:i+1) with
+// weights (1-f)/2, f, (1-f)/2, f = (1 + (ni-i)/(ni-1))/2 (1-based).
+// One workgroup per (m, kappa) line, the line in LDS.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_fftreat_mid_(double2 *e, const double *x, int nrdim, int npdim, int nr,
+                                                      int nrow, int ns, int ns0, int npc, int nzc, int nzcu) {
+  extern __shared__ double2 line[];  // 2 nrow
+  double2 *a = line, *b = line + nrow;
+  const int m = blockIdx.x, k = blockIdx.y, tid = threadIdx.x;
+  double2 *g = e + (size_t)k * nrdim * npdim + (size_t)nrdim * m;
+  const bool act = (k + 1 <= nzc) || (k + 1 >= nzcu);
+  if (!act) {
+    if (m >= 1)
+      for (int i = ns0 - 1 + tid; i < nrow; i += blockDim.x) g[i] = make_double2(0.0, 0.0);
+    return;
+  }
+  for (int i = tid; i < nrow; i += blockDim.x) {
+    double2 v = g[i];
+    if (i < nr) {
+      const double w = (1.0 - x[i]) * (1.0 - x[i]);
+      v = make_double2(v.x * w, v.y * w);
+    }
+    if (m >= 1 && i >= ns0 - 1) v = make_double2(0.0, 0.0);
+    a[i] = v;
+  }
+  __syncthreads();
+  const int o = ns - 1, ni = nrow - o;
+  for (int pass = 0; pass < 5; ++pass) {
+    for (int j = tid; j < ni; j += blockDim.x) {
+      double2 s = make_double2(0.0, 0.0);
+      if (j == 0) s = a[o];
+      // interior sources i = j-1, j, j+1 (1 <= i <= ni-2)
+      for (int i = j - 1; i <= j + 1; ++i) {
+        if (i < 1 || i > ni - 2) continue;
+        const double f = (1.0 + (double)(ni - (i + 1)) / (ni - 1.0)) * 0.5;
+        const double wgt = (i == j) ? f : (1.0 - f) / 2.0;
+        s.x += a[o + i].x * wgt;
+        s.y += a[o + i].y * wgt;
+      }
+      if (j >= ni - 3) {
+        const double fae = (j == ni - 3) ? 0.2 : (j == ni - 2) ? 0.3 : 0.5;
+        s.x += a[o + ni - 1].x * fae;
+        s.y += a[o + ni - 1].y * fae;
+      }
+      b[o + j] = s;
+    }
+    __syncthreads();
+    for (int j = tid; j < ni; j += blockDim.x) a[o + j] = b[o + j];
+    __syncthreads();
+  }
+  for (int i = tid; i < nrow; i += blockDim.x) {
+    double2 v = a[i];
+    if (i < nr) {
+      const double w = (1.0 - x[i]) * (1.0 - x[i]);
+      v = make_double2(v.x / w, v.y / w);
+    }
+    g[i] = v;
+  }
+}
+
 __global__ void k_vec2tp_up_ln_(double2 *up, const double2 *inf, double *psiln, const double *x, int nr) {
   double l = -1.0 / 2.0 * inf[0].x;
   if (blockIdx.x == 0 && threadIdx.x == 0) psiln[0] = l;
@@ -1270,6 +1339,20 @@ void k_ihelmp(mlx_ctx *c, double2 *e, double *sln, int power, double alpha, doub
     default: MLX_THROW(MLX_E_ARG, "ihelmp: power must be 4, 6 or 8");
   }
 #undef MLX_IHP
+  HIPCHK(hipGetLastError());
+}
+
+void k_fftreat_mid(mlx_ctx *c, double2 *e, int npc) {
+  const KitDev &k = c->kit;
+  const int ns = k.nr * 3 / 4, ns0 = (ns + 4 < k.nr) ? ns + 4 : k.nr;
+  const int nrow = k.nrdim_ref;
+  if (nrow - ns + 1 < 3) MLX_THROW(MLX_E_ARG, "smooth: the input length must be longer than or equal to 3.");
+  if (npc > k.npdim) MLX_THROW(MLX_E_ARG, "chop_index: chopping in p too large");
+  if (npc <= 0) return;
+  const size_t lds = 2 * sizeof(double2) * (size_t)nrow;
+  if (lds > 65536) MLX_THROW(MLX_E_ARG, "fftreat: the radial line exceeds the workgroup's LDS");
+  hipLaunchKernelGGL(k_fftreat_mid_, dim3(npc, k.nzdim), dim3(256), lds, c->stream, e, k.x, k.nrdim, k.npdim, k.nr,
+                     nrow, ns, ns0, npc, k.chopzl, k.chopzu);
   HIPCHK(hipGetLastError());
 }
 

@@ -105,7 +105,7 @@ module mlegs_amd
 
   public :: read_input, timestep_set
   public :: assignment(=)
-  public :: trans, chop, dealias, svv_filter, calcat0, calcat1, zeroat1
+  public :: trans, chop, dealias, svv_filter, calcat0, calcat1, zeroat1, fftreat
   public :: delsqp, idelsqp, xxdx, del2h, del2, idel2, ihelm, helmp, ihelmp, febe, abcn
   public :: vecprod, vec2tp, tp2vec, tp2curlvec, advection_rhs, check_stability, energy
   public :: msave, mload, vorticity_magnitude
@@ -139,6 +139,9 @@ module mlegs_amd
   end interface
   interface zeroat1
     module procedure scalar_zeroat1
+  end interface
+  interface fftreat
+    module procedure scalar_fftreat
   end interface
   interface delsqp
     module procedure scalar_delsqp
@@ -387,6 +390,10 @@ module mlegs_amd
       type(c_ptr), value :: ctx, f
     end function
     integer(c_int) function c_mlx_zeroat1(ctx, f) bind(C, name='mlx_zeroat1')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx, f
+    end function
+    integer(c_int) function c_mlx_fftreat(ctx, f) bind(C, name='mlx_fftreat')
       import :: c_int, c_ptr
       type(c_ptr), value :: ctx, f
     end function
@@ -834,6 +841,13 @@ contains
     class(scalar), intent(inout) :: s
     class(tfm_kit_3d), intent(in) :: tfm
     call one(c_mlx_zeroat1, s, tfm, 'zeroat1')
+  end subroutine
+
+  !> fftreat(s, tfm) (mlegs_scalar_ops.f90:1002-1063): far-field smoothing
+  subroutine scalar_fftreat(s, tfm)
+    class(scalar), intent(inout) :: s
+    class(tfm_kit_3d), intent(in) :: tfm
+    call one(c_mlx_fftreat, s, tfm, 'fftreat')
   end subroutine
 
   subroutine scalar_delsqp(s, tfm)

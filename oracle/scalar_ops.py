@@ -367,6 +367,64 @@ def idelsqp(s, kit):
     s.assign(so)
 
 
+def smooth(a):
+    """One smoothing pass along the first axis (mlegs_scalar_ops.f90:2149-2175):
+    out(1) = a(1); a(ni) spread over out(ni-2:ni) with 0.2, 0.3, 0.5; each
+    interior a(i) over out(i-1:i+1) with (1-f)/2, f, (1-f)/2,
+    f = (1 + (ni - i)/(ni - 1))/2 (1-based i)."""
+    ni = a.shape[0]
+    if ni < 3:
+        raise ValueError('smooth: the input length must be longer than or equal to 3.')
+    out = np.zeros_like(a)
+    out[0] = a[0]
+    for q, fae in enumerate((0.2, 0.3, 0.5)):
+        out[ni - 3 + q] += a[ni - 1] * fae
+    for i1 in range(2, ni):  # 1-based i = 2 .. ni-1
+        f = (1.0 + (ni - i1) / (ni - 1.0)) * 0.5
+        i = i1 - 1
+        out[i - 1] += a[i] * ((1.0 - f) / 2.0)
+        out[i] += a[i] * f
+        out[i + 1] += a[i] * ((1.0 - f) / 2.0)
+    return out
+
+
+def fftreat(s, kit):
+    """Far-field smoothing (mlegs_scalar_ops.f90:1002-1063): delsqp of s
+    without its log term, radial synthesis, (1-x)^2 weighting, rows >= ns0 of
+    columns m >= 1 zeroed (every slice), smooth five times over rows >= ns
+    on the chopped slices, weighting removed, radial analysis, idelsqp; the
+    log coefficient is restored and zeroat1 applied.  ns = 3 nr / 4 (integer),
+    ns0 = min(ns + 4, nr), 1-based; the smoothed lines run to the column's
+    end (row nrdim), past the physical rows."""
+    nrc, npc, nzc, nzcu, nrcs = chop_index(s, kit)
+    nr = kit.nr
+    ns = nr * 3 // 4
+    ns0 = min(ns + 4, nr)
+    so = s.copy()
+    ln = s.ln
+    so.ln = 0.0
+    delsqp(so, kit)
+    rtrans_backward(so, kit)
+    w = ((1.0 - kit.x[:nr]) ** 2)[:, None]
+    act = [kk for kk in range(kit.nzdim) if (kk + 1 <= nzc) or (kk + 1 >= nzcu)]
+    for mm in range(npc):
+        so.e[:nr, mm, act] = so.e[:nr, mm, act] * w
+    so.e[ns0 - 1:, 1:npc, :] = 0.0
+    for mm in range(npc):
+        line = so.e[ns - 1:, mm, act]
+        for _ in range(5):
+            line = smooth(line)
+        so.e[ns - 1:, mm, act] = line
+    for mm in range(npc):
+        so.e[:nr, mm, act] = so.e[:nr, mm, act] / w
+    so.space = 'FFF'
+    rtrans_forward(so, kit)
+    idelsqp(so, kit)
+    so.ln = ln
+    zeroat1(so, kit)
+    s.assign(so)
+
+
 # ----------------------------------------------------------------------------
 # banded operators
 # ----------------------------------------------------------------------------

@@ -443,3 +443,21 @@ def test_vortex_steps(setup):
         assert err < 1e-10, nm
     gp, gc = run.gains()
     assert abs(gp - orun.gain_psi) < 1e-9 and abs(gc - orun.gain_chi) < 1e-9
+
+
+def test_fftreat(setup):
+    """fftreat (mlegs_scalar_ops.f90:1002-1063) on a field with far-field
+    content and a log term, against the oracle."""
+    import mlegs_amd as M
+    from oracle import scalar_ops as so
+    name, p, tfm, okit = setup
+    print('[case] fftreat', name, flush=True)
+    e = C.spectral(tfm.dims, p, 41, decay=20.0)
+    g, o = C.pair(tfm, okit, e, 'FFF', ln=0.37)
+    M.fftreat(g, tfm)
+    so.fftreat(o, okit)
+    err = rel_err(g.e, o.e)
+    print('[case]   rel err %.2e, ln %.17g vs %.17g, changed by %.2e' % (err, g.ln, o.ln, rel_err(o.e, e)),
+          flush=True)
+    assert err < 1e-12 and abs(g.ln - o.ln) < 1e-15
+    assert rel_err(o.e, e) > 1e-6

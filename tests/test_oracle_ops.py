@@ -471,3 +471,77 @@ def test_svv_gain_on_a_two_entry_field():
     assert abs(g - gain) < 1e-15
     assert s.e[0, 0, 0] == 1.0
     assert abs(s.e[15, 0, 0] - 0.1j * math.exp(-0.4 * gain)) < 1e-16
+
+
+# ---------------------------------------------------------------------------
+# fftreat / smooth (mlegs_scalar_ops.f90:1002-1063, 2149-2175)
+# ---------------------------------------------------------------------------
+def test_smooth_conserves_and_stays_local():
+    """smooth is linear, conserves the sum of the line (every source's
+    weights add to one), moves each entry at most one place, and damps an
+    alternating line."""
+    rng = np.random.default_rng(3)
+    for ni in (3, 4, 7, 68):
+        a = rng.standard_normal(ni) + 1j * rng.standard_normal(ni)
+        out = so.smooth(a)
+        assert abs(out.sum() - a.sum()) < 1e-13 * np.abs(a).sum()
+        b = rng.standard_normal(ni)
+        assert np.allclose(so.smooth(2.0 * a + 3.0 * b), 2.0 * out + 3.0 * so.smooth(b), atol=1e-14, rtol=0)
+        for k in range(1, ni - 1):  # an interior source reaches k-1 .. k+1 only
+            e = np.zeros(ni)
+            e[k] = 1.0
+            o = so.smooth(e)
+            assert np.all(o[:max(k - 1, 0)] == 0.0) and np.all(o[k + 2:] == 0.0)
+            assert abs(o[k - 1] - o[k + 1]) < 1e-16 and o[k] >= 0.5 - 1e-15
+    alt = (-1.0) ** np.arange(40)
+    assert np.linalg.norm(so.smooth(alt)[1:-3]) < 0.6 * np.linalg.norm(alt[1:-3])
+    with pytest.raises(ValueError):
+        so.smooth(np.ones(2))
+
+
+def _fft_kit():
+    return _kit(nr=32, np=12, nz=6, nrchop=30, npchop=5, nzchop=3, ell=1.0, zlen=2.5)
+
+
+def test_fftreat_leaves_a_far_decaying_field_alone():
+    """A field whose values past r ~ 2 are below 1e-9 of its size has nothing
+    for the far-field treatment to zero or smooth: fftreat returns it, with
+    its log coefficient."""
+    kit = _fft_kit()
+    s = so.Scalar(kit, 'PPP')
+    s.e = _sample(kit, lambda r, ph, z: np.exp(-4 * r ** 2) * (1 + r * np.cos(ph) * np.cos(2 * np.pi * z / 2.5)
+                                                              + r ** 2 * np.sin(2 * ph)))
+    so.trans(s, 'FFF', kit)
+    so.chop(s, kit)
+    s.ln = 0.3
+    so.zeroat1(s, kit)
+    t = s.copy()
+    so.fftreat(t, kit)
+    assert t.space == 'FFF' and t.ln == 0.3
+    # what remains is the field's own far tail (2e-8 of it at this resolution)
+    assert _rel(t.e, s.e) < 1e-7
+
+
+def test_fftreat_is_linear_and_acts_on_the_far_field():
+    kit = _fft_kit()
+    rng = np.random.default_rng(5)
+    dims = (kit.nrdim, kit.npdim, kit.nzdim)
+    ch = (kit.nrchop, kit.npchop, kit.nzchop, kit.np, kit.nz)
+    f1 = so.Scalar(kit, 'FFF')
+    f1.e = inputs.transform_field(dims, ch, field_id=21, decay=20.0)
+    f1.ln = 0.2
+    f2 = so.Scalar(kit, 'FFF')
+    f2.e = inputs.transform_field(dims, ch, field_id=22, decay=20.0)
+    f2.ln = -0.1
+    c = so.Scalar(kit, 'FFF')
+    c.e = 2.0 * f1.e - 3.0 * f2.e
+    c.ln = 2.0 * f1.ln - 3.0 * f2.ln
+    for s in (f1, f2, c):
+        so.fftreat(s, kit)
+    assert _rel(c.e, 2.0 * f1.e - 3.0 * f2.e) < 1e-12
+    assert abs(c.ln - (2.0 * f1.ln - 3.0 * f2.ln)) < 1e-15
+    # a field with slowly decaying coefficients has far-field content: changed
+    g = so.Scalar(kit, 'FFF')
+    g.e = inputs.transform_field(dims, ch, field_id=21, decay=20.0)
+    g.ln = 0.2
+    assert _rel(f1.e, g.e) > 1e-3
