@@ -12,6 +12,8 @@ Precisions (--prec) of the Legendre contractions; everything else is FP64:
   mixed  (default) FP32 MFMA, except the m = 0 column of vec2tp's projection
          in FP64 MFMA (on a second stream, concurrent with the rest)
   mixedp FP32 MFMA, except vec2tp's whole projection in FP64 MFMA
+  bf16   3-term split-BF16 MFMA (six products, FP32 accumulation), the
+         projection in FP32 except its m = 0 column in FP64 MFMA
   fp32   FP32 MFMA everywhere
   fp64m  FP64 MFMA everywhere
   fp64   the plain FP64 path: FP64 vector-FMA contractions over every column,
@@ -38,8 +40,13 @@ Printed: one JSON line with
                    (fixed per kernel and chop).  The event stamping slows the
                    step, so `value` comes from the first, unprobed region and
                    the probed region's rate is reported as probed_steps_per_s
+  transform_c2     the transform-only workload (one 256^3 field, forward +
+                   backward 1000 times) per precision of trans: round trips/s,
+                   errors against plain FP64 (one backward, one round trip,
+                   after the 1000), the contractions' achieved rate
   cpu_baseline     the oracle (numpy/scipy FP64) timing one AB2/CN step at the
-                   same size on the host
+                   same size on the host, on the job's threads (`cores`) and
+                   on one thread, with the CPU model
 """
 import argparse
 import json
@@ -52,7 +59,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 METRIC = 'steps/s @256³ mixed-prec vs FP64 at ≤1e-6 rel.err; adjoint/forward time ratio'
-PEAK_TFLOPS = {'fp64': 78.6, 'fp32': 157.3}
+PEAK_TFLOPS = {'fp64': 78.6, 'fp32': 157.3, 'bf16': 2500.0}
 HBM_GBPS = 8000.0
 HBM_BOUND = ('fft_phi', 'fft_z', 'lu_solve', 'phi_cross')
 TRAFFIC_FILE = os.path.join(ROOT, 'profiles', 'traffic_256.json')
@@ -93,9 +100,9 @@ def rel(a, b):
     return float(np.linalg.norm((a - b).ravel()) / np.linalg.norm(b.ravel()))
 
 
-PRESETS = ('mixed', 'mixedp', 'fp32', 'fp64m', 'fp64')
+PRESETS = ('mixed', 'mixedp', 'bf16', 'fp32', 'fp64m', 'fp64')
 DTYPES = {'fp64': 'f64', 'fp64m': 'f64-mfma', 'fp32': 'f32-mfma+f64', 'mixed': 'f32-mfma+f64-mfma',
-          'mixedp': 'f32-mfma+f64-mfma'}
+          'mixedp': 'f32-mfma+f64-mfma', 'bf16': 'bf16x3-mfma+f64-mfma', 'bf16x3': 'bf16x3-mfma+f64'}
 GATE_TOL = 1e-6
 
 
@@ -103,11 +110,11 @@ def configure(tfm, name):
     """Contraction precisions of a preset (context precision + site overrides)."""
     import mlegs_amd as M
     base = {'fp64': M.PREC_FP64, 'fp32': M.PREC_FP32, 'fp64m': M.PREC_FP64_MFMA, 'mixed': M.PREC_FP32,
-            'mixedp': M.PREC_FP32}[name]
+            'mixedp': M.PREC_FP32, 'bf16': M.PREC_BF16X3, 'bf16x3': M.PREC_BF16X3}[name]
     tfm.set_precision(base)
     for st in (M.STAGE_SYNTH, M.STAGE_UP, M.STAGE_PROJ, M.STAGE_PROJ0):
         tfm.set_stage_precision(st, -1)
-    if name in ('mixed', 'mixedp'):
+    if name in ('mixed', 'mixedp', 'bf16'):
         tfm.set_stage_precision(M.STAGE_PROJ0, M.PREC_FP64_MFMA)
     if name == 'mixedp':
         tfm.set_stage_precision(M.STAGE_PROJ, M.PREC_FP64_MFMA)
@@ -215,6 +222,8 @@ def transform_error(tfm, p, name):
 def family_precision(name, family):
     if name in ('fp64', 'fp64m'):
         return 'fp64'
+    if name in ('bf16', 'bf16x3') and family in ('leg_fwd', 'leg_bwd'):
+        return 'bf16'
     if name == 'mixedp' and family == 'v2t':
         return 'fp64'
     return 'fp32' if family in ('leg_fwd', 'leg_bwd', 'v2t') else 'fp64'
@@ -263,15 +272,32 @@ def probe_table(probes):
     return out
 
 
-def cpu_baseline(p):
-    """The oracle (numpy/scipy, FP64) on the host: one AB2/CN step at the workload's
-    size from a synthetic state whose history (psi_prev, N, N_prev) is set
-    directly -- the step's cost does not depend on the values, so the
-    bootstrap is not run.  A bounded sample of the workload: one step."""
-    from threadpoolctl import threadpool_info
-    from oracle import kit as K, scalar_ops as so, vortex as V
+def host_info():
+    """CPU model, the host's logical CPUs, and the threads this process may use
+    (the job's CPU share: OMP_NUM_THREADS when set, else the affinity mask)."""
+    model = None
+    try:
+        with open('/proc/cpuinfo') as f:
+            for line in f:
+                if line.startswith('model name'):
+                    model = line.split(':', 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    avail = len(os.sched_getaffinity(0)) if hasattr(os, 'sched_getaffinity') else (os.cpu_count() or 1)
+    env = os.environ.get('OMP_NUM_THREADS')
+    threads = min(int(env), avail) if env and env.isdigit() and int(env) > 0 else avail
+    return model, os.cpu_count(), avail, threads
+
+
+def cpu_step_seconds(okit, p, threads):
+    """One AB2/CN step of the oracle at the workload's size with `threads`
+    BLAS threads and FFT workers, from a synthetic state whose history
+    (psi_prev, N, N_prev) is set directly: the step's cost does not depend on
+    the values, so the bootstrap is not run."""
+    from threadpoolctl import threadpool_limits
+    from oracle import scalar_ops as so, vortex as V
     from mlegs_amd import inputs
-    okit = K.Kit3D(p)
     dims = (okit.nrdim, okit.npdim, okit.nzdim)
     ch = (p.nrchop, p.npchop, p.nzchop, p.np, p.nz)
     psi = so.Scalar(okit, 'FFF')
@@ -282,13 +308,83 @@ def cpu_baseline(p):
     run.psi_prev, run.chi_prev = psi.copy(), chi.copy()
     run.nlpsi, run.nlchi = so.Scalar(okit, 'FFF'), so.Scalar(okit, 'FFF')
     run.nlpsi_prev, run.nlchi_prev = so.Scalar(okit, 'FFF'), so.Scalar(okit, 'FFF')
-    t0 = time.perf_counter()
-    run.step()
-    t = time.perf_counter() - t0
-    threads = max([i.get('num_threads', 1) for i in threadpool_info()] + [1])
-    return {'value': 1.0 / t, 'unit': 'steps/s', 'cores': threads, 'kind': 'port',
-            'sample': 'oracle AB2/CN step (numpy/scipy FP64, BLAS on %d threads, FFTs single-threaded) at '
-                      'NR=NP=NZ=%d, one step timed: %.2f s' % (threads, p.nr, t)}
+    saved = so.FFT_WORKERS
+    so.FFT_WORKERS = threads
+    try:
+        with threadpool_limits(limits=threads):
+            t0 = time.perf_counter()
+            run.step()
+            return time.perf_counter() - t0
+    finally:
+        so.FFT_WORKERS = saved
+
+
+def cpu_baseline(p, one_core=True):
+    """The oracle (numpy/scipy FP64) on the host's cores: a bounded sample of
+    the workload, one AB2/CN step, on all the threads the job may use and on
+    one thread."""
+    from oracle import kit as K
+    model, ncpu, avail, threads = host_info()
+    okit = K.Kit3D(p)
+    t_all = cpu_step_seconds(okit, p, threads)
+    out = {'value': 1.0 / t_all, 'unit': 'steps/s', 'cores': threads, 'kind': 'port',
+           'cpu_model': model, 'host_logical_cpus': ncpu, 'affinity_cpus': avail,
+           'sample': 'oracle AB2/CN step (numpy/scipy FP64; BLAS threads and scipy.fft workers = %d) at '
+                     'NR=NP=NZ=%d, one step timed: %.2f s' % (threads, p.nr, t_all)}
+    if one_core:
+        t1 = cpu_step_seconds(okit, p, 1)
+        out['one_core'] = {'value': 1.0 / t1, 'seconds_per_step': t1}
+    return out
+
+
+def transform_bench(tfm, p, names, n_rt, iters_probe=10):
+    """The transform-only workload: one 256^3 field, |c| ~ N(0,1) exp(-n/32),
+    forward + backward (FFF -> PPP -> FFF) n_rt times per precision of trans.
+    Errors against the plain FP64 path: one backward transform, one round
+    trip, and after the n_rt round trips.  A probed region of iters_probe
+    round trips gives each contraction's time and achieved rate."""
+    import mlegs_amd as M
+    from mlegs_amd import inputs
+    e = inputs.transform_field(tfm.dims, (p.nrchop, p.npchop, p.nzchop, p.np, p.nz), field_id=3)
+    out, ref = {}, None
+    for nm in names:
+        configure(tfm, nm)
+        g = M.Scalar(tfm).upload(e, space='FFF')
+        M.trans(g, 'PPP', tfm)
+        b1 = g.e[:p.nr, :p.np // 2].copy()
+        M.trans(g, 'FFF', tfm)
+        r1 = g.e.copy()
+        for _ in range(2):
+            M.trans(g, 'PPP', tfm)
+            M.trans(g, 'FFF', tfm)
+        g = M.Scalar(tfm).upload(e, space='FFF')
+        tfm.sync()
+        t0 = time.perf_counter()
+        for _ in range(n_rt):
+            M.trans(g, 'PPP', tfm)
+            M.trans(g, 'FFF', tfm)
+        tfm.sync()
+        t = time.perf_counter() - t0
+        rn = g.e.copy()
+        tfm.probe_start(16 * iters_probe)
+        for _ in range(iters_probe):
+            M.trans(g, 'PPP', tfm)
+            M.trans(g, 'FFF', tfm)
+        probes = tfm.probe_stop()
+        if nm == 'fp64':
+            ref = (b1, r1, rn)
+        res = {'round_trips_per_s': n_rt / t, 'ms_per_round_trip': 1e3 * t / n_rt, 'dtype': DTYPES[nm],
+               'kernels': probe_table(probes)}
+        if ref is not None and nm != 'fp64':
+            res['rel_err_vs_fp64'] = {'backward': rel(b1, ref[0]), 'round_trip': rel(r1, ref[1]),
+                                      'after_%d_round_trips' % n_rt: rel(rn, ref[2])}
+        leg = {k: v for k, v in probes.items() if k in ('leg_fwd', 'leg_bwd') and v[0] > 0}
+        if leg:
+            res['roofline'] = roofline(leg, nm)
+        out[nm] = res
+        log('transform', nm, '%.1f round trips/s' % res['round_trips_per_s'])
+        del g
+    return out
 
 
 def kernel_sample(tfm, p, name, iters):
@@ -318,6 +414,9 @@ def main():
     ap.add_argument('--traj-steps', type=int, default=500, help='length of the gate trajectory')
     ap.add_argument('--ke-every', type=int, default=50, help='perturbation-energy sampling interval')
     ap.add_argument('--others', type=int, default=1, help='also time the other precisions')
+    ap.add_argument('--transform', type=int, default=1000,
+                    help='round trips of the transform-only workload per precision (0: skip)')
+    ap.add_argument('--cpu-one-core', type=int, default=1, help='also time the CPU baseline on one thread')
     ap.add_argument('--compare-fp64', type=int, default=1,
                     help='also time the plain FP64 path (0: the selected precision only, e.g. under a profiler)')
     ap.add_argument('--kernel-sample', type=int, default=0,
@@ -401,6 +500,11 @@ def main():
         if not gate['pass']:
             log('WARNING: %s misses the accuracy gate (trajectory error above %.0e)' % (name, GATE_TOL))
 
+    transform = None
+    if args.transform:
+        log('transform-only workload: %d round trips per precision' % args.transform)
+        transform = transform_bench(tfm, p, ['fp64', 'fp32', 'bf16x3', 'fp64m'], args.transform)
+
     # the probed region: as many steps again, continuing the run
     configure(tfm, name)
     tfm.sync()
@@ -421,7 +525,7 @@ def main():
     cpu = None
     if args.cpu_baseline:
         log('cpu baseline (oracle step)')
-        cpu = cpu_baseline(p)
+        cpu = cpu_baseline(p, one_core=bool(args.cpu_one_core))
 
     out = {
         'metric': METRIC, 'value': rate, 'unit': 'steps/s', 'n_gpus': 1, 'steps': args.steps,
@@ -435,6 +539,7 @@ def main():
         'fp64_steps_per_s': fp64_rate, 'speedup_vs_fp64': rate / fp64_rate if fp64_rate else None,
         'rel_err_vs_fp64': err, 'ke_history': ke, 'other_precisions': others,
         'adjoint_forward_ratio': None, 'probed_steps_per_s': probed_rate,
+        'transform_c2': transform,
         'kernels': probe_table(probes),
         'roofline': roof, 'cpu_baseline': cpu,
     }

@@ -51,8 +51,11 @@ extern "C" {
 #define MLX_PREC_FP64   0    /* FP64 operands, FP64 accumulation (vector
                                 FMA), every column computed: the plain path */
 #define MLX_PREC_FP32   1    /* FP32 operands, FP32 MFMA accumulation       */
-#define MLX_PREC_BF16X3 2    /* 3-term split-BF16 on MFMA: reserved; rejected
-                                (MLX_E_ARG) by this build                   */
+#define MLX_PREC_BF16X3 2    /* 3-term split-BF16 on MFMA: each FP32-staged
+                                operand split into three BF16 terms, six
+                                products accumulated in FP32
+                                (v_mfma_f32_16x16x16_bf16); the vec2tp
+                                projection runs in FP32                    */
 #define MLX_PREC_FP64_MFMA 3 /* FP64 operands, FP64 MFMA accumulation
                                 (v_mfma_f64_16x16x4_f64)                    */
 
@@ -127,9 +130,9 @@ int mlx_kit_dims(mlx_ctx* ctx, int32_t* out9);
 int mlx_kit_get(mlx_ctx* ctx, const char* which, double* host, int64_t n);
 
 /* Legendre-contraction precision used by mlx_trans and mlx_vec2tp:
- * MLX_PREC_FP64, MLX_PREC_FP32 or MLX_PREC_FP64_MFMA (MLX_PREC_BF16X3
- * returns MLX_E_ARG).  Every precision but MLX_PREC_FP64 runs the MFMA
- * kernels, and the vortex step then skips the modes de-aliasing zeroes. */
+ * MLX_PREC_FP64, MLX_PREC_FP32, MLX_PREC_FP64_MFMA or MLX_PREC_BF16X3.
+ * Every precision but MLX_PREC_FP64 runs the MFMA kernels, and the vortex
+ * step then skips the modes de-aliasing zeroes. */
 int mlx_set_precision(mlx_ctx* ctx, int prec);
 /* Precision of one contraction site (MLX_STAGE_*) apart from the context's;
  * prec = -1 makes the site follow mlx_set_precision again.  No counterpart

@@ -403,7 +403,7 @@ bool tp2vec_back(mlx_ctx *c, mlx_field *ur, mlx_field *up, mlx_field *uz, int nc
   // physical rows only, on the MFMA path: the radial transform hands the
   // phi transform its FP32 even/odd sums (half the bytes of the FP64 rows,
   // which the phi transform forms from them exactly)
-  const bool pairs = lean && phys_rows && c->prec == MLX_PREC_FP32 && mfma_ok(k, 0);
+  const bool pairs = lean && phys_rows && (c->prec == MLX_PREC_FP32 || c->prec == MLX_PREC_BF16X3) && mfma_ok(k, 0);
   int q = 0;
   for (mlx_field *f : {ur, up, uz}) {
     f->off[0] = f->off[1] = f->off[2] = 0;
@@ -804,9 +804,7 @@ int mlx_kit_get(mlx_ctx *c, const char *which, double *host, int64_t n) {
 
 int mlx_set_precision(mlx_ctx *c, int prec) {
   return guarded(c, [&] {
-    if (prec == MLX_PREC_BF16X3)
-      MLX_THROW(MLX_E_ARG, "mlx_set_precision: the split-BF16 contractions are not part of this build");
-    if (prec != MLX_PREC_FP64 && prec != MLX_PREC_FP32 && prec != MLX_PREC_FP64_MFMA)
+    if (prec != MLX_PREC_FP64 && prec != MLX_PREC_FP32 && prec != MLX_PREC_FP64_MFMA && prec != MLX_PREC_BF16X3)
       MLX_THROW(MLX_E_ARG, "mlx_set_precision: unknown precision");
     c->prec = c->prec_ctx = prec;
   });
@@ -815,9 +813,8 @@ int mlx_set_precision(mlx_ctx *c, int prec) {
 int mlx_set_stage_precision(mlx_ctx *c, int stage, int prec) {
   return guarded(c, [&] {
     if (stage < 0 || stage >= MLX_STAGE_COUNT) MLX_THROW(MLX_E_ARG, "mlx_set_stage_precision: unknown stage");
-    if (prec == MLX_PREC_BF16X3)
-      MLX_THROW(MLX_E_ARG, "mlx_set_stage_precision: the split-BF16 contractions are not part of this build");
-    if (prec != -1 && prec != MLX_PREC_FP64 && prec != MLX_PREC_FP32 && prec != MLX_PREC_FP64_MFMA)
+    if (prec != -1 && prec != MLX_PREC_FP64 && prec != MLX_PREC_FP32 && prec != MLX_PREC_FP64_MFMA &&
+        prec != MLX_PREC_BF16X3)
       MLX_THROW(MLX_E_ARG, "mlx_set_stage_precision: unknown precision");
     c->stage_prec[stage] = prec;
   });

@@ -39,16 +39,33 @@ constexpr int BJP = BJ + 16;  // padded LDS row (elements)
 
 // the MFMA of one operand type: 16x16 accumulator (4 per lane), 16-byte
 // table loads of VE elements
+// Bf3: 3-term split-BF16.  Operands are staged in FP32 (from FP64, and the
+// FP32 tables); at the MFMA each FP32 value x is split exactly into three
+// BF16 terms h + m + l (h = bf16(x), m = bf16(x - h), l = bf16(x - h - m):
+// 24 significant bits), and the six products h h, h m, m h, h l, m m, l h
+// accumulate in FP32 on v_mfma_f32_16x16x16_bf16 (products of BF16 pairs
+// are exact in FP32; the dropped m l, l m, l l are below 2^-24).
+struct Bf3 {};
+typedef short shortx4 __attribute__((ext_vector_type(4)));
+
+// Per operand type: the MFMA accumulator, the 16-byte table load, the LDS
+// staging type, the contraction chunk (KCH) staged per barrier, the k step of
+// one instruction (KSTEP) and the values a lane feeds it per operand (NV, at
+// rows kk + NV (l/16) + j of the chunk), the packed operand (pk) and its MFMA.
 template <typename T>
 struct Mma;
 template <>
 struct Mma<float> {
   typedef floatx4 acc;
   typedef float4 vec;
-  static constexpr int VE = 4;
+  typedef float stage;
+  typedef float pk;
+  static constexpr int VE = 4, KCH = 8, KSTEP = 4, NV = 1;
   __device__ static inline acc op(float a, float b, acc c) {
     return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
   }
+  __device__ static inline pk prep(const stage *v) { return v[0]; }
+  __device__ static inline acc mma(pk a, pk b, acc c) { return op(a, b, c); }
   __device__ static inline acc zero() { return acc{0.f, 0.f, 0.f, 0.f}; }
   __device__ static inline vec vzero() { return make_float4(0.f, 0.f, 0.f, 0.f); }
   // field column that A row i carries (accumulator element r of lane l is row 4(l/16)+r)
@@ -58,16 +75,63 @@ template <>
 struct Mma<double> {
   typedef doublex4 acc;
   typedef double2 vec;
-  static constexpr int VE = 2;
+  typedef double stage;
+  typedef double pk;
+  static constexpr int VE = 2, KCH = 8, KSTEP = 4, NV = 1;
   __device__ static inline acc op(double a, double b, acc c) {
     return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
   }
+  __device__ static inline pk prep(const stage *v) { return v[0]; }
+  __device__ static inline acc mma(pk a, pk b, acc c) { return op(a, b, c); }
   __device__ static inline acc zero() { return acc{0.0, 0.0, 0.0, 0.0}; }
   __device__ static inline vec vzero() { return make_double2(0.0, 0.0); }
   // accumulator element r of lane l is row l/16 + 4r: A row i carries field
   // column 4(i%4) + i/4, which puts column 4(l/16)+r in that element
   __device__ static inline int arow(int i) { return 4 * (i & 3) + (i >> 2); }
 };
+template <>
+struct Mma<Bf3> {
+  typedef floatx4 acc;
+  typedef float4 vec;
+  typedef float stage;
+  struct pk {
+    shortx4 h, m, l;
+  };
+  static constexpr int VE = 4, KCH = 16, KSTEP = 16, NV = 4;
+  __device__ static inline acc zero() { return acc{0.f, 0.f, 0.f, 0.f}; }
+  __device__ static inline vec vzero() { return make_float4(0.f, 0.f, 0.f, 0.f); }
+  // the BF16 accumulator layout is the FP32 one (element r of lane l is row 4(l/16)+r)
+  __device__ static inline int arow(int i) { return i; }
+  __device__ static inline short bits(__bf16 x) { return __builtin_bit_cast(short, x); }
+  __device__ static inline pk prep(const stage *v) {
+    pk o;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const float x = v[j];
+      const __bf16 h = (__bf16)x;
+      const float r1 = x - (float)h;
+      const __bf16 m = (__bf16)r1;
+      const __bf16 l = (__bf16)(r1 - (float)m);
+      o.h[j] = bits(h);
+      o.m[j] = bits(m);
+      o.l[j] = bits(l);
+    }
+    return o;
+  }
+  __device__ static inline acc mm1(shortx4 a, shortx4 b, acc c) {
+    return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
+  }
+  // the small products first
+  __device__ static inline acc mma(const pk &a, const pk &b, acc c) {
+    c = mm1(a.l, b.h, c);
+    c = mm1(a.m, b.m, c);
+    c = mm1(a.h, b.l, c);
+    c = mm1(a.m, b.h, c);
+    c = mm1(a.h, b.m, c);
+    return mm1(a.h, b.h, c);
+  }
+};
+
 
 // ---------------------------------------------------------------------------
 // forward: out(2a+p, m, k) = sum_x P_p(a, x) fold_p(x, k)
@@ -75,16 +139,18 @@ struct Mma<double> {
 // ---------------------------------------------------------------------------
 template <typename T, int AT>  // 16-row tiles per wave: AP = 4 * AT * 16
 __global__ void __launch_bounds__(256) k_leg_fwd_mf(const double2 *__restrict__ in, double2 *__restrict__ out,
-                                                    const T *__restrict__ pfg, const double *__restrict__ w,
+                                                    const typename Mma<T>::stage *__restrict__ pfg, const double *__restrict__ w,
                                                      int nrdim, int npdim, int nz, int nr, int nrh, int nrc,
                                                      int npc, int nyt) {
   constexpr int AP = 4 * AT * 16;
   constexpr int APP = AP + 16;
   typedef Mma<T> M;
   typedef typename M::vec vec;
+  typedef typename M::stage S;
   constexpr int VE = M::VE;
-  __shared__ T Bs[2][KC][BJP];
-  __shared__ T As[2][KC][APP];
+  constexpr int KC_ = M::KCH;
+  __shared__ S Bs[2][KC_][BJP];
+  __shared__ S As[2][KC_][APP];
   int m, yt;
   xcd_map(blockIdx.x, nyt, m, yt);
   if (m >= npdim) return;
@@ -102,7 +168,7 @@ __global__ void __launch_bounds__(256) k_leg_fwd_mf(const double2 *__restrict__ 
   const int nn = nrc - m > 0 ? nrc - m : 0;
   const int Mmax = (nn + 1) / 2;
   const double2 *icol = in + (size_t)nrdim * m;
-  const T *Tg = pfg + (size_t)m * 2 * nrh * AP;
+  const S *Tg = pfg + (size_t)m * 2 * nrh * AP;
   typename M::acc acc[AT][4][2];
 #pragma unroll
   for (int s = 0; s < AT; ++s)
@@ -113,10 +179,10 @@ __global__ void __launch_bounds__(256) k_leg_fwd_mf(const double2 *__restrict__ 
 
   // Software pipeline: the next chunk's table and field loads are issued
   // into registers before the current chunk's MFMA work and stored to LDS
-  // after it.  Table: 2 KC AP / VE 16-byte loads per chunk; field: KC KT points,
+  // after it.  Table: 2 KC_ AP / VE 16-byte loads per chunk; field: KC_ KT points,
   // each the pair x, nr-1-x and the weight, folded when stored.
-  constexpr int NTA = (2 * KC * (AP / VE) + 255) / 256;
-  constexpr int NFA = (KC * KT + 255) / 256;
+  constexpr int NTA = (2 * KC_ * (AP / VE) + 255) / 256;
+  constexpr int NFA = (KC_ * KT + 255) / 256;
   vec rt[NTA];
   double2 ra[NFA], rb[NFA];
   double rw[NFA];
@@ -125,8 +191,8 @@ __global__ void __launch_bounds__(256) k_leg_fwd_mf(const double2 *__restrict__ 
     for (int q = 0; q < NTA; ++q) {
       const int idx = tid + 256 * q;
       vec v = M::vzero();
-      if (idx < 2 * KC * (AP / VE)) {
-        const int a = VE * (idx % (AP / VE)), x = (idx / (AP / VE)) % KC, p = idx / ((AP / VE) * KC);
+      if (idx < 2 * KC_ * (AP / VE)) {
+        const int a = VE * (idx % (AP / VE)), x = (idx / (AP / VE)) % KC_, p = idx / ((AP / VE) * KC_);
         const int xg = x0 + x;
         if (xg < nrh) v = *reinterpret_cast<const vec *>(Tg + ((size_t)p * nrh + xg) * AP + a);
       }
@@ -135,11 +201,11 @@ __global__ void __launch_bounds__(256) k_leg_fwd_mf(const double2 *__restrict__ 
 #pragma unroll
     for (int q = 0; q < NFA; ++q) {
       const int idx = tid + 256 * q;
-      const int x = idx % KC, kl = idx / KC;
+      const int x = idx % KC_, kl = idx / KC_;
       const int xg = x0 + x, kg = k0 + kl;
       ra[q] = rb[q] = make_double2(0.0, 0.0);
       rw[q] = 0.0;
-      if (idx < KC * KT && xg < nrh && kg < nz) {
+      if (idx < KC_ * KT && xg < nrh && kg < nz) {
         ra[q] = icol[(size_t)kg * zs + xg];
         rb[q] = icol[(size_t)kg * zs + (nr - 1 - xg)];
         rw[q] = w[xg];
@@ -147,45 +213,53 @@ __global__ void __launch_bounds__(256) k_leg_fwd_mf(const double2 *__restrict__ 
     }
   };
   fetch(0);
-  for (int x0 = 0; x0 < nrh; x0 += KC) {
+  for (int x0 = 0; x0 < nrh; x0 += KC_) {
 #pragma unroll
     for (int q = 0; q < NTA; ++q) {
       const int idx = tid + 256 * q;
-      if (idx < 2 * KC * (AP / VE)) {
-        const int a = VE * (idx % (AP / VE)), x = (idx / (AP / VE)) % KC, p = idx / ((AP / VE) * KC);
+      if (idx < 2 * KC_ * (AP / VE)) {
+        const int a = VE * (idx % (AP / VE)), x = (idx / (AP / VE)) % KC_, p = idx / ((AP / VE) * KC_);
         *reinterpret_cast<vec *>(&As[p][x][a]) = rt[q];
       }
     }
 #pragma unroll
     for (int q = 0; q < NFA; ++q) {
       const int idx = tid + 256 * q;
-      if (idx < KC * KT) {
-        const int x = idx % KC, kl = idx / KC;
+      if (idx < KC_ * KT) {
+        const int x = idx % KC_, kl = idx / KC_;
         const double2 a = ra[q], b = rb[q];
         const double wx = rw[q];
-        Bs[0][x][2 * kl] = (T)((a.x + b.x) * wx);
-        Bs[0][x][2 * kl + 1] = (T)((a.y + b.y) * wx);
-        Bs[1][x][2 * kl] = (T)((a.x - b.x) * wx);
-        Bs[1][x][2 * kl + 1] = (T)((a.y - b.y) * wx);
+        Bs[0][x][2 * kl] = (S)((a.x + b.x) * wx);
+        Bs[0][x][2 * kl + 1] = (S)((a.y + b.y) * wx);
+        Bs[1][x][2 * kl] = (S)((a.x - b.x) * wx);
+        Bs[1][x][2 * kl + 1] = (S)((a.y - b.y) * wx);
       }
     }
     __syncthreads();
-    if (x0 + KC < nrh) fetch(x0 + KC);
+    if (x0 + KC_ < nrh) fetch(x0 + KC_);
 #pragma unroll 2
-    for (int kk = 0; kk < KC; kk += 4) {
-      const int xr = kk + lane / 16;
+    for (int kk = 0; kk < KC_; kk += M::KSTEP) {
+      const int xr = kk + M::NV * (lane / 16);
 #pragma unroll
       for (int p = 0; p < 2; ++p) {
-        T fa[4];
+        typename M::pk fa[4];
 #pragma unroll
-        for (int t = 0; t < 4; ++t) fa[t] = Bs[p][xr][16 * t + M::arow(lane % 16)];
+        for (int t = 0; t < 4; ++t) {
+          S v[M::NV];
+#pragma unroll
+          for (int j = 0; j < M::NV; ++j) v[j] = Bs[p][xr + j][16 * t + M::arow(lane % 16)];
+          fa[t] = M::prep(v);
+        }
 #pragma unroll
         for (int s = 0; s < AT; ++s) {
           const int at = wv + 4 * s;
           if (at * 16 >= Mmax) continue;
-          T fb = As[p][xr][at * 16 + lane % 16];
+          S v[M::NV];
 #pragma unroll
-          for (int t = 0; t < 4; ++t) acc[s][t][p] = M::op(fa[t], fb, acc[s][t][p]);
+          for (int j = 0; j < M::NV; ++j) v[j] = As[p][xr + j][at * 16 + lane % 16];
+          const typename M::pk fb = M::prep(v);
+#pragma unroll
+          for (int t = 0; t < 4; ++t) acc[s][t][p] = M::mma(fa[t], fb, acc[s][t][p]);
         }
       }
     }
@@ -231,17 +305,19 @@ __global__ void __launch_bounds__(256) k_leg_fwd_mf(const double2 *__restrict__ 
 // ---------------------------------------------------------------------------
 template <typename T, int XT, bool PAIRS>  // 16-column tiles of x per wave: XP = 4 * XT * 16 >= nrh
 __global__ void __launch_bounds__(256) k_leg_bwd_mf(const double2 *__restrict__ in, double2 *__restrict__ out,
-                                                    float4 *__restrict__ pairs, const T *__restrict__ ptab,
+                                                    float4 *__restrict__ pairs, const typename Mma<T>::stage *__restrict__ ptab,
                                                      int nrdim, int npdim, int nz, int nr, int nrh, int ne, int nrc,
                                                      int npc, int nyt, int pstride) {
   constexpr int XP = 4 * XT * 16;
   constexpr int XPP = XP + 16;
   typedef Mma<T> M;
   typedef typename M::vec vec;
+  typedef typename M::stage S;
   constexpr int VE = M::VE;
-  static_assert(!PAIRS || sizeof(T) == 4, "radial pairs are FP32 sums");
-  __shared__ T Es[2][KC][BJP];
-  __shared__ T Ps[2][KC][XPP];
+  constexpr int KC_ = M::KCH;
+  static_assert(!PAIRS || sizeof(S) == 4, "radial pairs are FP32 sums");
+  __shared__ S Es[2][KC_][BJP];
+  __shared__ S Ps[2][KC_][XPP];
   int m, yt;
   xcd_map(blockIdx.x, nyt, m, yt);
   if (m >= npdim) return;
@@ -260,7 +336,7 @@ __global__ void __launch_bounds__(256) k_leg_bwd_mf(const double2 *__restrict__ 
   const int nn = nrc - m > 0 ? nrc - m : 0;
   const int Me = (nn + 1) / 2;
   const double2 *icol = in + (size_t)nrdim * m;
-  const T *P = ptab + (size_t)m * ne * nrh;
+  const S *P = ptab + (size_t)m * ne * nrh;
   typename M::acc acc[XT][4][2];
 #pragma unroll
   for (int s = 0; s < XT; ++s)
@@ -272,40 +348,48 @@ __global__ void __launch_bounds__(256) k_leg_bwd_mf(const double2 *__restrict__ 
   // MFMA work of one staged chunk (both parities)
   auto mfma_chunk = [&]() {
 #pragma unroll 2
-    for (int kk = 0; kk < KC; kk += 4) {
-      const int ar = kk + lane / 16;
+    for (int kk = 0; kk < KC_; kk += M::KSTEP) {
+      const int ar = kk + M::NV * (lane / 16);
 #pragma unroll
       for (int p = 0; p < 2; ++p) {
-        T fa[4];
+        typename M::pk fa[4];
 #pragma unroll
-        for (int t = 0; t < 4; ++t) fa[t] = Es[p][ar][16 * t + M::arow(lane % 16)];
+        for (int t = 0; t < 4; ++t) {
+          S v[M::NV];
+#pragma unroll
+          for (int j = 0; j < M::NV; ++j) v[j] = Es[p][ar + j][16 * t + M::arow(lane % 16)];
+          fa[t] = M::prep(v);
+        }
 #pragma unroll
         for (int s = 0; s < XT; ++s) {
           const int xt = wv + 4 * s;
           if (xt * 16 >= nrh) continue;
-          T fb = Ps[p][ar][xt * 16 + lane % 16];
+          S v[M::NV];
 #pragma unroll
-          for (int t = 0; t < 4; ++t) acc[s][t][p] = M::op(fa[t], fb, acc[s][t][p]);
+          for (int j = 0; j < M::NV; ++j) v[j] = Ps[p][ar + j][xt * 16 + lane % 16];
+          const typename M::pk fb = M::prep(v);
+#pragma unroll
+          for (int t = 0; t < 4; ++t) acc[s][t][p] = M::mma(fa[t], fb, acc[s][t][p]);
         }
       }
     }
   };
-  // field chunk rows 2 a0 .. 2 a0 + 2 KC - 1 of 32 slices (FP64 -> T)
+  // field chunk rows 2 a0 .. 2 a0 + 2 KC_ - 1 of 32 slices (FP64 -> T)
   auto field_at = [&](int a0, int idx) {
-    const int r = idx % (2 * KC), kl = idx / (2 * KC);
+    const int r = idx % (2 * KC_), kl = idx / (2 * KC_);
     const int n = 2 * a0 + r, kg = k0 + kl;
     return (n < nn && kg < nz) ? icol[(size_t)kg * zs + n] : make_double2(0.0, 0.0);
   };
   auto field_put = [&](int idx, double2 v) {
-    const int r = idx % (2 * KC), kl = idx / (2 * KC);
-    Es[r & 1][r >> 1][2 * kl] = (T)v.x;
-    Es[r & 1][r >> 1][2 * kl + 1] = (T)v.y;
+    const int r = idx % (2 * KC_), kl = idx / (2 * KC_);
+    Es[r & 1][r >> 1][2 * kl] = (S)v.x;
+    Es[r & 1][r >> 1][2 * kl + 1] = (S)v.y;
   };
   if ((nrh % VE) == 0) {
     // software pipeline: the next chunk's global loads are in flight while
     // the current chunk runs on MFMA; VE consecutive x per 16-byte load
-    constexpr int NTB = (2 * KC * (XP / VE) + 255) / 256;
-    constexpr int NFB = (2 * KC * KT + 255) / 256;
+    constexpr int NTB = (2 * KC_ * (XP / VE) + 255) / 256;
+    constexpr int NFB = (2 * KC_ * KT + 255) / 256;
     vec rt[NTB];
     double2 rf[NFB];
     auto fetch = [&](int a0) {
@@ -313,8 +397,8 @@ __global__ void __launch_bounds__(256) k_leg_bwd_mf(const double2 *__restrict__ 
       for (int q = 0; q < NTB; ++q) {
         const int idx = tid + 256 * q;
         vec v = M::vzero();
-        if (idx < 2 * KC * (XP / VE)) {
-          const int x = VE * (idx % (XP / VE)), a = (idx / (XP / VE)) % KC, p = idx / ((XP / VE) * KC);
+        if (idx < 2 * KC_ * (XP / VE)) {
+          const int x = VE * (idx % (XP / VE)), a = (idx / (XP / VE)) % KC_, p = idx / ((XP / VE) * KC_);
           const int n = 2 * (a0 + a) + p;
           if (x < nrh && n < nn) v = *reinterpret_cast<const vec *>(P + (size_t)n * nrh + x);
         }
@@ -323,37 +407,37 @@ __global__ void __launch_bounds__(256) k_leg_bwd_mf(const double2 *__restrict__ 
 #pragma unroll
       for (int q = 0; q < NFB; ++q) {
         const int idx = tid + 256 * q;
-        rf[q] = (idx < 2 * KC * KT) ? field_at(a0, idx) : make_double2(0.0, 0.0);
+        rf[q] = (idx < 2 * KC_ * KT) ? field_at(a0, idx) : make_double2(0.0, 0.0);
       }
     };
     fetch(0);
-    for (int a0 = 0; a0 < Me; a0 += KC) {
+    for (int a0 = 0; a0 < Me; a0 += KC_) {
 #pragma unroll
       for (int q = 0; q < NTB; ++q) {
         const int idx = tid + 256 * q;
-        if (idx < 2 * KC * (XP / VE)) {
-          const int x = VE * (idx % (XP / VE)), a = (idx / (XP / VE)) % KC, p = idx / ((XP / VE) * KC);
+        if (idx < 2 * KC_ * (XP / VE)) {
+          const int x = VE * (idx % (XP / VE)), a = (idx / (XP / VE)) % KC_, p = idx / ((XP / VE) * KC_);
           *reinterpret_cast<vec *>(&Ps[p][a][x]) = rt[q];
         }
       }
 #pragma unroll
       for (int q = 0; q < NFB; ++q) {
         const int idx = tid + 256 * q;
-        if (idx < 2 * KC * KT) field_put(idx, rf[q]);
+        if (idx < 2 * KC_ * KT) field_put(idx, rf[q]);
       }
       __syncthreads();
-      if (a0 + KC < Me) fetch(a0 + KC);
+      if (a0 + KC_ < Me) fetch(a0 + KC_);
       mfma_chunk();
       __syncthreads();
     }
   } else {
-    for (int a0 = 0; a0 < Me; a0 += KC) {
-      for (int idx = tid; idx < 2 * KC * XP; idx += 256) {
-        int x = idx % XP, a = (idx / XP) % KC, p = idx / (XP * KC);
+    for (int a0 = 0; a0 < Me; a0 += KC_) {
+      for (int idx = tid; idx < 2 * KC_ * XP; idx += 256) {
+        int x = idx % XP, a = (idx / XP) % KC_, p = idx / (XP * KC_);
         int n = 2 * (a0 + a) + p;
-        Ps[p][a][x] = (x < nrh && n < nn) ? P[(size_t)n * nrh + x] : (T)0;
+        Ps[p][a][x] = (x < nrh && n < nn) ? P[(size_t)n * nrh + x] : (S)0;
       }
-      for (int idx = tid; idx < 2 * KC * KT; idx += 256) field_put(idx, field_at(a0, idx));
+      for (int idx = tid; idx < 2 * KC_ * KT; idx += 256) field_put(idx, field_at(a0, idx));
       __syncthreads();
       mfma_chunk();
       __syncthreads();
@@ -647,12 +731,15 @@ void mfma_tables(mlx_ctx *c) {
 }
 
 namespace {
-// the operand type of the MFMA path at the context's current precision
-bool mfma_fp64(const mlx_ctx *c) {
-  if (c->prec == MLX_PREC_FP64_MFMA) return true;
-  if (c->prec == MLX_PREC_FP32) return false;
+// the operand type of the MFMA path at the context's current precision:
+// 0 FP32, 1 FP64, 2 split-BF16
+int mfma_kind(const mlx_ctx *c) {
+  if (c->prec == MLX_PREC_FP64_MFMA) return 1;
+  if (c->prec == MLX_PREC_FP32) return 0;
+  if (c->prec == MLX_PREC_BF16X3) return 2;
   MLX_THROW(MLX_E_ARG, "MFMA contractions: no kernels for this precision");
 }
+bool mfma_fp64(const mlx_ctx *c) { return mfma_kind(c) == 1; }
 }  // namespace
 
 // ncols > 0: columns m >= ncols are written as zero without being read
@@ -671,6 +758,9 @@ void leg_forward_mfma(mlx_ctx *c, const double2 *in, double2 *out, int ncols, bo
   if (f64) {
     if (at == 1) MLX_FWD(double, 1, k.pfg64);
     else MLX_FWD(double, 2, k.pfg64);
+  } else if (mfma_kind(c) == 2) {
+    if (at == 1) MLX_FWD(Bf3, 1, k.pfg32);
+    else MLX_FWD(Bf3, 2, k.pfg32);
   } else {
     if (at == 1) MLX_FWD(float, 1, k.pfg32);
     else MLX_FWD(float, 2, k.pfg32);
@@ -694,6 +784,9 @@ void leg_backward_mfma(mlx_ctx *c, const double2 *in, double2 *out, int ncols, b
   if (f64) {
     if (xt == 1) MLX_BWD(double, 1, k.pf);
     else MLX_BWD(double, 2, k.pf);
+  } else if (mfma_kind(c) == 2) {
+    if (xt == 1) MLX_BWD(Bf3, 1, k.pf32);
+    else MLX_BWD(Bf3, 2, k.pf32);
   } else {
     if (xt == 1) MLX_BWD(float, 1, k.pf32);
     else MLX_BWD(float, 2, k.pf32);
@@ -712,12 +805,19 @@ void leg_backward_pairs_f32(mlx_ctx *c, const double2 *in, float4 *pairs, int nc
   // input: the active rows; output: one float4 per radial pair
   const double bytes = leg_bytes(k, 0, 4, npc) - 8.0 * k.nr * npc * k.nzdim;
   ProbeScope probe(c, MLX_PROBE_LEG_BWD, bytes, leg_flops(k, 0, npc));
-  if (xt == 1)
-    probed_launch(probe, k_leg_bwd_mf<float, 1, true>, grid, dim3(256), 0, c->stream, in, nullptr, pairs, k.pf32,
-                  k.nrdim, k.npdim, k.nzdim, k.nr, k.nrh, k.ne, k.nrchop, npc, nyt, pair_stride(k));
-  else
-    probed_launch(probe, k_leg_bwd_mf<float, 2, true>, grid, dim3(256), 0, c->stream, in, nullptr, pairs, k.pf32,
-                  k.nrdim, k.npdim, k.nzdim, k.nr, k.nrh, k.ne, k.nrchop, npc, nyt, pair_stride(k));
+  const int kind = mfma_kind(c);
+  if (kind == 1) MLX_THROW(MLX_E_ARG, "rtrans_backward: radial pairs are FP32 sums");
+#define MLX_PAIRS(T_, XT_)                                                                                       \
+  probed_launch(probe, k_leg_bwd_mf<T_, XT_, true>, grid, dim3(256), 0, c->stream, in, nullptr, pairs, k.pf32, k.nrdim, \
+                k.npdim, k.nzdim, k.nr, k.nrh, k.ne, k.nrchop, npc, nyt, pair_stride(k))
+  if (kind == 2) {
+    if (xt == 1) MLX_PAIRS(Bf3, 1);
+    else MLX_PAIRS(Bf3, 2);
+  } else {
+    if (xt == 1) MLX_PAIRS(float, 1);
+    else MLX_PAIRS(float, 2);
+  }
+#undef MLX_PAIRS
   HIPCHK(hipGetLastError());
 }
 
@@ -738,7 +838,10 @@ void vec2tp_project_mfma(mlx_ctx *c, const double2 *ur, const double2 *up, const
   const int at = mfma_tiles(k);
   const bool f64 = mfma_fp64(c);
   if (prec_m0 == MLX_PREC_FP64) prec_m0 = MLX_PREC_FP64_MFMA;  // column 0 stays on MFMA
-  const bool split = prec_m0 >= 0 && prec_m0 != c->prec;
+  // the projection has no split-BF16 kernel: BF16X3 runs it in FP32
+  if (prec_m0 == MLX_PREC_BF16X3) prec_m0 = MLX_PREC_FP32;
+  const int prec_rest = c->prec == MLX_PREC_BF16X3 ? MLX_PREC_FP32 : c->prec;
+  const bool split = prec_m0 >= 0 && prec_m0 != prec_rest;
   bool f64_0 = f64;
   if (split) {
     if (prec_m0 != MLX_PREC_FP32 && prec_m0 != MLX_PREC_FP64_MFMA)

@@ -195,6 +195,4 @@ def test_stage_precision_rejects_bad_arguments():
         tfm.set_stage_precision(7, M.PREC_FP32)
     with pytest.raises(M.MlxError):
         tfm.set_stage_precision(M.STAGE_PROJ, 9)
-    with pytest.raises(M.MlxError):
-        tfm.set_stage_precision(M.STAGE_PROJ, M.PREC_BF16X3)
     tfm.close()

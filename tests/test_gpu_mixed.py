@@ -219,7 +219,8 @@ def test_transform_256_fp32_vs_oracle():
     so.trans(o, 'PPP', okit)
     ref_ppp = o.e.copy()
     so.trans(o, 'FFF', okit)
-    for prec, tol in ((M.PREC_FP64, 1e-12), (M.PREC_FP32, TOL_MIXED)):
+    for prec, tol in ((M.PREC_FP64, 1e-12), (M.PREC_FP32, TOL_MIXED), (M.PREC_FP64_MFMA, 1e-12),
+                      (M.PREC_BF16X3, TOL_MIXED)):
         tfm.set_precision(prec)
         g = M.Scalar(tfm).upload(e, space='FFF')
         M.trans(g, 'PPP', tfm)
@@ -231,11 +232,58 @@ def test_transform_256_fp32_vs_oracle():
     tfm.close()
 
 
-def test_split_bf16_is_rejected():
-    """MLX_PREC_BF16X3 is reserved: this build has no split-BF16 kernels and says so."""
+@pytest.mark.parametrize('cfg', ['C1', 'ODD', 'POW2'])
+def test_trans_bf16x3(cfg):
+    """3-term split-BF16 contractions (six products, FP32 accumulation on
+    v_mfma_f32_16x16x16_bf16): FP32-equivalent accuracy against the oracle,
+    from arithmetic other than the FP32 kernels'."""
     import mlegs_amd as M
-    p = C.params(C.SMALL)
-    tfm = M.Kit(p, device=0)
-    with pytest.raises(M.MlxError):
-        tfm.set_precision(M.PREC_BF16X3)
+    from oracle import kit as K, scalar_ops as so
+    p = C.params(getattr(C, cfg))
+    _say('trans bf16x3', cfg)
+    tfm = M.Kit(p, device=0, precision=M.PREC_BF16X3)
+    okit = K.Kit3D(p)
+    e = C.spectral(tfm.dims, p, 11)
+    g, o = C.pair(tfm, okit, e, 'FFF', ln=0.37)
+    M.trans(g, 'PPP', tfm)
+    so.trans(o, 'PPP', okit)
+    gb = g.e
+    err_b = rel_err(phys(gb, p.nr, p.np // 2), phys(o.e, p.nr, p.np // 2))
+    M.trans(g, 'FFF', tfm)
+    so.trans(o, 'FFF', okit)
+    err_f = rel_err(g.e, o.e)
+    tfm.set_precision(M.PREC_FP32)
+    h = M.Scalar(tfm).upload(e, space='FFF', ln=0.37)
+    M.trans(h, 'PPP', tfm)
+    d32 = rel_err(phys(gb, p.nr, p.np // 2), phys(h.e, p.nr, p.np // 2))
+    _say('  backward %.2e forward %.2e; vs the FP32 kernels %.2e' % (err_b, err_f, d32))
+    assert err_b < TOL_MIXED and err_f < TOL_MIXED
+    assert err_b > 1e-12 and d32 > 0.0
+    tfm.close()
+
+
+def test_vortex_bf16x3_gate():
+    """The step with split-BF16 contractions (the projection's column m = 0
+    in FP64 MFMA, as the mixed preset) against the plain FP64 path: bootstrap
+    + 200 steps within the 1e-6 gate at C1."""
+    import mlegs_amd as M
+    from oracle import kit as K
+    p = C.params(C.C1)
+    tfm = M.Kit(p, device=0, precision=M.PREC_FP64)
+    okit = K.Kit3D(p)
+    (gpsi, _), (gchi, _) = C.qvortex(tfm, okit, p, amp=1e-4)
+    out = {}
+    for mode in ('fp64', 'bf16x3'):
+        if mode == 'bf16x3':
+            tfm.set_precision(M.PREC_BF16X3)
+            tfm.set_stage_precision(M.STAGE_PROJ0, M.PREC_FP64_MFMA)
+        run = M.VortexRun(tfm, gpsi, gchi, M.Scalar(tfm))
+        run.bootstrap()
+        run.step(200)
+        out[mode] = (run.field('psi').e, run.field('chi').e)
+        del run
+    ep, ec = rel_err(out['bf16x3'][0], out['fp64'][0]), rel_err(out['bf16x3'][1], out['fp64'][1])
+    _say('bf16x3 trajectory: psi %.2e chi %.2e' % (ep, ec))
+    assert ep < TOL_MIXED and ec < TOL_MIXED
+    assert ep > 0.0
     tfm.close()

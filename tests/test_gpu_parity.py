@@ -350,20 +350,23 @@ PHI_SIZES = [dict(nr=24, np=32, nz=8, nrchop=24, npchop=17, nzchop=5),      # np
              dict(nr=16, np=256, nz=6, nrchop=16, npchop=129, nzchop=4)]    # np/2 = 128
 
 
-@pytest.mark.parametrize('prec', ['fp64', 'fp32'])
+@pytest.mark.parametrize('prec', ['fp64', 'fp64m', 'fp32'])
 @pytest.mark.parametrize('cfg', PHI_SIZES, ids=['np%d' % c['np'] for c in PHI_SIZES])
 def test_advection_rhs_phi_sizes(cfg, prec):
     """advection_rhs at the four-step phi kernels' sizes np/2 = 16, 64, 128
-    (its fused cross-product transform included), in FP64 and on the
-    FP32-MFMA path with the de-aliasing restrictions, against the composed
-    operators and, in FP64, against the oracle."""
+    (its fused cross-product transform included), in FP64, on FP64 MFMA and
+    on the FP32-MFMA path with the de-aliasing restrictions, against the
+    composed operators and, in FP64, against the oracle.  On the FP32 path
+    the fused form stores its six physical fields in FP32 while the composed
+    operators keep them in FP64, so there the two agree only to FP32 level
+    (N_chi, a difference of large products, to ~1e-2)."""
     import math
     import mlegs_amd as M
     from oracle import kit as K, scalar_ops as so, vortex as V
     d = dict(cfg, ell=1.3, zlen=2 * math.pi, visc=1e-3, dt=1e-3, hyperpow=0, hypervisc=0.0, is_svv=False)
     p = C.params(d)
     _say('advection phi sizes', cfg['np'], prec)
-    tfm = M.Kit(p, device=0, precision=M.PREC_FP64 if prec == 'fp64' else M.PREC_FP32)
+    tfm = M.Kit(p, device=0, precision={'fp64': M.PREC_FP64, 'fp64m': M.PREC_FP64_MFMA, 'fp32': M.PREC_FP32}[prec])
     okit = K.Kit3D(p)
     (gpsi, opsi), (gchi, ochi) = C.qvortex(tfm, okit, p, amp=0.5)
     ub = C.physical(tfm.dims, p, 33)
@@ -382,7 +385,10 @@ def test_advection_rhs_phi_sizes(cfg, prec):
     M.dealias(cc, tfm)
     fp, fc = rel_err(gnp.e, cp.e), rel_err(gnc.e, cc.e)
     _say('  vs composed: nlpsi %.2e nlchi %.2e' % (fp, fc))
-    assert fp < 1e-12 and fc < 1e-10
+    if prec == 'fp32':
+        assert fp < 1e-5 and fc < 1e-1
+    else:
+        assert fp < 1e-12 and fc < 1e-10
     if prec == 'fp64':
         ouz = so.Scalar(okit, 'PPP')
         ouz.e = ub.copy()
