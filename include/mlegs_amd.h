@@ -232,6 +232,14 @@ int mlx_fftreat(mlx_ctx* ctx, mlx_field* f);
 
 /* ---- time steppers (hyperpow = 0, or 4, 6, 8 through helmp / ihelmp) ---- */
 
+/* fefe(s, nl, dt, tfm) (mlegs_scalar.f90:316-325; mlegs_scalar_ops.f90:1065-1096):
+ * forward Euler on both terms, s <- s + dt (nl + visc del2 s). */
+int mlx_fefe(mlx_ctx* ctx, mlx_field* s, const mlx_field* nl, double dt);
+/* abab(s, s_p, nl, nl_p, dt, tfm, is_2nd_svis_p) (mlegs_scalar.f90:332-342;
+ * mlegs_scalar_ops.f90:1098-1155): Adams-Bashforth on both terms; s_p <- s,
+ * nl_p <- nl.  is_2nd_svis_p != 0: s_p holds the previous diffusive term. */
+int mlx_abab(mlx_ctx* ctx, mlx_field* s, mlx_field* s_p, const mlx_field* nl, mlx_field* nl_p, double dt,
+             int32_t is_2nd_svis_p);
 /* febe(s, nl, dt, tfm) (mlegs_scalar.f90:350-359; mlegs_scalar_ops.f90:1157-1198). */
 int mlx_febe(mlx_ctx* ctx, mlx_field* s, const mlx_field* nl, double dt);
 /* abcn(s, s_p, nl, nl_p, dt, tfm) (mlegs_scalar.f90:367-376;

@@ -19,7 +19,7 @@ from ._lib import STAGE_SYNTH, STAGE_UP, STAGE_PROJ, STAGE_PROJ0  # noqa: F401
 from .params import Params, read_input, timestep_set, InputError  # noqa: F401
 
 __all__ = ['Kit', 'Scalar', 'trans', 'chop', 'dealias', 'svv_filter', 'calcat0', 'calcat1', 'zeroat1', 'fftreat',
-           'delsqp', 'idelsqp', 'xxdx', 'del2h', 'del2', 'idel2', 'ihelm', 'helmp', 'ihelmp', 'febe', 'abcn', 'vecprod',
+           'delsqp', 'idelsqp', 'xxdx', 'del2h', 'del2', 'idel2', 'ihelm', 'helmp', 'ihelmp', 'fefe', 'abab', 'febe', 'abcn', 'vecprod',
            'vec2tp', 'tp2vec', 'tp2curlvec', 'advection_rhs', 'check_stability', 'VortexRun', 'energy',
            'Params', 'read_input', 'timestep_set', 'MlxError', 'InputError']
 
@@ -297,6 +297,14 @@ def helmp(s, power, alpha, beta, tfm):
 def ihelmp(s, power, alpha, beta, tfm):
     """Inverse of helmp by banded solves (mlegs_scalar_ops.f90:900-1000)."""
     _c(tfm, tfm.lib.mlx_ihelmp(tfm.ctx, s.h, int(power), float(alpha), float(beta)))
+
+
+def fefe(s, nl, dt, tfm):
+    _c(tfm, tfm.lib.mlx_fefe(tfm.ctx, s.h, nl.h, float(dt)))
+
+
+def abab(s, s_p, nl, nl_p, dt, tfm, is_2nd_svis_p=False):
+    _c(tfm, tfm.lib.mlx_abab(tfm.ctx, s.h, s_p.h, nl.h, nl_p.h, float(dt), int(bool(is_2nd_svis_p))))
 
 
 def febe(s, nl, dt, tfm):

@@ -85,6 +85,8 @@ SIGNATURES = {
     'mlx_helmp': ([_P, _P, ctypes.c_int, _D, _D], ctypes.c_int),
     'mlx_ihelmp': ([_P, _P, ctypes.c_int, _D, _D], ctypes.c_int),
     'mlx_febe': ([_P, _P, _P, _D], ctypes.c_int),
+    'mlx_fefe': ([_P, _P, _P, _D], ctypes.c_int),
+    'mlx_abab': ([_P, _P, _P, _P, _P, _D, _I], ctypes.c_int),
     'mlx_abcn': ([_P, _P, _P, _P, _P, _D], ctypes.c_int),
     'mlx_vecprod': ([_P, _P, _P, _P, _P, _P, _P], ctypes.c_int),
     'mlx_vec2tp': ([_P, _P, _P, _P, _P, _P], ctypes.c_int),

@@ -284,6 +284,68 @@ void step_hyper(mlx_ctx *c, mlx_field *s, const mlx_field *nl, const mlx_field *
   }
 }
 
+// The diffusive term visc del2 s (or the hyperviscous visc del2 + hypervisc
+// del^p) of fefe / abab into sv, a copy of s (mlegs_scalar_ops.f90:1077-1090).
+// Inviscid: the data is zeroed and the log term left as the copy's, as the
+// reference does.
+void diffusive_term(mlx_ctx *c, mlx_field *sv, const mlx_field *s) {
+  fcopy(c, sv, s);
+  const int pw = c->p.hyperpow;
+  if (pw == 0) {
+    if (c->p.visc < 5e-14) {
+      HIPCHK(hipMemsetAsync(sv->e, 0, sizeof(double2) * c->nelem, c->stream));
+    } else {
+      bandmul(c, sv, 0);
+      k_axpby(c, sv->e, sv->sc, c->p.visc, sv->e, sv->sc, 0.0);
+    }
+  } else {
+    const double g = c->p.hypervisc * (((pw / 2 + 1) % 2) ? -1.0 : 1.0);
+    helmp(c, sv, pw, 0.0, c->p.visc / g);
+    k_axpby(c, sv->e, sv->sc, g, sv->e, sv->sc, 0.0);
+  }
+}
+
+// fefe (mlegs_scalar_ops.f90:1065-1096): s <- s + dt (nl + D s)
+void fefe(mlx_ctx *c, mlx_field *s, const mlx_field *nl, double dt) {
+  check_step_inputs(c, {s, nl});
+  mlx_field *sv = tmp(c, 16);
+  diffusive_term(c, sv, s);
+  k_axpby(c, sv->e, sv->sc, 1.0, nl->e, nl->sc, 1.0);
+  k_axpby(c, s->e, s->sc, 1.0, sv->e, sv->sc, dt);
+}
+
+// abab (mlegs_scalar_ops.f90:1098-1155): s <- s + dt (1.5 (nl + D s) -
+// 0.5 (nl_p + D s_p)), then s_p <- s, nl_p <- nl.  is_2nd_svis_p: s_p already
+// holds D s_p.  Inviscid with s_p given: the reference zeroes D s (not
+// D s_p), so s_p itself enters as its diffusive term; kept.
+void abab(mlx_ctx *c, mlx_field *s, mlx_field *s_p, const mlx_field *nl, mlx_field *nl_p, double dt,
+          bool is_2nd_svis_p) {
+  check_step_inputs(c, {s, nl, s_p, nl_p});
+  mlx_field *sv = tmp(c, 16), *svp = tmp(c, 17);
+  diffusive_term(c, sv, s);
+  fcopy(c, svp, s_p);
+  if (!is_2nd_svis_p) {
+    const int pw = c->p.hyperpow;
+    if (pw == 0) {
+      if (!(c->p.visc < 5e-14)) {
+        bandmul(c, svp, 0);
+        k_axpby(c, svp->e, svp->sc, c->p.visc, svp->e, svp->sc, 0.0);
+      }
+    } else {
+      const double g = c->p.hypervisc * (((pw / 2 + 1) % 2) ? -1.0 : 1.0);
+      helmp(c, svp, pw, 0.0, c->p.visc / g);
+      k_axpby(c, svp->e, svp->sc, g, svp->e, svp->sc, 0.0);
+    }
+  }
+  // sv <- 1.5 (nl + sv) - 0.5 (nl_p + svp)
+  k_axpby(c, sv->e, sv->sc, 1.0, nl->e, nl->sc, 1.0);
+  k_axpby(c, svp->e, svp->sc, 1.0, nl_p->e, nl_p->sc, 1.0);
+  k_axpby(c, sv->e, sv->sc, 1.5, svp->e, svp->sc, -0.5);
+  k_axpby(c, s->e, s->sc, 1.0, sv->e, sv->sc, dt);
+  fcopy(c, s_p, s);
+  fcopy(c, nl_p, nl);
+}
+
 // febe (mlegs_scalar_ops.f90:1157-1198)
 void febe(mlx_ctx *c, mlx_field *s, const mlx_field *nl, double dt) {
   check_step_inputs(c, {s, nl});
@@ -757,7 +819,8 @@ int mlx_kit_init(mlx_ctx *c) {
     if (p.np != 1 && p.npchop * 2 > p.np + 2) MLX_THROW(MLX_E_ARG, "tfm_kit_init: npchop <= np/2 + 1 must be satisfied");
     if (p.nz != 1 && p.nzchop * 2 > p.nz + 2) MLX_THROW(MLX_E_ARG, "tfm_kit_init: nzchop <= nz/2 + 1 must be satisfied");
     if (p.nrchop < 1 || p.npchop < 1 || p.nzchop < 1) MLX_THROW(MLX_E_ARG, "tfm_kit_init: chops must be positive");
-    if (p.np == 1 || p.npchop > p.np / 2 + 1) MLX_THROW(MLX_E_ARG, "tfm_kit_init: this path needs the 2D/3D kit (np > 1)");
+    // np = 1 (the reference's 1D runs on its 3D kit): one azimuthal column
+    if (p.npchop > p.np / 2 + 1) MLX_THROW(MLX_E_ARG, "chop_index: chopping in p too large");
     HIPCHK(hipSetDevice(c->device));
     kit_build(c);
     const KitDev &k = c->kit;
@@ -1037,6 +1100,19 @@ int mlx_febe(mlx_ctx *c, mlx_field *s, const mlx_field *nl, double dt) {
     if (c->p.hyperpow != 0) clear_sing(c);
     febe(c, s, nl, dt);
     if (c->p.hyperpow != 0) check_sing(c);
+  });
+}
+int mlx_fefe(mlx_ctx *c, mlx_field *s, const mlx_field *nl, double dt) {
+  return guarded(c, [&] {
+    need_kit(c);
+    fefe(c, s, nl, dt);
+  });
+}
+int mlx_abab(mlx_ctx *c, mlx_field *s, mlx_field *s_p, const mlx_field *nl, mlx_field *nl_p, double dt,
+             int32_t is_2nd_svis_p) {
+  return guarded(c, [&] {
+    need_kit(c);
+    abab(c, s, s_p, nl, nl_p, dt, is_2nd_svis_p != 0);
   });
 }
 int mlx_abcn(mlx_ctx *c, mlx_field *s, mlx_field *s_p, const mlx_field *nl, mlx_field *nl_p, double dt) {

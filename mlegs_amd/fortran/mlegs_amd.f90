@@ -8,7 +8,7 @@
 !>   mlegs_scalar  type(scalar) with init/dealloc/chop_offset/assemble/disassemble,
 !>                 assignment(=) and the generic operators trans, chop, dealias,
 !>                 svv_filter, calcat0/1, zeroat1, delsqp, idelsqp, xxdx, del2h,
-!>                 del2, idel2, ihelm, helmp, ihelmp, febe, abcn, vecprod, vec2tp, tp2vec,
+!>                 del2, idel2, ihelm, helmp, ihelmp, fefe, abab, febe, abcn, vecprod, vec2tp, tp2vec,
 !>                 tp2curlvec (src/modules/mlegs_scalar.f90:15-428)
 !> Every operator calls the C ABI of include/mlegs_amd.h through iso_c_binding;
 !> the data of a scalar stays in GPU memory.  Differences from the reference
@@ -106,7 +106,7 @@ module mlegs_amd
   public :: read_input, timestep_set
   public :: assignment(=)
   public :: trans, chop, dealias, svv_filter, calcat0, calcat1, zeroat1, fftreat
-  public :: delsqp, idelsqp, xxdx, del2h, del2, idel2, ihelm, helmp, ihelmp, febe, abcn
+  public :: delsqp, idelsqp, xxdx, del2h, del2, idel2, ihelm, helmp, ihelmp, fefe, abab, febe, abcn
   public :: vecprod, vec2tp, tp2vec, tp2curlvec, advection_rhs, check_stability, energy
   public :: msave, mload, vorticity_magnitude
 
@@ -169,6 +169,12 @@ module mlegs_amd
   end interface
   interface ihelmp
     module procedure scalar_ihelmp
+  end interface
+  interface fefe
+    module procedure scalar_fefe
+  end interface
+  interface abab
+    module procedure scalar_abab
   end interface
   interface febe
     module procedure scalar_febe
@@ -334,6 +340,17 @@ module mlegs_amd
       type(c_ptr), value :: ctx, f
       integer(c_int), value :: power
       real(c_double), value :: alpha, beta
+    end function
+    integer(c_int) function mlx_fefe(ctx, s, nl, dt) bind(C, name='mlx_fefe')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx, s, nl
+      real(c_double), value :: dt
+    end function
+    integer(c_int) function mlx_abab(ctx, s, s_p, nl, nl_p, dt, is_2nd) bind(C, name='mlx_abab')
+      import :: c_int, c_ptr, c_double, c_int32_t
+      type(c_ptr), value :: ctx, s, s_p, nl, nl_p
+      real(c_double), value :: dt
+      integer(c_int32_t), value :: is_2nd
     end function
     integer(c_int) function mlx_febe(ctx, s, nl, dt) bind(C, name='mlx_febe')
       import :: c_int, c_ptr, c_double
@@ -920,6 +937,32 @@ contains
     class(tfm_kit_3d) :: tfm
     call need_set(tfm, 'ihelmp')
     call check(tfm%ctx, mlx_ihelmp(tfm%ctx, s%h, int(power, c_int), alpha, beta), 'ihelmp')
+  end subroutine
+
+  ! fefe(s, s_rhs_nonlin, dt, tfm) (mlegs_scalar.f90:316-325)
+  subroutine scalar_fefe(s, s_rhs_nonlin, dt, tfm)
+    class(scalar), intent(inout) :: s
+    class(scalar), intent(in) :: s_rhs_nonlin
+    real(p8), intent(in) :: dt
+    class(tfm_kit_3d) :: tfm
+    call need_set(tfm, 'fefe')
+    call check(tfm%ctx, mlx_fefe(tfm%ctx, s%h, s_rhs_nonlin%h, dt), 'fefe')
+  end subroutine
+
+  ! abab(s, s_p, s_rhs_nonlin, s_rhs_nonlin_p, dt, tfm, is_2nd_svis_p) (mlegs_scalar.f90:332-342)
+  subroutine scalar_abab(s, s_p, s_rhs_nonlin, s_rhs_nonlin_p, dt, tfm, is_2nd_svis_p)
+    class(scalar), intent(inout) :: s, s_rhs_nonlin
+    class(scalar), intent(inout) :: s_p, s_rhs_nonlin_p
+    real(p8), intent(in) :: dt
+    class(tfm_kit_3d) :: tfm
+    logical, optional :: is_2nd_svis_p
+    integer(c_int32_t) :: flag
+    flag = 0
+    if (present(is_2nd_svis_p)) then
+      if (is_2nd_svis_p) flag = 1
+    endif
+    call need_set(tfm, 'abab')
+    call check(tfm%ctx, mlx_abab(tfm%ctx, s%h, s_p%h, s_rhs_nonlin%h, s_rhs_nonlin_p%h, dt, flag), 'abab')
   end subroutine
 
   subroutine scalar_febe(s, s_rhs_nonlin, dt, tfm)

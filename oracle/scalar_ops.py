@@ -14,8 +14,12 @@ operators that are not idempotent (del2, del2h, xxdx, ihelm, idel2) are then
 applied twice to that slice, as the reference does.
 """
 import numpy as np
+import scipy.fft as sfft
 from scipy.linalg import solve_banded
 from scipy.linalg.lapack import zgbsv
+
+# threads of the oracle's FFTs (scipy.fft workers); the CPU baseline sets it
+FFT_WORKERS = 1
 
 PI = np.pi
 IU = 1j
@@ -175,7 +179,7 @@ def horizontal_fft_forward(s, kit):
     f = np.empty((s.e.shape[0], 2 * nph, s.e.shape[2]))
     f[:, 0::2, :] = s.e[:, :nph, :].real
     f[:, 1::2, :] = s.e[:, :nph, :].imag
-    c = np.fft.rfft(f, axis=1) / (2 * nph)
+    c = sfft.rfft(f, axis=1, workers=FFT_WORKERS) / (2 * nph)
     s.e[:, :nph + 1, :] = c
 
 
@@ -185,19 +189,19 @@ def horizontal_fft_backward(s, kit):
     Column np/2+1 is FFTE's scratch in the reference; here it is set to 0.
     """
     nph = kit.np // 2
-    f = np.fft.irfft(s.e[:, :nph + 1, :], n=2 * nph, axis=1) * (2 * nph)
+    f = sfft.irfft(s.e[:, :nph + 1, :], n=2 * nph, axis=1, workers=FFT_WORKERS) * (2 * nph)
     s.e[:, :nph, :] = f[:, 0::2, :] + IU * f[:, 1::2, :]
     s.e[:, nph, :] = 0.0
 
 
 def vertical_fft_forward(s, kit):
     """(1/nz) sum e^{-2 pi i jk/nz} along z for every (row, col) (:1721-1783)."""
-    s.e[:, :, :] = np.fft.fft(s.e, axis=2) / kit.nz
+    s.e[:, :, :] = sfft.fft(s.e, axis=2, workers=FFT_WORKERS) / kit.nz
 
 
 def vertical_fft_backward(s, kit):
     """sum e^{+2 pi i jk/nz} along z for every (row, col) (:1787-1848)."""
-    s.e[:, :, :] = np.fft.ifft(s.e, axis=2) * kit.nz
+    s.e[:, :, :] = sfft.ifft(s.e, axis=2, workers=FFT_WORKERS) * kit.nz
 
 
 # ----------------------------------------------------------------------------
@@ -729,6 +733,57 @@ def _need_fff(*ss):
     for s in ss:
         if s.space != 'FFF':
             raise ValueError('fefe: all input scalars must be in FFF for time stepping')
+
+
+def _diffusive(s, kit, p):
+    """visc del2 s, or visc del2 s + hypervisc del^p s (mlegs_scalar_ops.f90:1077-1090);
+    inviscid: the data zeroed, the copy's log term kept."""
+    sv = s.copy()
+    if p.hyperpow == 0:
+        if p.visc < 5e-14:
+            sv.e = np.zeros_like(sv.e)
+        else:
+            del2(sv, kit)
+            sv.e = sv.e * p.visc
+            sv.ln = sv.ln * p.visc
+    else:
+        g = p.hypervisc * (-1.0) ** (p.hyperpow // 2 + 1)
+        helmp(sv, p.hyperpow, 0.0, p.visc / g, kit)
+        sv.e = sv.e * g
+        sv.ln = sv.ln * g
+    return sv
+
+
+def fefe(s, nl, dt, kit, p):
+    """Forward Euler on both terms (mlegs_scalar_ops.f90:1065-1096)."""
+    _need_fff(s, nl)
+    sv = _diffusive(s, kit, p)
+    s.e = s.e + dt * (nl.e + sv.e)
+    s.ln = s.ln + dt * (nl.ln + sv.ln)
+
+
+def abab(s, s_p, nl, nl_p, dt, kit, p, is_2nd_svis_p=False):
+    """Adams-Bashforth on both terms (mlegs_scalar_ops.f90:1098-1155).  In the
+    inviscid case the reference zeroes the current diffusive term twice and
+    never the previous one, so s_p enters as its own diffusive term."""
+    _need_fff(s, nl, s_p, nl_p)
+    sv = _diffusive(s, kit, p)
+    svp = s_p.copy()
+    if not is_2nd_svis_p:
+        if p.hyperpow == 0:
+            if not p.visc < 5e-14:
+                del2(svp, kit)
+                svp.e = svp.e * p.visc
+                svp.ln = svp.ln * p.visc
+        else:
+            g = p.hypervisc * (-1.0) ** (p.hyperpow // 2 + 1)
+            helmp(svp, p.hyperpow, 0.0, p.visc / g, kit)
+            svp.e = svp.e * g
+            svp.ln = svp.ln * g
+    s.e = s.e + dt * (1.5 * (nl.e + sv.e) - 0.5 * (nl_p.e + svp.e))
+    s.ln = s.ln + dt * (1.5 * (nl.ln + sv.ln) - 0.5 * (nl_p.ln + svp.ln))
+    s_p.assign(s)
+    nl_p.assign(nl)
 
 
 def febe(s, nl, dt, kit, p):

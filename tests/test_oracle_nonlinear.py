@@ -304,3 +304,82 @@ def test_second_order_check_sees_swapped_ab2_weights(monkeypatch):
     CPU) loses the second order."""
     e = [_ab2cn_error(dt, monkeypatch, weights=(-0.5, 1.5)) for dt in (0.1, 0.05, 0.025)]
     assert not (e[0] / e[1] > 3.4 and e[1] / e[2] > 3.6), e
+
+
+# ---------------------------------------------------------------------------
+# explicit steppers, by self-convergence at T = 0.1 (differences of runs at
+# dt, dt/2, dt/4, dt/8; the spatial error of the nr = 16 grid cancels)
+# ---------------------------------------------------------------------------
+def _kit_explicit(dt):
+    p = Params(nr=16, np=4, nz=1, nrchop=16, npchop=2, nzchop=1, ell=ELL, zlen=1.0, hyperpow=0, visc=NU, dt=dt,
+               is_svv=False)
+    return K.Kit3D(p), p
+
+
+def _explicit_run(mode, dt, which, t_end=0.1):
+    kit, p = _kit_explicit(dt)
+    n = int(round(t_end / dt))
+    if mode == 'fefe':
+        s = _field(kit, which, 0.0, False)
+        for i in range(n):
+            so.fefe(s, _field(kit, which, i * dt, True), dt, kit, p)
+        return s
+    # two-level start from the exact s(0), s(dt)
+    s_p, s = _field(kit, which, 0.0, False), _field(kit, which, dt, False)
+    nl_p = _field(kit, which, 0.0, True)
+    for i in range(1, n):
+        nl = _field(kit, which, i * dt, True)
+        if mode == 'abab':
+            so.abab(s, s_p, nl, nl_p, dt, kit, p)
+        else:
+            # the caller hands in nu del^2 s_{n-1} (is_2nd_svis_p)
+            svp = s_p.copy()
+            so.del2(svp, kit)
+            svp.e = svp.e * NU
+            s_p = s.copy()
+            so.abab(s, svp, nl, nl_p, dt, kit, p, is_2nd_svis_p=True)
+    return s
+
+
+def _self_convergence(mode, which):
+    runs = [_explicit_run(mode, dt, which) for dt in (8e-4, 4e-4, 2e-4, 1e-4)]
+    d = [np.abs(a.e - b.e).max() + abs(a.ln - b.ln) for a, b in zip(runs, runs[1:])]
+    return d[0] / d[1], d[1] / d[2]
+
+
+@pytest.mark.parametrize('which', ['psi', 'chi'])
+def test_fefe_forced_first_order(which):
+    """Forward Euler on both terms (fefe, mlegs_scalar_ops.f90:1065-1096),
+    with the log term."""
+    r1, r2 = _self_convergence('fefe', which)
+    assert 1.9 < r1 < 2.1 and 1.9 < r2 < 2.1, (r1, r2)
+    kit, p = _kit_explicit(1e-4)
+    assert _err(kit, _explicit_run('fefe', 1e-4, which), which, 0.1) < 1e-4
+
+
+@pytest.mark.parametrize('which', ['psi', 'chi'])
+def test_abab_is_second_order_given_the_previous_diffusive_term(which):
+    """AB2 on both terms (mlegs_scalar_ops.f90:1098-1155) when the caller
+    passes nu del^2 s_{n-1} itself (is_2nd_svis_p): second order."""
+    r1, r2 = _self_convergence('abab2', which)
+    assert r1 > 3.8 and r2 > 3.8, (r1, r2)
+
+
+@pytest.mark.parametrize('which', ['psi', 'chi'])
+def test_abab_history_quirk_is_first_order(which):
+    """abab stores s_p = s after the update (mlegs_scalar_ops.f90:1151), so on
+    the next call the diffusive history is the current state and the term
+    1.5 D(s_n) - 0.5 D(s_p) reduces to D(s_n): forward Euler on diffusion,
+    AB2 on the nonlinear term, first order overall.  A dropped 0.5 or a
+    history written before the update would change these ratios."""
+    r1, r2 = _self_convergence('abab', which)
+    assert 1.9 < r1 < 2.1 and 1.9 < r2 < 2.1, (r1, r2)
+
+
+def test_abab_keeps_the_history():
+    kit, p = _kit_explicit(1e-4)
+    s, s_p = _field(kit, 'psi', 1e-4, False), _field(kit, 'psi', 0.0, False)
+    nl, nl_p = _field(kit, 'psi', 1e-4, True), _field(kit, 'psi', 0.0, True)
+    so.abab(s, s_p, nl, nl_p, 1e-4, kit, p)
+    assert np.array_equal(s_p.e, s.e) and s_p.ln == s.ln
+    assert np.array_equal(nl_p.e, nl.e) and nl_p.ln == nl.ln
