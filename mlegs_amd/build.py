@@ -3,9 +3,10 @@
 Each ``csrc/*.hip`` is compiled to an object in ``mlegs_amd/_build`` (in
 parallel), then linked into one shared library exposing the C ABI declared
 in ``include/mlegs_amd.h``.  The Fortran module ``fortran/mlegs_amd.f90``
-(iso_c_binding interface of that ABI) and the driver
-``fortran/vortical_flow_3d.f90`` are then compiled with amdflang into
-``mlegs_amd/bin/vortical_flow_3d``, which finds the library through its rpath.
+(iso_c_binding interface of that ABI), the vortex driver
+``fortran/vortical_flow_3d.f90`` and the programs under ``fortran/apps``
+are then compiled with amdflang into ``mlegs_amd/bin/<program>``, which find
+the library through their rpath.
 """
 import concurrent.futures
 import os
@@ -20,6 +21,7 @@ LIB = os.path.join(HERE, 'libmlegs_amd.so')
 HIPCC = os.environ.get('HIPCC', '/opt/rocm/bin/hipcc')
 FC = os.environ.get('FC_MLEGS', '/opt/rocm/bin/amdflang')
 FSRC = os.path.join(HERE, 'fortran')
+APPS = os.path.join(FSRC, 'apps')
 BIN = os.path.join(HERE, 'bin')
 ARCH = os.environ.get('MLEGS_AMD_ARCH', 'gfx950')
 
@@ -57,20 +59,34 @@ def build(verbose=True):
 
 
 def build_fortran(verbose=True):
-    """The Fortran host module and the vortex driver linked against the library."""
+    """The Fortran host module, then every program (the vortex driver and the
+    tutorial and 1D/2D programs under fortran/apps) linked against the
+    library into mlegs_amd/bin/<program>."""
     os.makedirs(BIN, exist_ok=True)
     mod_obj = os.path.join(OBJ, 'mlegs_amd_f90.o')
-    steps = [
-        [FC, '-O2', '-c', os.path.join(FSRC, 'mlegs_amd.f90'), '-module-dir', OBJ, '-o', mod_obj],
-        [FC, '-O2', os.path.join(FSRC, 'vortical_flow_3d.f90'), mod_obj, '-I', OBJ,
-         '-L' + HERE, '-lmlegs_amd', '-Wl,-rpath,$ORIGIN/..', '-o', os.path.join(BIN, 'vortical_flow_3d')],
-    ]
-    for cmd in steps:
+    r = subprocess.run([FC, '-O2', '-c', os.path.join(FSRC, 'mlegs_amd.f90'), '-module-dir', OBJ, '-o', mod_obj],
+                       capture_output=True, text=True, cwd=OBJ)
+    if r.returncode != 0:
+        raise RuntimeError('amdflang failed on mlegs_amd.f90:\n%s' % r.stderr)
+    apps = [os.path.join(FSRC, 'vortical_flow_3d.f90')]
+    apps += sorted(os.path.join(APPS, f) for f in os.listdir(APPS) if f.endswith('.f90'))
+
+    def link(src):
+        name = os.path.splitext(os.path.basename(src))[0]
+        out = os.path.join(BIN, name)
+        cmd = [FC, '-O2', src, mod_obj, '-I', OBJ, '-module-dir', os.path.join(OBJ, 'mod_' + name),
+               '-L' + HERE, '-lmlegs_amd', '-Wl,-rpath,$ORIGIN/..', '-o', out]
+        os.makedirs(os.path.join(OBJ, 'mod_' + name), exist_ok=True)
         r = subprocess.run(cmd, capture_output=True, text=True, cwd=OBJ)
         if r.returncode != 0:
             raise RuntimeError('amdflang failed:\n%s\n%s' % (' '.join(cmd), r.stderr))
+        return out
+
+    with concurrent.futures.ThreadPoolExecutor(max_workers=min(8, len(apps))) as ex:
+        outs = list(ex.map(link, apps))
     if verbose:
-        print('built', os.path.join(BIN, 'vortical_flow_3d'))
+        for out in outs:
+            print('built', out)
 
 
 if __name__ == '__main__':

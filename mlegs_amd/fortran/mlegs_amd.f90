@@ -101,6 +101,7 @@ module mlegs_amd
     procedure :: get_ln => scalar_get_ln
     procedure :: set_ln => scalar_set_ln
     procedure :: axpby => scalar_axpby
+    procedure :: zero_e => scalar_zero_e
   end type
 
   public :: read_input, timestep_set
@@ -108,13 +109,13 @@ module mlegs_amd
   public :: trans, chop, dealias, svv_filter, calcat0, calcat1, zeroat1, fftreat
   public :: delsqp, idelsqp, xxdx, del2h, del2, idel2, ihelm, helmp, ihelmp, fefe, abab, febe, abcn
   public :: vecprod, vec2tp, tp2vec, tp2curlvec, advection_rhs, check_stability, energy
-  public :: msave, mload, vorticity_magnitude
+  public :: msave, mload, vorticity_magnitude, colloc_pts
 
   interface assignment(=)
     module procedure scalar_copy
   end interface
   interface msave
-    module procedure msave_scalar
+    module procedure msave_scalar, msave_real1
   end interface
   interface mload
     module procedure mload_scalar
@@ -782,6 +783,16 @@ contains
     call check(this%ctx, mlx_field_axpby(this%ctx, this%h, a, that%h, b), 'axpby')
   end subroutine
 
+  !> s%e = 0 of the reference: the data zeroed, space, log term and offsets kept
+  subroutine scalar_zero_e(this)
+    class(scalar), intent(inout) :: this
+    complex(p8), dimension(:,:,:), allocatable :: z
+    allocate(z(this%glb_sz(1), this%glb_sz(2), this%glb_sz(3)))
+    z = (0.0_p8, 0.0_p8)
+    call this%disassemble(z)
+    deallocate(z)
+  end subroutine
+
   !> deep copy this = that (mlegs_scalar_init.f90:106-140)
   subroutine scalar_copy(this, that)
     class(scalar), intent(inout) :: this
@@ -1067,6 +1078,45 @@ contains
       if (.not. is_global) g = 0
     endif
     call check(s%ctx, mlx_field_save(s%ctx, s%h, trim(fn) // c_null_char, b, g), 'msave')
+  end subroutine
+
+  !> the collocation points r, phi (np > 1) and z (nz > 1) written to
+  !> <logdir>coords_{r,p,z}.dat, the colloc_pts routine of the reference's
+  !> programs (e.g. src/apps/scalar_diffusion_2d.f90, contains section)
+  subroutine colloc_pts(kit)
+    class(tfm_kit_3d), intent(in) :: kit
+    call need_set(kit, 'colloc_pts')
+    call msave(kit%r, trim(adjustl(logdir)) // 'coords_r.dat')
+    if (np .gt. 1) call msave(kit%p, trim(adjustl(logdir)) // 'coords_p.dat')
+    if (nz .gt. 1) call msave(kit%z, trim(adjustl(logdir)) // 'coords_z.dat')
+  end subroutine
+
+  !> msave of a real vector (mlegs_misc_io.f90:37-48, 484-520): the sizes
+  !> n 1 1 as three I10 fields, then one value per line in 1PE24.15E3
+  !> (binary: three int32 sizes, then the values)
+  subroutine msave_real1(a, fn, is_binary)
+    real(p8), dimension(:), intent(in) :: a
+    character(len=*), intent(in) :: fn
+    logical, intent(in), optional :: is_binary
+    integer :: fo, i
+    logical :: b
+    b = .false.
+    if (present(is_binary)) b = is_binary
+    if (b) then
+      open(newunit=fo, file=fn, form='unformatted', access='stream', status='replace')
+      write(fo) int(size(a), i4), 1_i4, 1_i4
+      write(fo) a
+    else
+      open(newunit=fo, file=fn, status='unknown')
+      write(unit=fo, fmt='(3(1X,I10))', advance='no') size(a), 1, 1
+      write(fo, *) ''
+      do i = 1, size(a)
+        write(unit=fo, fmt='(1PE24.15E3)', advance='no') a(i)
+        if (i .lt. size(a)) write(fo, *) ''
+      enddo
+    endif
+    close(fo)
+    write(*, '(I6, a, A)') size(a), ' x    1 1D real vector data written to ', trim(fn)
   end subroutine
 
   !> mload(fn, s, is_binary, is_global) (mlegs_scalar_io.f90:103-248)
