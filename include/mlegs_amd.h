@@ -324,6 +324,52 @@ int mlx_file_read(const char* path, double* host, int64_t n1, int64_t n2, int64_
                   char* space4, double* ln, int32_t* offsets, int32_t is_binary, int32_t is_global);
 const char* mlx_file_last_error(void);
 
+/* ---- ensembles (no reference counterpart) -------------------------------- */
+
+/* n independent vortex runs on one GPU.  The reference runs one kit per
+ * process (src/submodules/mlegs_spectfm_init.f90:15-16); an ensemble holds n
+ * contexts, member i built from p[i] (its own kit, fields and stream, every
+ * parameter free: sizes, ell, visc, dt) at precision prec (MLX_PREC_*).
+ * Members are stepped by replaying a HIP graph of each member's captured
+ * AB2/CN step (the step of mlx_vortex_step, dt = p[i].dt, no per-step host
+ * check), launched as one parent graph or on several streams.
+ *   mlx_ensemble_ctx: member i's context, for creating and filling its fields
+ *     with the ordinary entry points;
+ *   mlx_ensemble_set: member i's state (psi, chi in FFF, uz in PPP, fields of
+ *     member i's context; copied, as mlx_vortex_create does);
+ *   mlx_ensemble_bootstrap: the FE-BE + Richardson bootstrap of every member
+ *     (mlx_vortex_bootstrap);
+ *   mlx_ensemble_step: nsteps AB2/CN steps of every member.  mode -1 tunes
+ *     the launch configuration on the first call that replays graphs (the
+ *     tuning replays are steps of the run) and keeps it; mode 0 replays one
+ *     parent graph per round; mode 1 replays the member graphs on nstreams
+ *     streams; reps (1 or 4; modes 0 and 1) is the count of steps per
+ *     graph.  The first step of a member is eager, and so are the steps
+ *     left over when nsteps is not a multiple of reps;
+ *   mlx_ensemble_vortex: member i's run, to read fields with mlx_vortex_field
+ *     (on member i's context) after mlx_ensemble_sync;
+ *   mlx_ensemble_check: index of the first member whose state became
+ *     non-finite since its bootstrap, or -1;
+ *   mlx_ensemble_info: info[0..4] = mode, streams, steps per graph, phases
+ *     (graphs per member: the order of the step's buffer permutation),
+ *     members; tune (24 doubles, optional) = ms per step of all members for
+ *     each candidate [mode][streams 1, 2, 4, 8, 16, 32][steps per graph 1,
+ *     4], 0 where not tried.
+ * Results of a member equal those of its own single run: every launch
+ * configuration issues the same kernels in the same order per member. */
+typedef struct mlx_ensemble mlx_ensemble;
+int mlx_ensemble_create(const mlx_params* p, int32_t n, int device, int32_t prec, mlx_ensemble** out);
+int mlx_ensemble_destroy(mlx_ensemble* e);
+const char* mlx_ensemble_last_error(mlx_ensemble* e);
+int mlx_ensemble_ctx(mlx_ensemble* e, int32_t i, mlx_ctx** ctx);
+int mlx_ensemble_set(mlx_ensemble* e, int32_t i, const mlx_field* psi, const mlx_field* chi, const mlx_field* uz);
+int mlx_ensemble_bootstrap(mlx_ensemble* e);
+int mlx_ensemble_step(mlx_ensemble* e, int32_t nsteps, int32_t mode, int32_t nstreams, int32_t reps);
+int mlx_ensemble_sync(mlx_ensemble* e);
+int mlx_ensemble_vortex(mlx_ensemble* e, int32_t i, mlx_vortex** v);
+int mlx_ensemble_check(mlx_ensemble* e, int32_t* bad);
+int mlx_ensemble_info(mlx_ensemble* e, int32_t* info, double* tune);
+
 /* ---- diagnostics (no reference counterpart; FP64 reductions) ------------- */
 
 /* Kinetic energy of a PPP velocity field by FP64 physical quadrature:

@@ -35,15 +35,21 @@ def _params_struct(p):
 class Kit:
     """The transform kit on one GPU (tfm_kit_3d + tfm%init)."""
 
-    def __init__(self, params, device=0, precision=PREC_FP64):
+    def __init__(self, params, device=0, precision=PREC_FP64, _handle=None):
         self.lib = _lib.load()
         self.params = params
-        self._ctx = ctypes.c_void_p()
-        ps = _params_struct(params)
-        rc = self.lib.mlx_ctx_create(ctypes.byref(ps), int(device), None, ctypes.byref(self._ctx))
-        if rc != 0:
-            raise MlxError(rc, 'mlx_ctx_create failed')
-        self._check(self.lib.mlx_kit_init(self._ctx))
+        # _handle: an existing context with its kit built (an ensemble
+        # member's), used without taking ownership
+        self._own = _handle is None
+        if self._own:
+            self._ctx = ctypes.c_void_p()
+            ps = _params_struct(params)
+            rc = self.lib.mlx_ctx_create(ctypes.byref(ps), int(device), None, ctypes.byref(self._ctx))
+            if rc != 0:
+                raise MlxError(rc, 'mlx_ctx_create failed')
+            self._check(self.lib.mlx_kit_init(self._ctx))
+        else:
+            self._ctx = _handle
         d = (ctypes.c_int32 * 9)()
         self._check(self.lib.mlx_kit_dims(self._ctx, d))
         self.nrdim, self.npdim, self.nzdim, self.nrh, self.ne, self.chopzl, self.chopzu, self.nph, _ = list(d)
@@ -90,9 +96,9 @@ class Kit:
         self._check(self.lib.mlx_sync(self._ctx))
 
     def close(self):
-        if self._ctx:
+        if self._ctx and self._own:
             self.lib.mlx_ctx_destroy(self._ctx)
-            self._ctx = ctypes.c_void_p()
+        self._ctx = ctypes.c_void_p()
 
     def __del__(self):
         try:

@@ -95,6 +95,17 @@ SIGNATURES = {
     'mlx_advection_rhs': ([_P, _P, _P, _P, _P, _P], ctypes.c_int),
     'mlx_check_stability': ([_P, _P, _P], ctypes.c_int),
     'mlx_vortex_create': ([_P, _P, _P, _P, ctypes.POINTER(_P)], ctypes.c_int),
+    'mlx_ensemble_create': ([ctypes.c_void_p, _I, _I, _I, ctypes.POINTER(_P)], ctypes.c_int),
+    'mlx_ensemble_destroy': ([_P], ctypes.c_int),
+    'mlx_ensemble_last_error': ([_P], ctypes.c_char_p),
+    'mlx_ensemble_ctx': ([_P, _I, ctypes.POINTER(_P)], ctypes.c_int),
+    'mlx_ensemble_set': ([_P, _I, _P, _P, _P], ctypes.c_int),
+    'mlx_ensemble_bootstrap': ([_P], ctypes.c_int),
+    'mlx_ensemble_step': ([_P, _I, _I, _I, _I], ctypes.c_int),
+    'mlx_ensemble_sync': ([_P], ctypes.c_int),
+    'mlx_ensemble_vortex': ([_P, _I, ctypes.POINTER(_P)], ctypes.c_int),
+    'mlx_ensemble_check': ([_P, ctypes.POINTER(ctypes.c_int32)], ctypes.c_int),
+    'mlx_ensemble_info': ([_P, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_double)], ctypes.c_int),
     'mlx_vortex_destroy': ([_P, _P], ctypes.c_int),
     'mlx_vortex_bootstrap': ([_P, _P], ctypes.c_int),
     'mlx_vortex_step': ([_P, _P, _I, _D, _I], ctypes.c_int),

@@ -207,7 +207,7 @@ void helmp(mlx_ctx *c, mlx_field *f, int power, double alpha, double beta) {
 // ihelmp's singular-system flag (device int): set by a column whose
 // factorization meets a zero pivot, cleared and read by the entry points
 int *sing_flag(mlx_ctx *c) { return reinterpret_cast<int *>(c->dsmall + 4000 + 40); }
-void clear_sing(mlx_ctx *c) { HIPCHK(hipMemsetAsync(sing_flag(c), 0, sizeof(int), c->stream)); }
+void clear_sing(mlx_ctx *c) { k_set_flag(c, sing_flag(c), 0); }
 void check_sing(mlx_ctx *c) {
   int h = 0;
   HIPCHK(hipMemcpyAsync(&h, sing_flag(c), sizeof(int), hipMemcpyDeviceToHost, c->stream));
@@ -722,16 +722,6 @@ int guarded(mlx_ctx *c, F &&fn) {
 }
 
 }  // namespace
-
-// ---------------------------------------------------------------------------
-// vortex run state
-// ---------------------------------------------------------------------------
-struct mlx_vortex {
-  mlx_field *f[9] = {nullptr};  // psi chi nlpsi nlchi psi_prev chi_prev nlpsi_prev nlchi_prev uz
-  double *gains = nullptr;      // device [2]
-  int *flag = nullptr;          // device non-finite flag
-  bool prev_stale = false;      // psi_prev/chi_prev equal psi/chi but were not copied
-};
 
 extern "C" {
 

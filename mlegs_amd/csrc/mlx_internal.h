@@ -99,7 +99,14 @@ struct LuCache {
   uint64_t stamp = 0;
 };
 
-struct mlx_vortex;
+// the state of a vortex run: psi chi nlpsi nlchi psi_prev chi_prev nlpsi_prev
+// nlchi_prev uz, the SVV gains and the non-finite flag on the device
+struct mlx_vortex {
+  mlx_field *f[9] = {nullptr};
+  double *gains = nullptr;      // device [2]
+  int *flag = nullptr;          // device non-finite flag
+  bool prev_stale = false;      // psi_prev/chi_prev equal psi/chi but were not copied
+};
 
 // Kernel probes: while on, HIP events are recorded on the context stream
 // around each launch of the probed kernels, with the launch's algorithmic
@@ -212,10 +219,15 @@ struct ProbeScope {
   }
 };
 
+// (an ordinary launch when the probe is off, so that a step can be
+// captured into a HIP graph)
 template <typename... Args, typename F>
 inline void probed_launch(ProbeScope &ps, F kernel, dim3 grid, dim3 block, unsigned shm, hipStream_t s,
                           Args... args) {
-  hipExtLaunchKernelGGL(kernel, grid, block, shm, s, ps.start_event(), ps.stop_event(), 0u, args...);
+  if (ps.slot < 0)
+    hipLaunchKernelGGL(kernel, grid, block, shm, s, args...);
+  else
+    hipExtLaunchKernelGGL(kernel, grid, block, shm, s, ps.start_event(), ps.stop_event(), 0u, args...);
 }
 
 // active (n, m) pairs for a radial chop offset: sum over m < npchop of nrcs(m)
@@ -372,6 +384,9 @@ void k_nonfinite(mlx_ctx *c, const double2 *a, const double2 *b, int *flag);
 void k_energy(mlx_ctx *c, const double2 *vr, const double2 *vp, const double2 *vz, double *out);
 void k_vormag(mlx_ctx *c, double2 *out, const double2 *wr, const double2 *wp, const double2 *wz);
 void k_set_scalar(mlx_ctx *c, double *dst, double v);
+// one int set by a kernel: clears flags in code a caller may capture into a
+// HIP graph (no memset nodes there)
+void k_set_flag(mlx_ctx *c, int *dst, int v);
 void k_copy_scalar(mlx_ctx *c, double *dst, const double *src, double scale);
 
 }  // namespace mlx

@@ -1184,6 +1184,9 @@ __global__ void k_vormag_(double2 *__restrict__ o, const double2 *__restrict__ a
 __global__ void k_set_scalar_(double *d, double v) {
   if (threadIdx.x == 0 && blockIdx.x == 0) d[0] = v;
 }
+__global__ void k_set_flag_(int *d, int v) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) d[0] = v;
+}
 __global__ void k_copy_scalar_(double *d, const double *s, double scale) {
   if (threadIdx.x == 0 && blockIdx.x == 0) d[0] = scale * s[0];
 }
@@ -1478,6 +1481,9 @@ void k_vormag(mlx_ctx *c, double2 *out, const double2 *wr, const double2 *wp, co
 
 void k_set_scalar(mlx_ctx *c, double *dst, double v) {
   hipLaunchKernelGGL(k_set_scalar_, dim3(1), dim3(64), 0, c->stream, dst, v);
+}
+void k_set_flag(mlx_ctx *c, int *dst, int v) {
+  hipLaunchKernelGGL(k_set_flag_, dim3(1), dim3(64), 0, c->stream, dst, v);
 }
 void k_copy_scalar(mlx_ctx *c, double *dst, const double *src, double scale) {
   hipLaunchKernelGGL(k_copy_scalar_, dim3(1), dim3(64), 0, c->stream, dst, src, scale);
