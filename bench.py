@@ -49,6 +49,7 @@ Printed: one JSON line with
                    on one thread, with the CPU model
 """
 import argparse
+import ctypes
 import json
 import math
 import os
@@ -402,6 +403,82 @@ def kernel_sample(tfm, p, name, iters):
     print(json.dumps({'kernel_sample_us': out, 'iters': iters, 'precision': name}), flush=True)
 
 
+def ensemble_bench(n_members, steps, n_check, eager_steps=40, mode=-1, streams=4):
+    """The 2D Lamb-Oseen ensemble: n_members members (NR uniform over
+    {64, 96, 128, 192, 256}, NP = NR, ell ~ U[0.5, 2], visc ~ logU[1e-4,
+    1e-2], dt = 1e-2 min(1, 64 ell / NR), seeded 1e-3 perturbation),
+    bootstrap, then `steps`
+    AB2/CN steps replayed as HIP graphs with the tuned launch configuration
+    (the tuning replays are steps of the run and are inside the timed
+    region).  Also times the same members stepped eagerly (one launch per
+    kernel, the members' streams interleaved) and checks n_check members
+    against their own single runs."""
+    import mlegs_amd as M
+    from mlegs_amd import ensemble as E
+    params = [E.member_params(i) for i in range(n_members)]
+    log('ensemble: %d members, NR counts %s' % (n_members, {nr: sum(p.nr == nr for p in params)
+                                                            for nr in E.NR_CHOICES}))
+    ens = E.Ensemble(params)
+    for i in range(n_members):
+        ens.set(i, *E.lamb_oseen_tp(ens.kits[i], i))
+    ens.bootstrap()
+    ens.sync()
+    # eager reference: every member stepped by its own launches, members interleaved
+    runs = []
+    t0 = time.perf_counter()
+    for i in range(n_members):
+        v = ctypes.c_void_p()
+        ens._check(ens.lib.mlx_ensemble_vortex(ens.h, i, ctypes.byref(v)))
+        runs.append(v)
+    for _ in range(eager_steps):
+        for i, v in enumerate(runs):
+            tfm = ens.kits[i]
+            M._lib.check(tfm.ctx, ens.lib.mlx_vortex_step(tfm.ctx, v, 1, float(tfm.params.dt), 0))
+    ens.sync()
+    t_eager = time.perf_counter() - t0
+    log('eager: %.1f member-steps/s' % (n_members * eager_steps / t_eager))
+    t0 = time.perf_counter()
+    ens.step(steps, mode=mode, streams=streams)
+    ens.sync()
+    t = time.perf_counter() - t0
+    bad = ens.check()
+    info = ens.info()
+    rate = n_members * steps / t
+    log('graphs: %d steps in %.3f s, %.1f member-steps/s, %s' % (steps, t, rate,
+                                                                {k: v for k, v in info.items() if k != 'tune_ms'}))
+    checks = []
+    for i in range(min(n_check, n_members)):
+        ge = (ens.field(i, 'psi').e, ens.field(i, 'chi').e)
+        tfm = M.Kit(params[i], device=0)
+        run = M.VortexRun(tfm, *E.lamb_oseen_tp(tfm, i))
+        run.bootstrap()
+        run.step(eager_steps + steps, check=False)
+        se = (run.field('psi').e, run.field('chi').e)
+        checks.append({'member': i, 'nr': params[i].nr, 'psi': rel(ge[0], se[0]), 'chi': rel(ge[1], se[1])})
+        del run
+        tfm.close()
+    log('single-run checks', checks)
+    tune = info.pop('tune_ms')
+    table = {}
+    for md in range(3):
+        for si, ns in enumerate((1, 2, 4, 8, 16, 32)):
+            for ri, rp in enumerate((1, 4)):
+                v = tune[md, si, ri]
+                if v > 0:
+                    nm = 'parent' if md == 0 else ('streams%d' if md == 1 else 'threads%d') % ns
+                    table['%s_steps%d' % (nm, rp)] = round(n_members / (v * 1e-3), 1)
+    ens.close()
+    return {'metric': 'member-steps/s', 'value': rate, 'members': n_members, 'steps': steps, 'seconds': t,
+            'eager_member_steps_per_s': n_members * eager_steps / t_eager, 'speedup_vs_eager':
+            rate / (n_members * eager_steps / t_eager), 'launch': info, 'tuning_member_steps_per_s': table,
+            'nonfinite_member': bad, 'single_run_check': checks,
+            'max_rel_diff_vs_single': max(max(c['psi'], c['chi']) for c in checks) if checks else None,
+            'workload': '2D Lamb-Oseen (w0 = exp(-r^2/2)/(2 pi)) in TP form + seeded 1e-3 perturbation; NR in '
+                        '{64,96,128,192,256}, NP=NR, NZ=1, ell~U[0.5,2], visc~logU[1e-4,1e-2], '
+                        'dt=1e-2*min(1,64*ell/NR), '
+                        'FP64, SVV on (cutoff 0.5, target 1e-12)'}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
@@ -419,6 +496,13 @@ def main():
     ap.add_argument('--cpu-one-core', type=int, default=1, help='also time the CPU baseline on one thread')
     ap.add_argument('--compare-fp64', type=int, default=1,
                     help='also time the plain FP64 path (0: the selected precision only, e.g. under a profiler)')
+    ap.add_argument('--ensemble', type=int, default=64, help='members of the 2D ensemble leg (0: skip)')
+    ap.add_argument('--ensemble-steps', type=int, default=2000)
+    ap.add_argument('--ensemble-check', type=int, default=4, help='members checked against single runs')
+    ap.add_argument('--ensemble-only', type=int, default=0, help='run the ensemble leg alone and print it')
+    ap.add_argument('--ensemble-mode', type=int, default=-1,
+                    help='launch mode of the ensemble leg: -1 tuned, 0 parent graph, 1 streams, 2 threaded streams')
+    ap.add_argument('--ensemble-streams', type=int, default=4)
     ap.add_argument('--kernel-sample', type=int, default=0,
                     help='only launch each probed kernel family (and a plain field copy) this many times, '
                          'for hardware-counter runs; prints their mean durations')
@@ -429,6 +513,11 @@ def main():
         sys.exit(2)
 
     import mlegs_amd as M
+
+    if args.ensemble_only:
+        print(json.dumps(ensemble_bench(args.ensemble, args.ensemble_steps, args.ensemble_check,
+                                        mode=args.ensemble_mode, streams=args.ensemble_streams)), flush=True)
+        return
 
     p = workload(args.size)
     name = args.prec
@@ -522,6 +611,11 @@ def main():
     del run
     tfm.close()
 
+    ens = None
+    if args.ensemble:
+        ens = ensemble_bench(args.ensemble, args.ensemble_steps, args.ensemble_check,
+                             mode=args.ensemble_mode, streams=args.ensemble_streams)
+
     cpu = None
     if args.cpu_baseline:
         log('cpu baseline (oracle step)')
@@ -539,7 +633,7 @@ def main():
         'fp64_steps_per_s': fp64_rate, 'speedup_vs_fp64': rate / fp64_rate if fp64_rate else None,
         'rel_err_vs_fp64': err, 'ke_history': ke, 'other_precisions': others,
         'adjoint_forward_ratio': None, 'probed_steps_per_s': probed_rate,
-        'transform_c2': transform,
+        'transform_c2': transform, 'ensemble_c4': ens,
         'kernels': probe_table(probes),
         'roofline': roof, 'cpu_baseline': cpu,
     }

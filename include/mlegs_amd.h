@@ -343,7 +343,8 @@ const char* mlx_file_last_error(void);
  *     the launch configuration on the first call that replays graphs (the
  *     tuning replays are steps of the run) and keeps it; mode 0 replays one
  *     parent graph per round; mode 1 replays the member graphs on nstreams
- *     streams; reps (1 or 4; modes 0 and 1) is the count of steps per
+ *     streams; mode 2 does so with one host thread feeding each stream;
+ *     reps (1 or 4; modes 0 to 2) is the count of steps per
  *     graph.  The first step of a member is eager, and so are the steps
  *     left over when nsteps is not a multiple of reps;
  *   mlx_ensemble_vortex: member i's run, to read fields with mlx_vortex_field
@@ -352,9 +353,9 @@ const char* mlx_file_last_error(void);
  *     non-finite since its bootstrap, or -1;
  *   mlx_ensemble_info: info[0..4] = mode, streams, steps per graph, phases
  *     (graphs per member: the order of the step's buffer permutation),
- *     members; tune (24 doubles, optional) = ms per step of all members for
- *     each candidate [mode][streams 1, 2, 4, 8, 16, 32][steps per graph 1,
- *     4], 0 where not tried.
+ *     members; tune (36 doubles, optional) = ms per step of all members for
+ *     each candidate [mode 0, 1, 2][streams 1, 2, 4, 8, 16, 32][steps per
+ *     graph 1, 4], 0 where not tried.
  * Results of a member equal those of its own single run: every launch
  * configuration issues the same kernels in the same order per member. */
 typedef struct mlx_ensemble mlx_ensemble;

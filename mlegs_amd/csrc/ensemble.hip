@@ -25,10 +25,11 @@
 //
 // The launch configuration is tuned on the first call that replays graphs:
 // each candidate (one parent graph per phase holding every member's graph
-// as a child node; or the member graphs launched on 1, 2, 4, 8, 16 or 32
-// streams) runs 1 + 2 rounds and the fastest is kept; then the winner is
-// tried with graphs of four steps each (fewer, larger launches) and kept if
-// faster.  The tuning rounds are ordinary steps of the run: every
+// as a child node; the member graphs launched on 1, 2, 4, 8, 16 or 32
+// streams from one host thread; or the same with one host thread per
+// stream, as a graph launch costs host time per node) runs 1 + 2 rounds and
+// the fastest is kept; then the winner is tried with graphs of four steps
+// each (fewer, larger launches) and kept if faster.  The tuning rounds are ordinary steps of the run: every
 // configuration issues the same kernels in the same order per member, so a
 // member's results do not depend on the configuration.
 //
@@ -40,6 +41,7 @@
 #include <functional>
 #include <numeric>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "mlx_internal.h"
@@ -204,14 +206,15 @@ struct mlx_ensemble {
   std::vector<Member> m;
   int device = 0;
   std::string err;
-  // launch configuration: mode 0 parent graphs, mode 1 member graphs on `streams` streams
+  // launch configuration: mode 0 parent graphs, mode 1 member graphs on
+  // `streams` streams, mode 2 the same with one host thread per stream
   int mode = -1, streams = 0, steps = 1;
   std::vector<hipStream_t> s;        // launch streams (default priority)
   std::vector<hipGraph_t> parent;    // per phase (all members in the same phase)
   std::vector<hipGraphExec_t> parent_x;
   hipEvent_t ev = nullptr;
   std::vector<hipEvent_t> evs;
-  double tune_ms[2][6][2] = {};      // [mode][streams index][steps 1, 4] ms per step of all members
+  double tune_ms[3][6][2] = {};      // [mode][streams index][steps 1, 4] ms per step of all members
 };
 
 namespace {
@@ -324,6 +327,35 @@ void replay(mlx_ensemble *e, int mode, int nstreams) {
   }
 }
 
+// `rounds` rounds of mode 2: stream t is fed by its own host thread, which
+// launches the graphs of members t, t + nstreams, ... round after round
+// (members are independent, so each thread touches only its own members)
+void replay_threaded(mlx_ensemble *e, int nstreams, int rounds) {
+  std::vector<std::thread> th;
+  std::vector<std::string> errs(nstreams);
+  for (int t = 0; t < nstreams; ++t) {
+    th.emplace_back([e, t, nstreams, rounds, &errs] {
+      if (hipSetDevice(e->device) != hipSuccess) {
+        errs[t] = "hipSetDevice failed";
+        return;
+      }
+      for (int r = 0; r < rounds; ++r)
+        for (size_t i = t; i < e->m.size(); i += nstreams) {
+          Member &m = e->m[i];
+          hipError_t er = hipGraphLaunch(m.x[m.phase], e->s[t]);
+          if (er != hipSuccess) {
+            errs[t] = std::string("hipGraphLaunch: ") + hipGetErrorString(er);
+            return;
+          }
+          advance(m);
+        }
+    });
+  }
+  for (std::thread &x : th) x.join();
+  for (const std::string &s : errs)
+    if (!s.empty()) MLX_THROW(MLX_E_HIP, "ensemble: " + s);
+}
+
 void sync_streams(mlx_ensemble *e, int nstreams) {
   for (int i = 0; i < nstreams; ++i) HIPCHK(hipStreamSynchronize(e->s[i]));
 }
@@ -334,10 +366,13 @@ const int kStreamChoices[6] = {1, 2, 4, 8, 16, 32};
 double run_rounds(mlx_ensemble *e, int mode, int nstreams, int count) {
   if (mode == 0 && e->parent_x.empty()) build_parents(e);
   fence_members_to(e, nstreams);
-  replay(e, mode, nstreams);  // untimed: the first launch of an executable graph uploads it
+  replay(e, mode == 2 ? 1 : mode, nstreams);  // untimed: the first launch of an executable graph uploads it
   sync_streams(e, nstreams);
   auto t0 = std::chrono::steady_clock::now();
-  for (int r = 0; r < count; ++r) replay(e, mode, nstreams);
+  if (mode == 2)
+    replay_threaded(e, nstreams, count);
+  else
+    for (int r = 0; r < count; ++r) replay(e, mode, nstreams);
   sync_streams(e, nstreams);
   auto t1 = std::chrono::steady_clock::now();
   fence_streams_to_members(e, nstreams);
@@ -442,8 +477,9 @@ int mlx_ensemble_bootstrap(mlx_ensemble *e) {
 int mlx_ensemble_step(mlx_ensemble *e, int32_t nsteps, int32_t mode, int32_t nstreams, int32_t reps) {
   return guarded_e(e, [&] {
     if (nsteps < 0) MLX_THROW(MLX_E_ARG, "ensemble: nsteps must not be negative");
-    if (mode < -1 || mode > 1) MLX_THROW(MLX_E_ARG, "ensemble: mode is -1 (tune), 0 (parent graph) or 1 (streams)");
-    if (mode == 1 && (nstreams < 1 || nstreams > 64)) MLX_THROW(MLX_E_ARG, "ensemble: 1 to 64 streams");
+    if (mode < -1 || mode > 2)
+      MLX_THROW(MLX_E_ARG, "ensemble: mode is -1 (tune), 0 (parent graph), 1 (streams) or 2 (threaded streams)");
+    if (mode >= 1 && (nstreams < 1 || nstreams > 64)) MLX_THROW(MLX_E_ARG, "ensemble: 1 to 64 streams");
     if (mode >= 0 && reps != 1 && reps != 4) MLX_THROW(MLX_E_ARG, "ensemble: reps is 1 or 4");
     for (Member &m : e->m)
       if (!m.v) MLX_THROW(MLX_E_ARG, "ensemble: every member needs a state (mlx_ensemble_set)");
@@ -472,7 +508,7 @@ int mlx_ensemble_step(mlx_ensemble *e, int32_t nsteps, int32_t mode, int32_t nst
       ensure_streams(e, 32);
       double best = 1e300;
       int bm = 1, bs = 8;
-      for (int md = 0; md < 2; ++md) {
+      for (int md = 0; md < 3; ++md) {
         if (md == 0 && !parent_possible(e)) continue;
         for (int si = 0; si < 6; ++si) {
           if (md == 0 && si > 0) break;
@@ -493,7 +529,7 @@ int mlx_ensemble_step(mlx_ensemble *e, int32_t nsteps, int32_t mode, int32_t nst
       // graphs of four steps each, with the best launch configuration
       if (left >= 2 * 3 * 4) {
         capture_all(e, 4);
-        if (bm == 1 || parent_possible(e)) {
+        if (bm >= 1 || parent_possible(e)) {
           const double per_step = run_rounds(e, bm, bs, 2) / 4;
           left -= 3 * 4;
           int si = 0;
@@ -511,7 +547,10 @@ int mlx_ensemble_step(mlx_ensemble *e, int32_t nsteps, int32_t mode, int32_t nst
     if (rounds > 0) {
       if (e->mode == 0 && e->parent_x.empty()) build_parents(e);
       fence_members_to(e, e->streams);
-      for (int r = 0; r < rounds; ++r) replay(e, e->mode, e->streams);
+      if (e->mode == 2)
+        replay_threaded(e, e->streams, rounds);
+      else
+        for (int r = 0; r < rounds; ++r) replay(e, e->mode, e->streams);
       fence_streams_to_members(e, e->streams);
       left -= rounds * e->steps;
     }

@@ -107,7 +107,7 @@ def test_lamb_oseen_member_closed_form():
     1 / (2 pi (1 + 2 visc t))."""
     import mlegs_amd as M
     from mlegs_amd import ensemble as E
-    p = E.member_params(0).replace(nr=64, np=64, nrchop=64, npchop=33, ell=1.0, visc=1e-2)
+    p = E.member_params(0).replace(nr=64, np=64, nrchop=64, npchop=33, ell=1.0, visc=1e-2, is_svv=False)
     ens = E.Ensemble([p, p.replace(visc=1e-3)])
     for i in range(2):
         psi, chi, uz = E.lamb_oseen_tp(ens.kits[i], 0, amp=0.0)
