@@ -324,6 +324,34 @@ int mlx_file_read(const char* path, double* host, int64_t n1, int64_t n2, int64_
                   char* space4, double* ln, int32_t* offsets, int32_t is_binary, int32_t is_global);
 const char* mlx_file_last_error(void);
 
+/* ---- adjoints (no reference counterpart) --------------------------------- */
+
+/* Transposes of forward operators with respect to the Euclidean inner
+ * product of the stored arrays (real and imaginary parts of every entry,
+ * plus the log coefficient ln), for reverse-mode gradients.  They are exact
+ * transposes on real (Hermitian) fields in the chopped spectral support;
+ * fields with chopping offsets are rejected.
+ *   mlx_trans_adjoint: g holds a cotangent of trans(x, g's space) for x in
+ *     space `from` ("FFF" with g in PPP, or "PPP" with g in FFF); on return
+ *     g holds the cotangent of x, in `from` (transposes of
+ *     rtrans_forward / rtrans_backward, mlegs_scalar_ops.f90:1852-2008, and
+ *     of the phi and z FFTs, :1567-1848);
+ *   mlx_vecprod_adjoint: at the point (v, u) of v <- v x u (vector_product,
+ *     :1264-1306), a = (ar, ap, az) holds the cotangent of v x u on entry and
+ *     that of v on return; (ubr, ubp, ubz) receive that of u; all PPP;
+ *   mlx_abcn_adjoint: the transpose of abcn's map (s, nl, nl_p) -> s
+ *     (:1200-1262, with ihelm :791-854; hyperpow = 0 only): sbar holds the
+ *     cotangent of the new s on entry and that of the old s on return;
+ *     nlbar and nlpbar receive those of nl and nl_p.  Supported on the
+ *     chopped support without the Nyquist column and slice;
+ *   mlx_adjoint_project: zero an FFF field outside that support. */
+int mlx_trans_adjoint(mlx_ctx* ctx, mlx_field* g, const char* from);
+int mlx_vecprod_adjoint(mlx_ctx* ctx, mlx_field* ar, mlx_field* ap, mlx_field* az, const mlx_field* vr,
+                        const mlx_field* vp, const mlx_field* vz, const mlx_field* ur, const mlx_field* up,
+                        const mlx_field* uz, mlx_field* ubr, mlx_field* ubp, mlx_field* ubz);
+int mlx_abcn_adjoint(mlx_ctx* ctx, mlx_field* sbar, mlx_field* nlbar, mlx_field* nlpbar, double dt);
+int mlx_adjoint_project(mlx_ctx* ctx, mlx_field* f);
+
 /* ---- ensembles (no reference counterpart) -------------------------------- */
 
 /* n independent vortex runs on one GPU.  The reference runs one kit per

@@ -21,6 +21,7 @@ from .params import Params, read_input, timestep_set, InputError  # noqa: F401
 __all__ = ['Kit', 'Scalar', 'trans', 'chop', 'dealias', 'svv_filter', 'calcat0', 'calcat1', 'zeroat1', 'fftreat',
            'delsqp', 'idelsqp', 'xxdx', 'del2h', 'del2', 'idel2', 'ihelm', 'helmp', 'ihelmp', 'fefe', 'abab', 'febe', 'abcn', 'vecprod',
            'vec2tp', 'tp2vec', 'tp2curlvec', 'advection_rhs', 'check_stability', 'VortexRun', 'energy',
+           'trans_adjoint', 'vecprod_adjoint', 'abcn_adjoint', 'adjoint_project',
            'Params', 'read_input', 'timestep_set', 'MlxError', 'InputError']
 
 
@@ -323,6 +324,27 @@ def abcn(s, s_p, nl, nl_p, dt, tfm):
 
 def vecprod(vr, vp, vz, ur, up, uz, tfm):
     _c(tfm, tfm.lib.mlx_vecprod(tfm.ctx, vr.h, vp.h, vz.h, ur.h, up.h, uz.h))
+
+
+def trans_adjoint(g, from_space, tfm):
+    """g <- T^T g for T = trans(., g.space) applied to a field in from_space
+    ('FFF' with g in PPP or 'PPP' with g in FFF); see mlx_trans_adjoint."""
+    _c(tfm, tfm.lib.mlx_trans_adjoint(tfm.ctx, g.h, from_space.encode()))
+
+
+def vecprod_adjoint(ar, ap, az, vr, vp, vz, ur, up, uz, ubr, ubp, ubz, tfm):
+    """At (v, u): a <- u x a (the cotangent of v), ub <- a x v (that of u)."""
+    _c(tfm, tfm.lib.mlx_vecprod_adjoint(tfm.ctx, ar.h, ap.h, az.h, vr.h, vp.h, vz.h, ur.h, up.h, uz.h, ubr.h, ubp.h,
+                                        ubz.h))
+
+
+def abcn_adjoint(sbar, nlbar, nlpbar, dt, tfm):
+    """The transpose of abcn's (s, nl, nl_p) -> s: sbar in/out, nlbar, nlpbar out."""
+    _c(tfm, tfm.lib.mlx_abcn_adjoint(tfm.ctx, sbar.h, nlbar.h, nlpbar.h, float(dt)))
+
+
+def adjoint_project(f, tfm):
+    _c(tfm, tfm.lib.mlx_adjoint_project(tfm.ctx, f.h))
 
 
 def vec2tp(vr, vp, vz, psi, chi, tfm):

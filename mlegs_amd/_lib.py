@@ -95,6 +95,10 @@ SIGNATURES = {
     'mlx_advection_rhs': ([_P, _P, _P, _P, _P, _P], ctypes.c_int),
     'mlx_check_stability': ([_P, _P, _P], ctypes.c_int),
     'mlx_vortex_create': ([_P, _P, _P, _P, ctypes.POINTER(_P)], ctypes.c_int),
+    'mlx_trans_adjoint': ([_P, _P, ctypes.c_char_p], ctypes.c_int),
+    'mlx_vecprod_adjoint': ([_P] + [_P] * 12, ctypes.c_int),
+    'mlx_abcn_adjoint': ([_P, _P, _P, _P, _D], ctypes.c_int),
+    'mlx_adjoint_project': ([_P, _P], ctypes.c_int),
     'mlx_ensemble_create': ([ctypes.c_void_p, _I, _I, _I, ctypes.POINTER(_P)], ctypes.c_int),
     'mlx_ensemble_destroy': ([_P], ctypes.c_int),
     'mlx_ensemble_last_error': ([_P], ctypes.c_char_p),

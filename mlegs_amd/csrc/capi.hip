@@ -535,6 +535,95 @@ void vecprod(mlx_ctx *c, mlx_field *vr, mlx_field *vp, mlx_field *vz, const mlx_
   }
 }
 
+// ---------------------------------------------------------------------------
+// adjoints: transposes with respect to the Euclidean inner products of the
+// stored arrays (real and imaginary parts of every entry) plus the log
+// coefficient; see adjoint.hip for the transform identities
+// ---------------------------------------------------------------------------
+void adj_no_offsets(const mlx_field *f, const char *what) {
+  if (f->off[0] || f->off[1] || f->off[2])
+    MLX_THROW(MLX_E_ARG, std::string(what) + ": chopping offsets are not supported by the adjoint");
+}
+
+// g holds a cotangent of trans(x, space of g) for x in space `from` (FFF or
+// PPP, the other end); on return g holds the cotangent of x, in `from`
+void trans_adjoint(mlx_ctx *c, mlx_field *g, const char *from) {
+  need_kit(c);
+  adj_no_offsets(g, "trans adjoint");
+  const KitDev &k = c->kit;
+  const bool to_fff = !std::strncmp(from, "FFF", 3);
+  if (to_fff ? !is_ppp(g) : !(std::strncmp(from, "PPP", 3) == 0 && is_fff(g)))
+    MLX_THROW(MLX_E_ARG, "trans adjoint: the cotangent must be in PPP (from FFF) or FFF (from PPP)");
+  if (to_fff) {
+    // backward transform B = Phi_b Lambda L_b Z_b; B^T = Z_b^T L_b^T Lambda^T Phi_b^T
+    if (k.np > 1) {
+      fft_phi(c, g->e, g->e, -1);
+      k_adj_phi_scale(c, g->e, +1);
+    }
+    k_ln_adjoint(c, g->e, g->sc, +1.0);
+    k_adj_wscale(c, g->e, -1);
+    leg_forward(c, g->e, c->scr[0], 0);
+    swap_scr(c, g);
+    if (k.nz > 1) {
+      fft_z(c, g->e, -1, (double)k.nrdim * k.npdim);
+      k_adj_scale(c, g->e, (double)k.nz);
+    }
+    set_space(g, "FFF");
+  } else {
+    // forward transform F = Z_f L_f Lambda^- Phi_f; F^T = Phi_f^T Lambda^-T L_f^T Z_f^T
+    if (k.nz > 1) {
+      fft_z(c, g->e, +1, (double)k.nrdim * k.npdim);
+      k_adj_scale(c, g->e, 1.0 / k.nz);
+    }
+    leg_backward(c, g->e, c->scr[0], 0);
+    swap_scr(c, g);
+    k_adj_wscale(c, g->e, +1);
+    k_ln_adjoint(c, g->e, g->sc, -1.0);
+    if (k.np > 1) {
+      k_adj_phi_scale(c, g->e, -1);
+      fft_phi(c, g->e, g->e, +1);
+    }
+    set_space(g, "PPP");
+  }
+}
+
+// abcn (hyperpow = 0) maps (s, nl, nl_p) to s' = (del2 + a)^-1 E with
+// R = a (s + dt (1.5 nl - 0.5 nl_p) + dt/2 visc del2 s), L = s.ln + dt (1.5
+// nl.ln - 0.5 nl_p.ln), E = R - lap_ln(L), s'.ln = L, a = -2 / (dt visc)
+// (mlegs_scalar_ops.f90:1200-1262, ihelm :791-854), on the chopped support
+// (rows n < nrcs(m), m < npchop, the kept z slices; Nyquist column and
+// slice excluded).  sbar: the cotangent of s' on entry, of s on return;
+// nlbar, nlpbar: the cotangents of nl and nl_p (overwritten).
+void abcn_adjoint(mlx_ctx *c, mlx_field *sbar, mlx_field *nlbar, mlx_field *nlpbar, double dt) {
+  need_kit(c);
+  for (mlx_field *f : {sbar, nlbar, nlpbar}) adj_no_offsets(f, "abcn adjoint");
+  if (!is_fff(sbar)) MLX_THROW(MLX_E_LAYOUT, "abcn adjoint: the cotangent must be in FFF");
+  if (c->p.hyperpow != 0) MLX_THROW(MLX_E_ARG, "abcn adjoint: the hyperviscous stepper has no adjoint here");
+  if (c->p.visc < 5e-14) MLX_THROW(MLX_E_ARG, "abcn: inviscid case and no linear term in rhs. semi-implicit time adv is impossible");
+  const double a = -2.0 / (dt * c->p.visc);
+  mlx_field *y = sbar;
+  k_adj_project(c, y->e);
+  // y <- (del2 + a)^-T y; the cotangent of L gains -lap_ln^T y
+  k_helm_solve_t(c, y->e, a);
+  k_lap_ln_t(c, y->e, y->sc, -1.0);
+  // nl and nl_p
+  fcopy(c, nlbar, y);
+  k_axpby(c, nlbar->e, nullptr, 1.5 * a * dt, y->e, nullptr, 0.0);
+  k_copy_scalar(c, nlbar->sc, y->sc, 1.5 * dt);
+  fcopy(c, nlpbar, y);
+  k_axpby(c, nlpbar->e, nullptr, -0.5 * a * dt, y->e, nullptr, 0.0);
+  k_copy_scalar(c, nlpbar->sc, y->sc, -0.5 * dt);
+  // s: a (y + dt/2 visc del2^T y), and the log term's Laplacian inside del2
+  mlx_field *t = tmp(c, 3);
+  k_del2_t(c, t->e, y->e, true);
+  k_lap_ln_t(c, y->e, y->sc, a * dt / 2.0 * c->p.visc);
+  k_axpby(c, y->e, nullptr, a, t->e, nullptr, a * dt / 2.0 * c->p.visc);
+  k_adj_project(c, y->e);
+  set_space(y, "FFF");
+  set_space(nlbar, "FFF");
+  set_space(nlpbar, "FFF");
+}
+
 // dst = PFP form of the PPP field src (optionally r-weighted): metadata and log
 // term of src, then the phi transform from src into dst
 void phi_forward_from(mlx_ctx *c, mlx_field *dst, const mlx_field *src, int rscale) {
@@ -763,6 +852,7 @@ int mlx_ctx_destroy(mlx_ctx *c) {
   for (mlx_field *f : c->pool) field_free(f);
   c->pool.clear();
   lu_cache_free(c);
+  adj_cache_free(c);
   for (auto &v : c->probe.ev)
     for (hipEvent_t e : v) (void)hipEventDestroy(e);
   for (auto &kv : c->work)
@@ -1119,6 +1209,40 @@ int mlx_vecprod(mlx_ctx *c, mlx_field *vr, mlx_field *vp, mlx_field *vz, const m
   return guarded(c, [&] {
     need_kit(c);
     vecprod(c, vr, vp, vz, ur, up, uz);
+  });
+}
+
+int mlx_trans_adjoint(mlx_ctx *c, mlx_field *g, const char *from) {
+  return guarded(c, [&] { trans_adjoint(c, g, from); });
+}
+
+int mlx_vecprod_adjoint(mlx_ctx *c, mlx_field *ar, mlx_field *ap, mlx_field *az, const mlx_field *vr,
+                        const mlx_field *vp, const mlx_field *vz, const mlx_field *ur, const mlx_field *up,
+                        const mlx_field *uz, mlx_field *ubr, mlx_field *ubp, mlx_field *ubz) {
+  return guarded(c, [&] {
+    need_kit(c);
+    for (const mlx_field *f : {(const mlx_field *)ar, (const mlx_field *)ap, (const mlx_field *)az, vr, vp, vz, ur,
+                               up, uz})
+      if (!is_ppp(f)) MLX_THROW(MLX_E_LAYOUT, "vector_product adjoint: all inputs must be in PPP");
+    k_vecprod_adj(c, ar->e, ap->e, az->e, vr->e, vp->e, vz->e, ur->e, up->e, uz->e, ubr->e, ubp->e, ubz->e);
+    for (mlx_field *f : {ar, ap, az, ubr, ubp, ubz}) {
+      k_set_scalar(c, f->sc, 0.0);
+      f->off[0] = f->off[1] = f->off[2] = 0;
+      set_space(f, "PPP");
+    }
+  });
+}
+
+int mlx_abcn_adjoint(mlx_ctx *c, mlx_field *sbar, mlx_field *nlbar, mlx_field *nlpbar, double dt) {
+  return guarded(c, [&] { abcn_adjoint(c, sbar, nlbar, nlpbar, dt); });
+}
+
+int mlx_adjoint_project(mlx_ctx *c, mlx_field *f) {
+  return guarded(c, [&] {
+    need_kit(c);
+    if (!is_fff(f)) MLX_THROW(MLX_E_LAYOUT, "adjoint project: the field must be in FFF");
+    adj_no_offsets(f, "adjoint project");
+    k_adj_project(c, f->e);
   });
 }
 

@@ -127,6 +127,14 @@ struct WorkTable {
 };
 
 constexpr int kPartMax = 2 * 2 * 4096;  // two fields, (total, tail), up to 4096 workgroups each
+// LAPACK-band LU factors of the transposed (del2 + alpha) systems, one set
+// per alpha, for the adjoint of the Helmholtz solve
+struct AdjLu {
+  double alpha = 0.0;
+  double *fac = nullptr;
+  int8_t *piv = nullptr;
+};
+
 struct mlx_ctx {
   std::map<long, WorkTable> work;
   mlx_params p;
@@ -148,6 +156,7 @@ struct mlx_ctx {
   std::vector<mlx_field *> pool;  // temporaries (allocated on demand)
   std::vector<LuCache> lu;
   uint64_t lu_clock = 0;
+  std::vector<AdjLu> adj_lu;
   ProbeSet probe;
   // second stream (default priority) for work that overlaps the context
   // stream inside one operation, with its fork and join events; created on
@@ -384,6 +393,19 @@ void k_nonfinite(mlx_ctx *c, const double2 *a, const double2 *b, int *flag);
 void k_energy(mlx_ctx *c, const double2 *vr, const double2 *vp, const double2 *vz, double *out);
 void k_vormag(mlx_ctx *c, double2 *out, const double2 *wr, const double2 *wp, const double2 *wz);
 void k_set_scalar(mlx_ctx *c, double *dst, double v);
+void adj_cache_free(mlx_ctx *c);
+// adjoint kernels (adjoint.hip)
+void k_adj_phi_scale(mlx_ctx *c, double2 *e, int mode);
+void k_adj_wscale(mlx_ctx *c, double2 *e, int mode);
+void k_ln_adjoint(mlx_ctx *c, const double2 *e, double *ln_bar, double sign);
+void k_adj_scale(mlx_ctx *c, double2 *e, double s);
+void k_adj_project(mlx_ctx *c, double2 *e);
+void k_del2_t(mlx_ctx *c, double2 *out, const double2 *in, bool with_ak);
+void k_lap_ln_t(mlx_ctx *c, const double2 *e, double *ln_bar, double scale);
+void k_helm_solve_t(mlx_ctx *c, double2 *e, double alpha);
+void k_vecprod_adj(mlx_ctx *c, double2 *a1, double2 *a2, double2 *a3, const double2 *v1, const double2 *v2,
+                   const double2 *v3, const double2 *u1, const double2 *u2, const double2 *u3, double2 *ub1,
+                   double2 *ub2, double2 *ub3);
 // one int set by a kernel: clears flags in code a caller may capture into a
 // HIP graph (no memset nodes there)
 void k_set_flag(mlx_ctx *c, int *dst, int v);
