@@ -479,6 +479,59 @@ def ensemble_bench(n_members, steps, n_check, eager_steps=40, mode=-1, streams=4
                         'FP64, SVV on (cutoff 0.5, target 1e-12)'}
 
 
+def adjoint_bench(nsteps, ndir):
+    """The adjoint leg at 128 x 64 x 64 (L = 1, ZLEN = 2 pi, VISC = 1e-3,
+    dt = 1e-3, FP64): the nonlinear model step of mlegs_amd.adjoint (backward
+    transforms, v x u, forward transforms, de-aliasing, AB2/CN on three
+    fields) from seeded 0.5-amplitude fields; times nsteps forward steps
+    alone and the checkpointed gradient (forward with checkpoints every 5
+    steps, then the reverse sweep with recomputation), and checks the
+    gradient against central differences (eps = 1e-6) on ndir seeded
+    directions."""
+    import mlegs_amd as M
+    from mlegs_amd import adjoint as A, inputs
+    from mlegs_amd.params import Params
+    p = Params(nr=128, np=64, nz=64, nrchop=128, npchop=33, nzchop=33, ell=1.0, zlen=2 * math.pi, visc=1e-3,
+               dt=1e-3, hyperpow=0, hypervisc=0.0, is_svv=False)
+    tfm = M.Kit(p, device=0)
+    chops = (p.nrchop, p.npchop, p.nzchop, p.np, p.nz)
+    x0 = [M.Scalar(tfm).upload(inputs.perturbation(tfm.dims, chops, 0.5, field_id=11 + i), space='FFF',
+                               ln=0.1 * (i + 1)) for i in range(3)]
+    chain = A.ModelChain(tfm, p.dt, segment=5)
+    chain.forward(x0, 2)  # warm-up: factors and temporaries
+    tfm.sync()
+    t0 = time.perf_counter()
+    chain.forward(x0, nsteps)
+    tfm.sync()
+    t_fwd = time.perf_counter() - t0
+    timing = {}
+    J, g = chain.gradient(x0, nsteps, timing)
+    worst = worst_scaled = 0.0
+    eps = 1e-6
+    for d in range(ndir):
+        dirs = [M.Scalar(tfm).upload(inputs.perturbation(tfm.dims, chops, 1.0, field_id=1000 + 3 * d + i),
+                                     space='FFF', ln=0.05 * (d % 5 - 2)) for i in range(3)]
+        plus = [x.copy() for x in x0]
+        minus = [x.copy() for x in x0]
+        for a, b, dd in zip(plus, minus, dirs):
+            a.axpby(1.0, dd, eps)
+            b.axpby(1.0, dd, -eps)
+        fd = (chain.objective(chain.forward(plus, nsteps)) - chain.objective(chain.forward(minus, nsteps))) / (2 * eps)
+        ad = A.inner(g, dirs)
+        worst = max(worst, abs(fd - ad) / max(abs(fd), abs(ad), 1e-30))
+        worst_scaled = max(worst_scaled, abs(fd - ad) / math.sqrt(A.inner(g, g) * A.inner(dirs, dirs)))
+    tfm.close()
+    out = {'steps': nsteps, 'forward_ms_per_step': 1e3 * t_fwd / nsteps,
+           'reverse_ms_per_step': 1e3 * timing['reverse_s'] / nsteps,
+           'adjoint_forward_ratio': timing['reverse_s'] / t_fwd,
+           'checkpointed_forward_ms_per_step': 1e3 * timing['forward_s'] / nsteps,
+           'fd_directions': ndir, 'fd_eps': eps, 'max_rel_diff_vs_fd': worst,
+           'max_diff_vs_fd_over_norm_g_norm_d': worst_scaled, 'objective': J,
+           'workload': 'model step (B v, v x u, F, dealias, chop, AB2/CN on 3 fields) at 128x64x64, FP64'}
+    log('adjoint', out)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
@@ -503,6 +556,8 @@ def main():
     ap.add_argument('--ensemble-mode', type=int, default=-1,
                     help='launch mode of the ensemble leg: -1 tuned, 0 parent graph, 1 streams, 2 threaded streams')
     ap.add_argument('--ensemble-streams', type=int, default=4)
+    ap.add_argument('--adjoint', type=int, default=10, help='steps of the adjoint leg (0: skip)')
+    ap.add_argument('--adjoint-dirs', type=int, default=32, help='finite-difference directions of the adjoint leg')
     ap.add_argument('--kernel-sample', type=int, default=0,
                     help='only launch each probed kernel family (and a plain field copy) this many times, '
                          'for hardware-counter runs; prints their mean durations')
@@ -611,6 +666,10 @@ def main():
     del run
     tfm.close()
 
+    adj = None
+    if args.adjoint:
+        adj = adjoint_bench(args.adjoint, args.adjoint_dirs)
+
     ens = None
     if args.ensemble:
         ens = ensemble_bench(args.ensemble, args.ensemble_steps, args.ensemble_check,
@@ -632,7 +691,8 @@ def main():
         'gate': gate,
         'fp64_steps_per_s': fp64_rate, 'speedup_vs_fp64': rate / fp64_rate if fp64_rate else None,
         'rel_err_vs_fp64': err, 'ke_history': ke, 'other_precisions': others,
-        'adjoint_forward_ratio': None, 'probed_steps_per_s': probed_rate,
+        'adjoint_forward_ratio': adj['adjoint_forward_ratio'] if adj else None, 'adjoint_c5': adj,
+        'probed_steps_per_s': probed_rate,
         'transform_c2': transform, 'ensemble_c4': ens,
         'kernels': probe_table(probes),
         'roofline': roof, 'cpu_baseline': cpu,

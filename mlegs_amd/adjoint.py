@@ -146,9 +146,10 @@ class ModelChain:
             if k % seg == 0:
                 ckpt[k] = (copy_all(state), copy_all(hist) if hist is not None else None)
             state, hist = self.step(state, hist)
-        J = self.objective(state)
         self.tfm.sync()
         t1 = time.perf_counter()
+        J = self.objective(state)
+        t1b = time.perf_counter()
         sbar = copy_all(state)  # dJ/ds = s
         carry = None
         for k0 in sorted(ckpt, reverse=True):
@@ -164,5 +165,5 @@ class ModelChain:
         t2 = time.perf_counter()
         if timing is not None:
             timing['forward_s'] = t1 - t0
-            timing['reverse_s'] = t2 - t1
+            timing['reverse_s'] = t2 - t1b
         return J, sbar

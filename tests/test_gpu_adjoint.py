@@ -171,26 +171,35 @@ def _chain_case(cfg, nsteps, ndir, seed0, segment=4):
             b.axpby(1.0, dd, -eps)
         fd = (chain.objective(chain.forward(plus, nsteps)) - chain.objective(chain.forward(minus, nsteps))) / (2 * eps)
         ad = A.inner(g, dirs)
-        rows.append((fd, ad))
+        rows.append((fd, ad, math.sqrt(A.inner(g, g) * A.inner(dirs, dirs))))
     tfm.close()
     return J, rows, timing
 
 
+def _fd_errors(rows):
+    """Worst |fd - adjoint| relative to the derivative and to the Cauchy-Schwarz
+    scale |g| |d| (central differences at eps = 1e-6 carry ~1e-10 |J| / eps
+    of rounding, which dominates along directions with a small derivative)."""
+    rel = max(abs(fd - ad) / max(abs(fd), abs(ad), 1e-30) for fd, ad, _ in rows)
+    scaled = max(abs(fd - ad) / sc for fd, ad, sc in rows)
+    return rel, scaled
+
+
 def test_model_chain_gradient_c1():
     J, rows, timing = _chain_case(C.C1, 9, 6, 7)
-    worst = max(abs(fd - ad) / max(abs(fd), abs(ad), 1e-30) for fd, ad in rows)
-    _say('  J %.6e, worst rel. diff FD vs adjoint %.2e' % (J, worst), timing)
-    for fd, ad in rows:
+    rel, scaled = _fd_errors(rows)
+    _say('  J %.6e, FD vs adjoint: worst rel. %.2e, scaled %.2e' % (J, rel, scaled), timing)
+    for fd, ad, _ in rows:
         _say('    fd %.12e adjoint %.12e' % (fd, ad))
-    assert worst < 1e-6
+    assert rel < 1e-5 and scaled < 1e-8
 
 
 def test_model_chain_gradient_c5():
     """The C5 size (128 x 64 x 64), 10 steps, 32 seeded directions."""
     J, rows, timing = _chain_case(C5, 10, 32, 11, segment=5)
-    worst = max(abs(fd - ad) / max(abs(fd), abs(ad), 1e-30) for fd, ad in rows)
-    _say('  J %.6e, worst rel. diff over %d directions %.2e' % (J, len(rows), worst), timing)
-    assert worst < 1e-6
+    rel, scaled = _fd_errors(rows)
+    _say('  J %.6e, over %d directions: worst rel. %.2e, scaled %.2e' % (J, len(rows), rel, scaled), timing)
+    assert rel < 1e-5 and scaled < 1e-8
 
 
 def test_autograd_function_gradcheck():
