@@ -62,7 +62,7 @@ sys.path.insert(0, ROOT)
 METRIC = 'steps/s @256³ mixed-prec vs FP64 at ≤1e-6 rel.err; adjoint/forward time ratio'
 PEAK_TFLOPS = {'fp64': 78.6, 'fp32': 157.3, 'bf16': 2500.0}
 HBM_GBPS = 8000.0
-HBM_BOUND = ('fft_phi', 'fft_z', 'lu_solve', 'phi_cross')
+HBM_BOUND = ('fft_phi', 'fft_z', 'lu_solve', 'phi_cross', 'tp_front', 'abcn_rhs')
 TRAFFIC_FILE = os.path.join(ROOT, 'profiles', 'traffic_256.json')
 
 

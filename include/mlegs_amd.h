@@ -84,7 +84,9 @@ extern "C" {
 #define MLX_PROBE_V2T       4  /* vec2tp projection (5 contractions)       */
 #define MLX_PROBE_LU_SOLVE  5  /* banded LU solve of ihelm / idel2         */
 #define MLX_PROBE_PHI_CROSS 6  /* v x w formed inside three phi transforms */
-#define MLX_PROBE_COUNT     7
+#define MLX_PROBE_TP_FRONT  7  /* tp2vec / tp2curlvec column pass          */
+#define MLX_PROBE_ABCN_RHS  8  /* AB2/CN right-hand side with del2         */
+#define MLX_PROBE_COUNT     9
 
 typedef struct mlx_ctx mlx_ctx;
 typedef struct mlx_field mlx_field;

@@ -112,7 +112,7 @@ class Kit:
         self._check(self.lib.mlx_time_kernel(self._ctx, s.h, which.encode(), int(iters), ctypes.byref(ms)))
         return ms.value
 
-    PROBES = ('fft_phi', 'fft_z', 'leg_fwd', 'leg_bwd', 'v2t', 'lu_solve', 'phi_cross')
+    PROBES = ('fft_phi', 'fft_z', 'leg_fwd', 'leg_bwd', 'v2t', 'lu_solve', 'phi_cross', 'tp_front', 'abcn_rhs')
 
     def probe_start(self, max_launches=4096):
         """Bracket every launch of the probed kernels with HIP events from now on."""

@@ -1395,9 +1395,12 @@ void k_abcn_rhs_d2(mlx_ctx *c, double2 *e, double *ln, const double2 *nl, const 
   size_t lds = 3 * sizeof(double2) * (size_t)k.nrdim;
   // the log coefficient first: the data kernel reads s's ln for the (0,0) correction
   // with helm_pre the (0,0) workgroup also takes ihelm's log-term step
-  hipLaunchKernelGGL(k_abcn_rhs_d2_, grid, dim3(256), lds, c->stream, e, ln, nl, nlp, k.d2c, k.lognorm, k.ak, k.nrdim,
-                     k.npdim, k.ne, nrc, k.npchop, k.chopzl, k.chopzu, k.ell, dt, a, c->p.visc, pcut, zlo, zhi,
-                     helm_pre ? nlln : nullptr, helm_pre ? nlpln : nullptr);
+  // algorithmic bytes: s, N and N_p's active rows in, the right-hand side out
+  const double nzk = (double)k.nzdim - ((zhi >= zlo) ? (double)(zhi - zlo + 1) : 0.0);
+  ProbeScope probe(c, MLX_PROBE_ABCN_RHS, 64.0 * nzk * active_pairs(k, nroff, pcut), 0.0);
+  probed_launch(probe, k_abcn_rhs_d2_, grid, dim3(256), lds, c->stream, e, ln, nl, nlp, k.d2c, k.lognorm, k.ak,
+                k.nrdim, k.npdim, k.ne, nrc, k.npchop, k.chopzl, k.chopzu, k.ell, dt, a, c->p.visc, pcut, zlo, zhi,
+                helm_pre ? nlln : nullptr, helm_pre ? nlpln : nullptr);
   if (!helm_pre) hipLaunchKernelGGL(k_abcn_ln_, dim3(1), dim3(1), 0, c->stream, ln, ln, nlln, nlpln, dt, a);
   HIPCHK(hipGetLastError());
 }
@@ -1444,9 +1447,15 @@ void k_tp2vec_front2(mlx_ctx *c, double2 *vr, double2 *vp, double2 *vz, double *
   dim3 grid((ncols > 0 && ncols < k.npdim) ? ncols : k.npdim, k.nzdim);
   size_t lds = 4 * sizeof(double2) * (size_t)k.nrdim;
   const FrontOut v{vr, vp, vz, lvr, lvp, lvz}, w{wr, wp, wz, lwr, lwp, lwz};
-  hipLaunchKernelGGL(k_tp2vec_front2_, grid, dim3(256), lds, c->stream, v, w, psi, chi, psi_ln, chi_ln, md_off,
-                     k.d2c, k.xxc, k.lognorm, k.ak, k.nrdim, k.npdim, k.nzdim, k.ne, k.nrchop, k.npchop, k.chopzl,
-                     k.chopzu, k.ell, zlo, zhi);
+  // algorithmic bytes: psi and chi's active rows in, the six fields' rows of
+  // the +3 offset out, over the computed columns and z slices
+  const int nc = (ncols > 0 && ncols < k.npchop) ? ncols : k.npchop;
+  const double nzk = (double)k.nzdim - ((zhi >= zlo) ? (double)(zhi - zlo + 1) : 0.0);
+  ProbeScope probe(c, MLX_PROBE_TP_FRONT, 16.0 * nzk * (2.0 * active_pairs(k, 0, nc) + 6.0 * active_pairs(k, 3, nc)),
+                   0.0);
+  probed_launch(probe, k_tp2vec_front2_, grid, dim3(256), lds, c->stream, v, w, psi, chi, psi_ln, chi_ln, md_off,
+                k.d2c, k.xxc, k.lognorm, k.ak, k.nrdim, k.npdim, k.nzdim, k.ne, k.nrchop, k.npchop, k.chopzl, k.chopzu,
+                k.ell, zlo, zhi);
   HIPCHK(hipGetLastError());
 }
 
