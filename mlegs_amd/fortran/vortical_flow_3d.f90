@@ -7,15 +7,18 @@
 !> mlegs_amd module, so every field operation runs on the GPU.
 !> Differences: one process drives one GPU (no MPI); the 1e-10 noise of the
 !> initial condition is the library's seeded spectral perturbation instead of
-!> the compiler's rand(), so runs are reproducible; no field files are written
-!> (field I/O is not part of this library).
+!> the compiler's rand(), so runs are reproducible.  With ISFLDSAV the
+!> vorticity-magnitude snapshots vormagfld<n>_PPP are written to FLDDIR, as
+!> the reference's save_vort_mag writes them.
 !>
 !> Usage:  vortical_flow_3d [input.params [dump_prefix [precision]]]
 !>   input.params  default ./input.params (the built-in defaults when absent)
 !>   dump_prefix   write the final psi and chi to <prefix>_psi.bin and
 !>                 <prefix>_chi.bin: three int64 sizes, one float64 ln, then
 !>                 the global complex(8) array e(nrdim, npdim, nzdim)
-!>   precision     0 FP64 (default), 1 FP32-MFMA Legendre contractions
+!>   precision     Legendre contraction precision: 0 FP64 (default),
+!>                 1 FP32 MFMA, 2 split-BF16 MFMA, 3 FP64 MFMA, or 'mixed':
+!>                 FP32 MFMA with the projection's column m = 0 on FP64 MFMA
 program vortical_flow_3d
   use iso_c_binding
   use mlegs_amd
@@ -41,7 +44,7 @@ program vortical_flow_3d
   integer(i4) :: glb_sz(3), i, nargs, ios
   integer(i8) :: c0, c1, crate
   character(len=256) :: input_params_file = './input.params', dump_prefix = '', arg
-  logical :: exists
+  logical :: exists, mixed = .false.
   type(scalar) :: uz
   type(scalar) :: psi, nlpsi, chi, nlchi, psi_prev, nlpsi_prev, chi_prev, nlchi_prev
   type(scalar) :: psi_rich, chi_rich
@@ -51,8 +54,13 @@ program vortical_flow_3d
   if (nargs .ge. 2) call get_command_argument(2, dump_prefix)
   if (nargs .ge. 3) then
     call get_command_argument(3, arg)
-    read(arg, *, iostat=ios) mlx_precision
-    if (ios .ne. 0) error stop 'vortical_flow_3d: precision must be an integer'
+    if (trim(arg) .eq. 'mixed') then
+      mixed = .true.
+      mlx_precision = MLX_PREC_FP32
+    else
+      read(arg, *, iostat=ios) mlx_precision
+      if (ios .ne. 0) error stop "vortical_flow_3d: precision must be an integer or 'mixed'"
+    endif
   endif
   write(*, *) 'Program Started'
   call system_clock(c0, crate)
@@ -61,7 +69,9 @@ program vortical_flow_3d
   if (exists) call read_input(input_params_file)
   call timestep_set()
   call tfm%init()
-  write(*, '(a,3i6,a,i2)') ' Initialization: NR NP NZ =', nr, np, nz, ', precision', mlx_precision
+  if (mixed) call tfm%set_stage_precision(int(MLX_STAGE_PROJ0, i4), int(MLX_PREC_FP64_MFMA, i4))
+  write(*, '(a,3i6,a,i2,a,l2)') ' Initialization: NR NP NZ =', nr, np, nz, ', precision', mlx_precision, &
+                                ', mixed', mixed
 
   glb_sz = [tfm%nrdim, tfm%npdim, tfm%nzdim]
   call uz%init(glb_sz, [1, 0, 2]); call uz%set_space('PPP')

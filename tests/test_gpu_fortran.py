@@ -57,14 +57,17 @@ def _load_dump(fn):
     return e, ln
 
 
-def _python_run(pfile):
-    """vortical_flow_3d's sequence, operator by operator, from Python."""
+def _python_run(pfile, mixed=False):
+    """vortical_flow_3d's sequence, operator by operator, from Python; mixed:
+    FP32 MFMA contractions with the projection's column 0 on FP64 MFMA."""
     import mlegs_amd as M
     from mlegs_amd import inputs
     from mlegs_amd.params import read_input, timestep_set
     p = read_input(pfile, make_dirs=False)
     timestep_set(p)
-    tfm = M.Kit(p, device=0)
+    tfm = M.Kit(p, device=0, precision=M.PREC_FP32 if mixed else M.PREC_FP64)
+    if mixed:
+        tfm.set_stage_precision(M.STAGE_PROJ0, M.PREC_FP64_MFMA)
     nr, np_, nz = p.nr, p.np, p.nz
     r, x = tfm.r, tfm.x
     ph = 2 * math.pi / np_ * np.arange(np_ + 1)
@@ -143,6 +146,27 @@ def test_fortran_driver_matches_python_sequence(tmp_path):
     assert rel_err(fpsi, ppsi) < 1e-13 and rel_err(fchi, pchi) < 1e-13
     assert abs(fpsi_ln - ppsi_ln) <= 1e-13 * max(1.0, abs(ppsi_ln))
     assert abs(fchi_ln - pchi_ln) <= 1e-13 * max(1.0, abs(pchi_ln))
+
+
+def test_fortran_driver_mixed_precision(tmp_path):
+    """precision 'mixed': the driver's run equals the same sequence from Python
+    in that preset, and stays within 1e-6 of the FP64 run."""
+    print('[case] fortran vortical_flow_3d 48x32x16, 3 steps, mixed', flush=True)
+    assert os.path.exists(DRIVER)
+    pfile = tmp_path / 'input.params'
+    pfile.write_text(PARAMS)
+    r = subprocess.run([DRIVER, str(pfile), str(tmp_path / 'dump'), 'mixed'], cwd=str(tmp_path),
+                       capture_output=True, text=True, timeout=120)
+    print(r.stdout[-2000:], r.stderr[-2000:], flush=True)
+    assert r.returncode == 0
+    assert 'mixed T' in r.stdout and 'Program Finished' in r.stdout
+    fpsi, _ = _load_dump(tmp_path / 'dump_psi.bin')
+    fchi, _ = _load_dump(tmp_path / 'dump_chi.bin')
+    mpsi, _, mchi, _, n = _python_run(str(pfile), mixed=True)
+    assert n == 3
+    assert rel_err(fpsi, mpsi) < 1e-13 and rel_err(fchi, mchi) < 1e-13
+    dpsi, _, dchi, _, _ = _python_run(str(pfile))
+    assert 0 < rel_err(fchi, dchi) < 1e-6 and rel_err(fpsi, dpsi) < 1e-6
 
 
 def test_fortran_driver_rejects_bad_input(tmp_path):
