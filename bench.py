@@ -47,6 +47,16 @@ Printed: one JSON line with
   cpu_baseline     the oracle (numpy/scipy FP64) timing one AB2/CN step at the
                    same size on the host, on the job's threads (`cores`) and
                    on one thread, with the CPU model
+  ensemble_c4      64 2D Lamb-Oseen runs of uneven NR stepped together by
+                   HIP-graph replay: member-steps/s, tuned launch shape,
+                   members checked against their own single runs
+  adjoint_c5       the gradient of the perturbation kinetic energy after
+                   --adjoint steps at 128 x 64 x 64 (q = 4 vortex + 1e-4
+                   perturbation, VISC = 1e-3, dt = 1e-3, SVV off) with respect
+                   to the initial psi, chi: forward mixed, adjoints FP64,
+                   checkpoints every --adjoint-segment steps; reverse/forward
+                   time ratio (adjoint_forward_ratio), central differences on
+                   --adjoint-dirs directions, mixed vs FP64-forward gradient
 """
 import argparse
 import ctypes
@@ -479,55 +489,78 @@ def ensemble_bench(n_members, steps, n_check, eager_steps=40, mode=-1, streams=4
                         'FP64, SVV on (cutoff 0.5, target 1e-12)'}
 
 
-def adjoint_bench(nsteps, ndir):
-    """The adjoint leg at 128 x 64 x 64 (L = 1, ZLEN = 2 pi, VISC = 1e-3,
-    dt = 1e-3, FP64): the nonlinear model step of mlegs_amd.adjoint (backward
-    transforms, v x u, forward transforms, de-aliasing, AB2/CN on three
-    fields) from seeded 0.5-amplitude fields; times nsteps forward steps
-    alone and the checkpointed gradient (forward with checkpoints every 5
-    steps, then the reverse sweep with recomputation), and checks the
-    gradient against central differences (eps = 1e-6) on ndir seeded
-    directions."""
+def adjoint_bench(nsteps, ndir, segment=20, amp=1e-4):
+    """The adjoint leg: the gradient of the perturbation kinetic energy after
+    the bootstrap and nsteps steps (T = nsteps dt) with respect to the initial
+    psi and chi, for the q = 4 vortex plus a seeded perturbation of amplitude
+    amp at 128 x 64 x 64 (L = 1, ZLEN = 2 pi, VISC = 1e-3, dt = 1e-3, SVV off,
+    no background flow), checkpoints every `segment` steps.  The forward runs
+    in the mixed preset, the adjoints in FP64.  Times nsteps forward steps
+    alone and the reverse sweep (recomputation included); checks the
+    gradient of the FP64 forward against central differences (eps = 1e-6)
+    on ndir seeded directions, and the mixed-forward gradient against the
+    FP64-forward one."""
     import mlegs_amd as M
     from mlegs_amd import adjoint as A, inputs
     from mlegs_amd.params import Params
     p = Params(nr=128, np=64, nz=64, nrchop=128, npchop=33, nzchop=33, ell=1.0, zlen=2 * math.pi, visc=1e-3,
                dt=1e-3, hyperpow=0, hypervisc=0.0, is_svv=False)
     tfm = M.Kit(p, device=0)
-    chops = (p.nrchop, p.npchop, p.nzchop, p.np, p.nz)
-    x0 = [M.Scalar(tfm).upload(inputs.perturbation(tfm.dims, chops, 0.5, field_id=11 + i), space='FFF',
-                               ln=0.1 * (i + 1)) for i in range(3)]
-    chain = A.ModelChain(tfm, p.dt, segment=5)
-    chain.forward(x0, 2)  # warm-up: factors and temporaries
-    tfm.sync()
-    t0 = time.perf_counter()
-    chain.forward(x0, nsteps)
-    tfm.sync()
-    t_fwd = time.perf_counter() - t0
-    timing = {}
-    J, g = chain.gradient(x0, nsteps, timing)
-    worst = worst_scaled = 0.0
-    eps = 1e-6
-    for d in range(ndir):
-        dirs = [M.Scalar(tfm).upload(inputs.perturbation(tfm.dims, chops, 1.0, field_id=1000 + 3 * d + i),
-                                     space='FFF', ln=0.05 * (d % 5 - 2)) for i in range(3)]
-        plus = [x.copy() for x in x0]
-        minus = [x.copy() for x in x0]
-        for a, b, dd in zip(plus, minus, dirs):
-            a.axpby(1.0, dd, eps)
-            b.axpby(1.0, dd, -eps)
-        fd = (chain.objective(chain.forward(plus, nsteps)) - chain.objective(chain.forward(minus, nsteps))) / (2 * eps)
-        ad = A.inner(g, dirs)
-        worst = max(worst, abs(fd - ad) / max(abs(fd), abs(ad), 1e-30))
-        worst_scaled = max(worst_scaled, abs(fd - ad) / math.sqrt(A.inner(g, g) * A.inner(dirs, dirs)))
+    uz = M.Scalar(tfm)
+    q0 = initial_state(tfm, p, amp=amp)
+    qb = initial_state(tfm, p, amp=0.0)
+    out = {'steps': nsteps, 'segment': segment, 'perturbation_amplitude': amp}
+    grads = {}
+    for name in ('mixed', 'fp64'):
+        configure(tfm, name)
+        chain = A.VortexChain(tfm, uz, segment=segment, configure=lambda t, n=name: configure(t, n))
+        base = chain.forward(qb[0], qb[1], nsteps)
+        chain.forward(q0[0], q0[1], 2)  # warm-up: factors and temporaries
+        run = M.VortexRun(tfm, q0[0], q0[1], uz)
+        run.bootstrap()
+        tfm.sync()
+        t0 = time.perf_counter()
+        run.step(nsteps, check=False)
+        tfm.sync()
+        t_fwd = time.perf_counter() - t0
+        run.step(0, check=True)
+        del run
+        timing = {}
+        J, g = chain.gradient(q0[0], q0[1], nsteps, base, timing)
+        grads[name] = g
+        out[name] = {'forward_ms_per_step': 1e3 * t_fwd / nsteps,
+                     'reverse_ms_per_step': 1e3 * timing['reverse_s'] / nsteps,
+                     'adjoint_forward_ratio': timing['reverse_s'] / t_fwd,
+                     'checkpointed_forward_ms_per_step': 1e3 * timing['forward_s'] / nsteps, 'objective': J}
+        if name == 'fp64' and ndir:
+            worst = worst_scaled = 0.0
+            eps = 1e-6
+            gg = A.inner(g, g)
+            for d in range(ndir):
+                dirs = [M.Scalar(tfm).upload(inputs.perturbation(tfm.dims, (p.nrchop, p.npchop, p.nzchop, p.np, p.nz),
+                                                                 1.0, field_id=3000 + 2 * d + i),
+                                             space='FFF', ln=0.05 * (d % 3 - 1)) for i in range(2)]
+                Jpm = []
+                for sgn in (1.0, -1.0):
+                    a, b = q0[0].copy(), q0[1].copy()
+                    a.axpby(1.0, dirs[0], sgn * eps)
+                    b.axpby(1.0, dirs[1], sgn * eps)
+                    Jpm.append(chain.perturbation_energy(chain.forward(a, b, nsteps), base, grad=False))
+                fd = (Jpm[0] - Jpm[1]) / (2 * eps)
+                ad = A.inner(g, dirs)
+                worst = max(worst, abs(fd - ad) / max(abs(fd), abs(ad), 1e-300))
+                worst_scaled = max(worst_scaled, abs(fd - ad) / math.sqrt(gg * A.inner(dirs, dirs)))
+            out.update({'fd_directions': ndir, 'fd_eps': eps, 'max_rel_diff_vs_fd': worst,
+                        'max_diff_vs_fd_over_norm_g_norm_d': worst_scaled})
+    dg = [a.copy() for a in grads['mixed']]
+    for a, b in zip(dg, grads['fp64']):
+        a.axpby(1.0, b, -1.0)
+    out['mixed_vs_fp64_gradient_rel'] = math.sqrt(A.inner(dg, dg) / A.inner(grads['fp64'], grads['fp64']))
+    out['adjoint_forward_ratio'] = out['mixed']['adjoint_forward_ratio']
+    out['workload'] = ('vortex run (bootstrap + AB2/CN steps, dealias, zeroat1, advection_rhs) at 128x64x64, '
+                       'q = 4 vortex + seeded perturbation, objective 1/2 int |u - u_base|^2 at T, gradient '
+                       'w.r.t. initial psi, chi; forward mixed preset, adjoint FP64')
     tfm.close()
-    out = {'steps': nsteps, 'forward_ms_per_step': 1e3 * t_fwd / nsteps,
-           'reverse_ms_per_step': 1e3 * timing['reverse_s'] / nsteps,
-           'adjoint_forward_ratio': timing['reverse_s'] / t_fwd,
-           'checkpointed_forward_ms_per_step': 1e3 * timing['forward_s'] / nsteps,
-           'fd_directions': ndir, 'fd_eps': eps, 'max_rel_diff_vs_fd': worst,
-           'max_diff_vs_fd_over_norm_g_norm_d': worst_scaled, 'objective': J,
-           'workload': 'model step (B v, v x u, F, dealias, chop, AB2/CN on 3 fields) at 128x64x64, FP64'}
     log('adjoint', out)
     return out
 
@@ -556,7 +589,8 @@ def main():
     ap.add_argument('--ensemble-mode', type=int, default=-1,
                     help='launch mode of the ensemble leg: -1 tuned, 0 parent graph, 1 streams, 2 threaded streams')
     ap.add_argument('--ensemble-streams', type=int, default=4)
-    ap.add_argument('--adjoint', type=int, default=10, help='steps of the adjoint leg (0: skip)')
+    ap.add_argument('--adjoint', type=int, default=200, help='steps of the adjoint leg (0: skip)')
+    ap.add_argument('--adjoint-segment', type=int, default=20, help='checkpoint interval of the adjoint leg')
     ap.add_argument('--adjoint-dirs', type=int, default=32, help='finite-difference directions of the adjoint leg')
     ap.add_argument('--kernel-sample', type=int, default=0,
                     help='only launch each probed kernel family (and a plain field copy) this many times, '
@@ -668,7 +702,7 @@ def main():
 
     adj = None
     if args.adjoint:
-        adj = adjoint_bench(args.adjoint, args.adjoint_dirs)
+        adj = adjoint_bench(args.adjoint, args.adjoint_dirs, args.adjoint_segment)
 
     ens = None
     if args.ensemble:

@@ -346,13 +346,54 @@ const char* mlx_file_last_error(void);
  *     cotangent of the new s on entry and that of the old s on return;
  *     nlbar and nlpbar receive those of nl and nl_p.  Supported on the
  *     chopped support without the Nyquist column and slice;
- *   mlx_adjoint_project: zero an FFF field outside that support. */
+ *   mlx_adjoint_project: zero an FFF field outside that support;
+ *   mlx_febe_adjoint: the same for febe's (s, nl) -> s (:1157-1198);
+ *   mlx_zeroat1_adjoint: in place, the transpose of zeroat1 (:311-325);
+ *   mlx_tp2vec_adjoint / mlx_tp2curlvec_adjoint: the cotangents of (vr, vp,
+ *     vz) in PPP (overwritten) to those of (psi, chi), data and log
+ *     coefficients (vector_reconstruction :1455-1545, curl :1547-1560);
+ *   mlx_vec2tp_adjoint: the cotangents of (psi, chi) in FFF (overwritten)
+ *     to those of (vr, vp, vz) in PPP (vector_projection :1308-1453), the
+ *     log coefficients of psi and chi excluded (advection_rhs sets them to
+ *     zero);
+ *   mlx_advection_adjoint: the vector-Jacobian product of advection_rhs at
+ *     (psi, chi) with background uz (vortical_flow_3d.f90:353-395): the
+ *     cotangents of (nlpsi, nlchi) in FFF (overwritten) to those of (psi,
+ *     chi) in psib, chib; psi and chi are de-aliased in place, as
+ *     advection_rhs does;
+ *   mlx_vortex_reverse_step: one reverse step of mlx_vortex_step's map
+ *     (s' = zeroat1(dealias(abcn(s, N, N_prev))), N' = N(s'), N_prev' = N;
+ *     :160-181): from the cotangents (sb, nb, npb) of the state after the
+ *     step, whose psi and chi are p1, c1, to those of the state before it,
+ *     in place (p1, c1 de-aliased in place);
+ *   mlx_vortex_step_record: mlx_vortex_step without the stability check,
+ *     copying psi and chi after step i into rec[2 i] and rec[2 i + 1] (the
+ *     recomputation of a checkpointed segment);
+ *   mlx_energy_grad: in place, a PPP velocity (vr, vp, vz) to the gradient of
+ *     its kinetic energy (mlx_energy's quadrature).
+ * These linearise and transpose in plain FP64 whatever the context's
+ * precision; the Nyquist column and slice are outside their support. */
 int mlx_trans_adjoint(mlx_ctx* ctx, mlx_field* g, const char* from);
 int mlx_vecprod_adjoint(mlx_ctx* ctx, mlx_field* ar, mlx_field* ap, mlx_field* az, const mlx_field* vr,
                         const mlx_field* vp, const mlx_field* vz, const mlx_field* ur, const mlx_field* up,
                         const mlx_field* uz, mlx_field* ubr, mlx_field* ubp, mlx_field* ubz);
 int mlx_abcn_adjoint(mlx_ctx* ctx, mlx_field* sbar, mlx_field* nlbar, mlx_field* nlpbar, double dt);
 int mlx_adjoint_project(mlx_ctx* ctx, mlx_field* f);
+int mlx_febe_adjoint(mlx_ctx* ctx, mlx_field* sbar, mlx_field* nlbar, double dt);
+int mlx_zeroat1_adjoint(mlx_ctx* ctx, mlx_field* f);
+int mlx_tp2vec_adjoint(mlx_ctx* ctx, mlx_field* vrb, mlx_field* vpb, mlx_field* vzb, mlx_field* psib,
+                       mlx_field* chib);
+int mlx_tp2curlvec_adjoint(mlx_ctx* ctx, mlx_field* wrb, mlx_field* wpb, mlx_field* wzb, mlx_field* psib,
+                           mlx_field* chib);
+int mlx_vec2tp_adjoint(mlx_ctx* ctx, mlx_field* psib, mlx_field* chib, mlx_field* vrb, mlx_field* vpb,
+                       mlx_field* vzb);
+int mlx_advection_adjoint(mlx_ctx* ctx, mlx_field* psi, mlx_field* chi, const mlx_field* uz, mlx_field* nlpsib,
+                          mlx_field* nlchib, mlx_field* psib, mlx_field* chib);
+int mlx_vortex_reverse_step(mlx_ctx* ctx, mlx_field* p1, mlx_field* c1, const mlx_field* uz, mlx_field* sbpsi,
+                            mlx_field* sbchi, mlx_field* nbpsi, mlx_field* nbchi, mlx_field* npbpsi,
+                            mlx_field* npbchi, double dt);
+int mlx_vortex_step_record(mlx_ctx* ctx, mlx_vortex* v, int32_t nsteps, double dt, mlx_field** rec);
+int mlx_energy_grad(mlx_ctx* ctx, mlx_field* vr, mlx_field* vp, mlx_field* vz);
 
 /* ---- ensembles (no reference counterpart) -------------------------------- */
 

@@ -21,7 +21,8 @@ from .params import Params, read_input, timestep_set, InputError  # noqa: F401
 __all__ = ['Kit', 'Scalar', 'trans', 'chop', 'dealias', 'svv_filter', 'calcat0', 'calcat1', 'zeroat1', 'fftreat',
            'delsqp', 'idelsqp', 'xxdx', 'del2h', 'del2', 'idel2', 'ihelm', 'helmp', 'ihelmp', 'fefe', 'abab', 'febe', 'abcn', 'vecprod',
            'vec2tp', 'tp2vec', 'tp2curlvec', 'advection_rhs', 'check_stability', 'VortexRun', 'energy',
-           'trans_adjoint', 'vecprod_adjoint', 'abcn_adjoint', 'adjoint_project',
+           'trans_adjoint', 'vecprod_adjoint', 'abcn_adjoint', 'adjoint_project', 'febe_adjoint', 'zeroat1_adjoint',
+           'tp2vec_adjoint', 'tp2curlvec_adjoint', 'vec2tp_adjoint', 'advection_adjoint', 'energy_grad',
            'Params', 'read_input', 'timestep_set', 'MlxError', 'InputError']
 
 
@@ -345,6 +346,41 @@ def abcn_adjoint(sbar, nlbar, nlpbar, dt, tfm):
 
 def adjoint_project(f, tfm):
     _c(tfm, tfm.lib.mlx_adjoint_project(tfm.ctx, f.h))
+
+
+def febe_adjoint(sbar, nlbar, dt, tfm):
+    """The transpose of febe's (s, nl) -> s: sbar in/out, nlbar out."""
+    _c(tfm, tfm.lib.mlx_febe_adjoint(tfm.ctx, sbar.h, nlbar.h, float(dt)))
+
+
+def zeroat1_adjoint(f, tfm):
+    _c(tfm, tfm.lib.mlx_zeroat1_adjoint(tfm.ctx, f.h))
+
+
+def tp2vec_adjoint(vrb, vpb, vzb, psib, chib, tfm):
+    """Cotangents of (vr, vp, vz) in PPP (overwritten) -> those of (psi, chi)."""
+    _c(tfm, tfm.lib.mlx_tp2vec_adjoint(tfm.ctx, vrb.h, vpb.h, vzb.h, psib.h, chib.h))
+
+
+def tp2curlvec_adjoint(wrb, wpb, wzb, psib, chib, tfm):
+    _c(tfm, tfm.lib.mlx_tp2curlvec_adjoint(tfm.ctx, wrb.h, wpb.h, wzb.h, psib.h, chib.h))
+
+
+def vec2tp_adjoint(psib, chib, vrb, vpb, vzb, tfm):
+    """Cotangents of (psi, chi) in FFF (overwritten) -> those of (vr, vp, vz) in PPP."""
+    _c(tfm, tfm.lib.mlx_vec2tp_adjoint(tfm.ctx, psib.h, chib.h, vrb.h, vpb.h, vzb.h))
+
+
+def advection_adjoint(psi, chi, uz, nlpsib, nlchib, psib, chib, tfm):
+    """advection_rhs's vector-Jacobian product at (psi, chi): the cotangents
+    of (nlpsi, nlchi) (overwritten) -> those of (psi, chi); psi and chi are
+    de-aliased in place, as advection_rhs does."""
+    _c(tfm, tfm.lib.mlx_advection_adjoint(tfm.ctx, psi.h, chi.h, uz.h, nlpsib.h, nlchib.h, psib.h, chib.h))
+
+
+def energy_grad(vr, vp, vz, tfm):
+    """In place: a PPP velocity -> the gradient of its kinetic energy."""
+    _c(tfm, tfm.lib.mlx_energy_grad(tfm.ctx, vr.h, vp.h, vz.h))
 
 
 def vec2tp(vr, vp, vz, psi, chi, tfm):

@@ -1,5 +1,7 @@
 // Kernels of the adjoint (transposed) operators: the transforms, the
-// vector product, the Laplacian and the Helmholtz solve.  The reference has
+// vector product, the Laplacian, the toroidal-poloidal reconstruction and
+// projection, zeroat1 and the energy's gradient (the transposed Helmholtz
+// and idel2 solves run on the banded LU of solve.hip).  The reference has
 // no adjoint; these are the transposes of this library's forward operators
 // with respect to the Euclidean inner products of the stored arrays (the real
 // and imaginary parts of every entry, plus the log coefficient ln).
@@ -87,6 +89,40 @@ __global__ void k_adj_scale_(double2 *e, size_t n, double s) {
   }
 }
 
+// one pass of the diagonal factors of a transform's transpose:
+// e(i, m, k) *= s W(i) R(i) P(m), W = w(i) (wmode +1), 1/w(i) (-1) or 1 (0);
+// R = r(i) (rmode +1), 1/r(i) (-1) or 1 (0); P = np mu_m (phimode +1),
+// 1 / (np mu_m) (-1) or 1 (0), the imaginary parts at m = 0 and np/2 zeroed
+// when phimode != 0; rows i >= nr zeroed when wmode or rmode is set
+__global__ void k_adj_diag_(double2 *e, size_t n, const double *w, const double *r, int nr, int nrdim, int npdim,
+                            int nph, int np, int wmode, int rmode, int phimode, double s) {
+  for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (size_t)gridDim.x * blockDim.x) {
+    const int i = (int)(t % nrdim);
+    const int m = (int)((t / nrdim) % npdim);
+    double2 v = e[t];
+    double f = s;
+    bool zim = false;
+    if (wmode != 0 || rmode != 0) {
+      if (i >= nr) {
+        f = 0.0;
+      } else {
+        if (wmode > 0) f *= w[i];
+        if (wmode < 0) f /= w[i];
+        if (rmode > 0) f *= r[i];
+        if (rmode < 0) f /= r[i];
+      }
+    }
+    if (phimode != 0) {
+      const double mu = (m == 0 || m == nph) ? 1.0 : 2.0;
+      f = phimode > 0 ? f * (np * mu) : f / (np * mu);
+      zim = (m == 0 || m == nph);
+    }
+    v.x *= f;
+    v.y = zim ? 0.0 : v.y * f;
+    e[t] = v;
+  }
+}
+
 // zero outside the spectral support of a real, chopped field: rows
 // n >= nrcs(m), columns m >= npc, the z slices between the chops, and the
 // Nyquist column m = np/2 and slice kappa = nz/2
@@ -143,129 +179,21 @@ __global__ void k_lap_ln_t_(const double2 *e, double *ln_bar, const double *logn
   ln_bar[0] += scale * s;
 }
 
-constexpr int KL = 2, KU = 2, KV = 4, LDAB = 7;
-
-// (del2 + alpha)^T of system (m, kk): entry (i, j) of the transpose is the
-// band entry (j, i) of the forward matrix
-__device__ inline double d2t_entry(const double *d2c, int m, int ne, int i, int j, double akk, double alpha) {
-  const int d = i - j + 2;  // forward row j, column i
-  if (d < 0 || d > 4) return 0.0;
-  double v = d2c[((size_t)m * ne + j) * 5 + d];
-  if (d == 2) v = v - akk + alpha;
-  return v;
-}
-
-// one thread per (m, |kappa|) system: gbtf2 of the transposed matrix in
-// LAPACK band storage (element (i, j) at ab[(KV + i - j) + LDAB j])
-__global__ void k_lu_factor_t_(double *fac, int8_t *piv, const double *d2c, const double *ak, int nrdim, int nkk,
-                               int ne, int nrc, int npc, double alpha, int *info) {
-  const int kk = blockIdx.x * blockDim.x + threadIdx.x;
-  const int m = blockIdx.y;
-  if (kk >= nkk || m >= npc) return;
-  const int n = nrc - m;
-  if (n <= 0) return;
-  const size_t sys = (size_t)m * nkk + kk;
-  double *ab = fac + sys * (size_t)nrdim * LDAB;
-  int8_t *ip = piv + sys * (size_t)nrdim;
-  const double akk = ak[kk] * ak[kk];
-  for (int j = 0; j < n; ++j) {
-    for (int r = 0; r < LDAB; ++r) ab[r + LDAB * j] = 0.0;
-    for (int i = (j - KU > 0 ? j - KU : 0); i <= j + KL && i < n; ++i)
-      ab[(KV + i - j) + LDAB * j] = d2t_entry(d2c, m, ne, i, j, akk, alpha);
-  }
-  int ju = 0;
-  for (int j = 0; j < n; ++j) {
-    const int km = (KL < n - 1 - j) ? KL : n - 1 - j;
-    int jp = 0;
-    double best = fabs(ab[KV + LDAB * j]);
-    for (int r = 1; r <= km; ++r) {
-      const double v = fabs(ab[KV + r + LDAB * j]);
-      if (v > best) {
-        best = v;
-        jp = r;
-      }
-    }
-    ip[j] = (int8_t)jp;
-    if (ab[KV + jp + LDAB * j] != 0.0) {
-      int jnew = j + KU + jp;
-      if (jnew > n - 1) jnew = n - 1;
-      if (jnew > ju) ju = jnew;
-      if (jp != 0) {
-        for (int c = j; c <= ju; ++c) {
-          double *a = &ab[(KV + j - c) + LDAB * c];
-          double *b = &ab[(KV + j + jp - c) + LDAB * c];
-          const double t = *a;
-          *a = *b;
-          *b = t;
-        }
-      }
-      if (km > 0) {
-        const double rp = 1.0 / ab[KV + LDAB * j];
-        for (int r = 1; r <= km; ++r) ab[KV + r + LDAB * j] *= rp;
-        for (int c = j + 1; c <= ju; ++c) {
-          const double u = ab[(KV + j - c) + LDAB * c];
-          if (u != 0.0)
-            for (int r = 1; r <= km; ++r) ab[(KV + j + r - c) + LDAB * c] -= ab[KV + r + LDAB * j] * u;
-        }
-      }
-    } else {
-      atomicExch(info, 1);
-    }
-  }
-}
-
-// one thread per (m, z slice): gbtrs with the factors of |kappa| on rows
-// n < nrcs(m) of every slice inside the chops, each slice once
-__global__ void k_lu_solve_t_(const double *fac, const int8_t *piv, double2 *e, int nrdim, int npdim, int nz,
-                              int nkk, int nrc, int npc, int nzc, int nzcu) {
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  const int m = blockIdx.y;
-  if (k >= nz || m >= npc) return;
-  if (nz > 1 && !((k + 1 <= nzc) || (k + 1 >= nzcu))) return;
-  const int n = nrc - m;
-  if (n <= 0) return;
-  const int kk = k <= nz - k ? k : nz - k;
-  if (kk >= nkk) return;
-  const size_t sys = (size_t)m * nkk + kk;
-  const double *ab = fac + sys * (size_t)nrdim * LDAB;
-  const int8_t *ip = piv + sys * (size_t)nrdim;
-  double2 *b = e + (size_t)k * nrdim * npdim + (size_t)nrdim * m;
-  for (int j = 0; j < n - 1; ++j) {
-    const int lm = (KL < n - 1 - j) ? KL : n - 1 - j;
-    const int jp = ip[j];
-    if (jp != 0) {
-      const double2 t = b[j];
-      b[j] = b[j + jp];
-      b[j + jp] = t;
-    }
-    const double2 bj = b[j];
-    for (int r = 1; r <= lm; ++r) {
-      const double l = ab[KV + r + LDAB * j];
-      b[j + r].x -= l * bj.x;
-      b[j + r].y -= l * bj.y;
-    }
-  }
-  for (int j = n - 1; j >= 0; --j) {
-    const double d = ab[KV + LDAB * j];
-    double2 bj = b[j];
-    bj.x /= d;
-    bj.y /= d;
-    b[j] = bj;
-    const int i0 = j - KV > 0 ? j - KV : 0;
-    for (int i = i0; i < j; ++i) {
-      const double u = ab[(KV + i - j) + LDAB * j];
-      b[i].x -= u * bj.x;
-      b[i].y -= u * bj.y;
-    }
-  }
-}
-
 // the vector product's adjoint at (v, u): with a the cotangent of v x u,
 // v_bar = u x a and u_bar = a x v, on each packed real sample
+// (the product's samples: rows i < nr, packed columns < np/2; zero elsewhere)
 __global__ void k_vecprod_adj_(double2 *a1, double2 *a2, double2 *a3, const double2 *v1, const double2 *v2,
                                const double2 *v3, const double2 *u1, const double2 *u2, const double2 *u3,
-                               double2 *ub1, double2 *ub2, double2 *ub3, size_t n) {
+                               double2 *ub1, double2 *ub2, double2 *ub3, size_t n, int nr, int nrdim, int npdim,
+                               int nph) {
   for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (size_t)gridDim.x * blockDim.x) {
+    const int i = (int)(t % nrdim), j = (int)((t / nrdim) % npdim);
+    if (i >= nr || j >= nph) {
+      const double2 z2 = make_double2(0.0, 0.0);
+      a1[t] = a2[t] = a3[t] = z2;
+      ub1[t] = ub2[t] = ub3[t] = z2;
+      continue;
+    }
     const double2 A1 = a1[t], A2 = a2[t], A3 = a3[t];
     const double2 V1 = v1[t], V2 = v2[t], V3 = v3[t];
     const double2 U1 = u1[t], U2 = u2[t], U3 = u3[t];
@@ -277,6 +205,228 @@ __global__ void k_vecprod_adj_(double2 *a1, double2 *a2, double2 *a3, const doub
     ub1[t] = make_double2(A2.x * V3.x - A3.x * V2.x, A2.y * V3.y - A3.y * V2.y);
     ub2[t] = make_double2(A3.x * V1.x - A1.x * V3.x, A3.y * V1.y - A1.y * V3.y);
     ub3[t] = make_double2(A1.x * V2.x - A2.x * V1.x, A1.y * V2.y - A2.y * V1.y);
+  }
+}
+
+// ---- tp2vec transpose ----------------------------------------------------------
+// Forward (vector_reconstruction, mlegs_scalar_ops.f90:1455-1545), per (m,
+// kappa) column of a kept slice on rows j < nn = nrcs(m), before the radial
+// synthesis and the 1/r weight:
+//   ur_s = i m psi + i kappa X chi,  up_s = -X psi - m kappa chi,  uz_s = -H chi
+// with X the xxdx band (offsets -1, +1) and H the del2h band; psi.ln enters
+// up_s and chi.ln enters uz_s through the bands' log rows of column (0, 0).
+// The transpose:
+//   psi_b = -i m ur_b - X^T up_b,  chi_b = X^T (-i kappa ur_b) - m kappa up_b - H^T uz_b
+// on rows j < nn of the kept slices, zero elsewhere.
+// zlo..zhi (zlo > zhi: none) and columns m >= npc: taken as zero (written as
+// zero, their inputs not read), for cotangents that are de-aliased after
+__global__ void __launch_bounds__(64) k_tp2vec_t_(double2 *psib, double2 *chib, const double2 *urb,
+                                                  const double2 *upb, const double2 *uzb, const double *d2c,
+                                                  const double *xxc, const double *ak, int nrdim, int npdim, int nz,
+                                                  int ne, int nrc, int npc, int nzc, int nzcu, int zlo, int zhi) {
+  const int m = blockIdx.x, k = blockIdx.y;
+  const size_t base = (size_t)k * nrdim * npdim + (size_t)nrdim * m;
+  const int nn = (m < npc && nrc - m > 0) ? nrc - m : 0;
+  const bool slice = (nz == 1 || (k + 1 <= nzc) || (k + 1 >= nzcu)) && !(k >= zlo && k <= zhi);
+  const double kv = ak[k], mv = (double)m;
+  for (int j = threadIdx.x; j < nrdim; j += 64) {
+    double2 pb = make_double2(0.0, 0.0), cb = pb;
+    if (slice && j < nn) {
+      const double2 r = urb[base + j], p = upb[base + j];
+      pb = make_double2(mv * r.y, -mv * r.x);                 // -i m ur_b
+      cb = make_double2(-mv * kv * p.x, -mv * kv * p.y);      // -m kappa up_b
+      // X(i, j) for i = j - 1 (its +1 entry) and i = j + 1 (its -1 entry)
+      for (int s = -1; s <= 1; s += 2) {
+        const int i = j + s;
+        if (i < 0 || i >= nn) continue;
+        const double a = xxc[((size_t)m * ne + i) * 2 + (s < 0 ? 1 : 0)];
+        const double2 pi = upb[base + i], ri = urb[base + i];
+        pb.x -= a * pi.x;
+        pb.y -= a * pi.y;
+        cb.x += a * kv * ri.y;  // a (-i kappa) ur_b(i)
+        cb.y -= a * kv * ri.x;
+      }
+      for (int i = j - 2; i <= j + 2; ++i) {
+        if (i < 0 || i >= nn) continue;
+        const double a = d2c[((size_t)m * ne + i) * 5 + (j - i + 2)];
+        const double2 zi = uzb[base + i];
+        cb.x -= a * zi.x;
+        cb.y -= a * zi.y;
+      }
+    }
+    psib[base + j] = pb;
+    chib[base + j] = cb;
+  }
+}
+
+// the log rows: xxdx adds psi.ln / exp(lognorm(n, 0)) to rows n = 0, 1 of
+// column (0, 0) and up_s = -X psi; del2h adds (4/3, -2, 2/3) chi.ln / ell^2 /
+// exp(lognorm(n, 0)) to rows 0..2 and uz_s = -H chi
+__global__ void k_tp2vec_t_ln_(const double2 *upb, const double2 *uzb, double *psib_ln, double *chib_ln,
+                               const double *lognorm, double ell) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  const double ell2 = ell * ell;
+  psib_ln[0] = -(upb[0].x / exp(lognorm[0]) + upb[1].x / exp(lognorm[1]));
+  chib_ln[0] = -(4.0 / 3.0 / ell2 / exp(lognorm[0]) * uzb[0].x - 2.0 / 1.0 / ell2 / exp(lognorm[1]) * uzb[1].x +
+                 2.0 / 3.0 / ell2 / exp(lognorm[2]) * uzb[2].x);
+}
+
+// ---- vec2tp projection transpose ------------------------------------------------
+// Forward (vector_projection's five products, mlegs_scalar_ops.f90:1387-1436;
+// the plain FP64 kernel k_v2t_f64), per (m < npc, kept slice kappa) and table
+// row i < M = nrcs(m) + 3, with s_i = +1 (i even) or -1 (i odd), x < nr/2,
+// x~ = nr - 1 - x:
+//   A = V (ur(x) + s ur(x~)),  B = D (up(x) - s up(x~)),  C = D (ur(x) - s ur(x~)),
+//   Dd = V (up(x) + s up(x~)),  E = T (uz(x) + s uz(x~))   (A = Dd = 0 at m = 0)
+//   psi_i = -i m A - B,  chi_i = i kappa C + m kappa Dd - E
+// The transpose, from the cotangents of psi and chi:
+//   Ab = i m psib, Bb = -psib, Cb = -i kappa chib, Db = m kappa chib, Eb = -chib
+//   ur(x) += V Ab + D Cb, ur(x~) += s (V Ab - D Cb), up(x) += D Bb + V Db,
+//   up(x~) += s (V Db - D Bb), uz(x) += T Eb, uz(x~) += s T Eb
+// One workgroup per (m, kappa) column, one thread per x; rows staged in LDS
+// in chunks.  Every entry of the outputs is written (zero outside).
+// Workgroups of 256 threads: KB = 256 / NX kappa slices (NX = nr/2 rounded
+// up to 64, 128 or 256), one thread per (x, slice); each chunk of CH table
+// rows is staged in LDS once and read by all KB slices.
+constexpr int V2T_CH = 8;
+template <int NX>
+__global__ void __launch_bounds__(256) k_v2t_t_(double2 *urb, double2 *upb, double2 *uzb, const double2 *psib,
+                                                const double2 *chib, const double *tV, const double *tD,
+                                                const double *tT, const double *ak, int nrdim, int npdim, int nz,
+                                                int nr, int nrh, int nt, int nrc3, int npc, int nzc, int nzcu,
+                                                int zlo, int zhi) {
+  constexpr int KB = 256 / NX;
+  __shared__ double sT[3][V2T_CH][NX];
+  __shared__ double2 sQ[5][KB][V2T_CH];
+  const int m = blockIdx.x, tid = threadIdx.x, x = tid % NX, j = tid / NX;
+  const int k = blockIdx.y * KB + j;
+  int M = (m < npc && nrc3 - m > 0) ? nrc3 - m : 0;
+  if (M > nt) M = nt;
+  const double mv = (double)m;
+  const bool mz = (m == 0);
+  const size_t tb = (size_t)m * nt * nrh;
+  const double2 z2 = make_double2(0.0, 0.0);
+  double2 r0 = z2, r1 = z2, p0 = z2, p1 = z2, q0 = z2, q1 = z2;  // (x) and (x~) sums
+  for (int i0 = 0; i0 < M; i0 += V2T_CH) {
+    __syncthreads();
+    for (int idx = tid; idx < 3 * V2T_CH * NX; idx += 256) {
+      const int q = idx / (V2T_CH * NX), t = (idx / NX) % V2T_CH, xx = idx % NX, i = i0 + t;
+      const double *src = q == 0 ? tV : (q == 1 ? tD : tT);
+      sT[q][t][xx] = (i < M && xx < nrh) ? src[tb + (size_t)i * nrh + xx] : 0.0;
+    }
+    for (int idx = tid; idx < KB * V2T_CH; idx += 256) {
+      const int jj = idx / V2T_CH, t = idx % V2T_CH, i = i0 + t, kk = blockIdx.y * KB + jj;
+      double2 ps = z2, cs = z2;
+      double kv = 0.0;
+      if (i < M && kk < nz && (nz == 1 || (kk + 1 <= nzc) || (kk + 1 >= nzcu)) && !(kk >= zlo && kk <= zhi)) {
+        const size_t base = (size_t)kk * nrdim * npdim + (size_t)nrdim * m;
+        ps = psib[base + i];
+        cs = chib[base + i];
+        kv = ak[kk];
+      }
+      sQ[0][jj][t] = mz ? z2 : make_double2(-mv * ps.y, mv * ps.x);  // i m psib
+      sQ[1][jj][t] = make_double2(-ps.x, -ps.y);
+      sQ[2][jj][t] = make_double2(kv * cs.y, -kv * cs.x);            // -i kappa chib
+      sQ[3][jj][t] = mz ? z2 : make_double2(mv * kv * cs.x, mv * kv * cs.y);
+      sQ[4][jj][t] = make_double2(-cs.x, -cs.y);
+    }
+    __syncthreads();
+    const int iend = (M - i0 < V2T_CH) ? M - i0 : V2T_CH;
+#pragma unroll
+    for (int t = 0; t < V2T_CH; ++t) {
+      if (t >= iend) break;
+      const int i = i0 + t;
+      const double v = sT[0][t][x], d = sT[1][t][x], tt = sT[2][t][x];
+      const double sg = (i & 1) ? -1.0 : 1.0;
+      const double2 A = sQ[0][j][t], B = sQ[1][j][t], C = sQ[2][j][t], D = sQ[3][j][t], E = sQ[4][j][t];
+      const double vax = v * A.x, vay = v * A.y, dcx = d * C.x, dcy = d * C.y;
+      const double dbx = d * B.x, dby = d * B.y, vdx = v * D.x, vdy = v * D.y;
+      const double tex = tt * E.x, tey = tt * E.y;
+      r0.x += vax + dcx;
+      r0.y += vay + dcy;
+      r1.x += sg * (vax - dcx);
+      r1.y += sg * (vay - dcy);
+      p0.x += dbx + vdx;
+      p0.y += dby + vdy;
+      p1.x += sg * (vdx - dbx);
+      p1.y += sg * (vdy - dby);
+      q0.x += tex;
+      q0.y += tey;
+      q1.x += sg * tex;
+      q1.y += sg * tey;
+    }
+  }
+  if (k >= nz) return;
+  const size_t base = (size_t)k * nrdim * npdim + (size_t)nrdim * m;
+  for (int i = nr + x; i < nrdim; i += NX) {
+    urb[base + i] = z2;
+    upb[base + i] = z2;
+    uzb[base + i] = z2;
+  }
+  if (x < nrh) {
+    urb[base + x] = r0;
+    urb[base + nr - 1 - x] = r1;
+    upb[base + x] = p0;
+    upb[base + nr - 1 - x] = p1;
+    uzb[base + x] = q0;
+    uzb[base + nr - 1 - x] = q1;
+  }
+}
+
+// vec2tp's psi.ln = -1/2 Re calcat1(up)(kappa = 0) (:1371-1374) and the term
+// up(i, 0, 0) += psi.ln (1 + x_i), i < nr (:1377-1379): given the cotangent of
+// the PFF up (upP) and the FFF up (upF), upF(n, 0, 0) gains
+// -1/2 at1(n) sum_i (1 + x_i) Re upP(i, 0, 0) on rows n < nrchop.  One workgroup.
+__global__ void __launch_bounds__(256) k_v2t_up_ln_t_(double2 *upF, const double2 *upP, const double *x,
+                                                      const double *at1, int nr, int n) {
+  __shared__ double part[256];
+  double s = 0.0;
+  for (int i = threadIdx.x; i < nr; i += 256) s += (1.0 + x[i]) * upP[i].x;
+  part[threadIdx.x] = s;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (threadIdx.x < w) part[threadIdx.x] += part[threadIdx.x + w];
+    __syncthreads();
+  }
+  const double lb = part[0];
+  for (int i = threadIdx.x; i < n; i += 256) upF[i].x += -0.5 * at1[i] * lb;
+}
+
+// zeroat1 (mlegs_scalar_ops.f90:311-325) maps column m = 0 of each slice by
+// e(0) <- e(0) - sum_{n < N} at1(n) e(n) / at1(0); the transpose:
+// e(n) <- e(n) - at1(n) e(0) / at1(0) for n < N (e(0) becomes zero)
+__global__ void k_zeroat1_t_(double2 *e, const double *at1, int n, int nrdim, int npdim) {
+  double2 *c = e + (size_t)blockIdx.x * nrdim * npdim;
+  const double2 t = c[0];
+  __syncthreads();
+  const double d = at1[0];
+  for (int i = threadIdx.x; i < n; i += blockDim.x) {
+    const double f = at1[i] / d;
+    double2 v = c[i];
+    v.x -= f * t.x;
+    v.y -= f * t.y;
+    c[i] = v;
+  }
+}
+
+// the gradient of the kinetic energy 0.5 sum w_i ell^2/(1-x_i)^2 (2pi/np)
+// (zlen/nz) |v|^2 of a PPP velocity (k_energy_'s quadrature) with respect to
+// the stored samples: v <- cell w_i ell^2/(1-x_i)^2 v on rows i < nr and
+// packed columns < np/2, zero elsewhere
+__global__ void k_energy_grad_(double2 *vr, double2 *vp, double2 *vz, const double *x, const double *w, double ell,
+                               double cell, int nr, int nrdim, int npdim, int nph, size_t n) {
+  for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (size_t)gridDim.x * blockDim.x) {
+    const int i = (int)(t % nrdim);
+    const int j = (int)((t / nrdim) % npdim);
+    double f = 0.0;
+    if (i < nr && j < nph) {
+      const double om = 1.0 - x[i];
+      f = cell * w[i] * ell * ell / (om * om);
+    }
+    double2 a = vr[t], b = vp[t], c = vz[t];
+    vr[t] = make_double2(f * a.x, f * a.y);
+    vp[t] = make_double2(f * b.x, f * b.y);
+    vz[t] = make_double2(f * c.x, f * c.y);
   }
 }
 
@@ -306,6 +456,12 @@ void k_adj_scale(mlx_ctx *c, double2 *e, double s) {
   hipLaunchKernelGGL(k_adj_scale_, dim3(grid_for(c->nelem)), dim3(kT), 0, c->stream, e, c->nelem, s);
 }
 
+void k_adj_diag(mlx_ctx *c, double2 *e, int wmode, int rmode, int phimode, double s) {
+  const KitDev &k = c->kit;
+  hipLaunchKernelGGL(k_adj_diag_, dim3(grid_for(c->nelem)), dim3(kT), 0, c->stream, e, c->nelem, k.w, k.r, k.nr,
+                     k.nrdim, k.npdim, k.nph, k.np, wmode, rmode, phimode, s);
+}
+
 void k_adj_project(mlx_ctx *c, double2 *e) {
   const KitDev &k = c->kit;
   hipLaunchKernelGGL(k_adj_project_, dim3(grid_for(c->nelem)), dim3(kT), 0, c->stream, e, c->nelem, k.nrdim,
@@ -324,63 +480,101 @@ void k_lap_ln_t(mlx_ctx *c, const double2 *e, double *ln_bar, double scale) {
   hipLaunchKernelGGL(k_lap_ln_t_, dim3(1), dim3(64), 0, c->stream, e, ln_bar, c->kit.lognorm, c->kit.ell, scale);
 }
 
-// (del2 + alpha)^-T per (m, kappa) system on rows n < nrcs(m), factors
-// cached per alpha (two sets kept)
-void k_helm_solve_t(mlx_ctx *c, double2 *e, double alpha) {
-  const KitDev &k = c->kit;
-  const int nkk = (k.nzdim > 1) ? k.nzdim / 2 + 1 : 1;
-  AdjLu *hit = nullptr;
-  for (AdjLu &a : c->adj_lu)
-    if (a.alpha == alpha) hit = &a;
-  if (!hit) {
-    if (c->adj_lu.size() >= 2) {
-      HIPCHK(hipStreamSynchronize(c->stream));
-      (void)hipFree(c->adj_lu.front().fac);
-      (void)hipFree(c->adj_lu.front().piv);
-      c->adj_lu.erase(c->adj_lu.begin());
-    }
-    AdjLu a;
-    a.alpha = alpha;
-    const size_t nsys = (size_t)k.npchop * nkk;
-    if (hipMalloc(&a.fac, sizeof(double) * nsys * k.nrdim * LDAB) != hipSuccess ||
-        hipMalloc(&a.piv, nsys * k.nrdim) != hipSuccess) {
-      if (a.fac) (void)hipFree(a.fac);
-      MLX_THROW(MLX_E_ALLOC, "ihelm adjoint: cannot allocate LU factors");
-    }
-    int *dinfo = reinterpret_cast<int *>(c->dsmall + 4000);
-    k_set_flag(c, dinfo, 0);
-    hipLaunchKernelGGL(k_lu_factor_t_, dim3((nkk + 63) / 64, k.npchop), dim3(64), 0, c->stream, a.fac, a.piv, k.d2c,
-                       k.ak, k.nrdim, nkk, k.ne, k.nrchop, k.npchop, alpha, dinfo);
-    HIPCHK(hipGetLastError());
-    int info = 0;
-    HIPCHK(hipMemcpyAsync(&info, dinfo, sizeof(int), hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(hipStreamSynchronize(c->stream));
-    if (info != 0) {
-      (void)hipFree(a.fac);
-      (void)hipFree(a.piv);
-      MLX_THROW(MLX_E_ARG, "lurc: lu factorization resulted in failure");
-    }
-    c->adj_lu.push_back(a);
-    hit = &c->adj_lu.back();
+// (del2 + alpha)^-T per (m, kappa) system on rows n < nrcs(m) + nroff;
+// kind 1: idel2_proln's systems (alpha = 0, the (0,0) system bordered).  The
+// transposed matrices are factored and solved by the banded LU of solve.hip
+// (kind + 4), over every column and kept slice.
+void k_helm_solve_t(mlx_ctx *c, double2 *e, double alpha, int kind, int nroff, int pcut, int zlo, int zhi) {
+  k_helm_solve(c, e, alpha, kind | 4, nroff, pcut, zlo, zhi);
+}
+
+// abcn's transpose after the solve: nlbar = c1 y, nlpbar = c2 y, then
+// y = a y + c3 t (t = del2^T y), one pass
+__global__ void k_abcn_adj_out_(double2 *y, const double2 *t, double2 *nlbar, double2 *nlpbar, size_t n, double c1,
+                                double c2, double a, double c3) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+    const double2 v = y[i], w = t[i];
+    nlbar[i] = make_double2(c1 * v.x, c1 * v.y);
+    nlpbar[i] = make_double2(c2 * v.x, c2 * v.y);
+    y[i] = make_double2(a * v.x + c3 * w.x, a * v.y + c3 * w.y);
   }
-  hipLaunchKernelGGL(k_lu_solve_t_, dim3((k.nzdim + 63) / 64, k.npchop), dim3(64), 0, c->stream, hit->fac, hit->piv,
-                     e, k.nrdim, k.npdim, k.nzdim, nkk, k.nrchop, k.npchop, k.chopzl, k.chopzu);
+}
+
+__global__ void k_abcn_adj_ln_(const double *yln, double *nlln, double *nlpln, double f1, double f2) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    nlln[0] = f1 * yln[0];
+    nlpln[0] = f2 * yln[0];
+  }
+}
+
+void k_abcn_adj_out(mlx_ctx *c, double2 *y, const double2 *t, double2 *nlbar, double2 *nlpbar, double c1, double c2,
+                    double a, double c3) {
+  hipLaunchKernelGGL(k_abcn_adj_out_, dim3(grid_for(c->nelem)), dim3(kT), 0, c->stream, y, t, nlbar, nlpbar, c->nelem,
+                     c1, c2, a, c3);
+}
+
+void k_abcn_adj_ln(mlx_ctx *c, const double *yln, double *nlln, double *nlpln, double f1, double f2) {
+  hipLaunchKernelGGL(k_abcn_adj_ln_, dim3(1), dim3(64), 0, c->stream, yln, nlln, nlpln, f1, f2);
+}
+
+void k_tp2vec_t(mlx_ctx *c, double2 *psib, double2 *chib, const double2 *urb, const double2 *upb, const double2 *uzb,
+                double *psib_ln, double *chib_ln, int ncols, int zlo, int zhi) {
+  const KitDev &k = c->kit;
+  if (k.nrchop > k.ne || k.nrchop < 3) MLX_THROW(MLX_E_ARG, "tp2vec adjoint: table too small");
+  const int npc = (ncols > 0 && ncols < k.npchop) ? ncols : k.npchop;
+  hipLaunchKernelGGL(k_tp2vec_t_, dim3(k.npdim, k.nzdim), dim3(64), 0, c->stream, psib, chib, urb, upb, uzb, k.d2c,
+                     k.xxc, k.ak, k.nrdim, k.npdim, k.nzdim, k.ne, k.nrchop, npc, k.chopzl, k.chopzu, zlo, zhi);
+  hipLaunchKernelGGL(k_tp2vec_t_ln_, dim3(1), dim3(64), 0, c->stream, upb, uzb, psib_ln, chib_ln, k.lognorm, k.ell);
+  HIPCHK(hipGetLastError());
+}
+
+void k_v2t_t(mlx_ctx *c, double2 *urb, double2 *upb, double2 *uzb, const double2 *psib, const double2 *chib,
+             int ncols, int zlo, int zhi) {
+  const KitDev &k = c->kit;
+  if (k.nrh > 256 || (k.nr & 1)) MLX_THROW(MLX_E_ARG, "vec2tp adjoint: nr must be even and at most 512");
+  if (k.nt < k.nrchop + 3) MLX_THROW(MLX_E_ARG, "vec2tp adjoint: projection tables too small");
+  const int npc = (ncols > 0 && ncols < k.npchop) ? ncols : k.npchop;
+#define MLX_V2TT(NX_)                                                                                            \
+  hipLaunchKernelGGL(k_v2t_t_<NX_>, dim3(k.npdim, (k.nzdim + 256 / NX_ - 1) / (256 / NX_)), dim3(256), 0, c->stream, \
+                     urb, upb, uzb, psib, chib, k.tV, k.tD, k.tT, k.ak, k.nrdim, k.npdim, k.nzdim, k.nr, k.nrh, k.nt,  \
+                     k.nrchop + 3, npc, k.chopzl, k.chopzu, zlo, zhi)
+  if (k.nrh <= 64) MLX_V2TT(64);
+  else if (k.nrh <= 128) MLX_V2TT(128);
+  else MLX_V2TT(256);
+#undef MLX_V2TT
+  HIPCHK(hipGetLastError());
+}
+
+void k_v2t_up_ln_t(mlx_ctx *c, double2 *upF, const double2 *upP) {
+  const KitDev &k = c->kit;
+  hipLaunchKernelGGL(k_v2t_up_ln_t_, dim3(1), dim3(256), 0, c->stream, upF, upP, k.x, k.at1, k.nr,
+                     k.nrchop < k.nrdim ? k.nrchop : k.nrdim);
+  HIPCHK(hipGetLastError());
+}
+
+void k_zeroat1_t(mlx_ctx *c, double2 *e) {
+  const KitDev &k = c->kit;
+  const int n = k.nrchop < k.nrdim ? k.nrchop : k.nrdim;
+  hipLaunchKernelGGL(k_zeroat1_t_, dim3(k.nzdim), dim3(kT), 0, c->stream, e, k.at1, n, k.nrdim, k.npdim);
+  HIPCHK(hipGetLastError());
+}
+
+void k_energy_grad(mlx_ctx *c, double2 *vr, double2 *vp, double2 *vz) {
+  const KitDev &k = c->kit;
+  const double cell = (2.0 * 3.14159265358979323846 / k.np) * (k.zlen / k.nz);
+  hipLaunchKernelGGL(k_energy_grad_, dim3(grid_for(c->nelem)), dim3(kT), 0, c->stream, vr, vp, vz, k.x, k.w, k.ell,
+                     cell, k.nr, k.nrdim, k.npdim, k.nph, c->nelem);
   HIPCHK(hipGetLastError());
 }
 
 void k_vecprod_adj(mlx_ctx *c, double2 *a1, double2 *a2, double2 *a3, const double2 *v1, const double2 *v2,
                    const double2 *v3, const double2 *u1, const double2 *u2, const double2 *u3, double2 *ub1,
                    double2 *ub2, double2 *ub3) {
+  const KitDev &k = c->kit;
   hipLaunchKernelGGL(k_vecprod_adj_, dim3(grid_for(c->nelem)), dim3(kT), 0, c->stream, a1, a2, a3, v1, v2, v3, u1,
-                     u2, u3, ub1, ub2, ub3, c->nelem);
+                     u2, u3, ub1, ub2, ub3, c->nelem, k.nr, k.nrdim, k.npdim, k.nph);
 }
 
-void adj_cache_free(mlx_ctx *c) {
-  for (AdjLu &a : c->adj_lu) {
-    (void)hipFree(a.fac);
-    (void)hipFree(a.piv);
-  }
-  c->adj_lu.clear();
-}
+
 
 }  // namespace mlx

@@ -545,6 +545,48 @@ void adj_no_offsets(const mlx_field *f, const char *what) {
     MLX_THROW(MLX_E_ARG, std::string(what) + ": chopping offsets are not supported by the adjoint");
 }
 
+// The transpose of trans without its log term (for cotangents whose log
+// coefficient is not wanted), with a row factor r^rmode on the physical
+// side; the diagonal factors run as one pass.  to_fff: g in PPP holds the
+// cotangent of r^rmode B(x) and returns that of x, in FFF; otherwise g in
+// FFF holds the cotangent of F(r^rmode y) and returns that of y, in PPP.
+// lim (optional, with the four-step phi kernels): the FFF side is de-aliased
+// (columns m >= pcut and slices zlo..zhi zero on input, or not needed on
+// output: to_fff leaves them unwritten)
+void trans_adjoint_nolog(mlx_ctx *c, mlx_field *g, bool to_fff, int rmode, const DealiasLimits *lim = nullptr) {
+  const KitDev &k = c->kit;
+  const int nc = lim ? lim->pcut : -1;
+  const int zlo = lim ? lim->zlo : 1, zhi = lim ? lim->zhi : 0;
+  const int ncz = lim ? lim->pcut : 0;
+  if (to_fff) {
+    // B^T = Z_b^T L_b^T Phi_b^T (L_b^T = L_f W^-1, Z_b^T = nz Z_f)
+    if (k.np > 1) fft_phi(c, g->e, g->e, -1);
+    k_adj_diag(c, g->e, -1, rmode, k.np > 1 ? +1 : 0, k.nz > 1 ? (double)k.nz : 1.0);
+    leg_forward(c, g->e, c->scr[0], 0, true, nc, !lim);
+    swap_scr(c, g);
+    if (k.nz > 1) {
+      if (lim)
+        fft_z(c, g->e, -1, active_pairs(k, 0, nc), k.nrchop, ncz, 1, 0, zlo, zhi);
+      else
+        fft_z(c, g->e, -1, (double)k.nrdim * k.npdim);
+    }
+    set_space(g, "FFF");
+  } else {
+    // F^T = Phi_f^T L_f^T Z_f^T (L_f^T = W L_b, Z_f^T = Z_b / nz)
+    if (k.nz > 1) {
+      if (lim)
+        fft_z(c, g->e, +1, active_pairs(k, 0, nc), k.nrchop, ncz, zlo, zhi);
+      else
+        fft_z(c, g->e, +1, (double)k.nrdim * k.npdim);
+    }
+    leg_backward(c, g->e, c->scr[0], 0, true, nc, !lim);
+    swap_scr(c, g);
+    k_adj_diag(c, g->e, +1, rmode, k.np > 1 ? -1 : 0, k.nz > 1 ? 1.0 / k.nz : 1.0);
+    if (k.np > 1) fft_phi(c, g->e, g->e, +1, 0, lim ? lim->pcut : -1);
+    set_space(g, "PPP");
+  }
+}
+
 // g holds a cotangent of trans(x, space of g) for x in space `from` (FFF or
 // PPP, the other end); on return g holds the cotangent of x, in `from`
 void trans_adjoint(mlx_ctx *c, mlx_field *g, const char *from) {
@@ -606,22 +648,220 @@ void abcn_adjoint(mlx_ctx *c, mlx_field *sbar, mlx_field *nlbar, mlx_field *nlpb
   // y <- (del2 + a)^-T y; the cotangent of L gains -lap_ln^T y
   k_helm_solve_t(c, y->e, a);
   k_lap_ln_t(c, y->e, y->sc, -1.0);
-  // nl and nl_p
-  fcopy(c, nlbar, y);
-  k_axpby(c, nlbar->e, nullptr, 1.5 * a * dt, y->e, nullptr, 0.0);
-  k_copy_scalar(c, nlbar->sc, y->sc, 1.5 * dt);
-  fcopy(c, nlpbar, y);
-  k_axpby(c, nlpbar->e, nullptr, -0.5 * a * dt, y->e, nullptr, 0.0);
-  k_copy_scalar(c, nlpbar->sc, y->sc, -0.5 * dt);
-  // s: a (y + dt/2 visc del2^T y), and the log term's Laplacian inside del2
+  // nl and nl_p: 1.5 a dt y and -0.5 a dt y (log terms 1.5 dt and -0.5 dt
+  // of y's); s: a (y + dt/2 visc del2^T y), and the log term's Laplacian
+  // inside del2
+  k_abcn_adj_ln(c, y->sc, nlbar->sc, nlpbar->sc, 1.5 * dt, -0.5 * dt);
   mlx_field *t = tmp(c, 3);
   k_del2_t(c, t->e, y->e, true);
   k_lap_ln_t(c, y->e, y->sc, a * dt / 2.0 * c->p.visc);
-  k_axpby(c, y->e, nullptr, a, t->e, nullptr, a * dt / 2.0 * c->p.visc);
+  k_abcn_adj_out(c, y->e, t->e, nlbar->e, nlpbar->e, 1.5 * a * dt, -0.5 * a * dt, a, a * dt / 2.0 * c->p.visc);
+  for (mlx_field *f : {nlbar, nlpbar}) f->off[0] = f->off[1] = f->off[2] = 0;
   k_adj_project(c, y->e);
   set_space(y, "FFF");
   set_space(nlbar, "FFF");
   set_space(nlpbar, "FFF");
+}
+
+// febe (hyperpow = 0) maps (s, nl) to s' = (del2 + a)^-1 (a (s + dt nl) -
+// lap_ln(L)), s'.ln = L = s.ln + dt nl.ln, a = -1 / (dt visc)
+// (mlegs_scalar_ops.f90:1157-1198).  sbar: the cotangent of s' on entry, of
+// s on return; nlbar: the cotangent of nl (overwritten).
+void febe_adjoint(mlx_ctx *c, mlx_field *sbar, mlx_field *nlbar, double dt) {
+  need_kit(c);
+  for (mlx_field *f : {sbar, nlbar}) adj_no_offsets(f, "febe adjoint");
+  if (!is_fff(sbar)) MLX_THROW(MLX_E_LAYOUT, "febe adjoint: the cotangent must be in FFF");
+  if (c->p.hyperpow != 0) MLX_THROW(MLX_E_ARG, "febe adjoint: the hyperviscous stepper has no adjoint here");
+  if (c->p.visc < 5e-14) MLX_THROW(MLX_E_ARG, "febe: inviscid case and no linear term in rhs. semi-implicit time adv is impossible");
+  const double a = -1.0 / (dt * c->p.visc);
+  mlx_field *y = sbar;
+  k_adj_project(c, y->e);
+  k_helm_solve_t(c, y->e, a);
+  k_lap_ln_t(c, y->e, y->sc, -1.0);
+  fcopy(c, nlbar, y);
+  k_adj_scale(c, nlbar->e, a * dt);
+  k_copy_scalar(c, nlbar->sc, y->sc, dt);
+  k_adj_scale(c, y->e, a);
+  k_adj_project(c, y->e);
+  set_space(nlbar, "FFF");
+}
+
+// Runs the enclosed operations with every contraction in FP64 (on FP64
+// MFMA where the kit's shape allows it, whose transforms equal the plain
+// FP64 kernels' bit for bit) and restores the context's precision settings
+// afterwards: the adjoints linearise and transpose in FP64 whatever
+// precision the forward run used.
+struct Fp64Scope {
+  mlx_ctx *c;
+  int prec, prec_ctx, stage[MLX_STAGE_COUNT];
+  explicit Fp64Scope(mlx_ctx *c_) : c(c_), prec(c_->prec), prec_ctx(c_->prec_ctx) {
+    std::memcpy(stage, c->stage_prec, sizeof(stage));
+    c->prec = c->prec_ctx = MLX_PREC_FP64_MFMA;
+    for (int &v : c->stage_prec) v = -1;
+  }
+  ~Fp64Scope() {
+    c->prec = prec;
+    c->prec_ctx = prec_ctx;
+    std::memcpy(c->stage_prec, stage, sizeof(stage));
+  }
+};
+
+// host-side metadata of a cotangent (no offsets, its space); zero_ln: also
+// set its log coefficient to zero
+void clear_meta(mlx_ctx *c, mlx_field *f, const char *space, bool zero_ln = false) {
+  if (zero_ln) k_set_scalar(c, f->sc, 0.0);
+  f->off[0] = f->off[1] = f->off[2] = 0;
+  set_space(f, space);
+}
+
+// the transpose of y -> Z_f Phi_f (r^rmode y) (the phi and z transforms
+// without the radial one): g in PFF on entry, PPP on return
+void pff_adjoint(mlx_ctx *c, mlx_field *g, int rmode, const DealiasLimits *lim = nullptr) {
+  const KitDev &k = c->kit;
+  if (k.nz > 1) {
+    if (lim)
+      fft_z(c, g->e, +1, (double)k.nr * lim->pcut, 0, lim->pcut, lim->zlo, lim->zhi);
+    else
+      fft_z(c, g->e, +1, (double)k.nrdim * k.npdim);
+  }
+  k_adj_diag(c, g->e, 0, rmode, k.np > 1 ? -1 : 0, k.nz > 1 ? 1.0 / k.nz : 1.0);
+  if (k.np > 1) fft_phi(c, g->e, g->e, +1, 0, lim ? lim->pcut : -1);
+  set_space(g, "PPP");
+}
+
+// tp2vec's transpose (vector_reconstruction, mlegs_scalar_ops.f90:1455-1545):
+// urb, upb, uzb: the cotangents of vr, vp, vz in PPP (overwritten); psib,
+// chib: the cotangents of psi and chi (data and log coefficients) on return.
+// The chop of the intermediate fields restricts everything to rows n <
+// nrcs(m) of the kept slices.
+void tp2vec_adjoint(mlx_ctx *c, mlx_field *urb, mlx_field *upb, mlx_field *uzb, mlx_field *psib, mlx_field *chib,
+                    const DealiasLimits *lim = nullptr) {
+  need_kit(c);
+  for (mlx_field *f : {urb, upb, uzb})
+    if (!is_ppp(f)) MLX_THROW(MLX_E_LAYOUT, "vector_reconstruction adjoint: the cotangents must be in PPP");
+  // vr = B(ur_s) / r, vp = B(up_s) / r, vz = B(uz_s); the log terms of ur_s,
+  // up_s, uz_s are zero, so no cotangent of theirs is formed
+  trans_adjoint_nolog(c, urb, true, -1, lim);
+  trans_adjoint_nolog(c, upb, true, -1, lim);
+  trans_adjoint_nolog(c, uzb, true, 0, lim);
+  if (lim)
+    k_tp2vec_t(c, psib->e, chib->e, urb->e, upb->e, uzb->e, psib->sc, chib->sc, lim->pcut, lim->zlo, lim->zhi);
+  else
+    k_tp2vec_t(c, psib->e, chib->e, urb->e, upb->e, uzb->e, psib->sc, chib->sc);
+  clear_meta(c, urb, "FFF");
+  clear_meta(c, upb, "FFF");
+  clear_meta(c, uzb, "FFF");
+  for (mlx_field *f : {psib, chib}) {
+    f->off[0] = f->off[1] = f->off[2] = 0;
+    set_space(f, "FFF");
+  }
+}
+
+// tp2curlvec's transpose (curl V = tp2vec(-del2 chi, psi), :1547-1560)
+void tp2curlvec_adjoint(mlx_ctx *c, mlx_field *wrb, mlx_field *wpb, mlx_field *wzb, mlx_field *psib,
+                        mlx_field *chib, const DealiasLimits *lim = nullptr) {
+  mlx_field *md = tmp(c, 21);
+  // md = -del2 chi has a zero log coefficient: its cotangent is dropped
+  tp2vec_adjoint(c, wrb, wpb, wzb, md, psib, lim);
+  k_del2_t(c, chib->e, md->e, true);
+  k_adj_scale(c, chib->e, -1.0);
+  k_set_scalar(c, chib->sc, 0.0);
+  k_lap_ln_t(c, md->e, chib->sc, -1.0);
+  chib->off[0] = chib->off[1] = chib->off[2] = 0;
+  set_space(chib, "FFF");
+}
+
+// vec2tp's transpose (vector_projection, :1308-1453) with respect to the data
+// of its result (advection_rhs sets the result's log coefficients to zero,
+// so theirs are not part of it).  psib, chib: the cotangents of psi and chi
+// in FFF (overwritten); vrb, vpb, vzb: the cotangents of vr, vp, vz in PPP.
+// lim (optional): psib and chib are de-aliased, and so is what the caller
+// makes of the result, which lets every transform skip the zero modes
+void vec2tp_adjoint(mlx_ctx *c, mlx_field *psib, mlx_field *chib, mlx_field *vrb, mlx_field *vpb, mlx_field *vzb,
+                    const DealiasLimits *lim = nullptr) {
+  need_kit(c);
+  for (mlx_field *f : {psib, chib}) {
+    adj_no_offsets(f, "vec2tp adjoint");
+    if (!is_fff(f)) MLX_THROW(MLX_E_LAYOUT, "vector_projection adjoint: the cotangents must be in FFF");
+  }
+  // zeroat1 and the chop to offset 0
+  k_zeroat1_t(c, psib->e);
+  k_zeroat1_t(c, chib->e);
+  chop(c, psib);
+  chop(c, chib);
+  // idel2_proln on chi's own rows (offset 0: the projection's three extra
+  // rows are left as they are by the solve and chopped after it), the (0,0)
+  // system bordered
+  const int nc = lim ? lim->pcut : -1, zlo = lim ? lim->zlo : 1, zhi = lim ? lim->zhi : 0;
+  k_helm_solve_t(c, chib->e, 0.0, 1, 0, nc, zlo, zhi);
+  // the five projection products: cotangents of ur = F(r vr), up, uz = F(vz)
+  // in PFF
+  mlx_field *upP = tmp(c, 20);
+  k_v2t_t(c, vrb->e, upP->e, vzb->e, psib->e, chib->e, nc, zlo, zhi);
+  // up = L_b Z_f L_f Phi_f (r vp) + psi.ln (1 + x), psi.ln = -1/2 Re
+  // calcat1(Z_f L_f Phi_f (r vp))(0): L_b^T = L_f W^-1 on the PFF cotangent,
+  // then the calcat term, then the transpose of the forward transform
+  fcopy(c, vpb, upP);
+  k_adj_wscale(c, vpb->e, -1);
+  leg_forward(c, vpb->e, c->scr[0], 0, true, nc, true);
+  swap_scr(c, vpb);
+  k_v2t_up_ln_t(c, vpb->e, upP->e);
+  trans_adjoint_nolog(c, vpb, false, +1, lim);
+  // ur = Z_f Phi_f (r vr), uz = Z_f Phi_f vz
+  pff_adjoint(c, vrb, +1, lim);
+  pff_adjoint(c, vzb, 0, lim);
+  for (mlx_field *f : {vrb, vpb, vzb}) clear_meta(c, f, "PPP", true);
+}
+
+// advection_rhs's vector-Jacobian product at (psi, chi) (vortical_flow_3d.f90:
+// 353-395): N = dealias(vec2tp((tp2vec(psi, chi) + U) x tp2curlvec(psi, chi)))
+// with psi and chi de-aliased first.  nlpsib, nlchib: the cotangents of N
+// (overwritten); psib, chib: the cotangents of psi and chi on return.  The
+// operators run in FP64.
+// psi and chi are de-aliased in place, as advection_rhs does.
+void advection_adjoint(mlx_ctx *c, mlx_field *psi, mlx_field *chi, const mlx_field *uz, mlx_field *nlpsib,
+                       mlx_field *nlchib, mlx_field *psib, mlx_field *chib) {
+  need_kit(c);
+  if (!is_fff(psi) || !is_fff(chi)) MLX_THROW(MLX_E_LAYOUT, "advection_rhs adjoint: psi and chi must be in FFF");
+  if (!is_ppp(uz)) MLX_THROW(MLX_E_LAYOUT, "advection_rhs adjoint: uz must be in PPP");
+  for (const mlx_field *f : {psi, chi}) adj_no_offsets(f, "advection_rhs adjoint");
+  Fp64Scope fp64(c);
+  dealias(c, psi);
+  dealias(c, chi);
+  // v and w at (psi, chi): the fused column pass and the backward
+  // transforms of advection_rhs, skipping what de-aliasing zeroes, in FP64
+  const DealiasLimits dl = dealias_limits(c);
+  const int ncols = dl.on ? dl.pcut : -1;
+  const bool zskip = fft_z_fast(c->kit);
+  mlx_field *vr = tmp(c, 24), *vp = tmp(c, 25), *vz = tmp(c, 26);
+  mlx_field *wr = tmp(c, 27), *wp = tmp(c, 28), *wz = tmp(c, 29);
+  k_tp2vec_front2(c, vr->e, vp->e, vz->e, vr->sc, vp->sc, vz->sc, wr->e, wp->e, wz->e, wr->sc, wp->sc, wz->sc,
+                  psi->e, chi->e, psi->sc, chi->sc, chi->off[0], ncols, zskip ? dl.zlo : 1, zskip ? dl.zhi : 0);
+  tp2vec_back(c, vr, vp, vz, ncols, dl.zlo, dl.zhi, true, false);
+  tp2vec_back(c, wr, wp, wz, ncols, dl.zlo, dl.zhi, true, false);
+  k_axpby(c, vz->e, nullptr, 1.0, uz->e, nullptr, 1.0);
+  set_space(nlpsib, "FFF");
+  set_space(nlchib, "FFF");
+  dealias(c, nlpsib);
+  dealias(c, nlchib);
+  // the transforms below skip the de-aliased modes (the four-step phi
+  // kernels take the mode limit)
+  const DealiasLimits *lim = (dl.on && fft_phi_fast(c->kit)) ? &dl : nullptr;
+  mlx_field *ar = tmp(c, 30), *ap = tmp(c, 31), *az = tmp(c, 32);
+  vec2tp_adjoint(c, nlpsib, nlchib, ar, ap, az, lim);
+  // (v + U) x w: v_b = w x a, w_b = a x (v + U)
+  mlx_field *br = tmp(c, 33), *bp = tmp(c, 34), *bz = tmp(c, 35);
+  k_vecprod_adj(c, ar->e, ap->e, az->e, vr->e, vp->e, vz->e, wr->e, wp->e, wz->e, br->e, bp->e, bz->e);
+  set_space(br, "PPP");
+  set_space(bp, "PPP");
+  set_space(bz, "PPP");
+  tp2vec_adjoint(c, ar, ap, az, psib, chib, lim);
+  mlx_field *pb = tmp(c, 36), *cb = tmp(c, 37);
+  tp2curlvec_adjoint(c, br, bp, bz, pb, cb, lim);
+  k_axpby(c, psib->e, psib->sc, 1.0, pb->e, pb->sc, 1.0);
+  k_axpby(c, chib->e, chib->sc, 1.0, cb->e, cb->sc, 1.0);
+  dealias(c, psib);
+  dealias(c, chib);
 }
 
 // dst = PFP form of the PPP field src (optionally r-weighted): metadata and log
@@ -852,7 +1092,6 @@ int mlx_ctx_destroy(mlx_ctx *c) {
   for (mlx_field *f : c->pool) field_free(f);
   c->pool.clear();
   lu_cache_free(c);
-  adj_cache_free(c);
   for (auto &v : c->probe.ev)
     for (hipEvent_t e : v) (void)hipEventDestroy(e);
   for (auto &kv : c->work)
@@ -1237,6 +1476,83 @@ int mlx_abcn_adjoint(mlx_ctx *c, mlx_field *sbar, mlx_field *nlbar, mlx_field *n
   return guarded(c, [&] { abcn_adjoint(c, sbar, nlbar, nlpbar, dt); });
 }
 
+int mlx_febe_adjoint(mlx_ctx *c, mlx_field *sbar, mlx_field *nlbar, double dt) {
+  return guarded(c, [&] { febe_adjoint(c, sbar, nlbar, dt); });
+}
+
+int mlx_zeroat1_adjoint(mlx_ctx *c, mlx_field *f) {
+  return guarded(c, [&] {
+    need_kit(c);
+    if (!is_fff(f)) MLX_THROW(MLX_E_LAYOUT, "zeroat1 adjoint: the field must be in FFF");
+    adj_no_offsets(f, "zeroat1 adjoint");
+    k_zeroat1_t(c, f->e);
+  });
+}
+
+int mlx_tp2vec_adjoint(mlx_ctx *c, mlx_field *vrb, mlx_field *vpb, mlx_field *vzb, mlx_field *psib,
+                       mlx_field *chib) {
+  return guarded(c, [&] {
+    Fp64Scope fp64(c);
+    tp2vec_adjoint(c, vrb, vpb, vzb, psib, chib);
+  });
+}
+
+int mlx_tp2curlvec_adjoint(mlx_ctx *c, mlx_field *wrb, mlx_field *wpb, mlx_field *wzb, mlx_field *psib,
+                           mlx_field *chib) {
+  return guarded(c, [&] {
+    need_kit(c);
+    Fp64Scope fp64(c);
+    tp2curlvec_adjoint(c, wrb, wpb, wzb, psib, chib);
+  });
+}
+
+int mlx_vec2tp_adjoint(mlx_ctx *c, mlx_field *psib, mlx_field *chib, mlx_field *vrb, mlx_field *vpb,
+                       mlx_field *vzb) {
+  return guarded(c, [&] {
+    Fp64Scope fp64(c);
+    vec2tp_adjoint(c, psib, chib, vrb, vpb, vzb);
+  });
+}
+
+int mlx_advection_adjoint(mlx_ctx *c, mlx_field *psi, mlx_field *chi, const mlx_field *uz, mlx_field *nlpsib,
+                          mlx_field *nlchib, mlx_field *psib, mlx_field *chib) {
+  return guarded(c, [&] { advection_adjoint(c, psi, chi, uz, nlpsib, nlchib, psib, chib); });
+}
+
+// one reverse step of the vortex run (the map of mlx_vortex_step: s' =
+// zeroat1(dealias(abcn(s, N, N_prev))), N' = N(s'), N_prev' = N): from the
+// cotangents (sb, nb, npb) of the state after the step, whose psi and chi
+// are p1, c1, to those of the state before it, in place
+int mlx_vortex_reverse_step(mlx_ctx *c, mlx_field *p1, mlx_field *c1, const mlx_field *uz, mlx_field *sbpsi,
+                            mlx_field *sbchi, mlx_field *nbpsi, mlx_field *nbchi, mlx_field *npbpsi,
+                            mlx_field *npbchi, double dt) {
+  return guarded(c, [&] {
+    mlx_field *sb[2] = {sbpsi, sbchi}, *nb[2] = {nbpsi, nbchi}, *npb[2] = {npbpsi, npbchi};
+    mlx_field *g[2] = {tmp(c, 38), tmp(c, 39)};
+    advection_adjoint(c, p1, c1, uz, nb[0], nb[1], g[0], g[1]);
+    for (int i = 0; i < 2; ++i) {
+      k_axpby(c, sb[i]->e, sb[i]->sc, 1.0, g[i]->e, g[i]->sc, 1.0);
+      k_zeroat1_t(c, sb[i]->e);
+      dealias(c, sb[i]);
+      // abcn^T: nb receives the cotangent of N, npb's old value is the
+      // cotangent of the history N_prev' = N (added), the new npb that of N_prev
+      mlx_field *b = tmp(c, 40 + i);
+      abcn_adjoint(c, sb[i], nb[i], b, dt);
+      k_axpby(c, nb[i]->e, nb[i]->sc, 1.0, npb[i]->e, npb[i]->sc, 1.0);
+      fmove(npb[i], b);
+    }
+  });
+}
+
+int mlx_energy_grad(mlx_ctx *c, mlx_field *vr, mlx_field *vp, mlx_field *vz) {
+  return guarded(c, [&] {
+    need_kit(c);
+    for (const mlx_field *f : {vr, vp, vz})
+      if (!is_ppp(f)) MLX_THROW(MLX_E_LAYOUT, "energy gradient: velocity components must be in PPP");
+    k_energy_grad(c, vr->e, vp->e, vz->e);
+  });
+}
+
 int mlx_adjoint_project(mlx_ctx *c, mlx_field *f) {
   return guarded(c, [&] {
     need_kit(c);
@@ -1387,9 +1703,10 @@ int mlx_vortex_bootstrap(mlx_ctx *c, mlx_vortex *v) {
   });
 }
 
-// vortical_flow_3d.f90:160-181
-int mlx_vortex_step(mlx_ctx *c, mlx_vortex *v, int32_t nsteps, double dt, int32_t check) {
-  return guarded(c, [&] {
+// vortical_flow_3d.f90:160-181.  rec (optional): 2 nsteps fields that
+// receive psi and chi after each step
+void vortex_steps(mlx_ctx *c, mlx_vortex *v, int32_t nsteps, double dt, int32_t check, mlx_field **rec) {
+  {
     need_kit(c);
     mlx_field *psi = v->f[0], *chi = v->f[1], *nlpsi = v->f[2], *nlchi = v->f[3];
     mlx_field *psi_prev = v->f[4], *chi_prev = v->f[5], *nlpsi_prev = v->f[6], *nlchi_prev = v->f[7];
@@ -1416,6 +1733,10 @@ int mlx_vortex_step(mlx_ctx *c, mlx_vortex *v, int32_t nsteps, double dt, int32_
       zeroat1_pair(c, psi, chi);
       advection_rhs(c, psi, chi, nlpsi, nlchi, uz, true);
       if (!c->p.is_svv) check_finite(c, psi, chi, v->flag);
+      if (rec) {
+        fcopy(c, rec[2 * it], psi);
+        fcopy(c, rec[2 * it + 1], chi);
+      }
     }
     if (check) {
       int h = 0;
@@ -1424,6 +1745,19 @@ int mlx_vortex_step(mlx_ctx *c, mlx_vortex *v, int32_t nsteps, double dt, int32_
       if (h) MLX_THROW(MLX_E_NONFINITE, "ERROR: non-finite vortex state");
       if (c->p.hyperpow != 0 && nsteps > 0) check_sing(c);
     }
+  }
+}
+
+int mlx_vortex_step(mlx_ctx *c, mlx_vortex *v, int32_t nsteps, double dt, int32_t check) {
+  return guarded(c, [&] { vortex_steps(c, v, nsteps, dt, check, nullptr); });
+}
+
+int mlx_vortex_step_record(mlx_ctx *c, mlx_vortex *v, int32_t nsteps, double dt, mlx_field **rec) {
+  return guarded(c, [&] {
+    if (!rec && nsteps > 0) MLX_THROW(MLX_E_ARG, "mlx_vortex_step_record: no fields to record into");
+    for (int i = 0; i < 2 * nsteps; ++i)
+      if (!rec[i]) MLX_THROW(MLX_E_ARG, "mlx_vortex_step_record: null field");
+    vortex_steps(c, v, nsteps, dt, 0, rec);
   });
 }
 

@@ -87,7 +87,8 @@ struct mlx_field {
 // cached LU factors of (del2 + alpha) per (m, |kappa|) system (layout in solve.hip)
 struct LuCache {
   double alpha;
-  int kind;   // 0: del2 + alpha; 1: idel2 bordered (alpha = 0); 2: idel2, (0,0) excluded
+  int kind;   // 0: del2 + alpha; 1: idel2 bordered (alpha = 0); 2: idel2, (0,0) excluded;
+              // plus 4: the transposed systems
   int nroff;  // radial chop offset the systems were built for
   int seg = 0;               // rows per lane segment of the solve
   double *sol = nullptr;     // factor rows (segment layout)
@@ -129,11 +130,6 @@ struct WorkTable {
 constexpr int kPartMax = 2 * 2 * 4096;  // two fields, (total, tail), up to 4096 workgroups each
 // LAPACK-band LU factors of the transposed (del2 + alpha) systems, one set
 // per alpha, for the adjoint of the Helmholtz solve
-struct AdjLu {
-  double alpha = 0.0;
-  double *fac = nullptr;
-  int8_t *piv = nullptr;
-};
 
 struct mlx_ctx {
   std::map<long, WorkTable> work;
@@ -156,7 +152,6 @@ struct mlx_ctx {
   std::vector<mlx_field *> pool;  // temporaries (allocated on demand)
   std::vector<LuCache> lu;
   uint64_t lu_clock = 0;
-  std::vector<AdjLu> adj_lu;
   ProbeSet probe;
   // second stream (default priority) for work that overlaps the context
   // stream inside one operation, with its fork and join events; created on
@@ -393,16 +388,28 @@ void k_nonfinite(mlx_ctx *c, const double2 *a, const double2 *b, int *flag);
 void k_energy(mlx_ctx *c, const double2 *vr, const double2 *vp, const double2 *vz, double *out);
 void k_vormag(mlx_ctx *c, double2 *out, const double2 *wr, const double2 *wp, const double2 *wz);
 void k_set_scalar(mlx_ctx *c, double *dst, double v);
-void adj_cache_free(mlx_ctx *c);
 // adjoint kernels (adjoint.hip)
 void k_adj_phi_scale(mlx_ctx *c, double2 *e, int mode);
 void k_adj_wscale(mlx_ctx *c, double2 *e, int mode);
 void k_ln_adjoint(mlx_ctx *c, const double2 *e, double *ln_bar, double sign);
 void k_adj_scale(mlx_ctx *c, double2 *e, double s);
 void k_adj_project(mlx_ctx *c, double2 *e);
+void k_adj_diag(mlx_ctx *c, double2 *e, int wmode, int rmode, int phimode, double s);
 void k_del2_t(mlx_ctx *c, double2 *out, const double2 *in, bool with_ak);
 void k_lap_ln_t(mlx_ctx *c, const double2 *e, double *ln_bar, double scale);
-void k_helm_solve_t(mlx_ctx *c, double2 *e, double alpha);
+void k_helm_solve_t(mlx_ctx *c, double2 *e, double alpha, int kind = 0, int nroff = 0, int pcut = -1, int zlo = 1,
+                    int zhi = 0);
+void k_abcn_adj_out(mlx_ctx *c, double2 *y, const double2 *t, double2 *nlbar, double2 *nlpbar, double c1, double c2,
+                    double a, double c3);
+void k_abcn_adj_ln(mlx_ctx *c, const double *yln, double *nlln, double *nlpln, double f1, double f2);
+// ncols > 0, zlo..zhi: columns m >= ncols and slices zlo..zhi taken as zero
+void k_tp2vec_t(mlx_ctx *c, double2 *psib, double2 *chib, const double2 *urb, const double2 *upb, const double2 *uzb,
+                double *psib_ln, double *chib_ln, int ncols = -1, int zlo = 1, int zhi = 0);
+void k_v2t_t(mlx_ctx *c, double2 *urb, double2 *upb, double2 *uzb, const double2 *psib, const double2 *chib,
+             int ncols = -1, int zlo = 1, int zhi = 0);
+void k_zeroat1_t(mlx_ctx *c, double2 *e);
+void k_v2t_up_ln_t(mlx_ctx *c, double2 *upF, const double2 *upP);
+void k_energy_grad(mlx_ctx *c, double2 *vr, double2 *vp, double2 *vz);
 void k_vecprod_adj(mlx_ctx *c, double2 *a1, double2 *a2, double2 *a3, const double2 *v1, const double2 *v2,
                    const double2 *v3, const double2 *u1, const double2 *u2, const double2 *u3, double2 *ub1,
                    double2 *ub2, double2 *ub3);

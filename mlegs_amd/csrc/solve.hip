@@ -92,6 +92,8 @@ __global__ void k_lu_factor(double *fac, int8_t *piv, const double *d2c, const d
   if (kk >= nkk || m >= npc) return;
   int n = nrcs_dev(nrc, npc, m);
   if (n <= 0) return;
+  const bool tr = (kind & 4) != 0;  // the transposed matrix (the adjoint's solves)
+  kind &= 3;
   if (kind == 2 && m == 0 && kk == 0) return;  // idel2_preln: (0,0) solved densely
   size_t sys = (size_t)m * nkk + kk;
   double *ab = fac + sys * (size_t)nrdim * LDAB;
@@ -101,7 +103,8 @@ __global__ void k_lu_factor(double *fac, int8_t *piv, const double *d2c, const d
   for (int j = 0; j < n; ++j) {
     for (int r = 0; r < LDAB; ++r) ab[r + LDAB * j] = 0.0;
     for (int i = (j - KU > 0 ? j - KU : 0); i <= j + KL && i < n; ++i)
-      ab[(KV + i - j) + LDAB * j] = d2_entry(d2c, lognorm, m, ne, i, j, akk, alpha, bordered, ell);
+      ab[(KV + i - j) + LDAB * j] = tr ? d2_entry(d2c, lognorm, m, ne, j, i, akk, alpha, bordered, ell)
+                                       : d2_entry(d2c, lognorm, m, ne, i, j, akk, alpha, bordered, ell);
   }
   int ju = 0;
   for (int j = 0; j < n; ++j) {
@@ -151,6 +154,7 @@ __global__ void k_lu_repack(const double *fac, const int8_t *piv, FacPtrs F, uin
   int kk = blockIdx.x * blockDim.x + threadIdx.x;
   int m = blockIdx.y;
   if (kk >= nkk || m >= npc) return;
+  kind &= 3;
   if (kind == 2 && m == 0 && kk == 0) return;
   int n = nrcs_dev(nrc, npc, m);
   const size_t sysk = (size_t)kk * npc + m;  // segment-layout system index
@@ -537,7 +541,7 @@ LuCache &lu_get(mlx_ctx *c, double alpha, int kind, int nroff) {
   if (nrc > k.nrdim_ref) MLX_THROW(MLX_E_ARG, "chop_index: chopping in r too large");
   const int S = seg_rows(nrc);
   if (S == 0) MLX_THROW(MLX_E_ARG, "ihelm: more than 544 radial rows per system are not supported");
-  if (c->lu.size() >= 4) {  // evict the least recently used entry
+  if (c->lu.size() >= 6) {  // evict the least recently used entry
     size_t victim = 0;
     for (size_t i = 1; i < c->lu.size(); ++i)
       if (c->lu[i].stamp < c->lu[victim].stamp) victim = i;
@@ -646,7 +650,7 @@ void k_helm_solve(mlx_ctx *c, double2 *e, double alpha, int kind, int nroff, int
   }
   ProbeScope probe(c, MLX_PROBE_LU_SOLVE, 32.0 * rows * nf + fbytes, 2.0 * 4.0 * 5.0 * rows * nf);
   const FacPtrs F = fac_ptrs(lu.sol, fac_rows(k, lu.seg));
-  const int nrc = k.nrchop + nroff, sk = kind == 2 ? 1 : 0;
+  const int nrc = k.nrchop + nroff, sk = (kind & 3) == 2 ? 1 : 0;
 #define MLX_LU_LAUNCH(SV, PV)                                                                              \
   probed_launch(probe, k_lu_solve<SV, PV>, grid, dim3(64), 0, c->stream, e, e2 ? e2 : e, F, lu.pvw, lu.anyp,       \
                 k.nrdim, k.npdim, k.nzdim, nrc, k.npchop, k.chopzl, k.chopzu, sk, zlo, zhi)
