@@ -365,10 +365,13 @@ const char* mlx_file_last_error(void);
  *     (s' = zeroat1(dealias(abcn(s, N, N_prev))), N' = N(s'), N_prev' = N;
  *     :160-181): from the cotangents (sb, nb, npb) of the state after the
  *     step, whose psi and chi are p1, c1, to those of the state before it,
- *     in place (p1, c1 de-aliased in place);
+ *     in place (p1, c1 de-aliased in place); vw (optional, vw_fmt != 0): the
+ *     six fields v, w that mlx_vortex_step_record kept for that state;
  *   mlx_vortex_step_record: mlx_vortex_step without the stability check,
- *     copying psi and chi after step i into rec[2 i] and rec[2 i + 1] (the
- *     recomputation of a checkpointed segment);
+ *     copying psi and chi after step i into rec[2 i] and rec[2 i + 1] and,
+ *     with rec_vw, the step's v (without uz) and w into rec_vw[6 i .. 6 i +
+ *     5], *vw_fmt = 2 (stored in FP32), 1 (FP64) or 0 (not kept: the
+ *     kit's step does not keep them);
  *   mlx_energy_grad: in place, a PPP velocity (vr, vp, vz) to the gradient of
  *     its kinetic energy (mlx_energy's quadrature).
  * These linearise and transpose in plain FP64 whatever the context's
@@ -391,8 +394,9 @@ int mlx_advection_adjoint(mlx_ctx* ctx, mlx_field* psi, mlx_field* chi, const ml
                           mlx_field* nlchib, mlx_field* psib, mlx_field* chib);
 int mlx_vortex_reverse_step(mlx_ctx* ctx, mlx_field* p1, mlx_field* c1, const mlx_field* uz, mlx_field* sbpsi,
                             mlx_field* sbchi, mlx_field* nbpsi, mlx_field* nbchi, mlx_field* npbpsi,
-                            mlx_field* npbchi, double dt);
-int mlx_vortex_step_record(mlx_ctx* ctx, mlx_vortex* v, int32_t nsteps, double dt, mlx_field** rec);
+                            mlx_field* npbchi, double dt, mlx_field** vw, int32_t vw_fmt);
+int mlx_vortex_step_record(mlx_ctx* ctx, mlx_vortex* v, int32_t nsteps, double dt, mlx_field** rec,
+                           mlx_field** rec_vw, int32_t* vw_fmt);
 int mlx_energy_grad(mlx_ctx* ctx, mlx_field* vr, mlx_field* vp, mlx_field* vz);
 
 /* ---- ensembles (no reference counterpart) -------------------------------- */

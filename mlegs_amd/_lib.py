@@ -106,8 +106,8 @@ SIGNATURES = {
     'mlx_vec2tp_adjoint': ([_P, _P, _P, _P, _P, _P], ctypes.c_int),
     'mlx_advection_adjoint': ([_P, _P, _P, _P, _P, _P, _P, _P], ctypes.c_int),
     'mlx_energy_grad': ([_P, _P, _P, _P], ctypes.c_int),
-    'mlx_vortex_reverse_step': ([_P] * 10 + [_D], ctypes.c_int),
-    'mlx_vortex_step_record': ([_P, _P, ctypes.c_int32, _D, _P], ctypes.c_int),
+    'mlx_vortex_reverse_step': ([_P] * 10 + [_D, _P, _I], ctypes.c_int),
+    'mlx_vortex_step_record': ([_P, _P, _I, _D, _P, _P, _P], ctypes.c_int),
     'mlx_ensemble_create': ([ctypes.c_void_p, _I, _I, _I, ctypes.POINTER(_P)], ctypes.c_int),
     'mlx_ensemble_destroy': ([_P], ctypes.c_int),
     'mlx_ensemble_last_error': ([_P], ctypes.c_char_p),

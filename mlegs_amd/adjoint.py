@@ -306,6 +306,7 @@ class VortexChain:
         t = self.tfm
         if len(getattr(self, '_states', ())) < nseg:
             self._states = [(_fff(t), _fff(t)) for _ in range(nseg)]
+            self._vw = [tuple(Scalar(t) for _ in range(6)) for _ in range(nseg)]
         if not hasattr(self, '_g'):
             self._g = [_fff(t), _fff(t)]
 
@@ -316,19 +317,27 @@ class VortexChain:
         advection_adjoint(s[0], s[1], self.uz, nb[0], nb[1], g[0], g[1], self.tfm)
         return g
 
-    def _reverse_step(self, s1, sb, nb, npb):
+    def _reverse_step(self, j, sb, nb, npb):
         """From the cotangents (sb, nb, npb) of the state after a step, whose
-        (psi, chi) is s1, to those of the state before it, in place
-        (mlx_vortex_reverse_step)."""
+        psi, chi (and v, w) are the j-th recorded ones, to those of the state
+        before it, in place (mlx_vortex_reverse_step)."""
         t = self.tfm
+        s1 = self._states[j]
+        vw = (ctypes.c_void_p * 6)(*[f.h for f in self._vw[j]]) if self._vw_fmt else None
         t._check(t.lib.mlx_vortex_reverse_step(t.ctx, s1[0].h, s1[1].h, self.uz.h, sb[0].h, sb[1].h, nb[0].h,
-                                                nb[1].h, npb[0].h, npb[1].h, ctypes.c_double(self.dt)))
+                                                nb[1].h, npb[0].h, npb[1].h, ctypes.c_double(self.dt), vw,
+                                                int(self._vw_fmt)))
 
     def _record(self, run, n):
-        """n steps of the run, psi and chi after each into the state buffers."""
+        """n steps of the run; psi, chi and the step's v, w after each into
+        the state buffers."""
         t = self.tfm
         arr = (ctypes.c_void_p * (2 * n))(*[f.h for st in self._states[:n] for f in st])
-        t._check(t.lib.mlx_vortex_step_record(t.ctx, run.h, int(n), ctypes.c_double(self.dt), arr))
+        avw = (ctypes.c_void_p * (6 * n))(*[f.h for st in self._vw[:n] for f in st])
+        fmt = ctypes.c_int32()
+        t._check(t.lib.mlx_vortex_step_record(t.ctx, run.h, int(n), ctypes.c_double(self.dt), arr, avw,
+                                               ctypes.byref(fmt)))
+        self._vw_fmt = fmt.value
 
     def _reverse_bootstrap(self, B, sb, nb, npb):
         t, dt = self.tfm, self.dt
@@ -394,7 +403,7 @@ class VortexChain:
             self._load(run, ckpt[k0])
             self._record(run, k1 - k0)
             for k in range(k1 - 1, k0 - 1, -1):
-                self._reverse_step(self._states[k - k0], sb, nb, npb)
+                self._reverse_step(k - k0, sb, nb, npb)
         g = self._reverse_bootstrap(B, sb, nb, npb)
         t.sync()
         t3 = time.perf_counter()

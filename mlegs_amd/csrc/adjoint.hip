@@ -142,6 +142,32 @@ __global__ void k_adj_project_(double2 *e, size_t n, int nrdim, int npdim, int n
   }
 }
 
+// sb <- sb + g (data and log coefficient), kept on the chopped support
+// (as k_adj_project_) and outside the de-aliased modes (columns m >= pcut,
+// slices zcut .. zupper - 2), zero elsewhere: the sum, the projection and
+// dealias in one pass
+__global__ void k_adj_sum_mask_(double2 *sb, double *sbl, const double2 *g, const double *gl, size_t n, int nrdim,
+                                int npdim, int nz, int nrc, int npc, int nzc, int nzcu, int nph, int nzh, int pcut,
+                                int zcut, int zupper) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) sbl[0] += gl[0];
+  for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (size_t)gridDim.x * blockDim.x) {
+    const int i = (int)(t % nrdim);
+    const int m = (int)((t / nrdim) % npdim);
+    const int k = (int)(t / ((size_t)nrdim * npdim));
+    const int nn = m < npc ? nrc - m : 0;
+    bool keep = i < nn;
+    if (nz > 1) keep = keep && ((k + 1 <= nzc) || (k + 1 >= nzcu)) && k != nzh;
+    if (nph > 0 && m == nph) keep = false;
+    if ((pcut >= 0 && m >= pcut) || (zcut >= 0 && k >= zcut && k <= zupper - 2)) keep = false;
+    if (keep) {
+      const double2 a = sb[t], b = g[t];
+      sb[t] = make_double2(a.x + b.x, a.y + b.y);
+    } else {
+      sb[t] = make_double2(0.0, 0.0);
+    }
+  }
+}
+
 // out(j) = sum_i A(i, j) in(i), A = del2h band (plus -ak(k)^2 on the
 // diagonal when akk): the transpose of the Laplacian's band product on rows
 // j < nrcs(m) of every z slice inside the chops
@@ -181,11 +207,23 @@ __global__ void k_lap_ln_t_(const double2 *e, double *ln_bar, const double *logn
 
 // the vector product's adjoint at (v, u): with a the cotangent of v x u,
 // v_bar = u x a and u_bar = a x v, on each packed real sample
-// (the product's samples: rows i < nr, packed columns < np/2; zero elsewhere)
+// (the product's samples: rows i < nr, packed columns < np/2; zero elsewhere).
+// F32: v and u stored in FP32 (entry t at float pair t, as the forward's
+// fused product reads them); ub (optional): added to v3 (the background
+// axial flow)
+template <bool F32>
+__device__ inline double2 ld_vw(const double2 *p, size_t t) {
+  if (F32) {
+    const float2 f = reinterpret_cast<const float2 *>(p)[t];
+    return make_double2((double)f.x, (double)f.y);
+  }
+  return p[t];
+}
+template <bool F32>
 __global__ void k_vecprod_adj_(double2 *a1, double2 *a2, double2 *a3, const double2 *v1, const double2 *v2,
                                const double2 *v3, const double2 *u1, const double2 *u2, const double2 *u3,
-                               double2 *ub1, double2 *ub2, double2 *ub3, size_t n, int nr, int nrdim, int npdim,
-                               int nph) {
+                               double2 *ub1, double2 *ub2, double2 *ub3, const double2 *bg, size_t n, int nr,
+                               int nrdim, int npdim, int nph) {
   for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (size_t)gridDim.x * blockDim.x) {
     const int i = (int)(t % nrdim), j = (int)((t / nrdim) % npdim);
     if (i >= nr || j >= nph) {
@@ -195,8 +233,13 @@ __global__ void k_vecprod_adj_(double2 *a1, double2 *a2, double2 *a3, const doub
       continue;
     }
     const double2 A1 = a1[t], A2 = a2[t], A3 = a3[t];
-    const double2 V1 = v1[t], V2 = v2[t], V3 = v3[t];
-    const double2 U1 = u1[t], U2 = u2[t], U3 = u3[t];
+    const double2 V1 = ld_vw<F32>(v1, t), V2 = ld_vw<F32>(v2, t);
+    double2 V3 = ld_vw<F32>(v3, t);
+    if (bg) {
+      const double2 g = bg[t];
+      V3 = make_double2(V3.x + g.x, V3.y + g.y);
+    }
+    const double2 U1 = ld_vw<F32>(u1, t), U2 = ld_vw<F32>(u2, t), U3 = ld_vw<F32>(u3, t);
     // v_bar = u x a
     a1[t] = make_double2(U2.x * A3.x - U3.x * A2.x, U2.y * A3.y - U3.y * A2.y);
     a2[t] = make_double2(U3.x * A1.x - U1.x * A3.x, U3.y * A1.y - U1.y * A3.y);
@@ -285,91 +328,100 @@ __global__ void k_tp2vec_t_ln_(const double2 *upb, const double2 *uzb, double *p
 //   up(x~) += s (V Db - D Bb), uz(x) += T Eb, uz(x~) += s T Eb
 // One workgroup per (m, kappa) column, one thread per x; rows staged in LDS
 // in chunks.  Every entry of the outputs is written (zero outside).
-// Workgroups of 256 threads: KB = 256 / NX kappa slices (NX = nr/2 rounded
-// up to 64, 128 or 256), one thread per (x, slice); each chunk of CH table
-// rows is staged in LDS once and read by all KB slices.
-constexpr int V2T_CH = 8;
+// One workgroup of 256 threads per (m, kappa) column: RG = 256 / NX row
+// groups of NX threads (NX = nr/2 rounded up to 64, 128 or 256), one thread
+// per x; group g takes the table rows i = g, g + RG, ... of each staged
+// chunk, and the groups' partial sums meet in LDS at the end.
+constexpr int V2T_CH = 64;
 template <int NX>
 __global__ void __launch_bounds__(256) k_v2t_t_(double2 *urb, double2 *upb, double2 *uzb, const double2 *psib,
                                                 const double2 *chib, const double *tV, const double *tD,
                                                 const double *tT, const double *ak, int nrdim, int npdim, int nz,
                                                 int nr, int nrh, int nt, int nrc3, int npc, int nzc, int nzcu,
                                                 int zlo, int zhi) {
-  constexpr int KB = 256 / NX;
-  __shared__ double sT[3][V2T_CH][NX];
-  __shared__ double2 sQ[5][KB][V2T_CH];
-  const int m = blockIdx.x, tid = threadIdx.x, x = tid % NX, j = tid / NX;
-  const int k = blockIdx.y * KB + j;
+  constexpr int RG = 256 / NX;
+  __shared__ double2 sQ[5][V2T_CH];
+  __shared__ double2 sR[RG > 1 ? RG : 1][6][RG > 1 ? NX : 1];
+  const int m = blockIdx.x, k = blockIdx.y, tid = threadIdx.x, x = tid % NX, grp = tid / NX;
+  const size_t base = (size_t)k * nrdim * npdim + (size_t)nrdim * m;
   int M = (m < npc && nrc3 - m > 0) ? nrc3 - m : 0;
   if (M > nt) M = nt;
-  const double mv = (double)m;
+  const bool slice = (nz == 1 || (k + 1 <= nzc) || (k + 1 >= nzcu)) && !(k >= zlo && k <= zhi);
+  if (!slice) M = 0;
+  const double kv = ak[k], mv = (double)m;
   const bool mz = (m == 0);
   const size_t tb = (size_t)m * nt * nrh;
   const double2 z2 = make_double2(0.0, 0.0);
-  double2 r0 = z2, r1 = z2, p0 = z2, p1 = z2, q0 = z2, q1 = z2;  // (x) and (x~) sums
+  double2 acc[6] = {z2, z2, z2, z2, z2, z2};  // ur(x), ur(x~), up(x), up(x~), uz(x), uz(x~)
   for (int i0 = 0; i0 < M; i0 += V2T_CH) {
     __syncthreads();
-    for (int idx = tid; idx < 3 * V2T_CH * NX; idx += 256) {
-      const int q = idx / (V2T_CH * NX), t = (idx / NX) % V2T_CH, xx = idx % NX, i = i0 + t;
-      const double *src = q == 0 ? tV : (q == 1 ? tD : tT);
-      sT[q][t][xx] = (i < M && xx < nrh) ? src[tb + (size_t)i * nrh + xx] : 0.0;
-    }
-    for (int idx = tid; idx < KB * V2T_CH; idx += 256) {
-      const int jj = idx / V2T_CH, t = idx % V2T_CH, i = i0 + t, kk = blockIdx.y * KB + jj;
+    if (tid < V2T_CH) {
+      const int i = i0 + tid;
       double2 ps = z2, cs = z2;
-      double kv = 0.0;
-      if (i < M && kk < nz && (nz == 1 || (kk + 1 <= nzc) || (kk + 1 >= nzcu)) && !(kk >= zlo && kk <= zhi)) {
-        const size_t base = (size_t)kk * nrdim * npdim + (size_t)nrdim * m;
+      if (i < M) {
         ps = psib[base + i];
         cs = chib[base + i];
-        kv = ak[kk];
       }
-      sQ[0][jj][t] = mz ? z2 : make_double2(-mv * ps.y, mv * ps.x);  // i m psib
-      sQ[1][jj][t] = make_double2(-ps.x, -ps.y);
-      sQ[2][jj][t] = make_double2(kv * cs.y, -kv * cs.x);            // -i kappa chib
-      sQ[3][jj][t] = mz ? z2 : make_double2(mv * kv * cs.x, mv * kv * cs.y);
-      sQ[4][jj][t] = make_double2(-cs.x, -cs.y);
+      sQ[0][tid] = mz ? z2 : make_double2(-mv * ps.y, mv * ps.x);  // i m psib
+      sQ[1][tid] = make_double2(-ps.x, -ps.y);
+      sQ[2][tid] = make_double2(kv * cs.y, -kv * cs.x);            // -i kappa chib
+      sQ[3][tid] = mz ? z2 : make_double2(mv * kv * cs.x, mv * kv * cs.y);
+      sQ[4][tid] = make_double2(-cs.x, -cs.y);
     }
     __syncthreads();
-    const int iend = (M - i0 < V2T_CH) ? M - i0 : V2T_CH;
-#pragma unroll
-    for (int t = 0; t < V2T_CH; ++t) {
-      if (t >= iend) break;
-      const int i = i0 + t;
-      const double v = sT[0][t][x], d = sT[1][t][x], tt = sT[2][t][x];
-      const double sg = (i & 1) ? -1.0 : 1.0;
-      const double2 A = sQ[0][j][t], B = sQ[1][j][t], C = sQ[2][j][t], D = sQ[3][j][t], E = sQ[4][j][t];
-      const double vax = v * A.x, vay = v * A.y, dcx = d * C.x, dcy = d * C.y;
-      const double dbx = d * B.x, dby = d * B.y, vdx = v * D.x, vdy = v * D.y;
-      const double tex = tt * E.x, tey = tt * E.y;
-      r0.x += vax + dcx;
-      r0.y += vay + dcy;
-      r1.x += sg * (vax - dcx);
-      r1.y += sg * (vay - dcy);
-      p0.x += dbx + vdx;
-      p0.y += dby + vdy;
-      p1.x += sg * (vdx - dbx);
-      p1.y += sg * (vdy - dby);
-      q0.x += tex;
-      q0.y += tey;
-      q1.x += sg * tex;
-      q1.y += sg * tey;
+    if (x < nrh) {
+      const int iend = (M - i0 < V2T_CH) ? M - i0 : V2T_CH;
+#pragma unroll 4
+      for (int t = grp; t < iend; t += RG) {
+        const int i = i0 + t;
+        const size_t o = tb + (size_t)i * nrh + x;
+        const double v = tV[o], d = tD[o], tt = tT[o];
+        const double sg = (i & 1) ? -1.0 : 1.0;
+        const double2 A = sQ[0][t], B = sQ[1][t], C = sQ[2][t], D = sQ[3][t], E = sQ[4][t];
+        const double vax = v * A.x, vay = v * A.y, dcx = d * C.x, dcy = d * C.y;
+        const double dbx = d * B.x, dby = d * B.y, vdx = v * D.x, vdy = v * D.y;
+        const double tex = tt * E.x, tey = tt * E.y;
+        acc[0].x += vax + dcx;
+        acc[0].y += vay + dcy;
+        acc[1].x += sg * (vax - dcx);
+        acc[1].y += sg * (vay - dcy);
+        acc[2].x += dbx + vdx;
+        acc[2].y += dby + vdy;
+        acc[3].x += sg * (vdx - dbx);
+        acc[3].y += sg * (vdy - dby);
+        acc[4].x += tex;
+        acc[4].y += tey;
+        acc[5].x += sg * tex;
+        acc[5].y += sg * tey;
+      }
     }
   }
-  if (k >= nz) return;
-  const size_t base = (size_t)k * nrdim * npdim + (size_t)nrdim * m;
-  for (int i = nr + x; i < nrdim; i += NX) {
+  if (RG > 1) {
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < 6; ++q) sR[grp][q][x] = acc[q];
+    __syncthreads();
+    if (grp == 0) {
+      for (int g = 1; g < RG; ++g)
+#pragma unroll
+        for (int q = 0; q < 6; ++q) {
+          acc[q].x += sR[g][q][x].x;
+          acc[q].y += sR[g][q][x].y;
+        }
+    }
+  }
+  for (int i = nr + tid; i < nrdim; i += 256) {
     urb[base + i] = z2;
     upb[base + i] = z2;
     uzb[base + i] = z2;
   }
-  if (x < nrh) {
-    urb[base + x] = r0;
-    urb[base + nr - 1 - x] = r1;
-    upb[base + x] = p0;
-    upb[base + nr - 1 - x] = p1;
-    uzb[base + x] = q0;
-    uzb[base + nr - 1 - x] = q1;
+  if (grp == 0 && x < nrh) {
+    urb[base + x] = acc[0];
+    urb[base + nr - 1 - x] = acc[1];
+    upb[base + x] = acc[2];
+    upb[base + nr - 1 - x] = acc[3];
+    uzb[base + x] = acc[4];
+    uzb[base + nr - 1 - x] = acc[5];
   }
 }
 
@@ -469,6 +521,19 @@ void k_adj_project(mlx_ctx *c, double2 *e) {
                      k.nz > 1 ? k.nz / 2 : -1);
 }
 
+void k_adj_sum_mask(mlx_ctx *c, double2 *sb, double *sbl, const double2 *g, const double *gl) {
+  const KitDev &k = c->kit;
+  int pcut = -1, zcut = -1, zupper = 0;
+  if (k.np > 1) pcut = (k.np / 3 + 1 > 1) ? k.np / 3 + 1 : 1;
+  if (k.nz > 1) {
+    zcut = (k.nz / 3 + 1 > 1) ? k.nz / 3 + 1 : 1;
+    zupper = k.nz - zcut + 2;
+  }
+  hipLaunchKernelGGL(k_adj_sum_mask_, dim3(grid_for(c->nelem)), dim3(kT), 0, c->stream, sb, sbl, g, gl, c->nelem,
+                     k.nrdim, k.npdim, k.nzdim, k.nrchop, k.npchop, k.chopzl, k.chopzu, k.np > 1 ? k.nph : -1,
+                     k.nz > 1 ? k.nz / 2 : -1, pcut, zcut, zupper);
+}
+
 void k_del2_t(mlx_ctx *c, double2 *out, const double2 *in, bool with_ak) {
   const KitDev &k = c->kit;
   if (k.nrchop > k.ne) MLX_THROW(MLX_E_ARG, "del2 adjoint: table too small");
@@ -484,8 +549,9 @@ void k_lap_ln_t(mlx_ctx *c, const double2 *e, double *ln_bar, double scale) {
 // kind 1: idel2_proln's systems (alpha = 0, the (0,0) system bordered).  The
 // transposed matrices are factored and solved by the banded LU of solve.hip
 // (kind + 4), over every column and kept slice.
-void k_helm_solve_t(mlx_ctx *c, double2 *e, double alpha, int kind, int nroff, int pcut, int zlo, int zhi) {
-  k_helm_solve(c, e, alpha, kind | 4, nroff, pcut, zlo, zhi);
+void k_helm_solve_t(mlx_ctx *c, double2 *e, double alpha, int kind, int nroff, int pcut, int zlo, int zhi,
+                    double2 *e2) {
+  k_helm_solve(c, e, alpha, kind | 4, nroff, pcut, zlo, zhi, e2);
 }
 
 // abcn's transpose after the solve: nlbar = c1 y, nlpbar = c2 y, then
@@ -500,11 +566,19 @@ __global__ void k_abcn_adj_out_(double2 *y, const double2 *t, double2 *nlbar, do
   }
 }
 
-__global__ void k_abcn_adj_ln_(const double *yln, double *nlln, double *nlpln, double f1, double f2) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) {
-    nlln[0] = f1 * yln[0];
-    nlpln[0] = f2 * yln[0];
-  }
+// abcn's transpose, the log terms (one thread, before y's update): with
+// q(y) = (4/3, -2, 2/3) . Re y(0..2, 0, 0) / ell^2 / exp(lognorm(0..2)),
+// L = ysc - q(y), nl = 1.5 dt L, nlp = -0.5 dt L, ysc = L + c2 q(y)
+__global__ void k_abcn_adj_ln_(const double2 *y, double *ysc, double *nlln, double *nlpln, const double *lognorm,
+                               double ell, double dt, double c2) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  const double ell2 = ell * ell;
+  const double q = 4.0 / 3.0 / ell2 / exp(lognorm[0]) * y[0].x - 2.0 / 1.0 / ell2 / exp(lognorm[1]) * y[1].x +
+                   2.0 / 3.0 / ell2 / exp(lognorm[2]) * y[2].x;
+  const double L = ysc[0] - q;
+  nlln[0] = 1.5 * dt * L;
+  nlpln[0] = -0.5 * dt * L;
+  ysc[0] = L + c2 * q;
 }
 
 void k_abcn_adj_out(mlx_ctx *c, double2 *y, const double2 *t, double2 *nlbar, double2 *nlpbar, double c1, double c2,
@@ -513,8 +587,9 @@ void k_abcn_adj_out(mlx_ctx *c, double2 *y, const double2 *t, double2 *nlbar, do
                      c1, c2, a, c3);
 }
 
-void k_abcn_adj_ln(mlx_ctx *c, const double *yln, double *nlln, double *nlpln, double f1, double f2) {
-  hipLaunchKernelGGL(k_abcn_adj_ln_, dim3(1), dim3(64), 0, c->stream, yln, nlln, nlpln, f1, f2);
+void k_abcn_adj_ln(mlx_ctx *c, const double2 *y, double *ysc, double *nlln, double *nlpln, double dt, double c2) {
+  hipLaunchKernelGGL(k_abcn_adj_ln_, dim3(1), dim3(64), 0, c->stream, y, ysc, nlln, nlpln, c->kit.lognorm, c->kit.ell,
+                     dt, c2);
 }
 
 void k_tp2vec_t(mlx_ctx *c, double2 *psib, double2 *chib, const double2 *urb, const double2 *upb, const double2 *uzb,
@@ -535,7 +610,7 @@ void k_v2t_t(mlx_ctx *c, double2 *urb, double2 *upb, double2 *uzb, const double2
   if (k.nt < k.nrchop + 3) MLX_THROW(MLX_E_ARG, "vec2tp adjoint: projection tables too small");
   const int npc = (ncols > 0 && ncols < k.npchop) ? ncols : k.npchop;
 #define MLX_V2TT(NX_)                                                                                            \
-  hipLaunchKernelGGL(k_v2t_t_<NX_>, dim3(k.npdim, (k.nzdim + 256 / NX_ - 1) / (256 / NX_)), dim3(256), 0, c->stream, \
+  hipLaunchKernelGGL(k_v2t_t_<NX_>, dim3(k.npdim, k.nzdim), dim3(256), 0, c->stream,                             \
                      urb, upb, uzb, psib, chib, k.tV, k.tD, k.tT, k.ak, k.nrdim, k.npdim, k.nzdim, k.nr, k.nrh, k.nt,  \
                      k.nrchop + 3, npc, k.chopzl, k.chopzu, zlo, zhi)
   if (k.nrh <= 64) MLX_V2TT(64);
@@ -569,10 +644,14 @@ void k_energy_grad(mlx_ctx *c, double2 *vr, double2 *vp, double2 *vz) {
 
 void k_vecprod_adj(mlx_ctx *c, double2 *a1, double2 *a2, double2 *a3, const double2 *v1, const double2 *v2,
                    const double2 *v3, const double2 *u1, const double2 *u2, const double2 *u3, double2 *ub1,
-                   double2 *ub2, double2 *ub3) {
+                   double2 *ub2, double2 *ub3, bool f32, const double2 *bg) {
   const KitDev &k = c->kit;
-  hipLaunchKernelGGL(k_vecprod_adj_, dim3(grid_for(c->nelem)), dim3(kT), 0, c->stream, a1, a2, a3, v1, v2, v3, u1,
-                     u2, u3, ub1, ub2, ub3, c->nelem, k.nr, k.nrdim, k.npdim, k.nph);
+  if (f32)
+    hipLaunchKernelGGL(k_vecprod_adj_<true>, dim3(grid_for(c->nelem)), dim3(kT), 0, c->stream, a1, a2, a3, v1, v2, v3,
+                       u1, u2, u3, ub1, ub2, ub3, bg, c->nelem, k.nr, k.nrdim, k.npdim, k.nph);
+  else
+    hipLaunchKernelGGL(k_vecprod_adj_<false>, dim3(grid_for(c->nelem)), dim3(kT), 0, c->stream, a1, a2, a3, v1, v2,
+                       v3, u1, u2, u3, ub1, ub2, ub3, bg, c->nelem, k.nr, k.nrdim, k.npdim, k.nph);
 }
 
 

@@ -636,31 +636,41 @@ void trans_adjoint(mlx_ctx *c, mlx_field *g, const char *from) {
 // (rows n < nrcs(m), m < npchop, the kept z slices; Nyquist column and
 // slice excluded).  sbar: the cotangent of s' on entry, of s on return;
 // nlbar, nlpbar: the cotangents of nl and nl_p (overwritten).
-void abcn_adjoint(mlx_ctx *c, mlx_field *sbar, mlx_field *nlbar, mlx_field *nlpbar, double dt) {
+// projected: sbar is already zero outside that support.  nf = 2: a second
+// cotangent (sbar[1], nlbar[1], nlpbar[1]) through the same map, its solve
+// in the same launch (the same factors).
+void abcn_adjoint(mlx_ctx *c, mlx_field *const *sbar, mlx_field *const *nlbar, mlx_field *const *nlpbar, int nf,
+                  double dt, bool projected) {
   need_kit(c);
-  for (mlx_field *f : {sbar, nlbar, nlpbar}) adj_no_offsets(f, "abcn adjoint");
-  if (!is_fff(sbar)) MLX_THROW(MLX_E_LAYOUT, "abcn adjoint: the cotangent must be in FFF");
+  for (int f = 0; f < nf; ++f) {
+    for (mlx_field *x : {sbar[f], nlbar[f], nlpbar[f]}) adj_no_offsets(x, "abcn adjoint");
+    if (!is_fff(sbar[f])) MLX_THROW(MLX_E_LAYOUT, "abcn adjoint: the cotangent must be in FFF");
+  }
   if (c->p.hyperpow != 0) MLX_THROW(MLX_E_ARG, "abcn adjoint: the hyperviscous stepper has no adjoint here");
   if (c->p.visc < 5e-14) MLX_THROW(MLX_E_ARG, "abcn: inviscid case and no linear term in rhs. semi-implicit time adv is impossible");
-  const double a = -2.0 / (dt * c->p.visc);
-  mlx_field *y = sbar;
-  k_adj_project(c, y->e);
-  // y <- (del2 + a)^-T y; the cotangent of L gains -lap_ln^T y
-  k_helm_solve_t(c, y->e, a);
-  k_lap_ln_t(c, y->e, y->sc, -1.0);
-  // nl and nl_p: 1.5 a dt y and -0.5 a dt y (log terms 1.5 dt and -0.5 dt
-  // of y's); s: a (y + dt/2 visc del2^T y), and the log term's Laplacian
-  // inside del2
-  k_abcn_adj_ln(c, y->sc, nlbar->sc, nlpbar->sc, 1.5 * dt, -0.5 * dt);
-  mlx_field *t = tmp(c, 3);
-  k_del2_t(c, t->e, y->e, true);
-  k_lap_ln_t(c, y->e, y->sc, a * dt / 2.0 * c->p.visc);
-  k_abcn_adj_out(c, y->e, t->e, nlbar->e, nlpbar->e, 1.5 * a * dt, -0.5 * a * dt, a, a * dt / 2.0 * c->p.visc);
-  for (mlx_field *f : {nlbar, nlpbar}) f->off[0] = f->off[1] = f->off[2] = 0;
-  k_adj_project(c, y->e);
-  set_space(y, "FFF");
-  set_space(nlbar, "FFF");
-  set_space(nlpbar, "FFF");
+  const double a = -2.0 / (dt * c->p.visc), c3 = a * dt / 2.0 * c->p.visc;
+  if (!projected)
+    for (int f = 0; f < nf; ++f) k_adj_project(c, sbar[f]->e);
+  // y <- (del2 + a)^-T y
+  k_helm_solve_t(c, sbar[0]->e, a, 0, 0, -1, 1, 0, nf > 1 ? sbar[1]->e : nullptr);
+  for (int f = 0; f < nf; ++f) {
+    mlx_field *y = sbar[f], *nl = nlbar[f], *nlp = nlpbar[f];
+    // the log terms: L = y.ln - lap_ln^T y; nl.ln = 1.5 dt L, nl_p.ln =
+    // -0.5 dt L; s.ln = L + (a dt/2 visc) lap_ln^T y (del2's log rows)
+    k_abcn_adj_ln(c, y->e, y->sc, nl->sc, nlp->sc, dt, c3);
+    // nl = 1.5 a dt y, nl_p = -0.5 a dt y, s = a y + (a dt/2 visc) del2^T y
+    mlx_field *t = tmp(c, 3);
+    k_del2_t(c, t->e, y->e, true);
+    k_abcn_adj_out(c, y->e, t->e, nl->e, nlp->e, 1.5 * a * dt, -0.5 * a * dt, a, c3);
+    for (mlx_field *x : {nl, nlp}) x->off[0] = x->off[1] = x->off[2] = 0;
+    set_space(y, "FFF");
+    set_space(nl, "FFF");
+    set_space(nlp, "FFF");
+  }
+}
+
+void abcn_adjoint(mlx_ctx *c, mlx_field *sbar, mlx_field *nlbar, mlx_field *nlpbar, double dt) {
+  abcn_adjoint(c, &sbar, &nlbar, &nlpbar, 1, dt, false);
 }
 
 // febe (hyperpow = 0) maps (s, nl) to s' = (del2 + a)^-1 (a (s + dt nl) -
@@ -818,28 +828,42 @@ void vec2tp_adjoint(mlx_ctx *c, mlx_field *psib, mlx_field *chib, mlx_field *vrb
 // with psi and chi de-aliased first.  nlpsib, nlchib: the cotangents of N
 // (overwritten); psib, chib: the cotangents of psi and chi on return.  The
 // operators run in FP64.
-// psi and chi are de-aliased in place, as advection_rhs does.
+// psi and chi are de-aliased in place, as advection_rhs does.  vw
+// (optional): v and w at (psi, chi) as the forward step left them (v
+// without the background flow), in FP32 (vw_fmt 2) or FP64 (1); without
+// them they are formed here in FP64.
 void advection_adjoint(mlx_ctx *c, mlx_field *psi, mlx_field *chi, const mlx_field *uz, mlx_field *nlpsib,
-                       mlx_field *nlchib, mlx_field *psib, mlx_field *chib) {
+                       mlx_field *nlchib, mlx_field *psib, mlx_field *chib, mlx_field *const *vw = nullptr,
+                       int vw_fmt = 0) {
   need_kit(c);
   if (!is_fff(psi) || !is_fff(chi)) MLX_THROW(MLX_E_LAYOUT, "advection_rhs adjoint: psi and chi must be in FFF");
   if (!is_ppp(uz)) MLX_THROW(MLX_E_LAYOUT, "advection_rhs adjoint: uz must be in PPP");
   for (const mlx_field *f : {psi, chi}) adj_no_offsets(f, "advection_rhs adjoint");
   Fp64Scope fp64(c);
-  dealias(c, psi);
-  dealias(c, chi);
-  // v and w at (psi, chi): the fused column pass and the backward
-  // transforms of advection_rhs, skipping what de-aliasing zeroes, in FP64
   const DealiasLimits dl = dealias_limits(c);
-  const int ncols = dl.on ? dl.pcut : -1;
-  const bool zskip = fft_z_fast(c->kit);
-  mlx_field *vr = tmp(c, 24), *vp = tmp(c, 25), *vz = tmp(c, 26);
-  mlx_field *wr = tmp(c, 27), *wp = tmp(c, 28), *wz = tmp(c, 29);
-  k_tp2vec_front2(c, vr->e, vp->e, vz->e, vr->sc, vp->sc, vz->sc, wr->e, wp->e, wz->e, wr->sc, wp->sc, wz->sc,
-                  psi->e, chi->e, psi->sc, chi->sc, chi->off[0], ncols, zskip ? dl.zlo : 1, zskip ? dl.zhi : 0);
-  tp2vec_back(c, vr, vp, vz, ncols, dl.zlo, dl.zhi, true, false);
-  tp2vec_back(c, wr, wp, wz, ncols, dl.zlo, dl.zhi, true, false);
-  k_axpby(c, vz->e, nullptr, 1.0, uz->e, nullptr, 1.0);
+  const bool given = vw && (vw_fmt == 1 || vw_fmt == 2);
+  const double2 *v[3], *w[3];
+  if (given) {
+    for (int q = 0; q < 3; ++q) {
+      v[q] = vw[q]->e;
+      w[q] = vw[3 + q]->e;
+    }
+  } else {
+    dealias(c, psi);
+    dealias(c, chi);
+    // v and w at (psi, chi): the fused column pass and the backward
+    // transforms of advection_rhs, skipping what de-aliasing zeroes, in FP64
+    const int ncols = dl.on ? dl.pcut : -1;
+    const bool zskip = fft_z_fast(c->kit);
+    mlx_field *vr = tmp(c, 24), *vp = tmp(c, 25), *vz = tmp(c, 26);
+    mlx_field *wr = tmp(c, 27), *wp = tmp(c, 28), *wz = tmp(c, 29);
+    k_tp2vec_front2(c, vr->e, vp->e, vz->e, vr->sc, vp->sc, vz->sc, wr->e, wp->e, wz->e, wr->sc, wp->sc, wz->sc,
+                    psi->e, chi->e, psi->sc, chi->sc, chi->off[0], ncols, zskip ? dl.zlo : 1, zskip ? dl.zhi : 0);
+    tp2vec_back(c, vr, vp, vz, ncols, dl.zlo, dl.zhi, true, false);
+    tp2vec_back(c, wr, wp, wz, ncols, dl.zlo, dl.zhi, true, false);
+    v[0] = vr->e, v[1] = vp->e, v[2] = vz->e;
+    w[0] = wr->e, w[1] = wp->e, w[2] = wz->e;
+  }
   set_space(nlpsib, "FFF");
   set_space(nlchib, "FFF");
   dealias(c, nlpsib);
@@ -851,7 +875,8 @@ void advection_adjoint(mlx_ctx *c, mlx_field *psi, mlx_field *chi, const mlx_fie
   vec2tp_adjoint(c, nlpsib, nlchib, ar, ap, az, lim);
   // (v + U) x w: v_b = w x a, w_b = a x (v + U)
   mlx_field *br = tmp(c, 33), *bp = tmp(c, 34), *bz = tmp(c, 35);
-  k_vecprod_adj(c, ar->e, ap->e, az->e, vr->e, vp->e, vz->e, wr->e, wp->e, wz->e, br->e, bp->e, bz->e);
+  k_vecprod_adj(c, ar->e, ap->e, az->e, v[0], v[1], v[2], w[0], w[1], w[2], br->e, bp->e, bz->e,
+                given && vw_fmt == 2, uz->e);
   set_space(br, "PPP");
   set_space(bp, "PPP");
   set_space(bz, "PPP");
@@ -967,8 +992,10 @@ void vec2tp(mlx_ctx *c, const mlx_field *vr, const mlx_field *vp, const mlx_fiel
 // advection_rhs (vortical_flow_3d.f90:353-395).  dealiased: psi and chi are
 // already de-aliased (the stepper de-aliases them before svv_filter and
 // zeroat1, which keep a zero mode zero), so the opening dealias is a no-op.
-void advection_rhs(mlx_ctx *c, mlx_field *psi, mlx_field *chi, mlx_field *nlpsi, mlx_field *nlchi,
-                   const mlx_field *uz, bool dealiased = false) {
+// Returns how v and w (v without the background flow) are left in the
+// temporaries 4..9: 2 in FP32, 1 in FP64, 0 not at all (overwritten).
+int advection_rhs(mlx_ctx *c, mlx_field *psi, mlx_field *chi, mlx_field *nlpsi, mlx_field *nlchi,
+                  const mlx_field *uz, bool dealiased = false) {
   if (!is_fff(psi) || !is_fff(chi)) MLX_THROW(MLX_E_LAYOUT, "advection_rhs: psi and chi must be in FFF");
   if (!is_ppp(uz)) MLX_THROW(MLX_E_LAYOUT, "advection_rhs: uz must be in PPP");
   if (!dealiased) {
@@ -1008,13 +1035,14 @@ void advection_rhs(mlx_ctx *c, mlx_field *psi, mlx_field *chi, mlx_field *nlpsi,
       set_space(f, "PFP");
     }
     vec2tp_pfp(c, ur, up, uzp, nlpsi, nlchi, true, dl, true);
-  } else {
-    k_axpby(c, vz->e, vz->sc, 1.0, uz->e, nullptr, 1.0);
-    vecprod(c, vr, vp, vz, wr, wp, wz);
-    vec2tp(c, vr, vp, vz, nlpsi, nlchi, true);
-    k_set_scalar(c, nlpsi->sc, 0.0);
-    k_set_scalar(c, nlchi->sc, 0.0);
+    return f32phys ? 2 : 1;
   }
+  k_axpby(c, vz->e, vz->sc, 1.0, uz->e, nullptr, 1.0);
+  vecprod(c, vr, vp, vz, wr, wp, wz);
+  vec2tp(c, vr, vp, vz, nlpsi, nlchi, true);
+  k_set_scalar(c, nlpsi->sc, 0.0);
+  k_set_scalar(c, nlchi->sc, 0.0);
+  return 0;
 }
 
 // flag_dev (optional): the non-finite check of the field, on the same pass
@@ -1525,21 +1553,28 @@ int mlx_advection_adjoint(mlx_ctx *c, mlx_field *psi, mlx_field *chi, const mlx_
 // are p1, c1, to those of the state before it, in place
 int mlx_vortex_reverse_step(mlx_ctx *c, mlx_field *p1, mlx_field *c1, const mlx_field *uz, mlx_field *sbpsi,
                             mlx_field *sbchi, mlx_field *nbpsi, mlx_field *nbchi, mlx_field *npbpsi,
-                            mlx_field *npbchi, double dt) {
+                            mlx_field *npbchi, double dt, mlx_field **vw, int32_t vw_fmt) {
   return guarded(c, [&] {
+    if (vw && vw_fmt)
+      for (int q = 0; q < 6; ++q)
+        if (!vw[q]) MLX_THROW(MLX_E_ARG, "mlx_vortex_reverse_step: null v or w field");
     mlx_field *sb[2] = {sbpsi, sbchi}, *nb[2] = {nbpsi, nbchi}, *npb[2] = {npbpsi, npbchi};
     mlx_field *g[2] = {tmp(c, 38), tmp(c, 39)};
-    advection_adjoint(c, p1, c1, uz, nb[0], nb[1], g[0], g[1]);
+    advection_adjoint(c, p1, c1, uz, nb[0], nb[1], g[0], g[1], vw, vw ? vw_fmt : 0);
     for (int i = 0; i < 2; ++i) {
-      k_axpby(c, sb[i]->e, sb[i]->sc, 1.0, g[i]->e, g[i]->sc, 1.0);
+      // sb + g through zeroat1^T and dealias, and onto abcn's support (the
+      // three projections commute with zeroat1^T)
+      k_adj_sum_mask(c, sb[i]->e, sb[i]->sc, g[i]->e, g[i]->sc);
       k_zeroat1_t(c, sb[i]->e);
-      dealias(c, sb[i]);
-      // abcn^T: nb receives the cotangent of N, npb's old value is the
-      // cotangent of the history N_prev' = N (added), the new npb that of N_prev
-      mlx_field *b = tmp(c, 40 + i);
-      abcn_adjoint(c, sb[i], nb[i], b, dt);
+    }
+    // abcn^T of both fields: nb receives the cotangent of N, npb's old value
+    // is the cotangent of the history N_prev' = N (added), the new npb that
+    // of N_prev
+    mlx_field *b[2] = {tmp(c, 40), tmp(c, 41)};
+    abcn_adjoint(c, sb, nb, b, 2, dt, true);
+    for (int i = 0; i < 2; ++i) {
       k_axpby(c, nb[i]->e, nb[i]->sc, 1.0, npb[i]->e, npb[i]->sc, 1.0);
-      fmove(npb[i], b);
+      fmove(npb[i], b[i]);
     }
   });
 }
@@ -1704,8 +1739,11 @@ int mlx_vortex_bootstrap(mlx_ctx *c, mlx_vortex *v) {
 }
 
 // vortical_flow_3d.f90:160-181.  rec (optional): 2 nsteps fields that
-// receive psi and chi after each step
-void vortex_steps(mlx_ctx *c, mlx_vortex *v, int32_t nsteps, double dt, int32_t check, mlx_field **rec) {
+// receive psi and chi after each step; rec_vw (optional): 6 nsteps fields
+// that receive v (without the background flow) and w at them, where the
+// step leaves them (*vw_fmt: 2 FP32, 1 FP64, 0 not recorded)
+void vortex_steps(mlx_ctx *c, mlx_vortex *v, int32_t nsteps, double dt, int32_t check, mlx_field **rec,
+                  mlx_field **rec_vw = nullptr, int *vw_fmt = nullptr) {
   {
     need_kit(c);
     mlx_field *psi = v->f[0], *chi = v->f[1], *nlpsi = v->f[2], *nlchi = v->f[3];
@@ -1731,11 +1769,18 @@ void vortex_steps(mlx_ctx *c, mlx_vortex *v, int32_t nsteps, double dt, int32_t 
       // check after the step would raise
       svv(c, psi, v->gains, v->flag, &dl, chi, v->gains + 1);
       zeroat1_pair(c, psi, chi);
-      advection_rhs(c, psi, chi, nlpsi, nlchi, uz, true);
+      const int fmt = advection_rhs(c, psi, chi, nlpsi, nlchi, uz, true);
       if (!c->p.is_svv) check_finite(c, psi, chi, v->flag);
-      if (rec) {
+      // psi and chi are needed only where v and w are not kept
+      if (rec && !(rec_vw && fmt)) {
         fcopy(c, rec[2 * it], psi);
         fcopy(c, rec[2 * it + 1], chi);
+      }
+      if (rec_vw) {
+        if (vw_fmt) *vw_fmt = fmt;
+        // the temporaries' buffers go to the record (the record's come back
+        // as the next step's temporaries): no copy
+        for (int q = 0; fmt && q < 6; ++q) fmove(rec_vw[6 * it + q], tmp(c, 4 + q));
       }
     }
     if (check) {
@@ -1752,12 +1797,17 @@ int mlx_vortex_step(mlx_ctx *c, mlx_vortex *v, int32_t nsteps, double dt, int32_
   return guarded(c, [&] { vortex_steps(c, v, nsteps, dt, check, nullptr); });
 }
 
-int mlx_vortex_step_record(mlx_ctx *c, mlx_vortex *v, int32_t nsteps, double dt, mlx_field **rec) {
+int mlx_vortex_step_record(mlx_ctx *c, mlx_vortex *v, int32_t nsteps, double dt, mlx_field **rec,
+                           mlx_field **rec_vw, int32_t *vw_fmt) {
   return guarded(c, [&] {
     if (!rec && nsteps > 0) MLX_THROW(MLX_E_ARG, "mlx_vortex_step_record: no fields to record into");
     for (int i = 0; i < 2 * nsteps; ++i)
       if (!rec[i]) MLX_THROW(MLX_E_ARG, "mlx_vortex_step_record: null field");
-    vortex_steps(c, v, nsteps, dt, 0, rec);
+    for (int i = 0; rec_vw && i < 6 * nsteps; ++i)
+      if (!rec_vw[i]) MLX_THROW(MLX_E_ARG, "mlx_vortex_step_record: null v or w field");
+    int fmt = 0;
+    vortex_steps(c, v, nsteps, dt, 0, rec, rec_vw, &fmt);
+    if (vw_fmt) *vw_fmt = rec_vw ? fmt : 0;
   });
 }
 

@@ -395,13 +395,14 @@ void k_ln_adjoint(mlx_ctx *c, const double2 *e, double *ln_bar, double sign);
 void k_adj_scale(mlx_ctx *c, double2 *e, double s);
 void k_adj_project(mlx_ctx *c, double2 *e);
 void k_adj_diag(mlx_ctx *c, double2 *e, int wmode, int rmode, int phimode, double s);
+void k_adj_sum_mask(mlx_ctx *c, double2 *sb, double *sbl, const double2 *g, const double *gl);
 void k_del2_t(mlx_ctx *c, double2 *out, const double2 *in, bool with_ak);
 void k_lap_ln_t(mlx_ctx *c, const double2 *e, double *ln_bar, double scale);
 void k_helm_solve_t(mlx_ctx *c, double2 *e, double alpha, int kind = 0, int nroff = 0, int pcut = -1, int zlo = 1,
-                    int zhi = 0);
+                    int zhi = 0, double2 *e2 = nullptr);
 void k_abcn_adj_out(mlx_ctx *c, double2 *y, const double2 *t, double2 *nlbar, double2 *nlpbar, double c1, double c2,
                     double a, double c3);
-void k_abcn_adj_ln(mlx_ctx *c, const double *yln, double *nlln, double *nlpln, double f1, double f2);
+void k_abcn_adj_ln(mlx_ctx *c, const double2 *y, double *ysc, double *nlln, double *nlpln, double dt, double c2);
 // ncols > 0, zlo..zhi: columns m >= ncols and slices zlo..zhi taken as zero
 void k_tp2vec_t(mlx_ctx *c, double2 *psib, double2 *chib, const double2 *urb, const double2 *upb, const double2 *uzb,
                 double *psib_ln, double *chib_ln, int ncols = -1, int zlo = 1, int zhi = 0);
@@ -412,7 +413,7 @@ void k_v2t_up_ln_t(mlx_ctx *c, double2 *upF, const double2 *upP);
 void k_energy_grad(mlx_ctx *c, double2 *vr, double2 *vp, double2 *vz);
 void k_vecprod_adj(mlx_ctx *c, double2 *a1, double2 *a2, double2 *a3, const double2 *v1, const double2 *v2,
                    const double2 *v3, const double2 *u1, const double2 *u2, const double2 *u3, double2 *ub1,
-                   double2 *ub2, double2 *ub3);
+                   double2 *ub2, double2 *ub3, bool f32 = false, const double2 *bg = nullptr);
 // one int set by a kernel: clears flags in code a caller may capture into a
 // HIP graph (no memset nodes there)
 void k_set_flag(mlx_ctx *c, int *dst, int v);

@@ -257,3 +257,16 @@ def test_vortex_chain_gradient_c1():
     for fd, ad, _ in rows:
         _say('    fd %.12e adjoint %.12e' % (fd, ad))
     assert rel < 1e-5 and scaled < 1e-8
+
+
+def test_vortex_chain_gradient_pow2():
+    """The same at POW2 (power-of-two phi and z lengths: the forward keeps v
+    and w for the reverse steps, and the adjoint transforms skip the
+    de-aliased modes): 6 steps, checkpoints every 4, 3 directions."""
+    J, rows, timing = _chain_fd(C.POW2, 6, 3, 4)
+    rel = max(abs(fd - ad) / max(abs(fd), abs(ad), 1e-30) for fd, ad, _ in rows)
+    scaled = max(abs(fd - ad) / sc for fd, ad, sc in rows)
+    _say('  J %.6e, FD vs adjoint: worst rel. %.2e, scaled %.2e' % (J, rel, scaled), timing)
+    for fd, ad, _ in rows:
+        _say('    fd %.12e adjoint %.12e' % (fd, ad))
+    assert rel < 1e-5 and scaled < 1e-8
