@@ -1128,6 +1128,7 @@ int mlx_ctx_destroy(mlx_ctx *c) {
   if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
   if (c->ev_join) (void)hipEventDestroy(c->ev_join);
   if (c->aux) (void)hipStreamDestroy(c->aux);
+  if (c->v2t_part) (void)hipFree(c->v2t_part);
   for (double2 *s : c->scr)
     if (s) (void)hipFree(s);
   kit_free(c->kit);

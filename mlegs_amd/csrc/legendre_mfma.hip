@@ -491,7 +491,8 @@ __global__ void __launch_bounds__(256) k_v2t_mf(const double2 *__restrict__ ur, 
                                                 const T *__restrict__ tgD, const T *__restrict__ tgT,
                                                  const double *__restrict__ ak, int nrdim, int npdim, int nz, int nr,
                                                  int nrh, int nrc0, int npc, int nzc, int nzcu, int nyt, int zlo,
-                                                 int zhi, int mbeg, int mend, int asplit, int tpitch) {
+                                                 int zhi, int mbeg, int mend, int asplit, int tpitch, int ksplit,
+                                                 size_t zso, size_t kstride) {
   // AP: the table columns (radial rows a of one parity) this workgroup
   // stages; tpitch: the tables' row pitch (4 16 tiles of the widest kernel)
   constexpr int AP = 4 * AT * 16;
@@ -508,11 +509,18 @@ __global__ void __launch_bounds__(256) k_v2t_mf(const double2 *__restrict__ ur, 
   // this launch's columns: mbeg .. mend-1.  asplit workgroups share the
   // rows of one (m, parity, kappa tile): workgroup sp takes the 16-row tiles
   // wv + 4 (sp AT + s), s < AT, i.e. table columns acol0 .. acol0 + AP - 1
+  // ksplit > 1: workgroup ks contracts the radial points of its share
+  // xb .. xe - 1 only and writes its partial sums to psi/chi + ks kstride
+  // (slice stride zso); a second kernel adds the shares
   int m, y2;
-  xcd_map(blockIdx.x, 2 * nyt * asplit, m, y2);
+  xcd_map(blockIdx.x, 2 * nyt * asplit * ksplit, m, y2);
   m += mbeg;
-  const int p = y2 & 1, yt = (y2 >> 1) % nyt, sp = (y2 >> 1) / nyt;
+  const int p = y2 & 1, yt = (y2 >> 1) % nyt, sp = ((y2 >> 1) / nyt) % asplit, ks = ((y2 >> 1) / nyt) / asplit;
   if (m >= mend) return;
+  const int xshare = ((((nrh + KV - 1) / KV) + ksplit - 1) / ksplit) * KV;
+  const int xb = ks * xshare, xe = (xb + xshare < nrh) ? xb + xshare : nrh;
+  psi += (size_t)ks * kstride;
+  chi += (size_t)ks * kstride;
   const int k0 = yt * KT;
   const int tid = threadIdx.x, lane = tid % 64, wv = tid / 64;
   const size_t zs = (size_t)nrdim * npdim;
@@ -547,8 +555,8 @@ __global__ void __launch_bounds__(256) k_v2t_mf(const double2 *__restrict__ ur, 
   double rkv = 0.0;
   const int fx = tid % KV, fkl = tid / KV;
   if (active) {
-    for (int x0 = -KV; x0 < nrh; x0 += KV) {
-      if (x0 >= 0) {
+    for (int x0 = xb - KV; x0 < xe; x0 += KV) {
+      if (x0 >= xb) {
   #pragma unroll
         for (int j = 0; j < NTV; ++j) {
           const int idx = tid + 256 * j;
@@ -576,7 +584,7 @@ __global__ void __launch_bounds__(256) k_v2t_mf(const double2 *__restrict__ ur, 
         __syncthreads();
       }
       const int xn = x0 + KV;  // chunk fetched now, stored next iteration
-      if (xn < nrh) {
+      if (xn < xe) {
   #pragma unroll
         for (int j = 0; j < NTV; ++j) {
           const int idx = tid + 256 * j;
@@ -585,7 +593,7 @@ __global__ void __launch_bounds__(256) k_v2t_mf(const double2 *__restrict__ ur, 
             const int a = VE * (idx % (AP / VE)), x = (idx / (AP / VE)) % KV, q = idx / ((AP / VE) * KV);
             const int xg = xn + x;
             const T *src = (q == 0) ? tgV : (q == 1) ? tgD : tgT;
-            if (xg < nrh) v = *reinterpret_cast<const vec *>(src + tb + (size_t)xg * tpitch + a);
+            if (xg < xe) v = *reinterpret_cast<const vec *>(src + tb + (size_t)xg * tpitch + a);
           }
           rt[j] = v;
         }
@@ -593,7 +601,7 @@ __global__ void __launch_bounds__(256) k_v2t_mf(const double2 *__restrict__ ur, 
         const double2 z2 = make_double2(0.0, 0.0);
         rra = rrb = rpa = rpb = rza = rzb = z2;
         rkv = 0.0;
-        if (xg < nrh && kg < nz) {
+        if (xg < xe && kg < nz) {
           const size_t oa = (size_t)kg * zs + xg, ob = (size_t)kg * zs + (nr - 1 - xg);
           rra = cr[oa];
           rrb = cr[ob];
@@ -604,7 +612,7 @@ __global__ void __launch_bounds__(256) k_v2t_mf(const double2 *__restrict__ ur, 
           rkv = ak[kg];
         }
       }
-      if (x0 < 0) continue;
+      if (x0 < xb) continue;
   #pragma unroll
       for (int kk = 0; kk < KV; kk += 4) {
         const int xr = kk + lane / 16;
@@ -646,7 +654,7 @@ __global__ void __launch_bounds__(256) k_v2t_mf(const double2 *__restrict__ ur, 
         const int kg = k0 + 8 * t + 2 * (lane / 16) + h;
         if (kg >= nz) continue;
         const bool keep = active && a < M && ((kg + 1 <= nzc) || (kg + 1 >= nzcu)) && !(kg >= zlo && kg <= zhi);
-        size_t o = (size_t)kg * zs + (size_t)nrdim * m + n;
+        size_t o = (size_t)kg * zso + (size_t)nrdim * m + n;
         psi[o] = keep ? make_double2(aps[s][t][2 * h], aps[s][t][2 * h + 1]) : make_double2(0.0, 0.0);
         chi[o] = keep ? make_double2(ach[s][t][2 * h], ach[s][t][2 * h + 1]) : make_double2(0.0, 0.0);
       }
@@ -659,10 +667,31 @@ __global__ void __launch_bounds__(256) k_v2t_mf(const double2 *__restrict__ ur, 
   for (int idx = tid; idx < (nrest > 0 ? nrest : 0) * KT; idx += 256) {
     const int n = 2 * (A_END + idx % nrest) + p, kg = k0 + idx / nrest;
     if (kg < nz && n < nrdim) {
-      const size_t o = (size_t)kg * zs + (size_t)nrdim * m + n;
+      const size_t o = (size_t)kg * zso + (size_t)nrdim * m + n;
       psi[o] = make_double2(0.0, 0.0);
       chi[o] = make_double2(0.0, 0.0);
     }
+  }
+}
+
+// the shares of a split column-0 projection: psi(n, 0, kappa) = sum over the
+// ks shares in order (slice stride zs in psi and chi, nrdim in the shares)
+__global__ void k_v2t_col0_sum(double2 *__restrict__ psi, double2 *__restrict__ chi, const double2 *__restrict__ pp,
+                               const double2 *__restrict__ pc, int ksplit, size_t kstride, int nrdim, int nz,
+                               size_t zs) {
+  const size_t n = (size_t)nrdim * nz;
+  for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (size_t)gridDim.x * blockDim.x) {
+    double2 a = pp[t], b = pc[t];
+    for (int s = 1; s < ksplit; ++s) {
+      const double2 u = pp[(size_t)s * kstride + t], v = pc[(size_t)s * kstride + t];
+      a.x += u.x;
+      a.y += u.y;
+      b.x += v.x;
+      b.y += v.y;
+    }
+    const size_t o = (t / nrdim) * zs + t % nrdim;
+    psi[o] = a;
+    chi[o] = b;
   }
 }
 
@@ -855,12 +884,17 @@ void vec2tp_project_mfma(mlx_ctx *c, const double2 *ur, const double2 *up, const
   // rs: row splits; the launch of column 0 alone spreads its rows over
   // `at` workgroups of one tile per wave (more workgroups for its long
   // column, run beside the other columns' launch)
-  auto launch = [&](bool d, int mb, int me, hipStream_t s, int rs) {
-    dim3 grid = xcd_grid(me - mb, 2 * nyt * rs);
+  // ks > 1: the x contraction in ks shares written to (po, pc), slice
+  // stride nrdim, share stride kst
+  auto launch = [&](bool d, int mb, int me, hipStream_t s, int rs, int ks = 1, double2 *po = nullptr,
+                    double2 *pc = nullptr, size_t kst = 0) {
+    dim3 grid = xcd_grid(me - mb, 2 * nyt * rs * ks);
+    double2 *o0 = ks > 1 ? po : psi, *o1 = ks > 1 ? pc : chi;
+    const size_t zso = ks > 1 ? (size_t)k.nrdim : (size_t)k.nrdim * k.npdim;
 #define MLX_V2T(T_, AT_, V_, D_, T3_)                                                                    \
-  hipLaunchKernelGGL((k_v2t_mf<T_, AT_>), grid, dim3(256), 0, s, ur, up, uz, psi, chi, V_, D_, T3_, k.ak, k.nrdim, \
+  hipLaunchKernelGGL((k_v2t_mf<T_, AT_>), grid, dim3(256), 0, s, ur, up, uz, o0, o1, V_, D_, T3_, k.ak, k.nrdim, \
                      k.npdim, k.nzdim, k.nr, k.nrh, k.nrchop, npc, k.chopzl, k.chopzu, nyt, zlo, zhi, mb, me, rs, \
-                     k.mfma_ap)
+                     k.mfma_ap, ks, zso, kst)
     const int at_l = rs > 1 ? 1 : at;
     if (d) {
       if (at_l == 1) MLX_V2T(double, 1, k.tVg64, k.tDg64, k.tTg64);
@@ -879,9 +913,19 @@ void vec2tp_project_mfma(mlx_ctx *c, const double2 *ur, const double2 *up, const
     // fork: column 0 on the second stream once the inputs are ready, the
     // rest on the context stream; join before anything reads psi or chi
     aux_stream_setup(c);
+    // column 0 as 4 shares of its radial points (a quarter of the
+    // contraction's chain each), added in order by a second kernel
+    constexpr int KS = 4;
+    const size_t kst = (size_t)k.nrdim * k.nzdim;
+    if (!c->v2t_part) HIPCHK(hipMalloc(&c->v2t_part, sizeof(double2) * 2 * KS * kst));
+    double2 *po = c->v2t_part, *pc = c->v2t_part + KS * kst;
     HIPCHK(hipEventRecord(c->ev_fork, c->stream));
     HIPCHK(hipStreamWaitEvent(c->aux, c->ev_fork, 0));
-    launch(f64_0, 0, 1, c->aux, at);
+    launch(f64_0, 0, 1, c->aux, at, KS, po, pc, kst);
+    hipLaunchKernelGGL(k_v2t_col0_sum, dim3((unsigned)((kst + 255) / 256 < 1024 ? (kst + 255) / 256 : 1024)),
+                       dim3(256), 0, c->aux, psi, chi, po, pc, KS, kst, k.nrdim, k.nzdim,
+                       (size_t)k.nrdim * k.npdim);
+    HIPCHK(hipGetLastError());
     launch(f64, 1, k.npdim, c->stream, 1);
     HIPCHK(hipEventRecord(c->ev_join, c->aux));
     HIPCHK(hipStreamWaitEvent(c->stream, c->ev_join, 0));

@@ -158,6 +158,8 @@ struct mlx_ctx {
   // first use
   hipStream_t aux = nullptr;
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+  // partial sums of the split column-0 projection (allocated on first use)
+  double2 *v2t_part = nullptr;
 };
 
 inline void aux_stream_setup(mlx_ctx *c) {
