@@ -399,14 +399,15 @@ def transform_bench(tfm, p, names, n_rt, iters_probe=10):
 
 
 def kernel_sample(tfm, p, name, iters):
-    """Each probed kernel family on one field of the workload's size (plus the
-    copy), at the precision configure() has set."""
+    """Each probed kernel family on one field of the workload's size (plus a
+    plain copy, read-only and write-only pass), at the precision configure()
+    has set."""
     import mlegs_amd as M
     from mlegs_amd import inputs
     e = inputs.transform_field(tfm.dims, (p.nrchop, p.npchop, p.nzchop, p.np, p.nz), field_id=3)
     out = {}
-    for w in ('copy', 'fft_phi_fwd', 'fft_phi_bwd', 'fft_z_fwd', 'fft_z_bwd', 'leg_fwd', 'leg_bwd', 'v2t',
-              'lu_solve', 'phi_cross'):
+    for w in ('copy', 'read', 'write', 'fft_phi_fwd', 'fft_phi_bwd', 'fft_z_fwd', 'fft_z_bwd', 'leg_fwd', 'leg_bwd',
+              'v2t', 'lu_solve', 'phi_cross'):
         g = M.Scalar(tfm).upload(e, space='FFF')
         out[w] = round(1e3 * tfm.time_kernel(g, w, iters), 2)
         log(w, out[w], 'us')

@@ -461,9 +461,9 @@ int mlx_energy(mlx_ctx* ctx, const mlx_field* vr, const mlx_field* vp, const mlx
  * components), "lu_solve" (the ihelm solve of the AB2/CN step), "phi_cross"
  * (v x w formed inside the three forward phi transforms, the field as all
  * seven physical inputs), "copy" (a plain copy of the field, 32 bytes per
- * complex entry of HBM traffic).  The
- * field is used as operand and left in an unspecified state of the same
- * space. */
+ * complex entry of HBM traffic), "read" and "write" (a read-only and a
+ * write-only pass, 16 bytes per entry).  The field is used as operand and
+ * left in an unspecified state of the same space. */
 int mlx_time_kernel(mlx_ctx* ctx, mlx_field* f, const char* which, int32_t iters, double* ms);
 
 /* Seeded synthetic spectral field on the host (no context, no GPU): writes

@@ -208,13 +208,13 @@ __global__ void k_lap_ln_t_(const double2 *e, double *ln_bar, const double *logn
 // the vector product's adjoint at (v, u): with a the cotangent of v x u,
 // v_bar = u x a and u_bar = a x v, on each packed real sample
 // (the product's samples: rows i < nr, packed columns < np/2; zero elsewhere).
-// F32: v and u stored in FP32 (entry t at float pair t, as the forward's
-// fused product reads them); ub (optional): added to v3 (the background
+// F32: v and u stored in FP32 at f32_index (index t32), as the forward's
+// fused product reads them; ub (optional): added to v3 (the background
 // axial flow)
 template <bool F32>
-__device__ inline double2 ld_vw(const double2 *p, size_t t) {
+__device__ inline double2 ld_vw(const double2 *p, size_t t, size_t t32) {
   if (F32) {
-    const float2 f = reinterpret_cast<const float2 *>(p)[t];
+    const float2 f = reinterpret_cast<const float2 *>(p)[t32];
     return make_double2((double)f.x, (double)f.y);
   }
   return p[t];
@@ -232,14 +232,15 @@ __global__ void k_vecprod_adj_(double2 *a1, double2 *a2, double2 *a3, const doub
       ub1[t] = ub2[t] = ub3[t] = z2;
       continue;
     }
+    const size_t t32 = F32 ? f32_index(t / ((size_t)nrdim * npdim), j, i, nrdim, npdim) : t;
     const double2 A1 = a1[t], A2 = a2[t], A3 = a3[t];
-    const double2 V1 = ld_vw<F32>(v1, t), V2 = ld_vw<F32>(v2, t);
-    double2 V3 = ld_vw<F32>(v3, t);
+    const double2 V1 = ld_vw<F32>(v1, t, t32), V2 = ld_vw<F32>(v2, t, t32);
+    double2 V3 = ld_vw<F32>(v3, t, t32);
     if (bg) {
       const double2 g = bg[t];
       V3 = make_double2(V3.x + g.x, V3.y + g.y);
     }
-    const double2 U1 = ld_vw<F32>(u1, t), U2 = ld_vw<F32>(u2, t), U3 = ld_vw<F32>(u3, t);
+    const double2 U1 = ld_vw<F32>(u1, t, t32), U2 = ld_vw<F32>(u2, t, t32), U3 = ld_vw<F32>(u3, t, t32);
     // v_bar = u x a
     a1[t] = make_double2(U2.x * A3.x - U3.x * A2.x, U2.y * A3.y - U3.y * A2.y);
     a2[t] = make_double2(U3.x * A1.x - U1.x * A3.x, U3.y * A1.y - U1.y * A3.y);

@@ -1865,6 +1865,8 @@ int mlx_time_kernel(mlx_ctx *c, mlx_field *f, const char *which, int32_t iters, 
           MLX_THROW(MLX_E_ARG, "mlx_time_kernel: phi_cross is not built for this np");
       }
       else if (w == "copy") k_copy_field(c, c->scr[0], f->e);
+      else if (w == "read") k_read_field(c, f->e, reinterpret_cast<double *>(c->scr[0]));
+      else if (w == "write") k_write_field(c, c->scr[0], 0.0);
       else MLX_THROW(MLX_E_ARG, "mlx_time_kernel: unknown kernel " + w);
     };
     if (iters < 1) MLX_THROW(MLX_E_ARG, "mlx_time_kernel: iters must be positive");

@@ -329,8 +329,8 @@ __device__ __forceinline__ double2 twid(const double2 *tw, int idx, double sg) {
 // gs / es (each line of pairs starts on a 256-byte boundary); a tile holds
 // the pairs x0 .. x0 + TI/2 - 1 and synthesizes rows x = be + bo and
 // nr - 1 - x = be - bo (formed in FP64 on load), rows < nr only.
-// F32OUT (MODE 2): the output is stored in FP32, entry e at float pair e of
-// out's buffer (same indexing, half the bytes).
+// F32OUT (MODE 2): the output is stored in FP32 at f32_index (columns
+// padded to 128-byte lines, half the bytes).
 template <int N1, int N2, int TI, int MODE, bool PAIR = false, bool F32OUT = false>
 __global__ void __launch_bounds__(kThreads) k_fft4(double2 *out, const double2 *in, size_t es, size_t gs, int nrow,
                                                    int act_nrc, const double2 *tw, const double2 *twh, double sg,
@@ -503,11 +503,11 @@ __global__ void __launch_bounds__(kThreads) k_fft4(double2 *out, const double2 *
             double2 x = y2[t][k2];
             if (MODE == 0) x = make_double2(x.x * scale, x.y * scale);
             else if (dv != 1.0) x = make_double2(x.x / dv, x.y / dv);
-            if (F32OUT) st_nt_f32(out, base + (size_t)ko * es + i, x);
+            if (F32OUT) st_nt_f32(out, f32_index(blockIdx.y, ko, i, es, gs / es), x);
             else st_nt(&out[base + (size_t)ko * es + i], x);
           }
           if (MODE == 2 && k1 == 0) {
-            if (F32OUT) st_nt_f32(out, base + (size_t)N * es + i, make_double2(0.0, 0.0));
+            if (F32OUT) st_nt_f32(out, f32_index(blockIdx.y, N, i, es, gs / es), make_double2(0.0, 0.0));
             else st_nt(&out[base + (size_t)N * es + i], make_double2(0.0, 0.0));
           }
         }
@@ -548,8 +548,8 @@ __global__ void __launch_bounds__(kThreads) k_fft4(double2 *out, const double2 *
 // of the four-step transform forms the products while loading, so each
 // input is read once and no physical product field is written.  Rows
 // i >= nr give zero products, as vector_product leaves them.
-// F32IN: the six velocity and vorticity fields are stored in FP32 (ub, the
-// background flow, stays FP64).
+// F32IN: the six velocity and vorticity fields are stored in FP32 at
+// f32_index (ub, the background flow, stays FP64).
 template <int N1, int N2, int TI, bool F32IN = false>
 __global__ void __launch_bounds__(kThreads) k_fft4_cross(double2 *o0, double2 *o1, double2 *o2, const double2 *v0,
                                                          const double2 *v1, const double2 *v2, const double2 *w0,
@@ -590,12 +590,13 @@ __global__ void __launch_bounds__(kThreads) k_fft4_cross(double2 *o0, double2 *o
         if (i < nr) {
           const size_t o = base + (size_t)j * es + i;
           if (F32IN) {
-            a1 = ld_nt_f32(v0, o);
-            a2 = ld_nt_f32(v1, o);
-            a3 = ld_nt_f32(v2, o);
-            b1 = ld_nt_f32(w0, o);
-            b2 = ld_nt_f32(w1, o);
-            b3 = ld_nt_f32(w2, o);
+            const size_t o32 = f32_index(blockIdx.y, j, i, es, gs / es);
+            a1 = ld_nt_f32(v0, o32);
+            a2 = ld_nt_f32(v1, o32);
+            a3 = ld_nt_f32(v2, o32);
+            b1 = ld_nt_f32(w0, o32);
+            b2 = ld_nt_f32(w1, o32);
+            b3 = ld_nt_f32(w2, o32);
           } else {
             a1 = ld_nt(&v0[o]);
             a2 = ld_nt(&v1[o]);

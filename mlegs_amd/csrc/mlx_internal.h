@@ -360,6 +360,8 @@ void lu_cache_free(mlx_ctx *c);  // solve.hip
 void k_idel2_post(mlx_ctx *c, const double2 *e, double *ln);
 void k_idel2_preln00(mlx_ctx *c, double2 *e, double *ln, double preln, int nroff);
 void k_axpby(mlx_ctx *c, double2 *dst, double *dln, double a, const double2 *src, const double *sln, double b);
+void k_read_field(mlx_ctx *c, const double2 *src, double *part);
+void k_write_field(mlx_ctx *c, double2 *dst, double v);
 void k_copy_field(mlx_ctx *c, double2 *dst, const double2 *src);
 void k_abcn_rhs(mlx_ctx *c, double2 *sh, double *shln, const double2 *s, const double *sln,
                 const double2 *nl, const double *nlln, const double2 *nlp, const double *nlpln,
@@ -461,6 +463,16 @@ __device__ inline void xcd_map(int L, int nyt, int &u, int &y) {
   u = (q / nyt) * kXcd + x;
 }
 inline dim3 xcd_grid(int nu, int nyt) { return dim3(((nu + kXcd - 1) / kXcd) * kXcd * nyt); }
+
+// FP32 storage of a field's physical samples (the advection term's velocity
+// and vorticity): entry (row i, column j, slice k) at float pair
+// (k npdim + j) es32 + i of the field's buffer, es32 = nrdim rounded up to 16
+// so that every column starts on a 128-byte line (it fits: the FP64 field
+// holds 2 nrdim npdim nz float pairs)
+__host__ __device__ inline size_t f32_index(size_t k, size_t j, size_t i, size_t nrdim, size_t npdim) {
+  const size_t es32 = (nrdim + 15) & ~(size_t)15;
+  return (k * npdim + j) * es32 + i;
+}
 
 // Streaming (non-temporal) access to field data that a kernel reads or writes
 // once: the lines bypass the normal cache retention, so a pass over a 140 MB

@@ -1170,6 +1170,25 @@ __global__ void k_copy_field_(double2 *__restrict__ d, const double2 *__restrict
     d[i] = s[i];
 }
 
+// read-only and write-only passes over a whole field (the HBM ceilings of a
+// kernel that only reads or only writes): the read pass leaves one partial
+// sum per workgroup so the loads are not dropped
+__global__ void k_read_field_(const double2 *__restrict__ s, size_t n, double *__restrict__ part) {
+  __shared__ double sh[16];
+  double a = 0.0;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+    const double2 v = s[i];
+    a += v.x + v.y;
+  }
+  const double t = block_sum(a, sh);
+  if (threadIdx.x == 0) part[blockIdx.x] = t;
+}
+
+__global__ void k_write_field_(double2 *__restrict__ d, size_t n, double v) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+    d[i] = make_double2(v, v);
+}
+
 // |w| at each point, on the real and the imaginary parts separately (the two
 // azimuthal points a packed PPP entry holds): save_vort_mag
 // (vortical_flow_3d.f90:433-443)
@@ -1362,6 +1381,14 @@ void k_fftreat_mid(mlx_ctx *c, double2 *e, int npc) {
 void k_axpby(mlx_ctx *c, double2 *dst, double *dln, double a, const double2 *src, const double *sln, double b) {
   hipLaunchKernelGGL(k_axpby_, dim3(grid_for(c->nelem)), dim3(kT), 0, c->stream, dst, dln, a, src, sln, b,
                      c->nelem);
+}
+
+void k_read_field(mlx_ctx *c, const double2 *src, double *part) {
+  hipLaunchKernelGGL(k_read_field_, dim3(grid_for(c->nelem)), dim3(kT), 0, c->stream, src, c->nelem, part);
+}
+
+void k_write_field(mlx_ctx *c, double2 *dst, double v) {
+  hipLaunchKernelGGL(k_write_field_, dim3(grid_for(c->nelem)), dim3(kT), 0, c->stream, dst, c->nelem, v);
 }
 
 void k_copy_field(mlx_ctx *c, double2 *dst, const double2 *src) {
