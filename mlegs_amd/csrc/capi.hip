@@ -443,6 +443,16 @@ void abcn_pair(mlx_ctx *c, mlx_field *s1, mlx_field *s1p, mlx_field *nl1, mlx_fi
   fmove(nl2p, nl2);
 }
 
+// tp2vec_back's z transforms of the nf fields f[0..nf) in one launch
+void tp2vec_back_z(mlx_ctx *c, mlx_field *const *f, int nf, int ncols, int zlo, int zhi) {
+  const KitDev &k = c->kit;
+  if (k.nz <= 1) return;
+  const int nc = (ncols > 0 && ncols < k.npchop) ? ncols : k.npchop;
+  double2 *e[kZBatch];
+  for (int q = 0; q < nf; ++q) e[q] = f[q]->e;
+  fft_z_batch(c, e, nf, +1, active_pairs(k, 0, nc), k.nrchop, nc, zlo, zhi);
+}
+
 // Back half of tp2vec: the three FFF components (offset 0, no log term) to
 // PPP; ur and up are divided by r in the last phi transform.  ncols > 0: the
 // components are zero in columns m >= ncols, which the transforms skip.
@@ -453,8 +463,9 @@ void abcn_pair(mlx_ctx *c, mlx_field *s1, mlx_field *s1p, mlx_field *nl1, mlx_fi
 // f32_ok: the caller reads the outputs through fft_phi_cross's FP32 input;
 // when the radial pairs path runs (FP32 sums) the outputs are then stored in
 // FP32.  Returns whether they were.
+// z_done: the caller has run the z transforms (tp2vec_back_z)
 bool tp2vec_back(mlx_ctx *c, mlx_field *ur, mlx_field *up, mlx_field *uz, int ncols = -1, int zlo = 1,
-                 int zhi = 0, bool phys_rows = false, bool f32_ok = false) {
+                 int zhi = 0, bool phys_rows = false, bool f32_ok = false, bool z_done = false) {
   const KitDev &k = c->kit;
   PrecScope ps(c, MLX_STAGE_SYNTH);
   const int rs[3] = {-1, -1, 0};
@@ -466,11 +477,12 @@ bool tp2vec_back(mlx_ctx *c, mlx_field *ur, mlx_field *up, mlx_field *uz, int nc
   // phi transform its FP32 even/odd sums (half the bytes of the FP64 rows,
   // which the phi transform forms from them exactly)
   const bool pairs = lean && phys_rows && (c->prec == MLX_PREC_FP32 || c->prec == MLX_PREC_BF16X3) && mfma_ok(k, 0);
+  // the de-aliased kappa slices zlo..zhi of the input are zero
+  mlx_field *const three[3] = {ur, up, uz};
+  if (!z_done) tp2vec_back_z(c, three, 3, ncols, zlo, zhi);
   int q = 0;
-  for (mlx_field *f : {ur, up, uz}) {
+  for (mlx_field *f : three) {
     f->off[0] = f->off[1] = f->off[2] = 0;
-    // the de-aliased kappa slices zlo..zhi of the input are zero
-    if (k.nz > 1) fft_z(c, f->e, +1, active_pairs(k, 0, nc), k.nrchop, nc, zlo, zhi);
     if (pairs) {
       float4 *pr = reinterpret_cast<float4 *>(c->scr[0]);
       leg_backward_pairs_f32(c, f->e, pr, nc);
@@ -859,8 +871,10 @@ void advection_adjoint(mlx_ctx *c, mlx_field *psi, mlx_field *chi, const mlx_fie
     mlx_field *wr = tmp(c, 27), *wp = tmp(c, 28), *wz = tmp(c, 29);
     k_tp2vec_front2(c, vr->e, vp->e, vz->e, vr->sc, vp->sc, vz->sc, wr->e, wp->e, wz->e, wr->sc, wp->sc, wz->sc,
                     psi->e, chi->e, psi->sc, chi->sc, chi->off[0], ncols, zskip ? dl.zlo : 1, zskip ? dl.zhi : 0);
-    tp2vec_back(c, vr, vp, vz, ncols, dl.zlo, dl.zhi, true, false);
-    tp2vec_back(c, wr, wp, wz, ncols, dl.zlo, dl.zhi, true, false);
+    mlx_field *const six[6] = {vr, vp, vz, wr, wp, wz};
+    tp2vec_back_z(c, six, 6, ncols, dl.zlo, dl.zhi);
+    tp2vec_back(c, vr, vp, vz, ncols, dl.zlo, dl.zhi, true, false, true);
+    tp2vec_back(c, wr, wp, wz, ncols, dl.zlo, dl.zhi, true, false, true);
     v[0] = vr->e, v[1] = vp->e, v[2] = vz->e;
     w[0] = wr->e, w[1] = wp->e, w[2] = wz->e;
   }
@@ -924,7 +938,11 @@ void vec2tp_pfp(mlx_ctx *c, mlx_field *ur, mlx_field *up, mlx_field *uz, mlx_fie
   // the projection reads columns m < npchop (nc) only
   // outputs in the kappa slices the projection skips are not written
   const int zlo = dl.on ? dl.zlo : 1, zhi = dl.on ? dl.zhi : 0;
-  if (k.nz > 1) fft_z(c, ur->e, -1, (double)k.nr * nc, 0, nc, 1, 0, zlo, zhi);
+  // ur's and uz's z transforms (the same lines) in one launch
+  if (k.nz > 1) {
+    double2 *const e2[2] = {ur->e, uz->e};
+    fft_z_batch(c, e2, 2, -1, (double)k.nr * nc, 0, nc, 1, 0, zlo, zhi);
+  }
   // up to FFF as trans does it (PFP -> FFP -> FFF), over columns m < nc.
   // With the de-aliasing limits the columns m >= nc of up are read by
   // nothing below (the z transforms, the radial transforms and the
@@ -942,7 +960,6 @@ void vec2tp_pfp(mlx_ctx *c, mlx_field *ur, mlx_field *up, mlx_field *uz, mlx_fie
   leg_backward(c, up->e, c->scr[0], up->off[0], true, nc, !dl.on);
   swap_scr(c, up);
   k_vec2tp_up_ln(c, up->e, inf, psi->sc);
-  if (k.nz > 1) fft_z(c, uz->e, -1, (double)k.nr * nc, 0, nc, 1, 0, zlo, zhi);
   {
     const int prec_m0 = stage_precision(c, MLX_STAGE_PROJ0);
     PrecScope ps_proj(c, MLX_STAGE_PROJ);
@@ -1018,8 +1035,11 @@ int advection_rhs(mlx_ctx *c, mlx_field *psi, mlx_field *chi, mlx_field *nlpsi, 
     // the six physical fields are read only by the fused cross-product
     // transform below: with FP32 radial sums they are stored in FP32
     const bool cross = fft_phi_cross_ok(c->kit);
-    f32phys = tp2vec_back(c, vr, vp, vz, ncols, dl.zlo, dl.zhi, true, cross);
-    if (tp2vec_back(c, wr, wp, wz, ncols, dl.zlo, dl.zhi, true, cross) != f32phys)
+    // the six z transforms in one launch
+    mlx_field *const six[6] = {vr, vp, vz, wr, wp, wz};
+    tp2vec_back_z(c, six, 6, ncols, dl.zlo, dl.zhi);
+    f32phys = tp2vec_back(c, vr, vp, vz, ncols, dl.zlo, dl.zhi, true, cross, true);
+    if (tp2vec_back(c, wr, wp, wz, ncols, dl.zlo, dl.zhi, true, cross, true) != f32phys)
       MLX_THROW(MLX_E_ARG, "advection_rhs: velocity and vorticity stored at different precisions");
   }
   // de-aliased inside vec2tp before idel2; chop and zeroat1 keep zero modes

@@ -25,6 +25,11 @@ namespace {
 constexpr int kThreads = 256;
 constexpr int kLdsElems = 4096;  // complex doubles per workgroup (64 KiB)
 
+// The fields of a batched in-place z transform (one per blockIdx.z)
+struct ZFields {
+  double2 *p[mlx::kZBatch];
+};
+
 struct Cx {
   double x, y;
 };
@@ -331,13 +336,19 @@ __device__ __forceinline__ double2 twid(const double2 *tw, int idx, double sg) {
 // nr - 1 - x = be - bo (formed in FP64 on load), rows < nr only.
 // F32OUT (MODE 2): the output is stored in FP32 at f32_index (columns
 // padded to 128-byte lines, half the bytes).
+// MODE 0 with gridDim.z > 1: the in-place transform of the fields zf.p[z]
+// (same shape and limits), one launch for all.
 template <int N1, int N2, int TI, int MODE, bool PAIR = false, bool F32OUT = false>
 __global__ void __launch_bounds__(kThreads) k_fft4(double2 *out, const double2 *in, size_t es, size_t gs, int nrow,
                                                    int act_nrc, const double2 *tw, const double2 *twh, double sg,
                                                    double scale, const double *r, int nr, int rscale, int mlim,
                                                    int zin_lo, int zin_hi, int zout_lo, int zout_hi,
-                                                   int pstride) {
+                                                   int pstride, ZFields zf) {
   constexpr int N = N1 * N2;
+  if (MODE == 0 && gridDim.z > 1) {
+    out = zf.p[blockIdx.z];
+    in = out;
+  }
   constexpr int T1 = (TI * N2 + kThreads - 1) / kThreads;
   constexpr int T2 = (TI * N1 + kThreads - 1) / kThreads;
   constexpr bool HALF = (MODE == 0 && N == 256);
@@ -688,21 +699,23 @@ __global__ void __launch_bounds__(kThreads) k_fft4_cross(double2 *o0, double2 *o
   }
 }
 
+// grid_y.z > 1 (MODE 0): the batched in-place transform of the fields zf.p[0 .. grid_y.z)
 template <int MODE, bool PAIR = false, bool F32OUT = false>
 bool launch_fft4(mlx_ctx *c, ProbeScope &ps, int N, dim3 grid_y, double2 *out, const double2 *in, size_t es,
                  size_t gs, int nrow, int act_nrc, const double2 *tw, const double2 *twh, double sg, double scale,
                  int rscale, int mlim = 1 << 30, int zin_lo = 1, int zin_hi = 0, int zout_lo = 1, int zout_hi = 0,
-                 int pstride = 0) {
+                 int pstride = 0, ZFields zf = ZFields()) {
   const KitDev &k = c->kit;
-#define MLX_FFT4(N1_, N2_, TI_)                                                                               \
-  do {                                                                                                        \
-    constexpr int TH_ = PAIR ? TI_ / 2 : TI_;                                                                 \
-    dim3 grid((nrow + TH_ - 1) / TH_, grid_y.y);                                                              \
-    probed_launch(ps, k_fft4<N1_, N2_, TI_, MODE, PAIR, F32OUT>, grid, dim3(kThreads), 0, c->stream, out, in, es, gs, \
-                  nrow,                                                                                       \
-                  act_nrc, tw, twh, sg, scale, k.r, k.nr, rscale, mlim, zin_lo, zin_hi, zout_lo,     \
-                       zout_hi, pstride);                                                                    \
-    return true;                                                                                              \
+  if (grid_y.z > 1 && (MODE != 0 || grid_y.z > (unsigned)mlx::kZBatch))
+    MLX_THROW(MLX_E_ARG, "fft: batched launch outside the z transform's limits");
+#define MLX_FFT4(N1_, N2_, TI_)                                                                                    \
+  do {                                                                                                             \
+    constexpr int TH_ = PAIR ? TI_ / 2 : TI_;                                                                      \
+    dim3 grid((nrow + TH_ - 1) / TH_, grid_y.y, grid_y.z);                                                         \
+    probed_launch(ps, k_fft4<N1_, N2_, TI_, MODE, PAIR, F32OUT>, grid, dim3(kThreads), 0, c->stream, out, in, es, \
+                  gs, nrow, act_nrc, tw, twh, sg, scale, k.r, k.nr, rscale, mlim, zin_lo, zin_hi, zout_lo,         \
+                  zout_hi, pstride, zf);                                                                           \
+    return true;                                                                                                   \
   } while (0)
   switch (N) {
     case 16: MLX_FFT4(4, 4, 64);
@@ -827,11 +840,12 @@ bool fft_phi_cross(mlx_ctx *c, double2 *o0, double2 *o1, double2 *o2, const doub
   return true;
 }
 
-void fft_z(mlx_ctx *c, double2 *e, int dir, double ncol, int act_nrc, int ncols, int zin_lo, int zin_hi,
-           int zout_lo, int zout_hi) {
+void fft_z_batch(mlx_ctx *c, double2 *const *e, int nf, int dir, double ncol, int act_nrc, int ncols, int zin_lo,
+                 int zin_hi, int zout_lo, int zout_hi) {
   const KitDev &k = c->kit;
-  if (k.nz <= 1) return;
+  if (k.nz <= 1 || nf <= 0) return;
   if (k.nz > 512) MLX_THROW(MLX_E_ARG, "fft_z: nz above 512 is not supported");
+  if (nf > kZBatch) MLX_THROW(MLX_E_ARG, "fft_z: at most kZBatch fields per launch");
   if (ncols <= 0 || ncols > k.npdim) ncols = k.npdim;
   auto span = [&](int lo, int hi) {
     lo = lo < 0 ? 0 : lo;
@@ -839,19 +853,29 @@ void fft_z(mlx_ctx *c, double2 *e, int dir, double ncol, int act_nrc, int ncols,
     return hi >= lo ? hi - lo + 1 : 0;
   };
   // entries read and written per line (the skipped ranges neither)
-  ProbeScope probe(c, MLX_PROBE_FFT_Z, 16.0 * ncol * (2.0 * k.nz - span(zin_lo, zin_hi) - span(zout_lo, zout_hi)),
-                   5.0 * k.nz * std::log2((double)k.nz) * ncol);
-  if (launch_fft4<0>(c, probe, k.nz, dim3(1, ncols), e, e, (size_t)k.nrdim * k.npdim, k.nrdim, k.nrdim, act_nrc,
-                     k.tw_nz, nullptr, dir < 0 ? -1.0 : 1.0, dir < 0 ? 1.0 / k.nz : 1.0, 0, 1 << 30, zin_lo, zin_hi,
-                     zout_lo, zout_hi)) {
+  ProbeScope probe(c, MLX_PROBE_FFT_Z,
+                   16.0 * nf * ncol * (2.0 * k.nz - span(zin_lo, zin_hi) - span(zout_lo, zout_hi)),
+                   5.0 * nf * k.nz * std::log2((double)k.nz) * ncol);
+  ZFields zf;
+  for (int f = 0; f < kZBatch; ++f) zf.p[f] = e[f < nf ? f : 0];
+  if (launch_fft4<0>(c, probe, k.nz, dim3(1, ncols, nf), e[0], e[0], (size_t)k.nrdim * k.npdim, k.nrdim, k.nrdim,
+                     act_nrc, k.tw_nz, nullptr, dir < 0 ? -1.0 : 1.0, dir < 0 ? 1.0 / k.nz : 1.0, 0, 1 << 30, zin_lo,
+                     zin_hi, zout_lo, zout_hi, 0, zf)) {
     HIPCHK(hipGetLastError());
     return;
   }
+  // other lengths: the Stockham kernel, one launch per field
   int TI = tile_rows(k.nz, k.nrdim);
   dim3 grid((k.nrdim + TI - 1) / TI, k.npdim);
-  probed_launch(probe, k_fft_z, grid, dim3(kThreads), 0, c->stream, e, k.nrdim, k.npdim, k.nzdim, TI, k.plan_nz,
-                     k.tw_nz, dir);
+  for (int f = 0; f < nf; ++f)
+    probed_launch(probe, k_fft_z, grid, dim3(kThreads), 0, c->stream, e[f], k.nrdim, k.npdim, k.nzdim, TI,
+                  k.plan_nz, k.tw_nz, dir);
   HIPCHK(hipGetLastError());
+}
+
+void fft_z(mlx_ctx *c, double2 *e, int dir, double ncol, int act_nrc, int ncols, int zin_lo, int zin_hi,
+           int zout_lo, int zout_hi) {
+  fft_z_batch(c, &e, 1, dir, ncol, act_nrc, ncols, zin_lo, zin_hi, zout_lo, zout_hi);
 }
 
 }  // namespace mlx

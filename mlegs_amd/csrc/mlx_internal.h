@@ -292,6 +292,11 @@ bool fft_phi_cross_ok(const KitDev &k);  // the fused kernel is built for this n
 // the generic kernel reads and writes everything
 void fft_z(mlx_ctx *c, double2 *e, int dir, double ncol, int act_nrc = 0, int ncols = 0, int zin_lo = 1,
            int zin_hi = 0, int zout_lo = 1, int zout_hi = 0);
+// the same transform of the nf (<= kZBatch) fields e[0..nf) in one launch
+// (ncol: per field)
+constexpr int kZBatch = 6;
+void fft_z_batch(mlx_ctx *c, double2 *const *e, int nf, int dir, double ncol, int act_nrc = 0, int ncols = 0,
+                 int zin_lo = 1, int zin_hi = 0, int zout_lo = 1, int zout_hi = 0);
 
 // legendre.hip
 // ncols > 0 (MFMA path): only columns m < ncols are transformed, the others
