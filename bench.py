@@ -574,6 +574,34 @@ def adjoint_bench(nsteps, ndir, segment=20, amp=1e-4):
     return out
 
 
+def hyper_bench(n, steps, warmup=2, hyperpow=8, hypervisc=5e-7):
+    """The hyperviscous step: the headline workload with HYPERPOW = 8 and
+    HYPERVISC = 5e-7, the reference's shipped setting (input.params:15), so
+    that every Helmholtz solve is ihelmp's banded (kl = ku = 8) system.  Steps/s
+    in the mixed preset and in FP64, and the two runs' relative difference
+    after the timed steps."""
+    import mlegs_amd as M
+    import numpy as np
+    p = M.Params(nr=n, np=n, nz=n, nrchop=n, npchop=n // 2 + 1, nzchop=n // 2 + 1, ell=1.0, zlen=2 * math.pi,
+                 visc=5e-4, dt=5e-4, hyperpow=hyperpow, hypervisc=hypervisc, is_svv=True)
+    tfm = M.Kit(p, device=0, precision=M.PREC_FP64)
+    psi0, chi0 = initial_state(tfm, p)
+    out = {'workload': 'the 3D vortex step above with HYPERPOW=%d, HYPERVISC=%g' % (hyperpow, hypervisc),
+           'steps': steps}
+    final = {}
+    for nm in ('mixed', 'fp64'):
+        log('hyperviscous run', nm)
+        run, t = timed_run(tfm, nm, psi0, chi0, warmup, steps)
+        out[nm] = {'steps_per_s': steps / t, 'ms_per_step': 1e3 * t / steps}
+        final[nm] = (run.field('psi').download(), run.field('chi').download())
+        del run
+        log('  %.3f ms/step' % out[nm]['ms_per_step'])
+    out['mixed_vs_fp64'] = {'psi': rel(final['mixed'][0], final['fp64'][0]),
+                            'chi': rel(final['mixed'][1], final['fp64'][1])}
+    tfm.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
@@ -601,6 +629,8 @@ def main():
     ap.add_argument('--adjoint', type=int, default=200, help='steps of the adjoint leg (0: skip)')
     ap.add_argument('--adjoint-segment', type=int, default=20, help='checkpoint interval of the adjoint leg')
     ap.add_argument('--adjoint-dirs', type=int, default=32, help='finite-difference directions of the adjoint leg')
+    ap.add_argument('--hyper', type=int, default=10, help='steps of the HYPERPOW = 8 leg (0: skip)')
+    ap.add_argument('--hyper-only', type=int, default=0, help='run the HYPERPOW = 8 leg alone and print it')
     ap.add_argument('--kernel-sample', type=int, default=0,
                     help='only launch each probed kernel family (and a plain field copy) this many times, '
                          'for hardware-counter runs; prints their mean durations')
@@ -615,6 +645,9 @@ def main():
     if args.ensemble_only:
         print(json.dumps(ensemble_bench(args.ensemble, args.ensemble_steps, args.ensemble_check,
                                         mode=args.ensemble_mode, streams=args.ensemble_streams)), flush=True)
+        return
+    if args.hyper_only:
+        print(json.dumps(hyper_bench(args.size, args.hyper)), flush=True)
         return
 
     p = workload(args.size)
@@ -713,6 +746,10 @@ def main():
     if args.adjoint:
         adj = adjoint_bench(args.adjoint, args.adjoint_dirs, args.adjoint_segment)
 
+    hyp = None
+    if args.hyper:
+        hyp = hyper_bench(args.size, args.hyper)
+
     ens = None
     if args.ensemble:
         ens = ensemble_bench(args.ensemble, args.ensemble_steps, args.ensemble_check,
@@ -736,7 +773,7 @@ def main():
         'rel_err_vs_fp64': err, 'ke_history': ke, 'other_precisions': others,
         'adjoint_forward_ratio': adj['adjoint_forward_ratio'] if adj else None, 'adjoint_c5': adj,
         'probed_steps_per_s': probed_rate,
-        'transform_c2': transform, 'ensemble_c4': ens,
+        'transform_c2': transform, 'ensemble_c4': ens, 'hyperviscous_p8': hyp,
         'kernels': probe_table(probes),
         'roofline': roof, 'cpu_baseline': cpu,
     }

@@ -1140,6 +1140,7 @@ int mlx_ctx_destroy(mlx_ctx *c) {
   for (mlx_field *f : c->pool) field_free(f);
   c->pool.clear();
   lu_cache_free(c);
+  hyp_cache_free(c);
   for (auto &v : c->probe.ev)
     for (hipEvent_t e : v) (void)hipEventDestroy(e);
   for (auto &kv : c->work)
@@ -1193,6 +1194,13 @@ int mlx_kit_init(mlx_ctx *c) {
     kit_build(c);
     const KitDev &k = c->kit;
     c->nelem = (size_t)k.nrdim * k.npdim * k.nzdim;
+    // a buffer sized by an earlier kit is allocated again on first use
+    if (c->v2t_part) {
+      HIPCHK(hipStreamSynchronize(c->stream));
+      if (c->aux) HIPCHK(hipStreamSynchronize(c->aux));
+      (void)hipFree(c->v2t_part);
+      c->v2t_part = nullptr;
+    }
     for (double2 *&s : c->scr) {
       if (s) (void)hipFree(s);
       if (hipMalloc(&s, sizeof(double2) * c->nelem) != hipSuccess) MLX_THROW(MLX_E_ALLOC, "scratch allocation failed");

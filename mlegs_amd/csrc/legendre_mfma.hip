@@ -917,7 +917,8 @@ void vec2tp_project_mfma(mlx_ctx *c, const double2 *ur, const double2 *up, const
     // contraction's chain each), added in order by a second kernel
     constexpr int KS = 4;
     const size_t kst = (size_t)k.nrdim * k.nzdim;
-    if (!c->v2t_part) HIPCHK(hipMalloc(&c->v2t_part, sizeof(double2) * 2 * KS * kst));
+    lazy_alloc(c, reinterpret_cast<void **>(&c->v2t_part), sizeof(double2) * 2 * KS * kst,
+               "vector_projection: column-0 shares");
     double2 *po = c->v2t_part, *pc = c->v2t_part + KS * kst;
     HIPCHK(hipEventRecord(c->ev_fork, c->stream));
     HIPCHK(hipStreamWaitEvent(c->aux, c->ev_fork, 0));

@@ -100,6 +100,24 @@ struct LuCache {
   uint64_t stamp = 0;
 };
 
+// Factors of ihelmp's banded systems (del2^(P/2) + beta del2 + alpha, band
+// kl = ku = P, partial pivoting) for one (P, alpha, beta, radial offset):
+// one system per (m, |kappa|), sys = m * nkk + kk; row j of a system's
+// LAPACK band storage (LDAB = 3P + 1 entries: U's diagonal and 2P
+// superdiagonals at r = 2P - q, L's P multipliers at r = 2P + 1 ..) at
+// ab + (j * nsys + sys) * (3P + 2), entry r at r (r <= 2P) or r + 1 (the L
+// part starts 16-byte aligned); the row interchange of row j as the offset
+// to the pivot row at pv[j * nsys + sys].
+struct HypLu {
+  int power = 0;
+  double alpha = 0.0, beta = 0.0;
+  int nroff = 0;
+  int nkk = 0, nsys = 0;
+  double *ab = nullptr;
+  int8_t *pv = nullptr;
+  uint64_t stamp = 0;
+};
+
 // the state of a vortex run: psi chi nlpsi nlchi psi_prev chi_prev nlpsi_prev
 // nlchi_prev uz, the SVV gains and the non-finite flag on the device
 struct mlx_vortex {
@@ -151,6 +169,7 @@ struct mlx_ctx {
   double *hsmall = nullptr;     // pinned host mirror (4096 doubles)
   std::vector<mlx_field *> pool;  // temporaries (allocated on demand)
   std::vector<LuCache> lu;
+  std::vector<HypLu> hyp;  // ihelmp factors, at most kHypEntries (least recently used evicted)
   uint64_t lu_clock = 0;
   ProbeSet probe;
   // second stream (default priority) for work that overlaps the context
@@ -161,6 +180,21 @@ struct mlx_ctx {
   // partial sums of the split column-0 projection (allocated on first use)
   double2 *v2t_part = nullptr;
 };
+
+// A buffer the context allocates on first use (*p of bytes).  Under stream
+// capture nothing may be allocated: a step run once before the capture
+// allocates it.
+inline void lazy_alloc(mlx_ctx *c, void **p, size_t bytes, const char *what) {
+  if (*p) return;
+  hipStreamCaptureStatus st = hipStreamCaptureStatusNone;
+  HIPCHK(hipStreamIsCapturing(c->stream, &st));
+  if (st != hipStreamCaptureStatusNone)
+    MLX_THROW(MLX_E_ARG, std::string(what) + ": first use under stream capture (run one step before capturing)");
+  if (hipMalloc(p, bytes) != hipSuccess) {
+    *p = nullptr;
+    MLX_THROW(MLX_E_ALLOC, std::string(what) + ": allocation failed");
+  }
+}
 
 inline void aux_stream_setup(mlx_ctx *c) {
   if (c->aux) return;
@@ -362,6 +396,7 @@ void k_helm_pre(mlx_ctx *c, double2 *e, double *ln_out, const double *ln_in, dou
 void k_helm_solve(mlx_ctx *c, double2 *e, double alpha, int kind, int nroff, int pcut = -1, int zlo = 1,
                   int zhi = 0, double2 *e2 = nullptr);
 void lu_cache_free(mlx_ctx *c);  // solve.hip
+void hyp_cache_free(mlx_ctx *c);  // spectral.hip
 void k_idel2_post(mlx_ctx *c, const double2 *e, double *ln);
 void k_idel2_preln00(mlx_ctx *c, double2 *e, double *ln, double preln, int nroff);
 void k_axpby(mlx_ctx *c, double2 *dst, double *dln, double a, const double2 *src, const double *sln, double b);

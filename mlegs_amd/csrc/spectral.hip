@@ -873,29 +873,30 @@ __global__ void k_vecprod_(double2 *vr, double2 *vp, double2 *vz, const double2 
 // (leg_del2^(P/2) + beta leg_del2 + alpha) x = b of the nn x nn truncations,
 // banded with kl = ku = P, factored with partial pivoting as LAPACK gbtf2
 // (first maximum as pivot, multipliers scaled by the pivot's reciprocal) and
-// solved as gbtrs, once per visit of the slice.  One workgroup per column;
-// the band storage AB (2P + P + 1 rows, A(i, j) at AB[2P + i - j][j]), the
-// right-hand side and the pivots sit in LDS.  Workgroup (0, 0) first removes
-// the log term's contribution (bl, bl2 of the reference) and sets ln/alpha.
+// solved as gbtrs, once per visit of the slice.  The matrix depends on kappa
+// only through kappa^2, so one factorization per (m, |kappa|) serves both
+// slices, and it is kept (HypLu) for every later call with the same
+// (P, alpha, beta, offset).  Three kernels: the factorization (once), the
+// (0,0) column's log term, and the solve.
 // ---------------------------------------------------------------------------
+
+// One workgroup per (m, |kappa|): the band storage AB (2P + P + 1 rows,
+// A(i, j) at AB[2P + i - j][j]) formed and factored in LDS, then written out
+// in HypLu's layout.  A zero pivot sets *sing and leaves the system.
 template <int P>
-__global__ void __launch_bounds__(256) k_ihelmp_(double2 *e, double *sln, const double *d2c, const double *ak,
-                                                 const double *lognorm, double alpha, double beta, int nrdim,
-                                                 int npdim, int ne, int nrc, int npc, int nzc, int nzcu,
-                                                 double ell, int *sing) {
+__global__ void __launch_bounds__(256) k_ihelmp_fac_(double *__restrict__ ab, int8_t *__restrict__ pv,
+                                                     const double *d2c, const double *ak, double alpha, double beta,
+                                                     int ne, int nrc, int npc, int nkk, int *sing) {
   constexpr int Q = P / 2, KV = 2 * P, LDAB = 3 * P + 1, W = 2 * P + 1;
   extern __shared__ double hsm[];
-  const int m = blockIdx.x, k = blockIdx.y, tid = threadIdx.x;
-  const int cnt = z_count(k, nzc, nzcu);
+  const int m = blockIdx.x, kk = blockIdx.y, tid = threadIdx.x;
   const int nn = nrcs_dev(nrc, npc, m);
-  if (cnt == 0 || nn <= 0) return;
-  double *AB = hsm;  // LDAB * nn, rounded up to an even count: b is 16-byte aligned
-  double2 *b = reinterpret_cast<double2 *>(AB + (((size_t)LDAB * nn + 1) & ~(size_t)1));  // nn
-  double *v = reinterpret_cast<double *>(b + nn);              // nn
-  double *t = v + nn;                                          // nn
-  int *piv = reinterpret_cast<int *>(t + nn);                  // nn
+  if (nn <= 0) return;
+  const size_t nsys = (size_t)npc * nkk, sys = (size_t)m * nkk + kk;
+  double *AB = hsm;                                                             // LDAB * nn
+  int *piv = reinterpret_cast<int *>(AB + (((size_t)LDAB * nn + 1) & ~(size_t)1));  // nn
   __shared__ int s_jp, s_ju, s_sing;
-  const double akk = ak[k] * ak[k];
+  const double akk = ak[kk] * ak[kk];
   const double *dm = d2c + (size_t)m * ne * 5;
   auto dcoef = [&](int r, int d) { return dm[(size_t)r * 5 + d] - (d == 2 ? akk : 0.0); };
   for (int q = tid; q < LDAB * nn; q += blockDim.x) AB[q] = 0.0;
@@ -936,39 +937,6 @@ __global__ void __launch_bounds__(256) k_ihelmp_(double2 *e, double *sln, const 
       if (j >= 0 && j < nn) AB[(size_t)j * LDAB + (KV + i - j)] = r[o];
     }
   }
-  const size_t off = (size_t)k * nrdim * npdim + (size_t)nrdim * m;
-  const bool c00 = (m == 0 && k == 0);
-  double lnp = 0.0;
-  if (c00) {
-    // bl2: the log term's Laplacian rows; bl = leg_del2(0, ak_0)^(Q-1) bl2
-    lnp = sln[0] / alpha;
-    const double ell2 = ell * ell;
-    const double b0 = 4.0 / 3.0 / ell2 / exp(lognorm[0]) * lnp, b1 = -2.0 / ell2 / exp(lognorm[1]) * lnp,
-                 b2 = 2.0 / 3.0 / ell2 / exp(lognorm[2]) * lnp;
-    for (int i = tid; i < nn; i += blockDim.x) v[i] = (i == 0) ? b0 : (i == 1) ? b1 : (i == 2) ? b2 : 0.0;
-    __syncthreads();
-    for (int st = 0; st < Q - 1; ++st) {
-      for (int i = tid; i < nn; i += blockDim.x) {
-        double a = 0.0;
-        for (int d = 0; d < 5; ++d) {
-          const int j = i + d - 2;
-          if (j >= 0 && j < nn) a += dcoef(i, d) * v[j];
-        }
-        t[i] = a;
-      }
-      __syncthreads();
-      for (int i = tid; i < nn; i += blockDim.x) v[i] = t[i];
-      __syncthreads();
-    }
-    for (int i = tid; i < nn; i += blockDim.x) {
-      const double bl2 = (i == 0) ? b0 : (i == 1) ? b1 : (i == 2) ? b2 : 0.0;
-      double2 x = e[off + i];
-      x.x = x.x - v[i] - beta * bl2;
-      e[off + i] = x;
-    }
-    __syncthreads();
-  }
-  for (int i = tid; i < nn; i += blockDim.x) b[i] = e[off + i];
   if (tid == 0) s_ju = 0;
   __syncthreads();
   // gbtf2
@@ -980,7 +948,7 @@ __global__ void __launch_bounds__(256) k_ihelmp_(double2 *e, double *sln, const 
       double best = fabs(cj[KV]);
       for (int q = 1; q <= km; ++q)
         if (fabs(cj[KV + q]) > best) { best = fabs(cj[KV + q]); jp = q; }
-      piv[j] = j + jp;
+      piv[j] = jp;
       s_jp = jp;
       // the pivot test is made once, here, and read by every thread from
       // LDS: after the barrier tid 0 may already be swapping cj[KV + jp]
@@ -994,7 +962,7 @@ __global__ void __launch_bounds__(256) k_ihelmp_(double2 *e, double *sln, const 
     const int jp = s_jp, ju = s_ju;
     if (s_sing) {  // uniform over the workgroup: every thread leaves here
       if (tid == 0) atomicOr(sing, 1);
-      return;  // singular: left unsolved and reported (the reference stops)
+      return;  // singular: reported (the reference stops)
     }
     if (jp != 0)
       for (int cc = j + tid; cc <= ju; cc += blockDim.x) {
@@ -1015,43 +983,206 @@ __global__ void __launch_bounds__(256) k_ihelmp_(double2 *e, double *sln, const 
     }
     __syncthreads();
   }
-  // gbtrs, once per visit of the slice
-  for (int rep = 0; rep < cnt; ++rep) {
-    for (int j = 0; j < nn - 1; ++j) {
-      const int km = (P < nn - 1 - j) ? P : nn - 1 - j;
-      if (tid == 0) {
-        const int l = piv[j];
-        if (l != j) {
-          const double2 x = b[l];
-          b[l] = b[j];
-          b[j] = x;
+  // row j of the factors: U entries r = 0 .. 2P at r, L entries r = 2P + 1
+  // .. 3P at r + 1 (an even offset), row length 3P + 2
+  constexpr int ROW = 3 * P + 2;
+  for (int q = tid; q < LDAB * nn; q += blockDim.x) {
+    const int j = q / LDAB, r = q % LDAB;
+    ab[((size_t)j * nsys + sys) * ROW + (r <= KV ? r : r + 1)] = AB[q];
+  }
+  for (int j = tid; j < nn; j += blockDim.x) pv[(size_t)j * nsys + sys] = (int8_t)piv[j];
+}
+
+// The (0,0) column's log term before the solve (the reference's bl, bl2):
+// lnp = ln / alpha; b(:, 0, 0) -= leg_del2(0, ak_0)^(Q-1) bl2 + beta bl2,
+// with bl2 the log term's Laplacian rows; ln <- lnp.  One workgroup.
+template <int P>
+__global__ void __launch_bounds__(256) k_ihelmp_ln_(double2 *e, double *sln, const double *d2c, const double *ak,
+                                                    const double *lognorm, double alpha, double beta, int ne, int nn,
+                                                    double ell) {
+  constexpr int Q = P / 2;
+  extern __shared__ double hsm[];
+  double *v = hsm, *t = hsm + nn;
+  const int tid = threadIdx.x;
+  const double akk = ak[0] * ak[0];
+  auto dcoef = [&](int r, int d) { return d2c[(size_t)r * 5 + d] - (d == 2 ? akk : 0.0); };
+  const double lnp = sln[0] / alpha;
+  const double ell2 = ell * ell;
+  const double b0 = 4.0 / 3.0 / ell2 / exp(lognorm[0]) * lnp, b1 = -2.0 / ell2 / exp(lognorm[1]) * lnp,
+               b2 = 2.0 / 3.0 / ell2 / exp(lognorm[2]) * lnp;
+  for (int i = tid; i < nn; i += blockDim.x) v[i] = (i == 0) ? b0 : (i == 1) ? b1 : (i == 2) ? b2 : 0.0;
+  __syncthreads();
+  for (int st = 0; st < Q - 1; ++st) {
+    for (int i = tid; i < nn; i += blockDim.x) {
+      double a = 0.0;
+      for (int d = 0; d < 5; ++d) {
+        const int j = i + d - 2;
+        if (j >= 0 && j < nn) a += dcoef(i, d) * v[j];
+      }
+      t[i] = a;
+    }
+    __syncthreads();
+    for (int i = tid; i < nn; i += blockDim.x) v[i] = t[i];
+    __syncthreads();
+  }
+  for (int i = tid; i < nn; i += blockDim.x) {
+    const double bl2 = (i == 0) ? b0 : (i == 1) ? b1 : (i == 2) ? b2 : 0.0;
+    double2 x = e[i];
+    x.x = x.x - v[i] - beta * bl2;
+    e[i] = x;
+  }
+  __syncthreads();
+  if (tid == 0) sln[0] = lnp;
+}
+
+// gbtrs on the kept factors.  A 64-lane workgroup owns column m and 32
+// consecutive |kappa| values; lanes 2i and 2i + 1 solve slices kk and
+// nz - kk of system kk (they read the same factor entries).  Each lane runs
+// the two sweeps over its own column in registers, in the reference's order
+// of operations: the forward sweep keeps rows j .. j + P (interchange, then
+// the P eliminations of row j), the back substitution the 2P pending rows
+// above row j (x_j, then its 2P updates).  The factor entries of the rows
+// ahead are loaded RF (forward) and RB (back) rows early.  A slice visited
+// twice by the chop (z_count 2) is solved twice.
+template <int P>
+__global__ void __launch_bounds__(64) k_ihelmp_sol_(double2 *e, const double *__restrict__ ab,
+                                                    const int8_t *__restrict__ pv, int nrdim, int npdim, int nz,
+                                                    int nrc, int npc, int nkk, int nzc, int nzcu) {
+  constexpr int KV = 2 * P, ROW = 3 * P + 2, RF = 4, RB = 2;
+  const int m = blockIdx.y, lane = threadIdx.x;
+  const int nn = nrcs_dev(nrc, npc, m);  // uniform over the workgroup
+  if (nn <= 0) return;
+  const int kk0 = blockIdx.x * 32 + lane / 2, twin = lane & 1;
+  const bool have = kk0 < nkk;
+  const int kk = have ? kk0 : 0;
+  const int k = twin ? (nz - kk) % nz : kk;
+  const bool valid = have && (!twin || (kk != 0 && 2 * kk != nz));
+  const int cnt = valid ? z_count(k, nzc, nzcu) : 0;
+  const size_t nsys = (size_t)npc * nkk, sys = (size_t)m * nkk + kk;
+  double2 *col = e + (size_t)k * nrdim * npdim + (size_t)nrdim * m;
+  const double2 z2 = make_double2(0.0, 0.0);
+  // row j's factors: ab + (j nsys + sys) ROW (HypLu's layout)
+#define MLX_ROW(j) (ab + ((size_t)(j) * nsys + sys) * ROW)
+  // factor entries and the entering right-hand side of row j into ring slot u
+#define MLX_LOAD_F(j, u)                                              \
+  do {                                                                \
+    if ((j) < nn - 1) {                                               \
+      const double *rw = MLX_ROW(j) + KV + 2;                         \
+      _Pragma("unroll") for (int q = 0; q < P; ++q) L[u][q] = rw[q];  \
+      jp[u] = pv[(size_t)(j) * nsys + sys];                           \
+    }                                                                 \
+    nb[u] = z2;                                                       \
+    if ((j) + P + 1 < nn) nb[u] = col[(j) + P + 1];                   \
+  } while (0)
+#define MLX_LOAD_B(j, u)                                              \
+  do {                                                                \
+    if ((j) >= 0) {                                                   \
+      const double *rw = MLX_ROW(j);                                  \
+      _Pragma("unroll") for (int q = 0; q <= KV; ++q) U[u][q] = rw[KV - q]; \
+    }                                                                 \
+    yb[u] = z2;                                                       \
+    if ((j) - KV - 1 >= 0) yb[u] = col[(j) - KV - 1];                 \
+  } while (0)
+#pragma unroll 1
+  for (int rep = 0; rep < 2; ++rep) {
+    if (!__any(rep < cnt)) break;
+    const bool act = rep < cnt;
+    // forward sweep: F[q] = b(j + q)
+    {
+      double L[RF][P];
+      int jp[RF];
+      double2 nb[RF];  // b(j + P + 1) for the rows ahead
+      double2 F[P + 1];
+#pragma unroll
+      for (int q = 0; q <= P; ++q) {
+        F[q] = z2;
+        if (q < nn) F[q] = col[q];
+      }
+      MLX_LOAD_F(0, 0);
+      MLX_LOAD_F(1, 1);
+      MLX_LOAD_F(2, 2);
+      MLX_LOAD_F(3, 3);
+#pragma unroll 1
+      for (int j0 = 0; j0 < nn - 1; j0 += RF) {
+#pragma unroll
+        for (int u = 0; u < RF; ++u) {
+          const int j = j0 + u;
+          if (j < nn - 1) {
+            // interchange rows j and j + jp, as selects over the window: the
+            // empty asm hides that the mask has one bit, so the compiler keeps
+            // F in registers instead of indexing it through memory
+            unsigned pm = 1u << jp[u];
+            asm volatile("" : "+v"(pm));
+            // (component selects: a select of whole double2 values becomes
+            // a select of addresses, which also keeps F in memory)
+            double f0x = F[0].x, f0y = F[0].y;
+#pragma unroll
+            for (int q = 1; q <= P; ++q) {
+              const bool sw = (pm >> q) & 1u;
+              const double ax = F[q].x, ay = F[q].y;
+              F[q].x = sw ? F[0].x : ax;
+              F[q].y = sw ? F[0].y : ay;
+              f0x = sw ? ax : f0x;
+              f0y = sw ? ay : f0y;
+            }
+            F[0].x = f0x;
+            F[0].y = f0y;
+            const int km = (P < nn - 1 - j) ? P : nn - 1 - j;
+#pragma unroll
+            for (int q = 1; q <= P; ++q)
+              if (q <= km) {
+                F[q].x -= L[u][q - 1] * F[0].x;
+                F[q].y -= L[u][q - 1] * F[0].y;
+              }
+            if (act) col[j] = F[0];
+#pragma unroll
+            for (int q = 0; q < P; ++q) F[q] = F[q + 1];
+            F[P] = nb[u];
+            MLX_LOAD_F(j + RF, u);
+          }
         }
       }
-      __syncthreads();
-      const double2 bj = b[j];
-      const double *cj = AB + (size_t)j * LDAB;
-      for (int q = 1 + tid; q <= km; q += blockDim.x) {
-        b[j + q].x -= cj[KV + q] * bj.x;
-        b[j + q].y -= cj[KV + q] * bj.y;
-      }
-      __syncthreads();
+      if (act) col[nn - 1] = F[0];
     }
-    for (int j = nn - 1; j >= 0; --j) {
-      const double *cj = AB + (size_t)j * LDAB;
-      if (tid == 0) b[j] = make_double2(b[j].x / cj[KV], b[j].y / cj[KV]);
-      __syncthreads();
-      const double2 xj = b[j];
-      const int i0 = (j - KV > 0) ? j - KV : 0;
-      for (int i = i0 + tid; i < j; i += blockDim.x) {
-        const double a = cj[KV + i - j];
-        b[i].x -= a * xj.x;
-        b[i].y -= a * xj.y;
+    // back substitution: B[q] = b(j - q), pending
+    {
+      double U[RB][KV + 1];
+      double2 yb[RB];  // b(j - 2P - 1) for the rows ahead
+      double2 B[KV + 1];
+#pragma unroll
+      for (int q = 0; q <= KV; ++q) {
+        B[q] = z2;
+        if (nn - 1 - q >= 0) B[q] = col[nn - 1 - q];
       }
-      __syncthreads();
+      MLX_LOAD_B(nn - 1, 0);
+      MLX_LOAD_B(nn - 2, 1);
+#pragma unroll 1
+      for (int j0 = nn - 1; j0 >= 0; j0 -= RB) {
+#pragma unroll
+        for (int u = 0; u < RB; ++u) {
+          const int j = j0 - u;
+          if (j >= 0) {
+            const double d = U[u][0];
+            const double2 x = make_double2(B[0].x / d, B[0].y / d);
+            if (act) col[j] = x;
+#pragma unroll
+            for (int q = 1; q <= KV; ++q)
+              if (j - q >= 0) {
+                B[q].x -= U[u][q] * x.x;
+                B[q].y -= U[u][q] * x.y;
+              }
+#pragma unroll
+            for (int q = 0; q < KV; ++q) B[q] = B[q + 1];
+            B[KV] = yb[u];
+            MLX_LOAD_B(j - RB, u);
+          }
+        }
+      }
     }
   }
-  for (int i = tid; i < nn; i += blockDim.x) e[off + i] = b[i];
-  if (c00 && tid == 0) sln[0] = lnp;
+#undef MLX_LOAD_B
+#undef MLX_LOAD_F
+#undef MLX_ROW
 }
 
 // ---------------------------------------------------------------------------
@@ -1341,26 +1472,106 @@ void k_idel2_preln00(mlx_ctx *c, double2 *e, double *ln, double preln, int nroff
                      k.ell, work);
 }
 
+namespace {
+constexpr int kHypEntries = 4;
+
+void hyp_free(HypLu &h) {
+  if (h.ab) (void)hipFree(h.ab);
+  if (h.pv) (void)hipFree(h.pv);
+  h.ab = nullptr;
+  h.pv = nullptr;
+}
+
+// The factors for (power, alpha, beta, nroff): kept ones, else factored now
+// (a zero pivot throws, nothing is kept)
+HypLu &hyp_get(mlx_ctx *c, int power, double alpha, double beta, int nroff, int nrc, int *sing) {
+  for (HypLu &h : c->hyp)
+    if (h.power == power && h.alpha == alpha && h.beta == beta && h.nroff == nroff) {
+      h.stamp = ++c->lu_clock;
+      return h;
+    }
+  const KitDev &k = c->kit;
+  hipStreamCaptureStatus st = hipStreamCaptureStatusNone;
+  HIPCHK(hipStreamIsCapturing(c->stream, &st));
+  if (st != hipStreamCaptureStatusNone)
+    MLX_THROW(MLX_E_ARG, "ihelmp: first use of (power, alpha, beta) under stream capture (run one step before capturing)");
+  if ((int)c->hyp.size() >= kHypEntries) {  // evict the least recently used entry
+    size_t victim = 0;
+    for (size_t i = 1; i < c->hyp.size(); ++i)
+      if (c->hyp[i].stamp < c->hyp[victim].stamp) victim = i;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    hyp_free(c->hyp[victim]);
+    c->hyp.erase(c->hyp.begin() + victim);
+  }
+  HypLu h;
+  h.power = power;
+  h.alpha = alpha;
+  h.beta = beta;
+  h.nroff = nroff;
+  h.nkk = (k.nzdim > 1) ? k.nzdim / 2 + 1 : 1;
+  h.nsys = k.npchop * h.nkk;
+  const int LDAB = 3 * power + 1;
+  // factor rows of 3P + 2 doubles (HypLu's layout)
+  if (hipMalloc(&h.ab, sizeof(double) * (size_t)nrc * (LDAB + 1) * h.nsys) != hipSuccess ||
+      hipMalloc(&h.pv, (size_t)nrc * h.nsys) != hipSuccess) {
+    hyp_free(h);
+    MLX_THROW(MLX_E_ALLOC, "ihelmp: cannot allocate the banded factors");
+  }
+  const size_t lds = (((size_t)LDAB * nrc + 1) & ~(size_t)1) * sizeof(double) + (size_t)nrc * sizeof(int);
+  dim3 grid(k.npchop, h.nkk);
+#define MLX_IHF(P_)                                                                                           \
+  hipLaunchKernelGGL(k_ihelmp_fac_<P_>, grid, dim3(256), lds, c->stream, h.ab, h.pv, k.d2c, k.ak, alpha, beta, \
+                     k.ne, nrc, k.npchop, h.nkk, sing)
+  switch (power) {
+    case 4: MLX_IHF(4); break;
+    case 6: MLX_IHF(6); break;
+    default: MLX_IHF(8); break;
+  }
+#undef MLX_IHF
+  HIPCHK(hipGetLastError());
+  int hs = 0;
+  HIPCHK(hipMemcpyAsync(&hs, sing, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  if (hs) {
+    hyp_free(h);
+    MLX_THROW(MLX_E_ARG, "ihelmp: zero pivot in a banded system (singular); the field is left unsolved");
+  }
+  h.stamp = ++c->lu_clock;
+  c->hyp.push_back(h);
+  return c->hyp.back();
+}
+}  // namespace
+
+void hyp_cache_free(mlx_ctx *c) {
+  for (HypLu &h : c->hyp) hyp_free(h);
+  c->hyp.clear();
+}
+
 void k_ihelmp(mlx_ctx *c, double2 *e, double *sln, int power, double alpha, double beta, int nroff, int *sing) {
   const KitDev &k = c->kit;
   const int nrc = k.nrchop + nroff;
   // chop_index's check (mlegs_scalar_ops.f90:2053): the rows of a system lie
   // inside the field's column
   if (nrc > k.nrdim_ref || nrc > k.ne) MLX_THROW(MLX_E_ARG, "chop_index: chopping in r too large");
-  const size_t lds = (size_t)nrc * ((3 * power + 1) * sizeof(double) + sizeof(double2) + 2 * sizeof(double) +
-                                    sizeof(int)) + sizeof(double);
+  if (power != 4 && power != 6 && power != 8) MLX_THROW(MLX_E_ARG, "ihelmp: power must be 4, 6 or 8");
+  const size_t lds = (size_t)nrc * ((3 * power + 1) * sizeof(double) + sizeof(int)) + sizeof(double);
   if (lds > 65536) MLX_THROW(MLX_E_ARG, "ihelmp: the banded system of one column exceeds the workgroup's LDS");
-  dim3 grid(k.npchop, k.nzdim);
-#define MLX_IHP(P_)                                                                                             \
-  hipLaunchKernelGGL(k_ihelmp_<P_>, grid, dim3(256), lds, c->stream, e, sln, k.d2c, k.ak, k.lognorm, alpha, beta, \
-                     k.nrdim, k.npdim, k.ne, nrc, k.npchop, k.chopzl, k.chopzu, k.ell, sing)
+  if (nrc <= 0) return;
+  HypLu &h = hyp_get(c, power, alpha, beta, nroff, nrc, sing);
+  dim3 grid((h.nkk + 31) / 32, k.npchop);
+#define MLX_IHS(P_)                                                                                             \
+  do {                                                                                                          \
+    hipLaunchKernelGGL(k_ihelmp_ln_<P_>, dim3(1), dim3(256), 2 * sizeof(double) * nrc, c->stream, e, sln, k.d2c, \
+                       k.ak, k.lognorm, alpha, beta, k.ne, nrc, k.ell);                                        \
+    hipLaunchKernelGGL(k_ihelmp_sol_<P_>, grid, dim3(64), 0, c->stream, e, h.ab, h.pv, k.nrdim, k.npdim, k.nzdim, \
+                       nrc, k.npchop, h.nkk, k.chopzl, k.chopzu);                                              \
+  } while (0)
   switch (power) {
-    case 4: MLX_IHP(4); break;
-    case 6: MLX_IHP(6); break;
-    case 8: MLX_IHP(8); break;
-    default: MLX_THROW(MLX_E_ARG, "ihelmp: power must be 4, 6 or 8");
+    case 4: MLX_IHS(4); break;
+    case 6: MLX_IHS(6); break;
+    default: MLX_IHS(8); break;
   }
-#undef MLX_IHP
+#undef MLX_IHS
   HIPCHK(hipGetLastError());
 }
 
