@@ -215,13 +215,27 @@ void check_sing(mlx_ctx *c) {
   if (h) MLX_THROW(MLX_E_ARG, "ihelmp: zero pivot in a banded system (singular); the column is left unsolved");
 }
 
-void ihelmp(mlx_ctx *c, mlx_field *f, int power, double alpha, double beta) {
+// The modes de-aliasing zeroes (dealias, mlegs_scalar_ops.f90:305-352):
+// columns m >= pcut and kappa slices zlo..zhi.  When the advection term's
+// inputs are de-aliased and its result is de-aliased anyway, the MFMA path
+// transforms columns m < pcut only and the projection skips the zlo..zhi
+// slices (on = false: every column and slice, as without de-aliasing).
+struct DealiasLimits {
+  bool on = false;
+  int pcut = -1, zlo = 1, zhi = 0;
+};
+
+// dl (the time loop, which de-aliases f right after): the systems of the
+// modes de-aliasing zeroes are not solved
+void ihelmp(mlx_ctx *c, mlx_field *f, int power, double alpha, double beta, const DealiasLimits *dl = nullptr) {
   need_kit(c);
   check_power("ihelmp", power);
   to_fff(c, f);
   if (std::fabs(alpha) < 5e-14)
     MLX_THROW(MLX_E_ARG, "ihelmp: alpha equals to zero. Inversion of 0*identity is impossible");
-  k_ihelmp(c, f->e, f->sc, power, alpha, beta, f->off[0], sing_flag(c));
+  const bool lim = dl && dl->on;
+  k_ihelmp(c, f->e, f->sc, power, alpha, beta, f->off[0], sing_flag(c), lim ? dl->pcut : -1, lim ? dl->zlo : 1,
+           lim ? dl->zhi : 0);
 }
 
 // fftreat (mlegs_scalar_ops.f90:1002-1063): far-field smoothing.  delsqp
@@ -266,7 +280,9 @@ void check_step_inputs(mlx_ctx *c, std::initializer_list<const mlx_field *> fs) 
 // hypervisc del^p as g (del^p + b del2), g = hypervisc (-1)^(p/2+1),
 // b = visc / g, inverted by ihelmp with alpha = -1/(dt g) (febe) or
 // -2/(dt g) (abcn, whose right-hand side adds dt/2 g helmp(s, p, 0, b))
-void step_hyper(mlx_ctx *c, mlx_field *s, const mlx_field *nl, const mlx_field *nl_p, double dt) {
+// dl: as abcn's (the time loop de-aliases s right after)
+void step_hyper(mlx_ctx *c, mlx_field *s, const mlx_field *nl, const mlx_field *nl_p, double dt,
+                const DealiasLimits *dl = nullptr) {
   const int pw = c->p.hyperpow;
   const double g = c->p.hypervisc * (((pw / 2 + 1) % 2) ? -1.0 : 1.0);
   const double b = c->p.visc / g;
@@ -276,7 +292,7 @@ void step_hyper(mlx_ctx *c, mlx_field *s, const mlx_field *nl, const mlx_field *
     helmp(c, sv, pw, 0.0, b);
     const double a = -2.0 / (dt * g);
     k_abcn_rhs_g(c, s->e, s->sc, s->e, s->sc, nl->e, nl->sc, nl_p->e, nl_p->sc, sv->e, dt, a, 1, g);
-    ihelmp(c, s, pw, a, b);
+    ihelmp(c, s, pw, a, b, dl);
   } else {
     const double a = -1.0 / (dt * g);
     k_abcn_rhs_g(c, s->e, s->sc, s->e, s->sc, nl->e, nl->sc, nullptr, nullptr, nullptr, dt, a, 0, g);
@@ -359,15 +375,6 @@ void febe(mlx_ctx *c, mlx_field *s, const mlx_field *nl, double dt) {
   ihelm(c, s, a);
 }
 
-// The modes de-aliasing zeroes (dealias, mlegs_scalar_ops.f90:305-352):
-// columns m >= pcut and kappa slices zlo..zhi.  When the advection term's
-// inputs are de-aliased and its result is de-aliased anyway, the MFMA path
-// transforms columns m < pcut only and the projection skips the zlo..zhi
-// slices (on = false: every column and slice, as without de-aliasing).
-struct DealiasLimits {
-  bool on = false;
-  int pcut = -1, zlo = 1, zhi = 0;
-};
 DealiasLimits dealias_limits(mlx_ctx *c) {
   const KitDev &k = c->kit;
   DealiasLimits d;
@@ -396,7 +403,7 @@ void abcn(mlx_ctx *c, mlx_field *s, mlx_field *s_p, mlx_field *nl, mlx_field *nl
           bool swap_nl, const DealiasLimits *dl = nullptr) {
   check_step_inputs(c, {s, nl, s_p, nl_p});
   if (c->p.hyperpow != 0) {
-    step_hyper(c, s, nl, nl_p, dt);
+    step_hyper(c, s, nl, nl_p, dt, dl);
     if (copy_prev) fcopy(c, s_p, s);
     if (swap_nl) fmove(nl_p, nl);
     else fcopy(c, nl_p, nl);

@@ -115,6 +115,7 @@ struct HypLu {
   int nkk = 0, nsys = 0;
   double *ab = nullptr;
   int8_t *pv = nullptr;
+  double2 *sink = nullptr;  // the solve's work rows for lanes and rows outside any system
   uint64_t stamp = 0;
 };
 
@@ -363,10 +364,13 @@ void k_ln_column(mlx_ctx *c, double2 *e, const double *ln, double sign);
 void k_abcn_rhs_g(mlx_ctx *c, double2 *sh, double *shln, const double2 *s, const double *sln, const double2 *nl,
                   const double *nlln, const double2 *nlp, const double *nlpln, const double2 *svis, double dt,
                   double a, int ab2, double g);
-// ihelmp's banded solves (power 4, 6 or 8) and its log-term step, FP64; a
-// column whose factorization meets a zero pivot is left unsolved and sets
-// *sing (device int) to 1
-void k_ihelmp(mlx_ctx *c, double2 *e, double *sln, int power, double alpha, double beta, int nroff, int *sing);
+// ihelmp's banded solves (power 4, 6 or 8) and its log-term step, FP64, on
+// factors kept per (power, alpha, beta, nroff); a factorization that meets a
+// zero pivot sets *sing (device int) to 1 and throws before anything is solved
+// pcut > 0: only columns m < pcut are solved; slices zlo..zhi are not solved
+// (their right-hand sides are zero or are zeroed afterwards)
+void k_ihelmp(mlx_ctx *c, double2 *e, double *sln, int power, double alpha, double beta, int nroff, int *sing,
+              int pcut = -1, int zlo = 1, int zhi = 0);
 // fftreat's physical-space step on the PFF lines of columns m < npc
 void k_fftreat_mid(mlx_ctx *c, double2 *e, int npc);
 void k_chop(mlx_ctx *c, double2 *e, const char *space, int nroff);

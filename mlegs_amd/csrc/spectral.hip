@@ -1042,12 +1042,19 @@ __global__ void __launch_bounds__(256) k_ihelmp_ln_(double2 *e, double *sln, con
 // of operations: the forward sweep keeps rows j .. j + P (interchange, then
 // the P eliminations of row j), the back substitution the 2P pending rows
 // above row j (x_j, then its 2P updates).  The factor entries of the rows
-// ahead are loaded RF (forward) and RB (back) rows early.  A slice visited
-// twice by the chop (z_count 2) is solved twice.
+// ahead are loaded RF (forward) and RB (back) rows early.  The loops have no
+// branches around loads or stores, so the loads stay in flight across rows:
+// rows past the system's end run with zero multipliers and no interchange
+// and store into `sink`, band entries outside the matrix are stored zeros
+// (their updates leave values unchanged), and a lane with nothing to solve
+// (its slice chopped, the twin of a self-conjugate slice, past the last
+// |kappa|) works in `sink` (garbage in, garbage out, never read as data).
+// A slice visited twice by the chop (z_count 2) is solved twice.
 template <int P>
 __global__ void __launch_bounds__(64) k_ihelmp_sol_(double2 *e, const double *__restrict__ ab,
-                                                    const int8_t *__restrict__ pv, int nrdim, int npdim, int nz,
-                                                    int nrc, int npc, int nkk, int nzc, int nzcu) {
+                                                    const int8_t *__restrict__ pv, double2 *sink, int nrdim,
+                                                    int npdim, int nz, int nrc, int npc, int nkk, int nzc, int nzcu,
+                                                    int zlo, int zhi) {
   constexpr int KV = 2 * P, ROW = 3 * P + 2, RF = 4, RB = 2;
   const int m = blockIdx.y, lane = threadIdx.x;
   const int nn = nrcs_dev(nrc, npc, m);  // uniform over the workgroup
@@ -1057,37 +1064,48 @@ __global__ void __launch_bounds__(64) k_ihelmp_sol_(double2 *e, const double *__
   const int kk = have ? kk0 : 0;
   const int k = twin ? (nz - kk) % nz : kk;
   const bool valid = have && (!twin || (kk != 0 && 2 * kk != nz));
-  const int cnt = valid ? z_count(k, nzc, nzcu) : 0;
+  // slices zlo..zhi are left as they are
+  const int cnt = (valid && !(k >= zlo && k <= zhi)) ? z_count(k, nzc, nzcu) : 0;
   const size_t nsys = (size_t)npc * nkk, sys = (size_t)m * nkk + kk;
-  double2 *col = e + (size_t)k * nrdim * npdim + (size_t)nrdim * m;
+  double2 *const colv = e + (size_t)k * nrdim * npdim + (size_t)nrdim * m;
+  double2 *const snk = sink + (size_t)lane * (nrc + KV + RF);  // rows j < nrc + RF written, -KV-1.. read
   const double2 z2 = make_double2(0.0, 0.0);
-  // row j's factors: ab + (j nsys + sys) ROW (HypLu's layout)
-#define MLX_ROW(j) (ab + ((size_t)(j) * nsys + sys) * ROW)
-  // factor entries and the entering right-hand side of row j into ring slot u
-#define MLX_LOAD_F(j, u)                                              \
-  do {                                                                \
-    if ((j) < nn - 1) {                                               \
-      const double *rw = MLX_ROW(j) + KV + 2;                         \
-      _Pragma("unroll") for (int q = 0; q < P; ++q) L[u][q] = rw[q];  \
-      jp[u] = pv[(size_t)(j) * nsys + sys];                           \
-    }                                                                 \
-    nb[u] = z2;                                                       \
-    if ((j) + P + 1 < nn) nb[u] = col[(j) + P + 1];                   \
+  // row j's factors (clamped to the system's rows): ab + (j nsys + sys) ROW
+#define MLX_ROW(j) (ab + ((size_t)((j) < 0 ? 0 : ((j) > nn - 1 ? nn - 1 : (j))) * nsys + sys) * ROW)
+  // a right-hand-side entry (zero outside 0 .. nn - 1), read without a branch
+  // (component selects: a select of whole double2 values becomes a select of
+  // addresses)
+#define MLX_IDX(i) (((i) >= 0 && (i) < nn) ? (i) : 0)
+#define MLX_IN(i) ((i) >= 0 && (i) < nn)
+#define MLX_ZSEL(dst, in, v)      \
+  do {                            \
+    const bool in_ = (in);        \
+    dst.x = in_ ? (v).x : 0.0;    \
+    dst.y = in_ ? (v).y : 0.0;    \
   } while (0)
-#define MLX_LOAD_B(j, u)                                              \
-  do {                                                                \
-    if ((j) >= 0) {                                                   \
-      const double *rw = MLX_ROW(j);                                  \
-      _Pragma("unroll") for (int q = 0; q <= KV; ++q) U[u][q] = rw[KV - q]; \
-    }                                                                 \
-    yb[u] = z2;                                                       \
-    if ((j) - KV - 1 >= 0) yb[u] = col[(j) - KV - 1];                 \
+  // ring slot u: the raw entries of row j (loads only: the row-validity
+  // selects are applied where the values are used, so no instruction waits
+  // on a load before its row comes up)
+#define MLX_LOAD_F(j, u)                                                            \
+  do {                                                                              \
+    const double *rw = MLX_ROW(j) + KV + 2;                                         \
+    _Pragma("unroll") for (int q = 0; q < P; ++q) L[u][q] = rw[q];                  \
+    jp[u] = pv[(size_t)((j) > nn - 1 ? nn - 1 : (j)) * nsys + sys];                 \
+    nb[u] = col[MLX_IDX((j) + P + 1)];                                              \
+  } while (0)
+#define MLX_LOAD_B(j, u)                                                            \
+  do {                                                                              \
+    const double *rw = MLX_ROW(j);                                                  \
+    _Pragma("unroll") for (int q = 0; q <= KV; ++q) U[u][q] = rw[KV - q];           \
+    yb[u] = col[MLX_IDX((j) - KV - 1)];                                             \
   } while (0)
 #pragma unroll 1
   for (int rep = 0; rep < 2; ++rep) {
     if (!__any(rep < cnt)) break;
-    const bool act = rep < cnt;
-    // forward sweep: F[q] = b(j + q)
+    // the lane's working column: its slice, or the sink
+    double2 *const col = (rep < cnt) ? colv : snk + KV + 1;
+    // forward sweep over rows 0 .. nn - 1 (and padding to a multiple of RF):
+    // F[q] = b(j + q); row j stores y_j
     {
       double L[RF][P];
       int jp[RF];
@@ -1095,93 +1113,95 @@ __global__ void __launch_bounds__(64) k_ihelmp_sol_(double2 *e, const double *__
       double2 F[P + 1];
 #pragma unroll
       for (int q = 0; q <= P; ++q) {
-        F[q] = z2;
-        if (q < nn) F[q] = col[q];
+        const double2 v = col[MLX_IDX(q)];
+        MLX_ZSEL(F[q], MLX_IN(q), v);
       }
-      MLX_LOAD_F(0, 0);
-      MLX_LOAD_F(1, 1);
-      MLX_LOAD_F(2, 2);
-      MLX_LOAD_F(3, 3);
+#pragma unroll
+      for (int u = 0; u < RF; ++u) MLX_LOAD_F(u, u);
 #pragma unroll 1
-      for (int j0 = 0; j0 < nn - 1; j0 += RF) {
+      for (int j0 = 0; j0 < nn; j0 += RF) {
 #pragma unroll
         for (int u = 0; u < RF; ++u) {
           const int j = j0 + u;
-          if (j < nn - 1) {
-            // interchange rows j and j + jp, as selects over the window: the
-            // empty asm hides that the mask has one bit, so the compiler keeps
-            // F in registers instead of indexing it through memory
-            unsigned pm = 1u << jp[u];
-            asm volatile("" : "+v"(pm));
-            // (component selects: a select of whole double2 values becomes
-            // a select of addresses, which also keeps F in memory)
-            double f0x = F[0].x, f0y = F[0].y;
+          // interchange rows j and j + jp, as selects over the window: the
+          // empty asm hides that the mask has one bit, so the compiler keeps
+          // F in registers instead of indexing it through memory; component
+          // selects, since a select of whole double2 values becomes a select
+          // of addresses
+          const bool el = j < nn - 1;  // row j has eliminations
+          unsigned pm = 1u << (el ? jp[u] : 0);
+          asm volatile("" : "+v"(pm));
+          double f0x = F[0].x, f0y = F[0].y;
 #pragma unroll
-            for (int q = 1; q <= P; ++q) {
-              const bool sw = (pm >> q) & 1u;
-              const double ax = F[q].x, ay = F[q].y;
-              F[q].x = sw ? F[0].x : ax;
-              F[q].y = sw ? F[0].y : ay;
-              f0x = sw ? ax : f0x;
-              f0y = sw ? ay : f0y;
-            }
-            F[0].x = f0x;
-            F[0].y = f0y;
-            const int km = (P < nn - 1 - j) ? P : nn - 1 - j;
-#pragma unroll
-            for (int q = 1; q <= P; ++q)
-              if (q <= km) {
-                F[q].x -= L[u][q - 1] * F[0].x;
-                F[q].y -= L[u][q - 1] * F[0].y;
-              }
-            if (act) col[j] = F[0];
-#pragma unroll
-            for (int q = 0; q < P; ++q) F[q] = F[q + 1];
-            F[P] = nb[u];
-            MLX_LOAD_F(j + RF, u);
+          for (int q = 1; q <= P; ++q) {
+            const bool sw = (pm >> q) & 1u;
+            const double ax = F[q].x, ay = F[q].y;
+            F[q].x = sw ? F[0].x : ax;
+            F[q].y = sw ? F[0].y : ay;
+            f0x = sw ? ax : f0x;
+            f0y = sw ? ay : f0y;
           }
+          F[0].x = f0x;
+          F[0].y = f0y;
+          // multipliers of rows below nn - 1 are stored zeros
+#pragma unroll
+          for (int q = 1; q <= P; ++q) {
+            const double l = el ? L[u][q - 1] : 0.0;
+            F[q].x -= l * F[0].x;
+            F[q].y -= l * F[0].y;
+          }
+          double2 *const o = (j < nn) ? col + j : snk + j;
+          *o = F[0];
+#pragma unroll
+          for (int q = 0; q < P; ++q) F[q] = F[q + 1];
+          MLX_ZSEL(F[P], MLX_IN(j + P + 1), nb[u]);
+          MLX_LOAD_F(j + RF, u);
         }
       }
-      if (act) col[nn - 1] = F[0];
     }
-    // back substitution: B[q] = b(j - q), pending
+    // back substitution over rows nn - 1 .. 0 (and padding): B[q] = b(j - q),
+    // pending; row j stores x_j
     {
       double U[RB][KV + 1];
       double2 yb[RB];  // b(j - 2P - 1) for the rows ahead
       double2 B[KV + 1];
 #pragma unroll
       for (int q = 0; q <= KV; ++q) {
-        B[q] = z2;
-        if (nn - 1 - q >= 0) B[q] = col[nn - 1 - q];
+        const double2 v = col[MLX_IDX(nn - 1 - q)];
+        MLX_ZSEL(B[q], MLX_IN(nn - 1 - q), v);
       }
-      MLX_LOAD_B(nn - 1, 0);
-      MLX_LOAD_B(nn - 2, 1);
+#pragma unroll
+      for (int u = 0; u < RB; ++u) MLX_LOAD_B(nn - 1 - u, u);
 #pragma unroll 1
       for (int j0 = nn - 1; j0 >= 0; j0 -= RB) {
 #pragma unroll
         for (int u = 0; u < RB; ++u) {
           const int j = j0 - u;
-          if (j >= 0) {
-            const double d = U[u][0];
-            const double2 x = make_double2(B[0].x / d, B[0].y / d);
-            if (act) col[j] = x;
+          const bool el = j >= 0;  // a row of the system
+          const double d = el ? U[u][0] : 1.0;
+          const double2 x = make_double2(B[0].x / d, B[0].y / d);
+          double2 *const o = (j >= 0) ? col + j : snk + (j + KV + 1);
+          *o = x;
+          // U(j - q, j) of rows above the first is a stored zero
 #pragma unroll
-            for (int q = 1; q <= KV; ++q)
-              if (j - q >= 0) {
-                B[q].x -= U[u][q] * x.x;
-                B[q].y -= U[u][q] * x.y;
-              }
-#pragma unroll
-            for (int q = 0; q < KV; ++q) B[q] = B[q + 1];
-            B[KV] = yb[u];
-            MLX_LOAD_B(j - RB, u);
+          for (int q = 1; q <= KV; ++q) {
+            const double a = el ? U[u][q] : 0.0;
+            B[q].x -= a * x.x;
+            B[q].y -= a * x.y;
           }
+#pragma unroll
+          for (int q = 0; q < KV; ++q) B[q] = B[q + 1];
+          MLX_ZSEL(B[KV], MLX_IN(j - KV - 1), yb[u]);
+          MLX_LOAD_B(j - RB, u);
         }
       }
     }
   }
 #undef MLX_LOAD_B
 #undef MLX_LOAD_F
+#undef MLX_ZSEL
+#undef MLX_IN
+#undef MLX_IDX
 #undef MLX_ROW
 }
 
@@ -1478,9 +1498,14 @@ constexpr int kHypEntries = 4;
 void hyp_free(HypLu &h) {
   if (h.ab) (void)hipFree(h.ab);
   if (h.pv) (void)hipFree(h.pv);
+  if (h.sink) (void)hipFree(h.sink);
   h.ab = nullptr;
   h.pv = nullptr;
+  h.sink = nullptr;
 }
+
+// rows of the solve's sink per lane (k_ihelmp_sol_: nrc + 2P + 4)
+size_t hyp_sink_rows(int nrc, int power) { return (size_t)nrc + 2 * power + 4; }
 
 // The factors for (power, alpha, beta, nroff): kept ones, else factored now
 // (a zero pivot throws, nothing is kept)
@@ -1513,7 +1538,8 @@ HypLu &hyp_get(mlx_ctx *c, int power, double alpha, double beta, int nroff, int 
   const int LDAB = 3 * power + 1;
   // factor rows of 3P + 2 doubles (HypLu's layout)
   if (hipMalloc(&h.ab, sizeof(double) * (size_t)nrc * (LDAB + 1) * h.nsys) != hipSuccess ||
-      hipMalloc(&h.pv, (size_t)nrc * h.nsys) != hipSuccess) {
+      hipMalloc(&h.pv, (size_t)nrc * h.nsys) != hipSuccess ||
+      hipMalloc(&h.sink, sizeof(double2) * 64 * hyp_sink_rows(nrc, power)) != hipSuccess) {
     hyp_free(h);
     MLX_THROW(MLX_E_ALLOC, "ihelmp: cannot allocate the banded factors");
   }
@@ -1547,7 +1573,8 @@ void hyp_cache_free(mlx_ctx *c) {
   c->hyp.clear();
 }
 
-void k_ihelmp(mlx_ctx *c, double2 *e, double *sln, int power, double alpha, double beta, int nroff, int *sing) {
+void k_ihelmp(mlx_ctx *c, double2 *e, double *sln, int power, double alpha, double beta, int nroff, int *sing,
+              int pcut, int zlo, int zhi) {
   const KitDev &k = c->kit;
   const int nrc = k.nrchop + nroff;
   // chop_index's check (mlegs_scalar_ops.f90:2053): the rows of a system lie
@@ -1558,13 +1585,15 @@ void k_ihelmp(mlx_ctx *c, double2 *e, double *sln, int power, double alpha, doub
   if (lds > 65536) MLX_THROW(MLX_E_ARG, "ihelmp: the banded system of one column exceeds the workgroup's LDS");
   if (nrc <= 0) return;
   HypLu &h = hyp_get(c, power, alpha, beta, nroff, nrc, sing);
-  dim3 grid((h.nkk + 31) / 32, k.npchop);
+  const int npc_grid = (pcut > 0 && pcut < k.npchop) ? pcut : k.npchop;
+  dim3 grid((h.nkk + 31) / 32, npc_grid);
 #define MLX_IHS(P_)                                                                                             \
   do {                                                                                                          \
     hipLaunchKernelGGL(k_ihelmp_ln_<P_>, dim3(1), dim3(256), 2 * sizeof(double) * nrc, c->stream, e, sln, k.d2c, \
                        k.ak, k.lognorm, alpha, beta, k.ne, nrc, k.ell);                                        \
-    hipLaunchKernelGGL(k_ihelmp_sol_<P_>, grid, dim3(64), 0, c->stream, e, h.ab, h.pv, k.nrdim, k.npdim, k.nzdim, \
-                       nrc, k.npchop, h.nkk, k.chopzl, k.chopzu);                                              \
+    hipLaunchKernelGGL(k_ihelmp_sol_<P_>, grid, dim3(64), 0, c->stream, e, h.ab, h.pv, h.sink, k.nrdim, k.npdim, \
+                       k.nzdim, \
+                       nrc, k.npchop, h.nkk, k.chopzl, k.chopzu, zlo, zhi);                                    \
   } while (0)
   switch (power) {
     case 4: MLX_IHS(4); break;
