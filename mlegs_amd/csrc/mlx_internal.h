@@ -102,12 +102,15 @@ struct LuCache {
 
 // Factors of ihelmp's banded systems (del2^(P/2) + beta del2 + alpha, band
 // kl = ku = P, partial pivoting) for one (P, alpha, beta, radial offset):
-// one system per (m, |kappa|), sys = m * nkk + kk; row j of a system's
-// LAPACK band storage (LDAB = 3P + 1 entries: U's diagonal and 2P
+// one system per (m, |kappa|), in blocks of 32 consecutive |kappa| values
+// kk of one column m (block b = m * ceil(nkk / 32) + kk / 32 of nblk, place
+// i = kk % 32; nsys counts the blocks' slots); entry r of row j of a
+// system's LAPACK band storage (LDAB = 3P + 1 entries: U's diagonal and 2P
 // superdiagonals at r = 2P - q, L's P multipliers at r = 2P + 1 ..) at
-// ab + (j * nsys + sys) * (3P + 2), entry r at r (r <= 2P) or r + 1 (the L
-// part starts 16-byte aligned); the row interchange of row j as the offset
-// to the pivot row at pv[j * nsys + sys].
+// ab[((j * nblk + b) * (3P + 2) + r') * 32 + i], r' = r (r <= 2P) or r + 1:
+// the 32 systems of a block are adjacent for every entry; the row
+// interchange of row j as the offset to the pivot row at
+// pv[(j * nblk + b) * 32 + i].
 struct HypLu {
   int power = 0;
   double alpha = 0.0, beta = 0.0;

@@ -892,7 +892,9 @@ __global__ void __launch_bounds__(256) k_ihelmp_fac_(double *__restrict__ ab, in
   const int m = blockIdx.x, kk = blockIdx.y, tid = threadIdx.x;
   const int nn = nrcs_dev(nrc, npc, m);
   if (nn <= 0) return;
-  const size_t nsys = (size_t)npc * nkk, sys = (size_t)m * nkk + kk;
+  // HypLu's block of 32 systems and the system's place in it
+  const size_t nblk = (size_t)npc * ((nkk + 31) / 32), blk = (size_t)m * ((nkk + 31) / 32) + kk / 32;
+  const int ib = kk % 32;
   double *AB = hsm;                                                             // LDAB * nn
   int *piv = reinterpret_cast<int *>(AB + (((size_t)LDAB * nn + 1) & ~(size_t)1));  // nn
   __shared__ int s_jp, s_ju, s_sing;
@@ -983,14 +985,13 @@ __global__ void __launch_bounds__(256) k_ihelmp_fac_(double *__restrict__ ab, in
     }
     __syncthreads();
   }
-  // row j of the factors: U entries r = 0 .. 2P at r, L entries r = 2P + 1
-  // .. 3P at r + 1 (an even offset), row length 3P + 2
+  // HypLu's layout: entry r of row j (U's r = 0 .. 2P at r, L's at r + 1)
   constexpr int ROW = 3 * P + 2;
   for (int q = tid; q < LDAB * nn; q += blockDim.x) {
     const int j = q / LDAB, r = q % LDAB;
-    ab[((size_t)j * nsys + sys) * ROW + (r <= KV ? r : r + 1)] = AB[q];
+    ab[(((size_t)j * nblk + blk) * ROW + (r <= KV ? r : r + 1)) * 32 + ib] = AB[q];
   }
-  for (int j = tid; j < nn; j += blockDim.x) pv[(size_t)j * nsys + sys] = (int8_t)piv[j];
+  for (int j = tid; j < nn; j += blockDim.x) pv[((size_t)j * nblk + blk) * 32 + ib] = (int8_t)piv[j];
 }
 
 // The (0,0) column's log term before the solve (the reference's bl, bl2):
@@ -1066,12 +1067,15 @@ __global__ void __launch_bounds__(64) k_ihelmp_sol_(double2 *e, const double *__
   const bool valid = have && (!twin || (kk != 0 && 2 * kk != nz));
   // slices zlo..zhi are left as they are
   const int cnt = (valid && !(k >= zlo && k <= zhi)) ? z_count(k, nzc, nzcu) : 0;
-  const size_t nsys = (size_t)npc * nkk, sys = (size_t)m * nkk + kk;
+  // HypLu's block of the workgroup's 32 systems, the lane's place in it
+  const size_t nblk = (size_t)npc * ((nkk + 31) / 32), blk = (size_t)m * ((nkk + 31) / 32) + blockIdx.x;
+  const int ib = lane / 2;
   double2 *const colv = e + (size_t)k * nrdim * npdim + (size_t)nrdim * m;
   double2 *const snk = sink + (size_t)lane * (nrc + KV + RF);  // rows j < nrc + RF written, -KV-1.. read
   const double2 z2 = make_double2(0.0, 0.0);
-  // row j's factors (clamped to the system's rows): ab + (j nsys + sys) ROW
-#define MLX_ROW(j) (ab + ((size_t)((j) < 0 ? 0 : ((j) > nn - 1 ? nn - 1 : (j))) * nsys + sys) * ROW)
+  // row j's factors (clamped to the system's rows): entry r at rw[32 r]
+#define MLX_JC(j) ((j) < 0 ? 0 : ((j) > nn - 1 ? nn - 1 : (j)))
+#define MLX_ROW(j) (ab + ((size_t)MLX_JC(j) * nblk + blk) * ROW * 32 + ib)
   // a right-hand-side entry (zero outside 0 .. nn - 1), read without a branch
   // (component selects: a select of whole double2 values becomes a select of
   // addresses)
@@ -1088,15 +1092,15 @@ __global__ void __launch_bounds__(64) k_ihelmp_sol_(double2 *e, const double *__
   // on a load before its row comes up)
 #define MLX_LOAD_F(j, u)                                                            \
   do {                                                                              \
-    const double *rw = MLX_ROW(j) + KV + 2;                                         \
-    _Pragma("unroll") for (int q = 0; q < P; ++q) L[u][q] = rw[q];                  \
-    jp[u] = pv[(size_t)((j) > nn - 1 ? nn - 1 : (j)) * nsys + sys];                 \
+    const double *rw = MLX_ROW(j) + (KV + 2) * 32;                                  \
+    _Pragma("unroll") for (int q = 0; q < P; ++q) L[u][q] = rw[32 * q];             \
+    jp[u] = pv[((size_t)MLX_JC(j) * nblk + blk) * 32 + ib];                         \
     nb[u] = col[MLX_IDX((j) + P + 1)];                                              \
   } while (0)
 #define MLX_LOAD_B(j, u)                                                            \
   do {                                                                              \
     const double *rw = MLX_ROW(j);                                                  \
-    _Pragma("unroll") for (int q = 0; q <= KV; ++q) U[u][q] = rw[KV - q];           \
+    _Pragma("unroll") for (int q = 0; q <= KV; ++q) U[u][q] = rw[32 * (KV - q)];    \
     yb[u] = col[MLX_IDX((j) - KV - 1)];                                             \
   } while (0)
 #pragma unroll 1
@@ -1203,6 +1207,7 @@ __global__ void __launch_bounds__(64) k_ihelmp_sol_(double2 *e, const double *__
 #undef MLX_IN
 #undef MLX_IDX
 #undef MLX_ROW
+#undef MLX_JC
 }
 
 // ---------------------------------------------------------------------------
@@ -1534,7 +1539,7 @@ HypLu &hyp_get(mlx_ctx *c, int power, double alpha, double beta, int nroff, int 
   h.beta = beta;
   h.nroff = nroff;
   h.nkk = (k.nzdim > 1) ? k.nzdim / 2 + 1 : 1;
-  h.nsys = k.npchop * h.nkk;
+  h.nsys = k.npchop * ((h.nkk + 31) / 32) * 32;  // blocks of 32 systems
   const int LDAB = 3 * power + 1;
   // factor rows of 3P + 2 doubles (HypLu's layout)
   if (hipMalloc(&h.ab, sizeof(double) * (size_t)nrc * (LDAB + 1) * h.nsys) != hipSuccess ||
