@@ -57,6 +57,9 @@ Printed: one JSON line with
                    checkpoints every --adjoint-segment steps; reverse/forward
                    time ratio (adjoint_forward_ratio), central differences on
                    --adjoint-dirs directions, mixed vs FP64-forward gradient
+  hyperviscous_p8  the same 256^3 step with HYPERPOW = 8, HYPERVISC = 5e-7 (the
+                   reference's shipped setting): steps/s mixed and FP64 over
+                   --hyper steps, and the two runs' difference
 """
 import argparse
 import ctypes
@@ -264,8 +267,12 @@ def roofline(probes, name):
         with open(TRAFFIC_FILE) as f:
             t = json.load(f)
         entry = t.get('kernels', {}).get(fam)
+        # the counters hold for the launch they were taken on: the same
+        # family, precision and launch shape (its algorithmic bytes per launch)
         if entry and entry.get('precision') in (prec, 'any'):
-            traffic = entry.get('hbm_bytes_per_launch')
+            alg = entry.get('algorithmic_bytes_per_launch')
+            if alg and abs(alg - nbytes / n) <= 5e-3 * alg:
+                traffic = entry.get('hbm_bytes_per_launch')
     return {'bound': bound, 'kernel': fam, 'precision': prec, 'achieved': achieved, 'peak': peak, 'unit': unit,
             'frac': achieved / peak, 'traffic': traffic,
             'launches': n, 'avg_launch_us': 1e3 * ms / n,
