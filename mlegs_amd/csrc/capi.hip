@@ -571,39 +571,52 @@ void adj_no_offsets(const mlx_field *f, const char *what) {
 // FFF holds the cotangent of F(r^rmode y) and returns that of y, in PPP.
 // lim (optional, with the four-step phi kernels): the FFF side is de-aliased
 // (columns m >= pcut and slices zlo..zhi zero on input, or not needed on
-// output: to_fff leaves them unwritten)
-void trans_adjoint_nolog(mlx_ctx *c, mlx_field *g, bool to_fff, int rmode, const DealiasLimits *lim = nullptr) {
+// output: to_fff leaves them unwritten).  The nf (<= kZBatch) fields gs[q]
+// (row factors rmodes[q]) share one launch of the z transform.
+void trans_adjoint_nolog_n(mlx_ctx *c, mlx_field *const *gs, int nf, bool to_fff, const int *rmodes,
+                           const DealiasLimits *lim = nullptr) {
   const KitDev &k = c->kit;
   const int nc = lim ? lim->pcut : -1;
   const int zlo = lim ? lim->zlo : 1, zhi = lim ? lim->zhi : 0;
   const int ncz = lim ? lim->pcut : 0;
+  double2 *e[kZBatch];
+  auto z = [&](int dir) {
+    if (k.nz <= 1) return;
+    for (int q = 0; q < nf; ++q) e[q] = gs[q]->e;
+    if (lim) {
+      if (dir < 0) fft_z_batch(c, e, nf, -1, active_pairs(k, 0, nc), k.nrchop, ncz, 1, 0, zlo, zhi);
+      else fft_z_batch(c, e, nf, +1, active_pairs(k, 0, nc), k.nrchop, ncz, zlo, zhi);
+    } else {
+      fft_z_batch(c, e, nf, dir, (double)k.nrdim * k.npdim);
+    }
+  };
   if (to_fff) {
     // B^T = Z_b^T L_b^T Phi_b^T (L_b^T = L_f W^-1, Z_b^T = nz Z_f)
-    if (k.np > 1) fft_phi(c, g->e, g->e, -1);
-    k_adj_diag(c, g->e, -1, rmode, k.np > 1 ? +1 : 0, k.nz > 1 ? (double)k.nz : 1.0);
-    leg_forward(c, g->e, c->scr[0], 0, true, nc, !lim);
-    swap_scr(c, g);
-    if (k.nz > 1) {
-      if (lim)
-        fft_z(c, g->e, -1, active_pairs(k, 0, nc), k.nrchop, ncz, 1, 0, zlo, zhi);
-      else
-        fft_z(c, g->e, -1, (double)k.nrdim * k.npdim);
+    for (int q = 0; q < nf; ++q) {
+      mlx_field *g = gs[q];
+      if (k.np > 1) fft_phi(c, g->e, g->e, -1);
+      k_adj_diag(c, g->e, -1, rmodes[q], k.np > 1 ? +1 : 0, k.nz > 1 ? (double)k.nz : 1.0);
+      leg_forward(c, g->e, c->scr[0], 0, true, nc, !lim);
+      swap_scr(c, g);
     }
-    set_space(g, "FFF");
+    z(-1);
+    for (int q = 0; q < nf; ++q) set_space(gs[q], "FFF");
   } else {
     // F^T = Phi_f^T L_f^T Z_f^T (L_f^T = W L_b, Z_f^T = Z_b / nz)
-    if (k.nz > 1) {
-      if (lim)
-        fft_z(c, g->e, +1, active_pairs(k, 0, nc), k.nrchop, ncz, zlo, zhi);
-      else
-        fft_z(c, g->e, +1, (double)k.nrdim * k.npdim);
+    z(+1);
+    for (int q = 0; q < nf; ++q) {
+      mlx_field *g = gs[q];
+      leg_backward(c, g->e, c->scr[0], 0, true, nc, !lim);
+      swap_scr(c, g);
+      k_adj_diag(c, g->e, +1, rmodes[q], k.np > 1 ? -1 : 0, k.nz > 1 ? 1.0 / k.nz : 1.0);
+      if (k.np > 1) fft_phi(c, g->e, g->e, +1, 0, lim ? lim->pcut : -1);
+      set_space(g, "PPP");
     }
-    leg_backward(c, g->e, c->scr[0], 0, true, nc, !lim);
-    swap_scr(c, g);
-    k_adj_diag(c, g->e, +1, rmode, k.np > 1 ? -1 : 0, k.nz > 1 ? 1.0 / k.nz : 1.0);
-    if (k.np > 1) fft_phi(c, g->e, g->e, +1, 0, lim ? lim->pcut : -1);
-    set_space(g, "PPP");
   }
+}
+
+void trans_adjoint_nolog(mlx_ctx *c, mlx_field *g, bool to_fff, int rmode, const DealiasLimits *lim = nullptr) {
+  trans_adjoint_nolog_n(c, &g, 1, to_fff, &rmode, lim);
 }
 
 // g holds a cotangent of trans(x, space of g) for x in space `from` (FFF or
@@ -744,18 +757,24 @@ void clear_meta(mlx_ctx *c, mlx_field *f, const char *space, bool zero_ln = fals
 }
 
 // the transpose of y -> Z_f Phi_f (r^rmode y) (the phi and z transforms
-// without the radial one): g in PFF on entry, PPP on return
-void pff_adjoint(mlx_ctx *c, mlx_field *g, int rmode, const DealiasLimits *lim = nullptr) {
+// without the radial one): gs[q] (row factor rmodes[q]) in PFF on entry, PPP
+// on return; one launch of the z transform for the nf fields
+void pff_adjoint_n(mlx_ctx *c, mlx_field *const *gs, int nf, const int *rmodes, const DealiasLimits *lim = nullptr) {
   const KitDev &k = c->kit;
   if (k.nz > 1) {
+    double2 *e[kZBatch];
+    for (int q = 0; q < nf; ++q) e[q] = gs[q]->e;
     if (lim)
-      fft_z(c, g->e, +1, (double)k.nr * lim->pcut, 0, lim->pcut, lim->zlo, lim->zhi);
+      fft_z_batch(c, e, nf, +1, (double)k.nr * lim->pcut, 0, lim->pcut, lim->zlo, lim->zhi);
     else
-      fft_z(c, g->e, +1, (double)k.nrdim * k.npdim);
+      fft_z_batch(c, e, nf, +1, (double)k.nrdim * k.npdim);
   }
-  k_adj_diag(c, g->e, 0, rmode, k.np > 1 ? -1 : 0, k.nz > 1 ? 1.0 / k.nz : 1.0);
-  if (k.np > 1) fft_phi(c, g->e, g->e, +1, 0, lim ? lim->pcut : -1);
-  set_space(g, "PPP");
+  for (int q = 0; q < nf; ++q) {
+    mlx_field *g = gs[q];
+    k_adj_diag(c, g->e, 0, rmodes[q], k.np > 1 ? -1 : 0, k.nz > 1 ? 1.0 / k.nz : 1.0);
+    if (k.np > 1) fft_phi(c, g->e, g->e, +1, 0, lim ? lim->pcut : -1);
+    set_space(g, "PPP");
+  }
 }
 
 // tp2vec's transpose (vector_reconstruction, mlegs_scalar_ops.f90:1455-1545):
@@ -763,16 +782,25 @@ void pff_adjoint(mlx_ctx *c, mlx_field *g, int rmode, const DealiasLimits *lim =
 // chib: the cotangents of psi and chi (data and log coefficients) on return.
 // The chop of the intermediate fields restricts everything to rows n <
 // nrcs(m) of the kept slices.
+// tp2vec_adjoint_transforms: the transforms' part for one or two triples of
+// cotangents (vr = B(ur_s) / r, vp = B(up_s) / r, vz = B(uz_s); the log
+// terms of ur_s, up_s, uz_s are zero, so no cotangent of theirs is formed),
+// nf = 3 or 6 fields in one z launch
+void tp2vec_adjoint_transforms(mlx_ctx *c, mlx_field *const *fs, int nf, const DealiasLimits *lim) {
+  for (int q = 0; q < nf; ++q)
+    if (!is_ppp(fs[q])) MLX_THROW(MLX_E_LAYOUT, "vector_reconstruction adjoint: the cotangents must be in PPP");
+  const int rm[6] = {-1, -1, 0, -1, -1, 0};
+  trans_adjoint_nolog_n(c, fs, nf, true, rm, lim);
+}
+
+// pre: the cotangents have been through tp2vec_adjoint_transforms (FFF)
 void tp2vec_adjoint(mlx_ctx *c, mlx_field *urb, mlx_field *upb, mlx_field *uzb, mlx_field *psib, mlx_field *chib,
-                    const DealiasLimits *lim = nullptr) {
+                    const DealiasLimits *lim = nullptr, bool pre = false) {
   need_kit(c);
-  for (mlx_field *f : {urb, upb, uzb})
-    if (!is_ppp(f)) MLX_THROW(MLX_E_LAYOUT, "vector_reconstruction adjoint: the cotangents must be in PPP");
-  // vr = B(ur_s) / r, vp = B(up_s) / r, vz = B(uz_s); the log terms of ur_s,
-  // up_s, uz_s are zero, so no cotangent of theirs is formed
-  trans_adjoint_nolog(c, urb, true, -1, lim);
-  trans_adjoint_nolog(c, upb, true, -1, lim);
-  trans_adjoint_nolog(c, uzb, true, 0, lim);
+  if (!pre) {
+    mlx_field *const three[3] = {urb, upb, uzb};
+    tp2vec_adjoint_transforms(c, three, 3, lim);
+  }
   if (lim)
     k_tp2vec_t(c, psib->e, chib->e, urb->e, upb->e, uzb->e, psib->sc, chib->sc, lim->pcut, lim->zlo, lim->zhi);
   else
@@ -788,10 +816,10 @@ void tp2vec_adjoint(mlx_ctx *c, mlx_field *urb, mlx_field *upb, mlx_field *uzb, 
 
 // tp2curlvec's transpose (curl V = tp2vec(-del2 chi, psi), :1547-1560)
 void tp2curlvec_adjoint(mlx_ctx *c, mlx_field *wrb, mlx_field *wpb, mlx_field *wzb, mlx_field *psib,
-                        mlx_field *chib, const DealiasLimits *lim = nullptr) {
+                        mlx_field *chib, const DealiasLimits *lim = nullptr, bool pre = false) {
   mlx_field *md = tmp(c, 21);
   // md = -del2 chi has a zero log coefficient: its cotangent is dropped
-  tp2vec_adjoint(c, wrb, wpb, wzb, md, psib, lim);
+  tp2vec_adjoint(c, wrb, wpb, wzb, md, psib, lim, pre);
   k_del2_t(c, chib->e, md->e, true);
   k_adj_scale(c, chib->e, -1.0);
   k_set_scalar(c, chib->sc, 0.0);
@@ -837,8 +865,9 @@ void vec2tp_adjoint(mlx_ctx *c, mlx_field *psib, mlx_field *chib, mlx_field *vrb
   k_v2t_up_ln_t(c, vpb->e, upP->e);
   trans_adjoint_nolog(c, vpb, false, +1, lim);
   // ur = Z_f Phi_f (r vr), uz = Z_f Phi_f vz
-  pff_adjoint(c, vrb, +1, lim);
-  pff_adjoint(c, vzb, 0, lim);
+  mlx_field *const rz[2] = {vrb, vzb};
+  const int rzm[2] = {+1, 0};
+  pff_adjoint_n(c, rz, 2, rzm, lim);
   for (mlx_field *f : {vrb, vpb, vzb}) clear_meta(c, f, "PPP", true);
 }
 
@@ -901,9 +930,12 @@ void advection_adjoint(mlx_ctx *c, mlx_field *psi, mlx_field *chi, const mlx_fie
   set_space(br, "PPP");
   set_space(bp, "PPP");
   set_space(bz, "PPP");
-  tp2vec_adjoint(c, ar, ap, az, psib, chib, lim);
+  // the six cotangents' transforms, their z transforms in one launch
+  mlx_field *const six[6] = {ar, ap, az, br, bp, bz};
+  tp2vec_adjoint_transforms(c, six, 6, lim);
+  tp2vec_adjoint(c, ar, ap, az, psib, chib, lim, true);
   mlx_field *pb = tmp(c, 36), *cb = tmp(c, 37);
-  tp2curlvec_adjoint(c, br, bp, bz, pb, cb, lim);
+  tp2curlvec_adjoint(c, br, bp, bz, pb, cb, lim, true);
   k_axpby(c, psib->e, psib->sc, 1.0, pb->e, pb->sc, 1.0);
   k_axpby(c, chib->e, chib->sc, 1.0, cb->e, cb->sc, 1.0);
   dealias(c, psib);
