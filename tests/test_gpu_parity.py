@@ -160,6 +160,30 @@ def test_hyperviscous_ops(setup, power, alpha, beta):
     assert rel_err(back.e, s0.e) < 1e-8  # the oracle's own residual reaches 1e-9 on C1, power 8
 
 
+def test_ihelmp_kept_factors(setup):
+    """ihelmp keeps the factors of each (power, alpha, beta) (at most four
+    sets, least recently used evicted): six alpha values in turn, each
+    against the oracle; a repeated call (kept factors) and a call after
+    eviction (factored again) give the first call's result bit for bit."""
+    import mlegs_amd as M
+    from oracle import scalar_ops as so
+    name, p, tfm, okit = setup
+    _say('ihelmp kept factors', name)
+    e = C.spectral(tfm.dims, p, 29, admissible=True)
+    alphas = [1.0e6, 2.0e6, 3.0e6, 4.0e6, 5.0e6, 6.0e6]
+    first = {}
+    for a in alphas:
+        g, o = C.pair(tfm, okit, e, 'FFF', ln=0.11)
+        M.ihelmp(g, 8, a, -0.5, tfm)
+        so.ihelmp(o, 8, a, -0.5, okit)
+        assert rel_err(g.e, o.e) < 1e-6, a
+        first[a] = (g.e.copy(), g.ln)
+    for a in (alphas[-1], alphas[0]):  # kept; evicted and factored again
+        g = M.Scalar(tfm).upload(e, space='FFF', ln=0.11)
+        M.ihelmp(g, 8, a, -0.5, tfm)
+        assert np.array_equal(g.e, first[a][0]) and g.ln == first[a][1], a
+
+
 def test_hyperviscous_errors(setup):
     import mlegs_amd as M
     name, p, tfm, okit = setup
@@ -399,16 +423,19 @@ def test_advection_rhs_phi_sizes(cfg, prec):
     tfm.close()
 
 
-def test_vortex_steps_hyperviscous():
+@pytest.mark.parametrize('mfma', [False, True], ids=['fp64', 'fp64-mfma'])
+def test_vortex_steps_hyperviscous(mfma):
     """The vortex time loop (bootstrap + 3 AB2/CN steps) with HYPERPOW = 8,
     the shipped input.params' setting (input.params:15), against the
     oracle's loop.  The implicit solves' conditioning (see
-    test_steppers_hyperviscous) bounds the comparison at 1e-7."""
+    test_steppers_hyperviscous) bounds the comparison at 1e-7.  On FP64 MFMA
+    the loop runs with its de-aliasing restrictions, under which ihelmp
+    leaves the systems of the zeroed modes unsolved."""
     import mlegs_amd as M
     from oracle import kit as K, scalar_ops as so, vortex as V
     p = C.params(dict(C.SMALL, hyperpow=8, hypervisc=1e-7))
-    _say('vortex hyper', p.nr, p.np, p.nz)
-    tfm = M.Kit(p, device=0)
+    _say('vortex hyper', p.nr, p.np, p.nz, 'fp64-mfma' if mfma else 'fp64')
+    tfm = M.Kit(p, device=0, precision=M.PREC_FP64_MFMA if mfma else M.PREC_FP64)
     try:
         okit = K.Kit3D(p)
         (gpsi, opsi), (gchi, ochi) = C.qvortex(tfm, okit, p)
