@@ -485,7 +485,7 @@ __global__ void __launch_bounds__(256) k_abcn_rhs_d2_(double2 *e, double *sln, c
                                                       const double *ak, int nrdim, int npdim, int ne, int nrc,
                                                       int npc, int nzc, int nzcu, double ell, double dt, double a,
                                                       double visc, int pcut, int zlo, int zhi, const double *nlln,
-                                                      const double *nlpln) {
+                                                      const double *nlpln, int act) {
   extern __shared__ double2 lds[];
   double2 *S = lds, *V = lds + nrdim, *T = lds + 2 * nrdim;
   const int m = blockIdx.x, k = blockIdx.y;
@@ -495,7 +495,11 @@ __global__ void __launch_bounds__(256) k_abcn_rhs_d2_(double2 *e, double *sln, c
   const size_t off = (size_t)k * nrdim * npdim + (size_t)nrdim * m;
   const int cnt = z_count(k, nzc, nzcu);
   const int nn = (cnt > 0) ? nrcs_dev(nrc, npc, m) : 0;
-  for (int i = threadIdx.x; i < nrdim; i += blockDim.x) S[i] = e[off + i];
+  // act: s, N and N_p are chopped (rows >= nn zero), so the right-hand side
+  // is zero there too and only rows < nn are read and written (the band
+  // product itself reads rows < nn only)
+  const int nrow = act ? nn : nrdim;
+  for (int i = threadIdx.x; i < nrow; i += blockDim.x) S[i] = e[off + i];
   __syncthreads();
   if (nn > 0) {
     band_pow(V, S, T, nn, cnt, 0, m, ne, d2c, nullptr, ak[k] * ak[k]);
@@ -515,7 +519,7 @@ __global__ void __launch_bounds__(256) k_abcn_rhs_d2_(double2 *e, double *sln, c
     l = sh / a;
   }
   const double ell2 = ell * ell;
-  for (int i = threadIdx.x; i < nrdim; i += blockDim.x) {
+  for (int i = threadIdx.x; i < nrow; i += blockDim.x) {
     const double2 x = S[i], y = ld_nt(&nl[off + i]), z = ld_nt(&nlp[off + i]);
     const double2 v = (i < nn) ? V[i] : x;
     const double tx = x.x + dt * (1.5 * y.x - 0.5 * z.x);
@@ -1672,7 +1676,7 @@ void k_abcn_rhs_d2(mlx_ctx *c, double2 *e, double *ln, const double2 *nl, const 
   ProbeScope probe(c, MLX_PROBE_ABCN_RHS, 64.0 * nzk * active_pairs(k, nroff, pcut), 0.0);
   probed_launch(probe, k_abcn_rhs_d2_, grid, dim3(256), lds, c->stream, e, ln, nl, nlp, k.d2c, k.lognorm, k.ak,
                 k.nrdim, k.npdim, k.ne, nrc, k.npchop, k.chopzl, k.chopzu, k.ell, dt, a, c->p.visc, pcut, zlo, zhi,
-                helm_pre ? nlln : nullptr, helm_pre ? nlpln : nullptr);
+                helm_pre ? nlln : nullptr, helm_pre ? nlpln : nullptr, helm_pre ? 1 : 0);
   if (!helm_pre) hipLaunchKernelGGL(k_abcn_ln_, dim3(1), dim3(1), 0, c->stream, ln, ln, nlln, nlpln, dt, a);
   HIPCHK(hipGetLastError());
 }
