@@ -287,11 +287,11 @@ void step_hyper(mlx_ctx *c, mlx_field *s, const mlx_field *nl, const mlx_field *
   const double g = c->p.hypervisc * (((pw / 2 + 1) % 2) ? -1.0 : 1.0);
   const double b = c->p.visc / g;
   if (nl_p) {
-    mlx_field *sv = tmp(c, 14);
-    fcopy(c, sv, s);
-    helmp(c, sv, pw, 0.0, b);
+    // helmp(s, pw, 0, b) and the right-hand side in one column pass
     const double a = -2.0 / (dt * g);
-    k_abcn_rhs_g(c, s->e, s->sc, s->e, s->sc, nl->e, nl->sc, nl_p->e, nl_p->sc, sv->e, dt, a, 1, g);
+    const bool lim = dl && dl->on;
+    k_abcn_rhs_hyp(c, s->e, s->sc, nl->e, nl->sc, nl_p->e, nl_p->sc, dt, a, g, b, pw, s->off[0],
+                   lim ? dl->pcut : -1, lim ? dl->zlo : 1, lim ? dl->zhi : 0, lim);
     ihelmp(c, s, pw, a, b, dl);
   } else {
     const double a = -1.0 / (dt * g);

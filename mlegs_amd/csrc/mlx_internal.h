@@ -418,6 +418,12 @@ void k_abcn_rhs_d2(mlx_ctx *c, double2 *e, double *ln, const double2 *nl, const 
                    const double *nlpln, double dt, double a, int nroff, int pcut = -1,
                    int zlo = 1, int zhi = 0,
                    bool helm_pre = false);
+// abcn's hyperviscous right-hand side in place on e (= s), with svis =
+// helmp(s, power, 0, b) formed per column; act: s, N, N_p chopped (rows past
+// the system's rows zero), only the active rows read and written
+void k_abcn_rhs_hyp(mlx_ctx *c, double2 *e, double *ln, const double2 *nl, const double *nlln, const double2 *nlp,
+                    const double *nlpln, double dt, double a, double g, double b, int power, int nroff, int pcut,
+                    int zlo, int zhi, bool act);
 void k_scale_rows_r(mlx_ctx *c, double2 *e, int inverse);
 void k_tp2vec_combine(mlx_ctx *c, double2 *ur, double2 *up, const double2 *psi, const double2 *uz, int nroff);
 // mode 0: (P, C) = (psi, chi); mode 1: P = -del2(p_in = chi) over the rows of

@@ -539,6 +539,62 @@ __global__ void __launch_bounds__(256) k_abcn_rhs_d2_(double2 *e, double *sln, c
   }
 }
 
+// The hyperviscous AB2/CN right-hand side (abcn's HYPERPOW branch,
+// mlegs_scalar_ops.f90:1241-1255), one (m, kappa) column per workgroup:
+//   sh = a (s + dt (1.5 nl - 0.5 nl_p) + dt/2 g svis),
+//   svis = helmp(s, p, 0, b) = b del2(s) + del2^(p/2)(s)
+// with every del2 applied as often as the kk-loops visit the slice and the
+// first one taking the (0,0) log-term correction (helmp's bandmul steps);
+// rows at or past the system's rows keep s in both products, as bandmul
+// leaves them.  act: s, N and N_p are chopped, so only rows < nn are read
+// and written (the rest are zero on both sides).  The log coefficient is
+// k_abcn_ln_'s, launched after.
+__global__ void __launch_bounds__(256) k_abcn_rhs_hyp_(double2 *e, const double *sln, const double2 *nl,
+                                                       const double2 *nlp, const double *d2c, const double *lognorm,
+                                                       const double *ak, int nrdim, int npdim, int ne, int nrc,
+                                                       int npc, int nzc, int nzcu, double ell, double dt, double a,
+                                                       double g, double b, int half, int pcut, int zlo, int zhi,
+                                                       int act) {
+  extern __shared__ double2 lds[];
+  double2 *S = lds, *V = lds + nrdim, *W = lds + 2 * nrdim, *T = lds + 3 * nrdim;
+  const int m = blockIdx.x, k = blockIdx.y;
+  // columns m >= pcut and slices zlo..zhi: left as they are (the caller
+  // de-aliases the result)
+  if (m >= pcut || (k >= zlo && k <= zhi)) return;
+  const size_t off = (size_t)k * nrdim * npdim + (size_t)nrdim * m;
+  const int cnt = z_count(k, nzc, nzcu);
+  const int nn = (cnt > 0) ? nrcs_dev(nrc, npc, m) : 0;
+  const int nrow = act ? nn : nrdim;
+  for (int i = threadIdx.x; i < nrow; i += blockDim.x) S[i] = e[off + i];
+  __syncthreads();
+  if (nn > 0) {
+    const double akk = ak[k] * ak[k];
+    // V = del2(s) (the (0,0) log term after), W = del2^(half - 1)(V)
+    band_pow(V, S, T, nn, cnt, 0, m, ne, d2c, nullptr, akk);
+    if (m == 0 && k == 0 && threadIdx.x == 0) {
+      const double l = sln[0], ell2 = ell * ell;
+      V[0].x += 4.0 / 3.0 / ell2 / exp(lognorm[0]) * l;
+      if (nn > 1) V[1].x -= 2.0 / 1.0 / ell2 / exp(lognorm[1]) * l;
+      if (nn > 2) V[2].x += 2.0 / 3.0 / ell2 / exp(lognorm[2]) * l;
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < nn; i += blockDim.x) W[i] = V[i];
+    __syncthreads();
+    for (int st = 1; st < half; ++st) band_pow(W, W, T, nn, cnt, 0, m, ne, d2c, nullptr, akk);
+  }
+  const double hdt = dt / 2.0;
+  for (int i = threadIdx.x; i < nrow; i += blockDim.x) {
+    const double2 x = S[i], y = ld_nt(&nl[off + i]), z = ld_nt(&nlp[off + i]);
+    const double2 v = (i < nn) ? V[i] : x, w = (i < nn) ? W[i] : x;
+    // helmp's two axpby steps: 0 s + b del2(s), then + del2^(p/2)(s)
+    const double2 sv = make_double2((0.0 * x.x + b * v.x) + w.x, (0.0 * x.y + b * v.y) + w.y);
+    const double tx = x.x + dt * (1.5 * y.x - 0.5 * z.x);
+    const double ty = x.y + dt * (1.5 * y.y - 0.5 * z.y);
+    const double vx = sv.x * g, vy = sv.y * g;
+    e[off + i] = make_double2(a * (tx + hdt * vx), a * (ty + hdt * vy));
+  }
+}
+
 // sh.ln = a (s.ln + dt (1.5 nl.ln - 0.5 nl_p.ln))  (svis has no log term)
 __global__ void k_abcn_ln_(double *shln, const double *sln, const double *nlln, const double *nlpln, double dt,
                            double a) {
@@ -1678,6 +1734,23 @@ void k_abcn_rhs_d2(mlx_ctx *c, double2 *e, double *ln, const double2 *nl, const 
                 k.nrdim, k.npdim, k.ne, nrc, k.npchop, k.chopzl, k.chopzu, k.ell, dt, a, c->p.visc, pcut, zlo, zhi,
                 helm_pre ? nlln : nullptr, helm_pre ? nlpln : nullptr, helm_pre ? 1 : 0);
   if (!helm_pre) hipLaunchKernelGGL(k_abcn_ln_, dim3(1), dim3(1), 0, c->stream, ln, ln, nlln, nlpln, dt, a);
+  HIPCHK(hipGetLastError());
+}
+
+void k_abcn_rhs_hyp(mlx_ctx *c, double2 *e, double *ln, const double2 *nl, const double *nlln, const double2 *nlp,
+                    const double *nlpln, double dt, double a, double g, double b, int power, int nroff, int pcut,
+                    int zlo, int zhi, bool act) {
+  const KitDev &k = c->kit;
+  const int nrc = k.nrchop + nroff;
+  if (nrc > k.nrdim_ref) MLX_THROW(MLX_E_ARG, "chop_index: chopping in r too large");
+  if (pcut <= 0 || pcut > k.npdim) pcut = k.npdim;
+  dim3 grid(pcut, k.nzdim);
+  const size_t lds = 4 * sizeof(double2) * (size_t)k.nrdim;
+  hipLaunchKernelGGL(k_abcn_rhs_hyp_, grid, dim3(256), lds, c->stream, e, ln, nl, nlp, k.d2c, k.lognorm, k.ak,
+                     k.nrdim, k.npdim, k.ne, nrc, k.npchop, k.chopzl, k.chopzu, k.ell, dt, a, g, b, power / 2, pcut,
+                     zlo, zhi, act ? 1 : 0);
+  // the log coefficient after every read of s.ln above
+  hipLaunchKernelGGL(k_abcn_ln_, dim3(1), dim3(1), 0, c->stream, ln, ln, nlln, nlpln, dt, a);
   HIPCHK(hipGetLastError());
 }
 
