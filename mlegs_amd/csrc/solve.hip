@@ -469,11 +469,12 @@ __device__ __forceinline__ void solve_rows(double2 (&b)[S], const FacPtrs &F, co
 // Systems with and without row interchanges are solved by separate
 // instantiations (PIV), each returning at once for the other kind.
 // blockIdx.z selects the right-hand sides: e (0) or e2 (1), solved with the
-// same factors
+// same factors.  mixed = 0: every system is of this launch's kind (anyp is
+// not read).
 template <int S, bool PIV>
 __global__ void __launch_bounds__(64) k_lu_solve(double2 *e0, double2 *e2, FacPtrs F, const uint64_t *__restrict__ pw,
-                                                 const int *__restrict__ anyp, int nrdim, int npdim, int nz, int nrc,
-                                                 int npc, int nzc, int nzcu, int skip00, int zlo, int zhi) {
+                                                 const int *__restrict__ anyp, int mixed, int nrdim, int npdim, int nz,
+                                                 int nrc, int npc, int nzc, int nzcu, int skip00, int zlo, int zhi) {
   const int m = blockIdx.x, kk = blockIdx.y;
   double2 *e = blockIdx.z ? e2 : e0;
   // slices zlo..zhi left unsolved (both kk and nz - kk in the range)
@@ -484,7 +485,7 @@ __global__ void __launch_bounds__(64) k_lu_solve(double2 *e0, double2 *e2, FacPt
   const int n = nrc - m;
   if (n <= 0) return;
   const size_t sys = (size_t)kk * npc + m;
-  if ((anyp[sys] != 0) != PIV) return;
+  if (mixed && (anyp[sys] != 0) != PIV) return;
   const int k = twin ? nz - kk : kk;
   const bool valid = !twin || (kk != 0 && 2 * kk != nz);
   int cnt = valid ? z_count(k, nzc, nzcu) : 0;
@@ -651,8 +652,9 @@ void k_helm_solve(mlx_ctx *c, double2 *e, double alpha, int kind, int nroff, int
   ProbeScope probe(c, MLX_PROBE_LU_SOLVE, 32.0 * rows * nf + fbytes, 2.0 * 4.0 * 5.0 * rows * nf);
   const FacPtrs F = fac_ptrs(lu.sol, fac_rows(k, lu.seg));
   const int nrc = k.nrchop + nroff, sk = (kind & 3) == 2 ? 1 : 0;
+  const int mixed = lu.npiv > 0 && lu.npiv < lu.nsys;
 #define MLX_LU_LAUNCH(SV, PV)                                                                              \
-  probed_launch(probe, k_lu_solve<SV, PV>, grid, dim3(64), 0, c->stream, e, e2 ? e2 : e, F, lu.pvw, lu.anyp,       \
+  probed_launch(probe, k_lu_solve<SV, PV>, grid, dim3(64), 0, c->stream, e, e2 ? e2 : e, F, lu.pvw, lu.anyp, mixed, \
                 k.nrdim, k.npdim, k.nzdim, nrc, k.npchop, k.chopzl, k.chopzu, sk, zlo, zhi)
 #define MLX_LU_BOTH(SV)                               \
   do {                                                \
